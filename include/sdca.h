@@ -1,0 +1,160 @@
+/* sdca.h -- C ABI of the MI355X plmDCA engine (libsdca.so).
+ *
+ * The entry points replace the per-locus solve of SuperDCA (paths relative to the
+ * SuperDCA repository root):
+ *   - run_plmDCA<double,triallelic_state_t>     SuperDCA/include/plmDCA.hpp:436-437
+ *   - the TBB body plmDCA_solver                 SuperDCA/include/plmDCA.hpp:199-425
+ *   - cppoptlib::Problem::value_and_gradient     SuperDCA/include/LBFGS_interface_CppNumericalSolvers.hpp:111-117
+ *     -> plmDCA_objective_fval_and_gradient      SuperDCA/include/plmDCA_objective.hpp:38-249
+ *   - calculate_weights                          SuperDCA/include/plmDCA_optimizer_parameters.hpp:39-98
+ *   - B_eff / lambda initialisation              SuperDCA/include/plmDCA_optimizer_parameters.hpp:257-295
+ *
+ * Conventions: every call returns SDCA_OK (0) or a negative status; the message of
+ * the last failure on the calling thread is sdca_last_error().  The caller owns all
+ * host buffers; the library owns device memory and streams.  One context drives one
+ * GPU and is not thread-safe (one host thread per context).  Calls block until their
+ * results are in host memory.
+ */
+#ifndef SDCA_H
+#define SDCA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDCA_OK 0
+#define SDCA_E_ARG (-1)       /* invalid argument or shape */
+#define SDCA_E_HIP (-2)       /* HIP runtime failure (allocation, launch) */
+#define SDCA_E_NONFINITE (-3) /* objective became NaN/Inf */
+#define SDCA_E_NOCONV (-4)    /* at least one locus stopped without convergence */
+#define SDCA_E_STATE (-5)     /* call order violated (e.g. solve before weights) */
+#define SDCA_E_IO (-6)        /* file could not be read or written */
+
+/* Arithmetic of the energy and gradient contractions. */
+#define SDCA_PREC_FP32 0       /* FP32 MFMA (v_mfma_f32_32x32x2f32), FP32 X and W */
+#define SDCA_PREC_SPLIT_BF16 1 /* X exact in BF16, W and G as BF16 hi+lo, FP32 accumulation */
+
+/* Per-locus solve status (sdca_locus_stats.status). */
+#define SDCA_ST_CONVERGED 0
+#define SDCA_ST_ITER_CAP 1
+#define SDCA_ST_LS_FAIL 2
+#define SDCA_ST_NONFINITE 3
+
+typedef struct sdca_ctx sdca_ctx;
+
+typedef struct sdca_opts {
+    int32_t device;          /* HIP device ordinal (default 0) */
+    int32_t precision;       /* SDCA_PREC_* (default SDCA_PREC_FP32) */
+    int32_t batch_loci;      /* loci solved concurrently on the GPU (default 128) */
+    int32_t history;         /* L-BFGS history m (default 10, at most 16) */
+    int32_t max_iter;        /* iteration cap per locus (default 2000, plmDCA.hpp:219) */
+    int32_t max_ls;          /* function evaluations per line search (default 20) */
+    double grad_threshold;   /* stop when ||g||_inf < this (default 1e-3, plmDCA_options.cpp:61) */
+    int32_t verbose;         /* per-batch progress on stderr */
+    int32_t reserved[7];
+} sdca_opts;
+
+/* Defaults as documented above. */
+void sdca_default_opts(sdca_opts *opts);
+
+/* Per-locus result, plmDCA.hpp:306-315 (fval, gnorm, nfeval, iter) plus status. */
+typedef struct sdca_locus_stats {
+    double fval;
+    double gnorm;
+    int32_t nfeval;
+    int32_t iter;
+    int32_t status;
+    int32_t reserved;
+} sdca_locus_stats;
+
+/* Cumulative device timings of a context (milliseconds, HIP events). */
+typedef struct sdca_timing {
+    double gemm_energy_ms;   /* E = X W^T contractions */
+    double gemm_grad_ms;     /* grad W = G^T X contractions */
+    double softmax_ms;       /* LSE / softmax / node beliefs */
+    double lbfgs_ms;         /* direction, trial point and dot-product passes */
+    double score_ms;         /* gauge + Frobenius scoring */
+    double reweight_ms;      /* bit-plane popcount kernel */
+    double total_ms;         /* wall time inside sdca_solve_loci */
+    int64_t n_evals;         /* batched objective evaluations (rounds) */
+    int64_t n_energy_launches;
+    int64_t n_grad_launches;
+    double flops_energy;     /* algorithmic flops of all energy contractions (2*B*K_x*q per locus-eval) */
+    double flops_grad;
+    double bytes_lbfgs;      /* algorithmic HBM bytes of the L-BFGS vector passes */
+} sdca_timing;
+
+const char *sdca_last_error(void);
+
+/* Creates a context on opts->device over a B x L alignment of envelope states
+ * states[b*L + j] in [0, q_env), q_env in {2..8}, gap_state in [0, q_env) (or -1 if
+ * none).  The one-hot matrix X (B x K_x, K_x = observed states summed over loci)
+ * is built on the device. */
+int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L, int32_t q_env,
+                    int32_t gap_state, const sdca_opts *opts);
+void sdca_ctx_destroy(sdca_ctx *ctx);
+
+/* Number of observed-state columns K_x of the context's one-hot X. */
+int64_t sdca_num_columns(const sdca_ctx *ctx);
+
+/* calculate_weights (plmDCA_optimizer_parameters.hpp:39-98), bit-exact:
+ * identity = double(L - d_ij)/double(L) >= theta; int_w[i] = m_i + sum of m_j over
+ * such neighbours; w[i] = m_i / double(int_w[i]).  mult may be NULL (all 1);
+ * int_w_out may be NULL.  Does not install the weights. */
+int sdca_reweight(sdca_ctx *ctx, double theta, const uint32_t *mult, double *w_out,
+                  uint64_t *int_w_out);
+
+/* Installs sequence weights and derives B_eff and lambda_h, lambda_J as in
+ * plmDCA_optimizer_parameters.hpp:257-284 (negative user value = automatic rule).
+ * out3 (may be NULL) receives {B_eff, lambda_h, lambda_J}. */
+int sdca_set_weights(sdca_ctx *ctx, const double *w, double lambda_h_user, double lambda_J_user,
+                     double *out3);
+
+/* Parity entry point for plmDCA_objective_fval_and_gradient: for each of the K loci
+ * loci[k], evaluates f and g at x (K x n, reference layout: n = L*q*q + q, J_r[j](a,s)
+ * at 16-locus blocks, Matrix_kernel_access_order.hpp:28-34, h_r at L*q*q).  Rows of
+ * states never observed at j contribute only their L2 term, which is added on the host
+ * in FP64.  g (K x n) is overwritten; the block of locus r itself stays 0. */
+int sdca_eval_batch(sdca_ctx *ctx, const int32_t *loci, int32_t K, const double *x, double *f_out,
+                    double *g_out);
+
+/* Solves the K loci (any K; batched internally, converged slots are refilled) with the
+ * batched L-BFGS and writes each locus' mean-of-norms score row (K x L floats,
+ * plmDCA.hpp:366-394; row[k][loci[k]] = 0).  warm (K x n reference layout, may be NULL)
+ * gives start points; NULL = cold start from zero (plmDCA.hpp:298).  x_out (K x n,
+ * reference layout, may be NULL) receives the solutions.  st (K, may be NULL).
+ * Returns SDCA_E_NOCONV if any locus hit the iteration cap or a line-search failure
+ * (results are still written). */
+int sdca_solve_loci(sdca_ctx *ctx, const int32_t *loci, int32_t K, const double *warm,
+                    float *score_rows, double *x_out, sdca_locus_stats *st);
+
+/* Timings accumulated since the context was created or last reset. */
+int sdca_get_timing(sdca_ctx *ctx, sdca_timing *t, int32_t reset);
+
+/* ---- host-side alignment pipeline (the apegrunt contract the CLI relies on) ---- */
+
+/* MAF/gap filter and q_env-state transform (SuperDCA.cpp:274-277,354; README.md:32):
+ * chars is B x L_in (row-major, FASTA letters).  A locus is kept if it has more than
+ * one non-gap allele, minor-allele frequency (second most frequent allele over the
+ * non-gap count) >= maf_threshold and gap fraction <= gap_threshold, unless no_filter;
+ * alleles are then numbered by descending count (ties A<C<G<T), the gap is state
+ * q_env-1, and loci with more than q_env-1 alleles are dropped.  states_out (B x L_in
+ * capacity) and loci_out (L_in capacity, original column indices) receive the result;
+ * *L_out the number of kept loci. */
+int sdca_prepare_alignment(const char *chars, int32_t B, int32_t L_in, double maf_threshold,
+                           double gap_threshold, int32_t no_filter, int32_t q_env,
+                           uint8_t *states_out, int32_t *loci_out, int32_t *L_out);
+
+/* Writes the symmetrised couplings "score i j" (plmDCA.hpp:667-693): for r over the
+ * list in order and n before r: (float(S[r][n]) + float(S[n][r])) * 0.5 printed %.8f,
+ * then translation[r]+base and translation[n]+base.  S is L x L (row = locus). */
+int sdca_write_couplings(const char *path, const float *S, int32_t L, const int32_t *translation,
+                         int32_t base);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,234 @@
+"""superdca_amd -- MI355X plmDCA engine (Python face of libsdca.so).
+
+The compute path is libsdca.so (HIP kernels for gfx950 behind the C ABI declared in
+include/sdca.h).  This module binds that ABI with ctypes and mirrors the reference's
+per-locus interface (SuperDCA/include/plmDCA.hpp:436 run_plmDCA, the TBB body
+plmDCA_solver at :199-425 and the objective of plmDCA_objective.hpp:38-249) so that
+tests and benchmarks call exactly what a C++ caller would.  There is no fallback: if
+the library is missing or cannot be loaded every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsdca.so")
+
+SDCA_OK = 0
+SDCA_E_ARG = -1
+SDCA_E_HIP = -2
+SDCA_E_NONFINITE = -3
+SDCA_E_NOCONV = -4
+SDCA_E_STATE = -5
+SDCA_E_IO = -6
+
+PREC_FP32 = 0
+PREC_SPLIT_BF16 = 1
+
+ST_CONVERGED = 0
+ST_ITER_CAP = 1
+ST_LS_FAIL = 2
+ST_NONFINITE = 3
+
+# Every symbol include/sdca.h declares.
+EXPORTED = (
+    "sdca_default_opts", "sdca_last_error", "sdca_ctx_create", "sdca_ctx_destroy", "sdca_num_columns",
+    "sdca_reweight", "sdca_set_weights", "sdca_eval_batch", "sdca_solve_loci", "sdca_get_timing",
+    "sdca_prepare_alignment", "sdca_write_couplings",
+)
+
+
+class SdcaError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sdca error {code}: {msg}")
+        self.code = code
+
+
+class Opts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("precision", C.c_int32), ("batch_loci", C.c_int32),
+                ("history", C.c_int32), ("max_iter", C.c_int32), ("max_ls", C.c_int32),
+                ("grad_threshold", C.c_double), ("verbose", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+
+class LocusStats(C.Structure):
+    _fields_ = [("fval", C.c_double), ("gnorm", C.c_double), ("nfeval", C.c_int32), ("iter", C.c_int32),
+                ("status", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Timing(C.Structure):
+    _fields_ = [("gemm_energy_ms", C.c_double), ("gemm_grad_ms", C.c_double), ("softmax_ms", C.c_double),
+                ("lbfgs_ms", C.c_double), ("score_ms", C.c_double), ("reweight_ms", C.c_double),
+                ("total_ms", C.c_double), ("n_evals", C.c_int64), ("n_energy_launches", C.c_int64),
+                ("n_grad_launches", C.c_int64), ("flops_energy", C.c_double), ("flops_grad", C.c_double),
+                ("bytes_lbfgs", C.c_double)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libsdca.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SdcaError(SDCA_E_STATE, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    L.sdca_last_error.restype = C.c_char_p
+    L.sdca_default_opts.argtypes = [P(Opts)]
+    L.sdca_ctx_create.argtypes = [P(C.c_void_p), C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(Opts)]
+    L.sdca_ctx_destroy.argtypes = [C.c_void_p]
+    L.sdca_num_columns.argtypes = [C.c_void_p]
+    L.sdca_num_columns.restype = C.c_int64
+    L.sdca_reweight.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.sdca_set_weights.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p]
+    L.sdca_eval_batch.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.sdca_solve_loci.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.sdca_get_timing.argtypes = [C.c_void_p, P(Timing), C.c_int32]
+    L.sdca_prepare_alignment.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                         C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.sdca_write_couplings.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
+    for name in ("sdca_ctx_create", "sdca_reweight", "sdca_set_weights", "sdca_eval_batch", "sdca_solve_loci",
+                 "sdca_get_timing", "sdca_prepare_alignment", "sdca_write_couplings"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(rc: int, allow=()) -> int:
+    if rc != SDCA_OK and rc not in allow:
+        raise SdcaError(rc, lib().sdca_last_error().decode(errors="replace"))
+    return rc
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    lib().sdca_default_opts(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise TypeError(f"unknown option {k}")
+        setattr(o, k, v)
+    return o
+
+
+def nparams(L: int, q: int) -> int:
+    """Length of one locus' parameter vector in the reference layout (L*q*q + q)."""
+    return L * q * q + q
+
+
+class Context:
+    """One alignment on one GPU: the role of run_plmDCA's inputs (plmDCA.hpp:436-437)."""
+
+    def __init__(self, states: np.ndarray, q: int = 4, gap: int | None = None, **opts):
+        states = np.ascontiguousarray(states, dtype=np.uint8)
+        if states.ndim != 2:
+            raise ValueError("states must be B x L")
+        self.B, self.L = states.shape
+        self.q = q
+        self.gap = q - 1 if gap is None else gap
+        self.opts = default_opts(**opts)
+        h = C.c_void_p()
+        _check(lib().sdca_ctx_create(C.byref(h), _ptr(states), self.B, self.L, q, self.gap, C.byref(self.opts)))
+        self._h = h
+        self.B_eff = self.lambda_h = self.lambda_J = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sdca_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def num_columns(self) -> int:
+        return int(lib().sdca_num_columns(self._h))
+
+    def reweight(self, theta: float = 0.9, mult: np.ndarray | None = None):
+        """calculate_weights (plmDCA_optimizer_parameters.hpp:39-98) -> (w, int_w)."""
+        w = np.zeros(self.B, dtype=np.float64)
+        iw = np.zeros(self.B, dtype=np.uint64)
+        m = None if mult is None else np.ascontiguousarray(mult, dtype=np.uint32)
+        _check(lib().sdca_reweight(self._h, float(theta), _ptr(m), _ptr(w), _ptr(iw)))
+        return w, iw
+
+    def set_weights(self, w: np.ndarray, lambda_h: float = -1.0, lambda_J: float = -1.0):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        out = np.zeros(3, dtype=np.float64)
+        _check(lib().sdca_set_weights(self._h, _ptr(w), float(lambda_h), float(lambda_J), _ptr(out)))
+        self.B_eff, self.lambda_h, self.lambda_J = (float(v) for v in out)
+        return self.B_eff, self.lambda_h, self.lambda_J
+
+    def eval_batch(self, loci, x: np.ndarray):
+        """f and g of plmDCA_objective_fval_and_gradient for each locus (reference layout)."""
+        loci = np.ascontiguousarray(loci, dtype=np.int32)
+        K = loci.size
+        n = nparams(self.L, self.q)
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(K, n)
+        f = np.zeros(K, dtype=np.float64)
+        g = np.zeros((K, n), dtype=np.float64)
+        _check(lib().sdca_eval_batch(self._h, _ptr(loci), K, _ptr(x), _ptr(f), _ptr(g)))
+        return f, g
+
+    def solve(self, loci, warm: np.ndarray | None = None, want_x: bool = False, rows: bool = True):
+        """Batched L-BFGS solve -> (score_rows K x L float32, stats, x or None, rc)."""
+        loci = np.ascontiguousarray(loci, dtype=np.int32)
+        K = loci.size
+        n = nparams(self.L, self.q)
+        S = np.zeros((K, self.L), dtype=np.float32) if rows else None
+        X = np.zeros((K, n), dtype=np.float64) if want_x else None
+        st = (LocusStats * max(K, 1))()
+        wm = None if warm is None else np.ascontiguousarray(warm, dtype=np.float64).reshape(K, n)
+        rc = _check(lib().sdca_solve_loci(self._h, _ptr(loci), K, _ptr(wm), _ptr(S), _ptr(X),
+                                          C.cast(st, C.c_void_p)), allow=(SDCA_E_NOCONV,))
+        stats = {
+            "fval": np.array([st[k].fval for k in range(K)]),
+            "gnorm": np.array([st[k].gnorm for k in range(K)]),
+            "nfeval": np.array([st[k].nfeval for k in range(K)], dtype=np.int64),
+            "iter": np.array([st[k].iter for k in range(K)], dtype=np.int64),
+            "status": np.array([st[k].status for k in range(K)], dtype=np.int64),
+        }
+        return S, stats, X, rc
+
+    def timing(self, reset: bool = False) -> dict:
+        t = Timing()
+        _check(lib().sdca_get_timing(self._h, C.byref(t), int(reset)))
+        return {name: getattr(t, name) for name, _ in Timing._fields_}
+
+
+def prepare_alignment(chars: np.ndarray, maf_threshold: float = 0.01, gap_threshold: float = 0.15,
+                      no_filter: bool = False, q: int = 4):
+    """MAF/gap filter and envelope transform (SuperDCA.cpp:274-277,354) -> (states, loci)."""
+    chars = np.ascontiguousarray(chars, dtype=np.uint8)
+    B, L_in = chars.shape
+    states = np.zeros((B, L_in), dtype=np.uint8)
+    loci = np.zeros(L_in, dtype=np.int32)
+    L = C.c_int32(0)
+    _check(lib().sdca_prepare_alignment(_ptr(chars), B, L_in, float(maf_threshold), float(gap_threshold),
+                                        int(no_filter), q, _ptr(states), _ptr(loci), C.byref(L)))
+    Lk = L.value
+    return np.ascontiguousarray(states.reshape(-1)[: B * Lk].reshape(B, Lk)), loci[:Lk].copy()
+
+
+def write_couplings(path: str, S: np.ndarray, translation: np.ndarray, base: int = 1) -> None:
+    """Symmetrised "score i j" lines of plmDCA.hpp:667-693."""
+    S = np.ascontiguousarray(S, dtype=np.float32)
+    tr = np.ascontiguousarray(translation, dtype=np.int32)
+    _check(lib().sdca_write_couplings(path.encode(), _ptr(S), S.shape[0], _ptr(tr), int(base)))

@@ -1,0 +1,911 @@
+// sdca_api.cpp -- C ABI of libsdca.so: context, reweighting, objective parity entry and
+// the batched L-BFGS solve of many loci on one MI355X.
+//
+// Reference behaviour (paths relative to the SuperDCA repository root):
+//   run_plmDCA / plmDCA_solver          SuperDCA/include/plmDCA.hpp:199-437
+//   calculate_weights                   SuperDCA/include/plmDCA_optimizer_parameters.hpp:39-98
+//   B_eff and lambda                    SuperDCA/include/plmDCA_optimizer_parameters.hpp:257-284
+//   objective and gradient              SuperDCA/include/plmDCA_objective.hpp:38-249
+//
+// Batched L-BFGS.  Every slot of a batch holds one locus.  Its history lives in a ring
+// of P = 2m+3 device vectors: m+1 search directions d_i and m+2 gradients g_i, so that
+// s_i = alpha_i d_i and y_i = g_{i+1} - g_i are never materialised.  The host keeps the
+// P x P Gram matrix of the ring (FP64 dot products computed on the device while the
+// vectors are written) and runs the two-loop recursion on coefficient vectors over the
+// ring; the device then forms d = sum_p coef_p ring_p in one pass.  Each round of the
+// batch performs one objective evaluation for every active slot: trial point,
+// energies, softmax, gradient, dot products, one small device->host copy.  The line
+// search is the More-Thuente method, stepped on the host per slot.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "linesearch.hpp"
+#include "sdca_internal.hpp"
+
+namespace sdca {
+size_t ref_index(int L, int q, int j, int a, int s);  // sdca_host.cpp
+}
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                      \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess) return fail(SDCA_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(SDCA_E_HIP, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+    }
+    e = hipMemset(*p, 0, count * sizeof(T));
+    if (e != hipSuccess) return fail(SDCA_E_HIP, "hipMemset failed: %s", hipGetErrorString(e));
+    return SDCA_OK;
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+bool valid_history(int m) { return m == 1 || m == 3 || m == 5 || m == 10 || m == 16; }
+
+}  // namespace
+
+void sdca_ctx::free_workspace()
+{
+    dfree(d_ring);
+    dfree(d_Xtr);
+    d_Xc = nullptr;  // second half of the d_Xtr allocation
+    dfree(d_X0);
+    dfree(d_Whi);
+    dfree(d_Wlo);
+    dfree(d_G32);
+    dfree(d_Ghi);
+    dfree(d_Glo);
+    dfree(d_Epart);
+    dfree(d_upd_part);
+    dfree(d_dot_part);
+    dfree(d_sm_part);
+    dfree(d_res);
+    dfree(d_cmds);
+    dfree(d_rows);
+    dfree(d_src_off);
+    dfree(d_score_loci);
+    ws_slots = 0;
+    ws_m = 0;
+}
+
+sdca_ctx::~sdca_ctx()
+{
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    free_workspace();
+    dfree(d_states);
+    dfree(d_statesT);
+    dfree(d_state_col);
+    dfree(d_X);
+    dfree(d_Xt);
+    dfree(d_w);
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+
+// Allocates the batch workspace for nslots concurrent loci and history m.
+int ensure_workspace(sdca_ctx *c, int nslots, int m)
+{
+    if (c->ws_slots == nslots && c->ws_m == m) return SDCA_OK;
+    c->free_workspace();
+    const int P = 2 * m + 3;
+    c->n_slot = (int64_t)c->q * c->Kxp;
+    c->Mpad = (int)round_up((int64_t)nslots * c->q, sdca::kTile);
+    const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
+    int rc;
+    if ((rc = dalloc(&c->d_ring, (size_t)nslots * P * c->n_slot))) return rc;
+    if ((rc = dalloc(&c->d_Xtr, (size_t)2 * c->Mpad * c->Kxp))) return rc;
+    c->d_Xc = c->d_Xtr + (size_t)c->Mpad * c->Kxp;
+    if (split) {
+        if ((rc = dalloc(&c->d_Whi, (size_t)c->Mpad * c->Kxp))) return rc;
+        if ((rc = dalloc(&c->d_Wlo, (size_t)c->Mpad * c->Kxp))) return rc;
+        if ((rc = dalloc(&c->d_Ghi, (size_t)c->Mpad * c->Bp))) return rc;
+        if ((rc = dalloc(&c->d_Glo, (size_t)c->Mpad * c->Bp))) return rc;
+    } else {
+        if ((rc = dalloc(&c->d_G32, (size_t)c->Mpad * c->Bp))) return rc;
+    }
+    // split-K of the energy contraction: aim for >= 2048 workgroups
+    const int BK = split ? 64 : 32;
+    const int64_t tiles = (int64_t)(c->Mpad / sdca::kTile) * (c->Bp / sdca::kTile);
+    int64_t ks = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
+    ks = std::min<int64_t>(ks, c->Kxp / BK);
+    c->kchunk = (int)round_up((c->Kxp + ks - 1) / ks, BK);
+    c->ksplit = (int)((c->Kxp + c->kchunk - 1) / c->kchunk);
+    if ((rc = dalloc(&c->d_Epart, (size_t)c->ksplit * c->Mpad * c->Bp))) return rc;
+    c->nchunk = (int)std::min<int64_t>(64, std::max<int64_t>(1, (c->n_slot / 4 + 255) / 256));
+    c->nbchunk = (c->Bp + 255) / 256;
+    if ((rc = dalloc(&c->d_upd_part, (size_t)nslots * c->nchunk * (P + 2)))) return rc;
+    if ((rc = dalloc(&c->d_dot_part, (size_t)nslots * c->nchunk * (P + 1)))) return rc;
+    if ((rc = dalloc(&c->d_sm_part, (size_t)nslots * c->nbchunk))) return rc;
+    if ((rc = dalloc(&c->d_res, (size_t)nslots * (2 * P + 4)))) return rc;
+    if ((rc = dalloc(&c->d_cmds, (size_t)nslots))) return rc;
+    if ((rc = dalloc(&c->d_rows, (size_t)nslots * c->L))) return rc;
+    if ((rc = dalloc(&c->d_src_off, (size_t)nslots))) return rc;
+    if ((rc = dalloc(&c->d_score_loci, (size_t)nslots))) return rc;
+    c->ws_slots = nslots;
+    c->ws_m = m;
+    return SDCA_OK;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+// One batched evaluation round: trial points, energies, softmax, gradient, dots.
+// Results (ResLayout rows) land in res (host).
+int run_round(sdca_ctx *c, int nslots, const std::vector<sdca::SlotCmd> &cmds, std::vector<double> &res)
+{
+    const int m = c->ws_m, P = 2 * m + 3;
+    const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
+    hipStream_t st = c->stream;
+    HIP_TRY(hipMemcpyAsync(c->d_cmds, cmds.data(), nslots * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(c->ev[0], st));
+    sdca::kern::update(st, m, c->opts.precision, c->d_cmds, nslots, c->d_ring, c->d_Xc, c->d_Xtr,
+                       split ? c->d_Whi : nullptr, split ? c->d_Wlo : nullptr, c->d_X0, c->n_slot, (int)c->Kxp,
+                       (int)c->Kx, c->d_upd_part, c->nchunk);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[1], st));
+    sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_X, c->Mpad, c->Bp, (int)c->Kxp,
+                            c->ksplit, c->kchunk, c->d_Epart);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[2], st));
+    sdca::kern::softmax(st, c->opts.precision, c->d_Epart, c->ksplit, c->Mpad, c->Bp, c->B, c->q, nslots, c->d_cmds,
+                        c->d_statesT, c->d_w, c->d_G32, c->d_Ghi, c->d_Glo, c->d_sm_part, c->nbchunk);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[3], st));
+    sdca::kern::gemm_grad(st, c->opts.precision, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Xt, c->Mpad, c->Bp, (int)c->Kxp,
+                          (int)c->Kx, c->q, nslots, c->d_cmds, c->d_Xtr, c->d_ring, c->n_slot, P,
+                          (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[4], st));
+    sdca::kern::dots(st, m, c->d_cmds, nslots, c->d_ring, c->n_slot, c->d_dot_part, c->nchunk);
+    HIP_TRY(hipGetLastError());
+    sdca::kern::reduce_results(st, m, nslots, c->d_upd_part, c->d_dot_part, c->d_sm_part, c->nchunk, c->nbchunk,
+                               c->d_res);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->ev[5], st));
+    res.resize((size_t)nslots * (2 * P + 4));
+    HIP_TRY(hipMemcpyAsync(res.data(), c->d_res, res.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    auto &t = c->timing;
+    t.lbfgs += elapsed(c->ev[0], c->ev[1]) + elapsed(c->ev[4], c->ev[5]);
+    t.energy += elapsed(c->ev[1], c->ev[2]);
+    t.softmax += elapsed(c->ev[2], c->ev[3]);
+    t.grad += elapsed(c->ev[3], c->ev[4]);
+    t.evals += 1;
+    t.energy_launches += 1;
+    t.grad_launches += 1;
+    int active = 0;
+    double vec = 0.0;  // vector passes of the update and dot-product kernels
+    for (int i = 0; i < nslots; ++i) {
+        const int md = cmds[i].mode;
+        active += md != sdca::MODE_IDLE;
+        if (md == sdca::MODE_INIT) vec += 2 + (cmds[i].init_src != 0.f) + P;
+        else if (md == sdca::MODE_RETRY) vec += 3 + P;
+        else if (md == sdca::MODE_NEWDIR) vec += (P + 4) + P;
+        if (md != sdca::MODE_IDLE && split) vec += 1;  // hi + lo BF16 copies of x_t
+    }
+    t.bytes_lbfgs += vec * 4.0 * (double)c->n_slot;
+    t.flops_energy += 2.0 * c->B * (double)c->Kx * c->q * active;
+    t.flops_grad += 2.0 * c->B * (double)c->Kx * c->q * active;
+    return SDCA_OK;
+}
+
+// Reference layout (x of plmDCA_optimizer_parameters.hpp:176,219-247) -> device slot
+// vector (q x Kxp floats).  Returns the FP64 sum of squares of the entries whose state
+// is never observed (their only contribution is the L2 term).
+double ref_to_dev(const sdca_ctx *c, int r, const double *x, float *dev)
+{
+    std::fill(dev, dev + c->n_slot, 0.f);
+    double unobs = 0.0;
+    const int L = c->L, q = c->q;
+    for (int j = 0; j < L; ++j) {
+        if (j == r) continue;
+        for (int a = 0; a < q; ++a) {
+            const int col = c->state_col[(size_t)j * q + a];
+            const size_t o = sdca::ref_index(L, q, j, a, 0);
+            for (int s = 0; s < q; ++s) {
+                const double v = x[o + s];
+                if (col >= 0) dev[(size_t)s * c->Kxp + col] = (float)v;
+                else unobs += v * v;
+            }
+        }
+    }
+    for (int s = 0; s < q; ++s) dev[(size_t)s * c->Kxp + c->Kx] = (float)x[(size_t)L * q * q + s];
+    return unobs;
+}
+
+void dev_to_ref(const sdca_ctx *c, int r, const float *dev, double *x)
+{
+    const int L = c->L, q = c->q;
+    std::fill(x, x + (size_t)L * q * q + q, 0.0);
+    for (int j = 0; j < L; ++j) {
+        if (j == r) continue;
+        for (int a = 0; a < q; ++a) {
+            const int col = c->state_col[(size_t)j * q + a];
+            if (col < 0) continue;
+            const size_t o = sdca::ref_index(L, q, j, a, 0);
+            for (int s = 0; s < q; ++s) x[o + s] = dev[(size_t)s * c->Kxp + col];
+        }
+    }
+    for (int s = 0; s < q; ++s) x[(size_t)L * q * q + s] = dev[(size_t)s * c->Kxp + c->Kx];
+}
+
+sdca::SlotCmd idle_cmd()
+{
+    sdca::SlotCmd k;
+    std::memset(&k, 0, sizeof(k));
+    k.mode = sdca::MODE_IDLE;
+    k.locus = -1;
+    return k;
+}
+
+// ------------------------------------------------------- per-slot solver state
+
+struct Pair {
+    int d, g_old, g_new;
+    double alpha;
+};
+
+enum Phase { PH_IDLE, PH_INIT, PH_LS, PH_DONE };
+
+struct Slot {
+    int item = -1;  // index into the caller's loci list
+    int r = -1;
+    Phase phase = PH_IDLE;
+    std::deque<Pair> pairs;  // oldest first
+    int gk = -1, gt = -1, dk = -1;
+    double f = 0.0;      // objective at the accepted iterate
+    double stp = 0.0;    // current trial step
+    double stp0 = 0.0;   // initial step of the current search
+    sdca::MoreThuente ms;
+    int ls_evals = 0;
+    int nfeval = 0, iter = 0, status = -1;
+    double gnorm = 0.0;
+    bool from_xt = true;
+};
+
+class Solver {
+public:
+    Solver(sdca_ctx *c, int nslots) : c_(c), n_(nslots), m_(c->ws_m), P_(2 * c->ws_m + 3), slots_(nslots),
+                                       gram_((size_t)nslots * P_ * P_, 0.0), cmds_(nslots, idle_cmd()) {}
+
+    double &G(int i, int a, int b) { return gram_[((size_t)i * P_ + a) * P_ + b]; }
+
+    void assign(int i, int item, int r, bool warm)
+    {
+        Slot &S = slots_[i];
+        S = Slot();
+        S.item = item;
+        S.r = r;
+        S.phase = PH_INIT;
+        S.gt = m_ + 1;
+        sdca::SlotCmd &k = cmds_[i];
+        k = idle_cmd();
+        k.mode = sdca::MODE_INIT;
+        k.locus = r;
+        k.gt = S.gt;
+        k.dst = 0;
+        k.self_lo = c_->col_off[r];
+        k.self_hi = c_->col_off[r + 1];
+        k.init_src = warm ? 1.f : 0.f;
+    }
+
+    void set_idle(int i)
+    {
+        slots_[i] = Slot();
+        cmds_[i] = idle_cmd();
+    }
+
+    // Consumes the result row of slot i after a round.  Returns true when the slot's
+    // locus is finished (status set).
+    bool process(int i, const double *res)
+    {
+        Slot &S = slots_[i];
+        sdca::SlotCmd &k = cmds_[i];
+        const sdca::ResLayout R{P_};
+        const int pending = k.mode;
+        if (pending == sdca::MODE_NEWDIR)
+            for (int p = 0; p < P_; ++p) G(i, S.dk, p) = G(i, p, S.dk) = res[R.ddots() + p];
+        for (int p = 0; p < P_; ++p) G(i, S.gt, p) = G(i, p, S.gt) = res[R.gdots() + p];
+        const double f = res[R.fdata()] + c_->lambda_J * res[R.xsqJ()] + c_->lambda_h * res[R.xsqh()];
+        const double ginf = res[R.ginf()];
+        S.nfeval++;
+        if (!std::isfinite(f) || !std::isfinite(ginf)) return finish(i, SDCA_ST_NONFINITE, S.phase == PH_INIT);
+        if (S.phase == PH_INIT) {
+            S.gk = S.gt;
+            S.gt = -1;
+            S.f = f;
+            S.gnorm = ginf;
+            if (ginf < c_->opts.grad_threshold) return finish(i, SDCA_ST_CONVERGED, true);
+            newdir(i);
+            return false;
+        }
+        // line search in progress
+        if (pending == sdca::MODE_NEWDIR) {
+            const double dg0 = G(i, S.dk, S.gk);
+            if (!(dg0 < 0.0)) {  // rounding destroyed descent: restart from steepest descent
+                S.pairs.clear();
+                newdir(i);
+                return false;
+            }
+            S.ms.start(S.f, dg0, S.stp0);
+        }
+        const double dg = G(i, S.gt, S.dk);
+        const double stp_trial = S.stp;
+        const auto act = S.ms.step(f, dg, S.stp);
+        S.ls_evals++;
+        if (act == sdca::MoreThuente::Continue && S.ls_evals < c_->opts.max_ls) {
+            k.mode = sdca::MODE_RETRY;
+            k.dst = S.dk;
+            k.gt = S.gt;
+            k.alpha = (float)S.stp;
+            return false;
+        }
+        if (!(f < S.f)) return finish(i, SDCA_ST_LS_FAIL, false);
+        const double ys = stp_trial * (G(i, S.dk, S.gt) - G(i, S.dk, S.gk));
+        if (ys > 0.0) {
+            S.pairs.push_back(Pair{S.dk, S.gk, S.gt, stp_trial});
+            if ((int)S.pairs.size() > m_) S.pairs.pop_front();
+        } else {
+            S.pairs.clear();
+        }
+        S.gk = S.gt;
+        S.gt = -1;
+        S.dk = -1;
+        S.f = f;
+        S.gnorm = ginf;
+        S.iter++;
+        if (ginf < c_->opts.grad_threshold) return finish(i, SDCA_ST_CONVERGED, true);
+        if (S.iter >= c_->opts.max_iter) return finish(i, SDCA_ST_ITER_CAP, true);
+        newdir(i);
+        return false;
+    }
+
+    const Slot &slot(int i) const { return slots_[i]; }
+    const std::vector<sdca::SlotCmd> &cmds() const { return cmds_; }
+    std::vector<sdca::SlotCmd> &cmds() { return cmds_; }
+
+private:
+    bool finish(int i, int status, bool from_xt)
+    {
+        Slot &S = slots_[i];
+        S.status = status;
+        S.phase = PH_DONE;
+        S.from_xt = from_xt;
+        return true;
+    }
+
+    int free_d(int i) const
+    {
+        const Slot &S = slots_[i];
+        for (int d = 0; d <= m_; ++d) {
+            bool used = (d == S.dk);
+            for (const auto &p : S.pairs) used |= (p.d == d);
+            if (!used) return d;
+        }
+        return -1;
+    }
+    int free_g(int i) const
+    {
+        const Slot &S = slots_[i];
+        for (int g = m_ + 1; g < P_; ++g) {
+            bool used = (g == S.gk);
+            for (const auto &p : S.pairs) used |= (p.g_old == g || p.g_new == g);
+            if (!used) return g;
+        }
+        return -1;
+    }
+
+    // Two-loop recursion (Nocedal & Wright, Algorithm 7.4) on coefficient vectors over
+    // the ring: u . v = u^T Gram v, s_t = alpha_t e_{d_t}, y_t = e_{g_new} - e_{g_old}.
+    void two_loop(int i, std::vector<double> &delta)
+    {
+        Slot &S = slots_[i];
+        const int n = (int)S.pairs.size();
+        std::vector<double> qv(P_, 0.0), Gv(P_), a(n), rho(n);
+        auto gmul = [&](const std::vector<double> &v) {
+            for (int p = 0; p < P_; ++p) {
+                double acc = 0.0;
+                for (int u = 0; u < P_; ++u)
+                    if (v[u] != 0.0) acc += G(i, p, u) * v[u];
+                Gv[p] = acc;
+            }
+        };
+        qv[S.gk] = 1.0;
+        for (int t = n - 1; t >= 0; --t) {
+            const Pair &pr = S.pairs[t];
+            const double ys = pr.alpha * (G(i, pr.d, pr.g_new) - G(i, pr.d, pr.g_old));
+            rho[t] = 1.0 / ys;
+            gmul(qv);
+            a[t] = rho[t] * pr.alpha * Gv[pr.d];
+            qv[pr.g_new] -= a[t];
+            qv[pr.g_old] += a[t];
+        }
+        if (n > 0) {
+            const Pair &pr = S.pairs[n - 1];
+            const double ys = pr.alpha * (G(i, pr.d, pr.g_new) - G(i, pr.d, pr.g_old));
+            const double yy = G(i, pr.g_new, pr.g_new) - 2.0 * G(i, pr.g_new, pr.g_old) + G(i, pr.g_old, pr.g_old);
+            const double gam = ys / yy;
+            for (auto &v : qv) v *= gam;
+        }
+        for (int t = 0; t < n; ++t) {
+            const Pair &pr = S.pairs[t];
+            gmul(qv);
+            const double beta = rho[t] * (Gv[pr.g_new] - Gv[pr.g_old]);
+            qv[pr.d] += (a[t] - beta) * pr.alpha;
+        }
+        delta.assign(P_, 0.0);
+        for (int p = 0; p < P_; ++p) delta[p] = -qv[p];
+    }
+
+    void newdir(int i)
+    {
+        Slot &S = slots_[i];
+        std::vector<double> delta(P_, 0.0);
+        bool ok = false;
+        if (!S.pairs.empty()) {
+            two_loop(i, delta);
+            double dg = 0.0;
+            bool finite = true;
+            for (int p = 0; p < P_; ++p) {
+                dg += delta[p] * G(i, p, S.gk);
+                finite &= std::isfinite(delta[p]);
+            }
+            ok = finite && dg < 0.0;
+            if (!ok) S.pairs.clear();
+        }
+        if (!ok) {
+            std::fill(delta.begin(), delta.end(), 0.0);
+            delta[S.gk] = -1.0;
+        }
+        S.dk = free_d(i);
+        S.gt = free_g(i);
+        S.stp0 = S.pairs.empty() ? 1.0 / std::sqrt(G(i, S.gk, S.gk)) : 1.0;
+        S.stp = S.stp0;
+        S.ls_evals = 0;
+        S.phase = PH_LS;
+        sdca::SlotCmd &k = cmds_[i];
+        k.mode = sdca::MODE_NEWDIR;
+        k.dst = S.dk;
+        k.gt = S.gt;
+        k.alpha = (float)S.stp;
+        for (int p = 0; p < sdca::kMaxPhys; ++p) k.coef[p] = p < P_ ? (float)delta[p] : 0.f;
+        k.coef[S.dk] = 0.f;
+    }
+
+    sdca_ctx *c_;
+    int n_, m_, P_;
+    std::vector<Slot> slots_;
+    std::vector<double> gram_;
+    std::vector<sdca::SlotCmd> cmds_;
+};
+
+int check_ctx(sdca_ctx *c)
+{
+    if (!c) return fail(SDCA_E_ARG, "null context");
+    HIP_TRY(hipSetDevice(c->device));
+    return SDCA_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+const char *sdca_last_error(void) { return g_err.c_str(); }
+
+void sdca_default_opts(sdca_opts *o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->device = 0;
+    o->precision = SDCA_PREC_FP32;
+    o->batch_loci = 128;
+    o->history = 10;
+    o->max_iter = 2000;
+    o->max_ls = 20;
+    o->grad_threshold = 1e-3;
+    o->verbose = 0;
+}
+
+int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L, int32_t q, int32_t gap,
+                    const sdca_opts *opts)
+{
+    if (!out) return fail(SDCA_E_ARG, "out is null");
+    *out = nullptr;
+    if (!states || B < 1 || L < 2) return fail(SDCA_E_ARG, "need states, B >= 1 and L >= 2 (got B=%d L=%d)", B, L);
+    if (q < 2 || q > 8) return fail(SDCA_E_ARG, "q_env must be in [2, 8] (got %d)", q);
+    if (gap >= q) return fail(SDCA_E_ARG, "gap_state %d outside [0, %d)", gap, q);
+    sdca_opts o;
+    if (opts) o = *opts;
+    else sdca_default_opts(&o);
+    if (o.precision != SDCA_PREC_FP32 && o.precision != SDCA_PREC_SPLIT_BF16)
+        return fail(SDCA_E_ARG, "unknown precision %d", o.precision);
+    if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 10, 16 (got %d)", o.history);
+    if (o.batch_loci < 1) return fail(SDCA_E_ARG, "batch_loci must be >= 1");
+    if (o.max_iter < 1 || o.max_ls < 1) return fail(SDCA_E_ARG, "max_iter and max_ls must be >= 1");
+    if (!(o.grad_threshold >= 0.0)) return fail(SDCA_E_ARG, "grad_threshold must be >= 0");
+    for (int64_t e = 0; e < (int64_t)B * L; ++e)
+        if (states[e] >= q) return fail(SDCA_E_ARG, "state %d at element %lld outside [0, %d)", states[e], (long long)e, q);
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(SDCA_E_HIP, "no HIP device available");
+    if (o.device < 0 || o.device >= ndev) return fail(SDCA_E_ARG, "device %d not present (%d devices)", o.device, ndev);
+
+    auto *c = new sdca_ctx();
+    c->opts = o;
+    c->device = o.device;
+    c->B = B;
+    c->L = L;
+    c->q = q;
+    c->gap = gap;
+    c->h_states.assign(states, states + (size_t)B * L);
+    // observed states per locus -> columns of X
+    c->col_off.assign(L + 1, 0);
+    c->state_col.assign((size_t)L * q, -1);
+    std::vector<uint8_t> seen((size_t)L * q, 0);
+    for (int b = 0; b < B; ++b)
+        for (int j = 0; j < L; ++j) seen[(size_t)j * q + states[(size_t)b * L + j]] = 1;
+    int64_t col = 0;
+    for (int j = 0; j < L; ++j) {
+        c->col_off[j] = (int32_t)col;
+        for (int a = 0; a < q; ++a)
+            if (seen[(size_t)j * q + a]) c->state_col[(size_t)j * q + a] = (int32_t)col++;
+    }
+    c->col_off[L] = (int32_t)col;
+    c->Kx = col;
+    c->Kxp = round_up(c->Kx + 1, sdca::kTile);
+    c->Bp = (int)round_up(B, sdca::kTile);
+    if (c->Kxp >= (int64_t(1) << 31) / 8) {
+        delete c;
+        return fail(SDCA_E_ARG, "alignment too wide (K_x = %lld)", (long long)col);
+    }
+
+    int rc = SDCA_OK;
+    auto bail = [&](int code) {
+        delete c;
+        return code;
+    };
+    if (hipSetDevice(c->device) != hipSuccess) return bail(fail(SDCA_E_HIP, "hipSetDevice(%d) failed", c->device));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(SDCA_E_HIP, "hipStreamCreate failed"));
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) return bail(fail(SDCA_E_HIP, "hipEventCreate failed"));
+    std::vector<uint8_t> sT((size_t)B * L);
+    for (int b = 0; b < B; ++b)
+        for (int j = 0; j < L; ++j) sT[(size_t)j * B + b] = states[(size_t)b * L + j];
+    if ((rc = dalloc(&c->d_states, (size_t)B * L))) return bail(rc);
+    if ((rc = dalloc(&c->d_statesT, (size_t)B * L))) return bail(rc);
+    if ((rc = dalloc(&c->d_state_col, (size_t)L * q))) return bail(rc);
+    if ((rc = dalloc(&c->d_w, (size_t)c->Bp))) return bail(rc);
+    const size_t esz = o.precision == SDCA_PREC_FP32 ? 4 : 2;
+    if ((rc = dalloc((uint8_t **)&c->d_X, (size_t)c->Bp * c->Kxp * esz))) return bail(rc);
+    if ((rc = dalloc((uint8_t **)&c->d_Xt, (size_t)c->Bp * c->Kxp * esz))) return bail(rc);
+    if (hipMemcpy(c->d_states, states, (size_t)B * L, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_statesT, sT.data(), (size_t)B * L, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->d_state_col, c->state_col.data(), c->state_col.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(SDCA_E_HIP, "upload of the alignment failed"));
+    sdca::kern::build_x(c->stream, c->d_states, B, L, q, c->d_state_col, (int)c->Kx, c->Bp, (int)c->Kxp, o.precision,
+                        c->d_X, c->d_Xt);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+        return bail(fail(SDCA_E_HIP, "building X failed"));
+    *out = c;
+    return SDCA_OK;
+}
+
+void sdca_ctx_destroy(sdca_ctx *ctx) { delete ctx; }
+
+int64_t sdca_num_columns(const sdca_ctx *ctx) { return ctx ? ctx->Kx : -1; }
+
+int sdca_reweight(sdca_ctx *c, double theta, const uint32_t *mult, double *w_out, uint64_t *int_w_out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!w_out) return fail(SDCA_E_ARG, "w_out is null");
+    const int B = c->B, L = c->L, q = c->q;
+    const int W = (L + 63) / 64;
+    // largest mismatch count d with double(L - d)/double(L) >= theta
+    // (plmDCA_optimizer_parameters.hpp:67-72: size_t difference, then the FP64 division)
+    int64_t dmax = -1;
+    for (int64_t d = 0; d <= L; ++d) {
+        const double identity = (double)((size_t)L - (size_t)d) / (double)(size_t)L;
+        if (identity >= theta) dmax = d;
+        else break;  // identity is non-increasing in d
+    }
+    uint64_t *d_planes = nullptr;
+    unsigned long long *d_iw = nullptr;
+    uint32_t *d_mult = nullptr;
+    std::vector<unsigned long long> iw(B);
+    for (int i = 0; i < B; ++i) iw[i] = mult ? mult[i] : 1u;
+    auto cleanup = [&]() {
+        dfree(d_planes);
+        dfree(d_iw);
+        dfree(d_mult);
+    };
+    if ((rc = dalloc(&d_planes, (size_t)B * q * W)) || (rc = dalloc(&d_iw, (size_t)B))) {
+        cleanup();
+        return rc;
+    }
+    if (mult) {
+        if ((rc = dalloc(&d_mult, (size_t)B))) {
+            cleanup();
+            return rc;
+        }
+        (void)hipMemcpy(d_mult, mult, (size_t)B * 4, hipMemcpyHostToDevice);
+    }
+    (void)hipMemcpy(d_iw, iw.data(), (size_t)B * 8, hipMemcpyHostToDevice);
+    hipEvent_t e0 = c->ev[6], e1 = c->ev[7];
+    (void)hipEventRecord(e0, c->stream);
+    sdca::kern::bitplanes(c->stream, c->d_states, B, L, q, W, d_planes);
+    sdca::kern::hamming_neighbours(c->stream, d_planes, B, q, W, L, dmax, d_mult, d_iw);
+    (void)hipEventRecord(e1, c->stream);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(iw.data(), d_iw, (size_t)B * 8, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(SDCA_E_HIP, "reweighting kernels failed: %s", hipGetErrorString(e));
+    c->timing.reweight += elapsed(e0, e1);
+    for (int i = 0; i < B; ++i) {
+        const double mi = mult ? (double)mult[i] : 1.0;
+        w_out[i] = mi / (double)iw[i];  // :95
+        if (int_w_out) int_w_out[i] = iw[i];
+    }
+    return SDCA_OK;
+}
+
+int sdca_set_weights(sdca_ctx *c, const double *w, double lambda_h_user, double lambda_J_user, double *out3)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!w) return fail(SDCA_E_ARG, "weights are null");
+    double beff = 0.0;
+    for (int i = 0; i < c->B; ++i) {
+        if (!std::isfinite(w[i]) || w[i] < 0.0) return fail(SDCA_E_NONFINITE, "weight %d is %g", i, w[i]);
+        beff += w[i];  // std::accumulate from 0.0, plmDCA_optimizer_parameters.hpp:261
+    }
+    const double auto_lambda = (beff > 500.0) ? 0.1 : 1.0 - ((1.0 - 0.1) * beff / 500.0);  // :263
+    c->lambda_J = (lambda_J_user < 0.0) ? auto_lambda * beff / 2.0 : lambda_J_user * beff;  // :265-274
+    c->lambda_h = (lambda_h_user < 0.0) ? auto_lambda * beff : lambda_h_user * beff;        // :276-284
+    c->B_eff = beff;
+    std::vector<float> wf(c->Bp, 0.f);
+    for (int i = 0; i < c->B; ++i) wf[i] = (float)w[i];
+    HIP_TRY(hipMemcpy(c->d_w, wf.data(), (size_t)c->Bp * 4, hipMemcpyHostToDevice));
+    c->have_weights = true;
+    if (out3) {
+        out3[0] = beff;
+        out3[1] = c->lambda_h;
+        out3[2] = c->lambda_J;
+    }
+    return SDCA_OK;
+}
+
+int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x, double *f_out, double *g_out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->have_weights) return fail(SDCA_E_STATE, "sdca_set_weights must be called first");
+    if (K < 0 || (K > 0 && (!loci || !x || !f_out || !g_out))) return fail(SDCA_E_ARG, "bad arguments");
+    for (int k = 0; k < K; ++k)
+        if (loci[k] < 0 || loci[k] >= c->L) return fail(SDCA_E_ARG, "locus %d out of range", loci[k]);
+    if (K == 0) return SDCA_OK;
+    const int nslots = std::min(c->opts.batch_loci, K);
+    if ((rc = ensure_workspace(c, nslots, c->opts.history))) return rc;
+    if (!c->d_X0 && (rc = dalloc(&c->d_X0, (size_t)nslots * c->n_slot))) return rc;
+    const int m = c->ws_m, P = 2 * m + 3;
+    const size_t n = (size_t)c->L * c->q * c->q + c->q;
+    std::vector<float> stage((size_t)nslots * c->n_slot), gdev(c->n_slot);
+    std::vector<double> unobs(nslots), res;
+    std::vector<sdca::SlotCmd> cmds(nslots, idle_cmd());
+    for (int k0 = 0; k0 < K; k0 += nslots) {
+        const int cnt = std::min(nslots, K - k0);
+        for (int i = 0; i < nslots; ++i) {
+            cmds[i] = idle_cmd();
+            if (i >= cnt) continue;
+            const int r = loci[k0 + i];
+            unobs[i] = ref_to_dev(c, r, x + (size_t)(k0 + i) * n, stage.data() + (size_t)i * c->n_slot);
+            cmds[i].mode = sdca::MODE_INIT;
+            cmds[i].locus = r;
+            cmds[i].gt = m + 1;
+            cmds[i].self_lo = c->col_off[r];
+            cmds[i].self_hi = c->col_off[r + 1];
+            cmds[i].init_src = 1.f;
+        }
+        HIP_TRY(hipMemcpy(c->d_X0, stage.data(), (size_t)cnt * c->n_slot * 4, hipMemcpyHostToDevice));
+        if ((rc = run_round(c, nslots, cmds, res))) return rc;
+        const sdca::ResLayout R{P};
+        for (int i = 0; i < cnt; ++i) {
+            const int r = loci[k0 + i];
+            const double *rr = res.data() + (size_t)i * R.width();
+            f_out[k0 + i] = rr[R.fdata()] + c->lambda_J * rr[R.xsqJ()] + c->lambda_h * rr[R.xsqh()] +
+                            c->lambda_J * unobs[i];
+            HIP_TRY(hipMemcpy(gdev.data(), c->d_ring + ((size_t)i * P + (m + 1)) * c->n_slot, c->n_slot * 4,
+                              hipMemcpyDeviceToHost));
+            double *g = g_out + (size_t)(k0 + i) * n;
+            dev_to_ref(c, r, gdev.data(), g);
+            // states never observed at j: gradient 2 lambda_J x (plmDCA_objective.hpp:217)
+            const double *xr = x + (size_t)(k0 + i) * n;
+            for (int j = 0; j < c->L; ++j) {
+                if (j == r) continue;
+                for (int a = 0; a < c->q; ++a) {
+                    if (c->state_col[(size_t)j * c->q + a] >= 0) continue;
+                    const size_t o = sdca::ref_index(c->L, c->q, j, a, 0);
+                    for (int s = 0; s < c->q; ++s) g[o + s] = 2.0 * c->lambda_J * xr[o + s];
+                }
+            }
+        }
+    }
+    return SDCA_OK;
+}
+
+int sdca_solve_loci(sdca_ctx *c, const int32_t *loci, int32_t K, const double *warm, float *score_rows,
+                    double *x_out, sdca_locus_stats *st)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!c->have_weights) return fail(SDCA_E_STATE, "sdca_set_weights must be called first");
+    if (K < 0 || (K > 0 && !loci)) return fail(SDCA_E_ARG, "bad arguments");
+    for (int k = 0; k < K; ++k)
+        if (loci[k] < 0 || loci[k] >= c->L) return fail(SDCA_E_ARG, "locus %d out of range", loci[k]);
+    if (K == 0) return SDCA_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int nslots = std::min(c->opts.batch_loci, K);
+    if ((rc = ensure_workspace(c, nslots, c->opts.history))) return rc;
+    if (warm && !c->d_X0 && (rc = dalloc(&c->d_X0, (size_t)nslots * c->n_slot))) return rc;
+    const int m = c->ws_m, P = 2 * m + 3;
+    const size_t n = (size_t)c->L * c->q * c->q + c->q;
+    const sdca::ResLayout R{P};
+    Solver S(c, nslots);
+    std::vector<float> stage(warm ? c->n_slot : 0), xdev(x_out ? c->n_slot : 0), rowbuf;
+    std::vector<double> res;
+    int next = 0, done = 0, noconv = 0, nonfinite = 0;
+    auto fill = [&](int i) -> int {
+        if (next >= K) {
+            S.set_idle(i);
+            return SDCA_OK;
+        }
+        const int item = next++;
+        const int r = loci[item];
+        if (warm) {
+            (void)ref_to_dev(c, r, warm + (size_t)item * n, stage.data());
+            HIP_TRY(hipMemcpy(c->d_X0 + (size_t)i * c->n_slot, stage.data(), c->n_slot * 4, hipMemcpyHostToDevice));
+        }
+        S.assign(i, item, r, warm != nullptr);
+        return SDCA_OK;
+    };
+    for (int i = 0; i < nslots; ++i)
+        if ((rc = fill(i))) return rc;
+    std::vector<int> finished;
+    std::vector<int64_t> src_off;
+    std::vector<int32_t> sloci;
+    int64_t rounds = 0;
+    while (done < K) {
+        if ((rc = run_round(c, nslots, S.cmds(), res))) return rc;
+        ++rounds;
+        finished.clear();
+        for (int i = 0; i < nslots; ++i) {
+            if (S.slot(i).phase == PH_IDLE) continue;
+            if (S.process(i, res.data() + (size_t)i * R.width())) finished.push_back(i);
+        }
+        if (finished.empty()) continue;
+        // score rows of the finished loci, read before their slots are refilled
+        src_off.clear();
+        sloci.clear();
+        for (int i : finished) {
+            const Slot &sl = S.slot(i);
+            src_off.push_back((sl.from_xt ? 0 : (int64_t)c->Mpad * c->Kxp) + (int64_t)i * c->n_slot);
+            sloci.push_back(sl.r);
+        }
+        const int nf = (int)finished.size();
+        HIP_TRY(hipMemcpyAsync(c->d_src_off, src_off.data(), nf * 8, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->d_score_loci, sloci.data(), nf * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipEventRecord(c->ev[6], c->stream));
+        sdca::kern::score(c->stream, c->d_Xtr, c->d_src_off, c->d_score_loci, nf, c->L, c->q, c->gap, (int)c->Kxp,
+                          c->d_state_col, c->d_rows);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c->ev[7], c->stream));
+        if (score_rows) {
+            for (int u = 0; u < nf; ++u)
+                HIP_TRY(hipMemcpyAsync(score_rows + (size_t)S.slot(finished[u]).item * c->L, c->d_rows + (size_t)u * c->L,
+                                       (size_t)c->L * 4, hipMemcpyDeviceToHost, c->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->timing.score += elapsed(c->ev[6], c->ev[7]);
+        for (int u = 0; u < nf; ++u) {
+            const int i = finished[u];
+            const Slot &sl = S.slot(i);
+            if (x_out) {
+                HIP_TRY(hipMemcpy(xdev.data(), c->d_Xtr + src_off[u], c->n_slot * 4, hipMemcpyDeviceToHost));
+                dev_to_ref(c, sl.r, xdev.data(), x_out + (size_t)sl.item * n);
+            }
+            if (st) {
+                sdca_locus_stats &o = st[sl.item];
+                o.fval = sl.f;
+                o.gnorm = sl.gnorm;
+                o.nfeval = sl.nfeval;
+                o.iter = sl.iter;
+                o.status = sl.status;
+                o.reserved = 0;
+            }
+            noconv += (sl.status == SDCA_ST_ITER_CAP || sl.status == SDCA_ST_LS_FAIL);
+            nonfinite += (sl.status == SDCA_ST_NONFINITE);
+            ++done;
+            if ((rc = fill(i))) return rc;
+        }
+        if (c->opts.verbose)
+            fprintf(stderr, "sdca: %d / %d loci solved after %lld rounds\n", done, K, (long long)rounds);
+    }
+    c->timing.total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (nonfinite) return fail(SDCA_E_NONFINITE, "%d loci reached a non-finite objective", nonfinite);
+    if (noconv) return fail(SDCA_E_NOCONV, "%d loci stopped without reaching the gradient threshold", noconv);
+    return SDCA_OK;
+}
+
+int sdca_get_timing(sdca_ctx *c, sdca_timing *t, int32_t reset)
+{
+    if (!c || !t) return fail(SDCA_E_ARG, "null argument");
+    const auto &s = c->timing;
+    t->gemm_energy_ms = s.energy;
+    t->gemm_grad_ms = s.grad;
+    t->softmax_ms = s.softmax;
+    t->lbfgs_ms = s.lbfgs;
+    t->score_ms = s.score;
+    t->reweight_ms = s.reweight;
+    t->total_ms = s.total;
+    t->n_evals = s.evals;
+    t->n_energy_launches = s.energy_launches;
+    t->n_grad_launches = s.grad_launches;
+    t->flops_energy = s.flops_energy;
+    t->flops_grad = s.flops_grad;
+    t->bytes_lbfgs = s.bytes_lbfgs;
+    if (reset) c->timing = sdca::Timing();
+    return SDCA_OK;
+}
+
+}  // extern "C"

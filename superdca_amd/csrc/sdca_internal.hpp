@@ -1,0 +1,133 @@
+// sdca_internal.hpp -- shared definitions of the MI355X plmDCA engine (libsdca.so).
+//
+// Device data layout (one context = one alignment on one GPU):
+//   states   B x L   uint8, envelope states (row-major, as given by the caller)
+//   statesT  L x B   uint8, transposed copy: the column-r target states are contiguous
+//   X        Bp x Kxp  one-hot alignment (FP32 or BF16), column col(j,a) = col_off[j] +
+//            local index of state a among the states observed at locus j; column Kx is
+//            the constant 1 ("bias" column) whose weights are the fields h_r; pad rows
+//            b >= B and pad columns are zero
+//   Xt       Kxp x Bp  its transpose (the gradient contraction reads X with b contiguous)
+//   Per batch slot i (one locus r in flight) every L-BFGS vector is q x Kxp floats:
+//   row s (state at r) and column k of X, i.e. W_r[s][col(j,a)] = J_r[j](a,s) and
+//   W_r[s][Kx] = h_r(s).  Entries of the locus' own columns and of pad columns stay 0.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sdca.h"
+
+namespace sdca {
+
+constexpr int kTile = 128;        // GEMM output tile (rows and columns)
+constexpr int kMaxHistory = 16;   // largest L-BFGS history supported
+constexpr int kMaxPhys = 2 * kMaxHistory + 3;  // D ring (m+1) + G ring (m+2)
+
+// Per-slot command of one batched evaluation round (host -> device).
+enum SlotMode : int32_t { MODE_IDLE = 0, MODE_INIT = 1, MODE_RETRY = 2, MODE_NEWDIR = 3 };
+
+struct SlotCmd {
+    int32_t mode;     // SlotMode
+    int32_t dst;      // ring index of the direction (RETRY: read, NEWDIR: written)
+    int32_t gt;       // ring index that receives the trial gradient
+    int32_t locus;    // locus r of the slot, -1 if idle
+    int32_t self_lo;  // columns [self_lo, self_hi) belong to locus r: gradient masked to 0
+    int32_t self_hi;
+    float alpha;      // step length of the trial point x_t = x_c + alpha * d
+    float init_src;   // MODE_INIT: 1 = copy the staged start point, 0 = zero start
+    float coef[kMaxPhys];  // NEWDIR: d = sum_p coef[p] * ring[p]
+};
+
+// Layout of the per-slot result row (device -> host), doubles.
+struct ResLayout {
+    int P;
+    int gdots() const { return 0; }          // [0, P): g_t . ring[p]
+    int ginf() const { return P; }           // max |g_t|
+    int ddots() const { return P + 1; }      // [P+1, 2P+1): d . ring[p] (NEWDIR)
+    int xsqJ() const { return 2 * P + 1; }   // sum of squares of x_t, J part
+    int xsqh() const { return 2 * P + 2; }   // sum of squares of x_t, h part
+    int fdata() const { return 2 * P + 3; }  // sum_b w_b (LSE_b - E_b(x_br))
+    int width() const { return 2 * P + 4; }
+};
+
+struct Timing {
+    double energy = 0, grad = 0, softmax = 0, lbfgs = 0, score = 0, reweight = 0, total = 0;
+    int64_t evals = 0, energy_launches = 0, grad_launches = 0;
+    double flops_energy = 0, flops_grad = 0, bytes_lbfgs = 0;
+};
+
+// Kernel launchers (sdca_kernels.hip).
+namespace kern {
+void build_x(hipStream_t st, const uint8_t *d_states, int B, int L, int q, const int32_t *d_state_col,
+             int Kx, int Bp, int Kxp, int prec, void *X, void *Xt);
+void bitplanes(hipStream_t st, const uint8_t *d_states, int B, int L, int q, int W, uint64_t *planes);
+void hamming_neighbours(hipStream_t st, const uint64_t *planes, int B, int q, int W, int L, int64_t dmax,
+                        const uint32_t *d_mult, unsigned long long *d_iw);
+void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const void *X,
+                 int Mpad, int Bp, int Kxp, int ksplit, int kchunk, float *Epart);
+void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad, int Bp, int B, int q, int nslots,
+             const SlotCmd *cmds, const uint8_t *statesT, const float *wts, float *G32, uint16_t *Ghi,
+             uint16_t *Glo, double *fpart, int nbchunk);
+void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const void *Xt,
+               int Mpad, int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial,
+               float *ring, int64_t n_slot, int P, float lamJ2, float lamh2);
+void update(hipStream_t st, int m, int prec, const SlotCmd *cmds, int nslots, float *ring, float *Xc, float *Xtr,
+            uint16_t *Whi, uint16_t *Wlo, const float *X0, int64_t n_slot, int Kxp, int Kx, double *part,
+            int nchunk);
+void dots(hipStream_t st, int m, const SlotCmd *cmds, int nslots, const float *ring, int64_t n_slot, double *part,
+          int nchunk);
+void reduce_results(hipStream_t st, int m, int nslots, const double *upd_part, const double *dot_part,
+                    const double *sm_part, int nchunk, int nbchunk, double *res);
+void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *loci, int nscore, int L,
+           int q, int gap, int Kxp, const int32_t *state_col, float *rows);
+}  // namespace kern
+
+}  // namespace sdca
+
+struct sdca_ctx {
+    sdca_opts opts{};
+    int device = 0;
+    int B = 0, L = 0, q = 0, gap = -1;
+    int Bp = 0;       // B padded to kTile
+    int64_t Kx = 0;   // observed-state columns
+    int64_t Kxp = 0;  // Kx + 1 padded to kTile
+    std::vector<int32_t> col_off;    // L + 1
+    std::vector<int32_t> state_col;  // L x q: column of (j, a) or -1
+    std::vector<uint8_t> h_states;   // B x L copy (host)
+    hipStream_t stream = nullptr;
+
+    uint8_t *d_states = nullptr, *d_statesT = nullptr;
+    int32_t *d_state_col = nullptr;
+    void *d_X = nullptr, *d_Xt = nullptr;
+    float *d_w = nullptr;
+
+    bool have_weights = false;
+    double B_eff = 0, lambda_h = 0, lambda_J = 0;
+
+    // batch workspace, sized for nslots slots and history m
+    int ws_slots = 0, ws_m = 0;
+    int64_t n_slot = 0;
+    int Mpad = 0;
+    int ksplit = 1, kchunk = 0;
+    int nchunk = 0, nbchunk = 0;
+    float *d_ring = nullptr, *d_Xc = nullptr, *d_Xtr = nullptr, *d_X0 = nullptr;
+    uint16_t *d_Whi = nullptr, *d_Wlo = nullptr;
+    float *d_G32 = nullptr;
+    uint16_t *d_Ghi = nullptr, *d_Glo = nullptr;
+    float *d_Epart = nullptr;
+    double *d_upd_part = nullptr, *d_dot_part = nullptr, *d_sm_part = nullptr, *d_res = nullptr;
+    sdca::SlotCmd *d_cmds = nullptr;
+    float *d_rows = nullptr;
+    int64_t *d_src_off = nullptr;
+    int32_t *d_score_loci = nullptr;
+
+    hipEvent_t ev[8] = {};
+    sdca::Timing timing;
+
+    ~sdca_ctx();
+    void free_workspace();
+};

@@ -1,0 +1,735 @@
+// sdca_kernels.hip -- gfx950 kernels of the batched plmDCA solve.
+//
+// The per-locus asymmetric pseudolikelihood of SuperDCA
+// (SuperDCA/include/plmDCA_objective.hpp:38-249) is evaluated for a batch of loci as
+// two dense contractions over the shared one-hot alignment X:
+//   energies   E^T[r*q+s][b] = sum_k W[r*q+s][k] X[b][k]          (gemm_energy)
+//   gradient   dW[r*q+s][k]  = sum_b G^T[r*q+s][b] X^T[k][b] + 2 lambda W   (gemm_grad)
+// with G = w (softmax(E) - onehot(x_r)) from the softmax kernel.  Both contractions are
+// "NT" products (both operands contiguous along the reduction index) on 32x32 MFMA
+// tiles: FP32 (v_mfma_f32_32x32x2f32) or BF16 (v_mfma_f32_32x32x16_bf16) with W or G
+// split into hi + lo BF16 parts accumulated in FP32.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "sdca_internal.hpp"
+
+namespace sdca {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                                            // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ---------------------------------------------------------------- one-hot X
+
+// X[b][state_col[j*q + x_bj]] = 1 and the bias column X[b][Kx] = 1 (b < B);
+// the caller zero-fills X and Xt first.
+template <typename T>
+__global__ void k_build_x(const uint8_t *__restrict__ states, int B, int L, int q, const int32_t *__restrict__ state_col,
+                          int Kx, int Bp, int64_t Kxp, T *__restrict__ X, T *__restrict__ Xt, T one)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)B * (L + 1);
+    if (idx >= total) return;
+    const int b = (int)(idx / (L + 1));
+    const int j = (int)(idx - (int64_t)b * (L + 1));
+    int64_t col;
+    if (j == L) {
+        col = Kx;
+    } else {
+        const int a = states[(int64_t)b * L + j];
+        col = state_col[(int64_t)j * q + a];
+    }
+    X[(int64_t)b * Kxp + col] = one;
+    Xt[col * Bp + b] = one;
+}
+
+// ------------------------------------------------------------ reweighting
+
+// planes[b][v][w] bit t set iff states[b][64*w + t] == v.
+__global__ void k_bitplanes(const uint8_t *__restrict__ states, int B, int L, int q, int W, uint64_t *__restrict__ planes)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)B * W) return;
+    const int b = (int)(idx / W);
+    const int w = (int)(idx - (int64_t)b * W);
+    uint64_t word[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int l0 = w * 64;
+    const int l1 = min(L, l0 + 64);
+    for (int l = l0; l < l1; ++l) {
+        const int v = states[(int64_t)b * L + l];
+        for (int u = 0; u < 8; ++u)
+            if (u == v) word[u] |= 1ull << (l - l0);
+    }
+    for (int v = 0; v < q; ++v) planes[((int64_t)b * q + v) * W + w] = word[v];
+}
+
+// Lower-triangular 64x64 tiles of sequence pairs; each thread owns 4x4 pairs.
+// matches(i,j) = sum_v popcount(P_i^v & P_j^v); d = L - matches; the pair is a
+// neighbour iff d <= dmax, where dmax is the largest d with
+// double(L - d) / double(L) >= theta (computed on the host with that expression),
+// so the comparison is exactly the one of plmDCA_optimizer_parameters.hpp:67-72.
+constexpr int HT = 64;   // sequences per tile side
+constexpr int HWC = 4;   // words per LDS chunk
+__global__ __launch_bounds__(256) void k_hamming(const uint64_t *__restrict__ planes, int B, int q, int W, int L,
+                                                 int64_t dmax, const uint32_t *__restrict__ mult,
+                                                 unsigned long long *__restrict__ iw)
+{
+    __shared__ uint64_t sI[HT][8][HWC];
+    __shared__ uint64_t sJ[HT][8][HWC];
+    // blockIdx.x -> (ti, tj), ti >= tj
+    const int64_t t = blockIdx.x;
+    int ti = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(ti + 1) * (ti + 2) / 2 <= t) ++ti;
+    while ((int64_t)ti * (ti + 1) / 2 > t) --ti;
+    const int tj = (int)(t - (int64_t)ti * (ti + 1) / 2);
+    const int i0 = ti * HT, j0 = tj * HT;
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    int matches[4][4];
+    for (int a = 0; a < 4; ++a)
+        for (int c = 0; c < 4; ++c) matches[a][c] = 0;
+    for (int w0 = 0; w0 < W; w0 += HWC) {
+        __syncthreads();
+        for (int e = tid; e < HT * q * HWC; e += 256) {
+            const int s = e / (q * HWC);
+            const int rem = e - s * q * HWC;
+            const int v = rem / HWC;
+            const int ww = rem - v * HWC;
+            const int w = w0 + ww;
+            const int gi = i0 + s, gj = j0 + s;
+            sI[s][v][ww] = (gi < B && w < W) ? planes[((int64_t)gi * q + v) * W + w] : 0ull;
+            sJ[s][v][ww] = (gj < B && w < W) ? planes[((int64_t)gj * q + v) * W + w] : 0ull;
+        }
+        __syncthreads();
+        for (int v = 0; v < q; ++v)
+            for (int ww = 0; ww < HWC; ++ww) {
+                uint64_t pi[4], pj[4];
+                for (int a = 0; a < 4; ++a) pi[a] = sI[ty * 4 + a][v][ww];
+                for (int c = 0; c < 4; ++c) pj[c] = sJ[tx * 4 + c][v][ww];
+                for (int a = 0; a < 4; ++a)
+                    for (int c = 0; c < 4; ++c) matches[a][c] += __popcll(pi[a] & pj[c]);
+            }
+    }
+    unsigned long long addI[4] = {0, 0, 0, 0}, addJ[4] = {0, 0, 0, 0};
+    for (int a = 0; a < 4; ++a)
+        for (int c = 0; c < 4; ++c) {
+            const int gi = i0 + ty * 4 + a, gj = j0 + tx * 4 + c;
+            if (gi < B && gj < B && gi > gj) {
+                const int64_t d = (int64_t)L - matches[a][c];
+                if (d <= dmax) {
+                    addI[a] += mult ? mult[gj] : 1u;
+                    addJ[c] += mult ? mult[gi] : 1u;
+                }
+            }
+        }
+    for (int a = 0; a < 4; ++a) {
+        const int gi = i0 + ty * 4 + a;
+        if (addI[a] && gi < B) atomicAdd(&iw[gi], addI[a]);
+    }
+    for (int c = 0; c < 4; ++c) {
+        const int gj = j0 + tx * 4 + c;
+        if (addJ[c] && gj < B) atomicAdd(&iw[gj], addJ[c]);
+    }
+}
+
+// ------------------------------------------------------------------- GEMM
+
+// C[m][n] = sum_k A[m][k] * Bm[n][k], 128x128 tile per 256-thread workgroup, four
+// waves of 64x64 (2x2 MFMA 32x32 blocks), K staged through LDS in rows of 128 bytes.
+// NA = 2 sums two A variants (hi and lo BF16 parts) against the same B fragment.
+constexpr int ROWB = 144;  // LDS bytes per staged row (128 data + 16 pad)
+
+struct PolBF16 {
+    using T = uint16_t;
+    static constexpr int BK = 64;   // elements per staged row
+    static constexpr int KPM = 16;  // K per MFMA
+};
+struct PolF32 {
+    using T = float;
+    static constexpr int BK = 32;
+    static constexpr int KPM = 2;
+};
+
+template <class Pol, int NA>
+__device__ __forceinline__ void mma_tile(const uint8_t *sA0, const uint8_t *sA1, const uint8_t *sB, int wm, int wn,
+                                         int lane, floatx16 (&acc)[2][2])
+{
+    if constexpr (sizeof(typename Pol::T) == 2) {
+#pragma unroll
+        for (int kk = 0; kk < Pol::BK; kk += Pol::KPM) {
+            const int kb = (kk + (lane >> 5) * 8) * 2;
+            bf16x8 a[NA][2], b[2];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi) {
+                const int row = wm + mi * 32 + (lane & 31);
+                a[0][mi] = *reinterpret_cast<const bf16x8 *>(sA0 + row * ROWB + kb);
+                if constexpr (NA == 2) a[NA - 1][mi] = *reinterpret_cast<const bf16x8 *>(sA1 + row * ROWB + kb);
+            }
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni) {
+                const int row = wn + ni * 32 + (lane & 31);
+                b[ni] = *reinterpret_cast<const bf16x8 *>(sB + row * ROWB + kb);
+            }
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+                    for (int v = 0; v < NA; ++v)
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[v][mi], b[ni], acc[mi][ni], 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+        for (int kk = 0; kk < Pol::BK; kk += Pol::KPM) {
+            const int kb = (kk + (lane >> 5)) * 4;
+            float a[NA][2], b[2];
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi) {
+                const int row = wm + mi * 32 + (lane & 31);
+                a[0][mi] = *reinterpret_cast<const float *>(sA0 + row * ROWB + kb);
+                if constexpr (NA == 2) a[NA - 1][mi] = *reinterpret_cast<const float *>(sA1 + row * ROWB + kb);
+            }
+#pragma unroll
+            for (int ni = 0; ni < 2; ++ni) {
+                const int row = wn + ni * 32 + (lane & 31);
+                b[ni] = *reinterpret_cast<const float *>(sB + row * ROWB + kb);
+            }
+#pragma unroll
+            for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+                for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+                    for (int v = 0; v < NA; ++v)
+                        acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[v][mi], b[ni], acc[mi][ni], 0, 0, 0);
+        }
+    }
+}
+
+template <class Pol, int NA, class Epi>
+__global__ __launch_bounds__(256) void k_gemm_nt(const typename Pol::T *__restrict__ A0,
+                                                    const typename Pol::T *__restrict__ A1, int64_t lda,
+                                                    const typename Pol::T *__restrict__ Bm, int64_t ldb, int K,
+                                                    int kchunk, Epi epi)
+{
+    using T = typename Pol::T;
+    __shared__ __attribute__((aligned(16))) uint8_t sA[NA][kTile * ROWB];
+    __shared__ __attribute__((aligned(16))) uint8_t sB[kTile * ROWB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t m0 = (int64_t)blockIdx.y * kTile, n0 = (int64_t)blockIdx.x * kTile;
+    const int kb = blockIdx.z * kchunk;
+    const int ke = min(K, kb + kchunk);
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0.f;
+    const int lrow = tid >> 3;         // 0..31
+    const int lcol = (tid & 7) * 16;   // byte offset in the 128-byte row
+    for (int k0 = kb; k0 < ke; k0 += Pol::BK) {
+        u32x4 ra[NA][4], rb[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t row = lrow + 32 * i;
+            ra[0][i] = *reinterpret_cast<const u32x4 *>(reinterpret_cast<const uint8_t *>(A0 + (m0 + row) * lda + k0) + lcol);
+            if constexpr (NA == 2)
+                ra[NA - 1][i] =
+                    *reinterpret_cast<const u32x4 *>(reinterpret_cast<const uint8_t *>(A1 + (m0 + row) * lda + k0) + lcol);
+            rb[i] = *reinterpret_cast<const u32x4 *>(reinterpret_cast<const uint8_t *>(Bm + (n0 + row) * ldb + k0) + lcol);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = lrow + 32 * i;
+#pragma unroll
+            for (int v = 0; v < NA; ++v) *reinterpret_cast<u32x4 *>(&sA[v][row * ROWB + lcol]) = ra[v][i];
+            *reinterpret_cast<u32x4 *>(&sB[row * ROWB + lcol]) = rb[i];
+        }
+        __syncthreads();
+        mma_tile<Pol, NA>(sA[0], sA[NA - 1], sB, wm, wn, lane, acc);
+    }
+    // 32x32 MFMA result layout: register v of lane l holds row 8*(v/4) + 4*(l/32) + v%4,
+    // column l%32 of the block.
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < 2; ++ni)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int64_t m = m0 + wm + mi * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int64_t n = n0 + wn + ni * 32 + (lane & 31);
+                epi(m, n, acc[mi][ni][v]);
+            }
+}
+
+// Energy epilogue: split-K partial sums, Epart[z][m][n].
+struct EpiEnergy {
+    float *E;
+    int64_t ld, zstride;
+    __device__ __forceinline__ void operator()(int64_t m, int64_t n, float v) const
+    {
+        E[(int64_t)blockIdx.z * zstride + m * ld + n] = v;
+    }
+};
+
+// Gradient epilogue: row m = slot*q + s, column k.  Adds 2*lambda*x (lambda_h on the
+// bias column, lambda_J elsewhere: plmDCA_objective.hpp:192-244), zeroes the locus'
+// own columns (the j != r exclusions of :95,111,161,176) and pad columns, and writes
+// the slot's trial-gradient ring vector.
+struct EpiGrad {
+    float *ring;
+    const float *Xtrial;
+    const SlotCmd *cmds;
+    int64_t n_slot, Kxp;
+    int P, q, Kx, nslots;
+    float lamJ2, lamh2;
+    __device__ __forceinline__ void operator()(int64_t m, int64_t n, float v) const
+    {
+        const int i = (int)(m / q);
+        if (i >= nslots) return;
+        const SlotCmd &c = cmds[i];
+        if (c.mode == MODE_IDLE) return;
+        const int s = (int)(m - (int64_t)i * q);
+        float val = 0.f;
+        if (n <= Kx && !(n >= c.self_lo && n < c.self_hi))
+            val = v + (n == Kx ? lamh2 : lamJ2) * Xtrial[m * Kxp + n];
+        ring[((int64_t)i * P + c.gt) * n_slot + (int64_t)s * Kxp + n] = val;
+    }
+};
+
+// ---------------------------------------------------------------- softmax
+
+// One thread per (slot, sequence b): sums the split-K partial energies, forms the
+// log-partition lz = log sum_s exp E_b(s) (max-shifted; the same value as the
+// unshifted plmDCA_objective.hpp:140), the data term w_b (lz - E_b(x_br)) (:143) and the
+// node beliefs G_b(s) = w_b P_b(s) - w_b [s == x_br] (:147-148), with 1 - P(x_br)
+// formed as the sum of the other probabilities.  The data term is reduced in FP64.
+__global__ __launch_bounds__(256) void k_softmax(const float *__restrict__ Epart, int ksplit, int64_t zstride, int Bp,
+                                                 int B, int q, int nslots, const SlotCmd *__restrict__ cmds,
+                                                 const uint8_t *__restrict__ statesT, const float *__restrict__ wts,
+                                                 float *__restrict__ G32, uint16_t *__restrict__ Ghi,
+                                                 uint16_t *__restrict__ Glo, double *__restrict__ fpart, int nbchunk)
+{
+    const int i = blockIdx.y;
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    const int r = cmds[i].mode == MODE_IDLE ? -1 : cmds[i].locus;
+    double fb = 0.0;
+    if (b < Bp) {
+        float e[8], g[8];
+        if (r >= 0) {
+            for (int s = 0; s < q; ++s) {
+                float acc = 0.f;
+                for (int z = 0; z < ksplit; ++z) acc += Epart[(int64_t)z * zstride + (int64_t)(i * q + s) * Bp + b];
+                e[s] = acc;
+            }
+            const float w = wts[b];
+            const int xr = b < B ? statesT[(int64_t)r * B + b] : 0;
+            float mx = e[0];
+            for (int s = 1; s < q; ++s) mx = fmaxf(mx, e[s]);
+            float z = 0.f;
+            for (int s = 0; s < q; ++s) z += __expf(e[s] - mx);
+            const float lz = mx + __logf(z);
+            fb = (double)w * ((double)lz - (double)e[xr]);
+            float other = 0.f;
+            for (int s = 0; s < q; ++s) {
+                const float p = __expf(e[s] - lz);
+                g[s] = w * p;
+                if (s != xr) other += p;
+            }
+            g[xr] = -w * other;
+        } else {
+            for (int s = 0; s < q; ++s) g[s] = 0.f;
+        }
+        for (int s = 0; s < q; ++s) {
+            const int64_t o = (int64_t)(i * q + s) * Bp + b;
+            if (G32) {
+                G32[o] = g[s];
+            } else {
+                const uint16_t h = f2bf(g[s]);
+                Ghi[o] = h;
+                Glo[o] = f2bf(g[s] - bf2f(h));
+            }
+        }
+    }
+    // block reduction of fb
+    __shared__ double red[4];
+    for (int off = 32; off > 0; off >>= 1) fb += __shfl_down(fb, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = fb;
+    __syncthreads();
+    if (threadIdx.x == 0) fpart[(int64_t)i * nbchunk + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// ------------------------------------------------------ L-BFGS vector passes
+
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *out, const bool (&is_max)[NV])
+{
+    __shared__ double red[4][NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        double x = v[k];
+        for (int off = 32; off > 0; off >>= 1) {
+            const double y = __shfl_down(x, off, 64);
+            x = is_max[k] ? fmax(x, y) : x + y;
+        }
+        v[k] = x;
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[wave][k] = v[k];
+    __syncthreads();
+    for (int k = threadIdx.x; k < NV; k += blockDim.x)
+        out[k] = is_max[k] ? fmax(fmax(red[0][k], red[1][k]), fmax(red[2][k], red[3][k]))
+                           : (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+}
+
+__device__ __forceinline__ double dot4(float4 a, float4 b)
+{
+    return (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z + (double)a.w * b.w;
+}
+
+// Trial point of every slot (one launch per round):
+//   INIT    x_c = x0 (staged) or 0,           x_t = x_c
+//   RETRY   x_t = x_c + alpha d                (d = ring[dst])
+//   NEWDIR  x_c = x_t (accept), d = sum_p coef[p] ring[p] -> ring[dst], x_t = x_c + alpha d,
+//           and the dot products d . ring[p] (the Gram row of the new direction)
+// plus the squared norms of x_t (J and h parts) for the L2 terms.
+// part[slot][chunk][P + 3] = {d . ring[p]} , xsqJ, xsqh.
+template <int M>
+__global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds, float *__restrict__ ring,
+                                                float *__restrict__ Xc, float *__restrict__ Xtr,
+                                                uint16_t *__restrict__ Whi, uint16_t *__restrict__ Wlo,
+                                                const float *__restrict__ X0, int64_t n_slot, int Kxp, int Kx,
+                                                double *__restrict__ part, int nchunk)
+{
+    constexpr int P = 2 * M + 3;
+    constexpr int NV = P + 2;
+    const int i = blockIdx.y;
+    const SlotCmd &c = cmds[i];
+    double acc[NV];
+    bool is_max[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        acc[k] = 0.0;
+        is_max[k] = false;
+    }
+    const int mode = c.mode;
+    if (mode != MODE_IDLE) {
+        float *rb = ring + (int64_t)i * P * n_slot;
+        float *xc = Xc + (int64_t)i * n_slot;
+        float *xt = Xtr + (int64_t)i * n_slot;
+        const float *x0 = X0 ? X0 + (int64_t)i * n_slot : nullptr;
+        const float alpha = c.alpha;
+        const int dst = c.dst;
+        const int64_t n4 = n_slot >> 2;
+        for (int64_t e4 = (int64_t)blockIdx.x * 256 + threadIdx.x; e4 < n4; e4 += (int64_t)nchunk * 256) {
+            const int64_t e = e4 * 4;
+            float4 xn;
+            if (mode == MODE_INIT) {
+                xn = (c.init_src != 0.f && x0) ? *reinterpret_cast<const float4 *>(x0 + e) : make_float4(0.f, 0.f, 0.f, 0.f);
+                *reinterpret_cast<float4 *>(xc + e) = xn;
+            } else if (mode == MODE_RETRY) {
+                const float4 xv = *reinterpret_cast<const float4 *>(xc + e);
+                const float4 d = *reinterpret_cast<const float4 *>(rb + (int64_t)dst * n_slot + e);
+                xn = make_float4(xv.x + alpha * d.x, xv.y + alpha * d.y, xv.z + alpha * d.z, xv.w + alpha * d.w);
+            } else {  // NEWDIR
+                const float4 xv = *reinterpret_cast<const float4 *>(xt + e);
+                *reinterpret_cast<float4 *>(xc + e) = xv;
+                float4 v[P];
+                float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    v[p] = *reinterpret_cast<const float4 *>(rb + (int64_t)p * n_slot + e);
+                    const float cf = c.coef[p];
+                    if (cf != 0.f) {  // unused ring entries may hold stale values
+                        d.x += cf * v[p].x;
+                        d.y += cf * v[p].y;
+                        d.z += cf * v[p].z;
+                        d.w += cf * v[p].w;
+                    }
+                }
+                *reinterpret_cast<float4 *>(rb + (int64_t)dst * n_slot + e) = d;
+#pragma unroll
+                for (int p = 0; p < P; ++p) acc[p] += dot4(d, p == dst ? d : v[p]);
+                xn = make_float4(xv.x + alpha * d.x, xv.y + alpha * d.y, xv.z + alpha * d.z, xv.w + alpha * d.w);
+            }
+            *reinterpret_cast<float4 *>(xt + e) = xn;
+            if (Whi) {
+                const float xs[4] = {xn.x, xn.y, xn.z, xn.w};
+                ushort4 hi, lo;
+                uint16_t h[4], l[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    h[u] = f2bf(xs[u]);
+                    l[u] = f2bf(xs[u] - bf2f(h[u]));
+                }
+                hi = make_ushort4(h[0], h[1], h[2], h[3]);
+                lo = make_ushort4(l[0], l[1], l[2], l[3]);
+                *reinterpret_cast<ushort4 *>(Whi + (int64_t)i * n_slot + e) = hi;
+                *reinterpret_cast<ushort4 *>(Wlo + (int64_t)i * n_slot + e) = lo;
+            }
+            // column of each element: e % Kxp (Kxp is a multiple of 4, so all four share the row)
+            const int64_t col = e % Kxp;
+            const float xs[4] = {xn.x, xn.y, xn.z, xn.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double sq = (double)xs[u] * xs[u];
+                if (col + u == Kx) acc[P + 1] += sq;
+                else acc[P] += sq;
+            }
+        }
+    }
+    block_reduce_store<NV>(acc, part + ((int64_t)i * nchunk + blockIdx.x) * NV, is_max);
+}
+
+// Dot products of each active slot's trial gradient ring[gt] with every ring vector,
+// and its infinity norm: part[slot][chunk][P + 1].
+template <int M>
+__global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, const float *__restrict__ ring,
+                                              int64_t n_slot, double *__restrict__ part, int nchunk)
+{
+    constexpr int P = 2 * M + 3;
+    constexpr int NV = P + 1;
+    const int i = blockIdx.y;
+    const SlotCmd &c = cmds[i];
+    double acc[NV];
+    bool is_max[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        acc[k] = 0.0;
+        is_max[k] = (k == P);
+    }
+    if (c.mode != MODE_IDLE) {
+        const float *rb = ring + (int64_t)i * P * n_slot;
+        const float *g = rb + (int64_t)c.gt * n_slot;
+        const int64_t n4 = n_slot >> 2;
+        for (int64_t e4 = (int64_t)blockIdx.x * 256 + threadIdx.x; e4 < n4; e4 += (int64_t)nchunk * 256) {
+            const int64_t e = e4 * 4;
+            const float4 gv = *reinterpret_cast<const float4 *>(g + e);
+            acc[P] = fmax(acc[P], (double)fmaxf(fmaxf(fabsf(gv.x), fabsf(gv.y)), fmaxf(fabsf(gv.z), fabsf(gv.w))));
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const float4 v = *reinterpret_cast<const float4 *>(rb + (int64_t)p * n_slot + e);
+                acc[p] += dot4(gv, v);
+            }
+        }
+    }
+    block_reduce_store<NV>(acc, part + ((int64_t)i * nchunk + blockIdx.x) * NV, is_max);
+}
+
+// Per-slot reduction of the chunk partials into the result row (ResLayout).
+__global__ void k_reduce(int P, const double *__restrict__ upd_part, const double *__restrict__ dot_part,
+                         const double *__restrict__ sm_part, int nchunk, int nbchunk, double *__restrict__ res)
+{
+    const int i = blockIdx.x;
+    const int t = threadIdx.x;
+    const int W = 2 * P + 4;
+    double *out = res + (int64_t)i * W;
+    if (t < P + 1) {  // dots kernel: gdots then ginf
+        double a = 0.0;
+        for (int ch = 0; ch < nchunk; ++ch) {
+            const double v = dot_part[((int64_t)i * nchunk + ch) * (P + 1) + t];
+            a = (t == P) ? fmax(a, v) : a + v;
+        }
+        out[t] = a;
+    } else if (t < 2 * P + 3) {  // update kernel: ddots, xsqJ, xsqh
+        const int k = t - (P + 1);
+        double a = 0.0;
+        for (int ch = 0; ch < nchunk; ++ch) a += upd_part[((int64_t)i * nchunk + ch) * (P + 2) + k];
+        out[t] = a;
+    } else if (t == 2 * P + 3) {
+        double a = 0.0;
+        for (int ch = 0; ch < nbchunk; ++ch) a += sm_part[(int64_t)i * nbchunk + ch];
+        out[t] = a;
+    }
+}
+
+// ---------------------------------------------------------------- scoring
+
+// Score row of a solved locus r (plmDCA.hpp:378-392): for n != r the block
+// M[a][s] = J_r[n](a,s) (zero for states a not observed at n), the gauge
+// G[i][k] = ((M[i][k] - rowmean[k]) - colmean[i]) + mean(rowmean) of
+// plmDCA_utility.hpp:57-83 (on M^T for n < r), and the Frobenius norm over
+// i, k != gap (Matrix_math.hpp:134-178), accumulated in FP64 and stored as float.
+__global__ __launch_bounds__(256) void k_score(const float *__restrict__ xsrc, const int64_t *__restrict__ src_off,
+                                               const int32_t *__restrict__ loci, int L, int q, int gap, int Kxp,
+                                               const int32_t *__restrict__ state_col, float *__restrict__ rows)
+{
+    const int k = blockIdx.y;
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= L) return;
+    const int r = loci[k];
+    float *row = rows + (int64_t)k * L;
+    if (n == r) {
+        row[n] = 0.f;
+        return;
+    }
+    const float *x = xsrc + src_off[k];
+    double A[8][8];
+    const bool tr = n < r;
+    for (int a = 0; a < q; ++a) {
+        const int col = state_col[(int64_t)n * q + a];
+        for (int s = 0; s < q; ++s) {
+            const double v = col >= 0 ? (double)x[(int64_t)s * Kxp + col] : 0.0;
+            if (tr) A[s][a] = v;
+            else A[a][s] = v;
+        }
+    }
+    double m1[8], m2[8];
+    double mm = 0.0;
+    for (int a = 0; a < q; ++a) {
+        double t = 0.0;
+        for (int b2 = 0; b2 < q; ++b2) t = t + A[a][b2];
+        m1[a] = t / (double)q;
+    }
+    for (int a = 0; a < q; ++a) mm += m1[a];
+    mm = mm / (double)q;
+    for (int a = 0; a < q; ++a) {
+        double t = 0.0;
+        for (int b2 = 0; b2 < q; ++b2) t = t + A[b2][a];
+        m2[a] = t / (double)q;
+    }
+    double sum = 0.0;
+    for (int a = 0; a < q; ++a) {
+        if (a == gap) continue;
+        for (int s = 0; s < q; ++s) {
+            if (s == gap) continue;
+            const double g = ((A[a][s] - m1[s]) - m2[a]) + mm;
+            sum += g * g;
+        }
+    }
+    row[n] = (float)sqrt(sum);
+}
+
+}  // namespace
+
+// ============================================================== launchers
+
+namespace kern {
+
+static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+void build_x(hipStream_t st, const uint8_t *d_states, int B, int L, int q, const int32_t *d_state_col, int Kx,
+             int Bp, int Kxp, int prec, void *X, void *Xt)
+{
+    const int64_t total = (int64_t)B * (L + 1);
+    if (prec == SDCA_PREC_FP32)
+        hipLaunchKernelGGL(k_build_x<float>, dim3(cdiv(total, 256)), dim3(256), 0, st, d_states, B, L, q, d_state_col,
+                           Kx, Bp, (int64_t)Kxp, (float *)X, (float *)Xt, 1.0f);
+    else
+        hipLaunchKernelGGL(k_build_x<uint16_t>, dim3(cdiv(total, 256)), dim3(256), 0, st, d_states, B, L, q,
+                           d_state_col, Kx, Bp, (int64_t)Kxp, (uint16_t *)X, (uint16_t *)Xt, (uint16_t)0x3f80);
+}
+
+void bitplanes(hipStream_t st, const uint8_t *d_states, int B, int L, int q, int W, uint64_t *planes)
+{
+    hipLaunchKernelGGL(k_bitplanes, dim3(cdiv((int64_t)B * W, 256)), dim3(256), 0, st, d_states, B, L, q, W, planes);
+}
+
+void hamming_neighbours(hipStream_t st, const uint64_t *planes, int B, int q, int W, int L, int64_t dmax,
+                        const uint32_t *d_mult, unsigned long long *d_iw)
+{
+    const int64_t T = (B + HT - 1) / HT;
+    const int64_t ntiles = T * (T + 1) / 2;
+    hipLaunchKernelGGL(k_hamming, dim3((unsigned)ntiles), dim3(256), 0, st, planes, B, q, W, L, dmax, d_mult, d_iw);
+}
+
+void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const void *X, int Mpad,
+                 int Bp, int Kxp, int ksplit, int kchunk, float *Epart)
+{
+    EpiEnergy epi{Epart, (int64_t)Bp, (int64_t)Mpad * Bp};
+    const dim3 grid(Bp / kTile, Mpad / kTile, ksplit);
+    if (prec == SDCA_PREC_FP32)
+        hipLaunchKernelGGL((k_gemm_nt<PolF32, 1, EpiEnergy>), grid, dim3(256), 0, st, (const float *)W32,
+                           (const float *)W32, (int64_t)Kxp, (const float *)X, (int64_t)Kxp, Kxp, kchunk, epi);
+    else
+        hipLaunchKernelGGL((k_gemm_nt<PolBF16, 2, EpiEnergy>), grid, dim3(256), 0, st, (const uint16_t *)Whi,
+                           (const uint16_t *)Wlo, (int64_t)Kxp, (const uint16_t *)X, (int64_t)Kxp, Kxp, kchunk, epi);
+}
+
+void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad, int Bp, int B, int q, int nslots,
+             const SlotCmd *cmds, const uint8_t *statesT, const float *wts, float *G32, uint16_t *Ghi,
+             uint16_t *Glo, double *fpart, int nbchunk)
+{
+    hipLaunchKernelGGL(k_softmax, dim3(nbchunk, nslots), dim3(256), 0, st, Epart, ksplit, (int64_t)Mpad * Bp, Bp, B,
+                       q, nslots, cmds, statesT, wts, prec == SDCA_PREC_FP32 ? G32 : nullptr, Ghi, Glo, fpart,
+                       nbchunk);
+}
+
+void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const void *Xt, int Mpad,
+               int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial, float *ring,
+               int64_t n_slot, int P, float lamJ2, float lamh2)
+{
+    EpiGrad epi{ring, Xtrial, cmds, n_slot, (int64_t)Kxp, P, q, Kx, nslots, lamJ2, lamh2};
+    const dim3 grid(Kxp / kTile, Mpad / kTile, 1);
+    if (prec == SDCA_PREC_FP32)
+        hipLaunchKernelGGL((k_gemm_nt<PolF32, 1, EpiGrad>), grid, dim3(256), 0, st, (const float *)G32,
+                           (const float *)G32, (int64_t)Bp, (const float *)Xt, (int64_t)Bp, Bp, Bp, epi);
+    else
+        hipLaunchKernelGGL((k_gemm_nt<PolBF16, 2, EpiGrad>), grid, dim3(256), 0, st, (const uint16_t *)Ghi,
+                           (const uint16_t *)Glo, (int64_t)Bp, (const uint16_t *)Xt, (int64_t)Bp, Bp, Bp, epi);
+}
+
+#define SDCA_DISPATCH_M(m, CALL) \
+    switch (m) {                 \
+    case 1: CALL(1); break;      \
+    case 3: CALL(3); break;      \
+    case 5: CALL(5); break;      \
+    case 10: CALL(10); break;    \
+    case 16: CALL(16); break;    \
+    default: break;              \
+    }
+
+void update(hipStream_t st, int m, int prec, const SlotCmd *cmds, int nslots, float *ring, float *Xc, float *Xtr,
+            uint16_t *Whi, uint16_t *Wlo, const float *X0, int64_t n_slot, int Kxp, int Kx, double *part, int nchunk)
+{
+    (void)prec;
+#define SDCA_UPD(MM)                                                                                                  \
+    hipLaunchKernelGGL(k_update<MM>, dim3(nchunk, nslots), dim3(256), 0, st, cmds, ring, Xc, Xtr, Whi, Wlo, X0, n_slot, \
+                       Kxp, Kx, part, nchunk)
+    SDCA_DISPATCH_M(m, SDCA_UPD)
+#undef SDCA_UPD
+}
+
+void dots(hipStream_t st, int m, const SlotCmd *cmds, int nslots, const float *ring, int64_t n_slot, double *part,
+          int nchunk)
+{
+#define SDCA_DOT(MM) \
+    hipLaunchKernelGGL(k_dots<MM>, dim3(nchunk, nslots), dim3(256), 0, st, cmds, ring, n_slot, part, nchunk)
+    SDCA_DISPATCH_M(m, SDCA_DOT)
+#undef SDCA_DOT
+}
+
+void reduce_results(hipStream_t st, int m, int nslots, const double *upd_part, const double *dot_part,
+                    const double *sm_part, int nchunk, int nbchunk, double *res)
+{
+    const int P = 2 * m + 3;
+    const int W = 2 * P + 4;
+    hipLaunchKernelGGL(k_reduce, dim3(nslots), dim3((W + 63) / 64 * 64), 0, st, P, upd_part, dot_part, sm_part, nchunk,
+                       nbchunk, res);
+}
+
+void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *loci, int nscore, int L, int q,
+           int gap, int Kxp, const int32_t *state_col, float *rows)
+{
+    hipLaunchKernelGGL(k_score, dim3(cdiv(L, 256), nscore), dim3(256), 0, st, xsrc, src_off, loci, L, q, gap, Kxp,
+                       state_col, rows);
+}
+
+}  // namespace kern
+}  // namespace sdca

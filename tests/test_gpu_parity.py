@@ -1,0 +1,192 @@
+"""GPU parity: the HIP path (through the C ABI of libsdca.so) against the FP64 oracle.
+
+Bars: reweighting bit-exact; objective value and gradient within the stated FP32 /
+split-BF16 tolerances; solved scores against the oracle's optimum within the stated
+relative tolerance.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import superdca_amd as sd
+from superdca_amd import synth
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def c2():
+    d = synth.workload("C2")
+    states, loci = sd.prepare_alignment(d["chars"])
+    return {"states": states, "pairs": d["pairs"]}
+
+
+# ------------------------------------------------------------ reweighting
+
+@pytest.mark.parametrize("theta", [0.9, 0.95, 0.5, 1.0])
+def test_reweight_bit_exact_c1(c1, theta):
+    st = c1["states"]
+    with sd.Context(st) as ctx:
+        w, iw = ctx.reweight(theta)
+    wo, iwo = orc.weights(st, theta, nthreads=8)
+    np.testing.assert_array_equal(iw, iwo)
+    np.testing.assert_array_equal(w, wo)
+
+
+def test_reweight_bit_exact_with_multiplicity(c1):
+    st = c1["states"][:, :700]
+    mult = np.random.default_rng(4).integers(1, 5, st.shape[0]).astype(np.uint32)
+    with sd.Context(st) as ctx:
+        w, iw = ctx.reweight(0.9, mult)
+    wo, iwo = orc.weights(st, 0.9, mult=mult, nthreads=8)
+    np.testing.assert_array_equal(iw, iwo)
+    np.testing.assert_array_equal(w, wo)
+
+
+def test_reweight_bit_exact_c2(c2):
+    st = c2["states"]
+    with sd.Context(st) as ctx:
+        w, iw = ctx.reweight(0.9)
+    wo, iwo = orc.weights(st, 0.9, nthreads=16)
+    np.testing.assert_array_equal(iw, iwo)
+    np.testing.assert_array_equal(w, wo)
+
+
+# ---------------------------------------------------- objective and gradient
+
+EVAL_LOCI = [0, 1, 17, 500, 991, 999]  # 1000 % 16 = 8: 992..999 form the short last block
+
+
+@pytest.mark.parametrize("precision,ftol,gtol", [(sd.PREC_FP32, 2e-6, 2e-5), (sd.PREC_SPLIT_BF16, 2e-5, 2e-4)])
+def test_eval_batch_matches_oracle(c1, precision, ftol, gtol):
+    st = c1["states"]
+    B, L = st.shape
+    w, _ = orc.weights(st, 0.9, nthreads=8)
+    rng = np.random.default_rng(12)
+    n = orc.nparams(L, 4)
+    x = rng.normal(size=(len(EVAL_LOCI), n)) * 0.05
+    with sd.Context(st, precision=precision, batch_loci=4) as ctx:
+        beff, lh, lJ = ctx.set_weights(w)
+        f, g = ctx.eval_batch(EVAL_LOCI, x)
+    for k, r in enumerate(EVAL_LOCI):
+        fo, go = orc.objective(st, r, w, lh, lJ, x[k])
+        assert f[k] == pytest.approx(fo, rel=ftol), r
+        err = np.abs(g[k] - go).max() / np.abs(go).max()
+        assert err < gtol, (r, err)
+
+
+def test_eval_batch_cold_point_is_exact_softmax(c1):
+    # at x = 0 every E_b is 0: f = B_eff * log(q) exactly up to FP32 weight rounding
+    st = c1["states"]
+    w, _ = orc.weights(st, 0.9, nthreads=8)
+    L = st.shape[1]
+    with sd.Context(st) as ctx:
+        beff, lh, lJ = ctx.set_weights(w)
+        f, g = ctx.eval_batch([3], np.zeros((1, orc.nparams(L, 4))))
+    assert f[0] == pytest.approx(beff * np.log(4.0), rel=1e-6)
+    fo, go = orc.objective(st, 3, w, lh, lJ, np.zeros(orc.nparams(L, 4)))
+    np.testing.assert_allclose(g[0], go, atol=1e-5)
+
+
+# ------------------------------------------------------------------ solve
+
+SOLVE_LOCI = np.arange(0, 1000, 25)  # 40 loci spread over the alignment
+
+
+@pytest.fixture(scope="module")
+def c1_optimum(c1):
+    st = c1["states"]
+    w, _ = orc.weights(st, 0.9, nthreads=8)
+    beff, lh, lJ = orc.lambdas(w)
+    rows, X, stats = orc.solve_loci(st, w, lh, lJ, SOLVE_LOCI, gtol=1e-7, max_iter=10000, nthreads=8, want_x=True)
+    return {"w": w, "rows": rows, "x": X, "stats": stats, "lam": (lh, lJ)}
+
+
+def score_errors(rows, ref):
+    mask = np.ones_like(ref, dtype=bool)
+    for k, r in enumerate(SOLVE_LOCI):
+        mask[k, r] = False
+    big = mask & (ref > np.quantile(ref[mask], 0.9))
+    rel = np.abs(rows - ref)[big] / ref[big]
+    return np.abs(rows - ref)[mask].max(), rel.max()
+
+
+def test_solve_fp32_matches_oracle_optimum(c1, c1_optimum):
+    st = c1["states"]
+    with sd.Context(st, grad_threshold=1e-5, batch_loci=64) as ctx:
+        ctx.set_weights(c1_optimum["w"])
+        rows, stats, X, rc = ctx.solve(SOLVE_LOCI, want_x=True)
+    assert rc == sd.SDCA_OK
+    assert np.all(stats["status"] == sd.ST_CONVERGED)
+    assert np.all(stats["gnorm"] < 1e-5)
+    np.testing.assert_allclose(stats["fval"], c1_optimum["stats"]["fval"], rtol=1e-6)
+    for k, r in enumerate(SOLVE_LOCI):
+        assert rows[k, r] == 0.0
+    abs_err, rel_top = score_errors(rows, c1_optimum["rows"])
+    print(f"fp32 scores: max abs err {abs_err:.3e}, max rel err (top decile) {rel_top:.3e}")
+    assert rel_top < 1e-4
+
+
+def test_solve_split_bf16_matches_oracle_optimum(c1, c1_optimum):
+    st = c1["states"]
+    with sd.Context(st, precision=sd.PREC_SPLIT_BF16, grad_threshold=1e-4, batch_loci=64) as ctx:
+        ctx.set_weights(c1_optimum["w"])
+        rows, stats, X, rc = ctx.solve(SOLVE_LOCI)
+    assert rc == sd.SDCA_OK
+    np.testing.assert_allclose(stats["fval"], c1_optimum["stats"]["fval"], rtol=1e-5)
+    abs_err, rel_top = score_errors(rows, c1_optimum["rows"])
+    print(f"split-bf16 scores: max abs err {abs_err:.3e}, max rel err (top decile) {rel_top:.3e}")
+    assert rel_top < 1e-3
+
+
+def test_reference_mode_threshold_and_refill(c1, c1_optimum):
+    # default gradient threshold 1e-3 (plmDCA_options.cpp:61) with fewer slots than loci:
+    # converged slots are refilled from the queue
+    st = c1["states"]
+    with sd.Context(st, batch_loci=7) as ctx:
+        ctx.set_weights(c1_optimum["w"])
+        rows, stats, _, rc = ctx.solve(SOLVE_LOCI)
+    assert rc == sd.SDCA_OK
+    assert np.all(stats["gnorm"] < 1e-3)
+    np.testing.assert_allclose(stats["fval"], c1_optimum["stats"]["fval"], rtol=1e-4)
+
+
+def test_warm_start_reaches_cold_optimum(c1, c1_optimum):
+    st = c1["states"]
+    rng = np.random.default_rng(5)
+    warm = c1_optimum["x"] + rng.normal(size=c1_optimum["x"].shape) * 0.02
+    with sd.Context(st, grad_threshold=1e-5, batch_loci=64) as ctx:
+        ctx.set_weights(c1_optimum["w"])
+        _, sw, _, rc = ctx.solve(SOLVE_LOCI, warm=warm)
+        _, sc, _, rc2 = ctx.solve(SOLVE_LOCI)
+    assert rc == sd.SDCA_OK and rc2 == sd.SDCA_OK
+    np.testing.assert_allclose(sw["fval"], sc["fval"], rtol=1e-6)
+    assert np.median(sw["iter"]) < np.median(sc["iter"])
+
+
+def test_cli_end_to_end(tmp_path):
+    d = synth.generate(B=150, N=120, clades=6, pairs=4, seed=77)
+    fa = tmp_path / "toy.fasta"
+    synth.write_fasta(str(fa), d["chars"])
+    exe = os.path.join(ROOT, "superdca_amd", "sdca")
+    p = subprocess.run([exe, "-v", str(fa)], capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    assert p.returncode == 0, p.stderr
+    out = tmp_path / "toy.SuperDCA_couplings.1-based.all"
+    lines = out.read_text().splitlines()
+    states, loci = sd.prepare_alignment(d["chars"])
+    L = states.shape[1]
+    assert len(lines) == L * (L - 1) // 2
+    first = [int(t.split()[1]) > int(t.split()[2]) for t in lines]
+    assert all(first)
+    # same numbers as the library called directly
+    with sd.Context(states) as ctx:
+        w, _ = ctx.reweight(0.9)
+        ctx.set_weights(w)
+        rows, _, _, _ = ctx.solve(np.arange(L))
+    ref = tmp_path / "direct.all"
+    sd.write_couplings(str(ref), rows, loci, 1)
+    assert ref.read_text().splitlines() == lines

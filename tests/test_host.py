@@ -1,0 +1,130 @@
+"""CPU tests of the host side of libsdca.so: ABI surface, alignment pipeline, writer, CLI."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import superdca_amd as sd
+from superdca_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "sdca.h")).read()
+    return sorted(set(re.findall(r"\b(sdca_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(sd.LIB_PATH)
+    syms = declared_symbols()
+    assert len(syms) >= 12
+    for name in syms:
+        assert hasattr(lib, name), name
+    assert sorted(sd.EXPORTED) == syms
+
+
+def test_default_opts_match_reference_defaults():
+    o = sd.default_opts()
+    assert o.grad_threshold == 1e-3  # plmDCA_options.cpp:61
+    assert o.max_iter == 2000        # plmDCA.hpp:219
+    assert o.history == 10
+    assert o.precision == sd.PREC_FP32
+
+
+def test_prepare_alignment_matches_column_stats():
+    d = synth.generate(B=120, N=400, clades=6, pairs=5, seed=21)
+    chars = d["chars"].copy()
+    rng = np.random.default_rng(0)
+    # make some loci fail: monomorphic, too gappy, too rare
+    chars[:, 10] = ord("A")
+    chars[: int(0.2 * 120), 11] = ord("-")
+    chars[:, 12] = ord("C")
+    chars[0, 12] = ord("G")  # maf 1/120 < 1 %
+    chars[:, 13] = np.where(rng.random(120) < 0.5, ord("a"), ord("t"))  # lower case counts as alleles
+    states, loci = sd.prepare_alignment(chars)
+    keep = synth.passes_filter(chars)
+    np.testing.assert_array_equal(loci, np.flatnonzero(keep))
+    assert 10 not in loci and 11 not in loci and 12 not in loci and 13 in loci
+    st2, loci2 = sd.prepare_alignment(chars, no_filter=True)
+    assert loci2.size == chars.shape[1]
+
+
+def test_envelope_state_order():
+    # alleles by descending count, ties A<C<G<T, gap = q-1; four alleles dropped at q=4
+    col = np.array([b"T", b"T", b"T", b"G", b"G", b"C", b"C", b"-", b"N", b"T"]).view(np.uint8)
+    four = np.array([b"A", b"C", b"G", b"T", b"A", b"C", b"G", b"T", b"A", b"A"]).view(np.uint8)
+    chars = np.stack([col, four], axis=1)
+    states, loci = sd.prepare_alignment(chars, maf_threshold=0.0, gap_threshold=1.0)
+    np.testing.assert_array_equal(loci, [0])
+    # T (4) -> 0, C and G tie (2 each): C < G -> C=1, G=2, gaps and N -> 3
+    np.testing.assert_array_equal(states[:, 0], [0, 0, 0, 2, 2, 1, 1, 3, 3, 0])
+    states5, loci5 = sd.prepare_alignment(chars, maf_threshold=0.0, gap_threshold=1.0, q=5)
+    np.testing.assert_array_equal(loci5, [0, 1])
+    np.testing.assert_array_equal(states5[:, 1], [0, 1, 2, 3, 0, 1, 2, 3, 0, 0])
+    assert states5[7, 0] == 4
+
+
+def test_writer_matches_golden(tmp_path):
+    S = np.load(os.path.join(GOLD, "couplings_tiny_S.npy"))
+    out = tmp_path / "c.all"
+    sd.write_couplings(str(out), S, np.array([2, 5, 7, 11], dtype=np.int32), 1)
+    assert out.read_text() == open(os.path.join(GOLD, "couplings_tiny.all")).read()
+
+
+def test_writer_rounding_matches_printf(tmp_path):
+    rng = np.random.default_rng(3)
+    L = 300
+    S = (rng.random((L, L)) ** 3 * rng.choice([0.01, 0.1, 1.0, 7.0], (L, L))).astype(np.float32)
+    S[rng.random((L, L)) < 0.01] = 0.0
+    tr = np.sort(rng.choice(100000, L, replace=False)).astype(np.int32)
+    out = tmp_path / "r.all"
+    sd.write_couplings(str(out), S, tr, 1)
+    lines = out.read_text().splitlines()
+    assert len(lines) == L * (L - 1) // 2
+    k = 0
+    for r in range(L):
+        for n in range(r):
+            v = float(np.float32(S[r, n] + S[n, r])) * 0.5
+            assert lines[k] == "%.8f %d %d" % (v, tr[r] + 1, tr[n] + 1)
+            k += 1
+
+
+def test_generator_is_deterministic_and_filter_clean():
+    a = synth.generate(B=80, N=300, clades=5, pairs=4, seed=5)
+    b = synth.generate(B=80, N=300, clades=5, pairs=4, seed=5)
+    np.testing.assert_array_equal(a["chars"], b["chars"])
+    assert synth.passes_filter(a["chars"]).all()
+    assert a["pairs"].shape == (4, 2)
+    i, j = a["pairs"][0]
+    # the planted partner copies a fixed map of locus i in most sequences
+    ci, cj = a["chars"][:, i], a["chars"][:, j]
+    agree = max(np.mean([(cj[ci == v] == cj[ci == v][0]).mean() for v in np.unique(ci) if (ci == v).sum() > 3]), 0)
+    assert agree > 0.7
+
+
+def test_cli_help_and_errors(tmp_path):
+    exe = os.path.join(ROOT, "superdca_amd", "sdca")
+    if not os.path.exists(exe):
+        from superdca_amd import build
+        build.build_cli()
+    p = subprocess.run([exe, "--help"], capture_output=True, text=True)
+    assert p.returncode == 0 and "--gradient-threshold" in p.stdout and "--maf-threshold" in p.stdout
+    p = subprocess.run([exe], capture_output=True, text=True)
+    assert p.returncode != 0 and "No alignment file" in p.stderr
+    p = subprocess.run([exe, str(tmp_path / "missing.fasta")], capture_output=True, text=True)
+    assert p.returncode != 0 and "Could not get alignment" in p.stderr
+    p = subprocess.run([exe, "--bogus-flag", "x.fa"], capture_output=True, text=True)
+    assert p.returncode != 0
+
+
+def test_context_without_gpu_fails_loudly():
+    import torch  # noqa: F401  (only to learn whether a device is visible)
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(sd.SdcaError):
+        sd.Context(np.zeros((4, 3), dtype=np.uint8))
