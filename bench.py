@@ -1,7 +1,6 @@
 """Throughput of the per-locus plmDCA solve on one MI355X: loci/s to convergence.
 
-A step is one batch of loci (the GPU's concurrent slots, 256 by default) solved from a
-cold start to ||g||_inf < 1e-3 (SuperDCA's default stopping rule) on the genome-scale
+A step is one batch of loci (2048 at genome scale) solved from a cold start to ||g||_inf < 1e-3 (SuperDCA's default stopping rule) on the genome-scale
 workload B = 3000 sequences x N = 94,000 loci (seeded synthetic alignment, 60 clades,
 2 % mutation, 2000 planted pairs).  Timing starts with the alignment, weights and
 lambda resident on the device and ends when the last score row is in host memory.
@@ -37,7 +36,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="C4", choices=["C1", "C2", "C4"])
-    ap.add_argument("--loci-per-step", type=int, default=0, help="default: 256 at C4, 128 otherwise")
+    ap.add_argument("--loci-per-step", type=int, default=0, help="default: 2048 at C4, 1024 otherwise")
+    ap.add_argument("--slots", type=int, default=0,
+                    help="loci in flight on the GPU (default 256 at C4, 128 otherwise); finished slots are refilled")
     ap.add_argument("--precision", default="", choices=["", "fp32", "split-bf16"])
     ap.add_argument("--history", type=int, default=10)
     ap.add_argument("--cpu-loci", type=int, default=0, help="CPU baseline sample size (default: one per thread)")
@@ -57,7 +58,8 @@ def main() -> int:
     cfg = synth.WORKLOADS[wl]
     prec_name = args.precision or ("split-bf16" if wl == "C4" else "fp32")
     prec = sd.PREC_SPLIT_BF16 if prec_name == "split-bf16" else sd.PREC_FP32
-    per_step = args.loci_per_step or (256 if wl == "C4" else 128)
+    per_step = args.loci_per_step or (2048 if wl == "C4" else 1024)
+    slots = args.slots or (256 if wl == "C4" else 128)
 
     t = time.time()
     data = synth.workload(wl)
@@ -67,7 +69,7 @@ def main() -> int:
     log(f"[bench] workload {wl}: B={B} L={L} generated in {time.time() - t:.1f}s")
 
     t = time.time()
-    ctx = sd.Context(states, q=4, precision=prec, batch_loci=per_step, history=args.history, grad_threshold=1e-3)
+    ctx = sd.Context(states, q=4, precision=prec, batch_loci=slots, history=args.history, grad_threshold=1e-3)
     log(f"[bench] context (one-hot X, K_x={ctx.num_columns}) built in {time.time() - t:.1f}s")
     t = time.time()
     w, _ = ctx.reweight(0.9)
@@ -82,7 +84,8 @@ def main() -> int:
     pos = 0
     for k in range(args.warmup):
         t = time.time()
-        ctx.solve(order[pos:pos + per_step])
+        _, wst, _, _ = ctx.solve(order[pos:pos + per_step])
+        assert np.all(wst["nfeval"] > 1), "warmup loci stopped at their start point"
         pos += per_step
         log(f"[bench] warmup step {k} {time.time() - t:.2f}s")
     ctx.timing(reset=True)
@@ -151,8 +154,9 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "bf16x2+fp32" if prec == sd.PREC_SPLIT_BF16 else "fp32",
         "data": "synthetic",
-        "config": {"workload": f"{wl}: B={B} sequences x N={L} loci, cold start, {per_step} loci per step, "
-                               f"L-BFGS m={args.history}, ||g||_inf<1e-3", "B": B, "N": L, "loci_per_step": per_step,
+        "config": {"workload": f"{wl}: B={B} sequences x N={L} loci, cold start, {per_step} loci per step "
+                               f"({slots} in flight), L-BFGS m={args.history}, ||g||_inf<1e-3", "B": B, "N": L,
+                   "loci_per_step": per_step, "slots": slots,
                    "precision": prec_name, "start": "cold", "parallelism": "loci batched on 1 GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -116,13 +116,15 @@ def score_errors(rows, ref):
 
 
 def test_solve_fp32_matches_oracle_optimum(c1, c1_optimum):
+    # 5e-6 is near the FP32 floor: a locus may stop on a line-search stall there, but only
+    # with its gradient already below 1e-5
     st = c1["states"]
-    with sd.Context(st, grad_threshold=1e-5, batch_loci=64) as ctx:
+    with sd.Context(st, grad_threshold=5e-6, batch_loci=64) as ctx:
         ctx.set_weights(c1_optimum["w"])
         rows, stats, X, rc = ctx.solve(SOLVE_LOCI, want_x=True)
-    assert rc == sd.SDCA_OK
-    assert np.all(stats["status"] == sd.ST_CONVERGED)
-    assert np.all(stats["gnorm"] < 1e-5)
+    assert rc in (sd.SDCA_OK, sd.SDCA_E_NOCONV)
+    ok = (stats["status"] == sd.ST_CONVERGED) | ((stats["status"] == sd.ST_LS_FAIL) & (stats["gnorm"] < 1e-5))
+    assert ok.all(), stats
     np.testing.assert_allclose(stats["fval"], c1_optimum["stats"]["fval"], rtol=1e-6)
     for k, r in enumerate(SOLVE_LOCI):
         assert rows[k, r] == 0.0
@@ -132,15 +134,17 @@ def test_solve_fp32_matches_oracle_optimum(c1, c1_optimum):
 
 
 def test_solve_split_bf16_matches_oracle_optimum(c1, c1_optimum):
+    # hi + lo BF16 carries about 18 significant bits per product (|x - hi - lo| <= 2^-18 |x|):
+    # the gradient floor is ~2e-5 here and the top-decile score error ~3e-4
     st = c1["states"]
-    with sd.Context(st, precision=sd.PREC_SPLIT_BF16, grad_threshold=1e-4, batch_loci=64) as ctx:
+    with sd.Context(st, precision=sd.PREC_SPLIT_BF16, grad_threshold=3e-5, batch_loci=64) as ctx:
         ctx.set_weights(c1_optimum["w"])
         rows, stats, X, rc = ctx.solve(SOLVE_LOCI)
     assert rc == sd.SDCA_OK
-    np.testing.assert_allclose(stats["fval"], c1_optimum["stats"]["fval"], rtol=1e-5)
+    np.testing.assert_allclose(stats["fval"], c1_optimum["stats"]["fval"], rtol=1e-6)
     abs_err, rel_top = score_errors(rows, c1_optimum["rows"])
     print(f"split-bf16 scores: max abs err {abs_err:.3e}, max rel err (top decile) {rel_top:.3e}")
-    assert rel_top < 1e-3
+    assert rel_top < 5e-4
 
 
 def test_reference_mode_threshold_and_refill(c1, c1_optimum):
@@ -158,7 +162,8 @@ def test_reference_mode_threshold_and_refill(c1, c1_optimum):
 def test_warm_start_reaches_cold_optimum(c1, c1_optimum):
     st = c1["states"]
     rng = np.random.default_rng(5)
-    warm = c1_optimum["x"] + rng.normal(size=c1_optimum["x"].shape) * 0.02
+    # a start near the optimum (10 % multiplicative perturbation of every parameter)
+    warm = c1_optimum["x"] * (1.0 + 0.1 * rng.normal(size=c1_optimum["x"].shape))
     with sd.Context(st, grad_threshold=1e-5, batch_loci=64) as ctx:
         ctx.set_weights(c1_optimum["w"])
         _, sw, _, rc = ctx.solve(SOLVE_LOCI, warm=warm)
