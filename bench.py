@@ -107,9 +107,10 @@ def main() -> int:
 
     # dominant kernel group by device time
     groups = {
-        "gemm_grad": (tm["gemm_grad_ms"], tm["flops_grad"], "mfma"),
-        "gemm_energy": (tm["gemm_energy_ms"], tm["flops_energy"], "mfma"),
-        "lbfgs_vector_passes": (tm["lbfgs_ms"], tm["bytes_lbfgs"], "hbm"),
+        "k_gemm_nt<EpiGrad>": (tm["gemm_grad_ms"], tm["flops_grad"], "mfma"),
+        "k_gemm_nt<EpiEnergy>": (tm["gemm_energy_ms"], tm["flops_energy"], "mfma"),
+        "k_update": (tm["update_ms"], tm["bytes_update"], "hbm"),
+        "k_dots": (tm["dots_ms"], tm["bytes_dots"], "hbm"),
     }
     dom = max(groups, key=lambda k: groups[k][0])
     ms, work, bound = groups[dom]
@@ -164,8 +165,10 @@ def main() -> int:
             "converged": converged, "iter_median": float(np.median(iters)), "iter_p95": float(np.percentile(iters, 95)),
             "evals_median": float(np.median(evals)), "evals_p95": float(np.percentile(evals, 95)),
             "reweight_ms": round(reweight_s * 1e3, 2), "B_eff": beff,
-            "device_ms": {k: round(tm[k], 2) for k in ("gemm_energy_ms", "gemm_grad_ms", "softmax_ms", "lbfgs_ms",
-                                                       "score_ms")},
+            "device_ms": {k: round(tm[k], 2) for k in ("gemm_energy_ms", "gemm_grad_ms", "softmax_ms", "update_ms",
+                                                       "dots_ms", "score_ms")},
+            "kernel_rates": {k: round(v[1] / (v[0] * 1e-3) / (1e12 if v[2] == "mfma" else 1e9), 2)
+                             for k, v in groups.items() if v[0] > 0},
             "rounds": tm["n_evals"],
         },
     }

@@ -85,6 +85,10 @@ typedef struct sdca_timing {
     double flops_energy;     /* algorithmic flops of all energy contractions (2*B*K_x*q per locus-eval) */
     double flops_grad;
     double bytes_lbfgs;      /* algorithmic HBM bytes of the L-BFGS vector passes */
+    double update_ms;        /* trial point / new direction pass (part of lbfgs_ms) */
+    double dots_ms;          /* gradient dot products + reduction (part of lbfgs_ms) */
+    double bytes_update;     /* algorithmic HBM bytes of the update pass */
+    double bytes_dots;       /* algorithmic HBM bytes of the dot-product pass */
 } sdca_timing;
 
 const char *sdca_last_error(void);

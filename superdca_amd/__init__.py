@@ -63,7 +63,8 @@ class Timing(C.Structure):
                 ("lbfgs_ms", C.c_double), ("score_ms", C.c_double), ("reweight_ms", C.c_double),
                 ("total_ms", C.c_double), ("n_evals", C.c_int64), ("n_energy_launches", C.c_int64),
                 ("n_grad_launches", C.c_int64), ("flops_energy", C.c_double), ("flops_grad", C.c_double),
-                ("bytes_lbfgs", C.c_double)]
+                ("bytes_lbfgs", C.c_double), ("update_ms", C.c_double), ("dots_ms", C.c_double),
+                ("bytes_update", C.c_double), ("bytes_dots", C.c_double)]
 
 
 _lib = None

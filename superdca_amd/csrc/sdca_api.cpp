@@ -147,10 +147,13 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     } else {
         if ((rc = dalloc(&c->d_G32, (size_t)c->Mpad * c->Bp))) return rc;
     }
-    // split-K of the energy contraction: aim for >= 2048 workgroups
+    // split-K of the energy contraction: aim for >= 2048 workgroups, with the number of
+    // super-tiles a multiple of the eight XCDs
     const int BK = split ? 64 : 32;
     const int64_t tiles = (int64_t)(c->Mpad / sdca::kTile) * (c->Bp / sdca::kTile);
     int64_t ks = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
+    const int64_t sup = sdca::kern::energy_superiles(c->Mpad, c->Bp);
+    while ((ks * sup) % 8 != 0 && ks < 64) ++ks;
     ks = std::min<int64_t>(ks, c->Kxp / BK);
     c->kchunk = (int)round_up((c->Kxp + ks - 1) / ks, BK);
     c->ksplit = (int)((c->Kxp + c->kchunk - 1) / c->kchunk);
@@ -216,7 +219,10 @@ int run_round(sdca_ctx *c, int nslots, const std::vector<sdca::SlotCmd> &cmds, s
     HIP_TRY(hipMemcpyAsync(res.data(), c->d_res, res.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     auto &t = c->timing;
-    t.lbfgs += elapsed(c->ev[0], c->ev[1]) + elapsed(c->ev[4], c->ev[5]);
+    const double upd_ms = elapsed(c->ev[0], c->ev[1]), dot_ms = elapsed(c->ev[4], c->ev[5]);
+    t.lbfgs += upd_ms + dot_ms;
+    t.update += upd_ms;
+    t.dots += dot_ms;
     t.energy += elapsed(c->ev[1], c->ev[2]);
     t.softmax += elapsed(c->ev[2], c->ev[3]);
     t.grad += elapsed(c->ev[3], c->ev[4]);
@@ -224,16 +230,21 @@ int run_round(sdca_ctx *c, int nslots, const std::vector<sdca::SlotCmd> &cmds, s
     t.energy_launches += 1;
     t.grad_launches += 1;
     int active = 0;
-    double vec = 0.0;  // vector passes of the update and dot-product kernels
+    double vu = 0.0, vd = 0.0;  // vector passes of the update and dot-product kernels
     for (int i = 0; i < nslots; ++i) {
         const int md = cmds[i].mode;
         active += md != sdca::MODE_IDLE;
-        if (md == sdca::MODE_INIT) vec += 2 + (cmds[i].init_src != 0.f) + P;
-        else if (md == sdca::MODE_RETRY) vec += 3 + P;
-        else if (md == sdca::MODE_NEWDIR) vec += (P + 4) + P;
-        if (md != sdca::MODE_IDLE && split) vec += 1;  // hi + lo BF16 copies of x_t
+        if (md == sdca::MODE_INIT) vu += 2 + (cmds[i].init_src != 0.f);
+        else if (md == sdca::MODE_RETRY) vu += 3;
+        else if (md == sdca::MODE_NEWDIR) vu += P + 4;
+        if (md != sdca::MODE_IDLE) {
+            vd += P;
+            if (split) vu += 1;  // hi + lo BF16 copies of x_t
+        }
     }
-    t.bytes_lbfgs += vec * 4.0 * (double)c->n_slot;
+    t.bytes_update += vu * 4.0 * (double)c->n_slot;
+    t.bytes_dots += vd * 4.0 * (double)c->n_slot;
+    t.bytes_lbfgs += (vu + vd) * 4.0 * (double)c->n_slot;
     t.flops_energy += 2.0 * c->B * (double)c->Kx * c->q * active;
     t.flops_grad += 2.0 * c->B * (double)c->Kx * c->q * active;
     return SDCA_OK;
@@ -928,6 +939,10 @@ int sdca_get_timing(sdca_ctx *c, sdca_timing *t, int32_t reset)
     t->flops_energy = s.flops_energy;
     t->flops_grad = s.flops_grad;
     t->bytes_lbfgs = s.bytes_lbfgs;
+    t->update_ms = s.update;
+    t->dots_ms = s.dots;
+    t->bytes_update = s.bytes_update;
+    t->bytes_dots = s.bytes_dots;
     if (reset) c->timing = sdca::Timing();
     return SDCA_OK;
 }

@@ -58,6 +58,7 @@ struct Timing {
     double energy = 0, grad = 0, softmax = 0, lbfgs = 0, score = 0, reweight = 0, total = 0;
     int64_t evals = 0, energy_launches = 0, grad_launches = 0;
     double flops_energy = 0, flops_grad = 0, bytes_lbfgs = 0;
+    double update = 0, dots = 0, bytes_update = 0, bytes_dots = 0;
 };
 
 // Kernel launchers (sdca_kernels.hip).
@@ -67,6 +68,7 @@ void build_x(hipStream_t st, const uint8_t *d_states, int B, int L, int q, const
 void bitplanes(hipStream_t st, const uint8_t *d_states, int B, int L, int q, int W, uint64_t *planes);
 void hamming_neighbours(hipStream_t st, const uint64_t *planes, int B, int q, int W, int L, int64_t dmax,
                         const uint32_t *d_mult, unsigned long long *d_iw);
+int energy_superiles(int Mpad, int Bp);  // super-tiles per split-K slice
 void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const void *X,
                  int Mpad, int Bp, int Kxp, int ksplit, int kchunk, float *Epart);
 void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad, int Bp, int B, int q, int nslots,

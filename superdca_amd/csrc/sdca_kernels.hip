@@ -217,18 +217,45 @@ __device__ __forceinline__ void mma_tile(const uint8_t *sA0, const uint8_t *sA1,
     }
 }
 
+// XCD-aware tile order.  Workgroups are dispatched to the eight XCDs round-robin
+// (workgroup id % 8), so consecutive ids on one XCD are id, id+8, id+16, ...  Tiles are
+// grouped into super-tiles of GM x GN output tiles at one split-K index z; a super-tile
+// runs on a single XCD, whose L2 then serves each A rows-tile to GN workgroups and each
+// B rows-tile to GM workgroups.  Super-tiles are dealt to XCDs round-robin.
+struct TileMap {
+    int Mt, Nt, Z;   // output tiles along M and N, split-K slices
+    int GM, GN;      // super-tile shape
+    int MB, NB, S;   // super-tiles along M and N, total super-tiles
+};
+
+__device__ __forceinline__ bool map_tile(const TileMap &tm, int bid, int &mt, int &nt, int &z)
+{
+    const int xcd = bid & 7, local = bid >> 3;
+    const int per = tm.GM * tm.GN;
+    const int sidx = (local / per) * 8 + xcd;
+    if (sidx >= tm.S) return false;
+    const int t = local - (local / per) * per;
+    const int nb = sidx % tm.NB;
+    const int mb = (sidx / tm.NB) % tm.MB;
+    z = sidx / (tm.NB * tm.MB);
+    mt = mb * tm.GM + t % tm.GM;
+    nt = nb * tm.GN + t / tm.GM;
+    return mt < tm.Mt && nt < tm.Nt;
+}
+
 template <class Pol, int NA, class Epi>
 __global__ __launch_bounds__(256) void k_gemm_nt(const typename Pol::T *__restrict__ A0,
                                                     const typename Pol::T *__restrict__ A1, int64_t lda,
                                                     const typename Pol::T *__restrict__ Bm, int64_t ldb, int K,
-                                                    int kchunk, Epi epi)
+                                                    int kchunk, TileMap tm, Epi epi)
 {
-    using T = typename Pol::T;
     __shared__ __attribute__((aligned(16))) uint8_t sA[NA][kTile * ROWB];
     __shared__ __attribute__((aligned(16))) uint8_t sB[kTile * ROWB];
+    int mt, nt, z;
+    if (!map_tile(tm, blockIdx.x, mt, nt, z)) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t m0 = (int64_t)blockIdx.y * kTile, n0 = (int64_t)blockIdx.x * kTile;
-    const int kb = blockIdx.z * kchunk;
+    const int64_t m0 = (int64_t)mt * kTile, n0 = (int64_t)nt * kTile;
+    const int kb = z * kchunk;
     const int ke = min(K, kb + kchunk);
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     floatx16 acc[2][2];
@@ -240,17 +267,25 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const typename Pol::T *__restri
             for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0.f;
     const int lrow = tid >> 3;         // 0..31
     const int lcol = (tid & 7) * 16;   // byte offset in the 128-byte row
-    for (int k0 = kb; k0 < ke; k0 += Pol::BK) {
-        u32x4 ra[NA][4], rb[4];
+    const uint8_t *pa0 = reinterpret_cast<const uint8_t *>(A0 + (m0 + lrow) * lda) + lcol;
+    const uint8_t *pa1 = reinterpret_cast<const uint8_t *>(A1 + (m0 + lrow) * lda) + lcol;
+    const uint8_t *pb = reinterpret_cast<const uint8_t *>(Bm + (n0 + lrow) * ldb) + lcol;
+    const int64_t sa = 32 * lda * (int64_t)sizeof(typename Pol::T);  // 32 rows, in bytes
+    const int64_t sbb = 32 * ldb * (int64_t)sizeof(typename Pol::T);
+    constexpr int EB = (int)sizeof(typename Pol::T);
+    u32x4 ra[NA][4], rb[4];
+    // software pipeline: the next K slice is in flight from HBM while the current one
+    // is multiplied out of LDS
+    auto load = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int64_t row = lrow + 32 * i;
-            ra[0][i] = *reinterpret_cast<const u32x4 *>(reinterpret_cast<const uint8_t *>(A0 + (m0 + row) * lda + k0) + lcol);
-            if constexpr (NA == 2)
-                ra[NA - 1][i] =
-                    *reinterpret_cast<const u32x4 *>(reinterpret_cast<const uint8_t *>(A1 + (m0 + row) * lda + k0) + lcol);
-            rb[i] = *reinterpret_cast<const u32x4 *>(reinterpret_cast<const uint8_t *>(Bm + (n0 + row) * ldb + k0) + lcol);
+            ra[0][i] = *reinterpret_cast<const u32x4 *>(pa0 + i * sa + (int64_t)k0 * EB);
+            if constexpr (NA == 2) ra[NA - 1][i] = *reinterpret_cast<const u32x4 *>(pa1 + i * sa + (int64_t)k0 * EB);
+            rb[i] = *reinterpret_cast<const u32x4 *>(pb + i * sbb + (int64_t)k0 * EB);
         }
+    };
+    if (kb < ke) load(kb);
+    for (int k0 = kb; k0 < ke; k0 += Pol::BK) {
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -260,6 +295,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const typename Pol::T *__restri
             *reinterpret_cast<u32x4 *>(&sB[row * ROWB + lcol]) = rb[i];
         }
         __syncthreads();
+        if (k0 + Pol::BK < ke) load(k0 + Pol::BK);
         mma_tile<Pol, NA>(sA[0], sA[NA - 1], sB, wm, wn, lane, acc);
     }
     // 32x32 MFMA result layout: register v of lane l holds row 8*(v/4) + 4*(l/32) + v%4,
@@ -272,7 +308,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const typename Pol::T *__restri
             for (int v = 0; v < 16; ++v) {
                 const int64_t m = m0 + wm + mi * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
                 const int64_t n = n0 + wn + ni * 32 + (lane & 31);
-                epi(m, n, acc[mi][ni][v]);
+                epi(m, n, z, acc[mi][ni][v]);
             }
 }
 
@@ -280,9 +316,9 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const typename Pol::T *__restri
 struct EpiEnergy {
     float *E;
     int64_t ld, zstride;
-    __device__ __forceinline__ void operator()(int64_t m, int64_t n, float v) const
+    __device__ __forceinline__ void operator()(int64_t m, int64_t n, int z, float v) const
     {
-        E[(int64_t)blockIdx.z * zstride + m * ld + n] = v;
+        E[(int64_t)z * zstride + m * ld + n] = v;
     }
 };
 
@@ -297,7 +333,7 @@ struct EpiGrad {
     int64_t n_slot, Kxp;
     int P, q, Kx, nslots;
     float lamJ2, lamh2;
-    __device__ __forceinline__ void operator()(int64_t m, int64_t n, float v) const
+    __device__ __forceinline__ void operator()(int64_t m, int64_t n, int, float v) const
     {
         const int i = (int)(m / q);
         if (i >= nslots) return;
@@ -649,17 +685,40 @@ void hamming_neighbours(hipStream_t st, const uint64_t *planes, int B, int q, in
     hipLaunchKernelGGL(k_hamming, dim3((unsigned)ntiles), dim3(256), 0, st, planes, B, q, W, L, dmax, d_mult, d_iw);
 }
 
+static TileMap make_map(int Mt, int Nt, int Z)
+{
+    TileMap tm;
+    tm.Mt = Mt;
+    tm.Nt = Nt;
+    tm.Z = Z;
+    tm.GM = Mt < 8 ? Mt : 8;
+    tm.GN = Nt < 64 / tm.GM ? Nt : 64 / tm.GM;
+    tm.MB = (Mt + tm.GM - 1) / tm.GM;
+    tm.NB = (Nt + tm.GN - 1) / tm.GN;
+    tm.S = Z * tm.MB * tm.NB;
+    return tm;
+}
+
+static unsigned map_grid(const TileMap &tm) { return (unsigned)(((tm.S + 7) / 8) * 8 * tm.GM * tm.GN); }
+
+int energy_superiles(int Mpad, int Bp)
+{
+    const TileMap tm = make_map(Mpad / kTile, Bp / kTile, 1);
+    return tm.MB * tm.NB;
+}
+
 void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const void *X, int Mpad,
                  int Bp, int Kxp, int ksplit, int kchunk, float *Epart)
 {
     EpiEnergy epi{Epart, (int64_t)Bp, (int64_t)Mpad * Bp};
-    const dim3 grid(Bp / kTile, Mpad / kTile, ksplit);
+    const TileMap tm = make_map(Mpad / kTile, Bp / kTile, ksplit);
+    const dim3 grid(map_grid(tm));
     if (prec == SDCA_PREC_FP32)
         hipLaunchKernelGGL((k_gemm_nt<PolF32, 1, EpiEnergy>), grid, dim3(256), 0, st, (const float *)W32,
-                           (const float *)W32, (int64_t)Kxp, (const float *)X, (int64_t)Kxp, Kxp, kchunk, epi);
+                           (const float *)W32, (int64_t)Kxp, (const float *)X, (int64_t)Kxp, Kxp, kchunk, tm, epi);
     else
         hipLaunchKernelGGL((k_gemm_nt<PolBF16, 2, EpiEnergy>), grid, dim3(256), 0, st, (const uint16_t *)Whi,
-                           (const uint16_t *)Wlo, (int64_t)Kxp, (const uint16_t *)X, (int64_t)Kxp, Kxp, kchunk, epi);
+                           (const uint16_t *)Wlo, (int64_t)Kxp, (const uint16_t *)X, (int64_t)Kxp, Kxp, kchunk, tm, epi);
 }
 
 void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad, int Bp, int B, int q, int nslots,
@@ -676,13 +735,14 @@ void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const
                int64_t n_slot, int P, float lamJ2, float lamh2)
 {
     EpiGrad epi{ring, Xtrial, cmds, n_slot, (int64_t)Kxp, P, q, Kx, nslots, lamJ2, lamh2};
-    const dim3 grid(Kxp / kTile, Mpad / kTile, 1);
+    const TileMap tm = make_map(Mpad / kTile, Kxp / kTile, 1);
+    const dim3 grid(map_grid(tm));
     if (prec == SDCA_PREC_FP32)
         hipLaunchKernelGGL((k_gemm_nt<PolF32, 1, EpiGrad>), grid, dim3(256), 0, st, (const float *)G32,
-                           (const float *)G32, (int64_t)Bp, (const float *)Xt, (int64_t)Bp, Bp, Bp, epi);
+                           (const float *)G32, (int64_t)Bp, (const float *)Xt, (int64_t)Bp, Bp, Bp, tm, epi);
     else
         hipLaunchKernelGGL((k_gemm_nt<PolBF16, 2, EpiGrad>), grid, dim3(256), 0, st, (const uint16_t *)Ghi,
-                           (const uint16_t *)Glo, (int64_t)Bp, (const uint16_t *)Xt, (int64_t)Bp, Bp, Bp, epi);
+                           (const uint16_t *)Glo, (int64_t)Bp, (const uint16_t *)Xt, (int64_t)Bp, Bp, Bp, tm, epi);
 }
 
 #define SDCA_DISPATCH_M(m, CALL) \
