@@ -138,6 +138,24 @@ int sdca_solve_loci(sdca_ctx *ctx, const int32_t *loci, int32_t K, const double 
 /* Timings accumulated since the context was created or last reset. */
 int sdca_get_timing(sdca_ctx *ctx, sdca_timing *t, int32_t reset);
 
+/* ---- pair statistics for the warm start (no counterpart in SuperDCA) ---- */
+
+/* Exact co-occurrence counts N_ij(a,b) = #{sequences with state a at locus i and b at j}
+ * for npairs listed pairs (pairs[2p], pairs[2p+1]); counts_out is npairs x q x q int32.
+ * Popcounts of per-locus sequence bit-planes. */
+int sdca_pair_counts(sdca_ctx *ctx, const int32_t *pairs, int64_t npairs, int32_t *counts_out);
+
+/* Rows [i0*QP, (i0+nloci)*QP) of the full pair-count table X8 X8^T (QP = 4 for q <= 4,
+ * else 8; row i*QP + a, column j*QP + b holds N_ij(a,b)), computed with int8 MFMA into
+ * int32.  out is (nloci*QP) x (L*QP) int32.  For checking; the table is never stored whole. */
+int sdca_pair_count_table(sdca_ctx *ctx, int32_t i0, int32_t nloci, int32_t *out);
+
+/* Mutual information MI_ij = sum_ab f_ab log(f_ab / (f_a f_b)) of the unweighted pair
+ * frequencies for all locus pairs (int8-MFMA count tiles reduced in the epilogue), and
+ * for every locus the K partners of largest MI, ties to the lower index, listed in
+ * ascending index order: partners_out is L x K; mi_out (L x K, may be NULL) their MI. */
+int sdca_mi_topk(sdca_ctx *ctx, int32_t K, int32_t *partners_out, float *mi_out);
+
 /* ---- host-side alignment pipeline (the apegrunt contract the CLI relies on) ---- */
 
 /* MAF/gap filter and q_env-state transform (SuperDCA.cpp:274-277,354; README.md:32):

@@ -37,7 +37,8 @@ ST_NONFINITE = 3
 EXPORTED = (
     "sdca_default_opts", "sdca_last_error", "sdca_ctx_create", "sdca_ctx_destroy", "sdca_num_columns",
     "sdca_reweight", "sdca_set_weights", "sdca_eval_batch", "sdca_solve_loci", "sdca_get_timing",
-    "sdca_prepare_alignment", "sdca_write_couplings",
+    "sdca_prepare_alignment", "sdca_write_couplings", "sdca_pair_counts", "sdca_pair_count_table",
+    "sdca_mi_topk",
 )
 
 
@@ -93,8 +94,12 @@ def lib() -> C.CDLL:
     L.sdca_prepare_alignment.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
                                          C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
     L.sdca_write_couplings.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
+    L.sdca_pair_counts.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+    L.sdca_pair_count_table.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
+    L.sdca_mi_topk.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
     for name in ("sdca_ctx_create", "sdca_reweight", "sdca_set_weights", "sdca_eval_batch", "sdca_solve_loci",
-                 "sdca_get_timing", "sdca_prepare_alignment", "sdca_write_couplings"):
+                 "sdca_get_timing", "sdca_prepare_alignment", "sdca_write_couplings", "sdca_pair_counts",
+                 "sdca_pair_count_table", "sdca_mi_topk"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -207,6 +212,27 @@ class Context:
             "status": np.array([st[k].status for k in range(K)], dtype=np.int64),
         }
         return S, stats, X, rc
+
+    def pair_counts(self, pairs) -> np.ndarray:
+        """Exact q x q co-occurrence counts of listed locus pairs (popcount path)."""
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        out = np.zeros((pairs.shape[0], self.q, self.q), dtype=np.int32)
+        _check(lib().sdca_pair_counts(self._h, _ptr(pairs), pairs.shape[0], _ptr(out)))
+        return out
+
+    def pair_count_table(self, i0: int, nloci: int) -> np.ndarray:
+        """Rows of the full int8-MFMA count table X8 X8^T ((nloci*QP) x (L*QP))."""
+        QP = 4 if self.q <= 4 else 8
+        out = np.zeros((nloci * QP, self.L * QP), dtype=np.int32)
+        _check(lib().sdca_pair_count_table(self._h, int(i0), int(nloci), _ptr(out)))
+        return out
+
+    def mi_topk(self, K: int):
+        """(partners L x K int32, MI L x K float32): each locus' K largest-MI partners."""
+        idx = np.zeros((self.L, K), dtype=np.int32)
+        val = np.zeros((self.L, K), dtype=np.float32)
+        _check(lib().sdca_mi_topk(self._h, int(K), _ptr(idx), _ptr(val)))
+        return idx, val
 
     def timing(self, reset: bool = False) -> dict:
         t = Timing()

@@ -119,6 +119,9 @@ sdca_ctx::~sdca_ctx()
     dfree(d_X);
     dfree(d_Xt);
     dfree(d_w);
+    dfree(d_X8);
+    dfree(d_cnt);
+    dfree(d_SP);
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -554,6 +557,25 @@ private:
     double f_noise_;  // relative rounding noise of the objective (FP32 or split-BF16 energies)
 };
 
+// One-hot int8 X8 (L*QP rows, b contiguous) and the marginal counts, built on first use.
+int ensure_x8(sdca_ctx *c)
+{
+    if (c->d_X8) return SDCA_OK;
+    c->QP = c->q <= 4 ? 4 : 8;
+    const int64_t Kc = (int64_t)c->L * c->QP;
+    c->Kc_pad = round_up(Kc, sdca::kTile);
+    c->Kc_alloc = round_up(Kc, 1024) + 1024;  // a zero panel past the end for unaligned panels
+    int rc;
+    if ((rc = dalloc(&c->d_X8, (size_t)c->Kc_alloc * c->Bp))) return rc;
+    if ((rc = dalloc(&c->d_cnt, (size_t)c->Kc_alloc))) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    sdca::kern::build_x8(c->stream, c->d_states, c->B, c->L, c->QP, c->Bp, c->d_X8);
+    sdca::kern::marginals(c->stream, c->d_states, c->B, c->L, c->QP, c->d_cnt);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SDCA_OK;
+}
+
 int check_ctx(sdca_ctx *c)
 {
     if (!c) return fail(SDCA_E_ARG, "null context");
@@ -919,6 +941,103 @@ int sdca_solve_loci(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
     c->timing.total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (nonfinite) return fail(SDCA_E_NONFINITE, "%d loci reached a non-finite objective", nonfinite);
     if (noconv) return fail(SDCA_E_NOCONV, "%d loci stopped without reaching the gradient threshold", noconv);
+    return SDCA_OK;
+}
+
+int sdca_pair_counts(sdca_ctx *c, const int32_t *pairs, int64_t npairs, int32_t *counts_out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (npairs < 0 || (npairs > 0 && (!pairs || !counts_out))) return fail(SDCA_E_ARG, "bad arguments");
+    for (int64_t p = 0; p < 2 * npairs; ++p)
+        if (pairs[p] < 0 || pairs[p] >= c->L) return fail(SDCA_E_ARG, "pair locus %d out of range", pairs[p]);
+    if (npairs == 0) return SDCA_OK;
+    const int W = (c->B + 63) / 64;
+    if (!c->d_SP) {
+        if ((rc = dalloc(&c->d_SP, (size_t)c->L * c->q * W))) return rc;
+        HIP_TRY(hipDeviceSynchronize());
+        sdca::kern::seq_planes(c->stream, c->d_states, c->B, c->L, c->q, W, c->d_SP);
+        HIP_TRY(hipGetLastError());
+    }
+    int32_t *d_pairs = nullptr, *d_counts = nullptr;
+    const int64_t qq = (int64_t)c->q * c->q;
+    if ((rc = dalloc(&d_pairs, (size_t)npairs * 2)) || (rc = dalloc(&d_counts, (size_t)npairs * qq))) {
+        dfree(d_pairs);
+        return rc;
+    }
+    hipError_t e = hipMemcpy(d_pairs, pairs, (size_t)npairs * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        sdca::kern::pair_counts(c->stream, c->d_SP, c->q, W, d_pairs, npairs, d_counts);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(counts_out, d_counts, (size_t)npairs * qq * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(d_pairs);
+    dfree(d_counts);
+    if (e != hipSuccess) return fail(SDCA_E_HIP, "pair counts failed: %s", hipGetErrorString(e));
+    return SDCA_OK;
+}
+
+int sdca_pair_count_table(sdca_ctx *c, int32_t i0, int32_t nloci, int32_t *out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (i0 < 0 || nloci < 1 || i0 + nloci > c->L || !out) return fail(SDCA_E_ARG, "bad row range");
+    if ((rc = ensure_x8(c))) return rc;
+    const int QP = c->QP;
+    const int rows = nloci * QP;
+    const int Mrows = (int)round_up(rows, sdca::kTile);
+    int32_t *d_tab = nullptr;
+    if ((rc = dalloc(&d_tab, (size_t)Mrows * c->Kc_pad))) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    sdca::kern::pair_gemm(c->stream, 0, QP, c->d_X8 + (size_t)i0 * QP * c->Bp, c->d_X8, c->Bp, Mrows, (int)c->Kc_pad,
+                          i0, c->L, c->B, c->d_cnt, d_tab, nullptr);
+    hipError_t e = hipGetLastError();
+    const int64_t Kc = (int64_t)c->L * QP;
+    if (e == hipSuccess)
+        e = hipMemcpy2DAsync(out, Kc * 4, d_tab, c->Kc_pad * 4, Kc * 4, rows, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(d_tab);
+    if (e != hipSuccess) return fail(SDCA_E_HIP, "pair count table failed: %s", hipGetErrorString(e));
+    return SDCA_OK;
+}
+
+int sdca_mi_topk(sdca_ctx *c, int32_t K, int32_t *partners_out, float *mi_out)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (K < 1 || K > c->L - 1 || !partners_out) return fail(SDCA_E_ARG, "K must be in [1, L-1] (got %d)", K);
+    if ((rc = ensure_x8(c))) return rc;
+    const int QP = c->QP, L = c->L;
+    const int Mrows = (int)std::min<int64_t>(1024, round_up((int64_t)L * QP, sdca::kTile));
+    const int PL = Mrows / QP;  // loci per panel
+    float *d_mi = nullptr, *d_val = nullptr;
+    int32_t *d_idx = nullptr;
+    auto cleanup = [&]() {
+        dfree(d_mi);
+        dfree(d_val);
+        dfree(d_idx);
+    };
+    if ((rc = dalloc(&d_mi, (size_t)PL * L)) || (rc = dalloc(&d_idx, (size_t)L * K)) ||
+        (mi_out && (rc = dalloc(&d_val, (size_t)L * K)))) {
+        cleanup();
+        return rc;
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    hipError_t e = hipSuccess;
+    for (int p0 = 0; p0 < L && e == hipSuccess; p0 += PL) {
+        sdca::kern::pair_gemm(c->stream, 1, QP, c->d_X8 + (size_t)p0 * QP * c->Bp, c->d_X8, c->Bp, Mrows,
+                              (int)c->Kc_pad, p0, L, c->B, c->d_cnt, nullptr, d_mi);
+        sdca::kern::topk(c->stream, d_mi, std::min(PL, L - p0), L, K, p0, d_idx, d_val);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(partners_out, d_idx, (size_t)L * K * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && mi_out)
+        e = hipMemcpyAsync(mi_out, d_val, (size_t)L * K * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(SDCA_E_HIP, "MI top-K failed: %s", hipGetErrorString(e));
     return SDCA_OK;
 }
 

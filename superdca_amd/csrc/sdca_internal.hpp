@@ -86,6 +86,15 @@ void reduce_results(hipStream_t st, int m, int nslots, const double *upd_part, c
                     const double *sm_part, int nchunk, int nbchunk, double *res);
 void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *loci, int nscore, int L,
            int q, int gap, int Kxp, const int32_t *state_col, float *rows);
+// sdca_pairs.hip
+void build_x8(hipStream_t st, const uint8_t *states, int B, int L, int QP, int Bp, int8_t *X8);
+void marginals(hipStream_t st, const uint8_t *states, int B, int L, int QP, int32_t *cnt);
+void pair_gemm(hipStream_t st, int mode, int QP, const int8_t *Apan, const int8_t *Ball, int Bp, int Mrows,
+               int Ncols, int i0, int L, int B, const int32_t *cnt, int32_t *Cout, float *MI);
+void topk(hipStream_t st, const float *MI, int rows, int L, int K, int i0, int32_t *out_idx, float *out_val);
+void seq_planes(hipStream_t st, const uint8_t *states, int B, int L, int q, int W, uint64_t *SP);
+void pair_counts(hipStream_t st, const uint64_t *SP, int q, int W, const int32_t *pairs, int64_t npairs,
+                 int32_t *counts);
 }  // namespace kern
 
 }  // namespace sdca
@@ -126,6 +135,13 @@ struct sdca_ctx {
     float *d_rows = nullptr;
     int64_t *d_src_off = nullptr;
     int32_t *d_score_loci = nullptr;
+
+    // pair statistics (sdca_pairs.hip), built on first use
+    int QP = 0;
+    int64_t Kc_pad = 0, Kc_alloc = 0;
+    int8_t *d_X8 = nullptr;
+    int32_t *d_cnt = nullptr;
+    uint64_t *d_SP = nullptr;  // sequence bit-planes L x q x ceil(B/64)
 
     hipEvent_t ev[8] = {};
     sdca::Timing timing;
