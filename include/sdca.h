@@ -156,6 +156,40 @@ int sdca_pair_count_table(sdca_ctx *ctx, int32_t i0, int32_t nloci, int32_t *out
  * ascending index order: partners_out is L x K; mi_out (L x K, may be NULL) their MI. */
 int sdca_mi_topk(sdca_ctx *ctx, int32_t K, int32_t *partners_out, float *mi_out);
 
+/* ---- warm starts: pair features and the learned network (no counterpart in SuperDCA) ---- */
+
+/* Features of locus pairs (r, j) = (pairs[2p], pairs[2p+1]): the joint frequencies
+ * f_rj(s, a) at [s*q + a], the marginals f_r(s), f_j(a) and the pair's mutual information,
+ * q*q + 2q + 1 floats per pair (unweighted frequencies from exact counts). */
+int sdca_pair_features(sdca_ctx *ctx, const int32_t *pairs, int64_t npairs, float *feats);
+
+/* The warm-start network: features -> hidden -> hidden (ReLU) -> q*q outputs, where output
+ * s*q + a is the start value of J_r[j](a, s).  FP32 parameters on the device.
+ * sdca_mlp_init draws He-uniform weights from seed; sdca_mlp_set_params installs given
+ * weights (the layout W1 [H][Din], b1, W2 [H][H], b2, W3 [Dout][H], b3). */
+int sdca_mlp_init(sdca_ctx *ctx, int32_t hidden, uint64_t seed);
+int64_t sdca_mlp_num_params(const sdca_ctx *ctx);
+int sdca_mlp_get_params(sdca_ctx *ctx, float *out, int64_t n);
+int sdca_mlp_set_params(sdca_ctx *ctx, int32_t hidden, const float *params, int64_t n);
+int sdca_mlp_forward(sdca_ctx *ctx, const float *x, int64_t n, float *y);
+
+/* Masked mean squared error (1/sum(mask)) sum mask (y - t)^2 of n rows and its gradient
+ * with respect to all parameters (hand-written backward kernels). mask may be NULL. */
+int sdca_mlp_loss_grad(sdca_ctx *ctx, const float *x, const float *t, const float *mask, int64_t n, double *loss,
+                       float *grad);
+
+/* Adam training on the device: epochs over the n rows in seeded shuffled mini-batches;
+ * epoch_loss (epochs entries, may be NULL) receives the mean masked squared error. */
+int sdca_mlp_train(sdca_ctx *ctx, const float *x, const float *t, const float *mask, int64_t n, int32_t epochs,
+                   int32_t batch, float lr, uint64_t seed, double *epoch_loss);
+
+/* sdca_solve_loci with warm starts built on the device: for each locus its npartners
+ * largest-MI partners (sdca_mi_topk) receive the network's blocks, the fields start at
+ * the J = 0 optimum, all other couplings at zero.  The optimum is the cold start's
+ * (the problem is strictly convex); only the iteration count changes. */
+int sdca_solve_loci_mlp(sdca_ctx *ctx, const int32_t *loci, int32_t K, int32_t npartners, float *score_rows,
+                        double *x_out, sdca_locus_stats *st);
+
 /* ---- host-side alignment pipeline (the apegrunt contract the CLI relies on) ---- */
 
 /* MAF/gap filter and q_env-state transform (SuperDCA.cpp:274-277,354; README.md:32):

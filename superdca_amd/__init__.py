@@ -38,7 +38,8 @@ EXPORTED = (
     "sdca_default_opts", "sdca_last_error", "sdca_ctx_create", "sdca_ctx_destroy", "sdca_num_columns",
     "sdca_reweight", "sdca_set_weights", "sdca_eval_batch", "sdca_solve_loci", "sdca_get_timing",
     "sdca_prepare_alignment", "sdca_write_couplings", "sdca_pair_counts", "sdca_pair_count_table",
-    "sdca_mi_topk",
+    "sdca_mi_topk", "sdca_pair_features", "sdca_mlp_init", "sdca_mlp_num_params", "sdca_mlp_get_params",
+    "sdca_mlp_set_params", "sdca_mlp_forward", "sdca_mlp_loss_grad", "sdca_mlp_train", "sdca_solve_loci_mlp",
 )
 
 
@@ -97,9 +98,24 @@ def lib() -> C.CDLL:
     L.sdca_pair_counts.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
     L.sdca_pair_count_table.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
     L.sdca_mi_topk.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+    L.sdca_pair_features.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+    L.sdca_mlp_init.argtypes = [C.c_void_p, C.c_int32, C.c_uint64]
+    L.sdca_mlp_num_params.argtypes = [C.c_void_p]
+    L.sdca_mlp_num_params.restype = C.c_int64
+    L.sdca_mlp_get_params.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+    L.sdca_mlp_set_params.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]
+    L.sdca_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+    L.sdca_mlp_loss_grad.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p,
+                                     C.c_void_p]
+    L.sdca_mlp_train.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
+                                 C.c_float, C.c_uint64, C.c_void_p]
+    L.sdca_solve_loci_mlp.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                      C.c_void_p]
     for name in ("sdca_ctx_create", "sdca_reweight", "sdca_set_weights", "sdca_eval_batch", "sdca_solve_loci",
                  "sdca_get_timing", "sdca_prepare_alignment", "sdca_write_couplings", "sdca_pair_counts",
-                 "sdca_pair_count_table", "sdca_mi_topk"):
+                 "sdca_pair_count_table", "sdca_mi_topk", "sdca_pair_features", "sdca_mlp_init",
+                 "sdca_mlp_get_params", "sdca_mlp_set_params", "sdca_mlp_forward", "sdca_mlp_loss_grad",
+                 "sdca_mlp_train", "sdca_solve_loci_mlp"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -233,6 +249,68 @@ class Context:
         val = np.zeros((self.L, K), dtype=np.float32)
         _check(lib().sdca_mi_topk(self._h, int(K), _ptr(idx), _ptr(val)))
         return idx, val
+
+    # ---- warm starts
+
+    def pair_features(self, pairs) -> np.ndarray:
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        D = self.q * self.q + 2 * self.q + 1
+        out = np.zeros((pairs.shape[0], D), dtype=np.float32)
+        _check(lib().sdca_pair_features(self._h, _ptr(pairs), pairs.shape[0], _ptr(out)))
+        return out
+
+    def mlp_init(self, hidden: int = 128, seed: int = 0) -> None:
+        _check(lib().sdca_mlp_init(self._h, int(hidden), int(seed)))
+        self.mlp_hidden = int(hidden)
+
+    def mlp_params(self) -> np.ndarray:
+        n = int(lib().sdca_mlp_num_params(self._h))
+        out = np.zeros(max(n, 0), dtype=np.float32)
+        _check(lib().sdca_mlp_get_params(self._h, _ptr(out), n))
+        return out
+
+    def mlp_set_params(self, params: np.ndarray, hidden: int = 128) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        _check(lib().sdca_mlp_set_params(self._h, int(hidden), _ptr(p), p.size))
+        self.mlp_hidden = int(hidden)
+
+    def mlp_forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], self.q * self.q), dtype=np.float32)
+        _check(lib().sdca_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(y)))
+        return y
+
+    def mlp_loss_grad(self, x, t, mask=None):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
+        loss = np.zeros(1)
+        g = np.zeros(int(lib().sdca_mlp_num_params(self._h)), dtype=np.float32)
+        _check(lib().sdca_mlp_loss_grad(self._h, _ptr(x), _ptr(t), _ptr(m), x.shape[0], _ptr(loss), _ptr(g)))
+        return float(loss[0]), g
+
+    def mlp_train(self, x, t, mask=None, epochs=10, batch=4096, lr=1e-3, seed=0) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.float32)
+        losses = np.zeros(epochs)
+        _check(lib().sdca_mlp_train(self._h, _ptr(x), _ptr(t), _ptr(m), x.shape[0], int(epochs), int(batch),
+                                    float(lr), int(seed), _ptr(losses)))
+        return losses
+
+    def solve_mlp(self, loci, npartners: int, want_x: bool = False, rows: bool = True):
+        """Batched solve from network warm starts -> (rows, stats, x, rc) like solve()."""
+        loci = np.ascontiguousarray(loci, dtype=np.int32)
+        K = loci.size
+        n = nparams(self.L, self.q)
+        S = np.zeros((K, self.L), dtype=np.float32) if rows else None
+        X = np.zeros((K, n), dtype=np.float64) if want_x else None
+        st = (LocusStats * max(K, 1))()
+        rc = _check(lib().sdca_solve_loci_mlp(self._h, _ptr(loci), K, int(npartners), _ptr(S), _ptr(X),
+                                              C.cast(st, C.c_void_p)), allow=(SDCA_E_NOCONV,))
+        stats = {k: np.array([getattr(st[i], k) for i in range(K)]) for k in ("fval", "gnorm", "nfeval", "iter",
+                                                                               "status")}
+        return S, stats, X, rc
 
     def timing(self, reset: bool = False) -> dict:
         t = Timing()

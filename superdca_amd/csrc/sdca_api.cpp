@@ -36,8 +36,22 @@ size_t ref_index(int L, int q, int j, int a, int s);  // sdca_host.cpp
 }
 
 namespace {
-
 thread_local std::string g_err;
+}
+
+// shared with sdca_warm.cpp
+int sdca_fail(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+namespace {
 
 int fail(int code, const char *fmt, ...)
 {
@@ -122,6 +136,8 @@ sdca_ctx::~sdca_ctx()
     dfree(d_X8);
     dfree(d_cnt);
     dfree(d_SP);
+    dfree(d_topk);
+    delete mlp;
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -765,6 +781,7 @@ int sdca_set_weights(sdca_ctx *c, const double *w, double lambda_h_user, double 
     c->lambda_J = (lambda_J_user < 0.0) ? auto_lambda * beff / 2.0 : lambda_J_user * beff;  // :265-274
     c->lambda_h = (lambda_h_user < 0.0) ? auto_lambda * beff : lambda_h_user * beff;        // :276-284
     c->B_eff = beff;
+    c->h_w.assign(w, w + c->B);
     std::vector<float> wf(c->Bp, 0.f);
     for (int i = 0; i < c->B; ++i) wf[i] = (float)w[i];
     HIP_TRY(hipMemcpy(c->d_w, wf.data(), (size_t)c->Bp * 4, hipMemcpyHostToDevice));
@@ -841,6 +858,14 @@ int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x
 int sdca_solve_loci(sdca_ctx *c, const int32_t *loci, int32_t K, const double *warm, float *score_rows,
                     double *x_out, sdca_locus_stats *st)
 {
+    return sdca_solve_impl(c, loci, K, warm, nullptr, score_rows, x_out, st);
+}
+
+}  // extern "C"
+
+int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *warm, const sdca::CompactWarm *cw,
+                    float *score_rows, double *x_out, sdca_locus_stats *st)
+{
     int rc = check_ctx(c);
     if (rc) return rc;
     if (!c->have_weights) return fail(SDCA_E_STATE, "sdca_set_weights must be called first");
@@ -851,16 +876,27 @@ int sdca_solve_loci(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
     const auto t0 = std::chrono::steady_clock::now();
     const int nslots = std::min(c->opts.batch_loci, K);
     if ((rc = ensure_workspace(c, nslots, c->opts.history))) return rc;
-    if (warm && !c->d_X0) {
+    const bool any_warm = warm || cw;
+    if (any_warm && !c->d_X0) {
         if ((rc = dalloc(&c->d_X0, (size_t)nslots * c->n_slot))) return rc;
         HIP_TRY(hipDeviceSynchronize());
     }
+    int32_t *d_wl = nullptr;  // compact warm starts: slot, item, locus of each refill
+    if (cw) {
+        if ((rc = dalloc(&d_wl, (size_t)3 * nslots))) return rc;
+        HIP_TRY(hipDeviceSynchronize());
+    }
+    struct Guard {
+        int32_t *&p;
+        ~Guard() { dfree(p); }
+    } guard{d_wl};
     const int m = c->ws_m, P = 2 * m + 3;
     const size_t n = (size_t)c->L * c->q * c->q + c->q;
     const sdca::ResLayout R{P};
     Solver S(c, nslots);
     std::vector<float> stage(warm ? c->n_slot : 0), xdev(x_out ? c->n_slot : 0), rowbuf;
     std::vector<double> res;
+    std::vector<int32_t> wl_slot, wl_item, wl_locus;
     int next = 0, done = 0, noconv = 0, nonfinite = 0;
     auto fill = [&](int i) -> int {
         if (next >= K) {
@@ -873,11 +909,36 @@ int sdca_solve_loci(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
             (void)ref_to_dev(c, r, warm + (size_t)item * n, stage.data());
             HIP_TRY(hipMemcpy(c->d_X0 + (size_t)i * c->n_slot, stage.data(), c->n_slot * 4, hipMemcpyHostToDevice));
         }
-        S.assign(i, item, r, warm != nullptr);
+        if (cw) {
+            wl_slot.push_back(i);
+            wl_item.push_back(item);
+            wl_locus.push_back(r);
+        }
+        S.assign(i, item, r, any_warm);
+        return SDCA_OK;
+    };
+    // stage the compact warm starts of this round's refills into their slots' X0 vectors
+    auto stage_compact = [&]() -> int {
+        if (!cw || wl_slot.empty()) return SDCA_OK;
+        const int nw = (int)wl_slot.size();
+        std::vector<int32_t> packed(wl_slot);
+        packed.insert(packed.end(), wl_item.begin(), wl_item.end());
+        packed.insert(packed.end(), wl_locus.begin(), wl_locus.end());
+        HIP_TRY(hipMemcpyAsync(d_wl, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, c->stream));
+        for (int i : wl_slot)
+            HIP_TRY(hipMemsetAsync(c->d_X0 + (size_t)i * c->n_slot, 0, c->n_slot * 4, c->stream));
+        sdca::kern::scatter_warm(c->stream, d_wl, d_wl + nw, d_wl + 2 * nw, cw->d_partners, cw->d_J, cw->d_h, nw,
+                                 cw->P, c->q, c->Kxp, (int)c->Kx, c->d_state_col, c->n_slot, c->d_X0);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(c->stream));  // packed is a host temporary
+        wl_slot.clear();
+        wl_item.clear();
+        wl_locus.clear();
         return SDCA_OK;
     };
     for (int i = 0; i < nslots; ++i)
         if ((rc = fill(i))) return rc;
+    if ((rc = stage_compact())) return rc;
     std::vector<int> finished;
     std::vector<int64_t> src_off;
     std::vector<int32_t> sloci;
@@ -935,6 +996,7 @@ int sdca_solve_loci(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
             ++done;
             if ((rc = fill(i))) return rc;
         }
+        if ((rc = stage_compact())) return rc;
         if (c->opts.verbose)
             fprintf(stderr, "sdca: %d / %d loci solved after %lld rounds\n", done, K, (long long)rounds);
     }
@@ -943,6 +1005,8 @@ int sdca_solve_loci(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
     if (noconv) return fail(SDCA_E_NOCONV, "%d loci stopped without reaching the gradient threshold", noconv);
     return SDCA_OK;
 }
+
+extern "C" {
 
 int sdca_pair_counts(sdca_ctx *c, const int32_t *pairs, int64_t npairs, int32_t *counts_out)
 {
@@ -1036,6 +1100,12 @@ int sdca_mi_topk(sdca_ctx *c, int32_t K, int32_t *partners_out, float *mi_out)
     if (e == hipSuccess && mi_out)
         e = hipMemcpyAsync(mi_out, d_val, (size_t)L * K * 4, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) {  // keep the partner table for warm starts
+        dfree(c->d_topk);
+        c->d_topk = d_idx;
+        c->topk_K = K;
+        d_idx = nullptr;
+    }
     cleanup();
     if (e != hipSuccess) return fail(SDCA_E_HIP, "MI top-K failed: %s", hipGetErrorString(e));
     return SDCA_OK;

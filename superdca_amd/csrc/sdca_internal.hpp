@@ -95,9 +95,50 @@ void topk(hipStream_t st, const float *MI, int rows, int L, int K, int i0, int32
 void seq_planes(hipStream_t st, const uint8_t *states, int B, int L, int q, int W, uint64_t *SP);
 void pair_counts(hipStream_t st, const uint64_t *SP, int q, int W, const int32_t *pairs, int64_t npairs,
                  int32_t *counts);
+// sdca_mlp.hip
+void linear_fwd(hipStream_t st, bool relu, const float *X, const float *W, const float *b, float *Y, int64_t N, int I,
+                int O);
+void linear_bwd_dx(hipStream_t st, const float *dY, const float *W, const float *H, float *dX, int64_t N, int I, int O);
+void linear_bwd_dw(hipStream_t st, const float *dY, const float *X, float *dW, float *db, int64_t N, int I, int O);
+int mse(hipStream_t st, const float *Y, const float *T, const float *M, float *dY, int64_t total, float scale,
+        double *part, double *loss_acc);
+void adam(hipStream_t st, float *p, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2,
+          float eps, float c1, float c2);
+void gather_rows(hipStream_t st, const float *src, const int64_t *idx, int64_t nrows, int D, float *out);
+void pair_features(hipStream_t st, const int32_t *counts, int64_t npairs, int q, int B, float *feats);
+void scatter_warm(hipStream_t st, const int32_t *slot_of, const int32_t *item_of, const int32_t *locus_of,
+                  const int32_t *partners, const float *J, const float *h, int nwarm, int P, int q, int64_t Kxp, int Kx,
+                  const int32_t *state_col, int64_t n_slot, float *X0);
 }  // namespace kern
 
+// Warm starts of a solve call in compact device form: for item k (the k-th listed locus)
+// P partner loci, the network's q x q blocks for them and the fields h.
+struct CompactWarm {
+    int P = 0;
+    const int32_t *d_partners = nullptr;  // K x P
+    const float *d_J = nullptr;           // K x P x q*q
+    const float *d_h = nullptr;           // K x q
+};
+
+// The warm-start network (sdca_mlp.hip) and its training state.
+struct Mlp {
+    int din = 0, dh = 0, dout = 0;
+    int64_t np = 0, step = 0;
+    float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
+    int64_t cap = 0;  // rows of the scratch buffers below
+    float *d_x = nullptr, *d_t = nullptr, *d_msk = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_y = nullptr;
+    float *d_dy = nullptr, *d_dz2 = nullptr, *d_dz1 = nullptr;
+    double *d_part = nullptr, *d_loss = nullptr;
+    int64_t *d_idx = nullptr;
+    ~Mlp();
+};
+
 }  // namespace sdca
+
+// sdca_api.cpp: the batched solve with an optional warm-start source (dense reference
+// layout on the host, or compact on the device).
+int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *warm_dense,
+                    const sdca::CompactWarm *cw, float *score_rows, double *x_out, sdca_locus_stats *st);
 
 struct sdca_ctx {
     sdca_opts opts{};
@@ -142,6 +183,12 @@ struct sdca_ctx {
     int8_t *d_X8 = nullptr;
     int32_t *d_cnt = nullptr;
     uint64_t *d_SP = nullptr;  // sequence bit-planes L x q x ceil(B/64)
+
+    // MI top-K partner table kept on the device (sdca_mi_topk)
+    int32_t *d_topk = nullptr;
+    int topk_K = 0;
+    std::vector<double> h_w;  // installed weights (host copy)
+    sdca::Mlp *mlp = nullptr;
 
     hipEvent_t ev[8] = {};
     sdca::Timing timing;

@@ -297,3 +297,28 @@ def test_score_row_reads_block_of_each_partner():
         M = np.array([[x[orc.ref_index(L, q, n, a, s)] for s in range(q)] for a in range(q)])
         G = orc.ising_gauge(M, transpose=n < r)
         assert row[n] == np.float32(np.sqrt((G[:3, :3] ** 2).sum()))
+
+
+# ------------------------------------------------------- warm-start network
+
+def test_mlp_reference_gradient_finite_differences():
+    from oracle import mlp_ref
+
+    rng = np.random.default_rng(13)
+    din, dh, dout, n = 25, 12, 16, 9
+    npar = dh * din + dh + dh * dh + dh + dout * dh + dout
+    p = rng.normal(size=npar) * 0.4
+    x = rng.random((n, din))
+    t = rng.normal(size=(n, dout))
+    mask = (rng.random((n, dout)) < 0.8).astype(float)
+    loss, g = mlp_ref.loss_grad(p, x, t, mask, din, dh, dout)
+    h = 1e-6
+    for i in rng.choice(npar, 60, replace=False):
+        e = np.zeros(npar)
+        e[i] = h
+        fd = (mlp_ref.loss_grad(p + e, x, t, mask, din, dh, dout)[0] -
+              mlp_ref.loss_grad(p - e, x, t, mask, din, dh, dout)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i]))
+    # the loss is the masked mean squared error
+    y, _ = mlp_ref.forward(p, x, din, dh, dout)
+    assert loss == pytest.approx((((y - t) * mask) ** 2).sum() / mask.sum(), rel=1e-14)
