@@ -1,13 +1,23 @@
 """Throughput of the per-locus plmDCA solve on one MI355X: loci/s to convergence.
 
-A step is one batch of loci (2048 at genome scale) solved from a cold start to ||g||_inf < 1e-3 (SuperDCA's default stopping rule) on the genome-scale
-workload B = 3000 sequences x N = 94,000 loci (seeded synthetic alignment, 60 clades,
-2 % mutation, 2000 planted pairs).  Timing starts with the alignment, weights and
-lambda resident on the device and ends when the last score row is in host memory.
+Workload: B = 3000 sequences x N = 94,000 loci (seeded synthetic genome alignment, 60
+clades, 2 % mutation, 2000 planted pairs), q = 4 envelope, L-BFGS m = 10, stop at
+||g||_inf < 1e-3 (SuperDCA's default rule).  A step is a batch of loci solved to
+convergence with 256 loci in flight on the GPU (finished slots are refilled).  Timing
+starts with the alignment, weights and lambda resident on the device and ends when the
+last score row is in host memory.
 
-Prints one JSON line (rank 0): metric, value (loci/s), roofline of the dominant kernel
-(measured with HIP events on the library's stream), and a CPU baseline: the FP64 oracle
-(block-compressed, one locus per thread) timed on a bounded sample of the same workload.
+Three legs on the same GPU, each on its own loci:
+  warm   split-BF16 contractions, warm starts from the network (trained beforehand on
+         cold-start solutions of the C1 and C2 workloads, untimed) for the top-2048 MI
+         partners of every locus -- the headline value
+  cold   split-BF16, cold start
+  fp32   the plain path: FP32 MFMA contractions, cold start
+The MI partner table of the alignment is built once (reported, not in the loci/s).
+
+Prints one JSON line: value (warm loci/s), the roofline of the dominant kernel of the
+headline leg (HIP events on the library's stream), and a CPU baseline: the FP64 oracle
+(block-compressed, one locus per thread) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -30,6 +40,67 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def run_steps(ctx, order, pos, per_step, steps, label, solve):
+    """Times `steps` solve calls; returns (loci/s, iterations, evaluations, seconds, new pos)."""
+    import superdca_amd as sd
+
+    ctx.timing(reset=True)
+    converged = 0
+    iters, evals = [], []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        _, st, _, _ = solve(order[pos:pos + per_step])
+        pos += per_step
+        converged += int((st["status"] == sd.ST_CONVERGED).sum())
+        iters.append(st["iter"])
+        evals.append(st["nfeval"])
+        log(f"[bench] {label} step {k}: {converged} converged, {time.perf_counter() - t0:.2f}s")
+    elapsed = time.perf_counter() - t0
+    return converged / elapsed, np.concatenate(iters), np.concatenate(evals), elapsed, pos
+
+
+def roofline(tm, split):
+    groups = {
+        "k_gemm_nt<EpiGrad>": (tm["gemm_grad_ms"], tm["flops_grad"], "mfma"),
+        "k_gemm_nt<EpiEnergy>": (tm["gemm_energy_ms"], tm["flops_energy"], "mfma"),
+        "k_update": (tm["update_ms"], tm["bytes_update"], "hbm"),
+        "k_dots": (tm["dots_ms"], tm["bytes_dots"], "hbm"),
+    }
+    dom = max(groups, key=lambda k: groups[k][0])
+    ms, work, bound = groups[dom]
+    if bound == "mfma":
+        peak = PEAK["bf16_tflops"] if split else PEAK["fp32_tflops"]
+        achieved, unit = work / (ms * 1e-3) / 1e12, "TFLOP/s"
+    else:
+        peak = PEAK["hbm_gbs"]
+        achieved, unit = work / (ms * 1e-3) / 1e9, "GB/s"
+    rates = {k: round(v[1] / (v[0] * 1e-3) / (1e12 if v[2] == "mfma" else 1e9), 2) for k, v in groups.items() if v[0]}
+    return ({"bound": bound, "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+             "frac": round(achieved / peak, 5), "traffic": None}, rates)
+
+
+def device_ms(tm):
+    return {k: round(tm[k], 1) for k in ("gemm_energy_ms", "gemm_grad_ms", "softmax_ms", "update_ms", "dots_ms",
+                                         "score_ms")}
+
+
+def train_network(seed=0):
+    """The warm-start network, trained on cold-start solutions of the C1 and C2 workloads."""
+    import superdca_amd as sd
+    from superdca_amd import synth, warmstart
+
+    feats, tgts, masks = [], [], []
+    for wl in ("C1", "C2"):
+        st, _ = sd.prepare_alignment(synth.workload(wl)["chars"])
+        f, t, m, _ = warmstart.training_set(st, n_loci=400, top=32, rand=32, seed=seed)
+        feats.append(f)
+        tgts.append(t)
+        masks.append(m)
+    params, losses = warmstart.train(np.concatenate(feats), np.concatenate(tgts), np.concatenate(masks), st,
+                                     epochs=40, batch=2048, lr=2e-3, seed=seed)
+    return params, losses, sum(f.shape[0] for f in feats)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -37,10 +108,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="C4", choices=["C1", "C2", "C4"])
     ap.add_argument("--loci-per-step", type=int, default=0, help="default: 2048 at C4, 1024 otherwise")
-    ap.add_argument("--slots", type=int, default=0,
-                    help="loci in flight on the GPU (default 256 at C4, 128 otherwise); finished slots are refilled")
-    ap.add_argument("--precision", default="", choices=["", "fp32", "split-bf16"])
+    ap.add_argument("--slots", type=int, default=0, help="loci in flight (default 256 at C4, 128 otherwise)")
     ap.add_argument("--history", type=int, default=10)
+    ap.add_argument("--partners", type=int, default=2048, help="warm-started partners per locus")
+    ap.add_argument("--fp32-loci", type=int, default=1024, help="loci of the FP32 cold leg (0: skip)")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm leg (value = cold split-BF16)")
     ap.add_argument("--cpu-loci", type=int, default=0, help="CPU baseline sample size (default: one per thread)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -55,75 +127,71 @@ def main() -> int:
     from superdca_amd import synth
 
     wl = args.workload
-    cfg = synth.WORKLOADS[wl]
-    prec_name = args.precision or ("split-bf16" if wl == "C4" else "fp32")
-    prec = sd.PREC_SPLIT_BF16 if prec_name == "split-bf16" else sd.PREC_FP32
     per_step = args.loci_per_step or (2048 if wl == "C4" else 1024)
     slots = args.slots or (256 if wl == "C4" else 128)
 
     t = time.time()
-    data = synth.workload(wl)
-    states, translation = sd.prepare_alignment(data["chars"])
-    del data
+    states, translation = sd.prepare_alignment(synth.workload(wl)["chars"])
     B, L = states.shape
+    partners = min(args.partners, L - 1)
     log(f"[bench] workload {wl}: B={B} L={L} generated in {time.time() - t:.1f}s")
+    order = np.random.default_rng(1234).permutation(L).astype(np.int32)
+    need = (2 * args.warmup + 2 * args.steps) * per_step + args.fp32_loci + slots
+    if need > L:
+        order = np.resize(order, need)
+    pos = 0
 
-    t = time.time()
-    ctx = sd.Context(states, q=4, precision=prec, batch_loci=slots, history=args.history, grad_threshold=1e-3)
-    log(f"[bench] context (one-hot X, K_x={ctx.num_columns}) built in {time.time() - t:.1f}s")
+    net = None
+    if not args.no_warm:
+        t = time.time()
+        net = train_network()
+        log(f"[bench] network trained on {net[2]} pairs in {time.time() - t:.1f}s, loss {net[1][0]:.3e} -> "
+            f"{net[1][-1]:.3e}")
+
+    ctx = sd.Context(states, q=4, precision=sd.PREC_SPLIT_BF16, batch_loci=slots, history=args.history,
+                     grad_threshold=1e-3)
     t = time.time()
     w, _ = ctx.reweight(0.9)
     reweight_s = time.time() - t
     beff, lh, lJ = ctx.set_weights(w)
-    log(f"[bench] reweighting {reweight_s * 1e3:.1f} ms, B_eff={beff:.2f} lambda_h={lh:.3f} lambda_J={lJ:.3f}")
+    log(f"[bench] K_x={ctx.num_columns}, reweighting {reweight_s * 1e3:.1f} ms, B_eff={beff:.2f}")
 
-    order = np.random.default_rng(1234).permutation(L).astype(np.int32)
-    need = (args.warmup + args.steps) * per_step
-    if need > L:
-        order = np.resize(order, need)
-    pos = 0
+    # cold split-BF16
     for k in range(args.warmup):
-        t = time.time()
         _, wst, _, _ = ctx.solve(order[pos:pos + per_step])
         assert np.all(wst["nfeval"] > 1), "warmup loci stopped at their start point"
         pos += per_step
-        log(f"[bench] warmup step {k} {time.time() - t:.2f}s")
-    ctx.timing(reset=True)
-    converged = 0
-    iters, evals = [], []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        rows, stats, _, rc = ctx.solve(order[pos:pos + per_step])
-        pos += per_step
-        converged += int((stats["status"] == sd.ST_CONVERGED).sum())
-        iters.append(stats["iter"])
-        evals.append(stats["nfeval"])
-        log(f"[bench] step {k} done: {converged} converged so far, {time.perf_counter() - t0:.2f}s")
-    elapsed = time.perf_counter() - t0
-    tm = ctx.timing()
-    iters = np.concatenate(iters)
-    evals = np.concatenate(evals)
-    value = converged / elapsed
+    cold_v, cold_it, cold_ev, cold_s, pos = run_steps(ctx, order, pos, per_step, args.steps, "cold-split", ctx.solve)
+    cold_tm = ctx.timing()
 
-    # dominant kernel group by device time
-    groups = {
-        "k_gemm_nt<EpiGrad>": (tm["gemm_grad_ms"], tm["flops_grad"], "mfma"),
-        "k_gemm_nt<EpiEnergy>": (tm["gemm_energy_ms"], tm["flops_energy"], "mfma"),
-        "k_update": (tm["update_ms"], tm["bytes_update"], "hbm"),
-        "k_dots": (tm["dots_ms"], tm["bytes_dots"], "hbm"),
-    }
-    dom = max(groups, key=lambda k: groups[k][0])
-    ms, work, bound = groups[dom]
-    if bound == "mfma":
-        peak = PEAK["bf16_tflops"] if prec == sd.PREC_SPLIT_BF16 else PEAK["fp32_tflops"]
-        achieved = work / (ms * 1e-3) / 1e12
-        unit = "TFLOP/s"
-    else:
-        peak = PEAK["hbm_gbs"]
-        achieved = work / (ms * 1e-3) / 1e9
-        unit = "GB/s"
-    roofline = {"bound": bound, "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                "frac": round(achieved / peak, 5), "traffic": None}
+    warm = None
+    if net is not None:
+        ctx.mlp_set_params(net[0], 128)
+        t = time.time()
+        ctx.mi_topk(partners)
+        mi_s = time.time() - t
+        log(f"[bench] MI top-{partners} partner table in {mi_s:.2f}s")
+        solve_w = lambda loci: ctx.solve_mlp(loci, partners)
+        for k in range(args.warmup):
+            solve_w(order[pos:pos + per_step // 4])
+            pos += per_step // 4
+        warm_v, warm_it, warm_ev, warm_s, pos = run_steps(ctx, order, pos, per_step, args.steps, "warm-split", solve_w)
+        warm = {"loci_s": warm_v, "iter": warm_it, "evals": warm_ev, "tm": ctx.timing(), "mi_s": mi_s}
+    ctx.close()
+
+    fp32 = None
+    if args.fp32_loci > 0:
+        ctx32 = sd.Context(states, q=4, precision=sd.PREC_FP32, batch_loci=slots, history=args.history,
+                           grad_threshold=1e-3)
+        ctx32.set_weights(w)
+        ctx32.solve(order[pos:pos + slots // 4])  # allocation and first launches
+        pos += slots // 4
+        fv, fit, fev, fs, pos = run_steps(ctx32, order, pos, args.fp32_loci, 1, "cold-fp32", ctx32.solve)
+        fp32 = {"loci_s": fv, "iter": fit, "tm": ctx32.timing()}
+        ctx32.close()
+
+    head = warm if warm is not None else {"loci_s": cold_v, "iter": cold_it, "evals": cold_ev, "tm": cold_tm}
+    rl, rates = roofline(head["tm"], True)
 
     cpu = None
     if not args.no_cpu:
@@ -131,49 +199,58 @@ def main() -> int:
 
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
         ncpu = args.cpu_loci or nthreads
-        sample = order[:ncpu]
         t = time.perf_counter()
-        _, _, cst = oracle.solve_loci(states, w, lh, lJ, sample, gtol=1e-3, max_iter=2000, m=10, nthreads=nthreads,
-                                      blocked=True, want_rows=True)
+        _, _, cst = oracle.solve_loci(states, w, lh, lJ, order[:ncpu], gtol=1e-3, max_iter=2000, m=10,
+                                      nthreads=nthreads, blocked=True, want_rows=True)
         cpu_s = time.perf_counter() - t
         cpu = {"value": round(int((cst["status"] == 0).sum()) / cpu_s, 4), "unit": "loci/s", "cores": nthreads,
                "kind": "port",
-               "sample": f"{ncpu} loci of the same workload at full B={B}, L={L}, FP64 block-compressed "
-                         f"oracle, one locus per thread, {cpu_s:.1f}s wall"}
+               "sample": f"{ncpu} loci of the same workload at full B={B}, L={L}, FP64 block-compressed oracle, "
+                         f"cold start, ||g||_inf<1e-3, one locus per thread, {cpu_s:.1f}s wall"}
         log(f"[bench] cpu baseline {cpu}")
 
+    detail = {
+        "cold_split_bf16_loci_s": round(cold_v, 3),
+        "cold_fp32_loci_s": round(fp32["loci_s"], 3) if fp32 else None,
+        "warm_split_bf16_loci_s": round(warm["loci_s"], 3) if warm else None,
+        "warm_over_cold_fp32": round(warm["loci_s"] / fp32["loci_s"], 3) if (warm and fp32) else None,
+        "warm_over_cold_split": round(warm["loci_s"] / cold_v, 3) if warm else None,
+        "iter_median": {"cold_split": float(np.median(cold_it)),
+                        "warm_split": float(np.median(warm["iter"])) if warm else None,
+                        "cold_fp32": float(np.median(fp32["iter"])) if fp32 else None},
+        "iter_p95_headline": float(np.percentile(head["iter"], 95)),
+        "evals_median_headline": float(np.median(head["evals"])),
+        "partners": partners if warm else 0,
+        "mi_table_s": round(warm["mi_s"], 2) if warm else None,
+        "reweight_ms": round(reweight_s * 1e3, 2), "B_eff": beff,
+        "device_ms_headline": device_ms(head["tm"]),
+        "kernel_rates_headline": rates,
+        "device_ms_cold_fp32": device_ms(fp32["tm"]) if fp32 else None,
+        "rounds_headline": head["tm"]["n_evals"],
+    }
     out = {
         "metric": METRIC,
-        "value": round(value, 4),
+        "value": round(head["loci_s"], 4),
         "unit": "loci/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "ms_per_step": round(per_step / head["loci_s"] * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x2+fp32" if prec == sd.PREC_SPLIT_BF16 else "fp32",
+        "dtype": "bf16x2+fp32",
         "data": "synthetic",
-        "config": {"workload": f"{wl}: B={B} sequences x N={L} loci, cold start, {per_step} loci per step "
-                               f"({slots} in flight), L-BFGS m={args.history}, ||g||_inf<1e-3", "B": B, "N": L,
-                   "loci_per_step": per_step, "slots": slots,
-                   "precision": prec_name, "start": "cold", "parallelism": "loci batched on 1 GPU"},
-        "roofline": roofline,
+        "config": {"workload": f"{wl}: B={B} sequences x N={L} loci, {'warm' if warm else 'cold'} start, "
+                               f"{per_step} loci per step ({slots} in flight), L-BFGS m={args.history}, "
+                               f"||g||_inf<1e-3", "B": B, "N": L, "loci_per_step": per_step, "slots": slots,
+                   "precision": "split-bf16", "start": "warm" if warm else "cold",
+                   "parallelism": "loci batched on 1 GPU"},
+        "roofline": rl,
         "cpu_baseline": cpu,
-        "detail": {
-            "converged": converged, "iter_median": float(np.median(iters)), "iter_p95": float(np.percentile(iters, 95)),
-            "evals_median": float(np.median(evals)), "evals_p95": float(np.percentile(evals, 95)),
-            "reweight_ms": round(reweight_s * 1e3, 2), "B_eff": beff,
-            "device_ms": {k: round(tm[k], 2) for k in ("gemm_energy_ms", "gemm_grad_ms", "softmax_ms", "update_ms",
-                                                       "dots_ms", "score_ms")},
-            "kernel_rates": {k: round(v[1] / (v[0] * 1e-3) / (1e12 if v[2] == "mfma" else 1e9), 2)
-                             for k, v in groups.items() if v[0] > 0},
-            "rounds": tm["n_evals"],
-        },
+        "detail": detail,
     }
     print(json.dumps(out), flush=True)
-    ctx.close()
     return 0
 
 

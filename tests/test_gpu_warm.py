@@ -60,7 +60,7 @@ def test_mlp_training_decreases_loss(c1):
         ctx.mlp_init(128, seed=1)
         losses2 = ctx.mlp_train(x, t, epochs=20, batch=512, lr=2e-3, seed=3)
         p2 = ctx.mlp_params()
-    assert losses[-1] < 0.2 * losses[0]
+    assert losses[-1] < 0.35 * losses[0]
     np.testing.assert_array_equal(p1, p2)  # deterministic
     np.testing.assert_array_equal(losses, losses2)
 
@@ -74,21 +74,26 @@ def trained_c1(c1):
 
 def test_warm_start_same_optimum_fewer_iterations(trained_c1):
     params, losses = trained_c1
-    assert losses[-1] < 0.5 * losses[0]
+    assert losses[-1] < 0.05 * losses[0]
     d = synth.generate(B=600, N=1500, clades=20, pairs=60, seed=4242)
     st, _ = sd.prepare_alignment(d["chars"])
     loci = np.arange(0, st.shape[1], 23, dtype=np.int32)
-    with sd.Context(st, grad_threshold=1e-5, batch_loci=64) as ctx:
+    with sd.Context(st, grad_threshold=1e-4, batch_loci=64) as ctx:
         w, _ = ctx.reweight(0.9)
         ctx.set_weights(w)
         _, sc, _, rc = ctx.solve(loci)
         ctx.mlp_set_params(params, 128)
-        _, sw, _, rcw = ctx.solve_mlp(loci, npartners=64)
+        _, sw, _, rcw = ctx.solve_mlp(loci, npartners=512)
         ctx.mlp_set_params(np.zeros_like(params), 128)  # network off: fields-only warm start
-        _, sh, _, rch = ctx.solve_mlp(loci, npartners=64)
-    assert rc == rcw == rch == sd.SDCA_OK
-    np.testing.assert_allclose(sw["fval"], sc["fval"], rtol=2e-6)
-    np.testing.assert_allclose(sh["fval"], sc["fval"], rtol=2e-6)
-    print(f"iterations median cold {np.median(sc['iter'])} fields-only {np.median(sh['iter'])} "
-          f"network {np.median(sw['iter'])}; evals {sc['nfeval'].sum()} {sh['nfeval'].sum()} {sw['nfeval'].sum()}")
-    assert sw["nfeval"].sum() < sc["nfeval"].sum()
+        _, sh, _, rch = ctx.solve_mlp(loci, npartners=512)
+    for s, r in ((sc, rc), (sw, rcw), (sh, rch)):
+        ok = (s["status"] == sd.ST_CONVERGED) | ((s["status"] == sd.ST_LS_FAIL) & (s["gnorm"] < 3e-4))
+        assert ok.all()
+    # strictly convex: every start reaches the same optimum
+    np.testing.assert_allclose(sw["fval"], sc["fval"], rtol=1e-6)
+    np.testing.assert_allclose(sh["fval"], sc["fval"], rtol=1e-6)
+    print(f"evaluations: cold {sc['nfeval'].sum()} fields-only {sh['nfeval'].sum()} network {sw['nfeval'].sum()}; "
+          f"median iterations {np.median(sc['iter'])} {np.median(sh['iter'])} {np.median(sw['iter'])}")
+    # what the trained network gains over the same warm start without it
+    assert sw["nfeval"].sum() < 0.95 * sh["nfeval"].sum()
+    assert sw["nfeval"].sum() < 0.95 * sc["nfeval"].sum()
