@@ -101,6 +101,13 @@ def train_network(seed=0):
     return params, losses, sum(f.shape[0] for f in feats)
 
 
+WARM_MODES = {
+    "direction": lambda sd: sd.WARM_DIRECTION | sd.WARM_NO_FIELDS,
+    "direction-fields": lambda sd: sd.WARM_DIRECTION,
+    "point": lambda sd: sd.WARM_POINT,
+}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +118,9 @@ def main() -> int:
     ap.add_argument("--slots", type=int, default=0, help="loci in flight (default 256 at C4, 128 otherwise)")
     ap.add_argument("--history", type=int, default=10)
     ap.add_argument("--partners", type=int, default=2048, help="warm-started partners per locus")
+    ap.add_argument("--warm-mode", default="direction", choices=list(WARM_MODES),
+                    help="direction: network blocks as the first search direction from zero; direction-fields: "
+                         "the same from the J=0 field optimum; point: start at fields + blocks")
     ap.add_argument("--fp32-loci", type=int, default=1024, help="loci of the FP32 cold leg (0: skip)")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm leg (value = cold split-BF16)")
     ap.add_argument("--cpu-loci", type=int, default=0, help="CPU baseline sample size (default: one per thread)")
@@ -163,6 +173,8 @@ def main() -> int:
         pos += per_step
     cold_v, cold_it, cold_ev, cold_s, pos = run_steps(ctx, order, pos, per_step, args.steps, "cold-split", ctx.solve)
     cold_tm = ctx.timing()
+    gemm_cfg = ctx.gemm_config()
+    log(f"[bench] split-BF16 GEMM configuration {gemm_cfg}")
 
     warm = None
     if net is not None:
@@ -171,7 +183,8 @@ def main() -> int:
         ctx.mi_topk(partners)
         mi_s = time.time() - t
         log(f"[bench] MI top-{partners} partner table in {mi_s:.2f}s")
-        solve_w = lambda loci: ctx.solve_mlp(loci, partners)
+        mode = WARM_MODES[args.warm_mode](sd)
+        solve_w = lambda loci: ctx.solve_mlp(loci, partners, mode)
         for k in range(args.warmup):
             solve_w(order[pos:pos + per_step // 4])
             pos += per_step // 4
@@ -187,7 +200,7 @@ def main() -> int:
         ctx32.solve(order[pos:pos + slots // 4])  # allocation and first launches
         pos += slots // 4
         fv, fit, fev, fs, pos = run_steps(ctx32, order, pos, args.fp32_loci, 1, "cold-fp32", ctx32.solve)
-        fp32 = {"loci_s": fv, "iter": fit, "tm": ctx32.timing()}
+        fp32 = {"loci_s": fv, "iter": fit, "tm": ctx32.timing(), "gemm": ctx32.gemm_config()}
         ctx32.close()
 
     head = warm if warm is not None else {"loci_s": cold_v, "iter": cold_it, "evals": cold_ev, "tm": cold_tm}
@@ -221,12 +234,15 @@ def main() -> int:
         "iter_p95_headline": float(np.percentile(head["iter"], 95)),
         "evals_median_headline": float(np.median(head["evals"])),
         "partners": partners if warm else 0,
+        "warm_mode": args.warm_mode if warm else None,
         "mi_table_s": round(warm["mi_s"], 2) if warm else None,
         "reweight_ms": round(reweight_s * 1e3, 2), "B_eff": beff,
         "device_ms_headline": device_ms(head["tm"]),
         "kernel_rates_headline": rates,
         "device_ms_cold_fp32": device_ms(fp32["tm"]) if fp32 else None,
         "rounds_headline": head["tm"]["n_evals"],
+        "gemm_config_split": gemm_cfg,
+        "gemm_config_fp32": fp32["gemm"] if fp32 else None,
     }
     out = {
         "metric": METRIC,

@@ -54,8 +54,17 @@ typedef struct sdca_opts {
     int32_t max_ls;          /* function evaluations per line search (default 20) */
     double grad_threshold;   /* stop when ||g||_inf < this (default 1e-3, plmDCA_options.cpp:61) */
     int32_t verbose;         /* per-batch progress on stderr */
-    int32_t reserved[7];
+    int32_t autotune;        /* 1 (default): time the GEMM tile configurations once per shape */
+    int32_t gemm_ksplit;     /* > 0: force the energy split-K slices (no tuning) */
+    int32_t gemm_gm_energy;  /* > 0: force the energy XCD super-tile height, 1..64 (no tuning) */
+    int32_t gemm_gm_grad;    /* > 0: force the gradient XCD super-tile height, 1..64 (no tuning) */
+    int32_t reserved[3];
 } sdca_opts;
+
+/* How sdca_solve_loci_mlp uses the network's coupling blocks (flags). */
+#define SDCA_WARM_DIRECTION 0 /* start at the fields; the blocks are the first search direction */
+#define SDCA_WARM_POINT 1     /* start at the fields plus the blocks */
+#define SDCA_WARM_NO_FIELDS 2 /* or-ed in: fields start at zero instead of the J = 0 optimum */
 
 /* Defaults as documented above. */
 void sdca_default_opts(sdca_opts *opts);
@@ -138,6 +147,12 @@ int sdca_solve_loci(sdca_ctx *ctx, const int32_t *loci, int32_t K, const double 
 /* Timings accumulated since the context was created or last reset. */
 int sdca_get_timing(sdca_ctx *ctx, sdca_timing *t, int32_t reset);
 
+/* GEMM configuration of the current workspace (no counterpart in SuperDCA):
+ * cfg[0] = 1 if autotuned (or taken from the tuning cache), cfg[1] = energy split-K
+ * slices, cfg[2] = columns per slice, cfg[3] / cfg[4] = XCD super-tile height of the
+ * energy / gradient contraction.  SDCA_E_STATE before the first evaluation. */
+int sdca_gemm_config(const sdca_ctx *ctx, int32_t cfg[5]);
+
 /* ---- pair statistics for the warm start (no counterpart in SuperDCA) ---- */
 
 /* Exact co-occurrence counts N_ij(a,b) = #{sequences with state a at locus i and b at j}
@@ -187,8 +202,8 @@ int sdca_mlp_train(sdca_ctx *ctx, const float *x, const float *t, const float *m
  * largest-MI partners (sdca_mi_topk) receive the network's blocks, the fields start at
  * the J = 0 optimum, all other couplings at zero.  The optimum is the cold start's
  * (the problem is strictly convex); only the iteration count changes. */
-int sdca_solve_loci_mlp(sdca_ctx *ctx, const int32_t *loci, int32_t K, int32_t npartners, float *score_rows,
-                        double *x_out, sdca_locus_stats *st);
+int sdca_solve_loci_mlp(sdca_ctx *ctx, const int32_t *loci, int32_t K, int32_t npartners, int32_t warm_mode,
+                        float *score_rows, double *x_out, sdca_locus_stats *st);
 
 /* ---- host-side alignment pipeline (the apegrunt contract the CLI relies on) ---- */
 

@@ -33,6 +33,10 @@ ST_ITER_CAP = 1
 ST_LS_FAIL = 2
 ST_NONFINITE = 3
 
+WARM_DIRECTION = 0
+WARM_POINT = 1
+WARM_NO_FIELDS = 2
+
 # Every symbol include/sdca.h declares.
 EXPORTED = (
     "sdca_default_opts", "sdca_last_error", "sdca_ctx_create", "sdca_ctx_destroy", "sdca_num_columns",
@@ -40,6 +44,7 @@ EXPORTED = (
     "sdca_prepare_alignment", "sdca_write_couplings", "sdca_pair_counts", "sdca_pair_count_table",
     "sdca_mi_topk", "sdca_pair_features", "sdca_mlp_init", "sdca_mlp_num_params", "sdca_mlp_get_params",
     "sdca_mlp_set_params", "sdca_mlp_forward", "sdca_mlp_loss_grad", "sdca_mlp_train", "sdca_solve_loci_mlp",
+    "sdca_gemm_config",
 )
 
 
@@ -52,7 +57,9 @@ class SdcaError(RuntimeError):
 class Opts(C.Structure):
     _fields_ = [("device", C.c_int32), ("precision", C.c_int32), ("batch_loci", C.c_int32),
                 ("history", C.c_int32), ("max_iter", C.c_int32), ("max_ls", C.c_int32),
-                ("grad_threshold", C.c_double), ("verbose", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("grad_threshold", C.c_double), ("verbose", C.c_int32), ("autotune", C.c_int32),
+                ("gemm_ksplit", C.c_int32), ("gemm_gm_energy", C.c_int32), ("gemm_gm_grad", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class LocusStats(C.Structure):
@@ -109,13 +116,14 @@ def lib() -> C.CDLL:
                                      C.c_void_p]
     L.sdca_mlp_train.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
                                  C.c_float, C.c_uint64, C.c_void_p]
-    L.sdca_solve_loci_mlp.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
-                                      C.c_void_p]
+    L.sdca_solve_loci_mlp.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                                      C.c_void_p, C.c_void_p]
+    L.sdca_gemm_config.argtypes = [C.c_void_p, C.c_void_p]
     for name in ("sdca_ctx_create", "sdca_reweight", "sdca_set_weights", "sdca_eval_batch", "sdca_solve_loci",
                  "sdca_get_timing", "sdca_prepare_alignment", "sdca_write_couplings", "sdca_pair_counts",
                  "sdca_pair_count_table", "sdca_mi_topk", "sdca_pair_features", "sdca_mlp_init",
                  "sdca_mlp_get_params", "sdca_mlp_set_params", "sdca_mlp_forward", "sdca_mlp_loss_grad",
-                 "sdca_mlp_train", "sdca_solve_loci_mlp"):
+                 "sdca_mlp_train", "sdca_solve_loci_mlp", "sdca_gemm_config"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -298,7 +306,7 @@ class Context:
                                     float(lr), int(seed), _ptr(losses)))
         return losses
 
-    def solve_mlp(self, loci, npartners: int, want_x: bool = False, rows: bool = True):
+    def solve_mlp(self, loci, npartners: int, mode: int = WARM_DIRECTION, want_x: bool = False, rows: bool = True):
         """Batched solve from network warm starts -> (rows, stats, x, rc) like solve()."""
         loci = np.ascontiguousarray(loci, dtype=np.int32)
         K = loci.size
@@ -306,7 +314,7 @@ class Context:
         S = np.zeros((K, self.L), dtype=np.float32) if rows else None
         X = np.zeros((K, n), dtype=np.float64) if want_x else None
         st = (LocusStats * max(K, 1))()
-        rc = _check(lib().sdca_solve_loci_mlp(self._h, _ptr(loci), K, int(npartners), _ptr(S), _ptr(X),
+        rc = _check(lib().sdca_solve_loci_mlp(self._h, _ptr(loci), K, int(npartners), int(mode), _ptr(S), _ptr(X),
                                               C.cast(st, C.c_void_p)), allow=(SDCA_E_NOCONV,))
         stats = {k: np.array([getattr(st[i], k) for i in range(K)]) for k in ("fval", "gnorm", "nfeval", "iter",
                                                                                "status")}
@@ -316,6 +324,12 @@ class Context:
         t = Timing()
         _check(lib().sdca_get_timing(self._h, C.byref(t), int(reset)))
         return {name: getattr(t, name) for name, _ in Timing._fields_}
+
+    def gemm_config(self) -> dict:
+        """GEMM configuration of the current workspace (after the first evaluation)."""
+        cfg = (C.c_int32 * 5)()
+        _check(lib().sdca_gemm_config(self._h, cfg))
+        return {"tuned": bool(cfg[0]), "ksplit": cfg[1], "kchunk": cfg[2], "gm_energy": cfg[3], "gm_grad": cfg[4]}
 
 
 def prepare_alignment(chars: np.ndarray, maf_threshold: float = 0.01, gap_threshold: float = 0.15,

@@ -24,7 +24,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <deque>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -145,6 +148,199 @@ sdca_ctx::~sdca_ctx()
 
 namespace {
 
+// ---------------------------------------------------------------- GEMM autotuner
+//
+// Both contractions walk their output tiles in XCD super-tiles (gm tile rows by 64/gm
+// tile columns per group of eight workgroups); the energy contraction also splits K.
+// The best shape depends on the batch (M = slots x q), the alignment depth (N = B for
+// the energies) and K_x, so the first workspace of a shape times a small set of
+// candidates on the real buffers and keeps the fastest.  Choices are cached per
+// process and, when SDCA_TUNE_CACHE names a file, across processes (one text line per
+// shape: precision Mpad Bp Kxp ksplit kchunk gm_energy gm_grad).
+
+struct GemmKey {
+    int prec, Mpad, Bp;
+    int64_t Kxp;
+    bool operator<(const GemmKey &o) const
+    {
+        if (prec != o.prec) return prec < o.prec;
+        if (Mpad != o.Mpad) return Mpad < o.Mpad;
+        if (Bp != o.Bp) return Bp < o.Bp;
+        return Kxp < o.Kxp;
+    }
+};
+
+struct GemmConfig {
+    int ksplit = 1, kchunk = 0, gm_e = 8, gm_g = 8;
+};
+
+std::mutex g_tune_mu;
+std::map<GemmKey, GemmConfig> g_tune;
+bool g_tune_file_read = false;
+
+void read_tune_file()
+{
+    g_tune_file_read = true;
+    const char *path = std::getenv("SDCA_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return;
+    GemmKey k;
+    GemmConfig g;
+    long long kxp;
+    while (std::fscanf(f, "%d %d %d %lld %d %d %d %d", &k.prec, &k.Mpad, &k.Bp, &kxp, &g.ksplit, &g.kchunk, &g.gm_e,
+                       &g.gm_g) == 8) {
+        k.Kxp = kxp;
+        if (g.ksplit >= 1 && g.kchunk > 0 && (int64_t)g.ksplit * g.kchunk >= k.Kxp && g.gm_e >= 1 && g.gm_g >= 1)
+            g_tune[k] = g;
+    }
+    std::fclose(f);
+}
+
+bool lookup_config(const GemmKey &k, GemmConfig &g)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!g_tune_file_read) read_tune_file();
+    auto it = g_tune.find(k);
+    if (it == g_tune.end()) return false;
+    g = it->second;
+    return true;
+}
+
+void store_config(const GemmKey &k, const GemmConfig &g)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[k] = g;
+    const char *path = std::getenv("SDCA_TUNE_CACHE");
+    if (!path || !*path) return;
+    if (FILE *f = std::fopen(path, "a")) {
+        std::fprintf(f, "%d %d %d %lld %d %d %d %d\n", k.prec, k.Mpad, k.Bp, (long long)k.Kxp, g.ksplit, g.kchunk,
+                     g.gm_e, g.gm_g);
+        std::fclose(f);
+    }
+}
+
+// Split-K chunk for a requested number of slices (whole BK steps per slice).
+void set_split(const sdca_ctx *c, int64_t ks, GemmConfig &g)
+{
+    const int BK = c->opts.precision == SDCA_PREC_SPLIT_BF16 ? 64 : 32;
+    ks = std::max<int64_t>(1, std::min<int64_t>(ks, c->Kxp / BK));
+    g.kchunk = (int)round_up((c->Kxp + ks - 1) / ks, BK);
+    g.ksplit = (int)((c->Kxp + g.kchunk - 1) / g.kchunk);
+}
+
+// Default rule: >= 2048 energy workgroups, the number of super-tiles a multiple of the
+// eight XCDs, 8-row super-tiles for both contractions.
+GemmConfig default_config(const sdca_ctx *c, int gm)
+{
+    GemmConfig g;
+    g.gm_e = g.gm_g = gm;
+    const int64_t tiles = (int64_t)(c->Mpad / sdca::kTile) * (c->Bp / sdca::kTile);
+    int64_t ks = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
+    const int64_t sup = sdca::kern::energy_superiles(c->Mpad, c->Bp, gm);
+    while ((ks * sup) % 8 != 0 && ks < 64) ++ks;
+    set_split(c, ks, g);
+    return g;
+}
+
+// Energy candidates: super-tile heights 1..8 and half / default / double split-K.
+std::vector<GemmConfig> tuning_candidates(const sdca_ctx *c)
+{
+    std::vector<GemmConfig> v;
+    const int Mt = c->Mpad / sdca::kTile;
+    for (int gm = 1; gm <= 8; gm *= 2) {
+        if (gm > 1 && gm / 2 >= Mt) break;  // taller than the matrix: same as the previous one
+        const GemmConfig d = default_config(c, gm);
+        for (int64_t ks : {(int64_t)std::max(1, d.ksplit / 2), (int64_t)d.ksplit, (int64_t)d.ksplit * 2}) {
+            GemmConfig g = d;
+            set_split(c, ks, g);
+            bool dup = false;
+            for (const auto &o : v) dup |= o.ksplit == g.ksplit && o.gm_e == g.gm_e;
+            if (!dup) v.push_back(g);
+        }
+    }
+    return v;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b);
+
+// Times the candidates on the workspace's own buffers with synthetic slot commands
+// (every slot active, writing ring vector 0) and leaves the buffers zeroed again.
+int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand, GemmConfig &best)
+{
+    const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
+    hipStream_t st = c->stream;
+    std::vector<sdca::SlotCmd> cmds(nslots);
+    for (int i = 0; i < nslots; ++i) {
+        std::memset(&cmds[i], 0, sizeof(cmds[i]));
+        cmds[i].mode = sdca::MODE_INIT;
+        cmds[i].locus = i % c->L;
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_cmds, cmds.data(), nslots * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
+    const size_t nw = (size_t)c->Mpad * c->Kxp;
+    HIP_TRY(hipMemsetD32Async((hipDeviceptr_t)c->d_Xtr, 0x3c23d70au, nw, st));  // 0.01f
+    if (split) {
+        HIP_TRY(hipMemsetD16Async((hipDeviceptr_t)c->d_Whi, 0x3c24u, nw, st));  // bf16 0.01
+        HIP_TRY(hipMemsetD16Async((hipDeviceptr_t)c->d_Wlo, 0x3700u, nw, st));
+    }
+    const int reps = 3;
+    auto energy = [&](const GemmConfig &g) {
+        sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_X, c->Mpad, c->Bp,
+                                (int)c->Kxp, g.ksplit, g.kchunk, g.gm_e, c->d_Epart);
+        sdca::kern::softmax(st, c->opts.precision, c->d_Epart, g.ksplit, c->Mpad, c->Bp, c->B, c->q, nslots,
+                            c->d_cmds, c->d_statesT, c->d_w, c->d_G32, c->d_Ghi, c->d_Glo, c->d_sm_part, c->nbchunk);
+    };
+    auto grad = [&](int gm) {
+        sdca::kern::gemm_grad(st, c->opts.precision, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Xt, c->Mpad, c->Bp,
+                              (int)c->Kxp, (int)c->Kx, c->q, nslots, c->d_cmds, c->d_Xtr, c->d_ring, c->n_slot, P,
+                              (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), gm);
+    };
+    auto time = [&](auto &&launch) -> float {
+        launch();
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(c->ev[6], st));
+        for (int r = 0; r < reps; ++r) launch();
+        HIP_TRY(hipEventRecord(c->ev[7], st));
+        HIP_TRY(hipEventSynchronize(c->ev[7]));
+        return elapsed(c->ev[6], c->ev[7]) / reps;
+    };
+    float bt = 1e30f;
+    for (const auto &g : cand) {
+        const float t = time([&] { energy(g); });
+        if (t < 0) return SDCA_E_HIP;
+        if (c->opts.verbose)
+            std::fprintf(stderr, "sdca tune: energy+softmax ksplit %d gm %d: %.3f ms\n", g.ksplit, g.gm_e, t);
+        if (t < bt) {
+            bt = t;
+            best.ksplit = g.ksplit;
+            best.kchunk = g.kchunk;
+            best.gm_e = g.gm_e;
+        }
+    }
+    energy(best);  // node beliefs G for the gradient candidates
+    bt = 1e30f;
+    const int Mt = c->Mpad / sdca::kTile;
+    for (int gm = 1; gm <= 8; gm *= 2) {
+        if (gm > 1 && gm / 2 >= Mt) break;
+        const float t = time([&] { grad(gm); });
+        if (c->opts.verbose) std::fprintf(stderr, "sdca tune: grad gm %d: %.3f ms\n", gm, t);
+        if (t < bt) {
+            bt = t;
+            best.gm_g = gm;
+        }
+    }
+    // the solve starts from zeroed buffers
+    HIP_TRY(hipMemsetAsync(c->d_ring, 0, (size_t)nslots * P * c->n_slot * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(c->d_Xtr, 0, nw * sizeof(float), st));
+    if (split) {
+        HIP_TRY(hipMemsetAsync(c->d_Whi, 0, nw * sizeof(uint16_t), st));
+        HIP_TRY(hipMemsetAsync(c->d_Wlo, 0, nw * sizeof(uint16_t), st));
+    }
+    HIP_TRY(hipMemsetAsync(c->d_cmds, 0, nslots * sizeof(sdca::SlotCmd), st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SDCA_OK;
+}
+
 // Allocates the batch workspace for nslots concurrent loci and history m.
 int ensure_workspace(sdca_ctx *c, int nslots, int m)
 {
@@ -166,29 +362,45 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     } else {
         if ((rc = dalloc(&c->d_G32, (size_t)c->Mpad * c->Bp))) return rc;
     }
-    // split-K of the energy contraction: aim for >= 2048 workgroups, with the number of
-    // super-tiles a multiple of the eight XCDs
-    const int BK = split ? 64 : 32;
-    const int64_t tiles = (int64_t)(c->Mpad / sdca::kTile) * (c->Bp / sdca::kTile);
-    int64_t ks = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
-    const int64_t sup = sdca::kern::energy_superiles(c->Mpad, c->Bp);
-    while ((ks * sup) % 8 != 0 && ks < 64) ++ks;
-    ks = std::min<int64_t>(ks, c->Kxp / BK);
-    c->kchunk = (int)round_up((c->Kxp + ks - 1) / ks, BK);
-    c->ksplit = (int)((c->Kxp + c->kchunk - 1) / c->kchunk);
-    if ((rc = dalloc(&c->d_Epart, (size_t)c->ksplit * c->Mpad * c->Bp))) return rc;
+    // GEMM configuration: the cached/tuned one for this shape, else the default rule
+    const GemmKey key{c->opts.precision, c->Mpad, c->Bp, c->Kxp};
+    const sdca_opts &o = c->opts;
+    const bool forced = o.gemm_ksplit > 0 || o.gemm_gm_energy > 0 || o.gemm_gm_grad > 0;
+    GemmConfig cfg;
+    bool have = !forced && lookup_config(key, cfg);
+    const bool tune = !forced && !have && o.autotune;
+    if (!have) cfg = default_config(c, o.gemm_gm_energy > 0 ? o.gemm_gm_energy : 8);
+    if (forced) {
+        if (o.gemm_ksplit > 0) set_split(c, o.gemm_ksplit, cfg);
+        cfg.gm_g = o.gemm_gm_grad > 0 ? o.gemm_gm_grad : 8;
+    }
+    int max_ksplit = cfg.ksplit;
+    std::vector<GemmConfig> cand;
+    if (tune) {
+        cand = tuning_candidates(c);
+        for (const auto &k : cand) max_ksplit = std::max(max_ksplit, k.ksplit);
+    }
+    if ((rc = dalloc(&c->d_Epart, (size_t)max_ksplit * c->Mpad * c->Bp))) return rc;
     c->nchunk = (int)std::min<int64_t>(64, std::max<int64_t>(1, (c->n_slot / 4 + 255) / 256));
     c->nbchunk = (c->Bp + 255) / 256;
-    if ((rc = dalloc(&c->d_upd_part, (size_t)nslots * c->nchunk * (P + 2)))) return rc;
+    if ((rc = dalloc(&c->d_upd_part, (size_t)nslots * c->nchunk * 3))) return rc;
     if ((rc = dalloc(&c->d_dot_part, (size_t)nslots * c->nchunk * (P + 1)))) return rc;
     if ((rc = dalloc(&c->d_sm_part, (size_t)nslots * c->nbchunk))) return rc;
-    if ((rc = dalloc(&c->d_res, (size_t)nslots * (2 * P + 4)))) return rc;
+    if ((rc = dalloc(&c->d_res, (size_t)nslots * sdca::ResLayout{P}.width()))) return rc;
     if ((rc = dalloc(&c->d_cmds, (size_t)nslots))) return rc;
     if ((rc = dalloc(&c->d_rows, (size_t)nslots * c->L))) return rc;
     if ((rc = dalloc(&c->d_src_off, (size_t)nslots))) return rc;
     if ((rc = dalloc(&c->d_score_loci, (size_t)nslots))) return rc;
     // the zero fills above must be complete before any kernel of the solve runs
     HIP_TRY(hipDeviceSynchronize());
+    c->tuned = 0;
+    if (tune && (rc = autotune(c, nslots, P, cand, cfg))) return rc;
+    c->ksplit = cfg.ksplit;
+    c->kchunk = cfg.kchunk;
+    c->gm_energy = cfg.gm_e;
+    c->gm_grad = cfg.gm_g;
+    c->tuned = tune || have;
+    if (tune) store_config(key, cfg);
     c->ws_slots = nslots;
     c->ws_m = m;
     return SDCA_OK;
@@ -210,13 +422,13 @@ int run_round(sdca_ctx *c, int nslots, const std::vector<sdca::SlotCmd> &cmds, s
     hipStream_t st = c->stream;
     HIP_TRY(hipMemcpyAsync(c->d_cmds, cmds.data(), nslots * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(c->ev[0], st));
-    sdca::kern::update(st, m, c->opts.precision, c->d_cmds, nslots, c->d_ring, c->d_Xc, c->d_Xtr,
+    sdca::kern::update(st, m, c->q, c->d_cmds, nslots, c->d_ring, c->d_Xc, c->d_Xtr,
                        split ? c->d_Whi : nullptr, split ? c->d_Wlo : nullptr, c->d_X0, c->n_slot, (int)c->Kxp,
                        (int)c->Kx, c->d_upd_part, c->nchunk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[1], st));
     sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_X, c->Mpad, c->Bp, (int)c->Kxp,
-                            c->ksplit, c->kchunk, c->d_Epart);
+                            c->ksplit, c->kchunk, c->gm_energy, c->d_Epart);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[2], st));
     sdca::kern::softmax(st, c->opts.precision, c->d_Epart, c->ksplit, c->Mpad, c->Bp, c->B, c->q, nslots, c->d_cmds,
@@ -225,7 +437,7 @@ int run_round(sdca_ctx *c, int nslots, const std::vector<sdca::SlotCmd> &cmds, s
     HIP_TRY(hipEventRecord(c->ev[3], st));
     sdca::kern::gemm_grad(st, c->opts.precision, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Xt, c->Mpad, c->Bp, (int)c->Kxp,
                           (int)c->Kx, c->q, nslots, c->d_cmds, c->d_Xtr, c->d_ring, c->n_slot, P,
-                          (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h));
+                          (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), c->gm_grad);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[4], st));
     sdca::kern::dots(st, m, c->d_cmds, nslots, c->d_ring, c->n_slot, c->d_dot_part, c->nchunk);
@@ -234,7 +446,7 @@ int run_round(sdca_ctx *c, int nslots, const std::vector<sdca::SlotCmd> &cmds, s
                                c->d_res);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[5], st));
-    res.resize((size_t)nslots * (2 * P + 4));
+    res.resize((size_t)nslots * sdca::ResLayout{P}.width());
     HIP_TRY(hipMemcpyAsync(res.data(), c->d_res, res.size() * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     auto &t = c->timing;
@@ -255,7 +467,11 @@ int run_round(sdca_ctx *c, int nslots, const std::vector<sdca::SlotCmd> &cmds, s
         active += md != sdca::MODE_IDLE;
         if (md == sdca::MODE_INIT) vu += 2 + (cmds[i].init_src != 0.f);
         else if (md == sdca::MODE_RETRY) vu += 3;
-        else if (md == sdca::MODE_NEWDIR) vu += P + 4;
+        else if (md == sdca::MODE_NEWDIR) {
+            int support = 0;
+            for (int p = 0; p < P; ++p) support += cmds[i].coef[p] != 0.f;
+            vu += support + 4;
+        }
         if (md != sdca::MODE_IDLE) {
             vd += P;
             if (split) vu += 1;  // hi + lo BF16 copies of x_t
@@ -342,7 +558,15 @@ struct Slot {
     int nfeval = 0, iter = 0, status = -1;
     double gnorm = 0.0;
     bool from_xt = true;
+    bool preloaded = false;  // ring vector m holds a first search direction
+    int restarts = 0;        // line searches restarted from steepest descent
 };
+
+// A failed line search with curvature history is retried once more from steepest
+// descent at the same iterate (at most this many times per locus) before the locus
+// stops with SDCA_ST_LS_FAIL: near the FP32 rounding floor a stale history can give a
+// direction along which no decrease is measurable while the gradient is not yet small.
+constexpr int kMaxRestarts = 2;
 
 class Solver {
 public:
@@ -352,10 +576,11 @@ public:
 
     double &G(int i, int a, int b) { return gram_[((size_t)i * P_ + a) * P_ + b]; }
 
-    void assign(int i, int item, int r, bool warm)
+    void assign(int i, int item, int r, bool warm, bool preloaded = false)
     {
         Slot &S = slots_[i];
         S = Slot();
+        S.preloaded = preloaded;
         S.item = item;
         S.r = r;
         S.phase = PH_INIT;
@@ -385,8 +610,10 @@ public:
         sdca::SlotCmd &k = cmds_[i];
         const sdca::ResLayout R{P_};
         const int pending = k.mode;
-        if (pending == sdca::MODE_NEWDIR)
-            for (int p = 0; p < P_; ++p) G(i, S.dk, p) = G(i, p, S.dk) = res[R.ddots() + p];
+        if (pending == sdca::MODE_NEWDIR) {
+            derive_direction_row(i);
+            G(i, S.dk, S.gk) = G(i, S.gk, S.dk) = res[R.ddg()];
+        }
         for (int p = 0; p < P_; ++p) G(i, S.gt, p) = G(i, p, S.gt) = res[R.gdots() + p];
         const double f = res[R.fdata()] + c_->lambda_J * res[R.xsqJ()] + c_->lambda_h * res[R.xsqh()];
         const double ginf = res[R.ginf()];
@@ -398,7 +625,12 @@ public:
             S.f = f;
             S.gnorm = ginf;
             if (ginf < c_->opts.grad_threshold) return finish(i, SDCA_ST_CONVERGED, true);
-            newdir(i);
+            if (S.preloaded) {
+                S.preloaded = false;
+                newdir_preloaded(i);
+            } else {
+                newdir(i);
+            }
             return false;
         }
         // line search in progress
@@ -406,7 +638,7 @@ public:
             const double dg0 = G(i, S.dk, S.gk);
             if (!(dg0 < 0.0)) {  // rounding destroyed descent: restart from steepest descent
                 S.pairs.clear();
-                newdir(i);
+                newdir(i, false);  // the trial point along d is not accepted
                 return false;
             }
             S.dg0 = dg0;
@@ -432,7 +664,13 @@ public:
                 k.alpha = (float)S.stp;
                 return false;
             }
-            if (!(f < S.f)) return finish(i, SDCA_ST_LS_FAIL, false);
+            if (!(f < S.f)) {
+                if (S.pairs.empty() || S.restarts >= kMaxRestarts) return finish(i, SDCA_ST_LS_FAIL, false);
+                S.restarts++;
+                S.pairs.clear();
+                newdir(i, false);
+                return false;
+            }
         }
         const double ys = stp_trial * (G(i, S.dk, S.gt) - G(i, S.dk, S.gk));
         if (ys > 0.0) {
@@ -530,7 +768,51 @@ private:
         for (int p = 0; p < P_; ++p) delta[p] = -qv[p];
     }
 
-    void newdir(int i)
+    // Gram entries of a new direction d = sum_p coef_p ring_p (the FP32 coefficients the
+    // device used) with every gradient of the history: d . g = sum_p coef_p (ring_p . g).
+    // Only direction-gradient and gradient-gradient products enter the two-loop recursion
+    // and the line search, and products with gradients computed later come from k_dots.
+    void derive_direction_row(int i)
+    {
+        Slot &S = slots_[i];
+        const sdca::SlotCmd &k = cmds_[i];
+        std::vector<int> gs;
+        for (const auto &pr : S.pairs) gs.push_back(pr.g_old);
+        if (S.gk >= 0) gs.push_back(S.gk);
+        for (int g : gs) {
+            double acc = 0.0;
+            for (int p = 0; p < P_; ++p)
+                if (k.coef[p] != 0.f) acc += (double)k.coef[p] * G(i, p, g);
+            G(i, S.dk, g) = G(i, g, S.dk) = acc;
+        }
+    }
+
+    // First direction = the preloaded ring vector m (copied to a free direction slot so
+    // that the history ring keeps its usual roles), unit initial step.  If it turns out
+    // not to be a descent direction the line search start falls back to steepest descent.
+    void newdir_preloaded(int i)
+    {
+        Slot &S = slots_[i];
+        S.dk = 0;  // ring vector m is the preloaded one; 0 is free (no history yet)
+        S.gt = free_g(i);
+        S.stp0 = 1.0;
+        S.stp = 1.0;
+        S.ls_evals = 0;
+        S.phase = PH_LS;
+        sdca::SlotCmd &k = cmds_[i];
+        k.mode = sdca::MODE_NEWDIR;
+        k.dst = S.dk;
+        k.gt = S.gt;
+        k.gk = S.gk;
+        k.accept = 1;  // x_t = x_c after the start point's evaluation
+        k.alpha = 1.f;
+        for (int p = 0; p < sdca::kMaxPhys; ++p) k.coef[p] = 0.f;
+        k.coef[m_] = 1.f;
+    }
+
+    // New search direction from the iterate: the last trial point if accept, else the
+    // current x_c (restart after a failed or non-descent search).
+    void newdir(int i, bool accept = true)
     {
         Slot &S = slots_[i];
         std::vector<double> delta(P_, 0.0);
@@ -560,6 +842,8 @@ private:
         k.mode = sdca::MODE_NEWDIR;
         k.dst = S.dk;
         k.gt = S.gt;
+        k.gk = S.gk;
+        k.accept = accept ? 1 : 0;
         k.alpha = (float)S.stp;
         for (int p = 0; p < sdca::kMaxPhys; ++p) k.coef[p] = p < P_ ? (float)delta[p] : 0.f;
         k.coef[S.dk] = 0.f;
@@ -618,6 +902,7 @@ void sdca_default_opts(sdca_opts *o)
     o->max_ls = 20;
     o->grad_threshold = 1e-3;
     o->verbose = 0;
+    o->autotune = 1;
 }
 
 int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L, int32_t q, int32_t gap,
@@ -631,6 +916,9 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
     sdca_opts o;
     if (opts) o = *opts;
     else sdca_default_opts(&o);
+    if (o.gemm_ksplit < 0 || o.gemm_ksplit > 4096 || o.gemm_gm_energy < 0 || o.gemm_gm_energy > 64 ||
+        o.gemm_gm_grad < 0 || o.gemm_gm_grad > 64)
+        return fail(SDCA_E_ARG, "gemm_ksplit must be in [0, 4096] and gemm_gm_* in [0, 64]");
     if (o.precision != SDCA_PREC_FP32 && o.precision != SDCA_PREC_SPLIT_BF16)
         return fail(SDCA_E_ARG, "unknown precision %d", o.precision);
     if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 10, 16 (got %d)", o.history);
@@ -914,7 +1202,7 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
             wl_item.push_back(item);
             wl_locus.push_back(r);
         }
-        S.assign(i, item, r, any_warm);
+        S.assign(i, item, r, any_warm, cw && cw->as_direction);
         return SDCA_OK;
     };
     // stage the compact warm starts of this round's refills into their slots' X0 vectors
@@ -925,10 +1213,16 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
         packed.insert(packed.end(), wl_item.begin(), wl_item.end());
         packed.insert(packed.end(), wl_locus.begin(), wl_locus.end());
         HIP_TRY(hipMemcpyAsync(d_wl, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, c->stream));
-        for (int i : wl_slot)
+        // blocks: into the staged start, or into ring vector m (first search direction)
+        float *XJ = cw->as_direction ? c->d_ring + (size_t)m * c->n_slot : c->d_X0;
+        const int64_t jstride = cw->as_direction ? (int64_t)P * c->n_slot : c->n_slot;
+        for (int i : wl_slot) {
             HIP_TRY(hipMemsetAsync(c->d_X0 + (size_t)i * c->n_slot, 0, c->n_slot * 4, c->stream));
+            if (cw->as_direction)
+                HIP_TRY(hipMemsetAsync(XJ + (size_t)i * jstride, 0, c->n_slot * 4, c->stream));
+        }
         sdca::kern::scatter_warm(c->stream, d_wl, d_wl + nw, d_wl + 2 * nw, cw->d_partners, cw->d_J, cw->d_h, nw,
-                                 cw->P, c->q, c->Kxp, (int)c->Kx, c->d_state_col, c->n_slot, c->d_X0);
+                                 cw->P, c->q, c->Kxp, (int)c->Kx, c->d_state_col, c->n_slot, c->d_X0, XJ, jstride);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(c->stream));  // packed is a host temporary
         wl_slot.clear();
@@ -1108,6 +1402,18 @@ int sdca_mi_topk(sdca_ctx *c, int32_t K, int32_t *partners_out, float *mi_out)
     }
     cleanup();
     if (e != hipSuccess) return fail(SDCA_E_HIP, "MI top-K failed: %s", hipGetErrorString(e));
+    return SDCA_OK;
+}
+
+int sdca_gemm_config(const sdca_ctx *c, int32_t cfg[5])
+{
+    if (!c || !cfg) return fail(SDCA_E_ARG, "null argument");
+    if (!c->ws_slots) return fail(SDCA_E_STATE, "no workspace yet");
+    cfg[0] = c->tuned;
+    cfg[1] = c->ksplit;
+    cfg[2] = c->kchunk;
+    cfg[3] = c->gm_energy;
+    cfg[4] = c->gm_grad;
     return SDCA_OK;
 }
 

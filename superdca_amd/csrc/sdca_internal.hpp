@@ -37,6 +37,8 @@ struct SlotCmd {
     int32_t locus;    // locus r of the slot, -1 if idle
     int32_t self_lo;  // columns [self_lo, self_hi) belong to locus r: gradient masked to 0
     int32_t self_hi;
+    int32_t gk;       // NEWDIR: ring index of the gradient at the accepted point
+    int32_t accept;   // NEWDIR: 1 = the last trial point becomes the iterate (x_c = x_t) first
     float alpha;      // step length of the trial point x_t = x_c + alpha * d
     float init_src;   // MODE_INIT: 1 = copy the staged start point, 0 = zero start
     float coef[kMaxPhys];  // NEWDIR: d = sum_p coef[p] * ring[p]
@@ -45,13 +47,13 @@ struct SlotCmd {
 // Layout of the per-slot result row (device -> host), doubles.
 struct ResLayout {
     int P;
-    int gdots() const { return 0; }          // [0, P): g_t . ring[p]
-    int ginf() const { return P; }           // max |g_t|
-    int ddots() const { return P + 1; }      // [P+1, 2P+1): d . ring[p] (NEWDIR)
-    int xsqJ() const { return 2 * P + 1; }   // sum of squares of x_t, J part
-    int xsqh() const { return 2 * P + 2; }   // sum of squares of x_t, h part
-    int fdata() const { return 2 * P + 3; }  // sum_b w_b (LSE_b - E_b(x_br))
-    int width() const { return 2 * P + 4; }
+    int gdots() const { return 0; }         // [0, P): g_t . ring[p]
+    int ginf() const { return P; }          // max |g_t|
+    int xsqJ() const { return P + 1; }      // sum of squares of x_t, J part
+    int xsqh() const { return P + 2; }      // sum of squares of x_t, h part
+    int fdata() const { return P + 3; }     // sum_b w_b (LSE_b - E_b(x_br))
+    int ddg() const { return P + 4; }       // NEWDIR: d . g_k of the stored direction
+    int width() const { return P + 5; }
 };
 
 struct Timing {
@@ -68,16 +70,16 @@ void build_x(hipStream_t st, const uint8_t *d_states, int B, int L, int q, const
 void bitplanes(hipStream_t st, const uint8_t *d_states, int B, int L, int q, int W, uint64_t *planes);
 void hamming_neighbours(hipStream_t st, const uint64_t *planes, int B, int q, int W, int L, int64_t dmax,
                         const uint32_t *d_mult, unsigned long long *d_iw);
-int energy_superiles(int Mpad, int Bp);  // super-tiles per split-K slice
+int energy_superiles(int Mpad, int Bp, int gm);  // super-tiles per split-K slice
 void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const void *X,
-                 int Mpad, int Bp, int Kxp, int ksplit, int kchunk, float *Epart);
+                 int Mpad, int Bp, int Kxp, int ksplit, int kchunk, int gm, float *Epart);
 void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad, int Bp, int B, int q, int nslots,
              const SlotCmd *cmds, const uint8_t *statesT, const float *wts, float *G32, uint16_t *Ghi,
              uint16_t *Glo, double *fpart, int nbchunk);
 void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const void *Xt,
                int Mpad, int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial,
-               float *ring, int64_t n_slot, int P, float lamJ2, float lamh2);
-void update(hipStream_t st, int m, int prec, const SlotCmd *cmds, int nslots, float *ring, float *Xc, float *Xtr,
+               float *ring, int64_t n_slot, int P, float lamJ2, float lamh2, int gm);
+void update(hipStream_t st, int m, int q, const SlotCmd *cmds, int nslots, float *ring, float *Xc, float *Xtr,
             uint16_t *Whi, uint16_t *Wlo, const float *X0, int64_t n_slot, int Kxp, int Kx, double *part,
             int nchunk);
 void dots(hipStream_t st, int m, const SlotCmd *cmds, int nslots, const float *ring, int64_t n_slot, double *part,
@@ -108,7 +110,7 @@ void gather_rows(hipStream_t st, const float *src, const int64_t *idx, int64_t n
 void pair_features(hipStream_t st, const int32_t *counts, int64_t npairs, int q, int B, float *feats);
 void scatter_warm(hipStream_t st, const int32_t *slot_of, const int32_t *item_of, const int32_t *locus_of,
                   const int32_t *partners, const float *J, const float *h, int nwarm, int P, int q, int64_t Kxp, int Kx,
-                  const int32_t *state_col, int64_t n_slot, float *X0);
+                  const int32_t *state_col, int64_t n_slot, float *X0, float *XJ, int64_t jstride);
 }  // namespace kern
 
 // Warm starts of a solve call in compact device form: for item k (the k-th listed locus)
@@ -118,6 +120,9 @@ struct CompactWarm {
     const int32_t *d_partners = nullptr;  // K x P
     const float *d_J = nullptr;           // K x P x q*q
     const float *d_h = nullptr;           // K x q
+    // false: start at (J, h).  true: start at the fields h alone and take the blocks J as
+    // the first search direction, so the line search scales them to the alignment at hand.
+    bool as_direction = true;
 };
 
 // The warm-start network (sdca_mlp.hip) and its training state.
@@ -165,6 +170,8 @@ struct sdca_ctx {
     int64_t n_slot = 0;
     int Mpad = 0;
     int ksplit = 1, kchunk = 0;
+    int gm_energy = 8, gm_grad = 8;  // XCD super-tile heights (autotuned)
+    int tuned = 0;                   // 1 if the autotuner chose the GEMM configuration
     int nchunk = 0, nbchunk = 0;
     float *d_ring = nullptr, *d_Xc = nullptr, *d_Xtr = nullptr, *d_X0 = nullptr;
     uint16_t *d_Whi = nullptr, *d_Wlo = nullptr;

@@ -442,28 +442,24 @@ __device__ __forceinline__ double dot4(float4 a, float4 b)
 // Trial point of every slot (one launch per round):
 //   INIT    x_c = x0 (staged) or 0,           x_t = x_c
 //   RETRY   x_t = x_c + alpha d                (d = ring[dst])
-//   NEWDIR  x_c = x_t (accept), d = sum_p coef[p] ring[p] -> ring[dst], x_t = x_c + alpha d,
-//           and the dot products d . ring[p] (the Gram row of the new direction)
-// plus the squared norms of x_t (J and h parts) for the L2 terms.
-// part[slot][chunk][P + 3] = {d . ring[p]} , xsqJ, xsqh.
+//   NEWDIR  x_c = x_t (if accept), d = sum_p coef[p] ring[p] -> ring[dst], x_t = x_c + alpha d
+//           (only ring vectors with a nonzero coefficient are read; the host derives the
+//           Gram row of d from the coefficients, except d . g_k, the slope the line search
+//           starts from, which is taken from the stored FP32 d here)
+// plus the squared norms of x_t for the L2 terms: part[slot][chunk][3] = {sum x^2 over the
+// whole vector, sum over the h entries (column Kx of each row), d . g_k}.
 template <int M>
 __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds, float *__restrict__ ring,
                                                 float *__restrict__ Xc, float *__restrict__ Xtr,
                                                 uint16_t *__restrict__ Whi, uint16_t *__restrict__ Wlo,
-                                                const float *__restrict__ X0, int64_t n_slot, int Kxp, int Kx,
+                                                const float *__restrict__ X0, int64_t n_slot, int Kxp, int Kx, int q,
                                                 double *__restrict__ part, int nchunk)
 {
     constexpr int P = 2 * M + 3;
-    constexpr int NV = P + 2;
     const int i = blockIdx.y;
     const SlotCmd &c = cmds[i];
-    double acc[NV];
-    bool is_max[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) {
-        acc[k] = 0.0;
-        is_max[k] = false;
-    }
+    double acc[3] = {0.0, 0.0, 0.0};
+    const bool is_max[3] = {false, false, false};
     const int mode = c.mode;
     if (mode != MODE_IDLE) {
         float *rb = ring + (int64_t)i * P * n_slot;
@@ -473,6 +469,15 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
         const float alpha = c.alpha;
         const int dst = c.dst;
         const int64_t n4 = n_slot >> 2;
+        // the h entry of row s is element Kx % 4 of float4 number (s*Kxp + Kx)/4
+        const int64_t hq0 = Kx >> 2, hstep = Kxp >> 2;
+        const int hlane = Kx & 3;
+        float cf[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) cf[p] = c.coef[p];
+        const int gk = mode == MODE_NEWDIR ? c.gk : 0;
+        const bool accept = c.accept != 0;
+        const float cf_gk = c.coef[gk];
         for (int64_t e4 = (int64_t)blockIdx.x * 256 + threadIdx.x; e4 < n4; e4 += (int64_t)nchunk * 256) {
             const int64_t e = e4 * 4;
             float4 xn;
@@ -484,53 +489,46 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
                 const float4 d = *reinterpret_cast<const float4 *>(rb + (int64_t)dst * n_slot + e);
                 xn = make_float4(xv.x + alpha * d.x, xv.y + alpha * d.y, xv.z + alpha * d.z, xv.w + alpha * d.w);
             } else {  // NEWDIR
-                const float4 xv = *reinterpret_cast<const float4 *>(xt + e);
-                *reinterpret_cast<float4 *>(xc + e) = xv;
-                float4 v[P];
-                float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
+                const float4 xv = *reinterpret_cast<const float4 *>((accept ? xt : xc) + e);
+                if (accept) *reinterpret_cast<float4 *>(xc + e) = xv;
+                float4 d = make_float4(0.f, 0.f, 0.f, 0.f), g = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
                 for (int p = 0; p < P; ++p) {
-                    v[p] = *reinterpret_cast<const float4 *>(rb + (int64_t)p * n_slot + e);
-                    const float cf = c.coef[p];
-                    if (cf != 0.f) {  // unused ring entries may hold stale values
-                        d.x += cf * v[p].x;
-                        d.y += cf * v[p].y;
-                        d.z += cf * v[p].z;
-                        d.w += cf * v[p].w;
+                    if (cf[p] != 0.f) {  // unused ring entries are never read
+                        const float4 v = *reinterpret_cast<const float4 *>(rb + (int64_t)p * n_slot + e);
+                        if (p == gk) g = v;
+                        d.x += cf[p] * v.x;
+                        d.y += cf[p] * v.y;
+                        d.z += cf[p] * v.z;
+                        d.w += cf[p] * v.w;
                     }
                 }
+                if (cf_gk == 0.f) g = *reinterpret_cast<const float4 *>(rb + (int64_t)gk * n_slot + e);
                 *reinterpret_cast<float4 *>(rb + (int64_t)dst * n_slot + e) = d;
-#pragma unroll
-                for (int p = 0; p < P; ++p) acc[p] += dot4(d, p == dst ? d : v[p]);
+                acc[2] += dot4(d, g);
                 xn = make_float4(xv.x + alpha * d.x, xv.y + alpha * d.y, xv.z + alpha * d.z, xv.w + alpha * d.w);
             }
             *reinterpret_cast<float4 *>(xt + e) = xn;
             if (Whi) {
                 const float xs[4] = {xn.x, xn.y, xn.z, xn.w};
-                ushort4 hi, lo;
                 uint16_t h[4], l[4];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     h[u] = f2bf(xs[u]);
                     l[u] = f2bf(xs[u] - bf2f(h[u]));
                 }
-                hi = make_ushort4(h[0], h[1], h[2], h[3]);
-                lo = make_ushort4(l[0], l[1], l[2], l[3]);
-                *reinterpret_cast<ushort4 *>(Whi + (int64_t)i * n_slot + e) = hi;
-                *reinterpret_cast<ushort4 *>(Wlo + (int64_t)i * n_slot + e) = lo;
+                *reinterpret_cast<ushort4 *>(Whi + (int64_t)i * n_slot + e) = make_ushort4(h[0], h[1], h[2], h[3]);
+                *reinterpret_cast<ushort4 *>(Wlo + (int64_t)i * n_slot + e) = make_ushort4(l[0], l[1], l[2], l[3]);
             }
-            // column of each element: e % Kxp (Kxp is a multiple of 4, so all four share the row)
-            const int64_t col = e % Kxp;
-            const float xs[4] = {xn.x, xn.y, xn.z, xn.w};
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const double sq = (double)xs[u] * xs[u];
-                if (col + u == Kx) acc[P + 1] += sq;
-                else acc[P] += sq;
+            acc[0] += (double)xn.x * xn.x + (double)xn.y * xn.y + (double)xn.z * xn.z + (double)xn.w * xn.w;
+            const int64_t rel = e4 - hq0;
+            if (rel >= 0 && rel % hstep == 0 && rel / hstep < q) {
+                const float hv = hlane == 0 ? xn.x : hlane == 1 ? xn.y : hlane == 2 ? xn.z : xn.w;
+                acc[1] += (double)hv * hv;
             }
         }
     }
-    block_reduce_store<NV>(acc, part + ((int64_t)i * nchunk + blockIdx.x) * NV, is_max);
+    block_reduce_store<3>(acc, part + ((int64_t)i * nchunk + blockIdx.x) * 3, is_max);
 }
 
 // Dot products of each active slot's trial gradient ring[gt] with every ring vector,
@@ -568,13 +566,14 @@ __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, 
     block_reduce_store<NV>(acc, part + ((int64_t)i * nchunk + blockIdx.x) * NV, is_max);
 }
 
-// Per-slot reduction of the chunk partials into the result row (ResLayout).
+// Per-slot reduction of the chunk partials into the result row (ResLayout):
+// [0, P) g_t . ring[p], P max |g_t|, P+1 sum x_t^2 (J part), P+2 (h part), P+3 data term.
 __global__ void k_reduce(int P, const double *__restrict__ upd_part, const double *__restrict__ dot_part,
                          const double *__restrict__ sm_part, int nchunk, int nbchunk, double *__restrict__ res)
 {
     const int i = blockIdx.x;
     const int t = threadIdx.x;
-    const int W = 2 * P + 4;
+    const int W = P + 5;
     double *out = res + (int64_t)i * W;
     if (t < P + 1) {  // dots kernel: gdots then ginf
         double a = 0.0;
@@ -583,12 +582,17 @@ __global__ void k_reduce(int P, const double *__restrict__ upd_part, const doubl
             a = (t == P) ? fmax(a, v) : a + v;
         }
         out[t] = a;
-    } else if (t < 2 * P + 3) {  // update kernel: ddots, xsqJ, xsqh
-        const int k = t - (P + 1);
-        double a = 0.0;
-        for (int ch = 0; ch < nchunk; ++ch) a += upd_part[((int64_t)i * nchunk + ch) * (P + 2) + k];
-        out[t] = a;
-    } else if (t == 2 * P + 3) {
+    } else if (t == P + 1) {  // update kernel: squares (all, h) and d . g_k
+        double all = 0.0, h = 0.0, dg = 0.0;
+        for (int ch = 0; ch < nchunk; ++ch) {
+            all += upd_part[((int64_t)i * nchunk + ch) * 3];
+            h += upd_part[((int64_t)i * nchunk + ch) * 3 + 1];
+            dg += upd_part[((int64_t)i * nchunk + ch) * 3 + 2];
+        }
+        out[P + 1] = all - h;
+        out[P + 2] = h;
+        out[P + 4] = dg;
+    } else if (t == P + 3) {
         double a = 0.0;
         for (int ch = 0; ch < nbchunk; ++ch) a += sm_part[(int64_t)i * nbchunk + ch];
         out[t] = a;
@@ -685,13 +689,15 @@ void hamming_neighbours(hipStream_t st, const uint64_t *planes, int B, int q, in
     hipLaunchKernelGGL(k_hamming, dim3((unsigned)ntiles), dim3(256), 0, st, planes, B, q, W, L, dmax, d_mult, d_iw);
 }
 
-static TileMap make_map(int Mt, int Nt, int Z)
+static TileMap make_map(int Mt, int Nt, int Z, int gm)
 {
     TileMap tm;
     tm.Mt = Mt;
     tm.Nt = Nt;
     tm.Z = Z;
-    tm.GM = Mt < 8 ? Mt : 8;
+    if (gm < 1) gm = 8;
+    if (gm > 64) gm = 64;
+    tm.GM = Mt < gm ? Mt : gm;
     tm.GN = Nt < 64 / tm.GM ? Nt : 64 / tm.GM;
     tm.MB = (Mt + tm.GM - 1) / tm.GM;
     tm.NB = (Nt + tm.GN - 1) / tm.GN;
@@ -701,17 +707,17 @@ static TileMap make_map(int Mt, int Nt, int Z)
 
 static unsigned map_grid(const TileMap &tm) { return (unsigned)(((tm.S + 7) / 8) * 8 * tm.GM * tm.GN); }
 
-int energy_superiles(int Mpad, int Bp)
+int energy_superiles(int Mpad, int Bp, int gm)
 {
-    const TileMap tm = make_map(Mpad / kTile, Bp / kTile, 1);
+    const TileMap tm = make_map(Mpad / kTile, Bp / kTile, 1, gm);
     return tm.MB * tm.NB;
 }
 
 void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const void *X, int Mpad,
-                 int Bp, int Kxp, int ksplit, int kchunk, float *Epart)
+                 int Bp, int Kxp, int ksplit, int kchunk, int gm, float *Epart)
 {
     EpiEnergy epi{Epart, (int64_t)Bp, (int64_t)Mpad * Bp};
-    const TileMap tm = make_map(Mpad / kTile, Bp / kTile, ksplit);
+    const TileMap tm = make_map(Mpad / kTile, Bp / kTile, ksplit, gm);
     const dim3 grid(map_grid(tm));
     if (prec == SDCA_PREC_FP32)
         hipLaunchKernelGGL((k_gemm_nt<PolF32, 1, EpiEnergy>), grid, dim3(256), 0, st, (const float *)W32,
@@ -732,10 +738,10 @@ void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad,
 
 void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const void *Xt, int Mpad,
                int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial, float *ring,
-               int64_t n_slot, int P, float lamJ2, float lamh2)
+               int64_t n_slot, int P, float lamJ2, float lamh2, int gm)
 {
     EpiGrad epi{ring, Xtrial, cmds, n_slot, (int64_t)Kxp, P, q, Kx, nslots, lamJ2, lamh2};
-    const TileMap tm = make_map(Mpad / kTile, Kxp / kTile, 1);
+    const TileMap tm = make_map(Mpad / kTile, Kxp / kTile, 1, gm);
     const dim3 grid(map_grid(tm));
     if (prec == SDCA_PREC_FP32)
         hipLaunchKernelGGL((k_gemm_nt<PolF32, 1, EpiGrad>), grid, dim3(256), 0, st, (const float *)G32,
@@ -755,13 +761,12 @@ void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const
     default: break;              \
     }
 
-void update(hipStream_t st, int m, int prec, const SlotCmd *cmds, int nslots, float *ring, float *Xc, float *Xtr,
+void update(hipStream_t st, int m, int q, const SlotCmd *cmds, int nslots, float *ring, float *Xc, float *Xtr,
             uint16_t *Whi, uint16_t *Wlo, const float *X0, int64_t n_slot, int Kxp, int Kx, double *part, int nchunk)
 {
-    (void)prec;
 #define SDCA_UPD(MM)                                                                                                  \
     hipLaunchKernelGGL(k_update<MM>, dim3(nchunk, nslots), dim3(256), 0, st, cmds, ring, Xc, Xtr, Whi, Wlo, X0, n_slot, \
-                       Kxp, Kx, part, nchunk)
+                       Kxp, Kx, q, part, nchunk)
     SDCA_DISPATCH_M(m, SDCA_UPD)
 #undef SDCA_UPD
 }
@@ -779,7 +784,7 @@ void reduce_results(hipStream_t st, int m, int nslots, const double *upd_part, c
                     const double *sm_part, int nchunk, int nbchunk, double *res)
 {
     const int P = 2 * m + 3;
-    const int W = 2 * P + 4;
+    const int W = P + 5;
     hipLaunchKernelGGL(k_reduce, dim3(nslots), dim3((W + 63) / 64 * 64), 0, st, P, upd_part, dot_part, sm_part, nchunk,
                        nbchunk, res);
 }

@@ -405,11 +405,12 @@ int sdca_pair_features(sdca_ctx *c, const int32_t *pairs, int64_t npairs, float 
 
 int sdca_mi_topk(sdca_ctx *ctx, int32_t K, int32_t *partners_out, float *mi_out);
 
-int sdca_solve_loci_mlp(sdca_ctx *c, const int32_t *loci, int32_t K, int32_t npartners, float *score_rows,
-                        double *x_out, sdca_locus_stats *st)
+int sdca_solve_loci_mlp(sdca_ctx *c, const int32_t *loci, int32_t K, int32_t npartners, int32_t warm_mode,
+                        float *score_rows, double *x_out, sdca_locus_stats *st)
 {
     int rc = check(c);
     if (rc || (rc = need_mlp(c))) return rc;
+    if (warm_mode < 0 || warm_mode > 3) return sdca_fail(SDCA_E_ARG, "unknown warm_mode %d", warm_mode);
     if (!c->have_weights) return sdca_fail(SDCA_E_STATE, "sdca_set_weights must be called first");
     if (K < 0 || (K > 0 && !loci)) return sdca_fail(SDCA_E_ARG, "bad arguments");
     if (npartners < 1 || npartners > c->L - 1) return sdca_fail(SDCA_E_ARG, "npartners must be in [1, L-1]");
@@ -484,13 +485,14 @@ int sdca_solve_loci_mlp(sdca_ctx *c, const int32_t *loci, int32_t K, int32_t npa
         e = hipMemcpyAsync(d_J + p0 * q * q, m->d_y, cnt * q * q * 4, hipMemcpyDeviceToDevice, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // pairs is reused
     }
-    // fields of the J = 0 model
-    std::vector<float> hf((size_t)K * q);
+    // fields of the J = 0 model (or zero with SDCA_WARM_NO_FIELDS)
+    std::vector<float> hf((size_t)K * q, 0.f);
     std::vector<double> hd(q);
-    for (int k = 0; k < K; ++k) {
-        fields_j0(c, loci[k], hd.data());
-        for (int s = 0; s < q; ++s) hf[(size_t)k * q + s] = (float)hd[s];
-    }
+    if (!(warm_mode & SDCA_WARM_NO_FIELDS))
+        for (int k = 0; k < K; ++k) {
+            fields_j0(c, loci[k], hd.data());
+            for (int s = 0; s < q; ++s) hf[(size_t)k * q + s] = (float)hd[s];
+        }
     if (e == hipSuccess) e = hipMemcpy(d_h, hf.data(), hf.size() * 4, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         cleanup();
@@ -501,6 +503,7 @@ int sdca_solve_loci_mlp(sdca_ctx *c, const int32_t *loci, int32_t K, int32_t npa
     cw.d_partners = d_part;
     cw.d_J = d_J;
     cw.d_h = d_h;
+    cw.as_direction = !(warm_mode & SDCA_WARM_POINT);
     rc = sdca_solve_impl(c, loci, K, nullptr, &cw, score_rows, x_out, st);
     cleanup();
     return rc;

@@ -195,3 +195,36 @@ def test_cli_end_to_end(tmp_path):
     ref = tmp_path / "direct.all"
     sd.write_couplings(str(ref), rows, loci, 1)
     assert ref.read_text().splitlines() == lines
+
+
+# ------------------------------------------------------------ GEMM configurations
+
+@pytest.mark.parametrize("cfg", [dict(autotune=1), dict(gemm_ksplit=1, gemm_gm_energy=1, gemm_gm_grad=1),
+                                 dict(gemm_ksplit=3, gemm_gm_energy=2, gemm_gm_grad=4),
+                                 dict(gemm_ksplit=7, gemm_gm_energy=64, gemm_gm_grad=3)])
+@pytest.mark.parametrize("precision,ftol,gtol", [(sd.PREC_FP32, 2e-6, 2e-5), (sd.PREC_SPLIT_BF16, 2e-5, 2e-4)])
+def test_gemm_configurations_match_oracle(c1, cfg, precision, ftol, gtol):
+    # every tile walk / split-K the autotuner may choose computes the same objective;
+    # 20 slots x q = 80 rows pad to one 128-row tile, 48 slots to two
+    st = c1["states"]
+    w, _ = orc.weights(st, 0.9, nthreads=8)
+    n = orc.nparams(st.shape[1], 4)
+    loci = np.arange(3, 1000, 21)[:48]
+    x = np.random.default_rng(31).normal(size=(loci.size, n)) * 0.05
+    with sd.Context(st, precision=precision, batch_loci=48, **cfg) as ctx:
+        beff, lh, lJ = ctx.set_weights(w)
+        f, g = ctx.eval_batch(loci, x)
+        got = ctx.gemm_config()
+        kx = sd.lib().sdca_num_columns(ctx._h)
+    if "gemm_ksplit" in cfg:
+        assert not got["tuned"]
+        assert got["ksplit"] == cfg["gemm_ksplit"]
+        assert (got["gm_energy"], got["gm_grad"]) == (cfg["gemm_gm_energy"], cfg["gemm_gm_grad"])
+    else:
+        assert got["tuned"]
+    assert got["kchunk"] > 0 and got["ksplit"] * got["kchunk"] > kx  # K_x columns + the field column
+    for k in (0, 17, 47):
+        fo, go = orc.objective(st, int(loci[k]), w, lh, lJ, x[k])
+        assert f[k] == pytest.approx(fo, rel=ftol), (k, got)
+        err = np.abs(g[k] - go).max() / np.abs(go).max()
+        assert err < gtol, (k, err, got)
