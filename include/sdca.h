@@ -58,7 +58,9 @@ typedef struct sdca_opts {
     int32_t gemm_ksplit;     /* > 0: force the energy split-K slices (no tuning) */
     int32_t gemm_gm_energy;  /* > 0: force the energy XCD super-tile height, 1..64 (no tuning) */
     int32_t gemm_gm_grad;    /* > 0: force the gradient XCD super-tile height, 1..64 (no tuning) */
-    int32_t reserved[3];
+    int32_t gemm_tn_energy;  /* > 0: force the energy output tile width, 128 or 256 (no tuning) */
+    int32_t gemm_tn_grad;    /* > 0: force the gradient output tile width, 128 or 256 (no tuning) */
+    int32_t reserved[6];
 } sdca_opts;
 
 /* How sdca_solve_loci_mlp uses the network's coupling blocks (flags). */
@@ -150,8 +152,9 @@ int sdca_get_timing(sdca_ctx *ctx, sdca_timing *t, int32_t reset);
 /* GEMM configuration of the current workspace (no counterpart in SuperDCA):
  * cfg[0] = 1 if autotuned (or taken from the tuning cache), cfg[1] = energy split-K
  * slices, cfg[2] = columns per slice, cfg[3] / cfg[4] = XCD super-tile height of the
- * energy / gradient contraction.  SDCA_E_STATE before the first evaluation. */
-int sdca_gemm_config(const sdca_ctx *ctx, int32_t cfg[5]);
+ * energy / gradient contraction, cfg[5] / cfg[6] = their output tile widths.
+ * SDCA_E_STATE before the first evaluation. */
+int sdca_gemm_config(const sdca_ctx *ctx, int32_t cfg[7]);
 
 /* ---- pair statistics for the warm start (no counterpart in SuperDCA) ---- */
 

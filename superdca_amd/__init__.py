@@ -59,7 +59,7 @@ class Opts(C.Structure):
                 ("history", C.c_int32), ("max_iter", C.c_int32), ("max_ls", C.c_int32),
                 ("grad_threshold", C.c_double), ("verbose", C.c_int32), ("autotune", C.c_int32),
                 ("gemm_ksplit", C.c_int32), ("gemm_gm_energy", C.c_int32), ("gemm_gm_grad", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("gemm_tn_energy", C.c_int32), ("gemm_tn_grad", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
 class LocusStats(C.Structure):
@@ -327,9 +327,10 @@ class Context:
 
     def gemm_config(self) -> dict:
         """GEMM configuration of the current workspace (after the first evaluation)."""
-        cfg = (C.c_int32 * 5)()
+        cfg = (C.c_int32 * 7)()
         _check(lib().sdca_gemm_config(self._h, cfg))
-        return {"tuned": bool(cfg[0]), "ksplit": cfg[1], "kchunk": cfg[2], "gm_energy": cfg[3], "gm_grad": cfg[4]}
+        return {"tuned": bool(cfg[0]), "ksplit": cfg[1], "kchunk": cfg[2], "gm_energy": cfg[3], "gm_grad": cfg[4],
+                "tn_energy": cfg[5], "tn_grad": cfg[6]}
 
 
 def prepare_alignment(chars: np.ndarray, maf_threshold: float = 0.01, gap_threshold: float = 0.15,

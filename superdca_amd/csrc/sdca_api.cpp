@@ -156,7 +156,7 @@ namespace {
 // the energies) and K_x, so the first workspace of a shape times a small set of
 // candidates on the real buffers and keeps the fastest.  Choices are cached per
 // process and, when SDCA_TUNE_CACHE names a file, across processes (one text line per
-// shape: precision Mpad Bp Kxp ksplit kchunk gm_energy gm_grad).
+// shape: precision Mpad Bp Kxp ksplit kchunk gm_energy gm_grad tn_energy tn_grad).
 
 struct GemmKey {
     int prec, Mpad, Bp;
@@ -171,7 +171,7 @@ struct GemmKey {
 };
 
 struct GemmConfig {
-    int ksplit = 1, kchunk = 0, gm_e = 8, gm_g = 8;
+    int ksplit = 1, kchunk = 0, gm_e = 8, gm_g = 8, tn_e = 128, tn_g = 128;
 };
 
 std::mutex g_tune_mu;
@@ -188,10 +188,12 @@ void read_tune_file()
     GemmKey k;
     GemmConfig g;
     long long kxp;
-    while (std::fscanf(f, "%d %d %d %lld %d %d %d %d", &k.prec, &k.Mpad, &k.Bp, &kxp, &g.ksplit, &g.kchunk, &g.gm_e,
-                       &g.gm_g) == 8) {
+    while (std::fscanf(f, "%d %d %d %lld %d %d %d %d %d %d", &k.prec, &k.Mpad, &k.Bp, &kxp, &g.ksplit, &g.kchunk,
+                       &g.gm_e, &g.gm_g, &g.tn_e, &g.tn_g) == 10) {
         k.Kxp = kxp;
-        if (g.ksplit >= 1 && g.kchunk > 0 && (int64_t)g.ksplit * g.kchunk >= k.Kxp && g.gm_e >= 1 && g.gm_g >= 1)
+        const bool tn_ok = (g.tn_e == 128 || g.tn_e == 256) && (g.tn_g == 128 || g.tn_g == 256);
+        if (g.ksplit >= 1 && g.kchunk > 0 && (int64_t)g.ksplit * g.kchunk >= k.Kxp && g.gm_e >= 1 && g.gm_e <= 64 &&
+            g.gm_g >= 1 && g.gm_g <= 64 && tn_ok)
             g_tune[k] = g;
     }
     std::fclose(f);
@@ -214,8 +216,8 @@ void store_config(const GemmKey &k, const GemmConfig &g)
     const char *path = std::getenv("SDCA_TUNE_CACHE");
     if (!path || !*path) return;
     if (FILE *f = std::fopen(path, "a")) {
-        std::fprintf(f, "%d %d %d %lld %d %d %d %d\n", k.prec, k.Mpad, k.Bp, (long long)k.Kxp, g.ksplit, g.kchunk,
-                     g.gm_e, g.gm_g);
+        std::fprintf(f, "%d %d %d %lld %d %d %d %d %d %d\n", k.prec, k.Mpad, k.Bp, (long long)k.Kxp, g.ksplit,
+                     g.kchunk, g.gm_e, g.gm_g, g.tn_e, g.tn_g);
         std::fclose(f);
     }
 }
@@ -229,36 +231,39 @@ void set_split(const sdca_ctx *c, int64_t ks, GemmConfig &g)
     g.ksplit = (int)((c->Kxp + g.kchunk - 1) / g.kchunk);
 }
 
-// Default rule: >= 2048 energy workgroups, the number of super-tiles a multiple of the
-// eight XCDs, 8-row super-tiles for both contractions.
-GemmConfig default_config(const sdca_ctx *c, int gm)
+// Default rule: >= 2048 energy workgroups (128-wide tiles) with the number of
+// super-tiles a multiple of the eight XCDs, super-tiles gm tiles high.
+GemmConfig default_config(const sdca_ctx *c, int gm, int tn_e = 128)
 {
     GemmConfig g;
     g.gm_e = g.gm_g = gm;
+    g.tn_e = tn_e;
     const int64_t tiles = (int64_t)(c->Mpad / sdca::kTile) * (c->Bp / sdca::kTile);
     int64_t ks = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
-    const int64_t sup = sdca::kern::energy_superiles(c->Mpad, c->Bp, gm);
+    const int64_t sup = sdca::kern::energy_superiles(c->Mpad, c->Bp, gm, tn_e);
     while ((ks * sup) % 8 != 0 && ks < 64) ++ks;
     set_split(c, ks, g);
     return g;
 }
 
-// Energy candidates: super-tile heights 1..8 and half / default / double split-K.
+// Energy candidates: tile widths 128 and 256, super-tile heights 1..8 and half /
+// default / double split-K.
 std::vector<GemmConfig> tuning_candidates(const sdca_ctx *c)
 {
     std::vector<GemmConfig> v;
     const int Mt = c->Mpad / sdca::kTile;
-    for (int gm = 1; gm <= 8; gm *= 2) {
-        if (gm > 1 && gm / 2 >= Mt) break;  // taller than the matrix: same as the previous one
-        const GemmConfig d = default_config(c, gm);
-        for (int64_t ks : {(int64_t)std::max(1, d.ksplit / 2), (int64_t)d.ksplit, (int64_t)d.ksplit * 2}) {
-            GemmConfig g = d;
-            set_split(c, ks, g);
-            bool dup = false;
-            for (const auto &o : v) dup |= o.ksplit == g.ksplit && o.gm_e == g.gm_e;
-            if (!dup) v.push_back(g);
+    for (int tn : {128, 256})
+        for (int gm = 1; gm <= 8; gm *= 2) {
+            if (gm > 1 && gm / 2 >= Mt) break;  // taller than the matrix: same as the previous one
+            const GemmConfig d = default_config(c, gm, tn);
+            for (int64_t ks : {(int64_t)std::max(1, d.ksplit / 2), (int64_t)d.ksplit, (int64_t)d.ksplit * 2}) {
+                GemmConfig g = d;
+                set_split(c, ks, g);
+                bool dup = false;
+                for (const auto &o : v) dup |= o.ksplit == g.ksplit && o.gm_e == g.gm_e && o.tn_e == g.tn_e;
+                if (!dup) v.push_back(g);
+            }
         }
-    }
     return v;
 }
 
@@ -286,14 +291,14 @@ int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand
     const int reps = 3;
     auto energy = [&](const GemmConfig &g) {
         sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_X, c->Mpad, c->Bp,
-                                (int)c->Kxp, g.ksplit, g.kchunk, g.gm_e, c->d_Epart);
+                                (int)c->Kxp, g.ksplit, g.kchunk, g.gm_e, g.tn_e, c->d_Epart);
         sdca::kern::softmax(st, c->opts.precision, c->d_Epart, g.ksplit, c->Mpad, c->Bp, c->B, c->q, nslots,
                             c->d_cmds, c->d_statesT, c->d_w, c->d_G32, c->d_Ghi, c->d_Glo, c->d_sm_part, c->nbchunk);
     };
-    auto grad = [&](int gm) {
+    auto grad = [&](int gm, int tn) {
         sdca::kern::gemm_grad(st, c->opts.precision, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Xt, c->Mpad, c->Bp,
                               (int)c->Kxp, (int)c->Kx, c->q, nslots, c->d_cmds, c->d_Xtr, c->d_ring, c->n_slot, P,
-                              (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), gm);
+                              (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), gm, tn);
     };
     auto time = [&](auto &&launch) -> float {
         launch();
@@ -309,26 +314,30 @@ int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand
         const float t = time([&] { energy(g); });
         if (t < 0) return SDCA_E_HIP;
         if (c->opts.verbose)
-            std::fprintf(stderr, "sdca tune: energy+softmax ksplit %d gm %d: %.3f ms\n", g.ksplit, g.gm_e, t);
+            std::fprintf(stderr, "sdca tune: energy+softmax tn %d ksplit %d gm %d: %.3f ms\n", g.tn_e, g.ksplit,
+                         g.gm_e, t);
         if (t < bt) {
             bt = t;
             best.ksplit = g.ksplit;
             best.kchunk = g.kchunk;
             best.gm_e = g.gm_e;
+            best.tn_e = g.tn_e;
         }
     }
     energy(best);  // node beliefs G for the gradient candidates
     bt = 1e30f;
     const int Mt = c->Mpad / sdca::kTile;
-    for (int gm = 1; gm <= 8; gm *= 2) {
-        if (gm > 1 && gm / 2 >= Mt) break;
-        const float t = time([&] { grad(gm); });
-        if (c->opts.verbose) std::fprintf(stderr, "sdca tune: grad gm %d: %.3f ms\n", gm, t);
-        if (t < bt) {
-            bt = t;
-            best.gm_g = gm;
+    for (int tn : {128, 256})
+        for (int gm = 1; gm <= 8; gm *= 2) {
+            if (gm > 1 && gm / 2 >= Mt) break;
+            const float t = time([&] { grad(gm, tn); });
+            if (c->opts.verbose) std::fprintf(stderr, "sdca tune: grad tn %d gm %d: %.3f ms\n", tn, gm, t);
+            if (t < bt) {
+                bt = t;
+                best.gm_g = gm;
+                best.tn_g = tn;
+            }
         }
-    }
     // the solve starts from zeroed buffers
     HIP_TRY(hipMemsetAsync(c->d_ring, 0, (size_t)nslots * P * c->n_slot * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(c->d_Xtr, 0, nw * sizeof(float), st));
@@ -365,14 +374,18 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     // GEMM configuration: the cached/tuned one for this shape, else the default rule
     const GemmKey key{c->opts.precision, c->Mpad, c->Bp, c->Kxp};
     const sdca_opts &o = c->opts;
-    const bool forced = o.gemm_ksplit > 0 || o.gemm_gm_energy > 0 || o.gemm_gm_grad > 0;
+    const bool forced = o.gemm_ksplit > 0 || o.gemm_gm_energy > 0 || o.gemm_gm_grad > 0 || o.gemm_tn_energy > 0 ||
+                        o.gemm_tn_grad > 0;
     GemmConfig cfg;
     bool have = !forced && lookup_config(key, cfg);
     const bool tune = !forced && !have && o.autotune;
-    if (!have) cfg = default_config(c, o.gemm_gm_energy > 0 ? o.gemm_gm_energy : 8);
+    if (!have)
+        cfg = default_config(c, o.gemm_gm_energy > 0 ? o.gemm_gm_energy : 8,
+                             o.gemm_tn_energy > 0 ? o.gemm_tn_energy : 128);
     if (forced) {
         if (o.gemm_ksplit > 0) set_split(c, o.gemm_ksplit, cfg);
         cfg.gm_g = o.gemm_gm_grad > 0 ? o.gemm_gm_grad : 8;
+        cfg.tn_g = o.gemm_tn_grad > 0 ? o.gemm_tn_grad : 128;
     }
     int max_ksplit = cfg.ksplit;
     std::vector<GemmConfig> cand;
@@ -399,6 +412,8 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     c->kchunk = cfg.kchunk;
     c->gm_energy = cfg.gm_e;
     c->gm_grad = cfg.gm_g;
+    c->tn_energy = cfg.tn_e;
+    c->tn_grad = cfg.tn_g;
     c->tuned = tune || have;
     if (tune) store_config(key, cfg);
     c->ws_slots = nslots;
@@ -428,7 +443,7 @@ int run_round(sdca_ctx *c, int nslots, const std::vector<sdca::SlotCmd> &cmds, s
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[1], st));
     sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_X, c->Mpad, c->Bp, (int)c->Kxp,
-                            c->ksplit, c->kchunk, c->gm_energy, c->d_Epart);
+                            c->ksplit, c->kchunk, c->gm_energy, c->tn_energy, c->d_Epart);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[2], st));
     sdca::kern::softmax(st, c->opts.precision, c->d_Epart, c->ksplit, c->Mpad, c->Bp, c->B, c->q, nslots, c->d_cmds,
@@ -437,7 +452,7 @@ int run_round(sdca_ctx *c, int nslots, const std::vector<sdca::SlotCmd> &cmds, s
     HIP_TRY(hipEventRecord(c->ev[3], st));
     sdca::kern::gemm_grad(st, c->opts.precision, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Xt, c->Mpad, c->Bp, (int)c->Kxp,
                           (int)c->Kx, c->q, nslots, c->d_cmds, c->d_Xtr, c->d_ring, c->n_slot, P,
-                          (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), c->gm_grad);
+                          (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), c->gm_grad, c->tn_grad);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->ev[4], st));
     sdca::kern::dots(st, m, c->d_cmds, nslots, c->d_ring, c->n_slot, c->d_dot_part, c->nchunk);
@@ -919,6 +934,8 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
     if (o.gemm_ksplit < 0 || o.gemm_ksplit > 4096 || o.gemm_gm_energy < 0 || o.gemm_gm_energy > 64 ||
         o.gemm_gm_grad < 0 || o.gemm_gm_grad > 64)
         return fail(SDCA_E_ARG, "gemm_ksplit must be in [0, 4096] and gemm_gm_* in [0, 64]");
+    for (int tn : {o.gemm_tn_energy, o.gemm_tn_grad})
+        if (tn != 0 && tn != 128 && tn != 256) return fail(SDCA_E_ARG, "gemm_tn_* must be 0, 128 or 256");
     if (o.precision != SDCA_PREC_FP32 && o.precision != SDCA_PREC_SPLIT_BF16)
         return fail(SDCA_E_ARG, "unknown precision %d", o.precision);
     if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 10, 16 (got %d)", o.history);
@@ -954,8 +971,8 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
     }
     c->col_off[L] = (int32_t)col;
     c->Kx = col;
-    c->Kxp = round_up(c->Kx + 1, sdca::kTile);
-    c->Bp = (int)round_up(B, sdca::kTile);
+    c->Kxp = round_up(c->Kx + 1, sdca::kTileWide);
+    c->Bp = (int)round_up(B, sdca::kTileWide);
     if (c->Kxp >= (int64_t(1) << 31) / 8) {
         delete c;
         return fail(SDCA_E_ARG, "alignment too wide (K_x = %lld)", (long long)col);
@@ -1405,7 +1422,7 @@ int sdca_mi_topk(sdca_ctx *c, int32_t K, int32_t *partners_out, float *mi_out)
     return SDCA_OK;
 }
 
-int sdca_gemm_config(const sdca_ctx *c, int32_t cfg[5])
+int sdca_gemm_config(const sdca_ctx *c, int32_t cfg[7])
 {
     if (!c || !cfg) return fail(SDCA_E_ARG, "null argument");
     if (!c->ws_slots) return fail(SDCA_E_STATE, "no workspace yet");
@@ -1414,6 +1431,8 @@ int sdca_gemm_config(const sdca_ctx *c, int32_t cfg[5])
     cfg[2] = c->kchunk;
     cfg[3] = c->gm_energy;
     cfg[4] = c->gm_grad;
+    cfg[5] = c->tn_energy;
+    cfg[6] = c->tn_grad;
     return SDCA_OK;
 }
 

@@ -23,7 +23,8 @@
 
 namespace sdca {
 
-constexpr int kTile = 128;        // GEMM output tile (rows and columns)
+constexpr int kTile = 128;        // GEMM output tile rows (and narrow tile columns)
+constexpr int kTileWide = 256;    // wide GEMM tile columns; B and K_x are padded to this
 constexpr int kMaxHistory = 16;   // largest L-BFGS history supported
 constexpr int kMaxPhys = 2 * kMaxHistory + 3;  // D ring (m+1) + G ring (m+2)
 
@@ -70,15 +71,15 @@ void build_x(hipStream_t st, const uint8_t *d_states, int B, int L, int q, const
 void bitplanes(hipStream_t st, const uint8_t *d_states, int B, int L, int q, int W, uint64_t *planes);
 void hamming_neighbours(hipStream_t st, const uint64_t *planes, int B, int q, int W, int L, int64_t dmax,
                         const uint32_t *d_mult, unsigned long long *d_iw);
-int energy_superiles(int Mpad, int Bp, int gm);  // super-tiles per split-K slice
+int energy_superiles(int Mpad, int Bp, int gm, int tn);  // super-tiles per split-K slice
 void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const void *X,
-                 int Mpad, int Bp, int Kxp, int ksplit, int kchunk, int gm, float *Epart);
+                 int Mpad, int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn, float *Epart);
 void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad, int Bp, int B, int q, int nslots,
              const SlotCmd *cmds, const uint8_t *statesT, const float *wts, float *G32, uint16_t *Ghi,
              uint16_t *Glo, double *fpart, int nbchunk);
 void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const void *Xt,
                int Mpad, int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial,
-               float *ring, int64_t n_slot, int P, float lamJ2, float lamh2, int gm);
+               float *ring, int64_t n_slot, int P, float lamJ2, float lamh2, int gm, int tn);
 void update(hipStream_t st, int m, int q, const SlotCmd *cmds, int nslots, float *ring, float *Xc, float *Xtr,
             uint16_t *Whi, uint16_t *Wlo, const float *X0, int64_t n_slot, int Kxp, int Kx, double *part,
             int nchunk);
@@ -171,6 +172,7 @@ struct sdca_ctx {
     int Mpad = 0;
     int ksplit = 1, kchunk = 0;
     int gm_energy = 8, gm_grad = 8;  // XCD super-tile heights (autotuned)
+    int tn_energy = 128, tn_grad = 128;  // output tile widths (autotuned)
     int tuned = 0;                   // 1 if the autotuner chose the GEMM configuration
     int nchunk = 0, nbchunk = 0;
     float *d_ring = nullptr, *d_Xc = nullptr, *d_Xtr = nullptr, *d_X0 = nullptr;

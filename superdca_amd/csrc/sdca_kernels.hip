@@ -146,9 +146,11 @@ __global__ __launch_bounds__(256) void k_hamming(const uint64_t *__restrict__ pl
 
 // ------------------------------------------------------------------- GEMM
 
-// C[m][n] = sum_k A[m][k] * Bm[n][k], 128x128 tile per 256-thread workgroup, four
-// waves of 64x64 (2x2 MFMA 32x32 blocks), K staged through LDS in rows of 128 bytes.
-// NA = 2 sums two A variants (hi and lo BF16 parts) against the same B fragment.
+// C[m][n] = sum_k A[m][k] * Bm[n][k] on 256-thread workgroups of four waves (2 x 2).
+// A tile is 128 rows of A by TN = 64 * NI rows of Bm; each wave computes 64 x 32*NI
+// outputs as 2 x NI MFMA 32x32 blocks, with K staged through LDS in rows of 128 bytes.
+// NA = 2 sums two A variants (hi and lo BF16 parts) against the same B fragment.  The
+// wide tile (NI = 4) reads 0.5 LDS fragments per MFMA instead of 0.75 at NA = 2.
 constexpr int ROWB = 144;  // LDS bytes per staged row (128 data + 16 pad)
 
 struct PolBF16 {
@@ -162,15 +164,15 @@ struct PolF32 {
     static constexpr int KPM = 2;
 };
 
-template <class Pol, int NA>
+template <class Pol, int NA, int NI>
 __device__ __forceinline__ void mma_tile(const uint8_t *sA0, const uint8_t *sA1, const uint8_t *sB, int wm, int wn,
-                                         int lane, floatx16 (&acc)[2][2])
+                                         int lane, floatx16 (&acc)[2][NI])
 {
     if constexpr (sizeof(typename Pol::T) == 2) {
 #pragma unroll
         for (int kk = 0; kk < Pol::BK; kk += Pol::KPM) {
             const int kb = (kk + (lane >> 5) * 8) * 2;
-            bf16x8 a[NA][2], b[2];
+            bf16x8 a[NA][2], b[NI];
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi) {
                 const int row = wm + mi * 32 + (lane & 31);
@@ -178,14 +180,14 @@ __device__ __forceinline__ void mma_tile(const uint8_t *sA0, const uint8_t *sA1,
                 if constexpr (NA == 2) a[NA - 1][mi] = *reinterpret_cast<const bf16x8 *>(sA1 + row * ROWB + kb);
             }
 #pragma unroll
-            for (int ni = 0; ni < 2; ++ni) {
+            for (int ni = 0; ni < NI; ++ni) {
                 const int row = wn + ni * 32 + (lane & 31);
                 b[ni] = *reinterpret_cast<const bf16x8 *>(sB + row * ROWB + kb);
             }
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-                for (int ni = 0; ni < 2; ++ni)
+                for (int ni = 0; ni < NI; ++ni)
 #pragma unroll
                     for (int v = 0; v < NA; ++v)
                         acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[v][mi], b[ni], acc[mi][ni], 0, 0, 0);
@@ -194,7 +196,7 @@ __device__ __forceinline__ void mma_tile(const uint8_t *sA0, const uint8_t *sA1,
 #pragma unroll
         for (int kk = 0; kk < Pol::BK; kk += Pol::KPM) {
             const int kb = (kk + (lane >> 5)) * 4;
-            float a[NA][2], b[2];
+            float a[NA][2], b[NI];
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi) {
                 const int row = wm + mi * 32 + (lane & 31);
@@ -202,14 +204,14 @@ __device__ __forceinline__ void mma_tile(const uint8_t *sA0, const uint8_t *sA1,
                 if constexpr (NA == 2) a[NA - 1][mi] = *reinterpret_cast<const float *>(sA1 + row * ROWB + kb);
             }
 #pragma unroll
-            for (int ni = 0; ni < 2; ++ni) {
+            for (int ni = 0; ni < NI; ++ni) {
                 const int row = wn + ni * 32 + (lane & 31);
                 b[ni] = *reinterpret_cast<const float *>(sB + row * ROWB + kb);
             }
 #pragma unroll
             for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-                for (int ni = 0; ni < 2; ++ni)
+                for (int ni = 0; ni < NI; ++ni)
 #pragma unroll
                     for (int v = 0; v < NA; ++v)
                         acc[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[v][mi], b[ni], acc[mi][ni], 0, 0, 0);
@@ -243,26 +245,29 @@ __device__ __forceinline__ bool map_tile(const TileMap &tm, int bid, int &mt, in
     return mt < tm.Mt && nt < tm.Nt;
 }
 
-template <class Pol, int NA, class Epi>
-__global__ __launch_bounds__(256) void k_gemm_nt(const typename Pol::T *__restrict__ A0,
+// Two workgroups per CU (eight waves): the register budget of a wave is 256 (VGPRs +
+// AGPRs), which holds the wide tile's accumulators and one prefetch stage.
+template <class Pol, int NA, int NI, class Epi>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_nt(const typename Pol::T *__restrict__ A0,
                                                     const typename Pol::T *__restrict__ A1, int64_t lda,
                                                     const typename Pol::T *__restrict__ Bm, int64_t ldb, int K,
                                                     int kchunk, TileMap tm, Epi epi)
 {
+    constexpr int TN = 64 * NI;  // B rows (output columns) per tile
     __shared__ __attribute__((aligned(16))) uint8_t sA[NA][kTile * ROWB];
-    __shared__ __attribute__((aligned(16))) uint8_t sB[kTile * ROWB];
+    __shared__ __attribute__((aligned(16))) uint8_t sB[TN * ROWB];
     int mt, nt, z;
     if (!map_tile(tm, blockIdx.x, mt, nt, z)) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t m0 = (int64_t)mt * kTile, n0 = (int64_t)nt * kTile;
+    const int64_t m0 = (int64_t)mt * kTile, n0 = (int64_t)nt * TN;
     const int kb = z * kchunk;
     const int ke = min(K, kb + kchunk);
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-    floatx16 acc[2][2];
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * (TN / 2);
+    floatx16 acc[2][NI];
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
+        for (int ni = 0; ni < NI; ++ni)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0.f;
     const int lrow = tid >> 3;         // 0..31
@@ -273,59 +278,71 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const typename Pol::T *__restri
     const int64_t sa = 32 * lda * (int64_t)sizeof(typename Pol::T);  // 32 rows, in bytes
     const int64_t sbb = 32 * ldb * (int64_t)sizeof(typename Pol::T);
     constexpr int EB = (int)sizeof(typename Pol::T);
-    u32x4 ra[NA][4], rb[4];
-    // software pipeline: the next K slice is in flight from HBM while the current one
+    constexpr int RB = TN / 32;  // 32-row passes over the B tile
+    constexpr int BK = Pol::BK;
+    // software pipeline: the next K slice is in flight from memory while the current one
     // is multiplied out of LDS
+    u32x4 ra[NA][4], rb[RB];
     auto load = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             ra[0][i] = *reinterpret_cast<const u32x4 *>(pa0 + i * sa + (int64_t)k0 * EB);
             if constexpr (NA == 2) ra[NA - 1][i] = *reinterpret_cast<const u32x4 *>(pa1 + i * sa + (int64_t)k0 * EB);
-            rb[i] = *reinterpret_cast<const u32x4 *>(pb + i * sbb + (int64_t)k0 * EB);
         }
+#pragma unroll
+        for (int i = 0; i < RB; ++i) rb[i] = *reinterpret_cast<const u32x4 *>(pb + i * sbb + (int64_t)k0 * EB);
     };
     if (kb < ke) load(kb);
-    for (int k0 = kb; k0 < ke; k0 += Pol::BK) {
+    for (int k0 = kb; k0 < ke; k0 += BK) {
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int row = lrow + 32 * i;
 #pragma unroll
             for (int v = 0; v < NA; ++v) *reinterpret_cast<u32x4 *>(&sA[v][row * ROWB + lcol]) = ra[v][i];
-            *reinterpret_cast<u32x4 *>(&sB[row * ROWB + lcol]) = rb[i];
         }
+#pragma unroll
+        for (int i = 0; i < RB; ++i) *reinterpret_cast<u32x4 *>(&sB[(lrow + 32 * i) * ROWB + lcol]) = rb[i];
         __syncthreads();
-        if (k0 + Pol::BK < ke) load(k0 + Pol::BK);
-        mma_tile<Pol, NA>(sA[0], sA[NA - 1], sB, wm, wn, lane, acc);
+        if (k0 + BK < ke) load(k0 + BK);
+        mma_tile<Pol, NA, NI>(sA[0], sA[NA - 1], sB, wm, wn, lane, acc);
     }
-    // 32x32 MFMA result layout: register v of lane l holds row 8*(v/4) + 4*(l/32) + v%4,
-    // column l%32 of the block.
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < 2; ++ni)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) {
-                const int64_t m = m0 + wm + mi * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-                const int64_t n = n0 + wn + ni * 32 + (lane & 31);
-                epi(m, n, z, acc[mi][ni][v]);
-            }
+    epi.template tile<NI>((int)m0, wm, n0 + wn, lane, tid, acc, z);
 }
+
+// 32x32 MFMA result layout: register v of lane l holds row 8*(v/4) + 4*(l/32) + v%4,
+// column l%32 of the block.
+__device__ __forceinline__ int acc_row(int mi, int v, int lane) { return mi * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
+
+// Epilogues receive a wave's 64 x 32*NI block of outputs: rows m0 + wm + acc_row(),
+// columns nw + 32 ni + lane % 32.
 
 // Energy epilogue: split-K partial sums, Epart[z][m][n].
 struct EpiEnergy {
     float *E;
     int64_t ld, zstride;
-    __device__ __forceinline__ void operator()(int64_t m, int64_t n, int z, float v) const
+    template <int NI>
+    __device__ __forceinline__ void tile(int m0, int wm, int64_t nw, int lane, int, floatx16 (&acc)[2][NI],
+                                         int z) const
     {
-        E[(int64_t)z * zstride + m * ld + n] = v;
+        float *base = E + (int64_t)z * zstride + nw + (lane & 31);
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                float *row = base + (int64_t)(m0 + wm + acc_row(mi, v, lane)) * ld;
+#pragma unroll
+                for (int ni = 0; ni < NI; ++ni) row[ni * 32] = acc[mi][ni][v];
+            }
     }
 };
 
-// Gradient epilogue: row m = slot*q + s, column k.  Adds 2*lambda*x (lambda_h on the
+// Gradient epilogue: row m = slot*q + s, column n.  Adds 2*lambda*x (lambda_h on the
 // bias column, lambda_J elsewhere: plmDCA_objective.hpp:192-244), zeroes the locus'
 // own columns (the j != r exclusions of :95,111,161,176) and pad columns, and writes
-// the slot's trial-gradient ring vector.
+// the slot's trial-gradient ring vector.  The commands of the slots under the tile's
+// 128 rows are staged in LDS once; each group of four rows first loads its trial-point
+// values, then stores.
 struct EpiGrad {
     float *ring;
     const float *Xtrial;
@@ -333,17 +350,58 @@ struct EpiGrad {
     int64_t n_slot, Kxp;
     int P, q, Kx, nslots;
     float lamJ2, lamh2;
-    __device__ __forceinline__ void operator()(int64_t m, int64_t n, int, float v) const
+    template <int NI>
+    __device__ __forceinline__ void tile(int m0, int wm, int64_t nw, int lane, int tid, floatx16 (&acc)[2][NI],
+                                         int) const
     {
-        const int i = (int)(m / q);
-        if (i >= nslots) return;
-        const SlotCmd &c = cmds[i];
-        if (c.mode == MODE_IDLE) return;
-        const int s = (int)(m - (int64_t)i * q);
-        float val = 0.f;
-        if (n <= Kx && !(n >= c.self_lo && n < c.self_hi))
-            val = v + (n == Kx ? lamh2 : lamJ2) * Xtrial[m * Kxp + n];
-        ring[((int64_t)i * P + c.gt) * n_slot + (int64_t)s * Kxp + n] = val;
+        __shared__ int s_gt[kTile + 2], s_lo[kTile + 2], s_hi[kTile + 2];  // s_gt < 0: idle slot
+        const int i0 = m0 / q;
+        const int ns = min(nslots - i0, (kTile - 1) / q + 2);  // slots under rows [m0, m0 + 128)
+        if (tid < ns) {
+            const SlotCmd &c = cmds[i0 + tid];
+            s_gt[tid] = c.mode == MODE_IDLE ? -1 : c.gt;
+            s_lo[tid] = c.self_lo;
+            s_hi[tid] = c.self_hi;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+            for (int vq = 0; vq < 16; vq += 4) {  // four rows at a time
+                int gt[4], lo[4], hi[4];
+                int64_t roff[4];
+                float xv[4][NI];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int m = m0 + wm + acc_row(mi, vq + u, lane);
+                    const int i = m / q;
+                    const int il = i - i0;
+                    const bool ok = il < ns;
+                    gt[u] = ok ? s_gt[il] : -1;
+                    lo[u] = ok ? s_lo[il] : 0;
+                    hi[u] = ok ? s_hi[il] : 0;
+                    roff[u] = ((int64_t)i * P + (gt[u] < 0 ? 0 : gt[u])) * n_slot + (int64_t)(m - i * q) * Kxp;
+                    const float *xr = Xtrial + (int64_t)m * Kxp;
+#pragma unroll
+                    for (int ni = 0; ni < NI; ++ni) {
+                        const int64_t n = nw + ni * 32 + (lane & 31);
+                        const bool use = gt[u] >= 0 && n <= Kx && !(n >= lo[u] && n < hi[u]);
+                        xv[u][ni] = use ? xr[n] : 0.f;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    if (gt[u] < 0) continue;
+#pragma unroll
+                    for (int ni = 0; ni < NI; ++ni) {
+                        const int64_t n = nw + ni * 32 + (lane & 31);
+                        float val = 0.f;
+                        if (n <= Kx && !(n >= lo[u] && n < hi[u]))
+                            val = acc[mi][ni][vq + u] + (n == Kx ? lamh2 : lamJ2) * xv[u][ni];
+                        ring[roff[u] + n] = val;
+                    }
+                }
+            }
     }
 };
 
@@ -707,24 +765,36 @@ static TileMap make_map(int Mt, int Nt, int Z, int gm)
 
 static unsigned map_grid(const TileMap &tm) { return (unsigned)(((tm.S + 7) / 8) * 8 * tm.GM * tm.GN); }
 
-int energy_superiles(int Mpad, int Bp, int gm)
+int energy_superiles(int Mpad, int Bp, int gm, int tn)
 {
-    const TileMap tm = make_map(Mpad / kTile, Bp / kTile, 1, gm);
+    const TileMap tm = make_map(Mpad / kTile, Bp / tn, 1, gm);
     return tm.MB * tm.NB;
 }
 
+// Launches k_gemm_nt with output tiles 128 x tn (tn = 128 or 256; N must be a multiple).
+template <class Pol, int NA, class Epi>
+static void launch_gemm(hipStream_t st, int tn, const TileMap &tm, const void *A0, const void *A1, int64_t lda,
+                        const void *Bm, int64_t ldb, int K, int kchunk, const Epi &epi)
+{
+    using T = typename Pol::T;
+    const dim3 grid(map_grid(tm));
+    if (tn == 256)
+        hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 4, Epi>), grid, dim3(256), 0, st, (const T *)A0, (const T *)A1, lda,
+                           (const T *)Bm, ldb, K, kchunk, tm, epi);
+    else
+        hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 2, Epi>), grid, dim3(256), 0, st, (const T *)A0, (const T *)A1, lda,
+                           (const T *)Bm, ldb, K, kchunk, tm, epi);
+}
+
 void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const void *X, int Mpad,
-                 int Bp, int Kxp, int ksplit, int kchunk, int gm, float *Epart)
+                 int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn, float *Epart)
 {
     EpiEnergy epi{Epart, (int64_t)Bp, (int64_t)Mpad * Bp};
-    const TileMap tm = make_map(Mpad / kTile, Bp / kTile, ksplit, gm);
-    const dim3 grid(map_grid(tm));
+    const TileMap tm = make_map(Mpad / kTile, Bp / tn, ksplit, gm);
     if (prec == SDCA_PREC_FP32)
-        hipLaunchKernelGGL((k_gemm_nt<PolF32, 1, EpiEnergy>), grid, dim3(256), 0, st, (const float *)W32,
-                           (const float *)W32, (int64_t)Kxp, (const float *)X, (int64_t)Kxp, Kxp, kchunk, tm, epi);
+        launch_gemm<PolF32, 1>(st, tn, tm, W32, W32, Kxp, X, Kxp, Kxp, kchunk, epi);
     else
-        hipLaunchKernelGGL((k_gemm_nt<PolBF16, 2, EpiEnergy>), grid, dim3(256), 0, st, (const uint16_t *)Whi,
-                           (const uint16_t *)Wlo, (int64_t)Kxp, (const uint16_t *)X, (int64_t)Kxp, Kxp, kchunk, tm, epi);
+        launch_gemm<PolBF16, 2>(st, tn, tm, Whi, Wlo, Kxp, X, Kxp, Kxp, kchunk, epi);
 }
 
 void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad, int Bp, int B, int q, int nslots,
@@ -738,17 +808,14 @@ void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad,
 
 void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const void *Xt, int Mpad,
                int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial, float *ring,
-               int64_t n_slot, int P, float lamJ2, float lamh2, int gm)
+               int64_t n_slot, int P, float lamJ2, float lamh2, int gm, int tn)
 {
     EpiGrad epi{ring, Xtrial, cmds, n_slot, (int64_t)Kxp, P, q, Kx, nslots, lamJ2, lamh2};
-    const TileMap tm = make_map(Mpad / kTile, Kxp / kTile, 1, gm);
-    const dim3 grid(map_grid(tm));
+    const TileMap tm = make_map(Mpad / kTile, Kxp / tn, 1, gm);
     if (prec == SDCA_PREC_FP32)
-        hipLaunchKernelGGL((k_gemm_nt<PolF32, 1, EpiGrad>), grid, dim3(256), 0, st, (const float *)G32,
-                           (const float *)G32, (int64_t)Bp, (const float *)Xt, (int64_t)Bp, Bp, Bp, tm, epi);
+        launch_gemm<PolF32, 1>(st, tn, tm, G32, G32, Bp, Xt, Bp, Bp, Bp, epi);
     else
-        hipLaunchKernelGGL((k_gemm_nt<PolBF16, 2, EpiGrad>), grid, dim3(256), 0, st, (const uint16_t *)Ghi,
-                           (const uint16_t *)Glo, (int64_t)Bp, (const uint16_t *)Xt, (int64_t)Bp, Bp, Bp, tm, epi);
+        launch_gemm<PolBF16, 2>(st, tn, tm, Ghi, Glo, Bp, Xt, Bp, Bp, Bp, epi);
 }
 
 #define SDCA_DISPATCH_M(m, CALL) \

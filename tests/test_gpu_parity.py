@@ -199,9 +199,12 @@ def test_cli_end_to_end(tmp_path):
 
 # ------------------------------------------------------------ GEMM configurations
 
-@pytest.mark.parametrize("cfg", [dict(autotune=1), dict(gemm_ksplit=1, gemm_gm_energy=1, gemm_gm_grad=1),
-                                 dict(gemm_ksplit=3, gemm_gm_energy=2, gemm_gm_grad=4),
-                                 dict(gemm_ksplit=7, gemm_gm_energy=64, gemm_gm_grad=3)])
+@pytest.mark.parametrize("cfg", [
+    dict(autotune=1),
+    dict(gemm_ksplit=1, gemm_gm_energy=1, gemm_gm_grad=1, gemm_tn_energy=128, gemm_tn_grad=128),
+    dict(gemm_ksplit=3, gemm_gm_energy=2, gemm_gm_grad=4, gemm_tn_energy=256, gemm_tn_grad=256),
+    dict(gemm_ksplit=7, gemm_gm_energy=64, gemm_gm_grad=3, gemm_tn_energy=256, gemm_tn_grad=128),
+    dict(gemm_ksplit=2, gemm_gm_energy=1, gemm_gm_grad=8, gemm_tn_energy=128, gemm_tn_grad=256)])
 @pytest.mark.parametrize("precision,ftol,gtol", [(sd.PREC_FP32, 2e-6, 2e-5), (sd.PREC_SPLIT_BF16, 2e-5, 2e-4)])
 def test_gemm_configurations_match_oracle(c1, cfg, precision, ftol, gtol):
     # every tile walk / split-K the autotuner may choose computes the same objective;
@@ -220,6 +223,7 @@ def test_gemm_configurations_match_oracle(c1, cfg, precision, ftol, gtol):
         assert not got["tuned"]
         assert got["ksplit"] == cfg["gemm_ksplit"]
         assert (got["gm_energy"], got["gm_grad"]) == (cfg["gemm_gm_energy"], cfg["gemm_gm_grad"])
+        assert (got["tn_energy"], got["tn_grad"]) == (cfg["gemm_tn_energy"], cfg["gemm_tn_grad"])
     else:
         assert got["tuned"]
     assert got["kchunk"] > 0 and got["ksplit"] * got["kchunk"] > kx  # K_x columns + the field column
