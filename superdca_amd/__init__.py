@@ -59,7 +59,8 @@ class Opts(C.Structure):
                 ("history", C.c_int32), ("max_iter", C.c_int32), ("max_ls", C.c_int32),
                 ("grad_threshold", C.c_double), ("verbose", C.c_int32), ("autotune", C.c_int32),
                 ("gemm_ksplit", C.c_int32), ("gemm_gm_energy", C.c_int32), ("gemm_gm_grad", C.c_int32),
-                ("gemm_tn_energy", C.c_int32), ("gemm_tn_grad", C.c_int32), ("reserved", C.c_int32 * 6)]
+                ("gemm_tn_energy", C.c_int32), ("gemm_tn_grad", C.c_int32), ("pipeline_groups", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
 
 
 class LocusStats(C.Structure):

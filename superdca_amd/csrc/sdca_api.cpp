@@ -121,8 +121,13 @@ void sdca_ctx::free_workspace()
     dfree(d_rows);
     dfree(d_src_off);
     dfree(d_score_loci);
+    if (h_cmds) (void)hipHostFree(h_cmds);
+    if (h_res) (void)hipHostFree(h_res);
+    h_cmds = nullptr;
+    h_res = nullptr;
     ws_slots = 0;
     ws_m = 0;
+    ws_groups = 0;
 }
 
 sdca_ctx::~sdca_ctx()
@@ -143,6 +148,11 @@ sdca_ctx::~sdca_ctx()
     delete mlp;
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &g : gev)
+        for (auto &e : g)
+            if (e) (void)hipEventDestroy(e);
+    for (int g = 1; g < sdca::kMaxGroups; ++g)
+        if (gstream[g]) (void)hipStreamDestroy(gstream[g]);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -238,9 +248,9 @@ GemmConfig default_config(const sdca_ctx *c, int gm, int tn_e = 128)
     GemmConfig g;
     g.gm_e = g.gm_g = gm;
     g.tn_e = tn_e;
-    const int64_t tiles = (int64_t)(c->Mpad / sdca::kTile) * (c->Bp / sdca::kTile);
+    const int64_t tiles = (int64_t)(c->Mg / sdca::kTile) * (c->Bp / sdca::kTile);
     int64_t ks = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
-    const int64_t sup = sdca::kern::energy_superiles(c->Mpad, c->Bp, gm, tn_e);
+    const int64_t sup = sdca::kern::energy_superiles(c->Mg, c->Bp, gm, tn_e);
     while ((ks * sup) % 8 != 0 && ks < 64) ++ks;
     set_split(c, ks, g);
     return g;
@@ -251,7 +261,7 @@ GemmConfig default_config(const sdca_ctx *c, int gm, int tn_e = 128)
 std::vector<GemmConfig> tuning_candidates(const sdca_ctx *c)
 {
     std::vector<GemmConfig> v;
-    const int Mt = c->Mpad / sdca::kTile;
+    const int Mt = c->Mg / sdca::kTile;
     for (int tn : {128, 256})
         for (int gm = 1; gm <= 8; gm *= 2) {
             if (gm > 1 && gm / 2 >= Mt) break;  // taller than the matrix: same as the previous one
@@ -269,8 +279,9 @@ std::vector<GemmConfig> tuning_candidates(const sdca_ctx *c)
 
 float elapsed(hipEvent_t a, hipEvent_t b);
 
-// Times the candidates on the workspace's own buffers with synthetic slot commands
-// (every slot active, writing ring vector 0) and leaves the buffers zeroed again.
+// Times the candidates on the workspace's own buffers (the first slot group, nslots
+// slots) with synthetic slot commands (every slot active, writing ring vector 0) and
+// leaves the buffers zeroed again.
 int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand, GemmConfig &best)
 {
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
@@ -290,13 +301,13 @@ int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand
     }
     const int reps = 3;
     auto energy = [&](const GemmConfig &g) {
-        sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_X, c->Mpad, c->Bp,
+        sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_X, c->Mg, c->Bp,
                                 (int)c->Kxp, g.ksplit, g.kchunk, g.gm_e, g.tn_e, c->d_Epart);
-        sdca::kern::softmax(st, c->opts.precision, c->d_Epart, g.ksplit, c->Mpad, c->Bp, c->B, c->q, nslots,
+        sdca::kern::softmax(st, c->opts.precision, c->d_Epart, g.ksplit, c->Mg, c->Bp, c->B, c->q, nslots,
                             c->d_cmds, c->d_statesT, c->d_w, c->d_G32, c->d_Ghi, c->d_Glo, c->d_sm_part, c->nbchunk);
     };
     auto grad = [&](int gm, int tn) {
-        sdca::kern::gemm_grad(st, c->opts.precision, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Xt, c->Mpad, c->Bp,
+        sdca::kern::gemm_grad(st, c->opts.precision, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Xt, c->Mg, c->Bp,
                               (int)c->Kxp, (int)c->Kx, c->q, nslots, c->d_cmds, c->d_Xtr, c->d_ring, c->n_slot, P,
                               (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), gm, tn);
     };
@@ -326,7 +337,7 @@ int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand
     }
     energy(best);  // node beliefs G for the gradient candidates
     bt = 1e30f;
-    const int Mt = c->Mpad / sdca::kTile;
+    const int Mt = c->Mg / sdca::kTile;
     for (int tn : {128, 256})
         for (int gm = 1; gm <= 8; gm *= 2) {
             if (gm > 1 && gm / 2 >= Mt) break;
@@ -339,25 +350,41 @@ int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand
             }
         }
     // the solve starts from zeroed buffers
-    HIP_TRY(hipMemsetAsync(c->d_ring, 0, (size_t)nslots * P * c->n_slot * sizeof(float), st));
+    const size_t all = (size_t)c->ngroups * nslots;
+    HIP_TRY(hipMemsetAsync(c->d_ring, 0, all * P * c->n_slot * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(c->d_Xtr, 0, nw * sizeof(float), st));
     if (split) {
         HIP_TRY(hipMemsetAsync(c->d_Whi, 0, nw * sizeof(uint16_t), st));
         HIP_TRY(hipMemsetAsync(c->d_Wlo, 0, nw * sizeof(uint16_t), st));
     }
-    HIP_TRY(hipMemsetAsync(c->d_cmds, 0, nslots * sizeof(sdca::SlotCmd), st));
+    HIP_TRY(hipMemsetAsync(c->d_cmds, 0, all * sizeof(sdca::SlotCmd), st));
     HIP_TRY(hipStreamSynchronize(st));
     return SDCA_OK;
+}
+
+// Slot groups: two groups whose rounds run on separate streams when the slots split
+// evenly into halves of whole 128-row GEMM tiles (the compute-bound contractions of one
+// group then overlap the memory-bound vector passes of the other), else one.
+int choose_groups(const sdca_ctx *c, int nslots)
+{
+    const int want = c->opts.pipeline_groups;
+    if (want == 1) return 1;
+    const bool even = nslots % 2 == 0 && ((int64_t)(nslots / 2) * c->q) % sdca::kTile == 0;
+    return even ? 2 : 1;
 }
 
 // Allocates the batch workspace for nslots concurrent loci and history m.
 int ensure_workspace(sdca_ctx *c, int nslots, int m)
 {
-    if (c->ws_slots == nslots && c->ws_m == m) return SDCA_OK;
+    const int ng = choose_groups(c, nslots);
+    if (c->ws_slots == nslots && c->ws_m == m && c->ws_groups == ng) return SDCA_OK;
     c->free_workspace();
     const int P = 2 * m + 3;
     c->n_slot = (int64_t)c->q * c->Kxp;
-    c->Mpad = (int)round_up((int64_t)nslots * c->q, sdca::kTile);
+    c->ngroups = ng;
+    c->gslots = nslots / ng;
+    c->Mg = (int)round_up((int64_t)c->gslots * c->q, sdca::kTile);
+    c->Mpad = ng * c->Mg;
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
     int rc;
     if ((rc = dalloc(&c->d_ring, (size_t)nslots * P * c->n_slot))) return rc;
@@ -372,7 +399,7 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
         if ((rc = dalloc(&c->d_G32, (size_t)c->Mpad * c->Bp))) return rc;
     }
     // GEMM configuration: the cached/tuned one for this shape, else the default rule
-    const GemmKey key{c->opts.precision, c->Mpad, c->Bp, c->Kxp};
+    const GemmKey key{c->opts.precision, c->Mg, c->Bp, c->Kxp};
     const sdca_opts &o = c->opts;
     const bool forced = o.gemm_ksplit > 0 || o.gemm_gm_energy > 0 || o.gemm_gm_grad > 0 || o.gemm_tn_energy > 0 ||
                         o.gemm_tn_grad > 0;
@@ -393,7 +420,7 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
         cand = tuning_candidates(c);
         for (const auto &k : cand) max_ksplit = std::max(max_ksplit, k.ksplit);
     }
-    if ((rc = dalloc(&c->d_Epart, (size_t)max_ksplit * c->Mpad * c->Bp))) return rc;
+    if ((rc = dalloc(&c->d_Epart, (size_t)max_ksplit * c->Mpad * c->Bp))) return rc;  // per group: ksplit x Mg x Bp
     c->nchunk = (int)std::min<int64_t>(64, std::max<int64_t>(1, (c->n_slot / 4 + 255) / 256));
     c->nbchunk = (c->Bp + 255) / 256;
     if ((rc = dalloc(&c->d_upd_part, (size_t)nslots * c->nchunk * 3))) return rc;
@@ -404,10 +431,13 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     if ((rc = dalloc(&c->d_rows, (size_t)nslots * c->L))) return rc;
     if ((rc = dalloc(&c->d_src_off, (size_t)nslots))) return rc;
     if ((rc = dalloc(&c->d_score_loci, (size_t)nslots))) return rc;
+    HIP_TRY(hipHostMalloc((void **)&c->h_cmds, (size_t)nslots * sizeof(sdca::SlotCmd), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_res, (size_t)nslots * sdca::ResLayout{P}.width() * sizeof(double),
+                          hipHostMallocDefault));
     // the zero fills above must be complete before any kernel of the solve runs
     HIP_TRY(hipDeviceSynchronize());
     c->tuned = 0;
-    if (tune && (rc = autotune(c, nslots, P, cand, cfg))) return rc;
+    if (tune && (rc = autotune(c, c->gslots, P, cand, cfg))) return rc;
     c->ksplit = cfg.ksplit;
     c->kchunk = cfg.kchunk;
     c->gm_energy = cfg.gm_e;
@@ -418,6 +448,7 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     if (tune) store_config(key, cfg);
     c->ws_slots = nslots;
     c->ws_m = m;
+    c->ws_groups = ng;
     return SDCA_OK;
 }
 
@@ -428,56 +459,79 @@ float elapsed(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
-// One batched evaluation round: trial points, energies, softmax, gradient, dots.
-// Results (ResLayout rows) land in res (host).
-int run_round(sdca_ctx *c, int nslots, const std::vector<sdca::SlotCmd> &cmds, std::vector<double> &res)
+// One batched evaluation round of slot group g (slots [g*gslots, (g+1)*gslots)) on the
+// group's stream: trial points, energies, softmax, gradient, dots, and the copy of the
+// result rows (ResLayout) to c->h_res.  cmds points at the group's commands.
+int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
+{
+    const int m = c->ws_m, P = 2 * m + 3, W = sdca::ResLayout{P}.width();
+    const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
+    const int ns = c->gslots, s0 = g * ns;
+    const size_t vo = (size_t)s0 * c->n_slot;        // slot-vector offset of the group
+    const size_t go = (size_t)s0 * c->q * c->Bp;     // row offset of its node beliefs
+    float *ring = c->d_ring + (size_t)s0 * P * c->n_slot;
+    sdca::SlotCmd *dc = c->d_cmds + s0;
+    float *Epart = c->d_Epart + (size_t)g * c->ksplit * c->Mg * c->Bp;
+    hipStream_t st = c->gstream[g];
+    hipEvent_t *ev = c->gev[g];
+    std::memcpy(c->h_cmds + s0, cmds, ns * sizeof(sdca::SlotCmd));
+    HIP_TRY(hipMemcpyAsync(dc, c->h_cmds + s0, ns * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(ev[0], st));
+    sdca::kern::update(st, m, c->q, dc, ns, ring, c->d_Xc + vo, c->d_Xtr + vo, split ? c->d_Whi + vo : nullptr,
+                       split ? c->d_Wlo + vo : nullptr, c->d_X0 ? c->d_X0 + vo : nullptr, c->n_slot, (int)c->Kxp,
+                       (int)c->Kx, c->d_upd_part + (size_t)s0 * c->nchunk * 3, c->nchunk);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev[1], st));
+    sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr + vo, split ? c->d_Whi + vo : nullptr,
+                            split ? c->d_Wlo + vo : nullptr, c->d_X, c->Mg, c->Bp, (int)c->Kxp, c->ksplit, c->kchunk,
+                            c->gm_energy, c->tn_energy, Epart);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev[2], st));
+    sdca::kern::softmax(st, c->opts.precision, Epart, c->ksplit, c->Mg, c->Bp, c->B, c->q, ns, dc, c->d_statesT,
+                        c->d_w, split ? nullptr : c->d_G32 + go, split ? c->d_Ghi + go : nullptr,
+                        split ? c->d_Glo + go : nullptr, c->d_sm_part + (size_t)s0 * c->nbchunk, c->nbchunk);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev[3], st));
+    sdca::kern::gemm_grad(st, c->opts.precision, split ? nullptr : c->d_G32 + go, split ? c->d_Ghi + go : nullptr,
+                          split ? c->d_Glo + go : nullptr, c->d_Xt, c->Mg, c->Bp, (int)c->Kxp, (int)c->Kx, c->q, ns,
+                          dc, c->d_Xtr + vo, ring, c->n_slot, P, (float)(2.0 * c->lambda_J),
+                          (float)(2.0 * c->lambda_h), c->gm_grad, c->tn_grad);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev[4], st));
+    double *dot_part = c->d_dot_part + (size_t)s0 * c->nchunk * (P + 1);
+    sdca::kern::dots(st, m, dc, ns, ring, c->n_slot, dot_part, c->nchunk);
+    HIP_TRY(hipGetLastError());
+    sdca::kern::reduce_results(st, m, ns, c->d_upd_part + (size_t)s0 * c->nchunk * 3, dot_part,
+                               c->d_sm_part + (size_t)s0 * c->nbchunk, c->nchunk, c->nbchunk,
+                               c->d_res + (size_t)s0 * W);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev[5], st));
+    HIP_TRY(hipMemcpyAsync(c->h_res + (size_t)s0 * W, c->d_res + (size_t)s0 * W, (size_t)ns * W * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    return SDCA_OK;
+}
+
+// Waits for group g's round and accounts its kernel times and algorithmic traffic.
+int finish_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
 {
     const int m = c->ws_m, P = 2 * m + 3;
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
-    hipStream_t st = c->stream;
-    HIP_TRY(hipMemcpyAsync(c->d_cmds, cmds.data(), nslots * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipEventRecord(c->ev[0], st));
-    sdca::kern::update(st, m, c->q, c->d_cmds, nslots, c->d_ring, c->d_Xc, c->d_Xtr,
-                       split ? c->d_Whi : nullptr, split ? c->d_Wlo : nullptr, c->d_X0, c->n_slot, (int)c->Kxp,
-                       (int)c->Kx, c->d_upd_part, c->nchunk);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[1], st));
-    sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_X, c->Mpad, c->Bp, (int)c->Kxp,
-                            c->ksplit, c->kchunk, c->gm_energy, c->tn_energy, c->d_Epart);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[2], st));
-    sdca::kern::softmax(st, c->opts.precision, c->d_Epart, c->ksplit, c->Mpad, c->Bp, c->B, c->q, nslots, c->d_cmds,
-                        c->d_statesT, c->d_w, c->d_G32, c->d_Ghi, c->d_Glo, c->d_sm_part, c->nbchunk);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[3], st));
-    sdca::kern::gemm_grad(st, c->opts.precision, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Xt, c->Mpad, c->Bp, (int)c->Kxp,
-                          (int)c->Kx, c->q, nslots, c->d_cmds, c->d_Xtr, c->d_ring, c->n_slot, P,
-                          (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), c->gm_grad, c->tn_grad);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[4], st));
-    sdca::kern::dots(st, m, c->d_cmds, nslots, c->d_ring, c->n_slot, c->d_dot_part, c->nchunk);
-    HIP_TRY(hipGetLastError());
-    sdca::kern::reduce_results(st, m, nslots, c->d_upd_part, c->d_dot_part, c->d_sm_part, c->nchunk, c->nbchunk,
-                               c->d_res);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(c->ev[5], st));
-    res.resize((size_t)nslots * sdca::ResLayout{P}.width());
-    HIP_TRY(hipMemcpyAsync(res.data(), c->d_res, res.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipStreamSynchronize(c->gstream[g]));
+    hipEvent_t *ev = c->gev[g];
     auto &t = c->timing;
-    const double upd_ms = elapsed(c->ev[0], c->ev[1]), dot_ms = elapsed(c->ev[4], c->ev[5]);
+    const double upd_ms = elapsed(ev[0], ev[1]), dot_ms = elapsed(ev[4], ev[5]);
     t.lbfgs += upd_ms + dot_ms;
     t.update += upd_ms;
     t.dots += dot_ms;
-    t.energy += elapsed(c->ev[1], c->ev[2]);
-    t.softmax += elapsed(c->ev[2], c->ev[3]);
-    t.grad += elapsed(c->ev[3], c->ev[4]);
+    t.energy += elapsed(ev[1], ev[2]);
+    t.softmax += elapsed(ev[2], ev[3]);
+    t.grad += elapsed(ev[3], ev[4]);
     t.evals += 1;
     t.energy_launches += 1;
     t.grad_launches += 1;
     int active = 0;
     double vu = 0.0, vd = 0.0;  // vector passes of the update and dot-product kernels
-    for (int i = 0; i < nslots; ++i) {
+    for (int i = 0; i < c->gslots; ++i) {
         const int md = cmds[i].mode;
         active += md != sdca::MODE_IDLE;
         if (md == sdca::MODE_INIT) vu += 2 + (cmds[i].init_src != 0.f);
@@ -936,6 +990,8 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
         return fail(SDCA_E_ARG, "gemm_ksplit must be in [0, 4096] and gemm_gm_* in [0, 64]");
     for (int tn : {o.gemm_tn_energy, o.gemm_tn_grad})
         if (tn != 0 && tn != 128 && tn != 256) return fail(SDCA_E_ARG, "gemm_tn_* must be 0, 128 or 256");
+    if (o.pipeline_groups < 0 || o.pipeline_groups > sdca::kMaxGroups)
+        return fail(SDCA_E_ARG, "pipeline_groups must be 0, 1 or 2");
     if (o.precision != SDCA_PREC_FP32 && o.precision != SDCA_PREC_SPLIT_BF16)
         return fail(SDCA_E_ARG, "unknown precision %d", o.precision);
     if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 10, 16 (got %d)", o.history);
@@ -989,6 +1045,13 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
         return bail(fail(SDCA_E_HIP, "hipStreamCreate failed"));
     for (auto &e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return bail(fail(SDCA_E_HIP, "hipEventCreate failed"));
+    // one blocking stream per slot group (default priority); group 0 uses c->stream
+    c->gstream[0] = c->stream;
+    for (int g = 1; g < sdca::kMaxGroups; ++g)
+        if (hipStreamCreate(&c->gstream[g]) != hipSuccess) return bail(fail(SDCA_E_HIP, "hipStreamCreate failed"));
+    for (auto &g : c->gev)
+        for (auto &e : g)
+            if (hipEventCreate(&e) != hipSuccess) return bail(fail(SDCA_E_HIP, "hipEventCreate failed"));
     std::vector<uint8_t> sT((size_t)B * L);
     for (int b = 0; b < B; ++b)
         for (int j = 0; j < L; ++j) sT[(size_t)j * B + b] = states[(size_t)b * L + j];
@@ -1117,7 +1180,7 @@ int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x
     const int m = c->ws_m, P = 2 * m + 3;
     const size_t n = (size_t)c->L * c->q * c->q + c->q;
     std::vector<float> stage((size_t)nslots * c->n_slot), gdev(c->n_slot);
-    std::vector<double> unobs(nslots), res;
+    std::vector<double> unobs(nslots);
     std::vector<sdca::SlotCmd> cmds(nslots, idle_cmd());
     for (int k0 = 0; k0 < K; k0 += nslots) {
         const int cnt = std::min(nslots, K - k0);
@@ -1134,11 +1197,14 @@ int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x
             cmds[i].init_src = 1.f;
         }
         HIP_TRY(hipMemcpy(c->d_X0, stage.data(), (size_t)cnt * c->n_slot * 4, hipMemcpyHostToDevice));
-        if ((rc = run_round(c, nslots, cmds, res))) return rc;
+        for (int g = 0; g < c->ngroups; ++g)
+            if ((rc = launch_round(c, g, cmds.data() + (size_t)g * c->gslots))) return rc;
+        for (int g = 0; g < c->ngroups; ++g)
+            if ((rc = finish_round(c, g, cmds.data() + (size_t)g * c->gslots))) return rc;
         const sdca::ResLayout R{P};
         for (int i = 0; i < cnt; ++i) {
             const int r = loci[k0 + i];
-            const double *rr = res.data() + (size_t)i * R.width();
+            const double *rr = c->h_res + (size_t)i * R.width();
             f_out[k0 + i] = rr[R.fdata()] + c->lambda_J * rr[R.xsqJ()] + c->lambda_h * rr[R.xsqh()] +
                             c->lambda_J * unobs[i];
             HIP_TRY(hipMemcpy(gdev.data(), c->d_ring + ((size_t)i * P + (m + 1)) * c->n_slot, c->n_slot * 4,
@@ -1169,7 +1235,7 @@ int sdca_solve_loci(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
 }  // extern "C"
 
 int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *warm, const sdca::CompactWarm *cw,
-                    float *score_rows, double *x_out, sdca_locus_stats *st)
+                    float *score_rows, double *x_out, sdca_locus_stats *st_out)
 {
     int rc = check_ctx(c);
     if (rc) return rc;
@@ -1186,7 +1252,7 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
         if ((rc = dalloc(&c->d_X0, (size_t)nslots * c->n_slot))) return rc;
         HIP_TRY(hipDeviceSynchronize());
     }
-    int32_t *d_wl = nullptr;  // compact warm starts: slot, item, locus of each refill
+    int32_t *d_wl = nullptr;  // compact warm starts: slot, item, locus of each refill (per group)
     if (cw) {
         if ((rc = dalloc(&d_wl, (size_t)3 * nslots))) return rc;
         HIP_TRY(hipDeviceSynchronize());
@@ -1198,10 +1264,10 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
     const int m = c->ws_m, P = 2 * m + 3;
     const size_t n = (size_t)c->L * c->q * c->q + c->q;
     const sdca::ResLayout R{P};
+    const int ng = c->ngroups, gs = c->gslots;
     Solver S(c, nslots);
-    std::vector<float> stage(warm ? c->n_slot : 0), xdev(x_out ? c->n_slot : 0), rowbuf;
-    std::vector<double> res;
-    std::vector<int32_t> wl_slot, wl_item, wl_locus;
+    std::vector<float> stage(warm ? c->n_slot : 0), xdev(x_out ? c->n_slot : 0);
+    std::vector<int32_t> wl_slot[sdca::kMaxGroups], wl_item[sdca::kMaxGroups], wl_locus[sdca::kMaxGroups];
     int next = 0, done = 0, noconv = 0, nonfinite = 0;
     auto fill = [&](int i) -> int {
         if (next >= K) {
@@ -1215,101 +1281,140 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
             HIP_TRY(hipMemcpy(c->d_X0 + (size_t)i * c->n_slot, stage.data(), c->n_slot * 4, hipMemcpyHostToDevice));
         }
         if (cw) {
-            wl_slot.push_back(i);
-            wl_item.push_back(item);
-            wl_locus.push_back(r);
+            const int g = i / gs;
+            wl_slot[g].push_back(i);
+            wl_item[g].push_back(item);
+            wl_locus[g].push_back(r);
         }
         S.assign(i, item, r, any_warm, cw && cw->as_direction);
         return SDCA_OK;
     };
-    // stage the compact warm starts of this round's refills into their slots' X0 vectors
-    auto stage_compact = [&]() -> int {
-        if (!cw || wl_slot.empty()) return SDCA_OK;
-        const int nw = (int)wl_slot.size();
-        std::vector<int32_t> packed(wl_slot);
-        packed.insert(packed.end(), wl_item.begin(), wl_item.end());
-        packed.insert(packed.end(), wl_locus.begin(), wl_locus.end());
-        HIP_TRY(hipMemcpyAsync(d_wl, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, c->stream));
+    // stage the compact warm starts of group g's refills into their slots' X0 vectors
+    auto stage_compact = [&](int g) -> int {
+        if (!cw || wl_slot[g].empty()) return SDCA_OK;
+        hipStream_t st = c->gstream[g];
+        const int nw = (int)wl_slot[g].size();
+        std::vector<int32_t> packed(wl_slot[g]);
+        packed.insert(packed.end(), wl_item[g].begin(), wl_item[g].end());
+        packed.insert(packed.end(), wl_locus[g].begin(), wl_locus[g].end());
+        int32_t *dw = d_wl + (size_t)3 * g * gs;
+        HIP_TRY(hipMemcpyAsync(dw, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, st));
         // blocks: into the staged start, or into ring vector m (first search direction)
         float *XJ = cw->as_direction ? c->d_ring + (size_t)m * c->n_slot : c->d_X0;
         const int64_t jstride = cw->as_direction ? (int64_t)P * c->n_slot : c->n_slot;
-        for (int i : wl_slot) {
-            HIP_TRY(hipMemsetAsync(c->d_X0 + (size_t)i * c->n_slot, 0, c->n_slot * 4, c->stream));
-            if (cw->as_direction)
-                HIP_TRY(hipMemsetAsync(XJ + (size_t)i * jstride, 0, c->n_slot * 4, c->stream));
+        for (int i : wl_slot[g]) {
+            HIP_TRY(hipMemsetAsync(c->d_X0 + (size_t)i * c->n_slot, 0, c->n_slot * 4, st));
+            if (cw->as_direction) HIP_TRY(hipMemsetAsync(XJ + (size_t)i * jstride, 0, c->n_slot * 4, st));
         }
-        sdca::kern::scatter_warm(c->stream, d_wl, d_wl + nw, d_wl + 2 * nw, cw->d_partners, cw->d_J, cw->d_h, nw,
-                                 cw->P, c->q, c->Kxp, (int)c->Kx, c->d_state_col, c->n_slot, c->d_X0, XJ, jstride);
+        sdca::kern::scatter_warm(st, dw, dw + nw, dw + 2 * nw, cw->d_partners, cw->d_J, cw->d_h, nw, cw->P, c->q,
+                                 c->Kxp, (int)c->Kx, c->d_state_col, c->n_slot, c->d_X0, XJ, jstride);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipStreamSynchronize(c->stream));  // packed is a host temporary
-        wl_slot.clear();
-        wl_item.clear();
-        wl_locus.clear();
+        HIP_TRY(hipStreamSynchronize(st));  // packed is a host temporary
+        wl_slot[g].clear();
+        wl_item[g].clear();
+        wl_locus[g].clear();
         return SDCA_OK;
+    };
+    auto group_active = [&](int g) {
+        for (int i = g * gs; i < (g + 1) * gs; ++i)
+            if (S.slot(i).phase != PH_IDLE) return true;
+        return false;
     };
     for (int i = 0; i < nslots; ++i)
         if ((rc = fill(i))) return rc;
-    if ((rc = stage_compact())) return rc;
+    bool inflight[sdca::kMaxGroups] = {};
+    for (int g = 0; g < ng; ++g) {
+        if ((rc = stage_compact(g))) return rc;
+        if (group_active(g)) {
+            if ((rc = launch_round(c, g, S.cmds().data() + (size_t)g * gs))) return rc;
+            inflight[g] = true;
+        }
+    }
     std::vector<int> finished;
     std::vector<int64_t> src_off;
     std::vector<int32_t> sloci;
     int64_t rounds = 0;
+    // the groups' rounds alternate: while the host steps one group's line searches, the
+    // other group's round runs on the device
     while (done < K) {
-        if ((rc = run_round(c, nslots, S.cmds(), res))) return rc;
-        ++rounds;
-        finished.clear();
-        for (int i = 0; i < nslots; ++i) {
-            if (S.slot(i).phase == PH_IDLE) continue;
-            if (S.process(i, res.data() + (size_t)i * R.width())) finished.push_back(i);
-        }
-        if (finished.empty()) continue;
-        // score rows of the finished loci, read before their slots are refilled
-        src_off.clear();
-        sloci.clear();
-        for (int i : finished) {
-            const Slot &sl = S.slot(i);
-            src_off.push_back((sl.from_xt ? 0 : (int64_t)c->Mpad * c->Kxp) + (int64_t)i * c->n_slot);
-            sloci.push_back(sl.r);
-        }
-        const int nf = (int)finished.size();
-        HIP_TRY(hipMemcpyAsync(c->d_src_off, src_off.data(), nf * 8, hipMemcpyHostToDevice, c->stream));
-        HIP_TRY(hipMemcpyAsync(c->d_score_loci, sloci.data(), nf * 4, hipMemcpyHostToDevice, c->stream));
-        HIP_TRY(hipEventRecord(c->ev[6], c->stream));
-        sdca::kern::score(c->stream, c->d_Xtr, c->d_src_off, c->d_score_loci, nf, c->L, c->q, c->gap, (int)c->Kxp,
-                          c->d_state_col, c->d_rows);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(c->ev[7], c->stream));
-        if (score_rows) {
-            for (int u = 0; u < nf; ++u)
-                HIP_TRY(hipMemcpyAsync(score_rows + (size_t)S.slot(finished[u]).item * c->L, c->d_rows + (size_t)u * c->L,
-                                       (size_t)c->L * 4, hipMemcpyDeviceToHost, c->stream));
-        }
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        c->timing.score += elapsed(c->ev[6], c->ev[7]);
-        for (int u = 0; u < nf; ++u) {
-            const int i = finished[u];
-            const Slot &sl = S.slot(i);
-            if (x_out) {
-                HIP_TRY(hipMemcpy(xdev.data(), c->d_Xtr + src_off[u], c->n_slot * 4, hipMemcpyDeviceToHost));
-                dev_to_ref(c, sl.r, xdev.data(), x_out + (size_t)sl.item * n);
+        bool any = false;
+        for (int g = 0; g < ng; ++g) {
+            if (!inflight[g]) continue;
+            any = true;
+            inflight[g] = false;
+            if ((rc = finish_round(c, g, S.cmds().data() + (size_t)g * gs))) return rc;
+            ++rounds;
+            finished.clear();
+            for (int i = g * gs; i < (g + 1) * gs; ++i) {
+                if (S.slot(i).phase == PH_IDLE) continue;
+                if (S.process(i, c->h_res + (size_t)i * R.width())) finished.push_back(i);
             }
-            if (st) {
-                sdca_locus_stats &o = st[sl.item];
-                o.fval = sl.f;
-                o.gnorm = sl.gnorm;
-                o.nfeval = sl.nfeval;
-                o.iter = sl.iter;
-                o.status = sl.status;
-                o.reserved = 0;
+            if (!finished.empty()) {
+                // score rows of the finished loci, read before their slots are refilled
+                hipStream_t st = c->gstream[g];
+                hipEvent_t *ev = c->gev[g];
+                src_off.clear();
+                sloci.clear();
+                for (int i : finished) {
+                    const Slot &sl = S.slot(i);
+                    src_off.push_back((sl.from_xt ? 0 : (int64_t)c->Mpad * c->Kxp) + (int64_t)i * c->n_slot);
+                    sloci.push_back(sl.r);
+                }
+                const int nf = (int)finished.size();
+                int64_t *d_off = c->d_src_off + (size_t)g * gs;
+                int32_t *d_sl = c->d_score_loci + (size_t)g * gs;
+                float *d_rows = c->d_rows + (size_t)g * gs * c->L;
+                HIP_TRY(hipMemcpyAsync(d_off, src_off.data(), nf * 8, hipMemcpyHostToDevice, st));
+                HIP_TRY(hipMemcpyAsync(d_sl, sloci.data(), nf * 4, hipMemcpyHostToDevice, st));
+                HIP_TRY(hipEventRecord(ev[6], st));
+                sdca::kern::score(st, c->d_Xtr, d_off, d_sl, nf, c->L, c->q, c->gap, (int)c->Kxp, c->d_state_col,
+                                  d_rows);
+                HIP_TRY(hipGetLastError());
+                HIP_TRY(hipEventRecord(ev[7], st));
+                if (score_rows) {
+                    for (int u = 0; u < nf; ++u)
+                        HIP_TRY(hipMemcpyAsync(score_rows + (size_t)S.slot(finished[u]).item * c->L,
+                                               d_rows + (size_t)u * c->L, (size_t)c->L * 4, hipMemcpyDeviceToHost,
+                                               st));
+                }
+                if (x_out) {
+                    for (int u = 0; u < nf; ++u) {
+                        const Slot &sl = S.slot(finished[u]);
+                        HIP_TRY(hipMemcpyAsync(xdev.data(), c->d_Xtr + src_off[u], c->n_slot * 4,
+                                               hipMemcpyDeviceToHost, st));
+                        HIP_TRY(hipStreamSynchronize(st));
+                        dev_to_ref(c, sl.r, xdev.data(), x_out + (size_t)sl.item * n);
+                    }
+                }
+                HIP_TRY(hipStreamSynchronize(st));
+                c->timing.score += elapsed(ev[6], ev[7]);
+                for (int u = 0; u < nf; ++u) {
+                    const int i = finished[u];
+                    const Slot &sl = S.slot(i);
+                    if (st_out) {
+                        sdca_locus_stats &o = st_out[sl.item];
+                        o.fval = sl.f;
+                        o.gnorm = sl.gnorm;
+                        o.nfeval = sl.nfeval;
+                        o.iter = sl.iter;
+                        o.status = sl.status;
+                        o.reserved = 0;
+                    }
+                    noconv += (sl.status == SDCA_ST_ITER_CAP || sl.status == SDCA_ST_LS_FAIL);
+                    nonfinite += (sl.status == SDCA_ST_NONFINITE);
+                    ++done;
+                    if ((rc = fill(i))) return rc;
+                }
+                if ((rc = stage_compact(g))) return rc;
+                if (c->opts.verbose)
+                    fprintf(stderr, "sdca: %d / %d loci solved after %lld rounds\n", done, K, (long long)rounds);
             }
-            noconv += (sl.status == SDCA_ST_ITER_CAP || sl.status == SDCA_ST_LS_FAIL);
-            nonfinite += (sl.status == SDCA_ST_NONFINITE);
-            ++done;
-            if ((rc = fill(i))) return rc;
+            if (group_active(g)) {
+                if ((rc = launch_round(c, g, S.cmds().data() + (size_t)g * gs))) return rc;
+                inflight[g] = true;
+            }
         }
-        if ((rc = stage_compact())) return rc;
-        if (c->opts.verbose)
-            fprintf(stderr, "sdca: %d / %d loci solved after %lld rounds\n", done, K, (long long)rounds);
+        if (!any) break;  // every group idle
     }
     c->timing.total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (nonfinite) return fail(SDCA_E_NONFINITE, "%d loci reached a non-finite objective", nonfinite);

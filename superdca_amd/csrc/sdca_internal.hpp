@@ -27,6 +27,7 @@ constexpr int kTile = 128;        // GEMM output tile rows (and narrow tile colu
 constexpr int kTileWide = 256;    // wide GEMM tile columns; B and K_x are padded to this
 constexpr int kMaxHistory = 16;   // largest L-BFGS history supported
 constexpr int kMaxPhys = 2 * kMaxHistory + 3;  // D ring (m+1) + G ring (m+2)
+constexpr int kMaxGroups = 2;     // slot groups whose rounds are pipelined on separate streams
 
 // Per-slot command of one batched evaluation round (host -> device).
 enum SlotMode : int32_t { MODE_IDLE = 0, MODE_INIT = 1, MODE_RETRY = 2, MODE_NEWDIR = 3 };
@@ -170,6 +171,8 @@ struct sdca_ctx {
     int ws_slots = 0, ws_m = 0;
     int64_t n_slot = 0;
     int Mpad = 0;
+    int ngroups = 1, gslots = 0, Mg = 0;  // slot groups: slots and padded GEMM rows per group
+    int ws_groups = 0;
     int ksplit = 1, kchunk = 0;
     int gm_energy = 8, gm_grad = 8;  // XCD super-tile heights (autotuned)
     int tn_energy = 128, tn_grad = 128;  // output tile widths (autotuned)
@@ -182,6 +185,8 @@ struct sdca_ctx {
     float *d_Epart = nullptr;
     double *d_upd_part = nullptr, *d_dot_part = nullptr, *d_sm_part = nullptr, *d_res = nullptr;
     sdca::SlotCmd *d_cmds = nullptr;
+    sdca::SlotCmd *h_cmds = nullptr;  // pinned host staging of the slot commands
+    double *h_res = nullptr;          // pinned host result rows
     float *d_rows = nullptr;
     int64_t *d_src_off = nullptr;
     int32_t *d_score_loci = nullptr;
@@ -200,6 +205,8 @@ struct sdca_ctx {
     sdca::Mlp *mlp = nullptr;
 
     hipEvent_t ev[8] = {};
+    hipStream_t gstream[sdca::kMaxGroups] = {};  // gstream[0] is stream
+    hipEvent_t gev[sdca::kMaxGroups][8] = {};
     sdca::Timing timing;
 
     ~sdca_ctx();

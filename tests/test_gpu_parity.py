@@ -232,3 +232,24 @@ def test_gemm_configurations_match_oracle(c1, cfg, precision, ftol, gtol):
         assert f[k] == pytest.approx(fo, rel=ftol), (k, got)
         err = np.abs(g[k] - go).max() / np.abs(go).max()
         assert err < gtol, (k, err, got)
+
+
+@pytest.mark.parametrize("precision", [sd.PREC_FP32, sd.PREC_SPLIT_BF16])
+def test_pipelined_slot_groups_match_single_group(c1, precision):
+    # 80 loci through 64 slots: two groups of 32 slots (128 GEMM rows each) alternate on two
+    # streams, with refills in both; every locus must reach the same optimum as with one group
+    st = c1["states"]
+    w, _ = orc.weights(st, 0.9, nthreads=8)
+    loci = np.arange(0, 1000, 12)[:80]
+    out = {}
+    for groups in (1, 2):
+        with sd.Context(st, precision=precision, grad_threshold=1e-4, batch_loci=64, pipeline_groups=groups) as ctx:
+            ctx.set_weights(w)
+            rows, stats, _, rc = ctx.solve(loci)
+        assert rc == sd.SDCA_OK, groups
+        assert np.all(stats["gnorm"] < 1e-4)
+        out[groups] = (rows, stats)
+    np.testing.assert_allclose(out[2][1]["fval"], out[1][1]["fval"], rtol=1e-6)
+    big = out[1][0] > np.quantile(out[1][0], 0.9)
+    rel = np.abs(out[2][0] - out[1][0])[big] / out[1][0][big]
+    assert rel.max() < (2e-3 if precision == sd.PREC_SPLIT_BF16 else 1e-3), rel.max()
