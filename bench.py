@@ -117,6 +117,7 @@ def main() -> int:
     ap.add_argument("--loci-per-step", type=int, default=0, help="default: 2048 at C4, 1024 otherwise")
     ap.add_argument("--slots", type=int, default=0, help="loci in flight (default 256 at C4, 128 otherwise)")
     ap.add_argument("--history", type=int, default=10)
+    ap.add_argument("--groups", type=int, default=0, help="slot groups pipelined on two streams (0: automatic)")
     ap.add_argument("--partners", type=int, default=2048, help="warm-started partners per locus")
     ap.add_argument("--warm-mode", default="direction", choices=list(WARM_MODES),
                     help="direction: network blocks as the first search direction from zero; direction-fields: "
@@ -159,7 +160,7 @@ def main() -> int:
             f"{net[1][-1]:.3e}")
 
     ctx = sd.Context(states, q=4, precision=sd.PREC_SPLIT_BF16, batch_loci=slots, history=args.history,
-                     grad_threshold=1e-3)
+                     grad_threshold=1e-3, pipeline_groups=args.groups)
     t = time.time()
     w, _ = ctx.reweight(0.9)
     reweight_s = time.time() - t
@@ -195,7 +196,7 @@ def main() -> int:
     fp32 = None
     if args.fp32_loci > 0:
         ctx32 = sd.Context(states, q=4, precision=sd.PREC_FP32, batch_loci=slots, history=args.history,
-                           grad_threshold=1e-3)
+                           grad_threshold=1e-3, pipeline_groups=args.groups)
         ctx32.set_weights(w)
         ctx32.solve(order[pos:pos + slots // 4])  # allocation and first launches
         pos += slots // 4

@@ -103,7 +103,8 @@ bool valid_history(int m) { return m == 1 || m == 3 || m == 5 || m == 10 || m ==
 
 void sdca_ctx::free_workspace()
 {
-    dfree(d_ring);
+    dfree(d_gring);
+    dfree(d_dring);
     dfree(d_Xtr);
     d_Xc = nullptr;  // second half of the d_Xtr allocation
     dfree(d_X0);
@@ -280,7 +281,7 @@ std::vector<GemmConfig> tuning_candidates(const sdca_ctx *c)
 float elapsed(hipEvent_t a, hipEvent_t b);
 
 // Times the candidates on the workspace's own buffers (the first slot group, nslots
-// slots) with synthetic slot commands (every slot active, writing ring vector 0) and
+// slots) with synthetic slot commands (every slot active, writing its first gradient) and
 // leaves the buffers zeroed again.
 int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand, GemmConfig &best)
 {
@@ -291,6 +292,7 @@ int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand
         std::memset(&cmds[i], 0, sizeof(cmds[i]));
         cmds[i].mode = sdca::MODE_INIT;
         cmds[i].locus = i % c->L;
+        cmds[i].gt = c->ws_m + 1;  // the first gradient vector
     }
     HIP_TRY(hipMemcpyAsync(c->d_cmds, cmds.data(), nslots * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     const size_t nw = (size_t)c->Mpad * c->Kxp;
@@ -308,7 +310,7 @@ int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand
     };
     auto grad = [&](int gm, int tn) {
         sdca::kern::gemm_grad(st, c->opts.precision, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Xt, c->Mg, c->Bp,
-                              (int)c->Kxp, (int)c->Kx, c->q, nslots, c->d_cmds, c->d_Xtr, c->d_ring, c->n_slot, P,
+                              (int)c->Kxp, (int)c->Kx, c->q, nslots, c->d_cmds, c->d_Xtr, c->d_gring, c->n_slot, c->ws_m,
                               (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), gm, tn);
     };
     auto time = [&](auto &&launch) -> float {
@@ -351,7 +353,7 @@ int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand
         }
     // the solve starts from zeroed buffers
     const size_t all = (size_t)c->ngroups * nslots;
-    HIP_TRY(hipMemsetAsync(c->d_ring, 0, all * P * c->n_slot * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(c->d_gring, 0, all * (P + 1) / 2 * c->n_slot * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(c->d_Xtr, 0, nw * sizeof(float), st));
     if (split) {
         HIP_TRY(hipMemsetAsync(c->d_Whi, 0, nw * sizeof(uint16_t), st));
@@ -387,7 +389,9 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     c->Mpad = ng * c->Mg;
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
     int rc;
-    if ((rc = dalloc(&c->d_ring, (size_t)nslots * P * c->n_slot))) return rc;
+    c->ws_m = m;  // ring shapes below and the autotuner use it
+    if ((rc = dalloc(&c->d_gring, (size_t)nslots * (m + 2) * c->n_slot))) return rc;
+    if ((rc = dalloc(&c->d_dring, (size_t)nslots * (m + 1) * c->n_slot))) return rc;
     if ((rc = dalloc(&c->d_Xtr, (size_t)2 * c->Mpad * c->Kxp))) return rc;
     c->d_Xc = c->d_Xtr + (size_t)c->Mpad * c->Kxp;
     if (split) {
@@ -423,7 +427,7 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     if ((rc = dalloc(&c->d_Epart, (size_t)max_ksplit * c->Mpad * c->Bp))) return rc;  // per group: ksplit x Mg x Bp
     c->nchunk = (int)std::min<int64_t>(64, std::max<int64_t>(1, (c->n_slot / 4 + 255) / 256));
     c->nbchunk = (c->Bp + 255) / 256;
-    if ((rc = dalloc(&c->d_upd_part, (size_t)nslots * c->nchunk * 3))) return rc;
+    if ((rc = dalloc(&c->d_upd_part, (size_t)nslots * c->nchunk * (2 + m + 2)))) return rc;
     if ((rc = dalloc(&c->d_dot_part, (size_t)nslots * c->nchunk * (P + 1)))) return rc;
     if ((rc = dalloc(&c->d_sm_part, (size_t)nslots * c->nbchunk))) return rc;
     if ((rc = dalloc(&c->d_res, (size_t)nslots * sdca::ResLayout{P}.width()))) return rc;
@@ -469,7 +473,8 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     const int ns = c->gslots, s0 = g * ns;
     const size_t vo = (size_t)s0 * c->n_slot;        // slot-vector offset of the group
     const size_t go = (size_t)s0 * c->q * c->Bp;     // row offset of its node beliefs
-    float *ring = c->d_ring + (size_t)s0 * P * c->n_slot;
+    float *gring = c->d_gring + (size_t)s0 * (m + 2) * c->n_slot;
+    float *dring = c->d_dring + (size_t)s0 * (m + 1) * c->n_slot;
     sdca::SlotCmd *dc = c->d_cmds + s0;
     float *Epart = c->d_Epart + (size_t)g * c->ksplit * c->Mg * c->Bp;
     hipStream_t st = c->gstream[g];
@@ -477,9 +482,10 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     std::memcpy(c->h_cmds + s0, cmds, ns * sizeof(sdca::SlotCmd));
     HIP_TRY(hipMemcpyAsync(dc, c->h_cmds + s0, ns * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(ev[0], st));
-    sdca::kern::update(st, m, c->q, dc, ns, ring, c->d_Xc + vo, c->d_Xtr + vo, split ? c->d_Whi + vo : nullptr,
-                       split ? c->d_Wlo + vo : nullptr, c->d_X0 ? c->d_X0 + vo : nullptr, c->n_slot, (int)c->Kxp,
-                       (int)c->Kx, c->d_upd_part + (size_t)s0 * c->nchunk * 3, c->nchunk);
+    double *upd_part = c->d_upd_part + (size_t)s0 * c->nchunk * (2 + m + 2);
+    sdca::kern::update(st, m, c->q, dc, ns, gring, dring, c->d_Xc + vo, c->d_Xtr + vo,
+                       split ? c->d_Whi + vo : nullptr, split ? c->d_Wlo + vo : nullptr,
+                       c->d_X0 ? c->d_X0 + vo : nullptr, c->n_slot, (int)c->Kxp, (int)c->Kx, upd_part, c->nchunk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[1], st));
     sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr + vo, split ? c->d_Whi + vo : nullptr,
@@ -494,14 +500,14 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     HIP_TRY(hipEventRecord(ev[3], st));
     sdca::kern::gemm_grad(st, c->opts.precision, split ? nullptr : c->d_G32 + go, split ? c->d_Ghi + go : nullptr,
                           split ? c->d_Glo + go : nullptr, c->d_Xt, c->Mg, c->Bp, (int)c->Kxp, (int)c->Kx, c->q, ns,
-                          dc, c->d_Xtr + vo, ring, c->n_slot, P, (float)(2.0 * c->lambda_J),
+                          dc, c->d_Xtr + vo, gring, c->n_slot, m, (float)(2.0 * c->lambda_J),
                           (float)(2.0 * c->lambda_h), c->gm_grad, c->tn_grad);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[4], st));
     double *dot_part = c->d_dot_part + (size_t)s0 * c->nchunk * (P + 1);
-    sdca::kern::dots(st, m, dc, ns, ring, c->n_slot, dot_part, c->nchunk);
+    sdca::kern::dots(st, m, dc, ns, gring, dring, c->n_slot, dot_part, c->nchunk);
     HIP_TRY(hipGetLastError());
-    sdca::kern::reduce_results(st, m, ns, c->d_upd_part + (size_t)s0 * c->nchunk * 3, dot_part,
+    sdca::kern::reduce_results(st, m, ns, upd_part, dot_part,
                                c->d_sm_part + (size_t)s0 * c->nbchunk, c->nchunk, c->nbchunk,
                                c->d_res + (size_t)s0 * W);
     HIP_TRY(hipGetLastError());
@@ -530,25 +536,29 @@ int finish_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     t.energy_launches += 1;
     t.grad_launches += 1;
     int active = 0;
-    double vu = 0.0, vd = 0.0;  // vector passes of the update and dot-product kernels
+    const double db = 4.0;  // bytes per direction element
+    double bu = 0.0, bd = 0.0;  // bytes per element of the update and dot-product passes
     for (int i = 0; i < c->gslots; ++i) {
-        const int md = cmds[i].mode;
-        active += md != sdca::MODE_IDLE;
-        if (md == sdca::MODE_INIT) vu += 2 + (cmds[i].init_src != 0.f);
-        else if (md == sdca::MODE_RETRY) vu += 3;
-        else if (md == sdca::MODE_NEWDIR) {
-            int support = 0;
-            for (int p = 0; p < P; ++p) support += cmds[i].coef[p] != 0.f;
-            vu += support + 4;
+        const sdca::SlotCmd &k = cmds[i];
+        const int md = k.mode;
+        if (md == sdca::MODE_IDLE) continue;
+        ++active;
+        auto live = [&](int p) { return (k.live[p >> 5] >> (p & 31)) & 1u; };
+        if (md == sdca::MODE_INIT) bu += 4.0 * (2 + (k.init_src != 0.f));
+        else if (md == sdca::MODE_RETRY) bu += 8.0 + db;  // x_c and d read, x_t written
+        else {
+            bu += 4.0 * (2 + (k.accept != 0)) + db;  // x read, x_t (and x_c) written, d written
+            for (int p = 0; p <= m; ++p) bu += k.coef[p] != 0.f ? db : 0.0;
+            for (int p = m + 1; p < P; ++p) bu += (k.coef[p] != 0.f || (live(p) && p != k.gt)) ? 4.0 : 0.0;
         }
-        if (md != sdca::MODE_IDLE) {
-            vd += P;
-            if (split) vu += 1;  // hi + lo BF16 copies of x_t
-        }
+        if (split) bu += 4.0;  // hi + lo BF16 copies of x_t
+        bd += 4.0;             // g_t
+        for (int p = 0; p < P; ++p)
+            if (live(p) && p != k.gt) bd += p <= m ? db : 4.0;
     }
-    t.bytes_update += vu * 4.0 * (double)c->n_slot;
-    t.bytes_dots += vd * 4.0 * (double)c->n_slot;
-    t.bytes_lbfgs += (vu + vd) * 4.0 * (double)c->n_slot;
+    t.bytes_update += bu * (double)c->n_slot;
+    t.bytes_dots += bd * (double)c->n_slot;
+    t.bytes_lbfgs += (bu + bd) * (double)c->n_slot;
     t.flops_energy += 2.0 * c->B * (double)c->Kx * c->q * active;
     t.flops_grad += 2.0 * c->B * (double)c->Kx * c->q * active;
     return SDCA_OK;
@@ -679,10 +689,10 @@ public:
         sdca::SlotCmd &k = cmds_[i];
         const sdca::ResLayout R{P_};
         const int pending = k.mode;
-        if (pending == sdca::MODE_NEWDIR) {
-            derive_direction_row(i);
-            G(i, S.dk, S.gk) = G(i, S.gk, S.dk) = res[R.ddg()];
-        }
+        if (pending == sdca::MODE_NEWDIR)  // Gram row of the stored direction (live gradients)
+            for (int p = m_ + 1; p < P_; ++p)
+                if (((k.live[p >> 5] >> (p & 31)) & 1u) && p != k.gt)
+                    G(i, S.dk, p) = G(i, p, S.dk) = res[R.ddots() + (p - m_ - 1)];
         for (int p = 0; p < P_; ++p) G(i, S.gt, p) = G(i, p, S.gt) = res[R.gdots() + p];
         const double f = res[R.fdata()] + c_->lambda_J * res[R.xsqJ()] + c_->lambda_h * res[R.xsqh()];
         const double ginf = res[R.ginf()];
@@ -760,6 +770,30 @@ public:
         return false;
     }
 
+    // Marks the history vectors of slots [i0, i1) that the coming round must read or pair
+    // with: the trial gradient, the current gradient and direction, and every vector of
+    // the stored pairs.
+    void mark_live(int i0, int i1)
+    {
+        for (int i = i0; i < i1; ++i) {
+            const Slot &S = slots_[i];
+            sdca::SlotCmd &k = cmds_[i];
+            k.live[0] = k.live[1] = 0u;
+            if (S.phase == PH_IDLE || k.mode == sdca::MODE_IDLE) continue;
+            auto set = [&](int p) {
+                if (p >= 0) k.live[p >> 5] |= 1u << (p & 31);
+            };
+            set(S.gt);
+            set(S.gk);
+            set(S.dk);
+            for (const auto &pr : S.pairs) {
+                set(pr.d);
+                set(pr.g_old);
+                set(pr.g_new);
+            }
+        }
+    }
+
     const Slot &slot(int i) const { return slots_[i]; }
     const std::vector<sdca::SlotCmd> &cmds() const { return cmds_; }
     std::vector<sdca::SlotCmd> &cmds() { return cmds_; }
@@ -835,25 +869,6 @@ private:
         }
         delta.assign(P_, 0.0);
         for (int p = 0; p < P_; ++p) delta[p] = -qv[p];
-    }
-
-    // Gram entries of a new direction d = sum_p coef_p ring_p (the FP32 coefficients the
-    // device used) with every gradient of the history: d . g = sum_p coef_p (ring_p . g).
-    // Only direction-gradient and gradient-gradient products enter the two-loop recursion
-    // and the line search, and products with gradients computed later come from k_dots.
-    void derive_direction_row(int i)
-    {
-        Slot &S = slots_[i];
-        const sdca::SlotCmd &k = cmds_[i];
-        std::vector<int> gs;
-        for (const auto &pr : S.pairs) gs.push_back(pr.g_old);
-        if (S.gk >= 0) gs.push_back(S.gk);
-        for (int g : gs) {
-            double acc = 0.0;
-            for (int p = 0; p < P_; ++p)
-                if (k.coef[p] != 0.f) acc += (double)k.coef[p] * G(i, p, g);
-            G(i, S.dk, g) = G(i, g, S.dk) = acc;
-        }
     }
 
     // First direction = the preloaded ring vector m (copied to a free direction slot so
@@ -1192,6 +1207,7 @@ int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x
             cmds[i].mode = sdca::MODE_INIT;
             cmds[i].locus = r;
             cmds[i].gt = m + 1;
+            cmds[i].live[(m + 1) >> 5] = 1u << ((m + 1) & 31);
             cmds[i].self_lo = c->col_off[r];
             cmds[i].self_hi = c->col_off[r + 1];
             cmds[i].init_src = 1.f;
@@ -1207,8 +1223,8 @@ int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x
             const double *rr = c->h_res + (size_t)i * R.width();
             f_out[k0 + i] = rr[R.fdata()] + c->lambda_J * rr[R.xsqJ()] + c->lambda_h * rr[R.xsqh()] +
                             c->lambda_J * unobs[i];
-            HIP_TRY(hipMemcpy(gdev.data(), c->d_ring + ((size_t)i * P + (m + 1)) * c->n_slot, c->n_slot * 4,
-                              hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(gdev.data(), c->d_gring + (size_t)i * (m + 2) * c->n_slot, c->n_slot * 4,
+                              hipMemcpyDeviceToHost));  // gradient slot m + 1 is the first of the ring
             double *g = g_out + (size_t)(k0 + i) * n;
             dev_to_ref(c, r, gdev.data(), g);
             // states never observed at j: gradient 2 lambda_J x (plmDCA_objective.hpp:217)
@@ -1300,8 +1316,8 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
         int32_t *dw = d_wl + (size_t)3 * g * gs;
         HIP_TRY(hipMemcpyAsync(dw, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, st));
         // blocks: into the staged start, or into ring vector m (first search direction)
-        float *XJ = cw->as_direction ? c->d_ring + (size_t)m * c->n_slot : c->d_X0;
-        const int64_t jstride = cw->as_direction ? (int64_t)P * c->n_slot : c->n_slot;
+        float *XJ = cw->as_direction ? c->d_dring + (size_t)m * c->n_slot : c->d_X0;
+        const int64_t jstride = cw->as_direction ? (int64_t)(m + 1) * c->n_slot : c->n_slot;
         for (int i : wl_slot[g]) {
             HIP_TRY(hipMemsetAsync(c->d_X0 + (size_t)i * c->n_slot, 0, c->n_slot * 4, st));
             if (cw->as_direction) HIP_TRY(hipMemsetAsync(XJ + (size_t)i * jstride, 0, c->n_slot * 4, st));
@@ -1326,6 +1342,7 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
     for (int g = 0; g < ng; ++g) {
         if ((rc = stage_compact(g))) return rc;
         if (group_active(g)) {
+            S.mark_live(g * gs, (g + 1) * gs);
             if ((rc = launch_round(c, g, S.cmds().data() + (size_t)g * gs))) return rc;
             inflight[g] = true;
         }
@@ -1410,6 +1427,7 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
                     fprintf(stderr, "sdca: %d / %d loci solved after %lld rounds\n", done, K, (long long)rounds);
             }
             if (group_active(g)) {
+                S.mark_live(g * gs, (g + 1) * gs);
                 if ((rc = launch_round(c, g, S.cmds().data() + (size_t)g * gs))) return rc;
                 inflight[g] = true;
             }

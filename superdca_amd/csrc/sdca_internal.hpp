@@ -29,6 +29,16 @@ constexpr int kMaxHistory = 16;   // largest L-BFGS history supported
 constexpr int kMaxPhys = 2 * kMaxHistory + 3;  // D ring (m+1) + G ring (m+2)
 constexpr int kMaxGroups = 2;     // slot groups whose rounds are pipelined on separate streams
 
+// BF16 <-> FP32 (round to nearest even; NaN stays NaN).
+__device__ __forceinline__ uint16_t f2bf(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
 // Per-slot command of one batched evaluation round (host -> device).
 enum SlotMode : int32_t { MODE_IDLE = 0, MODE_INIT = 1, MODE_RETRY = 2, MODE_NEWDIR = 3 };
 
@@ -41,21 +51,24 @@ struct SlotCmd {
     int32_t self_hi;
     int32_t gk;       // NEWDIR: ring index of the gradient at the accepted point
     int32_t accept;   // NEWDIR: 1 = the last trial point becomes the iterate (x_c = x_t) first
+    uint32_t live[2]; // bit p: history vector p is referenced (only these are read for products)
     float alpha;      // step length of the trial point x_t = x_c + alpha * d
     float init_src;   // MODE_INIT: 1 = copy the staged start point, 0 = zero start
     float coef[kMaxPhys];  // NEWDIR: d = sum_p coef[p] * ring[p]
 };
 
+// History vectors of a slot, indexed p in [0, P): directions p <= m, gradients p > m
+// (stored in two rings).
 // Layout of the per-slot result row (device -> host), doubles.
 struct ResLayout {
     int P;
-    int gdots() const { return 0; }         // [0, P): g_t . ring[p]
+    int gdots() const { return 0; }         // [0, P): g_t . v_p (live p)
     int ginf() const { return P; }          // max |g_t|
     int xsqJ() const { return P + 1; }      // sum of squares of x_t, J part
     int xsqh() const { return P + 2; }      // sum of squares of x_t, h part
     int fdata() const { return P + 3; }     // sum_b w_b (LSE_b - E_b(x_br))
-    int ddg() const { return P + 4; }       // NEWDIR: d . g_k of the stored direction
-    int width() const { return P + 5; }
+    int ddots() const { return P + 4; }     // NEWDIR: [P+4, P+4+(P+1)/2): d . g for gradient p = m+1+k
+    int width() const { return P + 4 + (P + 1) / 2; }
 };
 
 struct Timing {
@@ -80,12 +93,12 @@ void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad,
              uint16_t *Glo, double *fpart, int nbchunk);
 void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const void *Xt,
                int Mpad, int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial,
-               float *ring, int64_t n_slot, int P, float lamJ2, float lamh2, int gm, int tn);
-void update(hipStream_t st, int m, int q, const SlotCmd *cmds, int nslots, float *ring, float *Xc, float *Xtr,
-            uint16_t *Whi, uint16_t *Wlo, const float *X0, int64_t n_slot, int Kxp, int Kx, double *part,
-            int nchunk);
-void dots(hipStream_t st, int m, const SlotCmd *cmds, int nslots, const float *ring, int64_t n_slot, double *part,
-          int nchunk);
+               float *gring, int64_t n_slot, int m, float lamJ2, float lamh2, int gm, int tn);
+void update(hipStream_t st, int m, int q, const SlotCmd *cmds, int nslots, const float *gring, float *dring,
+            float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0, int64_t n_slot, int Kxp, int Kx,
+            double *part, int nchunk);
+void dots(hipStream_t st, int m, const SlotCmd *cmds, int nslots, const float *gring, const float *dring,
+          int64_t n_slot, double *part, int nchunk);
 void reduce_results(hipStream_t st, int m, int nslots, const double *upd_part, const double *dot_part,
                     const double *sm_part, int nchunk, int nbchunk, double *res);
 void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *loci, int nscore, int L,
@@ -178,7 +191,9 @@ struct sdca_ctx {
     int tn_energy = 128, tn_grad = 128;  // output tile widths (autotuned)
     int tuned = 0;                   // 1 if the autotuner chose the GEMM configuration
     int nchunk = 0, nbchunk = 0;
-    float *d_ring = nullptr, *d_Xc = nullptr, *d_Xtr = nullptr, *d_X0 = nullptr;
+    float *d_gring = nullptr;   // gradients: slots x (m+2) x n_slot
+    float *d_dring = nullptr;   // directions: slots x (m+1) x n_slot
+    float *d_Xc = nullptr, *d_Xtr = nullptr, *d_X0 = nullptr;
     uint16_t *d_Whi = nullptr, *d_Wlo = nullptr;
     float *d_G32 = nullptr;
     uint16_t *d_Ghi = nullptr, *d_Glo = nullptr;

@@ -22,14 +22,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t f2bf(float x)
-{
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // NaN stays NaN
-    u += 0x7fffu + ((u >> 16) & 1u);                                            // round to nearest even
-    return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // ---------------------------------------------------------------- one-hot X
 
@@ -340,15 +332,15 @@ struct EpiEnergy {
 // Gradient epilogue: row m = slot*q + s, column n.  Adds 2*lambda*x (lambda_h on the
 // bias column, lambda_J elsewhere: plmDCA_objective.hpp:192-244), zeroes the locus'
 // own columns (the j != r exclusions of :95,111,161,176) and pad columns, and writes
-// the slot's trial-gradient ring vector.  The commands of the slots under the tile's
+// the slot's trial gradient g_t (gradient ring entry gt - g0).  The commands of the slots under the tile's
 // 128 rows are staged in LDS once; each group of four rows first loads its trial-point
 // values, then stores.
 struct EpiGrad {
-    float *ring;
+    float *gring;
     const float *Xtrial;
     const SlotCmd *cmds;
     int64_t n_slot, Kxp;
-    int P, q, Kx, nslots;
+    int ng, g0, q, Kx, nslots;  // gradients per slot, ring index of the first
     float lamJ2, lamh2;
     template <int NI>
     __device__ __forceinline__ void tile(int m0, int wm, int64_t nw, int lane, int tid, floatx16 (&acc)[2][NI],
@@ -359,7 +351,7 @@ struct EpiGrad {
         const int ns = min(nslots - i0, (kTile - 1) / q + 2);  // slots under rows [m0, m0 + 128)
         if (tid < ns) {
             const SlotCmd &c = cmds[i0 + tid];
-            s_gt[tid] = c.mode == MODE_IDLE ? -1 : c.gt;
+            s_gt[tid] = (c.mode == MODE_IDLE || c.gt < g0 || c.gt >= g0 + ng) ? -1 : c.gt;
             s_lo[tid] = c.self_lo;
             s_hi[tid] = c.self_hi;
         }
@@ -380,7 +372,7 @@ struct EpiGrad {
                     gt[u] = ok ? s_gt[il] : -1;
                     lo[u] = ok ? s_lo[il] : 0;
                     hi[u] = ok ? s_hi[il] : 0;
-                    roff[u] = ((int64_t)i * P + (gt[u] < 0 ? 0 : gt[u])) * n_slot + (int64_t)(m - i * q) * Kxp;
+                    roff[u] = ((int64_t)i * ng + (gt[u] < 0 ? 0 : gt[u] - g0)) * n_slot + (int64_t)(m - i * q) * Kxp;
                     const float *xr = Xtrial + (int64_t)m * Kxp;
 #pragma unroll
                     for (int ni = 0; ni < NI; ++ni) {
@@ -398,7 +390,7 @@ struct EpiGrad {
                         float val = 0.f;
                         if (n <= Kx && !(n >= lo[u] && n < hi[u]))
                             val = acc[mi][ni][vq + u] + (n == Kx ? lamh2 : lamJ2) * xv[u][ni];
-                        ring[roff[u] + n] = val;
+                        gring[roff[u] + n] = val;
                     }
                 }
             }
@@ -497,30 +489,43 @@ __device__ __forceinline__ double dot4(float4 a, float4 b)
     return (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z + (double)a.w * b.w;
 }
 
+// History vectors of a slot: directions p in [0, M] and gradients p in [M+1, 2M+2], in
+// separate rings.
+__device__ __forceinline__ float4 ld4(const float *v, int64_t e) { return *reinterpret_cast<const float4 *>(v + e); }
+__device__ __forceinline__ bool is_live(const SlotCmd &c, int p) { return (c.live[p >> 5] >> (p & 31)) & 1u; }
+
 // Trial point of every slot (one launch per round):
 //   INIT    x_c = x0 (staged) or 0,           x_t = x_c
-//   RETRY   x_t = x_c + alpha d                (d = ring[dst])
-//   NEWDIR  x_c = x_t (if accept), d = sum_p coef[p] ring[p] -> ring[dst], x_t = x_c + alpha d
-//           (only ring vectors with a nonzero coefficient are read; the host derives the
-//           Gram row of d from the coefficients, except d . g_k, the slope the line search
-//           starts from, which is taken from the stored FP32 d here)
-// plus the squared norms of x_t for the L2 terms: part[slot][chunk][3] = {sum x^2 over the
-// whole vector, sum over the h entries (column Kx of each row), d . g_k}.
+//   RETRY   x_t = x_c + alpha d                (d = direction dst)
+//   NEWDIR  x_c = x_t (if accept), d = sum_p coef[p] v_p -> direction dst,
+//           x_t = x_c + alpha d, and d . g for every live gradient g other than the trial
+//           slot gt (the Gram row of the stored d).  Only vectors with a nonzero
+//           coefficient or a needed product are read.
+// plus the squared norms of x_t for the L2 terms: part[slot][chunk][M + 4] = {sum x^2 over
+// the whole vector, sum over the h entries (column Kx of each row), d . g_k for the M+2
+// gradient slots k}.
 template <int M>
-__global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds, float *__restrict__ ring,
-                                                float *__restrict__ Xc, float *__restrict__ Xtr,
+__global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds, const float *__restrict__ gring,
+                                                float *__restrict__ dring, float *__restrict__ Xc, float *__restrict__ Xtr,
                                                 uint16_t *__restrict__ Whi, uint16_t *__restrict__ Wlo,
                                                 const float *__restrict__ X0, int64_t n_slot, int Kxp, int Kx, int q,
                                                 double *__restrict__ part, int nchunk)
 {
-    constexpr int P = 2 * M + 3;
+    constexpr int ND = M + 1, NG = M + 2, NV = 2 + NG;
     const int i = blockIdx.y;
     const SlotCmd &c = cmds[i];
-    double acc[3] = {0.0, 0.0, 0.0};
-    const bool is_max[3] = {false, false, false};
+    double acc[NV];
+    bool is_max[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        acc[k] = 0.0;
+        is_max[k] = false;
+    }
     const int mode = c.mode;
-    if (mode != MODE_IDLE) {
-        float *rb = ring + (int64_t)i * P * n_slot;
+    const bool dst_ok = mode == MODE_INIT || (c.dst >= 0 && c.dst < ND);
+    if (mode != MODE_IDLE && dst_ok) {
+        const float *gb = gring + (int64_t)i * NG * n_slot;
+        float *db = dring + (int64_t)i * ND * n_slot;
         float *xc = Xc + (int64_t)i * n_slot;
         float *xt = Xtr + (int64_t)i * n_slot;
         const float *x0 = X0 ? X0 + (int64_t)i * n_slot : nullptr;
@@ -530,12 +535,13 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
         // the h entry of row s is element Kx % 4 of float4 number (s*Kxp + Kx)/4
         const int64_t hq0 = Kx >> 2, hstep = Kxp >> 2;
         const int hlane = Kx & 3;
-        float cf[P];
-#pragma unroll
-        for (int p = 0; p < P; ++p) cf[p] = c.coef[p];
-        const int gk = mode == MODE_NEWDIR ? c.gk : 0;
         const bool accept = c.accept != 0;
-        const float cf_gk = c.coef[gk];
+        float cf[2 * M + 3];
+        bool dotg[NG];  // product with gradient k wanted
+#pragma unroll
+        for (int p = 0; p < 2 * M + 3; ++p) cf[p] = c.coef[p];
+#pragma unroll
+        for (int k = 0; k < NG; ++k) dotg[k] = mode == MODE_NEWDIR && is_live(c, ND + k) && ND + k != c.gt;
         for (int64_t e4 = (int64_t)blockIdx.x * 256 + threadIdx.x; e4 < n4; e4 += (int64_t)nchunk * 256) {
             const int64_t e = e4 * 4;
             float4 xn;
@@ -544,26 +550,39 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
                 *reinterpret_cast<float4 *>(xc + e) = xn;
             } else if (mode == MODE_RETRY) {
                 const float4 xv = *reinterpret_cast<const float4 *>(xc + e);
-                const float4 d = *reinterpret_cast<const float4 *>(rb + (int64_t)dst * n_slot + e);
+                const float4 d = ld4(db + (int64_t)dst * n_slot, e);
                 xn = make_float4(xv.x + alpha * d.x, xv.y + alpha * d.y, xv.z + alpha * d.z, xv.w + alpha * d.w);
             } else {  // NEWDIR
                 const float4 xv = *reinterpret_cast<const float4 *>((accept ? xt : xc) + e);
                 if (accept) *reinterpret_cast<float4 *>(xc + e) = xv;
-                float4 d = make_float4(0.f, 0.f, 0.f, 0.f), g = make_float4(0.f, 0.f, 0.f, 0.f);
+                float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-                for (int p = 0; p < P; ++p) {
-                    if (cf[p] != 0.f) {  // unused ring entries are never read
-                        const float4 v = *reinterpret_cast<const float4 *>(rb + (int64_t)p * n_slot + e);
-                        if (p == gk) g = v;
+                for (int p = 0; p < ND; ++p) {
+                    if (cf[p] != 0.f) {  // unused vectors are never read
+                        const float4 v = ld4(db + (int64_t)p * n_slot, e);
                         d.x += cf[p] * v.x;
                         d.y += cf[p] * v.y;
                         d.z += cf[p] * v.z;
                         d.w += cf[p] * v.w;
                     }
                 }
-                if (cf_gk == 0.f) g = *reinterpret_cast<const float4 *>(rb + (int64_t)gk * n_slot + e);
-                *reinterpret_cast<float4 *>(rb + (int64_t)dst * n_slot + e) = d;
-                acc[2] += dot4(d, g);
+                float4 gv[NG];
+#pragma unroll
+                for (int k = 0; k < NG; ++k) {
+                    const float w = cf[ND + k];
+                    gv[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (w != 0.f || dotg[k]) gv[k] = *reinterpret_cast<const float4 *>(gb + (int64_t)k * n_slot + e);
+                    if (w != 0.f) {
+                        d.x += w * gv[k].x;
+                        d.y += w * gv[k].y;
+                        d.z += w * gv[k].z;
+                        d.w += w * gv[k].w;
+                    }
+                }
+                *reinterpret_cast<float4 *>(db + (int64_t)dst * n_slot + e) = d;
+#pragma unroll
+                for (int k = 0; k < NG; ++k)
+                    if (dotg[k]) acc[2 + k] += dot4(d, gv[k]);
                 xn = make_float4(xv.x + alpha * d.x, xv.y + alpha * d.y, xv.z + alpha * d.z, xv.w + alpha * d.w);
             }
             *reinterpret_cast<float4 *>(xt + e) = xn;
@@ -586,17 +605,17 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
             }
         }
     }
-    block_reduce_store<3>(acc, part + ((int64_t)i * nchunk + blockIdx.x) * 3, is_max);
+    block_reduce_store<NV>(acc, part + ((int64_t)i * nchunk + blockIdx.x) * NV, is_max);
 }
 
-// Dot products of each active slot's trial gradient ring[gt] with every ring vector,
-// and its infinity norm: part[slot][chunk][P + 1].
+// Dot products of each active slot's trial gradient g_t with every live history vector
+// (the others are not read), and its infinity norm: part[slot][chunk][P + 1].
 template <int M>
-__global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, const float *__restrict__ ring,
-                                              int64_t n_slot, double *__restrict__ part, int nchunk)
+__global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, const float *__restrict__ gring,
+                                              const float *__restrict__ dring, int64_t n_slot, double *__restrict__ part,
+                                              int nchunk)
 {
-    constexpr int P = 2 * M + 3;
-    constexpr int NV = P + 1;
+    constexpr int ND = M + 1, NG = M + 2, P = 2 * M + 3, NV = P + 1;
     const int i = blockIdx.y;
     const SlotCmd &c = cmds[i];
     double acc[NV];
@@ -606,32 +625,45 @@ __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, 
         acc[k] = 0.0;
         is_max[k] = (k == P);
     }
-    if (c.mode != MODE_IDLE) {
-        const float *rb = ring + (int64_t)i * P * n_slot;
-        const float *g = rb + (int64_t)c.gt * n_slot;
+    if (c.mode != MODE_IDLE && c.gt >= ND && c.gt < P) {
+        const float *gb = gring + (int64_t)i * NG * n_slot;
+        const float *db = dring + (int64_t)i * ND * n_slot;
+        const int gt = c.gt;
+        const float *g = gb + (int64_t)(gt - ND) * n_slot;
+        const uint32_t l0 = c.live[0], l1 = c.live[1];
+        auto live = [&](int p) -> bool { return p < 32 ? (l0 >> p) & 1u : (l1 >> (p - 32)) & 1u; };
+        double self = 0.0;  // g_t . g_t
         const int64_t n4 = n_slot >> 2;
         for (int64_t e4 = (int64_t)blockIdx.x * 256 + threadIdx.x; e4 < n4; e4 += (int64_t)nchunk * 256) {
             const int64_t e = e4 * 4;
             const float4 gv = *reinterpret_cast<const float4 *>(g + e);
             acc[P] = fmax(acc[P], (double)fmaxf(fmaxf(fabsf(gv.x), fabsf(gv.y)), fmaxf(fabsf(gv.z), fabsf(gv.w))));
 #pragma unroll
-            for (int p = 0; p < P; ++p) {
-                const float4 v = *reinterpret_cast<const float4 *>(rb + (int64_t)p * n_slot + e);
-                acc[p] += dot4(gv, v);
+            for (int p = 0; p < ND; ++p)
+                if (live(p)) acc[p] += dot4(gv, ld4(db + (int64_t)p * n_slot, e));
+            self += dot4(gv, gv);
+#pragma unroll
+            for (int k = 0; k < NG; ++k) {
+                if (!live(ND + k) || ND + k == gt) continue;
+                acc[ND + k] += dot4(gv, *reinterpret_cast<const float4 *>(gb + (int64_t)k * n_slot + e));
             }
         }
+#pragma unroll
+        for (int k = 0; k < NG; ++k)
+            if (ND + k == gt) acc[ND + k] = self;
     }
     block_reduce_store<NV>(acc, part + ((int64_t)i * nchunk + blockIdx.x) * NV, is_max);
 }
 
 // Per-slot reduction of the chunk partials into the result row (ResLayout):
-// [0, P) g_t . ring[p], P max |g_t|, P+1 sum x_t^2 (J part), P+2 (h part), P+3 data term.
+// [0, P) g_t . v_p, P max |g_t|, P+1 sum x_t^2 (J part), P+2 (h part), P+3 data term,
+// [P+4, P+4+NG) d . g_k of a new direction.
 __global__ void k_reduce(int P, const double *__restrict__ upd_part, const double *__restrict__ dot_part,
                          const double *__restrict__ sm_part, int nchunk, int nbchunk, double *__restrict__ res)
 {
     const int i = blockIdx.x;
     const int t = threadIdx.x;
-    const int W = P + 5;
+    const int NG = (P + 1) / 2, NU = 2 + NG, W = P + 4 + NG;
     double *out = res + (int64_t)i * W;
     if (t < P + 1) {  // dots kernel: gdots then ginf
         double a = 0.0;
@@ -640,19 +672,22 @@ __global__ void k_reduce(int P, const double *__restrict__ upd_part, const doubl
             a = (t == P) ? fmax(a, v) : a + v;
         }
         out[t] = a;
-    } else if (t == P + 1) {  // update kernel: squares (all, h) and d . g_k
-        double all = 0.0, h = 0.0, dg = 0.0;
+    } else if (t == P + 1) {  // update kernel: squares (all, h)
+        double all = 0.0, h = 0.0;
         for (int ch = 0; ch < nchunk; ++ch) {
-            all += upd_part[((int64_t)i * nchunk + ch) * 3];
-            h += upd_part[((int64_t)i * nchunk + ch) * 3 + 1];
-            dg += upd_part[((int64_t)i * nchunk + ch) * 3 + 2];
+            all += upd_part[((int64_t)i * nchunk + ch) * NU];
+            h += upd_part[((int64_t)i * nchunk + ch) * NU + 1];
         }
         out[P + 1] = all - h;
         out[P + 2] = h;
-        out[P + 4] = dg;
     } else if (t == P + 3) {
         double a = 0.0;
         for (int ch = 0; ch < nbchunk; ++ch) a += sm_part[(int64_t)i * nbchunk + ch];
+        out[t] = a;
+    } else if (t >= P + 4 && t < W) {  // update kernel: products of the new direction
+        const int k = t - (P + 4);
+        double a = 0.0;
+        for (int ch = 0; ch < nchunk; ++ch) a += upd_part[((int64_t)i * nchunk + ch) * NU + 2 + k];
         out[t] = a;
     }
 }
@@ -807,10 +842,10 @@ void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad,
 }
 
 void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const void *Xt, int Mpad,
-               int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial, float *ring,
-               int64_t n_slot, int P, float lamJ2, float lamh2, int gm, int tn)
+               int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial, float *gring,
+               int64_t n_slot, int m, float lamJ2, float lamh2, int gm, int tn)
 {
-    EpiGrad epi{ring, Xtrial, cmds, n_slot, (int64_t)Kxp, P, q, Kx, nslots, lamJ2, lamh2};
+    EpiGrad epi{gring, Xtrial, cmds, n_slot, (int64_t)Kxp, m + 2, m + 1, q, Kx, nslots, lamJ2, lamh2};
     const TileMap tm = make_map(Mpad / kTile, Kxp / tn, 1, gm);
     if (prec == SDCA_PREC_FP32)
         launch_gemm<PolF32, 1>(st, tn, tm, G32, G32, Bp, Xt, Bp, Bp, Bp, epi);
@@ -828,21 +863,22 @@ void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const
     default: break;              \
     }
 
-void update(hipStream_t st, int m, int q, const SlotCmd *cmds, int nslots, float *ring, float *Xc, float *Xtr,
-            uint16_t *Whi, uint16_t *Wlo, const float *X0, int64_t n_slot, int Kxp, int Kx, double *part, int nchunk)
+void update(hipStream_t st, int m, int q, const SlotCmd *cmds, int nslots, const float *gring, float *dring,
+            float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0, int64_t n_slot, int Kxp, int Kx,
+            double *part, int nchunk)
 {
 #define SDCA_UPD(MM)                                                                                                  \
-    hipLaunchKernelGGL(k_update<MM>, dim3(nchunk, nslots), dim3(256), 0, st, cmds, ring, Xc, Xtr, Whi, Wlo, X0, n_slot, \
-                       Kxp, Kx, q, part, nchunk)
+    hipLaunchKernelGGL(k_update<MM>, dim3(nchunk, nslots), dim3(256), 0, st, cmds, gring, dring, Xc, Xtr, Whi, Wlo, \
+                       X0, n_slot, Kxp, Kx, q, part, nchunk)
     SDCA_DISPATCH_M(m, SDCA_UPD)
 #undef SDCA_UPD
 }
 
-void dots(hipStream_t st, int m, const SlotCmd *cmds, int nslots, const float *ring, int64_t n_slot, double *part,
-          int nchunk)
+void dots(hipStream_t st, int m, const SlotCmd *cmds, int nslots, const float *gring, const float *dring,
+          int64_t n_slot, double *part, int nchunk)
 {
-#define SDCA_DOT(MM) \
-    hipLaunchKernelGGL(k_dots<MM>, dim3(nchunk, nslots), dim3(256), 0, st, cmds, ring, n_slot, part, nchunk)
+#define SDCA_DOT(MM)                                                                                                 \
+    hipLaunchKernelGGL(k_dots<MM>, dim3(nchunk, nslots), dim3(256), 0, st, cmds, gring, dring, n_slot, part, nchunk)
     SDCA_DISPATCH_M(m, SDCA_DOT)
 #undef SDCA_DOT
 }
@@ -851,7 +887,7 @@ void reduce_results(hipStream_t st, int m, int nslots, const double *upd_part, c
                     const double *sm_part, int nchunk, int nbchunk, double *res)
 {
     const int P = 2 * m + 3;
-    const int W = P + 5;
+    const int W = P + 4 + (m + 2);
     hipLaunchKernelGGL(k_reduce, dim3(nslots), dim3((W + 63) / 64 * 64), 0, st, P, upd_part, dot_part, sm_part, nchunk,
                        nbchunk, res);
 }
