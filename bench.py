@@ -117,7 +117,7 @@ def main() -> int:
     ap.add_argument("--loci-per-step", type=int, default=0, help="default: 2048 at C4, 1024 otherwise")
     ap.add_argument("--slots", type=int, default=0, help="loci in flight (default 256 at C4, 128 otherwise)")
     ap.add_argument("--history", type=int, default=10)
-    ap.add_argument("--groups", type=int, default=0, help="slot groups pipelined on two streams (0: automatic)")
+    ap.add_argument("--groups", type=int, default=1, help="slot groups pipelined on two streams (1 or 2)")
     ap.add_argument("--partners", type=int, default=2048, help="warm-started partners per locus")
     ap.add_argument("--warm-mode", default="direction", choices=list(WARM_MODES),
                     help="direction: network blocks as the first search direction from zero; direction-fields: "

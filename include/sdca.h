@@ -60,7 +60,7 @@ typedef struct sdca_opts {
     int32_t gemm_gm_grad;    /* > 0: force the gradient XCD super-tile height, 1..64 (no tuning) */
     int32_t gemm_tn_energy;  /* > 0: force the energy output tile width, 128 or 256 (no tuning) */
     int32_t gemm_tn_grad;    /* > 0: force the gradient output tile width, 128 or 256 (no tuning) */
-    int32_t pipeline_groups; /* slot groups whose rounds overlap on two streams: 0 = automatic, 1 or 2 */
+    int32_t pipeline_groups; /* 2: two slot groups whose rounds overlap on two streams; 0 or 1: one */
     int32_t reserved[5];
 } sdca_opts;
 

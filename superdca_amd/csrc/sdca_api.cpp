@@ -7,15 +7,17 @@
 //   B_eff and lambda                    SuperDCA/include/plmDCA_optimizer_parameters.hpp:257-284
 //   objective and gradient              SuperDCA/include/plmDCA_objective.hpp:38-249
 //
-// Batched L-BFGS.  Every slot of a batch holds one locus.  Its history lives in a ring
-// of P = 2m+3 device vectors: m+1 search directions d_i and m+2 gradients g_i, so that
-// s_i = alpha_i d_i and y_i = g_{i+1} - g_i are never materialised.  The host keeps the
-// P x P Gram matrix of the ring (FP64 dot products computed on the device while the
-// vectors are written) and runs the two-loop recursion on coefficient vectors over the
-// ring; the device then forms d = sum_p coef_p ring_p in one pass.  Each round of the
-// batch performs one objective evaluation for every active slot: trial point,
-// energies, softmax, gradient, dot products, one small device->host copy.  The line
-// search is the More-Thuente method, stepped on the host per slot.
+// Batched L-BFGS.  Every slot of a batch holds one locus.  Its history is m curvature
+// pairs s_p = alpha d and y_p = g_new - g_old stored on the device (BF16 under split
+// precision, FP32 otherwise), plus two gradient buffers, the search direction and the
+// iterate / trial point.  The host keeps the inner products the two-loop recursion
+// needs (s.y, y.y, s.g, y.g, g.g; FP64 products computed on the device while the
+// vectors stream, or derived from them when a pair is formed) and runs the recursion on
+// coefficient vectors; the device then forms d = c_g g + sum_p (c_s s_p + c_y y_p) in
+// the same pass that stores the newest pair.  Each round of the batch performs one
+// objective evaluation for every active slot: trial point, energies, softmax, gradient,
+// dot products, one small device->host copy.  The line search is the More-Thuente
+// method, stepped on the host per slot.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -103,8 +105,10 @@ bool valid_history(int m) { return m == 1 || m == 3 || m == 5 || m == 10 || m ==
 
 void sdca_ctx::free_workspace()
 {
-    dfree(d_gring);
-    dfree(d_dring);
+    dfree(d_gbuf);
+    dfree(d_dbuf);
+    dfree(d_S);
+    dfree(d_Y);
     dfree(d_Xtr);
     d_Xc = nullptr;  // second half of the d_Xtr allocation
     dfree(d_X0);
@@ -283,7 +287,7 @@ float elapsed(hipEvent_t a, hipEvent_t b);
 // Times the candidates on the workspace's own buffers (the first slot group, nslots
 // slots) with synthetic slot commands (every slot active, writing its first gradient) and
 // leaves the buffers zeroed again.
-int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand, GemmConfig &best)
+int autotune(sdca_ctx *c, int nslots, const std::vector<GemmConfig> &cand, GemmConfig &best)
 {
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
     hipStream_t st = c->stream;
@@ -292,7 +296,9 @@ int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand
         std::memset(&cmds[i], 0, sizeof(cmds[i]));
         cmds[i].mode = sdca::MODE_INIT;
         cmds[i].locus = i % c->L;
-        cmds[i].gt = c->ws_m + 1;  // the first gradient vector
+        cmds[i].gt = 0;  // the first gradient buffer
+        cmds[i].gk = -1;
+        cmds[i].pnew = -1;
     }
     HIP_TRY(hipMemcpyAsync(c->d_cmds, cmds.data(), nslots * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     const size_t nw = (size_t)c->Mpad * c->Kxp;
@@ -310,7 +316,7 @@ int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand
     };
     auto grad = [&](int gm, int tn) {
         sdca::kern::gemm_grad(st, c->opts.precision, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Xt, c->Mg, c->Bp,
-                              (int)c->Kxp, (int)c->Kx, c->q, nslots, c->d_cmds, c->d_Xtr, c->d_gring, c->n_slot, c->ws_m,
+                              (int)c->Kxp, (int)c->Kx, c->q, nslots, c->d_cmds, c->d_Xtr, c->d_gbuf, c->n_slot,
                               (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), gm, tn);
     };
     auto time = [&](auto &&launch) -> float {
@@ -353,7 +359,7 @@ int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand
         }
     // the solve starts from zeroed buffers
     const size_t all = (size_t)c->ngroups * nslots;
-    HIP_TRY(hipMemsetAsync(c->d_gring, 0, all * (P + 1) / 2 * c->n_slot * sizeof(float), st));
+    HIP_TRY(hipMemsetAsync(c->d_gbuf, 0, all * 2 * c->n_slot * sizeof(float), st));
     HIP_TRY(hipMemsetAsync(c->d_Xtr, 0, nw * sizeof(float), st));
     if (split) {
         HIP_TRY(hipMemsetAsync(c->d_Whi, 0, nw * sizeof(uint16_t), st));
@@ -364,13 +370,18 @@ int autotune(sdca_ctx *c, int nslots, int P, const std::vector<GemmConfig> &cand
     return SDCA_OK;
 }
 
-// Slot groups: two groups whose rounds run on separate streams when the slots split
-// evenly into halves of whole 128-row GEMM tiles (the compute-bound contractions of one
-// group then overlap the memory-bound vector passes of the other), else one.
+// Bytes per element of a stored curvature pair: BF16 under split precision (s and y
+// are differences, accurate to their own magnitude after rounding; every product uses
+// the rounded stored vector), FP32 otherwise.
+inline size_t pair_bytes(const sdca_ctx *c) { return c->opts.precision == SDCA_PREC_SPLIT_BF16 ? 2 : 4; }
+
+// Slot groups: with pipeline_groups = 2 the slots split into two halves of whole
+// 128-row GEMM tiles whose rounds run on separate streams (the host steps one group while
+// the other computes); otherwise one group.  One group is the default: the two groups'
+// kernels compete for the same CUs, and at the C4 shape one group is faster.
 int choose_groups(const sdca_ctx *c, int nslots)
 {
-    const int want = c->opts.pipeline_groups;
-    if (want == 1) return 1;
+    if (c->opts.pipeline_groups != 2) return 1;
     const bool even = nslots % 2 == 0 && ((int64_t)(nslots / 2) * c->q) % sdca::kTile == 0;
     return even ? 2 : 1;
 }
@@ -381,7 +392,6 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     const int ng = choose_groups(c, nslots);
     if (c->ws_slots == nslots && c->ws_m == m && c->ws_groups == ng) return SDCA_OK;
     c->free_workspace();
-    const int P = 2 * m + 3;
     c->n_slot = (int64_t)c->q * c->Kxp;
     c->ngroups = ng;
     c->gslots = nslots / ng;
@@ -390,8 +400,10 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
     int rc;
     c->ws_m = m;  // ring shapes below and the autotuner use it
-    if ((rc = dalloc(&c->d_gring, (size_t)nslots * (m + 2) * c->n_slot))) return rc;
-    if ((rc = dalloc(&c->d_dring, (size_t)nslots * (m + 1) * c->n_slot))) return rc;
+    if ((rc = dalloc(&c->d_gbuf, (size_t)nslots * 2 * c->n_slot))) return rc;
+    if ((rc = dalloc(&c->d_dbuf, (size_t)nslots * c->n_slot))) return rc;
+    if ((rc = dalloc((uint8_t **)&c->d_S, (size_t)nslots * m * c->n_slot * pair_bytes(c)))) return rc;
+    if ((rc = dalloc((uint8_t **)&c->d_Y, (size_t)nslots * m * c->n_slot * pair_bytes(c)))) return rc;
     if ((rc = dalloc(&c->d_Xtr, (size_t)2 * c->Mpad * c->Kxp))) return rc;
     c->d_Xc = c->d_Xtr + (size_t)c->Mpad * c->Kxp;
     if (split) {
@@ -427,21 +439,21 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     if ((rc = dalloc(&c->d_Epart, (size_t)max_ksplit * c->Mpad * c->Bp))) return rc;  // per group: ksplit x Mg x Bp
     c->nchunk = (int)std::min<int64_t>(64, std::max<int64_t>(1, (c->n_slot / 4 + 255) / 256));
     c->nbchunk = (c->Bp + 255) / 256;
-    if ((rc = dalloc(&c->d_upd_part, (size_t)nslots * c->nchunk * (2 + m + 2)))) return rc;
-    if ((rc = dalloc(&c->d_dot_part, (size_t)nslots * c->nchunk * (P + 1)))) return rc;
+    if ((rc = dalloc(&c->d_upd_part, (size_t)nslots * c->nchunk * sdca::ResLayout{m}.nupd()))) return rc;
+    if ((rc = dalloc(&c->d_dot_part, (size_t)nslots * c->nchunk * sdca::ResLayout{m}.ndots()))) return rc;
     if ((rc = dalloc(&c->d_sm_part, (size_t)nslots * c->nbchunk))) return rc;
-    if ((rc = dalloc(&c->d_res, (size_t)nslots * sdca::ResLayout{P}.width()))) return rc;
+    if ((rc = dalloc(&c->d_res, (size_t)nslots * sdca::ResLayout{m}.width()))) return rc;
     if ((rc = dalloc(&c->d_cmds, (size_t)nslots))) return rc;
     if ((rc = dalloc(&c->d_rows, (size_t)nslots * c->L))) return rc;
     if ((rc = dalloc(&c->d_src_off, (size_t)nslots))) return rc;
     if ((rc = dalloc(&c->d_score_loci, (size_t)nslots))) return rc;
     HIP_TRY(hipHostMalloc((void **)&c->h_cmds, (size_t)nslots * sizeof(sdca::SlotCmd), hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void **)&c->h_res, (size_t)nslots * sdca::ResLayout{P}.width() * sizeof(double),
+    HIP_TRY(hipHostMalloc((void **)&c->h_res, (size_t)nslots * sdca::ResLayout{m}.width() * sizeof(double),
                           hipHostMallocDefault));
     // the zero fills above must be complete before any kernel of the solve runs
     HIP_TRY(hipDeviceSynchronize());
     c->tuned = 0;
-    if (tune && (rc = autotune(c, c->gslots, P, cand, cfg))) return rc;
+    if (tune && (rc = autotune(c, c->gslots, cand, cfg))) return rc;
     c->ksplit = cfg.ksplit;
     c->kchunk = cfg.kchunk;
     c->gm_energy = cfg.gm_e;
@@ -468,13 +480,15 @@ float elapsed(hipEvent_t a, hipEvent_t b)
 // result rows (ResLayout) to c->h_res.  cmds points at the group's commands.
 int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
 {
-    const int m = c->ws_m, P = 2 * m + 3, W = sdca::ResLayout{P}.width();
+    const int m = c->ws_m, W = sdca::ResLayout{m}.width();
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
     const int ns = c->gslots, s0 = g * ns;
     const size_t vo = (size_t)s0 * c->n_slot;        // slot-vector offset of the group
     const size_t go = (size_t)s0 * c->q * c->Bp;     // row offset of its node beliefs
-    float *gring = c->d_gring + (size_t)s0 * (m + 2) * c->n_slot;
-    float *dring = c->d_dring + (size_t)s0 * (m + 1) * c->n_slot;
+    float *gbuf = c->d_gbuf + (size_t)s0 * 2 * c->n_slot;
+    float *dbuf = c->d_dbuf + vo;
+    const size_t po = (size_t)s0 * m * c->n_slot * pair_bytes(c);
+    void *Sp = (uint8_t *)c->d_S + po, *Yp = (uint8_t *)c->d_Y + po;
     sdca::SlotCmd *dc = c->d_cmds + s0;
     float *Epart = c->d_Epart + (size_t)g * c->ksplit * c->Mg * c->Bp;
     hipStream_t st = c->gstream[g];
@@ -482,8 +496,8 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     std::memcpy(c->h_cmds + s0, cmds, ns * sizeof(sdca::SlotCmd));
     HIP_TRY(hipMemcpyAsync(dc, c->h_cmds + s0, ns * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(ev[0], st));
-    double *upd_part = c->d_upd_part + (size_t)s0 * c->nchunk * (2 + m + 2);
-    sdca::kern::update(st, m, c->q, dc, ns, gring, dring, c->d_Xc + vo, c->d_Xtr + vo,
+    double *upd_part = c->d_upd_part + (size_t)s0 * c->nchunk * sdca::ResLayout{m}.nupd();
+    sdca::kern::update(st, m, c->q, split, dc, ns, gbuf, dbuf, Sp, Yp, c->d_Xc + vo, c->d_Xtr + vo,
                        split ? c->d_Whi + vo : nullptr, split ? c->d_Wlo + vo : nullptr,
                        c->d_X0 ? c->d_X0 + vo : nullptr, c->n_slot, (int)c->Kxp, (int)c->Kx, upd_part, c->nchunk);
     HIP_TRY(hipGetLastError());
@@ -500,12 +514,12 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     HIP_TRY(hipEventRecord(ev[3], st));
     sdca::kern::gemm_grad(st, c->opts.precision, split ? nullptr : c->d_G32 + go, split ? c->d_Ghi + go : nullptr,
                           split ? c->d_Glo + go : nullptr, c->d_Xt, c->Mg, c->Bp, (int)c->Kxp, (int)c->Kx, c->q, ns,
-                          dc, c->d_Xtr + vo, gring, c->n_slot, m, (float)(2.0 * c->lambda_J),
+                          dc, c->d_Xtr + vo, gbuf, c->n_slot, (float)(2.0 * c->lambda_J),
                           (float)(2.0 * c->lambda_h), c->gm_grad, c->tn_grad);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[4], st));
-    double *dot_part = c->d_dot_part + (size_t)s0 * c->nchunk * (P + 1);
-    sdca::kern::dots(st, m, dc, ns, gring, dring, c->n_slot, dot_part, c->nchunk);
+    double *dot_part = c->d_dot_part + (size_t)s0 * c->nchunk * sdca::ResLayout{m}.ndots();
+    sdca::kern::dots(st, m, split, dc, ns, gbuf, dbuf, Sp, Yp, c->n_slot, dot_part, c->nchunk);
     HIP_TRY(hipGetLastError());
     sdca::kern::reduce_results(st, m, ns, upd_part, dot_part,
                                c->d_sm_part + (size_t)s0 * c->nbchunk, c->nchunk, c->nbchunk,
@@ -520,7 +534,7 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
 // Waits for group g's round and accounts its kernel times and algorithmic traffic.
 int finish_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
 {
-    const int m = c->ws_m, P = 2 * m + 3;
+    const int m = c->ws_m;
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
     HIP_TRY(hipStreamSynchronize(c->gstream[g]));
     hipEvent_t *ev = c->gev[g];
@@ -536,25 +550,27 @@ int finish_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     t.energy_launches += 1;
     t.grad_launches += 1;
     int active = 0;
-    const double db = 4.0;  // bytes per direction element
+    const double pb = (double)pair_bytes(c);
     double bu = 0.0, bd = 0.0;  // bytes per element of the update and dot-product passes
     for (int i = 0; i < c->gslots; ++i) {
         const sdca::SlotCmd &k = cmds[i];
         const int md = k.mode;
         if (md == sdca::MODE_IDLE) continue;
         ++active;
-        auto live = [&](int p) { return (k.live[p >> 5] >> (p & 31)) & 1u; };
+        int npairs = 0;
+        for (int p = 0; p < m; ++p) npairs += (k.live >> p) & 1u;
         if (md == sdca::MODE_INIT) bu += 4.0 * (2 + (k.init_src != 0.f));
-        else if (md == sdca::MODE_RETRY) bu += 8.0 + db;  // x_c and d read, x_t written
+        else if (md == sdca::MODE_RETRY) bu += 12.0;  // x_c and d read, x_t written
         else {
-            bu += 4.0 * (2 + (k.accept != 0)) + db;  // x read, x_t (and x_c) written, d written
-            for (int p = 0; p <= m; ++p) bu += k.coef[p] != 0.f ? db : 0.0;
-            for (int p = m + 1; p < P; ++p) bu += (k.coef[p] != 0.f || (live(p) && p != k.gt)) ? 4.0 : 0.0;
+            // x, d and g_k read; x_t, d (and x_c) written; the new pair from g_old
+            bu += 4.0 * (5 + (k.accept != 0)) + (k.pnew >= 0 ? 4.0 + 2.0 * pb : 0.0);
+            for (int p = 0; p < m; ++p) {
+                if (p == k.pnew) continue;
+                bu += (k.cs[p] != 0.f ? pb : 0.0) + ((k.cy[p] != 0.f || ((k.live >> p) & 1u)) ? pb : 0.0);
+            }
         }
         if (split) bu += 4.0;  // hi + lo BF16 copies of x_t
-        bd += 4.0;             // g_t
-        for (int p = 0; p < P; ++p)
-            if (live(p) && p != k.gt) bd += p <= m ? db : 4.0;
+        bd += 4.0 * (md == sdca::MODE_INIT ? 1 : 3) + 2.0 * pb * npairs;
     }
     t.bytes_update += bu * (double)c->n_slot;
     t.bytes_dots += bd * (double)c->n_slot;
@@ -615,30 +631,34 @@ sdca::SlotCmd idle_cmd()
 
 // ------------------------------------------------------- per-slot solver state
 
-struct Pair {
-    int d, g_old, g_new;
-    double alpha;
-};
-
 enum Phase { PH_IDLE, PH_INIT, PH_LS, PH_DONE };
 
+// Host side of one slot: line-search state and the inner products of the stored vectors
+// that the two-loop recursion needs.  Pairs p hold s_p = alpha d and y_p = g_new - g_old;
+// with g the current gradient, d the current direction and t the trial gradient:
+//   SY[p][q] = s_p.y_q, YY[p][q] = y_p.y_q, SG[p] = s_p.g, YG[p] = y_p.g, GG = g.g,
+//   DG = d.g, DY[p] = d.y_p, and the trial's TS[p], TY[p], TT, TG = t.g, TD = t.d.
 struct Slot {
     int item = -1;  // index into the caller's loci list
     int r = -1;
     Phase phase = PH_IDLE;
-    std::deque<Pair> pairs;  // oldest first
-    int gk = -1, gt = -1, dk = -1;
-    double f = 0.0;      // objective at the accepted iterate
-    double stp = 0.0;    // current trial step
-    double stp0 = 0.0;   // initial step of the current search
-    double dg0 = 0.0;    // directional derivative at the start of the search
+    std::deque<int> pairs;  // pair slots, oldest first
+    int gk = -1, gt = 0;    // gradient buffers: current (-1 before the first) and trial
+    double f = 0.0;         // objective at the accepted iterate
+    double stp = 0.0;       // current trial step
+    double stp0 = 0.0;      // initial step of the current search
+    double dg0 = 0.0;       // directional derivative at the start of the search
     sdca::MoreThuente ms;
     int ls_evals = 0;
     int nfeval = 0, iter = 0, status = -1;
     double gnorm = 0.0;
     bool from_xt = true;
-    bool preloaded = false;  // ring vector m holds a first search direction
+    bool preloaded = false;  // the direction buffer holds a first search direction
     int restarts = 0;        // line searches restarted from steepest descent
+    double SY[sdca::kMaxHistory][sdca::kMaxHistory], YY[sdca::kMaxHistory][sdca::kMaxHistory];
+    double SG[sdca::kMaxHistory], YG[sdca::kMaxHistory], DY[sdca::kMaxHistory];
+    double TS[sdca::kMaxHistory], TY[sdca::kMaxHistory];
+    double GG = 0, DG = 0, TT = 0, TG = 0, TD = 0;
 };
 
 // A failed line search with curvature history is retried once more from steepest
@@ -649,11 +669,8 @@ constexpr int kMaxRestarts = 2;
 
 class Solver {
 public:
-    Solver(sdca_ctx *c, int nslots) : c_(c), n_(nslots), m_(c->ws_m), P_(2 * c->ws_m + 3), slots_(nslots),
-                                       gram_((size_t)nslots * P_ * P_, 0.0), cmds_(nslots, idle_cmd()),
+    Solver(sdca_ctx *c, int nslots) : c_(c), n_(nslots), m_(c->ws_m), slots_(nslots), cmds_(nslots, idle_cmd()),
                                        f_noise_(c->opts.precision == SDCA_PREC_SPLIT_BF16 ? 1e-5 : 1e-6) {}
-
-    double &G(int i, int a, int b) { return gram_[((size_t)i * P_ + a) * P_ + b]; }
 
     void assign(int i, int item, int r, bool warm, bool preloaded = false)
     {
@@ -663,13 +680,14 @@ public:
         S.item = item;
         S.r = r;
         S.phase = PH_INIT;
-        S.gt = m_ + 1;
+        S.gt = 0;
         sdca::SlotCmd &k = cmds_[i];
         k = idle_cmd();
         k.mode = sdca::MODE_INIT;
         k.locus = r;
         k.gt = S.gt;
-        k.dst = 0;
+        k.gk = -1;
+        k.pnew = -1;
         k.self_lo = c_->col_off[r];
         k.self_hi = c_->col_off[r + 1];
         k.init_src = warm ? 1.f : 0.f;
@@ -687,20 +705,27 @@ public:
     {
         Slot &S = slots_[i];
         sdca::SlotCmd &k = cmds_[i];
-        const sdca::ResLayout R{P_};
+        const sdca::ResLayout R{m_};
         const int pending = k.mode;
-        if (pending == sdca::MODE_NEWDIR)  // Gram row of the stored direction (live gradients)
-            for (int p = m_ + 1; p < P_; ++p)
-                if (((k.live[p >> 5] >> (p & 31)) & 1u) && p != k.gt)
-                    G(i, S.dk, p) = G(i, p, S.dk) = res[R.ddots() + (p - m_ - 1)];
-        for (int p = 0; p < P_; ++p) G(i, S.gt, p) = G(i, p, S.gt) = res[R.gdots() + p];
+        if (pending == sdca::MODE_NEWDIR) {  // products of the stored new direction
+            S.DG = res[R.dg()];
+            for (int p : S.pairs) S.DY[p] = res[R.dy(p)];
+        }
+        for (int p : S.pairs) {
+            S.TS[p] = res[R.ts(p)];
+            S.TY[p] = res[R.ty(p)];
+        }
+        S.TT = res[R.tt()];
+        S.TG = res[R.tg()];
+        S.TD = res[R.td()];
         const double f = res[R.fdata()] + c_->lambda_J * res[R.xsqJ()] + c_->lambda_h * res[R.xsqh()];
         const double ginf = res[R.ginf()];
         S.nfeval++;
         if (!std::isfinite(f) || !std::isfinite(ginf)) return finish(i, SDCA_ST_NONFINITE, S.phase == PH_INIT);
         if (S.phase == PH_INIT) {
             S.gk = S.gt;
-            S.gt = -1;
+            S.gt = 1 - S.gk;
+            S.GG = S.TT;
             S.f = f;
             S.gnorm = ginf;
             if (ginf < c_->opts.grad_threshold) return finish(i, SDCA_ST_CONVERGED, true);
@@ -708,22 +733,22 @@ public:
                 S.preloaded = false;
                 newdir_preloaded(i);
             } else {
-                newdir(i);
+                newdir(i, true, -1, 0.0);
             }
             return false;
         }
         // line search in progress
         if (pending == sdca::MODE_NEWDIR) {
-            const double dg0 = G(i, S.dk, S.gk);
+            const double dg0 = S.DG;
             if (!(dg0 < 0.0)) {  // rounding destroyed descent: restart from steepest descent
                 S.pairs.clear();
-                newdir(i, false);  // the trial point along d is not accepted
+                newdir(i, false, -1, 0.0);  // the trial point along d is not accepted
                 return false;
             }
             S.dg0 = dg0;
             S.ms.start(S.f, dg0, S.stp0);
         }
-        const double dg = G(i, S.gt, S.dk);
+        const double dg = S.TD;
         const double stp_trial = S.stp;
         // Approximate Wolfe conditions (W. W. Hager and H. Zhang, SIAM J. Optim. 16 (2005)
         // 170-192): near the optimum the objective, summed from FP32 energies, changes by
@@ -738,8 +763,8 @@ public:
             S.ls_evals++;
             if (act == sdca::MoreThuente::Continue && S.ls_evals < c_->opts.max_ls) {
                 k.mode = sdca::MODE_RETRY;
-                k.dst = S.dk;
                 k.gt = S.gt;
+                k.gk = S.gk;
                 k.alpha = (float)S.stp;
                 return false;
             }
@@ -747,50 +772,58 @@ public:
                 if (S.pairs.empty() || S.restarts >= kMaxRestarts) return finish(i, SDCA_ST_LS_FAIL, false);
                 S.restarts++;
                 S.pairs.clear();
-                newdir(i, false);
+                newdir(i, false, -1, 0.0);
                 return false;
             }
         }
-        const double ys = stp_trial * (G(i, S.dk, S.gt) - G(i, S.dk, S.gk));
+        // accept the trial: the new pair s = stp d, y = t - g (kept if s.y > 0)
+        const double ys = stp_trial * (S.TD - S.DG);
+        int pnew = -1;
         if (ys > 0.0) {
-            S.pairs.push_back(Pair{S.dk, S.gk, S.gt, stp_trial});
-            if ((int)S.pairs.size() > m_) S.pairs.pop_front();
+            if ((int)S.pairs.size() >= m_) {
+                pnew = S.pairs.front();
+                S.pairs.pop_front();
+            } else {
+                pnew = free_pair(i);
+            }
+            for (int q : S.pairs) {
+                S.SY[pnew][q] = stp_trial * S.DY[q];
+                S.SY[q][pnew] = S.TS[q] - S.SG[q];
+                S.YY[pnew][q] = S.YY[q][pnew] = S.TY[q] - S.YG[q];
+            }
+            S.SY[pnew][pnew] = ys;
+            S.YY[pnew][pnew] = S.TT - 2.0 * S.TG + S.GG;
+            // products with the new current gradient t
+            for (int q : S.pairs) {
+                S.SG[q] = S.TS[q];
+                S.YG[q] = S.TY[q];
+            }
+            S.SG[pnew] = stp_trial * S.TD;
+            S.YG[pnew] = S.TT - S.TG;
+            S.pairs.push_back(pnew);
         } else {
             S.pairs.clear();
         }
-        S.gk = S.gt;
-        S.gt = -1;
-        S.dk = -1;
+        S.GG = S.TT;
+        std::swap(S.gk, S.gt);
         S.f = f;
         S.gnorm = ginf;
         S.iter++;
         if (ginf < c_->opts.grad_threshold) return finish(i, SDCA_ST_CONVERGED, true);
         if (S.iter >= c_->opts.max_iter) return finish(i, SDCA_ST_ITER_CAP, true);
-        newdir(i);
+        newdir(i, true, pnew, stp_trial);
         return false;
     }
 
-    // Marks the history vectors of slots [i0, i1) that the coming round must read or pair
-    // with: the trial gradient, the current gradient and direction, and every vector of
-    // the stored pairs.
+    // Marks the stored pairs of slots [i0, i1) for the coming round's products.
     void mark_live(int i0, int i1)
     {
         for (int i = i0; i < i1; ++i) {
             const Slot &S = slots_[i];
             sdca::SlotCmd &k = cmds_[i];
-            k.live[0] = k.live[1] = 0u;
+            k.live = 0u;
             if (S.phase == PH_IDLE || k.mode == sdca::MODE_IDLE) continue;
-            auto set = [&](int p) {
-                if (p >= 0) k.live[p >> 5] |= 1u << (p & 31);
-            };
-            set(S.gt);
-            set(S.gk);
-            set(S.dk);
-            for (const auto &pr : S.pairs) {
-                set(pr.d);
-                set(pr.g_old);
-                set(pr.g_new);
-            }
+            for (int p : S.pairs) k.live |= 1u << p;
         }
     }
 
@@ -808,135 +841,122 @@ private:
         return true;
     }
 
-    int free_d(int i) const
+    int free_pair(int i) const
     {
         const Slot &S = slots_[i];
-        for (int d = 0; d <= m_; ++d) {
-            bool used = (d == S.dk);
-            for (const auto &p : S.pairs) used |= (p.d == d);
-            if (!used) return d;
-        }
-        return -1;
-    }
-    int free_g(int i) const
-    {
-        const Slot &S = slots_[i];
-        for (int g = m_ + 1; g < P_; ++g) {
-            bool used = (g == S.gk);
-            for (const auto &p : S.pairs) used |= (p.g_old == g || p.g_new == g);
-            if (!used) return g;
+        for (int p = 0; p < m_; ++p) {
+            bool used = false;
+            for (int q : S.pairs) used |= q == p;
+            if (!used) return p;
         }
         return -1;
     }
 
     // Two-loop recursion (Nocedal & Wright, Algorithm 7.4) on coefficient vectors over
-    // the ring: u . v = u^T Gram v, s_t = alpha_t e_{d_t}, y_t = e_{g_new} - e_{g_old}.
-    void two_loop(int i, std::vector<double> &delta)
+    // {s_p, y_p, g}: q = g; for pairs newest first a_p = rho_p s_p.q, q -= a_p y_p;
+    // r = gamma q; for pairs oldest first b = rho_p y_p.r, r += (a_p - b) s_p; d = -r.
+    void two_loop(int i, double *cs, double *cy, double &cg)
     {
         Slot &S = slots_[i];
         const int n = (int)S.pairs.size();
-        std::vector<double> qv(P_, 0.0), Gv(P_), a(n), rho(n);
-        auto gmul = [&](const std::vector<double> &v) {
-            for (int p = 0; p < P_; ++p) {
-                double acc = 0.0;
-                for (int u = 0; u < P_; ++u)
-                    if (v[u] != 0.0) acc += G(i, p, u) * v[u];
-                Gv[p] = acc;
-            }
-        };
-        qv[S.gk] = 1.0;
+        for (int p = 0; p < m_; ++p) cs[p] = cy[p] = 0.0;
+        cg = 1.0;
+        double a[sdca::kMaxHistory];
         for (int t = n - 1; t >= 0; --t) {
-            const Pair &pr = S.pairs[t];
-            const double ys = pr.alpha * (G(i, pr.d, pr.g_new) - G(i, pr.d, pr.g_old));
-            rho[t] = 1.0 / ys;
-            gmul(qv);
-            a[t] = rho[t] * pr.alpha * Gv[pr.d];
-            qv[pr.g_new] -= a[t];
-            qv[pr.g_old] += a[t];
+            const int p = S.pairs[t];
+            double sq = cg * S.SG[p];  // s_p . q, q = cg g + sum cy y
+            for (int u : S.pairs) sq += cy[u] * S.SY[p][u];
+            a[p] = sq / S.SY[p][p];
+            cy[p] -= a[p];
         }
         if (n > 0) {
-            const Pair &pr = S.pairs[n - 1];
-            const double ys = pr.alpha * (G(i, pr.d, pr.g_new) - G(i, pr.d, pr.g_old));
-            const double yy = G(i, pr.g_new, pr.g_new) - 2.0 * G(i, pr.g_new, pr.g_old) + G(i, pr.g_old, pr.g_old);
-            const double gam = ys / yy;
-            for (auto &v : qv) v *= gam;
+            const int p = S.pairs[n - 1];
+            const double gam = S.SY[p][p] / S.YY[p][p];
+            cg *= gam;
+            for (int u = 0; u < m_; ++u) cy[u] *= gam;
         }
         for (int t = 0; t < n; ++t) {
-            const Pair &pr = S.pairs[t];
-            gmul(qv);
-            const double beta = rho[t] * (Gv[pr.g_new] - Gv[pr.g_old]);
-            qv[pr.d] += (a[t] - beta) * pr.alpha;
+            const int p = S.pairs[t];
+            double yr = cg * S.YG[p];  // y_p . r, r = cg g + sum cy y + sum cs s
+            for (int u : S.pairs) yr += cy[u] * S.YY[p][u] + cs[u] * S.SY[u][p];
+            cs[p] += a[p] - yr / S.SY[p][p];
         }
-        delta.assign(P_, 0.0);
-        for (int p = 0; p < P_; ++p) delta[p] = -qv[p];
+        for (int p = 0; p < m_; ++p) {
+            cs[p] = -cs[p];
+            cy[p] = -cy[p];
+        }
+        cg = -cg;
     }
 
-    // First direction = the preloaded ring vector m (copied to a free direction slot so
-    // that the history ring keeps its usual roles), unit initial step.  If it turns out
-    // not to be a descent direction the line search start falls back to steepest descent.
+    // First direction = the preloaded direction buffer, unit initial step.  If it turns
+    // out not to be a descent direction the line search start falls back to steepest
+    // descent.
     void newdir_preloaded(int i)
     {
         Slot &S = slots_[i];
-        S.dk = 0;  // ring vector m is the preloaded one; 0 is free (no history yet)
-        S.gt = free_g(i);
         S.stp0 = 1.0;
         S.stp = 1.0;
         S.ls_evals = 0;
         S.phase = PH_LS;
         sdca::SlotCmd &k = cmds_[i];
         k.mode = sdca::MODE_NEWDIR;
-        k.dst = S.dk;
         k.gt = S.gt;
         k.gk = S.gk;
         k.accept = 1;  // x_t = x_c after the start point's evaluation
+        k.pnew = -1;
+        k.pstep = 0.f;
         k.alpha = 1.f;
-        for (int p = 0; p < sdca::kMaxPhys; ++p) k.coef[p] = 0.f;
-        k.coef[m_] = 1.f;
+        k.cd = 1.f;
+        k.cg = 0.f;
+        for (int p = 0; p < sdca::kMaxHistory; ++p) k.cs[p] = k.cy[p] = 0.f;
     }
 
-    // New search direction from the iterate: the last trial point if accept, else the
-    // current x_c (restart after a failed or non-descent search).
-    void newdir(int i, bool accept = true)
+    // New search direction from the iterate: the last trial point if accept (with the new
+    // pair pnew of step pstep, if any), else the current x_c (restart after a failed or
+    // non-descent search).
+    void newdir(int i, bool accept, int pnew, double pstep)
     {
         Slot &S = slots_[i];
-        std::vector<double> delta(P_, 0.0);
+        double cs[sdca::kMaxHistory], cy[sdca::kMaxHistory], cg = -1.0;
         bool ok = false;
         if (!S.pairs.empty()) {
-            two_loop(i, delta);
-            double dg = 0.0;
-            bool finite = true;
-            for (int p = 0; p < P_; ++p) {
-                dg += delta[p] * G(i, p, S.gk);
-                finite &= std::isfinite(delta[p]);
+            two_loop(i, cs, cy, cg);
+            double dg = cg * S.GG;
+            bool finite = std::isfinite(cg);
+            for (int p : S.pairs) {
+                dg += cs[p] * S.SG[p] + cy[p] * S.YG[p];
+                finite &= std::isfinite(cs[p]) && std::isfinite(cy[p]);
             }
             ok = finite && dg < 0.0;
             if (!ok) S.pairs.clear();
         }
         if (!ok) {
-            std::fill(delta.begin(), delta.end(), 0.0);
-            delta[S.gk] = -1.0;
+            for (int p = 0; p < m_; ++p) cs[p] = cy[p] = 0.0;
+            cg = -1.0;
         }
-        S.dk = free_d(i);
-        S.gt = free_g(i);
-        S.stp0 = S.pairs.empty() ? 1.0 / std::sqrt(G(i, S.gk, S.gk)) : 1.0;
+        S.stp0 = S.pairs.empty() ? 1.0 / std::sqrt(S.GG) : 1.0;
         S.stp = S.stp0;
         S.ls_evals = 0;
         S.phase = PH_LS;
         sdca::SlotCmd &k = cmds_[i];
         k.mode = sdca::MODE_NEWDIR;
-        k.dst = S.dk;
         k.gt = S.gt;
         k.gk = S.gk;
         k.accept = accept ? 1 : 0;
+        k.pnew = pnew;
+        k.pstep = (float)pstep;
         k.alpha = (float)S.stp;
-        for (int p = 0; p < sdca::kMaxPhys; ++p) k.coef[p] = p < P_ ? (float)delta[p] : 0.f;
-        k.coef[S.dk] = 0.f;
+        k.cd = 0.f;
+        k.cg = (float)cg;
+        for (int p = 0; p < sdca::kMaxHistory; ++p) {
+            k.cs[p] = p < m_ ? (float)cs[p] : 0.f;
+            k.cy[p] = p < m_ ? (float)cy[p] : 0.f;
+        }
     }
 
     sdca_ctx *c_;
-    int n_, m_, P_;
+    int n_, m_;
     std::vector<Slot> slots_;
-    std::vector<double> gram_;
     std::vector<sdca::SlotCmd> cmds_;
     double f_noise_;  // relative rounding noise of the objective (FP32 or split-BF16 energies)
 };
@@ -1192,7 +1212,7 @@ int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x
         if ((rc = dalloc(&c->d_X0, (size_t)nslots * c->n_slot))) return rc;
         HIP_TRY(hipDeviceSynchronize());
     }
-    const int m = c->ws_m, P = 2 * m + 3;
+    const int m = c->ws_m;
     const size_t n = (size_t)c->L * c->q * c->q + c->q;
     std::vector<float> stage((size_t)nslots * c->n_slot), gdev(c->n_slot);
     std::vector<double> unobs(nslots);
@@ -1206,8 +1226,9 @@ int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x
             unobs[i] = ref_to_dev(c, r, x + (size_t)(k0 + i) * n, stage.data() + (size_t)i * c->n_slot);
             cmds[i].mode = sdca::MODE_INIT;
             cmds[i].locus = r;
-            cmds[i].gt = m + 1;
-            cmds[i].live[(m + 1) >> 5] = 1u << ((m + 1) & 31);
+            cmds[i].gt = 0;
+            cmds[i].gk = -1;
+            cmds[i].pnew = -1;
             cmds[i].self_lo = c->col_off[r];
             cmds[i].self_hi = c->col_off[r + 1];
             cmds[i].init_src = 1.f;
@@ -1217,14 +1238,14 @@ int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x
             if ((rc = launch_round(c, g, cmds.data() + (size_t)g * c->gslots))) return rc;
         for (int g = 0; g < c->ngroups; ++g)
             if ((rc = finish_round(c, g, cmds.data() + (size_t)g * c->gslots))) return rc;
-        const sdca::ResLayout R{P};
+        const sdca::ResLayout R{m};
         for (int i = 0; i < cnt; ++i) {
             const int r = loci[k0 + i];
             const double *rr = c->h_res + (size_t)i * R.width();
             f_out[k0 + i] = rr[R.fdata()] + c->lambda_J * rr[R.xsqJ()] + c->lambda_h * rr[R.xsqh()] +
                             c->lambda_J * unobs[i];
-            HIP_TRY(hipMemcpy(gdev.data(), c->d_gring + (size_t)i * (m + 2) * c->n_slot, c->n_slot * 4,
-                              hipMemcpyDeviceToHost));  // gradient slot m + 1 is the first of the ring
+            HIP_TRY(hipMemcpy(gdev.data(), c->d_gbuf + (size_t)i * 2 * c->n_slot, c->n_slot * 4,
+                              hipMemcpyDeviceToHost));  // gradient buffer 0
             double *g = g_out + (size_t)(k0 + i) * n;
             dev_to_ref(c, r, gdev.data(), g);
             // states never observed at j: gradient 2 lambda_J x (plmDCA_objective.hpp:217)
@@ -1277,9 +1298,9 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
         int32_t *&p;
         ~Guard() { dfree(p); }
     } guard{d_wl};
-    const int m = c->ws_m, P = 2 * m + 3;
+    const int m = c->ws_m;
     const size_t n = (size_t)c->L * c->q * c->q + c->q;
-    const sdca::ResLayout R{P};
+    const sdca::ResLayout R{m};
     const int ng = c->ngroups, gs = c->gslots;
     Solver S(c, nslots);
     std::vector<float> stage(warm ? c->n_slot : 0), xdev(x_out ? c->n_slot : 0);
@@ -1316,8 +1337,8 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
         int32_t *dw = d_wl + (size_t)3 * g * gs;
         HIP_TRY(hipMemcpyAsync(dw, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, st));
         // blocks: into the staged start, or into ring vector m (first search direction)
-        float *XJ = cw->as_direction ? c->d_dring + (size_t)m * c->n_slot : c->d_X0;
-        const int64_t jstride = cw->as_direction ? (int64_t)(m + 1) * c->n_slot : c->n_slot;
+        float *XJ = cw->as_direction ? c->d_dbuf : c->d_X0;
+        const int64_t jstride = c->n_slot;
         for (int i : wl_slot[g]) {
             HIP_TRY(hipMemsetAsync(c->d_X0 + (size_t)i * c->n_slot, 0, c->n_slot * 4, st));
             if (cw->as_direction) HIP_TRY(hipMemsetAsync(XJ + (size_t)i * jstride, 0, c->n_slot * 4, st));

@@ -26,7 +26,6 @@ namespace sdca {
 constexpr int kTile = 128;        // GEMM output tile rows (and narrow tile columns)
 constexpr int kTileWide = 256;    // wide GEMM tile columns; B and K_x are padded to this
 constexpr int kMaxHistory = 16;   // largest L-BFGS history supported
-constexpr int kMaxPhys = 2 * kMaxHistory + 3;  // D ring (m+1) + G ring (m+2)
 constexpr int kMaxGroups = 2;     // slot groups whose rounds are pipelined on separate streams
 
 // BF16 <-> FP32 (round to nearest even; NaN stays NaN).
@@ -40,35 +39,47 @@ __device__ __forceinline__ uint16_t f2bf(float x)
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // Per-slot command of one batched evaluation round (host -> device).
+//
+// L-BFGS state of a slot on the device: iterate x_c and trial point x_t, two gradient
+// buffers (current g_k and trial g_t alternate), the search direction d, and m stored
+// curvature pairs s_p = alpha d, y_p = g_new - g_old (BF16 under split precision, FP32
+// otherwise).
 enum SlotMode : int32_t { MODE_IDLE = 0, MODE_INIT = 1, MODE_RETRY = 2, MODE_NEWDIR = 3 };
 
 struct SlotCmd {
     int32_t mode;     // SlotMode
-    int32_t dst;      // ring index of the direction (RETRY: read, NEWDIR: written)
-    int32_t gt;       // ring index that receives the trial gradient
     int32_t locus;    // locus r of the slot, -1 if idle
     int32_t self_lo;  // columns [self_lo, self_hi) belong to locus r: gradient masked to 0
     int32_t self_hi;
-    int32_t gk;       // NEWDIR: ring index of the gradient at the accepted point
+    int32_t gt;       // gradient buffer (0 or 1) that receives the trial gradient
+    int32_t gk;       // gradient buffer of the current iterate (after NEWDIR's accept); -1 at INIT
     int32_t accept;   // NEWDIR: 1 = the last trial point becomes the iterate (x_c = x_t) first
-    uint32_t live[2]; // bit p: history vector p is referenced (only these are read for products)
-    float alpha;      // step length of the trial point x_t = x_c + alpha * d
+    int32_t pnew;     // NEWDIR: pair slot receiving s = pstep d_old, y = g[gk] - g[1-gk]; -1: none
+    uint32_t live;    // bit p: pair slot p holds a stored pair
+    float alpha;      // step of the trial point x_t = x_c + alpha d
+    float pstep;      // NEWDIR with pnew >= 0: step of the accepted trial
     float init_src;   // MODE_INIT: 1 = copy the staged start point, 0 = zero start
-    float coef[kMaxPhys];  // NEWDIR: d = sum_p coef[p] * ring[p]
+    float cd, cg;     // NEWDIR: d = cd d_old + cg g_k + sum_p (cs[p] s_p + cy[p] y_p)
+    float cs[kMaxHistory], cy[kMaxHistory];
 };
 
-// History vectors of a slot, indexed p in [0, P): directions p <= m, gradients p > m
-// (stored in two rings).
-// Layout of the per-slot result row (device -> host), doubles.
+// Layout of the per-slot result row (device -> host), doubles, for history m.
 struct ResLayout {
-    int P;
-    int gdots() const { return 0; }         // [0, P): g_t . v_p (live p)
-    int ginf() const { return P; }          // max |g_t|
-    int xsqJ() const { return P + 1; }      // sum of squares of x_t, J part
-    int xsqh() const { return P + 2; }      // sum of squares of x_t, h part
-    int fdata() const { return P + 3; }     // sum_b w_b (LSE_b - E_b(x_br))
-    int ddots() const { return P + 4; }     // NEWDIR: [P+4, P+4+(P+1)/2): d . g for gradient p = m+1+k
-    int width() const { return P + 4 + (P + 1) / 2; }
+    int m;
+    int ts(int p) const { return p; }          // g_t . s_p (live p)
+    int ty(int p) const { return m + p; }      // g_t . y_p
+    int tt() const { return 2 * m; }           // g_t . g_t
+    int tg() const { return 2 * m + 1; }       // g_t . g_k
+    int td() const { return 2 * m + 2; }       // g_t . d
+    int ginf() const { return 2 * m + 3; }     // max |g_t|
+    int xsqJ() const { return 2 * m + 4; }     // sum of squares of x_t, J part
+    int xsqh() const { return 2 * m + 5; }     // sum of squares of x_t, h part
+    int dg() const { return 2 * m + 6; }       // NEWDIR: d . g_k
+    int dy(int p) const { return 2 * m + 7 + p; }  // NEWDIR: d . y_p
+    int fdata() const { return 3 * m + 7; }    // sum_b w_b (LSE_b - E_b(x_br))
+    int width() const { return 3 * m + 8; }
+    int ndots() const { return 2 * m + 4; }    // entries produced by the dot-product pass
+    int nupd() const { return m + 3; }         // entries produced by the trial-point pass
 };
 
 struct Timing {
@@ -93,12 +104,14 @@ void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad,
              uint16_t *Glo, double *fpart, int nbchunk);
 void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const void *Xt,
                int Mpad, int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial,
-               float *gring, int64_t n_slot, int m, float lamJ2, float lamh2, int gm, int tn);
-void update(hipStream_t st, int m, int q, const SlotCmd *cmds, int nslots, const float *gring, float *dring,
-            float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0, int64_t n_slot, int Kxp, int Kx,
-            double *part, int nchunk);
-void dots(hipStream_t st, int m, const SlotCmd *cmds, int nslots, const float *gring, const float *dring,
-          int64_t n_slot, double *part, int nchunk);
+               float *gbuf, int64_t n_slot, float lamJ2, float lamh2, int gm, int tn);
+// Slot vectors: gbuf [slot][2][n_slot], dbuf [slot][n_slot], S and Y [slot][m][n_slot]
+// (uint16_t BF16 when pair_bf16, else float).
+void update(hipStream_t st, int m, int q, bool pair_bf16, const SlotCmd *cmds, int nslots, const float *gbuf,
+            float *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0,
+            int64_t n_slot, int Kxp, int Kx, double *part, int nchunk);
+void dots(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int nslots, const float *gbuf,
+          const float *dbuf, const void *S, const void *Y, int64_t n_slot, double *part, int nchunk);
 void reduce_results(hipStream_t st, int m, int nslots, const double *upd_part, const double *dot_part,
                     const double *sm_part, int nchunk, int nbchunk, double *res);
 void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *loci, int nscore, int L,
@@ -191,8 +204,9 @@ struct sdca_ctx {
     int tn_energy = 128, tn_grad = 128;  // output tile widths (autotuned)
     int tuned = 0;                   // 1 if the autotuner chose the GEMM configuration
     int nchunk = 0, nbchunk = 0;
-    float *d_gring = nullptr;   // gradients: slots x (m+2) x n_slot
-    float *d_dring = nullptr;   // directions: slots x (m+1) x n_slot
+    float *d_gbuf = nullptr;    // gradients: slots x 2 x n_slot (current and trial alternate)
+    float *d_dbuf = nullptr;    // search directions: slots x n_slot
+    void *d_S = nullptr, *d_Y = nullptr;  // curvature pairs: slots x m x n_slot (BF16 or float)
     float *d_Xc = nullptr, *d_Xtr = nullptr, *d_X0 = nullptr;
     uint16_t *d_Whi = nullptr, *d_Wlo = nullptr;
     float *d_G32 = nullptr;

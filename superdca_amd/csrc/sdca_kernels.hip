@@ -332,7 +332,7 @@ struct EpiEnergy {
 // Gradient epilogue: row m = slot*q + s, column n.  Adds 2*lambda*x (lambda_h on the
 // bias column, lambda_J elsewhere: plmDCA_objective.hpp:192-244), zeroes the locus'
 // own columns (the j != r exclusions of :95,111,161,176) and pad columns, and writes
-// the slot's trial gradient g_t (gradient ring entry gt - g0).  The commands of the slots under the tile's
+// the slot's trial gradient (gradient buffer gt).  The commands of the slots under the tile's
 // 128 rows are staged in LDS once; each group of four rows first loads its trial-point
 // values, then stores.
 struct EpiGrad {
@@ -489,29 +489,86 @@ __device__ __forceinline__ double dot4(float4 a, float4 b)
     return (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z + (double)a.w * b.w;
 }
 
-// History vectors of a slot: directions p in [0, M] and gradients p in [M+1, 2M+2], in
-// separate rings.
-__device__ __forceinline__ float4 ld4(const float *v, int64_t e) { return *reinterpret_cast<const float4 *>(v + e); }
-__device__ __forceinline__ bool is_live(const SlotCmd &c, int p) { return (c.live[p >> 5] >> (p & 31)) & 1u; }
+// The vector passes move 8 consecutive elements per thread and iteration: two float4 of
+// an FP32 vector, or one 16-byte load of a BF16 curvature pair.  Pair loads widen to
+// float; stores round and hand back the rounded value (every product uses the stored
+// vector).
+struct f8 {
+    float4 a, b;
+};
+__device__ __forceinline__ f8 zero8() { return {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)}; }
+__device__ __forceinline__ f8 ld8(const float *v, int64_t e)
+{
+    return {*reinterpret_cast<const float4 *>(v + e), *reinterpret_cast<const float4 *>(v + e + 4)};
+}
+__device__ __forceinline__ f8 ld8(const uint16_t *v, int64_t e)
+{
+    const u32x4 u = *reinterpret_cast<const u32x4 *>(v + e);
+    auto lo = [](uint32_t w) { return __uint_as_float(w << 16); };
+    auto hi = [](uint32_t w) { return __uint_as_float(w & 0xffff0000u); };
+    return {make_float4(lo(u.x), hi(u.x), lo(u.y), hi(u.y)), make_float4(lo(u.z), hi(u.z), lo(u.w), hi(u.w))};
+}
+__device__ __forceinline__ void st8(float *v, int64_t e, const f8 &x)
+{
+    *reinterpret_cast<float4 *>(v + e) = x.a;
+    *reinterpret_cast<float4 *>(v + e + 4) = x.b;
+}
+__device__ __forceinline__ void st8(float *v, int64_t e, f8 &x, int) { st8(v, e, x); }
+__device__ __forceinline__ void st8(uint16_t *v, int64_t e, f8 &x, int)
+{
+    const float f[8] = {x.a.x, x.a.y, x.a.z, x.a.w, x.b.x, x.b.y, x.b.z, x.b.w};
+    uint32_t w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = (uint32_t)f2bf(f[2 * k]) | ((uint32_t)f2bf(f[2 * k + 1]) << 16);
+    u32x4 u;
+    u.x = w[0];
+    u.y = w[1];
+    u.z = w[2];
+    u.w = w[3];
+    *reinterpret_cast<u32x4 *>(v + e) = u;
+    x = ld8(reinterpret_cast<const uint16_t *>(&u), 0);
+}
+__device__ __forceinline__ f8 lin(float a, const f8 &x, float b, const f8 &y)
+{
+    return {make_float4(a * x.a.x + b * y.a.x, a * x.a.y + b * y.a.y, a * x.a.z + b * y.a.z, a * x.a.w + b * y.a.w),
+            make_float4(a * x.b.x + b * y.b.x, a * x.b.y + b * y.b.y, a * x.b.z + b * y.b.z, a * x.b.w + b * y.b.w)};
+}
+__device__ __forceinline__ void axpy8(f8 &d, float a, const f8 &v)
+{
+    d.a.x += a * v.a.x;
+    d.a.y += a * v.a.y;
+    d.a.z += a * v.a.z;
+    d.a.w += a * v.a.w;
+    d.b.x += a * v.b.x;
+    d.b.y += a * v.b.y;
+    d.b.z += a * v.b.z;
+    d.b.w += a * v.b.w;
+}
+__device__ __forceinline__ double dot8(const f8 &x, const f8 &y) { return dot4(x.a, y.a) + dot4(x.b, y.b); }
+__device__ __forceinline__ float amax8(const f8 &x)
+{
+    return fmaxf(fmaxf(fmaxf(fabsf(x.a.x), fabsf(x.a.y)), fmaxf(fabsf(x.a.z), fabsf(x.a.w))),
+                 fmaxf(fmaxf(fabsf(x.b.x), fabsf(x.b.y)), fmaxf(fabsf(x.b.z), fabsf(x.b.w))));
+}
 
 // Trial point of every slot (one launch per round):
 //   INIT    x_c = x0 (staged) or 0,           x_t = x_c
-//   RETRY   x_t = x_c + alpha d                (d = direction dst)
-//   NEWDIR  x_c = x_t (if accept), d = sum_p coef[p] v_p -> direction dst,
-//           x_t = x_c + alpha d, and d . g for every live gradient g other than the trial
-//           slot gt (the Gram row of the stored d).  Only vectors with a nonzero
-//           coefficient or a needed product are read.
-// plus the squared norms of x_t for the L2 terms: part[slot][chunk][M + 4] = {sum x^2 over
-// the whole vector, sum over the h entries (column Kx of each row), d . g_k for the M+2
-// gradient slots k}.
-template <int M>
-__global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds, const float *__restrict__ gring,
-                                                float *__restrict__ dring, float *__restrict__ Xc, float *__restrict__ Xtr,
+//   RETRY   x_t = x_c + alpha d
+//   NEWDIR  x_c = x_t (if accept); the new pair s = pstep d, y = g[gk] - g[1-gk] into slot
+//           pnew (if any); d <- cd d + cg g_k + sum_p (cs[p] s_p + cy[p] y_p);
+//           x_t = x_c + alpha d; and the products d . g_k, d . y_p (live p) of the stored d.
+//           Pairs with a zero coefficient are read only for their product.
+// plus the squared norms of x_t for the L2 terms: part[slot][chunk][M + 3] = {sum x^2 over
+// the whole vector, sum over the h entries (column Kx of each row), d . g_k, d . y_p}.
+template <int M, class PT>
+__global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds, const float *__restrict__ gbuf,
+                                                float *__restrict__ dbuf, PT *__restrict__ S, PT *__restrict__ Y,
+                                                float *__restrict__ Xc, float *__restrict__ Xtr,
                                                 uint16_t *__restrict__ Whi, uint16_t *__restrict__ Wlo,
                                                 const float *__restrict__ X0, int64_t n_slot, int Kxp, int Kx, int q,
                                                 double *__restrict__ part, int nchunk)
 {
-    constexpr int ND = M + 1, NG = M + 2, NV = 2 + NG;
+    constexpr int NV = M + 3;
     const int i = blockIdx.y;
     const SlotCmd &c = cmds[i];
     double acc[NV];
@@ -522,85 +579,101 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
         is_max[k] = false;
     }
     const int mode = c.mode;
-    const bool dst_ok = mode == MODE_INIT || (c.dst >= 0 && c.dst < ND);
-    if (mode != MODE_IDLE && dst_ok) {
-        const float *gb = gring + (int64_t)i * NG * n_slot;
-        float *db = dring + (int64_t)i * ND * n_slot;
+    const bool ok = mode == MODE_INIT || mode == MODE_RETRY ||
+                    (mode == MODE_NEWDIR && (c.gk == 0 || c.gk == 1) && c.pnew < M);
+    if (mode != MODE_IDLE && ok) {
         float *xc = Xc + (int64_t)i * n_slot;
         float *xt = Xtr + (int64_t)i * n_slot;
+        float *dv = dbuf + (int64_t)i * n_slot;
         const float *x0 = X0 ? X0 + (int64_t)i * n_slot : nullptr;
-        const float alpha = c.alpha;
-        const int dst = c.dst;
-        const int64_t n4 = n_slot >> 2;
-        // the h entry of row s is element Kx % 4 of float4 number (s*Kxp + Kx)/4
-        const int64_t hq0 = Kx >> 2, hstep = Kxp >> 2;
-        const int hlane = Kx & 3;
+        const float *gk = gbuf + ((int64_t)i * 2 + (c.gk == 1)) * n_slot;
+        const float *gold = gbuf + ((int64_t)i * 2 + (c.gk != 1)) * n_slot;
+        PT *Sb = S + (int64_t)i * M * n_slot;
+        PT *Yb = Y + (int64_t)i * M * n_slot;
+        const float alpha = c.alpha, pstep = c.pstep, cd = c.cd, cg = c.cg;
+        const int pnew = c.pnew;
+        const int64_t n8 = n_slot >> 3;
         const bool accept = c.accept != 0;
-        float cf[2 * M + 3];
-        bool dotg[NG];  // product with gradient k wanted
+        const bool from_x0 = c.init_src != 0.f && x0;
+        float cs[M], cy[M];
+        bool live[M];
 #pragma unroll
-        for (int p = 0; p < 2 * M + 3; ++p) cf[p] = c.coef[p];
-#pragma unroll
-        for (int k = 0; k < NG; ++k) dotg[k] = mode == MODE_NEWDIR && is_live(c, ND + k) && ND + k != c.gt;
-        for (int64_t e4 = (int64_t)blockIdx.x * 256 + threadIdx.x; e4 < n4; e4 += (int64_t)nchunk * 256) {
-            const int64_t e = e4 * 4;
-            float4 xn;
+        for (int p = 0; p < M; ++p) {
+            cs[p] = c.cs[p];
+            cy[p] = c.cy[p];
+            live[p] = (c.live >> p) & 1u;
+        }
+        for (int64_t e8 = (int64_t)blockIdx.x * 256 + threadIdx.x; e8 < n8; e8 += (int64_t)nchunk * 256) {
+            const int64_t e = e8 * 8;
+            f8 xn;
             if (mode == MODE_INIT) {
-                xn = (c.init_src != 0.f && x0) ? *reinterpret_cast<const float4 *>(x0 + e) : make_float4(0.f, 0.f, 0.f, 0.f);
-                *reinterpret_cast<float4 *>(xc + e) = xn;
+                xn = from_x0 ? ld8(x0, e) : zero8();
+                st8(xc, e, xn);
             } else if (mode == MODE_RETRY) {
-                const float4 xv = *reinterpret_cast<const float4 *>(xc + e);
-                const float4 d = ld4(db + (int64_t)dst * n_slot, e);
-                xn = make_float4(xv.x + alpha * d.x, xv.y + alpha * d.y, xv.z + alpha * d.z, xv.w + alpha * d.w);
+                xn = lin(1.f, ld8(xc, e), alpha, ld8(dv, e));
             } else {  // NEWDIR
-                const float4 xv = *reinterpret_cast<const float4 *>((accept ? xt : xc) + e);
-                if (accept) *reinterpret_cast<float4 *>(xc + e) = xv;
-                float4 d = make_float4(0.f, 0.f, 0.f, 0.f);
+                const f8 xv = ld8(accept ? xt : xc, e);
+                if (accept) st8(xc, e, xv);
+                const f8 dold = ld8(dv, e);
+                const f8 g = ld8(gk, e);
+                f8 sn = zero8(), yn = zero8();
+                if (pnew >= 0) {
+                    sn = lin(pstep, dold, 0.f, dold);
+                    yn = lin(1.f, g, -1.f, ld8(gold, e));
+                    st8(Sb + (int64_t)pnew * n_slot, e, sn, 0);
+                    st8(Yb + (int64_t)pnew * n_slot, e, yn, 0);
+                }
+                f8 d = lin(cd, dold, cg, g);
 #pragma unroll
-                for (int p = 0; p < ND; ++p) {
-                    if (cf[p] != 0.f) {  // unused vectors are never read
-                        const float4 v = ld4(db + (int64_t)p * n_slot, e);
-                        d.x += cf[p] * v.x;
-                        d.y += cf[p] * v.y;
-                        d.z += cf[p] * v.z;
-                        d.w += cf[p] * v.w;
+                for (int p = 0; p < M; ++p) {
+                    const bool mine = p == pnew;
+                    if (cs[p] != 0.f) {
+                        f8 sv = mine ? zero8() : ld8(Sb + (int64_t)p * n_slot, e);
+                        if (mine) sv = sn;
+                        axpy8(d, cs[p], sv);
+                    }
+                    if (cy[p] != 0.f) {
+                        f8 yv = mine ? zero8() : ld8(Yb + (int64_t)p * n_slot, e);
+                        if (mine) yv = yn;
+                        axpy8(d, cy[p], yv);
                     }
                 }
-                float4 gv[NG];
+                st8(dv, e, d);
+                acc[2] += dot8(d, g);
+                // products with the pairs: the y chunks were just read by this thread and
+                // are re-read from cache rather than held in registers
 #pragma unroll
-                for (int k = 0; k < NG; ++k) {
-                    const float w = cf[ND + k];
-                    gv[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (w != 0.f || dotg[k]) gv[k] = *reinterpret_cast<const float4 *>(gb + (int64_t)k * n_slot + e);
-                    if (w != 0.f) {
-                        d.x += w * gv[k].x;
-                        d.y += w * gv[k].y;
-                        d.z += w * gv[k].z;
-                        d.w += w * gv[k].w;
-                    }
+                for (int p = 0; p < M; ++p) {
+                    if (!live[p]) continue;
+                    f8 yv = p == pnew ? zero8() : ld8(Yb + (int64_t)p * n_slot, e);
+                    if (p == pnew) yv = yn;
+                    acc[3 + p] += dot8(d, yv);
                 }
-                *reinterpret_cast<float4 *>(db + (int64_t)dst * n_slot + e) = d;
-#pragma unroll
-                for (int k = 0; k < NG; ++k)
-                    if (dotg[k]) acc[2 + k] += dot4(d, gv[k]);
-                xn = make_float4(xv.x + alpha * d.x, xv.y + alpha * d.y, xv.z + alpha * d.z, xv.w + alpha * d.w);
+                xn = lin(1.f, xv, alpha, d);
             }
-            *reinterpret_cast<float4 *>(xt + e) = xn;
+            st8(xt, e, xn);
+            const float xs[8] = {xn.a.x, xn.a.y, xn.a.z, xn.a.w, xn.b.x, xn.b.y, xn.b.z, xn.b.w};
             if (Whi) {
-                const float xs[4] = {xn.x, xn.y, xn.z, xn.w};
-                uint16_t h[4], l[4];
+                uint32_t h[4], l[4];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    h[u] = f2bf(xs[u]);
-                    l[u] = f2bf(xs[u] - bf2f(h[u]));
+                for (int k = 0; k < 4; ++k) {
+                    const uint16_t h0 = f2bf(xs[2 * k]), h1 = f2bf(xs[2 * k + 1]);
+                    const uint16_t l0 = f2bf(xs[2 * k] - bf2f(h0)), l1 = f2bf(xs[2 * k + 1] - bf2f(h1));
+                    h[k] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+                    l[k] = (uint32_t)l0 | ((uint32_t)l1 << 16);
                 }
-                *reinterpret_cast<ushort4 *>(Whi + (int64_t)i * n_slot + e) = make_ushort4(h[0], h[1], h[2], h[3]);
-                *reinterpret_cast<ushort4 *>(Wlo + (int64_t)i * n_slot + e) = make_ushort4(l[0], l[1], l[2], l[3]);
+                u32x4 hv, lv;
+                hv.x = h[0], hv.y = h[1], hv.z = h[2], hv.w = h[3];
+                lv.x = l[0], lv.y = l[1], lv.z = l[2], lv.w = l[3];
+                *reinterpret_cast<u32x4 *>(Whi + (int64_t)i * n_slot + e) = hv;
+                *reinterpret_cast<u32x4 *>(Wlo + (int64_t)i * n_slot + e) = lv;
             }
-            acc[0] += (double)xn.x * xn.x + (double)xn.y * xn.y + (double)xn.z * xn.z + (double)xn.w * xn.w;
-            const int64_t rel = e4 - hq0;
-            if (rel >= 0 && rel % hstep == 0 && rel / hstep < q) {
-                const float hv = hlane == 0 ? xn.x : hlane == 1 ? xn.y : hlane == 2 ? xn.z : xn.w;
+            acc[0] += dot8(xn, xn);
+            // the h entry of row s is column Kx (K_x is padded to a multiple of 8, so an
+            // 8-element chunk lies in one row)
+            const int64_t row = e / Kxp, col = e - row * Kxp;
+            if (row < q && Kx >= col && Kx < col + 8) {
+                const float hv = xs[Kx - col];
                 acc[1] += (double)hv * hv;
             }
         }
@@ -608,14 +681,16 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
     block_reduce_store<NV>(acc, part + ((int64_t)i * nchunk + blockIdx.x) * NV, is_max);
 }
 
-// Dot products of each active slot's trial gradient g_t with every live history vector
-// (the others are not read), and its infinity norm: part[slot][chunk][P + 1].
-template <int M>
-__global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, const float *__restrict__ gring,
-                                              const float *__restrict__ dring, int64_t n_slot, double *__restrict__ part,
+// Products of each active slot's trial gradient g_t with the stored pairs, itself, the
+// current gradient and the direction, and its infinity norm:
+// part[slot][chunk][2M + 4] = {g_t.s_p, g_t.y_p, g_t.g_t, g_t.g_k, g_t.d, max |g_t|}.
+template <int M, class PT>
+__global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, const float *__restrict__ gbuf,
+                                              const float *__restrict__ dbuf, const PT *__restrict__ S,
+                                              const PT *__restrict__ Y, int64_t n_slot, double *__restrict__ part,
                                               int nchunk)
 {
-    constexpr int ND = M + 1, NG = M + 2, P = 2 * M + 3, NV = P + 1;
+    constexpr int NV = 2 * M + 4;
     const int i = blockIdx.y;
     const SlotCmd &c = cmds[i];
     double acc[NV];
@@ -623,73 +698,67 @@ __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, 
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         acc[k] = 0.0;
-        is_max[k] = (k == P);
+        is_max[k] = (k == 2 * M + 3);
     }
-    if (c.mode != MODE_IDLE && c.gt >= ND && c.gt < P) {
-        const float *gb = gring + (int64_t)i * NG * n_slot;
-        const float *db = dring + (int64_t)i * ND * n_slot;
-        const int gt = c.gt;
-        const float *g = gb + (int64_t)(gt - ND) * n_slot;
-        const uint32_t l0 = c.live[0], l1 = c.live[1];
-        auto live = [&](int p) -> bool { return p < 32 ? (l0 >> p) & 1u : (l1 >> (p - 32)) & 1u; };
-        double self = 0.0;  // g_t . g_t
-        const int64_t n4 = n_slot >> 2;
-        for (int64_t e4 = (int64_t)blockIdx.x * 256 + threadIdx.x; e4 < n4; e4 += (int64_t)nchunk * 256) {
-            const int64_t e = e4 * 4;
-            const float4 gv = *reinterpret_cast<const float4 *>(g + e);
-            acc[P] = fmax(acc[P], (double)fmaxf(fmaxf(fabsf(gv.x), fabsf(gv.y)), fmaxf(fabsf(gv.z), fabsf(gv.w))));
+    if (c.mode != MODE_IDLE && (c.gt == 0 || c.gt == 1)) {
+        const float *g = gbuf + ((int64_t)i * 2 + c.gt) * n_slot;
+        const bool cur = c.mode != MODE_INIT;  // a current gradient and a direction exist
+        const float *gk = gbuf + ((int64_t)i * 2 + (1 - c.gt)) * n_slot;
+        const float *dv = dbuf + (int64_t)i * n_slot;
+        const PT *Sb = S + (int64_t)i * M * n_slot;
+        const PT *Yb = Y + (int64_t)i * M * n_slot;
+        const uint32_t live = c.live;
+        const int64_t n8 = n_slot >> 3;
+        for (int64_t e8 = (int64_t)blockIdx.x * 256 + threadIdx.x; e8 < n8; e8 += (int64_t)nchunk * 256) {
+            const int64_t e = e8 * 8;
+            const f8 gv = ld8(g, e);
+            acc[2 * M + 3] = fmax(acc[2 * M + 3], (double)amax8(gv));
+            acc[2 * M] += dot8(gv, gv);
+            if (cur) {
+                acc[2 * M + 1] += dot8(gv, ld8(gk, e));
+                acc[2 * M + 2] += dot8(gv, ld8(dv, e));
+            }
 #pragma unroll
-            for (int p = 0; p < ND; ++p)
-                if (live(p)) acc[p] += dot4(gv, ld4(db + (int64_t)p * n_slot, e));
-            self += dot4(gv, gv);
-#pragma unroll
-            for (int k = 0; k < NG; ++k) {
-                if (!live(ND + k) || ND + k == gt) continue;
-                acc[ND + k] += dot4(gv, *reinterpret_cast<const float4 *>(gb + (int64_t)k * n_slot + e));
+            for (int p = 0; p < M; ++p) {
+                if (!((live >> p) & 1u)) continue;
+                acc[p] += dot8(gv, ld8(Sb + (int64_t)p * n_slot, e));
+                acc[M + p] += dot8(gv, ld8(Yb + (int64_t)p * n_slot, e));
             }
         }
-#pragma unroll
-        for (int k = 0; k < NG; ++k)
-            if (ND + k == gt) acc[ND + k] = self;
     }
     block_reduce_store<NV>(acc, part + ((int64_t)i * nchunk + blockIdx.x) * NV, is_max);
 }
 
-// Per-slot reduction of the chunk partials into the result row (ResLayout):
-// [0, P) g_t . v_p, P max |g_t|, P+1 sum x_t^2 (J part), P+2 (h part), P+3 data term,
-// [P+4, P+4+NG) d . g_k of a new direction.
-__global__ void k_reduce(int P, const double *__restrict__ upd_part, const double *__restrict__ dot_part,
+// Per-slot reduction of the chunk partials into the result row (ResLayout): the dots
+// pass fills [0, 2m+4), the trial-point pass [2m+4, 3m+7) (two squares, d.g_k, d.y_p;
+// the first square is the whole vector, stored as the J part), the softmax the data term.
+__global__ void k_reduce(int m, const double *__restrict__ upd_part, const double *__restrict__ dot_part,
                          const double *__restrict__ sm_part, int nchunk, int nbchunk, double *__restrict__ res)
 {
     const int i = blockIdx.x;
     const int t = threadIdx.x;
-    const int NG = (P + 1) / 2, NU = 2 + NG, W = P + 4 + NG;
+    const int ND = 2 * m + 4, NU = m + 3, W = 3 * m + 8;
     double *out = res + (int64_t)i * W;
-    if (t < P + 1) {  // dots kernel: gdots then ginf
+    if (t < ND) {
+        const bool mx = t == ND - 1;
         double a = 0.0;
         for (int ch = 0; ch < nchunk; ++ch) {
-            const double v = dot_part[((int64_t)i * nchunk + ch) * (P + 1) + t];
-            a = (t == P) ? fmax(a, v) : a + v;
+            const double v = dot_part[((int64_t)i * nchunk + ch) * ND + t];
+            a = mx ? fmax(a, v) : a + v;
         }
         out[t] = a;
-    } else if (t == P + 1) {  // update kernel: squares (all, h)
-        double all = 0.0, h = 0.0;
-        for (int ch = 0; ch < nchunk; ++ch) {
-            all += upd_part[((int64_t)i * nchunk + ch) * NU];
-            h += upd_part[((int64_t)i * nchunk + ch) * NU + 1];
-        }
-        out[P + 1] = all - h;
-        out[P + 2] = h;
-    } else if (t == P + 3) {
+    } else if (t < ND + NU) {
+        const int u = t - ND;
+        double a = 0.0;
+        for (int ch = 0; ch < nchunk; ++ch) a += upd_part[((int64_t)i * nchunk + ch) * NU + u];
+        out[t] = a;
+    } else if (t == W - 1) {
         double a = 0.0;
         for (int ch = 0; ch < nbchunk; ++ch) a += sm_part[(int64_t)i * nbchunk + ch];
         out[t] = a;
-    } else if (t >= P + 4 && t < W) {  // update kernel: products of the new direction
-        const int k = t - (P + 4);
-        double a = 0.0;
-        for (int ch = 0; ch < nchunk; ++ch) a += upd_part[((int64_t)i * nchunk + ch) * NU + 2 + k];
-        out[t] = a;
     }
+    __syncthreads();
+    if (t == 0) out[ND] -= out[ND + 1];  // J part = all - h
 }
 
 // ---------------------------------------------------------------- scoring
@@ -842,10 +911,10 @@ void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad,
 }
 
 void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const void *Xt, int Mpad,
-               int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial, float *gring,
-               int64_t n_slot, int m, float lamJ2, float lamh2, int gm, int tn)
+               int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial, float *gbuf,
+               int64_t n_slot, float lamJ2, float lamh2, int gm, int tn)
 {
-    EpiGrad epi{gring, Xtrial, cmds, n_slot, (int64_t)Kxp, m + 2, m + 1, q, Kx, nslots, lamJ2, lamh2};
+    EpiGrad epi{gbuf, Xtrial, cmds, n_slot, (int64_t)Kxp, 2, 0, q, Kx, nslots, lamJ2, lamh2};
     const TileMap tm = make_map(Mpad / kTile, Kxp / tn, 1, gm);
     if (prec == SDCA_PREC_FP32)
         launch_gemm<PolF32, 1>(st, tn, tm, G32, G32, Bp, Xt, Bp, Bp, Bp, epi);
@@ -863,22 +932,31 @@ void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const
     default: break;              \
     }
 
-void update(hipStream_t st, int m, int q, const SlotCmd *cmds, int nslots, const float *gring, float *dring,
-            float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0, int64_t n_slot, int Kxp, int Kx,
-            double *part, int nchunk)
+void update(hipStream_t st, int m, int q, bool pair_bf16, const SlotCmd *cmds, int nslots, const float *gbuf,
+            float *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0,
+            int64_t n_slot, int Kxp, int Kx, double *part, int nchunk)
 {
-#define SDCA_UPD(MM)                                                                                                  \
-    hipLaunchKernelGGL(k_update<MM>, dim3(nchunk, nslots), dim3(256), 0, st, cmds, gring, dring, Xc, Xtr, Whi, Wlo, \
-                       X0, n_slot, Kxp, Kx, q, part, nchunk)
+#define SDCA_UPD(MM)                                                                                                \
+    if (pair_bf16)                                                                                                  \
+        hipLaunchKernelGGL((k_update<MM, uint16_t>), dim3(nchunk, nslots), dim3(256), 0, st, cmds, gbuf, dbuf,    \
+                           (uint16_t *)S, (uint16_t *)Y, Xc, Xtr, Whi, Wlo, X0, n_slot, Kxp, Kx, q, part, nchunk); \
+    else                                                                                                            \
+        hipLaunchKernelGGL((k_update<MM, float>), dim3(nchunk, nslots), dim3(256), 0, st, cmds, gbuf, dbuf,       \
+                           (float *)S, (float *)Y, Xc, Xtr, Whi, Wlo, X0, n_slot, Kxp, Kx, q, part, nchunk)
     SDCA_DISPATCH_M(m, SDCA_UPD)
 #undef SDCA_UPD
 }
 
-void dots(hipStream_t st, int m, const SlotCmd *cmds, int nslots, const float *gring, const float *dring,
-          int64_t n_slot, double *part, int nchunk)
+void dots(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int nslots, const float *gbuf,
+          const float *dbuf, const void *S, const void *Y, int64_t n_slot, double *part, int nchunk)
 {
-#define SDCA_DOT(MM)                                                                                                 \
-    hipLaunchKernelGGL(k_dots<MM>, dim3(nchunk, nslots), dim3(256), 0, st, cmds, gring, dring, n_slot, part, nchunk)
+#define SDCA_DOT(MM)                                                                                                \
+    if (pair_bf16)                                                                                                  \
+        hipLaunchKernelGGL((k_dots<MM, uint16_t>), dim3(nchunk, nslots), dim3(256), 0, st, cmds, gbuf, dbuf,      \
+                           (const uint16_t *)S, (const uint16_t *)Y, n_slot, part, nchunk);                         \
+    else                                                                                                            \
+        hipLaunchKernelGGL((k_dots<MM, float>), dim3(nchunk, nslots), dim3(256), 0, st, cmds, gbuf, dbuf,         \
+                           (const float *)S, (const float *)Y, n_slot, part, nchunk)
     SDCA_DISPATCH_M(m, SDCA_DOT)
 #undef SDCA_DOT
 }
@@ -886,9 +964,8 @@ void dots(hipStream_t st, int m, const SlotCmd *cmds, int nslots, const float *g
 void reduce_results(hipStream_t st, int m, int nslots, const double *upd_part, const double *dot_part,
                     const double *sm_part, int nchunk, int nbchunk, double *res)
 {
-    const int P = 2 * m + 3;
-    const int W = P + 4 + (m + 2);
-    hipLaunchKernelGGL(k_reduce, dim3(nslots), dim3((W + 63) / 64 * 64), 0, st, P, upd_part, dot_part, sm_part, nchunk,
+    const int W = 3 * m + 8;
+    hipLaunchKernelGGL(k_reduce, dim3(nslots), dim3((W + 63) / 64 * 64), 0, st, m, upd_part, dot_part, sm_part, nchunk,
                        nbchunk, res);
 }
 

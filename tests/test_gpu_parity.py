@@ -238,6 +238,7 @@ def test_gemm_configurations_match_oracle(c1, cfg, precision, ftol, gtol):
 def test_pipelined_slot_groups_match_single_group(c1, precision):
     # 80 loci through 64 slots: two groups of 32 slots (128 GEMM rows each) alternate on two
     # streams, with refills in both; every locus must reach the same optimum as with one group
+    # (and the two-group layout must stay usable although one group is the default)
     st = c1["states"]
     w, _ = orc.weights(st, 0.9, nthreads=8)
     loci = np.arange(0, 1000, 12)[:80]
