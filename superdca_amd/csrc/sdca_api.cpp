@@ -99,7 +99,7 @@ void dfree(T *&p)
 
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
-bool valid_history(int m) { return m == 1 || m == 3 || m == 5 || m == 10 || m == 16; }
+bool valid_history(int m) { return m == 1 || m == 3 || m == 5 || m == 8 || m == 10 || m == 16; }
 
 }  // namespace
 
@@ -1029,7 +1029,7 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
         return fail(SDCA_E_ARG, "pipeline_groups must be 0, 1 or 2");
     if (o.precision != SDCA_PREC_FP32 && o.precision != SDCA_PREC_SPLIT_BF16)
         return fail(SDCA_E_ARG, "unknown precision %d", o.precision);
-    if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 10, 16 (got %d)", o.history);
+    if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 8, 10, 16 (got %d)", o.history);
     if (o.batch_loci < 1) return fail(SDCA_E_ARG, "batch_loci must be >= 1");
     if (o.max_iter < 1 || o.max_ls < 1) return fail(SDCA_E_ARG, "max_iter and max_ls must be >= 1");
     if (!(o.grad_threshold >= 0.0)) return fail(SDCA_E_ARG, "grad_threshold must be >= 0");
@@ -1094,16 +1094,15 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
     if ((rc = dalloc(&c->d_statesT, (size_t)B * L))) return bail(rc);
     if ((rc = dalloc(&c->d_state_col, (size_t)L * q))) return bail(rc);
     if ((rc = dalloc(&c->d_w, (size_t)c->Bp))) return bail(rc);
-    const size_t esz = o.precision == SDCA_PREC_FP32 ? 4 : 2;
-    if ((rc = dalloc((uint8_t **)&c->d_X, (size_t)c->Bp * c->Kxp * esz))) return bail(rc);
-    if ((rc = dalloc((uint8_t **)&c->d_Xt, (size_t)c->Bp * c->Kxp * esz))) return bail(rc);
+    if ((rc = dalloc(&c->d_X, (size_t)c->Bp * (c->Kxp / 32)))) return bail(rc);
+    if ((rc = dalloc(&c->d_Xt, (size_t)c->Kxp * (c->Bp / 32)))) return bail(rc);
     if (hipMemcpy(c->d_states, states, (size_t)B * L, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d_statesT, sT.data(), (size_t)B * L, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->d_state_col, c->state_col.data(), c->state_col.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(SDCA_E_HIP, "upload of the alignment failed"));
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(SDCA_E_HIP, "zero fill of X failed"));
-    sdca::kern::build_x(c->stream, c->d_states, B, L, q, c->d_state_col, (int)c->Kx, c->Bp, (int)c->Kxp, o.precision,
-                        c->d_X, c->d_Xt);
+    sdca::kern::build_x(c->stream, c->d_states, B, L, q, c->d_state_col, (int)c->Kx, c->Bp, (int)c->Kxp, c->d_X,
+                        c->d_Xt);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
         return bail(fail(SDCA_E_HIP, "building X failed"));
     *out = c;

@@ -92,17 +92,17 @@ struct Timing {
 // Kernel launchers (sdca_kernels.hip).
 namespace kern {
 void build_x(hipStream_t st, const uint8_t *d_states, int B, int L, int q, const int32_t *d_state_col,
-             int Kx, int Bp, int Kxp, int prec, void *X, void *Xt);
+             int Kx, int Bp, int Kxp, uint32_t *X, uint32_t *Xt);
 void bitplanes(hipStream_t st, const uint8_t *d_states, int B, int L, int q, int W, uint64_t *planes);
 void hamming_neighbours(hipStream_t st, const uint64_t *planes, int B, int q, int W, int L, int64_t dmax,
                         const uint32_t *d_mult, unsigned long long *d_iw);
 int energy_superiles(int Mpad, int Bp, int gm, int tn);  // super-tiles per split-K slice
-void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const void *X,
+void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const uint32_t *X,
                  int Mpad, int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn, float *Epart);
 void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad, int Bp, int B, int q, int nslots,
              const SlotCmd *cmds, const uint8_t *statesT, const float *wts, float *G32, uint16_t *Ghi,
              uint16_t *Glo, double *fpart, int nbchunk);
-void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const void *Xt,
+void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const uint32_t *Xt,
                int Mpad, int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial,
                float *gbuf, int64_t n_slot, float lamJ2, float lamh2, int gm, int tn);
 // Slot vectors: gbuf [slot][2][n_slot], dbuf [slot][n_slot], S and Y [slot][m][n_slot]
@@ -187,7 +187,7 @@ struct sdca_ctx {
 
     uint8_t *d_states = nullptr, *d_statesT = nullptr;
     int32_t *d_state_col = nullptr;
-    void *d_X = nullptr, *d_Xt = nullptr;
+    uint32_t *d_X = nullptr, *d_Xt = nullptr;  // one-hot X (Bp x Kxp) and X^T as bits
     float *d_w = nullptr;
 
     bool have_weights = false;

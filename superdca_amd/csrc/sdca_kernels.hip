@@ -25,11 +25,11 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------- one-hot X
 
-// X[b][state_col[j*q + x_bj]] = 1 and the bias column X[b][Kx] = 1 (b < B);
-// the caller zero-fills X and Xt first.
-template <typename T>
+// One-hot matrices as bits: X[b][col] set for col = state_col[j*q + x_bj] and the bias
+// column Kx (b < B), rows of Kxp/32 words; Xt[col][b] likewise, rows of Bp/32 words.
+// The caller zero-fills both first.
 __global__ void k_build_x(const uint8_t *__restrict__ states, int B, int L, int q, const int32_t *__restrict__ state_col,
-                          int Kx, int Bp, int64_t Kxp, T *__restrict__ X, T *__restrict__ Xt, T one)
+                          int Kx, int Bp, int64_t Kxp, uint32_t *__restrict__ X, uint32_t *__restrict__ Xt)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t total = (int64_t)B * (L + 1);
@@ -43,8 +43,8 @@ __global__ void k_build_x(const uint8_t *__restrict__ states, int B, int L, int 
         const int a = states[(int64_t)b * L + j];
         col = state_col[(int64_t)j * q + a];
     }
-    X[(int64_t)b * Kxp + col] = one;
-    Xt[col * Bp + b] = one;
+    atomicOr(&X[(int64_t)b * (Kxp >> 5) + (col >> 5)], 1u << (col & 31));
+    atomicOr(&Xt[col * (Bp >> 5) + (b >> 5)], 1u << (b & 31));
 }
 
 // ------------------------------------------------------------ reweighting
@@ -138,11 +138,14 @@ __global__ __launch_bounds__(256) void k_hamming(const uint64_t *__restrict__ pl
 
 // ------------------------------------------------------------------- GEMM
 
-// C[m][n] = sum_k A[m][k] * Bm[n][k] on 256-thread workgroups of four waves (2 x 2).
-// A tile is 128 rows of A by TN = 64 * NI rows of Bm; each wave computes 64 x 32*NI
+// C[m][n] = sum_k A[m][k] * B[n][k] on 256-thread workgroups of four waves (2 x 2).
+// A tile is 128 rows of A by TN = 64 * NI rows of B; each wave computes 64 x 32*NI
 // outputs as 2 x NI MFMA 32x32 blocks, with K staged through LDS in rows of 128 bytes.
 // NA = 2 sums two A variants (hi and lo BF16 parts) against the same B fragment.  The
 // wide tile (NI = 4) reads 0.5 LDS fragments per MFMA instead of 0.75 at NA = 2.
+// B is a 0/1 matrix held as bits (one-hot X or its transpose); each thread loads the
+// bits of its share of a K slice and expands them to 1.0 / 0.0 while staging to LDS,
+// so B costs 1/16 (BF16) or 1/32 (FP32) of its element traffic.
 constexpr int ROWB = 144;  // LDS bytes per staged row (128 data + 16 pad)
 
 struct PolBF16 {
@@ -242,7 +245,7 @@ __device__ __forceinline__ bool map_tile(const TileMap &tm, int bid, int &mt, in
 template <class Pol, int NA, int NI, class Epi>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_nt(const typename Pol::T *__restrict__ A0,
                                                     const typename Pol::T *__restrict__ A1, int64_t lda,
-                                                    const typename Pol::T *__restrict__ Bm, int64_t ldb, int K,
+                                                    const uint32_t *__restrict__ Bbits, int64_t ldbw, int K,
                                                     int kchunk, TileMap tm, Epi epi)
 {
     constexpr int TN = 64 * NI;  // B rows (output columns) per tile
@@ -266,23 +269,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     const int lcol = (tid & 7) * 16;   // byte offset in the 128-byte row
     const uint8_t *pa0 = reinterpret_cast<const uint8_t *>(A0 + (m0 + lrow) * lda) + lcol;
     const uint8_t *pa1 = reinterpret_cast<const uint8_t *>(A1 + (m0 + lrow) * lda) + lcol;
-    const uint8_t *pb = reinterpret_cast<const uint8_t *>(Bm + (n0 + lrow) * ldb) + lcol;
     const int64_t sa = 32 * lda * (int64_t)sizeof(typename Pol::T);  // 32 rows, in bytes
-    const int64_t sbb = 32 * ldb * (int64_t)sizeof(typename Pol::T);
     constexpr int EB = (int)sizeof(typename Pol::T);
-    constexpr int RB = TN / 32;  // 32-row passes over the B tile
     constexpr int BK = Pol::BK;
+    // B bits: EPT elements of one row per thread and K slice (TPR threads per row)
+    constexpr int EPT = TN * BK / 256, TPR = BK / EPT, NW = EPT >= 32 ? EPT / 32 : 1;
+    static_assert(EPT == 16 || EPT == 32 || EPT == 64, "B bits per thread");
+    const int brow = tid / TPR, bseg = tid - brow * TPR;
+    const uint32_t *pbw = Bbits + (n0 + brow) * ldbw;
+    uint8_t *sBrow = sB + brow * ROWB + bseg * EPT * EB;
     // software pipeline: the next K slice is in flight from memory while the current one
     // is multiplied out of LDS
-    u32x4 ra[NA][4], rb[RB];
+    u32x4 ra[NA][4];
+    uint32_t rbw[NW];
     auto load = [&](int k0) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             ra[0][i] = *reinterpret_cast<const u32x4 *>(pa0 + i * sa + (int64_t)k0 * EB);
             if constexpr (NA == 2) ra[NA - 1][i] = *reinterpret_cast<const u32x4 *>(pa1 + i * sa + (int64_t)k0 * EB);
         }
-#pragma unroll
-        for (int i = 0; i < RB; ++i) rb[i] = *reinterpret_cast<const u32x4 *>(pb + i * sbb + (int64_t)k0 * EB);
+        const int bit0 = k0 + bseg * EPT;
+        if constexpr (NW == 2) {
+            const uint2 w = *reinterpret_cast<const uint2 *>(pbw + (bit0 >> 5));
+            rbw[0] = w.x;
+            rbw[NW - 1] = w.y;
+        } else {
+            rbw[0] = pbw[bit0 >> 5] >> (bit0 & 31);
+        }
     };
     if (kb < ke) load(kb);
     for (int k0 = kb; k0 < ke; k0 += BK) {
@@ -293,8 +306,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int v = 0; v < NA; ++v) *reinterpret_cast<u32x4 *>(&sA[v][row * ROWB + lcol]) = ra[v][i];
         }
+        // expand the B bits: 16-byte chunks of 8 BF16 or 4 FP32 ones and zeros
+        {
+            constexpr int EPC = 16 / EB;  // elements per chunk
 #pragma unroll
-        for (int i = 0; i < RB; ++i) *reinterpret_cast<u32x4 *>(&sB[(lrow + 32 * i) * ROWB + lcol]) = rb[i];
+            for (int ch = 0; ch < EPT / EPC; ++ch) {
+                const int b0 = ch * EPC;
+                const uint32_t w = rbw[b0 >> 5] >> (b0 & 31);
+                u32x4 v;
+                if constexpr (EB == 2) {
+                    v.x = (w & 1u) * 0x3f80u + ((w >> 1) & 1u) * 0x3f800000u;
+                    v.y = ((w >> 2) & 1u) * 0x3f80u + ((w >> 3) & 1u) * 0x3f800000u;
+                    v.z = ((w >> 4) & 1u) * 0x3f80u + ((w >> 5) & 1u) * 0x3f800000u;
+                    v.w = ((w >> 6) & 1u) * 0x3f80u + ((w >> 7) & 1u) * 0x3f800000u;
+                } else {
+                    v.x = (w & 1u) * 0x3f800000u;
+                    v.y = ((w >> 1) & 1u) * 0x3f800000u;
+                    v.z = ((w >> 2) & 1u) * 0x3f800000u;
+                    v.w = ((w >> 3) & 1u) * 0x3f800000u;
+                }
+                *reinterpret_cast<u32x4 *>(sBrow + ch * 16) = v;
+            }
+        }
         __syncthreads();
         if (k0 + BK < ke) load(k0 + BK);
         mma_tile<Pol, NA, NI>(sA[0], sA[NA - 1], sB, wm, wn, lane, acc);
@@ -827,15 +860,11 @@ namespace kern {
 static inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 void build_x(hipStream_t st, const uint8_t *d_states, int B, int L, int q, const int32_t *d_state_col, int Kx,
-             int Bp, int Kxp, int prec, void *X, void *Xt)
+             int Bp, int Kxp, uint32_t *X, uint32_t *Xt)
 {
     const int64_t total = (int64_t)B * (L + 1);
-    if (prec == SDCA_PREC_FP32)
-        hipLaunchKernelGGL(k_build_x<float>, dim3(cdiv(total, 256)), dim3(256), 0, st, d_states, B, L, q, d_state_col,
-                           Kx, Bp, (int64_t)Kxp, (float *)X, (float *)Xt, 1.0f);
-    else
-        hipLaunchKernelGGL(k_build_x<uint16_t>, dim3(cdiv(total, 256)), dim3(256), 0, st, d_states, B, L, q,
-                           d_state_col, Kx, Bp, (int64_t)Kxp, (uint16_t *)X, (uint16_t *)Xt, (uint16_t)0x3f80);
+    hipLaunchKernelGGL(k_build_x, dim3(cdiv(total, 256)), dim3(256), 0, st, d_states, B, L, q, d_state_col, Kx, Bp,
+                       (int64_t)Kxp, X, Xt);
 }
 
 void bitplanes(hipStream_t st, const uint8_t *d_states, int B, int L, int q, int W, uint64_t *planes)
@@ -878,27 +907,27 @@ int energy_superiles(int Mpad, int Bp, int gm, int tn)
 // Launches k_gemm_nt with output tiles 128 x tn (tn = 128 or 256; N must be a multiple).
 template <class Pol, int NA, class Epi>
 static void launch_gemm(hipStream_t st, int tn, const TileMap &tm, const void *A0, const void *A1, int64_t lda,
-                        const void *Bm, int64_t ldb, int K, int kchunk, const Epi &epi)
+                        const uint32_t *Bbits, int64_t ldbw, int K, int kchunk, const Epi &epi)
 {
     using T = typename Pol::T;
     const dim3 grid(map_grid(tm));
     if (tn == 256)
         hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 4, Epi>), grid, dim3(256), 0, st, (const T *)A0, (const T *)A1, lda,
-                           (const T *)Bm, ldb, K, kchunk, tm, epi);
+                           Bbits, ldbw, K, kchunk, tm, epi);
     else
         hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 2, Epi>), grid, dim3(256), 0, st, (const T *)A0, (const T *)A1, lda,
-                           (const T *)Bm, ldb, K, kchunk, tm, epi);
+                           Bbits, ldbw, K, kchunk, tm, epi);
 }
 
-void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const void *X, int Mpad,
+void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const uint32_t *X, int Mpad,
                  int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn, float *Epart)
 {
     EpiEnergy epi{Epart, (int64_t)Bp, (int64_t)Mpad * Bp};
     const TileMap tm = make_map(Mpad / kTile, Bp / tn, ksplit, gm);
     if (prec == SDCA_PREC_FP32)
-        launch_gemm<PolF32, 1>(st, tn, tm, W32, W32, Kxp, X, Kxp, Kxp, kchunk, epi);
+        launch_gemm<PolF32, 1>(st, tn, tm, W32, W32, Kxp, X, Kxp / 32, Kxp, kchunk, epi);
     else
-        launch_gemm<PolBF16, 2>(st, tn, tm, Whi, Wlo, Kxp, X, Kxp, Kxp, kchunk, epi);
+        launch_gemm<PolBF16, 2>(st, tn, tm, Whi, Wlo, Kxp, X, Kxp / 32, Kxp, kchunk, epi);
 }
 
 void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad, int Bp, int B, int q, int nslots,
@@ -910,16 +939,16 @@ void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad,
                        nbchunk);
 }
 
-void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const void *Xt, int Mpad,
+void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const uint32_t *Xt, int Mpad,
                int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial, float *gbuf,
                int64_t n_slot, float lamJ2, float lamh2, int gm, int tn)
 {
     EpiGrad epi{gbuf, Xtrial, cmds, n_slot, (int64_t)Kxp, 2, 0, q, Kx, nslots, lamJ2, lamh2};
     const TileMap tm = make_map(Mpad / kTile, Kxp / tn, 1, gm);
     if (prec == SDCA_PREC_FP32)
-        launch_gemm<PolF32, 1>(st, tn, tm, G32, G32, Bp, Xt, Bp, Bp, Bp, epi);
+        launch_gemm<PolF32, 1>(st, tn, tm, G32, G32, Bp, Xt, Bp / 32, Bp, Bp, epi);
     else
-        launch_gemm<PolBF16, 2>(st, tn, tm, Ghi, Glo, Bp, Xt, Bp, Bp, Bp, epi);
+        launch_gemm<PolBF16, 2>(st, tn, tm, Ghi, Glo, Bp, Xt, Bp / 32, Bp, Bp, epi);
 }
 
 #define SDCA_DISPATCH_M(m, CALL) \
@@ -927,6 +956,7 @@ void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const
     case 1: CALL(1); break;      \
     case 3: CALL(3); break;      \
     case 5: CALL(5); break;      \
+    case 8: CALL(8); break;      \
     case 10: CALL(10); break;    \
     case 16: CALL(16); break;    \
     default: break;              \
