@@ -84,19 +84,20 @@ def device_ms(tm):
                                          "score_ms")}
 
 
-def train_network(seed=0):
-    """The warm-start network, trained on cold-start solutions of the C1 and C2 workloads."""
+def train_network(q, seed=0):
+    """The warm-start network, trained on cold-start solutions of the C1 and C2 workloads
+    (prepared with the same state envelope q as the benchmarked alignment)."""
     import superdca_amd as sd
     from superdca_amd import synth, warmstart
 
     feats, tgts, masks = [], [], []
     for wl in ("C1", "C2"):
-        st, _ = sd.prepare_alignment(synth.workload(wl)["chars"])
-        f, t, m, _ = warmstart.training_set(st, n_loci=400, top=32, rand=32, seed=seed)
+        st, _ = sd.prepare_alignment(synth.workload(wl)["chars"], q=q)
+        f, t, m, _ = warmstart.training_set(st, q=q, n_loci=400, top=32, rand=32, seed=seed)
         feats.append(f)
         tgts.append(t)
         masks.append(m)
-    params, losses = warmstart.train(np.concatenate(feats), np.concatenate(tgts), np.concatenate(masks), st,
+    params, losses = warmstart.train(np.concatenate(feats), np.concatenate(tgts), np.concatenate(masks), st, q=q,
                                      epochs=40, batch=2048, lr=2e-3, seed=seed)
     return params, losses, sum(f.shape[0] for f in feats)
 
@@ -114,6 +115,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="C4", choices=["C1", "C2", "C4"])
+    ap.add_argument("--q", type=int, default=0,
+                    help="state envelope (softmax width): default 5 ({A,C,G,T,-}) at C4, 4 (SuperDCA's) otherwise")
     ap.add_argument("--loci-per-step", type=int, default=0, help="default: 2048 at C4, 1024 otherwise")
     ap.add_argument("--slots", type=int, default=0, help="loci in flight (default 256 at C4, 128 otherwise)")
     ap.add_argument("--history", type=int, default=10)
@@ -138,11 +141,12 @@ def main() -> int:
     from superdca_amd import synth
 
     wl = args.workload
+    q = args.q or (5 if wl == "C4" else 4)
     per_step = args.loci_per_step or (2048 if wl == "C4" else 1024)
     slots = args.slots or (256 if wl == "C4" else 128)
 
     t = time.time()
-    states, translation = sd.prepare_alignment(synth.workload(wl)["chars"])
+    states, translation = sd.prepare_alignment(synth.workload(wl)["chars"], q=q)
     B, L = states.shape
     partners = min(args.partners, L - 1)
     log(f"[bench] workload {wl}: B={B} L={L} generated in {time.time() - t:.1f}s")
@@ -155,11 +159,11 @@ def main() -> int:
     net = None
     if not args.no_warm:
         t = time.time()
-        net = train_network()
+        net = train_network(q)
         log(f"[bench] network trained on {net[2]} pairs in {time.time() - t:.1f}s, loss {net[1][0]:.3e} -> "
             f"{net[1][-1]:.3e}")
 
-    ctx = sd.Context(states, q=4, precision=sd.PREC_SPLIT_BF16, batch_loci=slots, history=args.history,
+    ctx = sd.Context(states, q=q, precision=sd.PREC_SPLIT_BF16, batch_loci=slots, history=args.history,
                      grad_threshold=1e-3, pipeline_groups=args.groups)
     t = time.time()
     w, _ = ctx.reweight(0.9)
@@ -195,7 +199,7 @@ def main() -> int:
 
     fp32 = None
     if args.fp32_loci > 0:
-        ctx32 = sd.Context(states, q=4, precision=sd.PREC_FP32, batch_loci=slots, history=args.history,
+        ctx32 = sd.Context(states, q=q, precision=sd.PREC_FP32, batch_loci=slots, history=args.history,
                            grad_threshold=1e-3, pipeline_groups=args.groups)
         ctx32.set_weights(w)
         ctx32.solve(order[pos:pos + slots // 4])  # allocation and first launches
@@ -214,12 +218,12 @@ def main() -> int:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
         ncpu = args.cpu_loci or nthreads
         t = time.perf_counter()
-        _, _, cst = oracle.solve_loci(states, w, lh, lJ, order[:ncpu], gtol=1e-3, max_iter=2000, m=10,
+        _, _, cst = oracle.solve_loci(states, w, lh, lJ, order[:ncpu], q=q, gtol=1e-3, max_iter=2000, m=10,
                                       nthreads=nthreads, blocked=True, want_rows=True)
         cpu_s = time.perf_counter() - t
         cpu = {"value": round(int((cst["status"] == 0).sum()) / cpu_s, 4), "unit": "loci/s", "cores": nthreads,
                "kind": "port",
-               "sample": f"{ncpu} loci of the same workload at full B={B}, L={L}, FP64 block-compressed oracle, "
+               "sample": f"{ncpu} loci of the same workload at full B={B}, L={L}, q={q}, FP64 block-compressed oracle, "
                          f"cold start, ||g||_inf<1e-3, one locus per thread, {cpu_s:.1f}s wall"}
         log(f"[bench] cpu baseline {cpu}")
 
@@ -258,9 +262,10 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "bf16x2+fp32",
         "data": "synthetic",
-        "config": {"workload": f"{wl}: B={B} sequences x N={L} loci, {'warm' if warm else 'cold'} start, "
+        "config": {"workload": f"{wl}: B={B} sequences x N={L} loci, q_env={q}, {'warm' if warm else 'cold'} start, "
                                f"{per_step} loci per step ({slots} in flight), L-BFGS m={args.history}, "
-                               f"||g||_inf<1e-3", "B": B, "N": L, "loci_per_step": per_step, "slots": slots,
+                               f"||g||_inf<1e-3", "B": B, "N": L, "q_env": q, "loci_per_step": per_step,
+                   "slots": slots,
                    "precision": "split-bf16", "start": "warm" if warm else "cold",
                    "parallelism": "loci batched on 1 GPU"},
         "roofline": rl,

@@ -254,3 +254,26 @@ def test_pipelined_slot_groups_match_single_group(c1, precision):
     big = out[1][0] > np.quantile(out[1][0], 0.9)
     rel = np.abs(out[2][0] - out[1][0])[big] / out[1][0][big]
     assert rel.max() < (2e-3 if precision == sd.PREC_SPLIT_BF16 else 1e-3), rel.max()
+
+
+@pytest.mark.parametrize("precision,ftol,gtol", [(sd.PREC_FP32, 2e-6, 2e-5), (sd.PREC_SPLIT_BF16, 2e-5, 2e-4)])
+def test_eval_and_solve_q5_envelope(c1, precision, ftol, gtol):
+    # the genome-scale envelope {A,C,G,T,-}: q_env = 5, gap = state 4, with an envelope
+    # state that no locus of C1 observes (three alleles at most)
+    st5, _ = sd.prepare_alignment(c1["chars"], q=5)
+    assert st5.max() <= 4
+    w, _ = orc.weights(st5, 0.9, nthreads=8)
+    n = orc.nparams(st5.shape[1], 5)
+    loci = [0, 17, 500, 999]
+    x = np.random.default_rng(41).normal(size=(len(loci), n)) * 0.05
+    with sd.Context(st5, q=5, precision=precision, batch_loci=4) as ctx:
+        beff, lh, lJ = ctx.set_weights(w)
+        f, g = ctx.eval_batch(loci, x)
+        rows, stats, _, rc = ctx.solve(loci)
+    for k, r in enumerate(loci):
+        fo, go = orc.objective(st5, r, w, lh, lJ, x[k], q=5)
+        assert f[k] == pytest.approx(fo, rel=ftol), r
+        assert np.abs(g[k] - go).max() / np.abs(go).max() < gtol, r
+    assert rc == sd.SDCA_OK and np.all(stats["gnorm"] < 1e-3)
+    ro, _, so = orc.solve_loci(st5, w, lh, lJ, np.array(loci), q=5, gtol=1e-7, max_iter=10000, nthreads=4)
+    np.testing.assert_allclose(stats["fval"], so["fval"], rtol=1e-5)
