@@ -156,8 +156,8 @@ sdca_ctx::~sdca_ctx()
     for (auto &g : gev)
         for (auto &e : g)
             if (e) (void)hipEventDestroy(e);
-    for (int g = 1; g < sdca::kMaxGroups; ++g)
-        if (gstream[g]) (void)hipStreamDestroy(gstream[g]);
+    for (int g = 0; g < sdca::kMaxGroups; ++g)
+        if (gstream[g] && gstream[g] != stream) (void)hipStreamDestroy(gstream[g]);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -290,7 +290,7 @@ float elapsed(hipEvent_t a, hipEvent_t b);
 int autotune(sdca_ctx *c, int nslots, const std::vector<GemmConfig> &cand, GemmConfig &best)
 {
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
-    hipStream_t st = c->stream;
+    hipStream_t st = c->gstream[0];  // the first group's stream
     std::vector<sdca::SlotCmd> cmds(nslots);
     for (int i = 0; i < nslots; ++i) {
         std::memset(&cmds[i], 0, sizeof(cmds[i]));
