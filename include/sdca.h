@@ -79,7 +79,7 @@ typedef struct sdca_locus_stats {
     int32_t nfeval;
     int32_t iter;
     int32_t status;
-    int32_t reserved;
+    int32_t nfeval_first;  /* evaluations up to the first accepted step (start point included) */
 } sdca_locus_stats;
 
 /* Cumulative device timings of a context (milliseconds, HIP events). */

@@ -65,7 +65,7 @@ class Opts(C.Structure):
 
 class LocusStats(C.Structure):
     _fields_ = [("fval", C.c_double), ("gnorm", C.c_double), ("nfeval", C.c_int32), ("iter", C.c_int32),
-                ("status", C.c_int32), ("reserved", C.c_int32)]
+                ("status", C.c_int32), ("nfeval_first", C.c_int32)]
 
 
 class Timing(C.Structure):
@@ -235,6 +235,7 @@ class Context:
             "nfeval": np.array([st[k].nfeval for k in range(K)], dtype=np.int64),
             "iter": np.array([st[k].iter for k in range(K)], dtype=np.int64),
             "status": np.array([st[k].status for k in range(K)], dtype=np.int64),
+            "nfeval_first": np.array([st[k].nfeval_first for k in range(K)], dtype=np.int64),
         }
         return S, stats, X, rc
 
@@ -318,7 +319,7 @@ class Context:
         rc = _check(lib().sdca_solve_loci_mlp(self._h, _ptr(loci), K, int(npartners), int(mode), _ptr(S), _ptr(X),
                                               C.cast(st, C.c_void_p)), allow=(SDCA_E_NOCONV,))
         stats = {k: np.array([getattr(st[i], k) for i in range(K)]) for k in ("fval", "gnorm", "nfeval", "iter",
-                                                                               "status")}
+                                                                               "status", "nfeval_first")}
         return S, stats, X, rc
 
     def timing(self, reset: bool = False) -> dict:

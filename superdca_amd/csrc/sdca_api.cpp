@@ -655,6 +655,7 @@ struct Slot {
     bool from_xt = true;
     bool preloaded = false;  // the direction buffer holds a first search direction
     int restarts = 0;        // line searches restarted from steepest descent
+    int nfeval_first = 0;    // evaluations up to the first accepted step
     double SY[sdca::kMaxHistory][sdca::kMaxHistory], YY[sdca::kMaxHistory][sdca::kMaxHistory];
     double SG[sdca::kMaxHistory], YG[sdca::kMaxHistory], DY[sdca::kMaxHistory];
     double TS[sdca::kMaxHistory], TY[sdca::kMaxHistory];
@@ -666,6 +667,12 @@ struct Slot {
 // stops with SDCA_ST_LS_FAIL: near the FP32 rounding floor a stale history can give a
 // direction along which no decrease is measurable while the gradient is not yet small.
 constexpr int kMaxRestarts = 2;
+
+// Steepest-descent steps (no curvature pairs yet) start at kFirstStep / ||g||_2.  The
+// data term's curvature along g makes the accepted first step about 1/16 of 1/||g||_2 on
+// the synthetic genomes: starting there takes the first line search from four
+// evaluations to two (C4: median 62 instead of 63 evaluations per locus).
+constexpr double kFirstStep = 1.0 / 16.0;
 
 class Solver {
 public:
@@ -809,6 +816,7 @@ public:
         S.f = f;
         S.gnorm = ginf;
         S.iter++;
+        if (S.iter == 1) S.nfeval_first = S.nfeval;
         if (ginf < c_->opts.grad_threshold) return finish(i, SDCA_ST_CONVERGED, true);
         if (S.iter >= c_->opts.max_iter) return finish(i, SDCA_ST_ITER_CAP, true);
         newdir(i, true, pnew, stp_trial);
@@ -934,7 +942,7 @@ private:
             for (int p = 0; p < m_; ++p) cs[p] = cy[p] = 0.0;
             cg = -1.0;
         }
-        S.stp0 = S.pairs.empty() ? 1.0 / std::sqrt(S.GG) : 1.0;
+        S.stp0 = S.pairs.empty() ? kFirstStep / std::sqrt(S.GG) : 1.0;
         S.stp = S.stp0;
         S.ls_evals = 0;
         S.phase = PH_LS;
@@ -1435,7 +1443,7 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
                         o.nfeval = sl.nfeval;
                         o.iter = sl.iter;
                         o.status = sl.status;
-                        o.reserved = 0;
+                        o.nfeval_first = sl.nfeval_first;
                     }
                     noconv += (sl.status == SDCA_ST_ITER_CAP || sl.status == SDCA_ST_LS_FAIL);
                     nonfinite += (sl.status == SDCA_ST_NONFINITE);
