@@ -299,6 +299,8 @@ int autotune(sdca_ctx *c, int nslots, const std::vector<GemmConfig> &cand, GemmC
         cmds[i].gt = 0;  // the first gradient buffer
         cmds[i].gk = -1;
         cmds[i].pnew = -1;
+        cmds[i].slot = i;
+        cmds[i].row = i * c->q;
     }
     HIP_TRY(hipMemcpyAsync(c->d_cmds, cmds.data(), nslots * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     const size_t nw = (size_t)c->Mpad * c->Kxp;
@@ -309,7 +311,7 @@ int autotune(sdca_ctx *c, int nslots, const std::vector<GemmConfig> &cand, GemmC
     }
     const int reps = 3;
     auto energy = [&](const GemmConfig &g) {
-        sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_X, c->Mg, c->Bp,
+        sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_X, c->Mg, c->Mg, c->Bp,
                                 (int)c->Kxp, g.ksplit, g.kchunk, g.gm_e, g.tn_e, c->d_Epart);
         sdca::kern::softmax(st, c->opts.precision, c->d_Epart, g.ksplit, c->Mg, c->Bp, c->B, c->q, nslots,
                             c->d_cmds, c->d_statesT, c->d_w, c->d_G32, c->d_Ghi, c->d_Glo, c->d_sm_part, c->nbchunk);
@@ -450,6 +452,7 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     HIP_TRY(hipHostMalloc((void **)&c->h_cmds, (size_t)nslots * sizeof(sdca::SlotCmd), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&c->h_res, (size_t)nslots * sdca::ResLayout{m}.width() * sizeof(double),
                           hipHostMallocDefault));
+    c->h_xt_off.assign(nslots, -1);
     // the zero fills above must be complete before any kernel of the solve runs
     HIP_TRY(hipDeviceSynchronize());
     c->tuned = 0;
@@ -483,8 +486,23 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     const int m = c->ws_m, W = sdca::ResLayout{m}.width();
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
     const int ns = c->gslots, s0 = g * ns;
-    const size_t vo = (size_t)s0 * c->n_slot;        // slot-vector offset of the group
-    const size_t go = (size_t)s0 * c->q * c->Bp;     // row offset of its node beliefs
+    // command list: the active slots in slot order, each with its packed rows
+    std::vector<int> &lst = c->gcmd_slot[g];
+    lst.clear();
+    sdca::SlotCmd *hc = c->h_cmds + s0;
+    for (int i = 0; i < ns; ++i) {
+        if (cmds[i].mode == sdca::MODE_IDLE) continue;
+        hc[lst.size()] = cmds[i];
+        hc[lst.size()].slot = i;
+        hc[lst.size()].row = (int)lst.size() * c->q;
+        lst.push_back(i);
+    }
+    const int nc = (int)lst.size();
+    if (nc == 0) return SDCA_OK;
+    const int Mrows = (int)round_up((int64_t)nc * c->q, sdca::kTile);
+    const size_t vo = (size_t)s0 * c->n_slot;              // slot-vector offset of the group
+    const size_t ro = (size_t)g * c->Mg * c->Kxp;          // its packed trial-point rows
+    const size_t go = (size_t)g * c->Mg * c->Bp;           // its packed node-belief rows
     float *gbuf = c->d_gbuf + (size_t)s0 * 2 * c->n_slot;
     float *dbuf = c->d_dbuf + vo;
     const size_t po = (size_t)s0 * m * c->n_slot * pair_bytes(c);
@@ -493,50 +511,61 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     float *Epart = c->d_Epart + (size_t)g * c->ksplit * c->Mg * c->Bp;
     hipStream_t st = c->gstream[g];
     hipEvent_t *ev = c->gev[g];
-    std::memcpy(c->h_cmds + s0, cmds, ns * sizeof(sdca::SlotCmd));
-    HIP_TRY(hipMemcpyAsync(dc, c->h_cmds + s0, ns * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(dc, hc, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(ev[0], st));
     double *upd_part = c->d_upd_part + (size_t)s0 * c->nchunk * sdca::ResLayout{m}.nupd();
-    sdca::kern::update(st, m, c->q, split, dc, ns, gbuf, dbuf, Sp, Yp, c->d_Xc + vo, c->d_Xtr + vo,
-                       split ? c->d_Whi + vo : nullptr, split ? c->d_Wlo + vo : nullptr,
+    sdca::kern::update(st, m, c->q, split, dc, nc, gbuf, dbuf, Sp, Yp, c->d_Xc + vo, c->d_Xtr + ro,
+                       split ? c->d_Whi + ro : nullptr, split ? c->d_Wlo + ro : nullptr,
                        c->d_X0 ? c->d_X0 + vo : nullptr, c->n_slot, (int)c->Kxp, (int)c->Kx, upd_part, c->nchunk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[1], st));
-    sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr + vo, split ? c->d_Whi + vo : nullptr,
-                            split ? c->d_Wlo + vo : nullptr, c->d_X, c->Mg, c->Bp, (int)c->Kxp, c->ksplit, c->kchunk,
-                            c->gm_energy, c->tn_energy, Epart);
+    sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr + ro, split ? c->d_Whi + ro : nullptr,
+                            split ? c->d_Wlo + ro : nullptr, c->d_X, Mrows, c->Mg, c->Bp, (int)c->Kxp, c->ksplit,
+                            c->kchunk, c->gm_energy, c->tn_energy, Epart);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[2], st));
-    sdca::kern::softmax(st, c->opts.precision, Epart, c->ksplit, c->Mg, c->Bp, c->B, c->q, ns, dc, c->d_statesT,
+    sdca::kern::softmax(st, c->opts.precision, Epart, c->ksplit, c->Mg, c->Bp, c->B, c->q, nc, dc, c->d_statesT,
                         c->d_w, split ? nullptr : c->d_G32 + go, split ? c->d_Ghi + go : nullptr,
                         split ? c->d_Glo + go : nullptr, c->d_sm_part + (size_t)s0 * c->nbchunk, c->nbchunk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[3], st));
     sdca::kern::gemm_grad(st, c->opts.precision, split ? nullptr : c->d_G32 + go, split ? c->d_Ghi + go : nullptr,
-                          split ? c->d_Glo + go : nullptr, c->d_Xt, c->Mg, c->Bp, (int)c->Kxp, (int)c->Kx, c->q, ns,
-                          dc, c->d_Xtr + vo, gbuf, c->n_slot, (float)(2.0 * c->lambda_J),
+                          split ? c->d_Glo + go : nullptr, c->d_Xt, Mrows, c->Bp, (int)c->Kxp, (int)c->Kx, c->q, nc,
+                          dc, c->d_Xtr + ro, gbuf, c->n_slot, (float)(2.0 * c->lambda_J),
                           (float)(2.0 * c->lambda_h), c->gm_grad, c->tn_grad);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[4], st));
     double *dot_part = c->d_dot_part + (size_t)s0 * c->nchunk * sdca::ResLayout{m}.ndots();
-    sdca::kern::dots(st, m, split, dc, ns, gbuf, dbuf, Sp, Yp, c->n_slot, dot_part, c->nchunk);
+    sdca::kern::dots(st, m, split, dc, nc, gbuf, dbuf, Sp, Yp, c->n_slot, dot_part, c->nchunk);
     HIP_TRY(hipGetLastError());
-    sdca::kern::reduce_results(st, m, ns, upd_part, dot_part,
+    sdca::kern::reduce_results(st, m, nc, upd_part, dot_part,
                                c->d_sm_part + (size_t)s0 * c->nbchunk, c->nchunk, c->nbchunk,
                                c->d_res + (size_t)s0 * W);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[5], st));
-    HIP_TRY(hipMemcpyAsync(c->h_res + (size_t)s0 * W, c->d_res + (size_t)s0 * W, (size_t)ns * W * sizeof(double),
+    HIP_TRY(hipMemcpyAsync(c->h_res + (size_t)s0 * W, c->d_res + (size_t)s0 * W, (size_t)nc * W * sizeof(double),
                            hipMemcpyDeviceToHost, st));
     return SDCA_OK;
 }
 
-// Waits for group g's round and accounts its kernel times and algorithmic traffic.
+// Waits for group g's round and accounts its kernel times and algorithmic traffic.  The
+// result rows arrive in command order and are moved to their slots; each listed slot's
+// trial point is recorded for scoring.
 int finish_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
 {
     const int m = c->ws_m;
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
+    const std::vector<int> &lst = c->gcmd_slot[g];
+    if (lst.empty()) return SDCA_OK;
     HIP_TRY(hipStreamSynchronize(c->gstream[g]));
+    {
+        const int W = sdca::ResLayout{m}.width(), s0 = g * c->gslots;
+        double *res = c->h_res + (size_t)s0 * W;
+        for (int k = (int)lst.size() - 1; k >= 0; --k) {  // lst[k] >= k: rows move up only
+            if (lst[k] != k) std::memcpy(res + (size_t)lst[k] * W, res + (size_t)k * W, W * sizeof(double));
+            c->h_xt_off[s0 + lst[k]] = (int64_t)((size_t)g * c->Mg + (size_t)k * c->q) * c->Kxp;
+        }
+    }
     hipEvent_t *ev = c->gev[g];
     auto &t = c->timing;
     const double upd_ms = elapsed(ev[0], ev[1]), dot_ms = elapsed(ev[4], ev[5]);
@@ -740,7 +769,7 @@ public:
                 S.preloaded = false;
                 newdir_preloaded(i);
             } else {
-                newdir(i, true, -1, 0.0);
+                newdir(i, false, -1, 0.0);  // x_c already holds the start point
             }
             return false;
         }
@@ -910,7 +939,7 @@ private:
         k.mode = sdca::MODE_NEWDIR;
         k.gt = S.gt;
         k.gk = S.gk;
-        k.accept = 1;  // x_t = x_c after the start point's evaluation
+        k.accept = 0;  // x_c already holds the start point
         k.pnew = -1;
         k.pstep = 0.f;
         k.alpha = 1.f;
@@ -919,9 +948,9 @@ private:
         for (int p = 0; p < sdca::kMaxHistory; ++p) k.cs[p] = k.cy[p] = 0.f;
     }
 
-    // New search direction from the iterate: the last trial point if accept (with the new
-    // pair pnew of step pstep, if any), else the current x_c (restart after a failed or
-    // non-descent search).
+    // New search direction from the iterate: the last trial point x_c + pstep d if accept
+    // (with the new pair pnew of that step, if any), else the current x_c (the start
+    // point, or a restart after a failed or non-descent search).
     void newdir(int i, bool accept, int pnew, double pstep)
     {
         Slot &S = slots_[i];
@@ -1402,7 +1431,7 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
                 sloci.clear();
                 for (int i : finished) {
                     const Slot &sl = S.slot(i);
-                    src_off.push_back((sl.from_xt ? 0 : (int64_t)c->Mpad * c->Kxp) + (int64_t)i * c->n_slot);
+                    src_off.push_back(sl.from_xt ? c->h_xt_off[i] : (int64_t)c->Mpad * c->Kxp + (int64_t)i * c->n_slot);
                     sloci.push_back(sl.r);
                 }
                 const int nf = (int)finished.size();

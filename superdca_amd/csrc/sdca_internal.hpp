@@ -40,10 +40,15 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 
 // Per-slot command of one batched evaluation round (host -> device).
 //
-// L-BFGS state of a slot on the device: iterate x_c and trial point x_t, two gradient
-// buffers (current g_k and trial g_t alternate), the search direction d, and m stored
-// curvature pairs s_p = alpha d, y_p = g_new - g_old (BF16 under split precision, FP32
-// otherwise).
+// L-BFGS state of a slot on the device: iterate x_c, two gradient buffers (current g_k
+// and trial g_t alternate), the search direction d, and m stored curvature pairs
+// s_p = alpha d, y_p = g_new - g_old (BF16 under split precision, FP32 otherwise).
+//
+// A round's command list holds the group's active slots only, in slot order.  The slot
+// vectors stay at their slot; the round's trial point x_t (and its BF16 copies), energies
+// and node beliefs are packed at the command's rows, so that the GEMMs of the round cover
+// active slots only.  x_t lives for one round: an accepted trial point is re-formed as
+// x_c + pstep d_old by the next NEWDIR (the same fused multiply-add, hence the same value).
 enum SlotMode : int32_t { MODE_IDLE = 0, MODE_INIT = 1, MODE_RETRY = 2, MODE_NEWDIR = 3 };
 
 struct SlotCmd {
@@ -51,13 +56,15 @@ struct SlotCmd {
     int32_t locus;    // locus r of the slot, -1 if idle
     int32_t self_lo;  // columns [self_lo, self_hi) belong to locus r: gradient masked to 0
     int32_t self_hi;
+    int32_t slot;     // slot of the command within its group
+    int32_t row;      // first of its q rows in the round's packed layout
     int32_t gt;       // gradient buffer (0 or 1) that receives the trial gradient
     int32_t gk;       // gradient buffer of the current iterate (after NEWDIR's accept); -1 at INIT
-    int32_t accept;   // NEWDIR: 1 = the last trial point becomes the iterate (x_c = x_t) first
+    int32_t accept;   // NEWDIR: 1 = the last trial point becomes the iterate first (x_c += pstep d_old)
     int32_t pnew;     // NEWDIR: pair slot receiving s = pstep d_old, y = g[gk] - g[1-gk]; -1: none
     uint32_t live;    // bit p: pair slot p holds a stored pair
     float alpha;      // step of the trial point x_t = x_c + alpha d
-    float pstep;      // NEWDIR with pnew >= 0: step of the accepted trial
+    float pstep;      // NEWDIR with accept: step of the accepted trial
     float init_src;   // MODE_INIT: 1 = copy the staged start point, 0 = zero start
     float cd, cg;     // NEWDIR: d = cd d_old + cg g_k + sum_p (cs[p] s_p + cy[p] y_p)
     float cs[kMaxHistory], cy[kMaxHistory];
@@ -97,22 +104,26 @@ void bitplanes(hipStream_t st, const uint8_t *d_states, int B, int L, int q, int
 void hamming_neighbours(hipStream_t st, const uint64_t *planes, int B, int q, int W, int L, int64_t dmax,
                         const uint32_t *d_mult, unsigned long long *d_iw);
 int energy_superiles(int Mpad, int Bp, int gm, int tn);  // super-tiles per split-K slice
+// Packed rows (trial points W, energies, beliefs): Mrows rows computed (a multiple of
+// kTile), Mz rows between split-K slices of Epart.  Command k of a launch owns rows
+// [k q, k q + q) of it.
 void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const uint32_t *X,
-                 int Mpad, int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn, float *Epart);
-void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad, int Bp, int B, int q, int nslots,
+                 int Mrows, int Mz, int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn, float *Epart);
+void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mz, int Bp, int B, int q, int ncmd,
              const SlotCmd *cmds, const uint8_t *statesT, const float *wts, float *G32, uint16_t *Ghi,
              uint16_t *Glo, double *fpart, int nbchunk);
 void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const uint32_t *Xt,
-               int Mpad, int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial,
+               int Mrows, int Bp, int Kxp, int Kx, int q, int ncmd, const SlotCmd *cmds, const float *Xtrial,
                float *gbuf, int64_t n_slot, float lamJ2, float lamh2, int gm, int tn);
 // Slot vectors: gbuf [slot][2][n_slot], dbuf [slot][n_slot], S and Y [slot][m][n_slot]
-// (uint16_t BF16 when pair_bf16, else float).
-void update(hipStream_t st, int m, int q, bool pair_bf16, const SlotCmd *cmds, int nslots, const float *gbuf,
+// (uint16_t BF16 when pair_bf16, else float); Xtr, Whi and Wlo are packed rows.  One
+// block row per command (ncmd commands).
+void update(hipStream_t st, int m, int q, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
             float *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0,
             int64_t n_slot, int Kxp, int Kx, double *part, int nchunk);
-void dots(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int nslots, const float *gbuf,
+void dots(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
           const float *dbuf, const void *S, const void *Y, int64_t n_slot, double *part, int nchunk);
-void reduce_results(hipStream_t st, int m, int nslots, const double *upd_part, const double *dot_part,
+void reduce_results(hipStream_t st, int m, int ncmd, const double *upd_part, const double *dot_part,
                     const double *sm_part, int nchunk, int nbchunk, double *res);
 void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *loci, int nscore, int L,
            int q, int gap, int Kxp, const int32_t *state_col, float *rows);
@@ -214,7 +225,9 @@ struct sdca_ctx {
     float *d_Epart = nullptr;
     double *d_upd_part = nullptr, *d_dot_part = nullptr, *d_sm_part = nullptr, *d_res = nullptr;
     sdca::SlotCmd *d_cmds = nullptr;
-    sdca::SlotCmd *h_cmds = nullptr;  // pinned host staging of the slot commands
+    sdca::SlotCmd *h_cmds = nullptr;  // pinned host staging of the command lists
+    std::vector<int> gcmd_slot[sdca::kMaxGroups];  // slot (in its group) of each listed command in flight
+    std::vector<int64_t> h_xt_off;                 // per slot: d_Xtr offset of its last trial point
     double *h_res = nullptr;          // pinned host result rows
     float *d_rows = nullptr;
     int64_t *d_src_off = nullptr;

@@ -362,31 +362,33 @@ struct EpiEnergy {
     }
 };
 
-// Gradient epilogue: row m = slot*q + s, column n.  Adds 2*lambda*x (lambda_h on the
-// bias column, lambda_J elsewhere: plmDCA_objective.hpp:192-244), zeroes the locus'
+// Gradient epilogue: row m = k*q + s of command k, column n.  Adds 2*lambda*x (lambda_h on
+// the bias column, lambda_J elsewhere: plmDCA_objective.hpp:192-244), zeroes the locus'
 // own columns (the j != r exclusions of :95,111,161,176) and pad columns, and writes
-// the slot's trial gradient (gradient buffer gt).  The commands of the slots under the tile's
-// 128 rows are staged in LDS once; each group of four rows first loads its trial-point
-// values, then stores.
+// the trial gradient into gradient buffer gt of the command's slot.  The commands under
+// the tile's 128 rows are staged in LDS once; each group of four rows first loads its
+// trial-point values (packed rows, like the beliefs), then stores.
 struct EpiGrad {
     float *gring;
     const float *Xtrial;
     const SlotCmd *cmds;
     int64_t n_slot, Kxp;
-    int ng, g0, q, Kx, nslots;  // gradients per slot, ring index of the first
+    int ng, g0, q, Kx, ncmd;  // gradients per slot, ring index of the first, commands
     float lamJ2, lamh2;
     template <int NI>
     __device__ __forceinline__ void tile(int m0, int wm, int64_t nw, int lane, int tid, floatx16 (&acc)[2][NI],
                                          int) const
     {
-        __shared__ int s_gt[kTile + 2], s_lo[kTile + 2], s_hi[kTile + 2];  // s_gt < 0: idle slot
+        // s_gt < 0: no gradient to write
+        __shared__ int s_gt[kTile + 2], s_lo[kTile + 2], s_hi[kTile + 2], s_slot[kTile + 2];
         const int i0 = m0 / q;
-        const int ns = min(nslots - i0, (kTile - 1) / q + 2);  // slots under rows [m0, m0 + 128)
+        const int ns = min(ncmd - i0, (kTile - 1) / q + 2);  // commands under rows [m0, m0 + 128)
         if (tid < ns) {
             const SlotCmd &c = cmds[i0 + tid];
             s_gt[tid] = (c.mode == MODE_IDLE || c.gt < g0 || c.gt >= g0 + ng) ? -1 : c.gt;
             s_lo[tid] = c.self_lo;
             s_hi[tid] = c.self_hi;
+            s_slot[tid] = c.slot;
         }
         __syncthreads();
 #pragma unroll
@@ -405,7 +407,9 @@ struct EpiGrad {
                     gt[u] = ok ? s_gt[il] : -1;
                     lo[u] = ok ? s_lo[il] : 0;
                     hi[u] = ok ? s_hi[il] : 0;
-                    roff[u] = ((int64_t)i * ng + (gt[u] < 0 ? 0 : gt[u] - g0)) * n_slot + (int64_t)(m - i * q) * Kxp;
+                    const int slot = ok ? s_slot[il] : 0;
+                    roff[u] = ((int64_t)slot * ng + (gt[u] < 0 ? 0 : gt[u] - g0)) * n_slot +
+                              (int64_t)(m - i * q) * Kxp;
                     const float *xr = Xtrial + (int64_t)m * Kxp;
 #pragma unroll
                     for (int ni = 0; ni < NI; ++ni) {
@@ -432,13 +436,13 @@ struct EpiGrad {
 
 // ---------------------------------------------------------------- softmax
 
-// One thread per (slot, sequence b): sums the split-K partial energies, forms the
+// One thread per (command, sequence b): sums the split-K partial energies, forms the
 // log-partition lz = log sum_s exp E_b(s) (max-shifted; the same value as the
 // unshifted plmDCA_objective.hpp:140), the data term w_b (lz - E_b(x_br)) (:143) and the
 // node beliefs G_b(s) = w_b P_b(s) - w_b [s == x_br] (:147-148), with 1 - P(x_br)
 // formed as the sum of the other probabilities.  The data term is reduced in FP64.
 __global__ __launch_bounds__(256) void k_softmax(const float *__restrict__ Epart, int ksplit, int64_t zstride, int Bp,
-                                                 int B, int q, int nslots, const SlotCmd *__restrict__ cmds,
+                                                 int B, int q, const SlotCmd *__restrict__ cmds,
                                                  const uint8_t *__restrict__ statesT, const float *__restrict__ wts,
                                                  float *__restrict__ G32, uint16_t *__restrict__ Ghi,
                                                  uint16_t *__restrict__ Glo, double *__restrict__ fpart, int nbchunk)
@@ -446,13 +450,14 @@ __global__ __launch_bounds__(256) void k_softmax(const float *__restrict__ Epart
     const int i = blockIdx.y;
     const int b = blockIdx.x * 256 + threadIdx.x;
     const int r = cmds[i].mode == MODE_IDLE ? -1 : cmds[i].locus;
+    const int64_t row0 = cmds[i].row;
     double fb = 0.0;
     if (b < Bp) {
         float e[8], g[8];
         if (r >= 0) {
             for (int s = 0; s < q; ++s) {
                 float acc = 0.f;
-                for (int z = 0; z < ksplit; ++z) acc += Epart[(int64_t)z * zstride + (int64_t)(i * q + s) * Bp + b];
+                for (int z = 0; z < ksplit; ++z) acc += Epart[(int64_t)z * zstride + (row0 + s) * Bp + b];
                 e[s] = acc;
             }
             const float w = wts[b];
@@ -474,7 +479,7 @@ __global__ __launch_bounds__(256) void k_softmax(const float *__restrict__ Epart
             for (int s = 0; s < q; ++s) g[s] = 0.f;
         }
         for (int s = 0; s < q; ++s) {
-            const int64_t o = (int64_t)(i * q + s) * Bp + b;
+            const int64_t o = (row0 + s) * Bp + b;
             if (G32) {
                 G32[o] = g[s];
             } else {
@@ -577,6 +582,15 @@ __device__ __forceinline__ void axpy8(f8 &d, float a, const f8 &v)
     d.b.z += a * v.b.z;
     d.b.w += a * v.b.w;
 }
+// x + a d with one rounding per element: an accepted trial point is formed again from
+// x_c and d by the same operation, which reproduces it exactly.
+__device__ __forceinline__ f8 step8(const f8 &x, float a, const f8 &d)
+{
+    return {make_float4(__builtin_fmaf(a, d.a.x, x.a.x), __builtin_fmaf(a, d.a.y, x.a.y),
+                        __builtin_fmaf(a, d.a.z, x.a.z), __builtin_fmaf(a, d.a.w, x.a.w)),
+            make_float4(__builtin_fmaf(a, d.b.x, x.b.x), __builtin_fmaf(a, d.b.y, x.b.y),
+                        __builtin_fmaf(a, d.b.z, x.b.z), __builtin_fmaf(a, d.b.w, x.b.w))};
+}
 __device__ __forceinline__ double dot8(const f8 &x, const f8 &y) { return dot4(x.a, y.a) + dot4(x.b, y.b); }
 __device__ __forceinline__ float amax8(const f8 &x)
 {
@@ -584,14 +598,16 @@ __device__ __forceinline__ float amax8(const f8 &x)
                  fmaxf(fmaxf(fabsf(x.b.x), fabsf(x.b.y)), fmaxf(fabsf(x.b.z), fabsf(x.b.w))));
 }
 
-// Trial point of every slot (one launch per round):
+// Trial point of every command's slot (one launch per round):
 //   INIT    x_c = x0 (staged) or 0,           x_t = x_c
 //   RETRY   x_t = x_c + alpha d
-//   NEWDIR  x_c = x_t (if accept); the new pair s = pstep d, y = g[gk] - g[1-gk] into slot
-//           pnew (if any); d <- cd d + cg g_k + sum_p (cs[p] s_p + cy[p] y_p);
+//   NEWDIR  x_c = x_c + pstep d_old (if accept: the last trial point); the new pair
+//           s = pstep d, y = g[gk] - g[1-gk] into pair slot pnew (if any);
+//           d <- cd d + cg g_k + sum_p (cs[p] s_p + cy[p] y_p);
 //           x_t = x_c + alpha d; and the products d . g_k, d . y_p (live p) of the stored d.
 //           Pairs with a zero coefficient are read only for their product.
-// plus the squared norms of x_t for the L2 terms: part[slot][chunk][M + 3] = {sum x^2 over
+// x_t (and its BF16 hi/lo copies) is written at the command's packed rows.  Plus the
+// squared norms of x_t for the L2 terms: part[command][chunk][M + 3] = {sum x^2 over
 // the whole vector, sum over the h entries (column Kx of each row), d . g_k, d . y_p}.
 template <int M, class PT>
 __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds, const float *__restrict__ gbuf,
@@ -602,8 +618,8 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
                                                 double *__restrict__ part, int nchunk)
 {
     constexpr int NV = M + 3;
-    const int i = blockIdx.y;
-    const SlotCmd &c = cmds[i];
+    const int j = blockIdx.y;
+    const SlotCmd &c = cmds[j];
     double acc[NV];
     bool is_max[NV];
 #pragma unroll
@@ -615,8 +631,9 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
     const bool ok = mode == MODE_INIT || mode == MODE_RETRY ||
                     (mode == MODE_NEWDIR && (c.gk == 0 || c.gk == 1) && c.pnew < M);
     if (mode != MODE_IDLE && ok) {
-        float *xc = Xc + (int64_t)i * n_slot;
-        float *xt = Xtr + (int64_t)i * n_slot;
+        const int64_t i = c.slot, xo = (int64_t)c.row * Kxp;
+        float *xc = Xc + i * n_slot;
+        float *xt = Xtr + xo;
         float *dv = dbuf + (int64_t)i * n_slot;
         const float *x0 = X0 ? X0 + (int64_t)i * n_slot : nullptr;
         const float *gk = gbuf + ((int64_t)i * 2 + (c.gk == 1)) * n_slot;
@@ -643,11 +660,14 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
                 xn = from_x0 ? ld8(x0, e) : zero8();
                 st8(xc, e, xn);
             } else if (mode == MODE_RETRY) {
-                xn = lin(1.f, ld8(xc, e), alpha, ld8(dv, e));
+                xn = step8(ld8(xc, e), alpha, ld8(dv, e));
             } else {  // NEWDIR
-                const f8 xv = ld8(accept ? xt : xc, e);
-                if (accept) st8(xc, e, xv);
                 const f8 dold = ld8(dv, e);
+                f8 xv = ld8(xc, e);
+                if (accept) {
+                    xv = step8(xv, pstep, dold);
+                    st8(xc, e, xv);
+                }
                 const f8 g = ld8(gk, e);
                 f8 sn = zero8(), yn = zero8();
                 if (pnew >= 0) {
@@ -682,7 +702,7 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
                     if (p == pnew) yv = yn;
                     acc[3 + p] += dot8(d, yv);
                 }
-                xn = lin(1.f, xv, alpha, d);
+                xn = step8(xv, alpha, d);
             }
             st8(xt, e, xn);
             const float xs[8] = {xn.a.x, xn.a.y, xn.a.z, xn.a.w, xn.b.x, xn.b.y, xn.b.z, xn.b.w};
@@ -698,8 +718,8 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
                 u32x4 hv, lv;
                 hv.x = h[0], hv.y = h[1], hv.z = h[2], hv.w = h[3];
                 lv.x = l[0], lv.y = l[1], lv.z = l[2], lv.w = l[3];
-                *reinterpret_cast<u32x4 *>(Whi + (int64_t)i * n_slot + e) = hv;
-                *reinterpret_cast<u32x4 *>(Wlo + (int64_t)i * n_slot + e) = lv;
+                *reinterpret_cast<u32x4 *>(Whi + xo + e) = hv;
+                *reinterpret_cast<u32x4 *>(Wlo + xo + e) = lv;
             }
             acc[0] += dot8(xn, xn);
             // the h entry of row s is column Kx (K_x is padded to a multiple of 8, so an
@@ -711,12 +731,12 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
             }
         }
     }
-    block_reduce_store<NV>(acc, part + ((int64_t)i * nchunk + blockIdx.x) * NV, is_max);
+    block_reduce_store<NV>(acc, part + ((int64_t)j * nchunk + blockIdx.x) * NV, is_max);
 }
 
-// Products of each active slot's trial gradient g_t with the stored pairs, itself, the
+// Products of each command slot's trial gradient g_t with the stored pairs, itself, the
 // current gradient and the direction, and its infinity norm:
-// part[slot][chunk][2M + 4] = {g_t.s_p, g_t.y_p, g_t.g_t, g_t.g_k, g_t.d, max |g_t|}.
+// part[command][chunk][2M + 4] = {g_t.s_p, g_t.y_p, g_t.g_t, g_t.g_k, g_t.d, max |g_t|}.
 template <int M, class PT>
 __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, const float *__restrict__ gbuf,
                                               const float *__restrict__ dbuf, const PT *__restrict__ S,
@@ -724,8 +744,8 @@ __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, 
                                               int nchunk)
 {
     constexpr int NV = 2 * M + 4;
-    const int i = blockIdx.y;
-    const SlotCmd &c = cmds[i];
+    const int j = blockIdx.y;
+    const SlotCmd &c = cmds[j];
     double acc[NV];
     bool is_max[NV];
 #pragma unroll
@@ -734,12 +754,13 @@ __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, 
         is_max[k] = (k == 2 * M + 3);
     }
     if (c.mode != MODE_IDLE && (c.gt == 0 || c.gt == 1)) {
-        const float *g = gbuf + ((int64_t)i * 2 + c.gt) * n_slot;
+        const int64_t i = c.slot;
+        const float *g = gbuf + (i * 2 + c.gt) * n_slot;
         const bool cur = c.mode != MODE_INIT;  // a current gradient and a direction exist
-        const float *gk = gbuf + ((int64_t)i * 2 + (1 - c.gt)) * n_slot;
-        const float *dv = dbuf + (int64_t)i * n_slot;
-        const PT *Sb = S + (int64_t)i * M * n_slot;
-        const PT *Yb = Y + (int64_t)i * M * n_slot;
+        const float *gk = gbuf + (i * 2 + (1 - c.gt)) * n_slot;
+        const float *dv = dbuf + i * n_slot;
+        const PT *Sb = S + i * M * n_slot;
+        const PT *Yb = Y + i * M * n_slot;
         const uint32_t live = c.live;
         const int64_t n8 = n_slot >> 3;
         for (int64_t e8 = (int64_t)blockIdx.x * 256 + threadIdx.x; e8 < n8; e8 += (int64_t)nchunk * 256) {
@@ -759,10 +780,10 @@ __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, 
             }
         }
     }
-    block_reduce_store<NV>(acc, part + ((int64_t)i * nchunk + blockIdx.x) * NV, is_max);
+    block_reduce_store<NV>(acc, part + ((int64_t)j * nchunk + blockIdx.x) * NV, is_max);
 }
 
-// Per-slot reduction of the chunk partials into the result row (ResLayout): the dots
+// Per-command reduction of the chunk partials into the result row (ResLayout): the dots
 // pass fills [0, 2m+4), the trial-point pass [2m+4, 3m+7) (two squares, d.g_k, d.y_p;
 // the first square is the whole vector, stored as the J part), the softmax the data term.
 __global__ void k_reduce(int m, const double *__restrict__ upd_part, const double *__restrict__ dot_part,
@@ -919,32 +940,31 @@ static void launch_gemm(hipStream_t st, int tn, const TileMap &tm, const void *A
                            Bbits, ldbw, K, kchunk, tm, epi);
 }
 
-void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const uint32_t *X, int Mpad,
-                 int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn, float *Epart)
+void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const uint32_t *X, int Mrows,
+                 int Mz, int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn, float *Epart)
 {
-    EpiEnergy epi{Epart, (int64_t)Bp, (int64_t)Mpad * Bp};
-    const TileMap tm = make_map(Mpad / kTile, Bp / tn, ksplit, gm);
+    EpiEnergy epi{Epart, (int64_t)Bp, (int64_t)Mz * Bp};
+    const TileMap tm = make_map(Mrows / kTile, Bp / tn, ksplit, gm);
     if (prec == SDCA_PREC_FP32)
         launch_gemm<PolF32, 1>(st, tn, tm, W32, W32, Kxp, X, Kxp / 32, Kxp, kchunk, epi);
     else
         launch_gemm<PolBF16, 2>(st, tn, tm, Whi, Wlo, Kxp, X, Kxp / 32, Kxp, kchunk, epi);
 }
 
-void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mpad, int Bp, int B, int q, int nslots,
+void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mz, int Bp, int B, int q, int ncmd,
              const SlotCmd *cmds, const uint8_t *statesT, const float *wts, float *G32, uint16_t *Ghi,
              uint16_t *Glo, double *fpart, int nbchunk)
 {
-    hipLaunchKernelGGL(k_softmax, dim3(nbchunk, nslots), dim3(256), 0, st, Epart, ksplit, (int64_t)Mpad * Bp, Bp, B,
-                       q, nslots, cmds, statesT, wts, prec == SDCA_PREC_FP32 ? G32 : nullptr, Ghi, Glo, fpart,
-                       nbchunk);
+    hipLaunchKernelGGL(k_softmax, dim3(nbchunk, ncmd), dim3(256), 0, st, Epart, ksplit, (int64_t)Mz * Bp, Bp, B, q,
+                       cmds, statesT, wts, prec == SDCA_PREC_FP32 ? G32 : nullptr, Ghi, Glo, fpart, nbchunk);
 }
 
-void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const uint32_t *Xt, int Mpad,
-               int Bp, int Kxp, int Kx, int q, int nslots, const SlotCmd *cmds, const float *Xtrial, float *gbuf,
+void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const uint32_t *Xt, int Mrows,
+               int Bp, int Kxp, int Kx, int q, int ncmd, const SlotCmd *cmds, const float *Xtrial, float *gbuf,
                int64_t n_slot, float lamJ2, float lamh2, int gm, int tn)
 {
-    EpiGrad epi{gbuf, Xtrial, cmds, n_slot, (int64_t)Kxp, 2, 0, q, Kx, nslots, lamJ2, lamh2};
-    const TileMap tm = make_map(Mpad / kTile, Kxp / tn, 1, gm);
+    EpiGrad epi{gbuf, Xtrial, cmds, n_slot, (int64_t)Kxp, 2, 0, q, Kx, ncmd, lamJ2, lamh2};
+    const TileMap tm = make_map(Mrows / kTile, Kxp / tn, 1, gm);
     if (prec == SDCA_PREC_FP32)
         launch_gemm<PolF32, 1>(st, tn, tm, G32, G32, Bp, Xt, Bp / 32, Bp, Bp, epi);
     else
@@ -962,40 +982,40 @@ void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const
     default: break;              \
     }
 
-void update(hipStream_t st, int m, int q, bool pair_bf16, const SlotCmd *cmds, int nslots, const float *gbuf,
+void update(hipStream_t st, int m, int q, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
             float *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0,
             int64_t n_slot, int Kxp, int Kx, double *part, int nchunk)
 {
 #define SDCA_UPD(MM)                                                                                                \
     if (pair_bf16)                                                                                                  \
-        hipLaunchKernelGGL((k_update<MM, uint16_t>), dim3(nchunk, nslots), dim3(256), 0, st, cmds, gbuf, dbuf,    \
+        hipLaunchKernelGGL((k_update<MM, uint16_t>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf, dbuf,    \
                            (uint16_t *)S, (uint16_t *)Y, Xc, Xtr, Whi, Wlo, X0, n_slot, Kxp, Kx, q, part, nchunk); \
     else                                                                                                            \
-        hipLaunchKernelGGL((k_update<MM, float>), dim3(nchunk, nslots), dim3(256), 0, st, cmds, gbuf, dbuf,       \
+        hipLaunchKernelGGL((k_update<MM, float>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf, dbuf,       \
                            (float *)S, (float *)Y, Xc, Xtr, Whi, Wlo, X0, n_slot, Kxp, Kx, q, part, nchunk)
     SDCA_DISPATCH_M(m, SDCA_UPD)
 #undef SDCA_UPD
 }
 
-void dots(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int nslots, const float *gbuf,
+void dots(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
           const float *dbuf, const void *S, const void *Y, int64_t n_slot, double *part, int nchunk)
 {
 #define SDCA_DOT(MM)                                                                                                \
     if (pair_bf16)                                                                                                  \
-        hipLaunchKernelGGL((k_dots<MM, uint16_t>), dim3(nchunk, nslots), dim3(256), 0, st, cmds, gbuf, dbuf,      \
+        hipLaunchKernelGGL((k_dots<MM, uint16_t>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf, dbuf,      \
                            (const uint16_t *)S, (const uint16_t *)Y, n_slot, part, nchunk);                         \
     else                                                                                                            \
-        hipLaunchKernelGGL((k_dots<MM, float>), dim3(nchunk, nslots), dim3(256), 0, st, cmds, gbuf, dbuf,         \
+        hipLaunchKernelGGL((k_dots<MM, float>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf, dbuf,         \
                            (const float *)S, (const float *)Y, n_slot, part, nchunk)
     SDCA_DISPATCH_M(m, SDCA_DOT)
 #undef SDCA_DOT
 }
 
-void reduce_results(hipStream_t st, int m, int nslots, const double *upd_part, const double *dot_part,
+void reduce_results(hipStream_t st, int m, int ncmd, const double *upd_part, const double *dot_part,
                     const double *sm_part, int nchunk, int nbchunk, double *res)
 {
     const int W = 3 * m + 8;
-    hipLaunchKernelGGL(k_reduce, dim3(nslots), dim3((W + 63) / 64 * 64), 0, st, m, upd_part, dot_part, sm_part, nchunk,
+    hipLaunchKernelGGL(k_reduce, dim3(ncmd), dim3((W + 63) / 64 * 64), 0, st, m, upd_part, dot_part, sm_part, nchunk,
                        nbchunk, res);
 }
 
