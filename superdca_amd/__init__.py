@@ -60,7 +60,7 @@ class Opts(C.Structure):
                 ("grad_threshold", C.c_double), ("verbose", C.c_int32), ("autotune", C.c_int32),
                 ("gemm_ksplit", C.c_int32), ("gemm_gm_energy", C.c_int32), ("gemm_gm_grad", C.c_int32),
                 ("gemm_tn_energy", C.c_int32), ("gemm_tn_grad", C.c_int32), ("pipeline_groups", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("coarse_factor", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class LocusStats(C.Structure):

@@ -311,13 +311,13 @@ int autotune(sdca_ctx *c, int nslots, const std::vector<GemmConfig> &cand, GemmC
     }
     const int reps = 3;
     auto energy = [&](const GemmConfig &g) {
-        sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_X, c->Mg, c->Mg, c->Bp,
+        sdca::kern::gemm_energy(st, c->opts.precision, c->Mg, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_X, c->Mg, c->Mg, c->Bp,
                                 (int)c->Kxp, g.ksplit, g.kchunk, g.gm_e, g.tn_e, c->d_Epart);
         sdca::kern::softmax(st, c->opts.precision, c->d_Epart, g.ksplit, c->Mg, c->Bp, c->B, c->q, nslots,
                             c->d_cmds, c->d_statesT, c->d_w, c->d_G32, c->d_Ghi, c->d_Glo, c->d_sm_part, c->nbchunk);
     };
     auto grad = [&](int gm, int tn) {
-        sdca::kern::gemm_grad(st, c->opts.precision, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Xt, c->Mg, c->Bp,
+        sdca::kern::gemm_grad(st, c->opts.precision, c->Mg, nslots, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Xt, c->Mg, c->Bp,
                               (int)c->Kxp, (int)c->Kx, c->q, nslots, c->d_cmds, c->d_Xtr, c->d_gbuf, c->n_slot,
                               (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), gm, tn);
     };
@@ -399,6 +399,8 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     c->gslots = nslots / ng;
     c->Mg = (int)round_up((int64_t)c->gslots * c->q, sdca::kTile);
     c->Mpad = ng * c->Mg;
+    c->Mgp = c->Mg + sdca::kTile;
+    const size_t prow = (size_t)ng * c->Mgp;  // packed rows of all groups
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
     int rc;
     c->ws_m = m;  // ring shapes below and the autotuner use it
@@ -406,15 +408,16 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     if ((rc = dalloc(&c->d_dbuf, (size_t)nslots * c->n_slot))) return rc;
     if ((rc = dalloc((uint8_t **)&c->d_S, (size_t)nslots * m * c->n_slot * pair_bytes(c)))) return rc;
     if ((rc = dalloc((uint8_t **)&c->d_Y, (size_t)nslots * m * c->n_slot * pair_bytes(c)))) return rc;
-    if ((rc = dalloc(&c->d_Xtr, (size_t)2 * c->Mpad * c->Kxp))) return rc;
-    c->d_Xc = c->d_Xtr + (size_t)c->Mpad * c->Kxp;
+    c->xc_off = (int64_t)prow * c->Kxp;
+    if ((rc = dalloc(&c->d_Xtr, (prow + c->Mpad) * c->Kxp))) return rc;
+    c->d_Xc = c->d_Xtr + c->xc_off;
     if (split) {
-        if ((rc = dalloc(&c->d_Whi, (size_t)c->Mpad * c->Kxp))) return rc;
-        if ((rc = dalloc(&c->d_Wlo, (size_t)c->Mpad * c->Kxp))) return rc;
-        if ((rc = dalloc(&c->d_Ghi, (size_t)c->Mpad * c->Bp))) return rc;
-        if ((rc = dalloc(&c->d_Glo, (size_t)c->Mpad * c->Bp))) return rc;
+        if ((rc = dalloc(&c->d_Whi, prow * c->Kxp))) return rc;
+        if ((rc = dalloc(&c->d_Wlo, prow * c->Kxp))) return rc;
+        if ((rc = dalloc(&c->d_Ghi, prow * c->Bp))) return rc;
+        if ((rc = dalloc(&c->d_Glo, prow * c->Bp))) return rc;
     } else {
-        if ((rc = dalloc(&c->d_G32, (size_t)c->Mpad * c->Bp))) return rc;
+        if ((rc = dalloc(&c->d_G32, prow * c->Bp))) return rc;
     }
     // GEMM configuration: the cached/tuned one for this shape, else the default rule
     const GemmKey key{c->opts.precision, c->Mg, c->Bp, c->Kxp};
@@ -438,7 +441,7 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
         cand = tuning_candidates(c);
         for (const auto &k : cand) max_ksplit = std::max(max_ksplit, k.ksplit);
     }
-    if ((rc = dalloc(&c->d_Epart, (size_t)max_ksplit * c->Mpad * c->Bp))) return rc;  // per group: ksplit x Mg x Bp
+    if ((rc = dalloc(&c->d_Epart, (size_t)max_ksplit * prow * c->Bp))) return rc;  // per group: ksplit x Mgp x Bp
     c->nchunk = (int)std::min<int64_t>(64, std::max<int64_t>(1, (c->n_slot / 4 + 255) / 256));
     c->nbchunk = (c->Bp + 255) / 256;
     if ((rc = dalloc(&c->d_upd_part, (size_t)nslots * c->nchunk * sdca::ResLayout{m}.nupd()))) return rc;
@@ -486,29 +489,38 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     const int m = c->ws_m, W = sdca::ResLayout{m}.width();
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
     const int ns = c->gslots, s0 = g * ns;
-    // command list: the active slots in slot order, each with its packed rows
+    // command list: the active slots, full-precision ones first, each part in slot order;
+    // the coarse part starts on a tile boundary of the packed rows
     std::vector<int> &lst = c->gcmd_slot[g];
     lst.clear();
     sdca::SlotCmd *hc = c->h_cmds + s0;
-    for (int i = 0; i < ns; ++i) {
-        if (cmds[i].mode == sdca::MODE_IDLE) continue;
-        hc[lst.size()] = cmds[i];
-        hc[lst.size()].slot = i;
-        hc[lst.size()].row = (int)lst.size() * c->q;
-        lst.push_back(i);
+    int ncoarse = 0, row = 0;
+    for (int part = 0; part <= 1; ++part) {
+        if (part == 1) row = (int)round_up(row, sdca::kTile);
+        for (int i = 0; i < ns; ++i) {
+            if (cmds[i].mode == sdca::MODE_IDLE || (cmds[i].coarse != 0) != (part == 1)) continue;
+            hc[lst.size()] = cmds[i];
+            hc[lst.size()].slot = i;
+            hc[lst.size()].row = row;
+            row += c->q;
+            lst.push_back(i);
+            ncoarse += part;
+        }
     }
-    const int nc = (int)lst.size();
+    const int nc = (int)lst.size(), nfull = nc - ncoarse;
     if (nc == 0) return SDCA_OK;
-    const int Mrows = (int)round_up((int64_t)nc * c->q, sdca::kTile);
+    const int Mc = (int)round_up((int64_t)ncoarse * c->q, sdca::kTile);  // packed rows of each part
+    const int Mf = (int)round_up((int64_t)nfull * c->q, sdca::kTile);
     const size_t vo = (size_t)s0 * c->n_slot;              // slot-vector offset of the group
-    const size_t ro = (size_t)g * c->Mg * c->Kxp;          // its packed trial-point rows
-    const size_t go = (size_t)g * c->Mg * c->Bp;           // its packed node-belief rows
+    const size_t ro = (size_t)g * c->Mgp * c->Kxp;         // its packed trial-point rows
+    const size_t go = (size_t)g * c->Mgp * c->Bp;          // its packed node-belief rows
+
     float *gbuf = c->d_gbuf + (size_t)s0 * 2 * c->n_slot;
     float *dbuf = c->d_dbuf + vo;
     const size_t po = (size_t)s0 * m * c->n_slot * pair_bytes(c);
     void *Sp = (uint8_t *)c->d_S + po, *Yp = (uint8_t *)c->d_Y + po;
     sdca::SlotCmd *dc = c->d_cmds + s0;
-    float *Epart = c->d_Epart + (size_t)g * c->ksplit * c->Mg * c->Bp;
+    float *Epart = c->d_Epart + (size_t)g * c->ksplit * c->Mgp * c->Bp;
     hipStream_t st = c->gstream[g];
     hipEvent_t *ev = c->gev[g];
     HIP_TRY(hipMemcpyAsync(dc, hc, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
@@ -519,20 +531,22 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
                        c->d_X0 ? c->d_X0 + vo : nullptr, c->n_slot, (int)c->Kxp, (int)c->Kx, upd_part, c->nchunk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[1], st));
-    sdca::kern::gemm_energy(st, c->opts.precision, c->d_Xtr + ro, split ? c->d_Whi + ro : nullptr,
-                            split ? c->d_Wlo + ro : nullptr, c->d_X, Mrows, c->Mg, c->Bp, (int)c->Kxp, c->ksplit,
+    // full rows [0, Mf): hi + lo (or FP32); coarse rows: one MFMA pass over the BF16 high parts
+    sdca::kern::gemm_energy(st, c->opts.precision, Mf, c->d_Xtr + ro, split ? c->d_Whi + ro : nullptr,
+                            split ? c->d_Wlo + ro : nullptr, c->d_X, Mc + Mf, c->Mgp, c->Bp, (int)c->Kxp, c->ksplit,
                             c->kchunk, c->gm_energy, c->tn_energy, Epart);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[2], st));
-    sdca::kern::softmax(st, c->opts.precision, Epart, c->ksplit, c->Mg, c->Bp, c->B, c->q, nc, dc, c->d_statesT,
+    sdca::kern::softmax(st, c->opts.precision, Epart, c->ksplit, c->Mgp, c->Bp, c->B, c->q, nc, dc, c->d_statesT,
                         c->d_w, split ? nullptr : c->d_G32 + go, split ? c->d_Ghi + go : nullptr,
                         split ? c->d_Glo + go : nullptr, c->d_sm_part + (size_t)s0 * c->nbchunk, c->nbchunk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[3], st));
-    sdca::kern::gemm_grad(st, c->opts.precision, split ? nullptr : c->d_G32 + go, split ? c->d_Ghi + go : nullptr,
-                          split ? c->d_Glo + go : nullptr, c->d_Xt, Mrows, c->Bp, (int)c->Kxp, (int)c->Kx, c->q, nc,
-                          dc, c->d_Xtr + ro, gbuf, c->n_slot, (float)(2.0 * c->lambda_J),
-                          (float)(2.0 * c->lambda_h), c->gm_grad, c->tn_grad);
+    sdca::kern::gemm_grad(st, c->opts.precision, Mf, nfull, split ? nullptr : c->d_G32 + go,
+                          split ? c->d_Ghi + go : nullptr, split ? c->d_Glo + go : nullptr, c->d_Xt, Mc + Mf, c->Bp,
+                          (int)c->Kxp, (int)c->Kx, c->q, nc, dc, c->d_Xtr + ro, gbuf, c->n_slot,
+                          (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), c->gm_grad, c->tn_grad);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[4], st));
     double *dot_part = c->d_dot_part + (size_t)s0 * c->nchunk * sdca::ResLayout{m}.ndots();
@@ -549,8 +563,8 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
 }
 
 // Waits for group g's round and accounts its kernel times and algorithmic traffic.  The
-// result rows arrive in command order and are moved to their slots; each listed slot's
-// trial point is recorded for scoring.
+// result rows arrive in command order (full-precision commands first) and are moved to their
+// slots; each listed slot's trial point is recorded for scoring.
 int finish_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
 {
     const int m = c->ws_m;
@@ -561,9 +575,11 @@ int finish_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     {
         const int W = sdca::ResLayout{m}.width(), s0 = g * c->gslots;
         double *res = c->h_res + (size_t)s0 * W;
-        for (int k = (int)lst.size() - 1; k >= 0; --k) {  // lst[k] >= k: rows move up only
-            if (lst[k] != k) std::memcpy(res + (size_t)lst[k] * W, res + (size_t)k * W, W * sizeof(double));
-            c->h_xt_off[s0 + lst[k]] = (int64_t)((size_t)g * c->Mg + (size_t)k * c->q) * c->Kxp;
+        std::vector<double> &tmp = c->h_res_cmd;
+        tmp.assign(res, res + lst.size() * W);
+        for (size_t k = 0; k < lst.size(); ++k) {
+            std::memcpy(res + (size_t)lst[k] * W, tmp.data() + k * W, W * sizeof(double));
+            c->h_xt_off[s0 + lst[k]] = (int64_t)((size_t)g * c->Mgp + (size_t)c->h_cmds[s0 + k].row) * c->Kxp;
         }
     }
     hipEvent_t *ev = c->gev[g];
@@ -598,7 +614,7 @@ int finish_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
                 bu += (k.cs[p] != 0.f ? pb : 0.0) + ((k.cy[p] != 0.f || ((k.live >> p) & 1u)) ? pb : 0.0);
             }
         }
-        if (split) bu += 4.0;  // hi + lo BF16 copies of x_t
+        if (split) bu += k.coarse ? 2.0 : 4.0;  // BF16 copies of x_t: hi (+ lo)
         bd += 4.0 * (md == sdca::MODE_INIT ? 1 : 3) + 2.0 * pb * npairs;
     }
     t.bytes_update += bu * (double)c->n_slot;
@@ -660,7 +676,8 @@ sdca::SlotCmd idle_cmd()
 
 // ------------------------------------------------------- per-slot solver state
 
-enum Phase { PH_IDLE, PH_INIT, PH_LS, PH_DONE };
+// PH_REBASE: the iterate is evaluated again at full precision after the coarse phase.
+enum Phase { PH_IDLE, PH_INIT, PH_LS, PH_REBASE, PH_DONE };
 
 // Host side of one slot: line-search state and the inner products of the stored vectors
 // that the two-loop recursion needs.  Pairs p hold s_p = alpha d and y_p = g_new - g_old;
@@ -683,6 +700,7 @@ struct Slot {
     double gnorm = 0.0;
     bool from_xt = true;
     bool preloaded = false;  // the direction buffer holds a first search direction
+    bool coarse = false;     // evaluations use the BF16 high parts only (coarse phase)
     int restarts = 0;        // line searches restarted from steepest descent
     int nfeval_first = 0;    // evaluations up to the first accepted step
     double SY[sdca::kMaxHistory][sdca::kMaxHistory], YY[sdca::kMaxHistory][sdca::kMaxHistory];
@@ -705,14 +723,19 @@ constexpr double kFirstStep = 1.0 / 16.0;
 
 class Solver {
 public:
-    Solver(sdca_ctx *c, int nslots) : c_(c), n_(nslots), m_(c->ws_m), slots_(nslots), cmds_(nslots, idle_cmd()),
-                                       f_noise_(c->opts.precision == SDCA_PREC_SPLIT_BF16 ? 1e-5 : 1e-6) {}
+    Solver(sdca_ctx *c, int nslots)
+        : c_(c), n_(nslots), m_(c->ws_m), slots_(nslots), cmds_(nslots, idle_cmd()),
+          f_noise_(c->opts.precision == SDCA_PREC_SPLIT_BF16 ? 1e-5 : 1e-6),
+          coarse_gnorm_(c->opts.precision == SDCA_PREC_SPLIT_BF16 && c->opts.coarse_factor > 0
+                            ? c->opts.coarse_factor * c->opts.grad_threshold
+                            : 0.0) {}
 
     void assign(int i, int item, int r, bool warm, bool preloaded = false)
     {
         Slot &S = slots_[i];
         S = Slot();
         S.preloaded = preloaded;
+        S.coarse = coarse_gnorm_ > 0.0;
         S.item = item;
         S.r = r;
         S.phase = PH_INIT;
@@ -758,6 +781,24 @@ public:
         const double ginf = res[R.ginf()];
         S.nfeval++;
         if (!std::isfinite(f) || !std::isfinite(ginf)) return finish(i, SDCA_ST_NONFINITE, S.phase == PH_INIT);
+        if (S.phase == PH_INIT && S.coarse && ginf < coarse_gnorm_) {
+            S.coarse = false;  // near the end already: evaluate the start point again in full
+            return false;
+        }
+        if (S.phase == PH_REBASE) {
+            // the full-precision gradient at x_c becomes the current one
+            std::swap(S.gk, S.gt);
+            S.GG = S.TT;
+            for (int p : S.pairs) {
+                S.SG[p] = S.TS[p];
+                S.YG[p] = S.TY[p];
+            }
+            S.f = f;
+            S.gnorm = ginf;
+            if (ginf < c_->opts.grad_threshold) return finish(i, SDCA_ST_CONVERGED, true);  // x_t = x_c
+            newdir(i, false, -1, 0.0);
+            return false;
+        }
         if (S.phase == PH_INIT) {
             S.gk = S.gt;
             S.gt = 1 - S.gk;
@@ -778,6 +819,7 @@ public:
             const double dg0 = S.DG;
             if (!(dg0 < 0.0)) {  // rounding destroyed descent: restart from steepest descent
                 S.pairs.clear();
+                if (S.coarse) return rebase(i);
                 newdir(i, false, -1, 0.0);  // the trial point along d is not accepted
                 return false;
             }
@@ -798,6 +840,10 @@ public:
             act = S.ms.step(f, dg, S.stp);
             S.ls_evals++;
             if (act == sdca::MoreThuente::Continue && S.ls_evals < c_->opts.max_ls) {
+                // a quasi-Newton step rejected twice in the coarse phase: the steps have
+                // become small against the BF16 rounding of x, where the coarse objective
+                // is too rough for the line search
+                if (S.coarse && !S.pairs.empty() && S.ls_evals >= 2) return rebase(i);
                 k.mode = sdca::MODE_RETRY;
                 k.gt = S.gt;
                 k.gk = S.gk;
@@ -805,6 +851,7 @@ public:
                 return false;
             }
             if (!(f < S.f)) {
+                if (S.coarse) return rebase(i);  // continue at full precision from x_c
                 if (S.pairs.empty() || S.restarts >= kMaxRestarts) return finish(i, SDCA_ST_LS_FAIL, false);
                 S.restarts++;
                 S.pairs.clear();
@@ -846,9 +893,32 @@ public:
         S.gnorm = ginf;
         S.iter++;
         if (S.iter == 1) S.nfeval_first = S.nfeval;
+        if (S.coarse && ginf < coarse_gnorm_) {
+            // end of the coarse phase: make the trial point the iterate (storing its pair)
+            // and evaluate it again at full precision
+            S.coarse = false;
+            newdir_identity(i, pnew, stp_trial);
+            S.phase = PH_REBASE;
+            return false;
+        }
         if (ginf < c_->opts.grad_threshold) return finish(i, SDCA_ST_CONVERGED, true);
         if (S.iter >= c_->opts.max_iter) return finish(i, SDCA_ST_ITER_CAP, true);
         newdir(i, true, pnew, stp_trial);
+        return false;
+    }
+
+    // End of the coarse phase without an accepted step: x_c is evaluated again at full
+    // precision (a trial step of zero along the current direction).
+    bool rebase(int i)
+    {
+        Slot &S = slots_[i];
+        S.coarse = false;
+        S.phase = PH_REBASE;
+        sdca::SlotCmd &k = cmds_[i];
+        k.mode = sdca::MODE_RETRY;
+        k.gt = S.gt;
+        k.gk = S.gk;
+        k.alpha = 0.f;
         return false;
     }
 
@@ -859,6 +929,7 @@ public:
             const Slot &S = slots_[i];
             sdca::SlotCmd &k = cmds_[i];
             k.live = 0u;
+            k.coarse = S.coarse ? 1 : 0;
             if (S.phase == PH_IDLE || k.mode == sdca::MODE_IDLE) continue;
             for (int p : S.pairs) k.live |= 1u << p;
         }
@@ -948,6 +1019,23 @@ private:
         for (int p = 0; p < sdca::kMaxHistory; ++p) k.cs[p] = k.cy[p] = 0.f;
     }
 
+    // x_c <- x_c + pstep d (the accepted trial point) with its pair pnew (if any), the
+    // direction kept, and a trial step of zero: the round evaluates the new iterate.
+    void newdir_identity(int i, int pnew, double pstep)
+    {
+        sdca::SlotCmd &k = cmds_[i];
+        k.mode = sdca::MODE_NEWDIR;
+        k.gt = slots_[i].gt;
+        k.gk = slots_[i].gk;
+        k.accept = 1;
+        k.pnew = pnew;
+        k.pstep = (float)pstep;
+        k.alpha = 0.f;
+        k.cd = 1.f;
+        k.cg = 0.f;
+        for (int p = 0; p < sdca::kMaxHistory; ++p) k.cs[p] = k.cy[p] = 0.f;
+    }
+
     // New search direction from the iterate: the last trial point x_c + pstep d if accept
     // (with the new pair pnew of that step, if any), else the current x_c (the start
     // point, or a restart after a failed or non-descent search).
@@ -996,6 +1084,8 @@ private:
     std::vector<Slot> slots_;
     std::vector<sdca::SlotCmd> cmds_;
     double f_noise_;  // relative rounding noise of the objective (FP32 or split-BF16 energies)
+    // split precision: coarse phase (BF16 high parts only) while ||g||_inf >= this; 0: none
+    double coarse_gnorm_;
 };
 
 // One-hot int8 X8 (L*QP rows, b contiguous) and the marginal counts, built on first use.
@@ -1044,6 +1134,7 @@ void sdca_default_opts(sdca_opts *o)
     o->grad_threshold = 1e-3;
     o->verbose = 0;
     o->autotune = 1;
+    o->coarse_factor = 4;
 }
 
 int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L, int32_t q, int32_t gap,
@@ -1062,6 +1153,8 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
         return fail(SDCA_E_ARG, "gemm_ksplit must be in [0, 4096] and gemm_gm_* in [0, 64]");
     for (int tn : {o.gemm_tn_energy, o.gemm_tn_grad})
         if (tn != 0 && tn != 128 && tn != 256) return fail(SDCA_E_ARG, "gemm_tn_* must be 0, 128 or 256");
+    if (o.coarse_factor < 0 || o.coarse_factor > 1000000)
+        return fail(SDCA_E_ARG, "coarse_factor must be 0 (off) or a factor >= 1 (got %d)", o.coarse_factor);
     if (o.pipeline_groups < 0 || o.pipeline_groups > sdca::kMaxGroups)
         return fail(SDCA_E_ARG, "pipeline_groups must be 0, 1 or 2");
     if (o.precision != SDCA_PREC_FP32 && o.precision != SDCA_PREC_SPLIT_BF16)
@@ -1431,7 +1524,7 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
                 sloci.clear();
                 for (int i : finished) {
                     const Slot &sl = S.slot(i);
-                    src_off.push_back(sl.from_xt ? c->h_xt_off[i] : (int64_t)c->Mpad * c->Kxp + (int64_t)i * c->n_slot);
+                    src_off.push_back(sl.from_xt ? c->h_xt_off[i] : c->xc_off + (int64_t)i * c->n_slot);
                     sloci.push_back(sl.r);
                 }
                 const int nf = (int)finished.size();

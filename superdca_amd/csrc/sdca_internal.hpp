@@ -58,6 +58,7 @@ struct SlotCmd {
     int32_t self_hi;
     int32_t slot;     // slot of the command within its group
     int32_t row;      // first of its q rows in the round's packed layout
+    int32_t coarse;   // 1: the round's GEMMs use the BF16 high parts only (split precision)
     int32_t gt;       // gradient buffer (0 or 1) that receives the trial gradient
     int32_t gk;       // gradient buffer of the current iterate (after NEWDIR's accept); -1 at INIT
     int32_t accept;   // NEWDIR: 1 = the last trial point becomes the iterate first (x_c += pstep d_old)
@@ -105,16 +106,19 @@ void hamming_neighbours(hipStream_t st, const uint64_t *planes, int B, int q, in
                         const uint32_t *d_mult, unsigned long long *d_iw);
 int energy_superiles(int Mpad, int Bp, int gm, int tn);  // super-tiles per split-K slice
 // Packed rows (trial points W, energies, beliefs): Mrows rows computed (a multiple of
-// kTile), Mz rows between split-K slices of Epart.  Command k of a launch owns rows
-// [k q, k q + q) of it.
-void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const uint32_t *X,
-                 int Mrows, int Mz, int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn, float *Epart);
+// kTile), Mz rows between split-K slices of Epart.  Command k owns rows [row, row + q).
+// Split precision: rows from m_lo on (coarse commands; m_lo a multiple of kTile) take one
+// MFMA pass over the BF16 high parts, the rows before them two (hi and lo).
+void gemm_energy(hipStream_t st, int prec, int m_lo, const void *W32, const void *Whi, const void *Wlo,
+                 const uint32_t *X, int Mrows, int Mz, int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn,
+                 float *Epart);
 void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mz, int Bp, int B, int q, int ncmd,
              const SlotCmd *cmds, const uint8_t *statesT, const float *wts, float *G32, uint16_t *Ghi,
              uint16_t *Glo, double *fpart, int nbchunk);
-void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const uint32_t *Xt,
-               int Mrows, int Bp, int Kxp, int Kx, int q, int ncmd, const SlotCmd *cmds, const float *Xtrial,
-               float *gbuf, int64_t n_slot, float lamJ2, float lamh2, int gm, int tn);
+// The first nfull commands own rows [0, nfull q), the coarse ones start at row m_lo.
+void gemm_grad(hipStream_t st, int prec, int m_lo, int nfull, const void *G32, const void *Ghi, const void *Glo,
+               const uint32_t *Xt, int Mrows, int Bp, int Kxp, int Kx, int q, int ncmd, const SlotCmd *cmds,
+               const float *Xtrial, float *gbuf, int64_t n_slot, float lamJ2, float lamh2, int gm, int tn);
 // Slot vectors: gbuf [slot][2][n_slot], dbuf [slot][n_slot], S and Y [slot][m][n_slot]
 // (uint16_t BF16 when pair_bf16, else float); Xtr, Whi and Wlo are packed rows.  One
 // block row per command (ncmd commands).
@@ -209,6 +213,8 @@ struct sdca_ctx {
     int64_t n_slot = 0;
     int Mpad = 0;
     int ngroups = 1, gslots = 0, Mg = 0;  // slot groups: slots and padded GEMM rows per group
+    int Mgp = 0;        // packed rows per group: Mg plus one tile (coarse and full parts pad separately)
+    int64_t xc_off = 0;  // offset of the iterates x_c in the d_Xtr allocation
     int ws_groups = 0;
     int ksplit = 1, kchunk = 0;
     int gm_energy = 8, gm_grad = 8;  // XCD super-tile heights (autotuned)
@@ -228,6 +234,7 @@ struct sdca_ctx {
     sdca::SlotCmd *h_cmds = nullptr;  // pinned host staging of the command lists
     std::vector<int> gcmd_slot[sdca::kMaxGroups];  // slot (in its group) of each listed command in flight
     std::vector<int64_t> h_xt_off;                 // per slot: d_Xtr offset of its last trial point
+    std::vector<double> h_res_cmd;                 // result rows in command order (host scratch)
     double *h_res = nullptr;          // pinned host result rows
     float *d_rows = nullptr;
     int64_t *d_src_off = nullptr;

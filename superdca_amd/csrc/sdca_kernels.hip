@@ -218,7 +218,9 @@ __device__ __forceinline__ void mma_tile(const uint8_t *sA0, const uint8_t *sA1,
 // (workgroup id % 8), so consecutive ids on one XCD are id, id+8, id+16, ...  Tiles are
 // grouped into super-tiles of GM x GN output tiles at one split-K index z; a super-tile
 // runs on a single XCD, whose L2 then serves each A rows-tile to GN workgroups and each
-// B rows-tile to GM workgroups.  Super-tiles are dealt to XCDs round-robin.
+// B rows-tile to GM workgroups.  Super-tiles are dealt to XCDs round-robin in the order
+// N fastest, then split-K, then M: the row tiles start in increasing order, so the rows
+// of a launch that take two MFMA passes (placed first) are not left for its end.
 struct TileMap {
     int Mt, Nt, Z;   // output tiles along M and N, split-K slices
     int GM, GN;      // super-tile shape
@@ -233,8 +235,8 @@ __device__ __forceinline__ bool map_tile(const TileMap &tm, int bid, int &mt, in
     if (sidx >= tm.S) return false;
     const int t = local - (local / per) * per;
     const int nb = sidx % tm.NB;
-    const int mb = (sidx / tm.NB) % tm.MB;
-    z = sidx / (tm.NB * tm.MB);
+    z = (sidx / tm.NB) % tm.Z;
+    const int mb = sidx / (tm.NB * tm.Z);
     mt = mb * tm.GM + t % tm.GM;
     nt = nb * tm.GN + t / tm.GM;
     return mt < tm.Mt && nt < tm.Nt;
@@ -242,17 +244,17 @@ __device__ __forceinline__ bool map_tile(const TileMap &tm, int bid, int &mt, in
 
 // Two workgroups per CU (eight waves): the register budget of a wave is 256 (VGPRs +
 // AGPRs), which holds the wide tile's accumulators and one prefetch stage.
+// One 128 x 64 NI output tile, split-K slice z, of C = A B^T with A = A0 (+ A1): the body
+// of k_gemm_nt for NA operand parts.
 template <class Pol, int NA, int NI, class Epi>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_nt(const typename Pol::T *__restrict__ A0,
-                                                    const typename Pol::T *__restrict__ A1, int64_t lda,
-                                                    const uint32_t *__restrict__ Bbits, int64_t ldbw, int K,
-                                                    int kchunk, TileMap tm, Epi epi)
+__device__ __forceinline__ void gemm_tile(uint8_t *sA0, uint8_t *sA1, uint8_t *sB,
+                                          const typename Pol::T *__restrict__ A0,
+                                          const typename Pol::T *__restrict__ A1, int64_t lda,
+                                          const uint32_t *__restrict__ Bbits, int64_t ldbw, int K, int kchunk,
+                                          int mt, int nt, int z, const Epi &epi)
 {
     constexpr int TN = 64 * NI;  // B rows (output columns) per tile
-    __shared__ __attribute__((aligned(16))) uint8_t sA[NA][kTile * ROWB];
-    __shared__ __attribute__((aligned(16))) uint8_t sB[TN * ROWB];
-    int mt, nt, z;
-    if (!map_tile(tm, blockIdx.x, mt, nt, z)) return;
+    uint8_t *sA[2] = {sA0, sA1};
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t m0 = (int64_t)mt * kTile, n0 = (int64_t)nt * TN;
     const int kb = z * kchunk;
@@ -304,7 +306,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         for (int i = 0; i < 4; ++i) {
             const int row = lrow + 32 * i;
 #pragma unroll
-            for (int v = 0; v < NA; ++v) *reinterpret_cast<u32x4 *>(&sA[v][row * ROWB + lcol]) = ra[v][i];
+            for (int v = 0; v < NA; ++v) *reinterpret_cast<u32x4 *>(sA[v] + row * ROWB + lcol) = ra[v][i];
         }
         // expand the B bits: 16-byte chunks of 8 BF16 or 4 FP32 ones and zeros
         {
@@ -335,6 +337,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     epi.template tile<NI>((int)m0, wm, n0 + wn, lane, tid, acc, z);
 }
 
+// Tiles of rows from m_lo on (a multiple of kTile) use the first operand part alone: under
+// split precision those are the rows of coarse commands, multiplied with the BF16 high
+// parts in one MFMA pass; the rows before them take both passes.  One launch serves both
+// kinds; the dearer tiles have the lower row indices and are dispatched first, so the
+// cheaper ones fill the end of the launch.
+template <class Pol, int NA, int NI, class Epi>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_nt(const typename Pol::T *__restrict__ A0,
+                                                    const typename Pol::T *__restrict__ A1, int64_t lda,
+                                                    const uint32_t *__restrict__ Bbits, int64_t ldbw, int K,
+                                                    int kchunk, int m_lo, TileMap tm, Epi epi)
+{
+    constexpr int TN = 64 * NI;
+    __shared__ __attribute__((aligned(16))) uint8_t sA[NA][kTile * ROWB];
+    __shared__ __attribute__((aligned(16))) uint8_t sB[TN * ROWB];
+    int mt, nt, z;
+    if (!map_tile(tm, blockIdx.x, mt, nt, z)) return;
+    if constexpr (NA == 2) {
+        if (mt * kTile >= m_lo) {
+            gemm_tile<Pol, 1, NI>(sA[0], sA[0], sB, A0, A0, lda, Bbits, ldbw, K, kchunk, mt, nt, z, epi);
+            return;
+        }
+    }
+    gemm_tile<Pol, NA, NI>(sA[0], sA[NA - 1], sB, A0, A1, lda, Bbits, ldbw, K, kchunk, mt, nt, z, epi);
+}
+
 // 32x32 MFMA result layout: register v of lane l holds row 8*(v/4) + 4*(l/32) + v%4,
 // column l%32 of the block.
 __device__ __forceinline__ int acc_row(int mi, int v, int lane) { return mi * 32 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3); }
@@ -362,7 +389,7 @@ struct EpiEnergy {
     }
 };
 
-// Gradient epilogue: row m = k*q + s of command k, column n.  Adds 2*lambda*x (lambda_h on
+// Gradient epilogue: packed row m = row_of(k) + s of command k, column n.  Adds 2*lambda*x (lambda_h on
 // the bias column, lambda_J elsewhere: plmDCA_objective.hpp:192-244), zeroes the locus'
 // own columns (the j != r exclusions of :95,111,161,176) and pad columns, and writes
 // the trial gradient into gradient buffer gt of the command's slot.  The commands under
@@ -374,14 +401,23 @@ struct EpiGrad {
     const SlotCmd *cmds;
     int64_t n_slot, Kxp;
     int ng, g0, q, Kx, ncmd;  // gradients per slot, ring index of the first, commands
+    int mcoarse, nfull;       // the coarse commands start at row mcoarse, after nfull others
     float lamJ2, lamh2;
+    // command owning packed row m (ncmd for the pad rows after the full-precision part),
+    // and its first row
+    __device__ __forceinline__ int cmd_of(int m) const
+    {
+        if (m >= mcoarse) return nfull + (m - mcoarse) / q;
+        return m < nfull * q ? m / q : ncmd;
+    }
+    __device__ __forceinline__ int row_of(int i) const { return i < nfull ? i * q : mcoarse + (i - nfull) * q; }
     template <int NI>
     __device__ __forceinline__ void tile(int m0, int wm, int64_t nw, int lane, int tid, floatx16 (&acc)[2][NI],
                                          int) const
     {
         // s_gt < 0: no gradient to write
         __shared__ int s_gt[kTile + 2], s_lo[kTile + 2], s_hi[kTile + 2], s_slot[kTile + 2];
-        const int i0 = m0 / q;
+        const int i0 = cmd_of(m0);
         const int ns = min(ncmd - i0, (kTile - 1) / q + 2);  // commands under rows [m0, m0 + 128)
         if (tid < ns) {
             const SlotCmd &c = cmds[i0 + tid];
@@ -401,7 +437,7 @@ struct EpiGrad {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int m = m0 + wm + acc_row(mi, vq + u, lane);
-                    const int i = m / q;
+                    const int i = cmd_of(m);
                     const int il = i - i0;
                     const bool ok = il < ns;
                     gt[u] = ok ? s_gt[il] : -1;
@@ -409,7 +445,7 @@ struct EpiGrad {
                     hi[u] = ok ? s_hi[il] : 0;
                     const int slot = ok ? s_slot[il] : 0;
                     roff[u] = ((int64_t)slot * ng + (gt[u] < 0 ? 0 : gt[u] - g0)) * n_slot +
-                              (int64_t)(m - i * q) * Kxp;
+                              (int64_t)(ok ? m - row_of(i) : 0) * Kxp;
                     const float *xr = Xtrial + (int64_t)m * Kxp;
 #pragma unroll
                     for (int ni = 0; ni < NI; ++ni) {
@@ -451,6 +487,7 @@ __global__ __launch_bounds__(256) void k_softmax(const float *__restrict__ Epart
     const int b = blockIdx.x * 256 + threadIdx.x;
     const int r = cmds[i].mode == MODE_IDLE ? -1 : cmds[i].locus;
     const int64_t row0 = cmds[i].row;
+    const bool coarse = cmds[i].coarse != 0;  // BF16 high parts only
     double fb = 0.0;
     if (b < Bp) {
         float e[8], g[8];
@@ -485,7 +522,7 @@ __global__ __launch_bounds__(256) void k_softmax(const float *__restrict__ Epart
             } else {
                 const uint16_t h = f2bf(g[s]);
                 Ghi[o] = h;
-                Glo[o] = f2bf(g[s] - bf2f(h));
+                if (!coarse) Glo[o] = f2bf(g[s] - bf2f(h));
             }
         }
     }
@@ -606,7 +643,8 @@ __device__ __forceinline__ float amax8(const f8 &x)
 //           d <- cd d + cg g_k + sum_p (cs[p] s_p + cy[p] y_p);
 //           x_t = x_c + alpha d; and the products d . g_k, d . y_p (live p) of the stored d.
 //           Pairs with a zero coefficient are read only for their product.
-// x_t (and its BF16 hi/lo copies) is written at the command's packed rows.  Plus the
+// x_t (and its BF16 hi/lo copies; hi only for a coarse command) is written at the
+// command's packed rows.  Plus the
 // squared norms of x_t for the L2 terms: part[command][chunk][M + 3] = {sum x^2 over
 // the whole vector, sum over the h entries (column Kx of each row), d . g_k, d . y_p}.
 template <int M, class PT>
@@ -644,6 +682,7 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
         const int pnew = c.pnew;
         const int64_t n8 = n_slot >> 3;
         const bool accept = c.accept != 0;
+        const bool coarse = c.coarse != 0;  // the GEMMs read the BF16 high part only
         const bool from_x0 = c.init_src != 0.f && x0;
         float cs[M], cy[M];
         bool live[M];
@@ -719,7 +758,7 @@ __global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds
                 hv.x = h[0], hv.y = h[1], hv.z = h[2], hv.w = h[3];
                 lv.x = l[0], lv.y = l[1], lv.z = l[2], lv.w = l[3];
                 *reinterpret_cast<u32x4 *>(Whi + xo + e) = hv;
-                *reinterpret_cast<u32x4 *>(Wlo + xo + e) = lv;
+                if (!coarse) *reinterpret_cast<u32x4 *>(Wlo + xo + e) = lv;
             }
             acc[0] += dot8(xn, xn);
             // the h entry of row s is column Kx (K_x is padded to a multiple of 8, so an
@@ -928,27 +967,28 @@ int energy_superiles(int Mpad, int Bp, int gm, int tn)
 // Launches k_gemm_nt with output tiles 128 x tn (tn = 128 or 256; N must be a multiple).
 template <class Pol, int NA, class Epi>
 static void launch_gemm(hipStream_t st, int tn, const TileMap &tm, const void *A0, const void *A1, int64_t lda,
-                        const uint32_t *Bbits, int64_t ldbw, int K, int kchunk, const Epi &epi)
+                        const uint32_t *Bbits, int64_t ldbw, int K, int kchunk, int m_lo, const Epi &epi)
 {
     using T = typename Pol::T;
     const dim3 grid(map_grid(tm));
     if (tn == 256)
         hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 4, Epi>), grid, dim3(256), 0, st, (const T *)A0, (const T *)A1, lda,
-                           Bbits, ldbw, K, kchunk, tm, epi);
+                           Bbits, ldbw, K, kchunk, m_lo, tm, epi);
     else
         hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 2, Epi>), grid, dim3(256), 0, st, (const T *)A0, (const T *)A1, lda,
-                           Bbits, ldbw, K, kchunk, tm, epi);
+                           Bbits, ldbw, K, kchunk, m_lo, tm, epi);
 }
 
-void gemm_energy(hipStream_t st, int prec, const void *W32, const void *Whi, const void *Wlo, const uint32_t *X, int Mrows,
-                 int Mz, int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn, float *Epart)
+void gemm_energy(hipStream_t st, int prec, int m_lo, const void *W32, const void *Whi, const void *Wlo,
+                 const uint32_t *X, int Mrows, int Mz, int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn,
+                 float *Epart)
 {
     EpiEnergy epi{Epart, (int64_t)Bp, (int64_t)Mz * Bp};
     const TileMap tm = make_map(Mrows / kTile, Bp / tn, ksplit, gm);
     if (prec == SDCA_PREC_FP32)
-        launch_gemm<PolF32, 1>(st, tn, tm, W32, W32, Kxp, X, Kxp / 32, Kxp, kchunk, epi);
+        launch_gemm<PolF32, 1>(st, tn, tm, W32, W32, Kxp, X, Kxp / 32, Kxp, kchunk, Mrows, epi);
     else
-        launch_gemm<PolBF16, 2>(st, tn, tm, Whi, Wlo, Kxp, X, Kxp / 32, Kxp, kchunk, epi);
+        launch_gemm<PolBF16, 2>(st, tn, tm, Whi, Wlo, Kxp, X, Kxp / 32, Kxp, kchunk, m_lo, epi);
 }
 
 void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mz, int Bp, int B, int q, int ncmd,
@@ -959,16 +999,16 @@ void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mz, i
                        cmds, statesT, wts, prec == SDCA_PREC_FP32 ? G32 : nullptr, Ghi, Glo, fpart, nbchunk);
 }
 
-void gemm_grad(hipStream_t st, int prec, const void *G32, const void *Ghi, const void *Glo, const uint32_t *Xt, int Mrows,
-               int Bp, int Kxp, int Kx, int q, int ncmd, const SlotCmd *cmds, const float *Xtrial, float *gbuf,
-               int64_t n_slot, float lamJ2, float lamh2, int gm, int tn)
+void gemm_grad(hipStream_t st, int prec, int m_lo, int nfull, const void *G32, const void *Ghi, const void *Glo,
+               const uint32_t *Xt, int Mrows, int Bp, int Kxp, int Kx, int q, int ncmd, const SlotCmd *cmds,
+               const float *Xtrial, float *gbuf, int64_t n_slot, float lamJ2, float lamh2, int gm, int tn)
 {
-    EpiGrad epi{gbuf, Xtrial, cmds, n_slot, (int64_t)Kxp, 2, 0, q, Kx, ncmd, lamJ2, lamh2};
+    EpiGrad epi{gbuf, Xtrial, cmds, n_slot, (int64_t)Kxp, 2, 0, q, Kx, ncmd, m_lo, nfull, lamJ2, lamh2};
     const TileMap tm = make_map(Mrows / kTile, Kxp / tn, 1, gm);
     if (prec == SDCA_PREC_FP32)
-        launch_gemm<PolF32, 1>(st, tn, tm, G32, G32, Bp, Xt, Bp / 32, Bp, Bp, epi);
+        launch_gemm<PolF32, 1>(st, tn, tm, G32, G32, Bp, Xt, Bp / 32, Bp, Bp, Mrows, epi);
     else
-        launch_gemm<PolBF16, 2>(st, tn, tm, Ghi, Glo, Bp, Xt, Bp / 32, Bp, Bp, epi);
+        launch_gemm<PolBF16, 2>(st, tn, tm, Ghi, Glo, Bp, Xt, Bp / 32, Bp, Bp, m_lo, epi);
 }
 
 #define SDCA_DISPATCH_M(m, CALL) \

@@ -256,6 +256,32 @@ def test_pipelined_slot_groups_match_single_group(c1, precision):
     assert rel.max() < (2e-3 if precision == sd.PREC_SPLIT_BF16 else 1e-3), rel.max()
 
 
+def test_coarse_phase_reaches_the_full_precision_optimum(c1):
+    # split-BF16 with the coarse phase (BF16 high parts only while ||g||_inf >= factor *
+    # threshold) off, at the default factor, and so large that every start point is
+    # already below it (re-evaluated in full at once); 80 loci through 64 slots, so coarse
+    # and full-precision slots share rounds and refills
+    st = c1["states"]
+    w, _ = orc.weights(st, 0.9, nthreads=8)
+    loci = np.arange(3, 1000, 12)[:80]
+    out = {}
+    for factor in (0, 4, 1000000):
+        with sd.Context(st, precision=sd.PREC_SPLIT_BF16, grad_threshold=1e-4, batch_loci=64,
+                        coarse_factor=factor) as ctx:
+            ctx.set_weights(w)
+            rows, stats, _, rc = ctx.solve(loci)
+        assert rc == sd.SDCA_OK, factor
+        assert np.all(stats["status"] == sd.ST_CONVERGED), factor
+        assert np.all(stats["gnorm"] < 1e-4), factor
+        out[factor] = (rows, stats)
+    for factor in (4, 1000000):
+        np.testing.assert_allclose(out[factor][1]["fval"], out[0][1]["fval"], rtol=1e-6)
+        big = out[0][0] > np.quantile(out[0][0], 0.9)
+        rel = np.abs(out[factor][0] - out[0][0])[big] / out[0][0][big]
+        assert rel.max() < 2e-3, (factor, rel.max())
+    print("coarse phase evaluations", {f: int(o[1]["nfeval"].sum()) for f, o in out.items()})
+
+
 @pytest.mark.parametrize("precision,ftol,gtol", [(sd.PREC_FP32, 2e-6, 2e-5), (sd.PREC_SPLIT_BF16, 2e-5, 2e-4)])
 def test_eval_and_solve_q5_envelope(c1, precision, ftol, gtol):
     # the genome-scale envelope {A,C,G,T,-}: q_env = 5, gap = state 4, with an envelope
