@@ -648,7 +648,7 @@ __device__ __forceinline__ float amax8(const f8 &x)
 // squared norms of x_t for the L2 terms: part[command][chunk][M + 3] = {sum x^2 over
 // the whole vector, sum over the h entries (column Kx of each row), d . g_k, d . y_p}.
 template <int M, class PT>
-__global__ __launch_bounds__(256) void k_update(const SlotCmd *__restrict__ cmds, const float *__restrict__ gbuf,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 : 4))) void k_update(const SlotCmd *__restrict__ cmds, const float *__restrict__ gbuf,
                                                 float *__restrict__ dbuf, PT *__restrict__ S, PT *__restrict__ Y,
                                                 float *__restrict__ Xc, float *__restrict__ Xtr,
                                                 uint16_t *__restrict__ Whi, uint16_t *__restrict__ Wlo,
