@@ -611,7 +611,7 @@ int finish_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
             bu += 4.0 * (5 + (k.accept != 0)) + (k.pnew >= 0 ? 4.0 + 2.0 * pb : 0.0);
             for (int p = 0; p < m; ++p) {
                 if (p == k.pnew) continue;
-                bu += (k.cs[p] != 0.f ? pb : 0.0) + ((k.cy[p] != 0.f || ((k.live >> p) & 1u)) ? pb : 0.0);
+                bu += (k.cs[p] != 0.f ? pb : 0.0) + (k.cy[p] != 0.f ? pb : 0.0);
             }
         }
         if (split) bu += k.coarse ? 2.0 : 4.0;  // BF16 copies of x_t: hi (+ lo)
@@ -683,7 +683,8 @@ enum Phase { PH_IDLE, PH_INIT, PH_LS, PH_REBASE, PH_DONE };
 // that the two-loop recursion needs.  Pairs p hold s_p = alpha d and y_p = g_new - g_old;
 // with g the current gradient, d the current direction and t the trial gradient:
 //   SY[p][q] = s_p.y_q, YY[p][q] = y_p.y_q, SG[p] = s_p.g, YG[p] = y_p.g, GG = g.g,
-//   DG = d.g, DY[p] = d.y_p, and the trial's TS[p], TY[p], TT, TG = t.g, TD = t.d.
+//   DG = d.g, DY[p] = d.y_p (from d's coefficients and SY, YY, YG), and the trial's
+//   TS[p], TY[p], TT, TG = t.g, TD = t.d.
 struct Slot {
     int item = -1;  // index into the caller's loci list
     int r = -1;
@@ -766,10 +767,7 @@ public:
         sdca::SlotCmd &k = cmds_[i];
         const sdca::ResLayout R{m_};
         const int pending = k.mode;
-        if (pending == sdca::MODE_NEWDIR) {  // products of the stored new direction
-            S.DG = res[R.dg()];
-            for (int p : S.pairs) S.DY[p] = res[R.dy(p)];
-        }
+        if (pending == sdca::MODE_NEWDIR) S.DG = res[R.dg()];  // d . g_k of the stored new direction
         for (int p : S.pairs) {
             S.TS[p] = res[R.ts(p)];
             S.TY[p] = res[R.ty(p)];
@@ -1023,6 +1021,8 @@ private:
     // direction kept, and a trial step of zero: the round evaluates the new iterate.
     void newdir_identity(int i, int pnew, double pstep)
     {
+        Slot &S = slots_[i];
+        if (pnew >= 0) S.DY[pnew] = S.TD - S.DG;  // d . (g_new - g_old); the older d . y_p stand
         sdca::SlotCmd &k = cmds_[i];
         k.mode = sdca::MODE_NEWDIR;
         k.gt = slots_[i].gt;
@@ -1076,6 +1076,13 @@ private:
         for (int p = 0; p < sdca::kMaxHistory; ++p) {
             k.cs[p] = p < m_ ? (float)cs[p] : 0.f;
             k.cy[p] = p < m_ ? (float)cy[p] : 0.f;
+        }
+        // d . y_p of the new direction from the pair products, with the coefficients as
+        // the device applies them: d = cg g_k + sum_q (cs_q s_q + cy_q y_q)
+        for (int p : S.pairs) {
+            double v = (double)k.cg * S.YG[p];
+            for (int q : S.pairs) v += (double)k.cs[q] * S.SY[q][p] + (double)k.cy[q] * S.YY[q][p];
+            S.DY[p] = v;
         }
     }
 

@@ -83,11 +83,10 @@ struct ResLayout {
     int xsqJ() const { return 2 * m + 4; }     // sum of squares of x_t, J part
     int xsqh() const { return 2 * m + 5; }     // sum of squares of x_t, h part
     int dg() const { return 2 * m + 6; }       // NEWDIR: d . g_k
-    int dy(int p) const { return 2 * m + 7 + p; }  // NEWDIR: d . y_p
-    int fdata() const { return 3 * m + 7; }    // sum_b w_b (LSE_b - E_b(x_br))
-    int width() const { return 3 * m + 8; }
+    int fdata() const { return 2 * m + 7; }    // sum_b w_b (LSE_b - E_b(x_br))
+    int width() const { return 2 * m + 8; }
     int ndots() const { return 2 * m + 4; }    // entries produced by the dot-product pass
-    int nupd() const { return m + 3; }         // entries produced by the trial-point pass
+    int nupd() const { return 3; }             // entries produced by the trial-point pass
 };
 
 struct Timing {

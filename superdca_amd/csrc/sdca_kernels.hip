@@ -641,12 +641,13 @@ __device__ __forceinline__ float amax8(const f8 &x)
 //   NEWDIR  x_c = x_c + pstep d_old (if accept: the last trial point); the new pair
 //           s = pstep d, y = g[gk] - g[1-gk] into pair slot pnew (if any);
 //           d <- cd d + cg g_k + sum_p (cs[p] s_p + cy[p] y_p);
-//           x_t = x_c + alpha d; and the products d . g_k, d . y_p (live p) of the stored d.
-//           Pairs with a zero coefficient are read only for their product.
+//           x_t = x_c + alpha d; and the product d . g_k of the stored d.  Pairs with a
+//           zero coefficient are not read.  (The products d . y_p follow on the host from
+//           the coefficients and the pair products.)
 // x_t (and its BF16 hi/lo copies; hi only for a coarse command) is written at the
-// command's packed rows.  Plus the
-// squared norms of x_t for the L2 terms: part[command][chunk][M + 3] = {sum x^2 over
-// the whole vector, sum over the h entries (column Kx of each row), d . g_k, d . y_p}.
+// command's packed rows.  Plus the squared norms of x_t for the L2 terms:
+// part[command][chunk][3] = {sum x^2 over the whole vector, sum over the h entries
+// (column Kx of each row), d . g_k}.
 template <int M, class PT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 : 4))) void k_update(const SlotCmd *__restrict__ cmds, const float *__restrict__ gbuf,
                                                 float *__restrict__ dbuf, PT *__restrict__ S, PT *__restrict__ Y,
@@ -655,7 +656,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
                                                 const float *__restrict__ X0, int64_t n_slot, int Kxp, int Kx, int q,
                                                 double *__restrict__ part, int nchunk)
 {
-    constexpr int NV = M + 3;
+    constexpr int NV = 3;
     const int j = blockIdx.y;
     const SlotCmd &c = cmds[j];
     double acc[NV];
@@ -685,12 +686,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
         const bool coarse = c.coarse != 0;  // the GEMMs read the BF16 high part only
         const bool from_x0 = c.init_src != 0.f && x0;
         float cs[M], cy[M];
-        bool live[M];
 #pragma unroll
         for (int p = 0; p < M; ++p) {
             cs[p] = c.cs[p];
             cy[p] = c.cy[p];
-            live[p] = (c.live >> p) & 1u;
         }
         for (int64_t e8 = (int64_t)blockIdx.x * 256 + threadIdx.x; e8 < n8; e8 += (int64_t)nchunk * 256) {
             const int64_t e = e8 * 8;
@@ -732,15 +731,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
                 }
                 st8(dv, e, d);
                 acc[2] += dot8(d, g);
-                // products with the pairs: the y chunks were just read by this thread and
-                // are re-read from cache rather than held in registers
-#pragma unroll
-                for (int p = 0; p < M; ++p) {
-                    if (!live[p]) continue;
-                    f8 yv = p == pnew ? zero8() : ld8(Yb + (int64_t)p * n_slot, e);
-                    if (p == pnew) yv = yn;
-                    acc[3 + p] += dot8(d, yv);
-                }
                 xn = step8(xv, alpha, d);
             }
             st8(xt, e, xn);
@@ -823,14 +813,14 @@ __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, 
 }
 
 // Per-command reduction of the chunk partials into the result row (ResLayout): the dots
-// pass fills [0, 2m+4), the trial-point pass [2m+4, 3m+7) (two squares, d.g_k, d.y_p;
-// the first square is the whole vector, stored as the J part), the softmax the data term.
+// pass fills [0, 2m+4), the trial-point pass [2m+4, 2m+7) (two squares and d.g_k; the
+// first square is the whole vector, stored as the J part), the softmax the data term.
 __global__ void k_reduce(int m, const double *__restrict__ upd_part, const double *__restrict__ dot_part,
                          const double *__restrict__ sm_part, int nchunk, int nbchunk, double *__restrict__ res)
 {
     const int i = blockIdx.x;
     const int t = threadIdx.x;
-    const int ND = 2 * m + 4, NU = m + 3, W = 3 * m + 8;
+    const int ND = 2 * m + 4, NU = 3, W = 2 * m + 8;
     double *out = res + (int64_t)i * W;
     if (t < ND) {
         const bool mx = t == ND - 1;
@@ -1054,7 +1044,7 @@ void dots(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int ncmd, 
 void reduce_results(hipStream_t st, int m, int ncmd, const double *upd_part, const double *dot_part,
                     const double *sm_part, int nchunk, int nbchunk, double *res)
 {
-    const int W = 3 * m + 8;
+    const int W = ResLayout{m}.width();
     hipLaunchKernelGGL(k_reduce, dim3(ncmd), dim3((W + 63) / 64 * 64), 0, st, m, upd_part, dot_part, sm_part, nchunk,
                        nbchunk, res);
 }
