@@ -442,6 +442,7 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
         for (const auto &k : cand) max_ksplit = std::max(max_ksplit, k.ksplit);
     }
     if ((rc = dalloc(&c->d_Epart, (size_t)max_ksplit * prow * c->Bp))) return rc;  // per group: ksplit x Mgp x Bp
+    c->ksplit_alloc = max_ksplit;
     c->nchunk = (int)std::min<int64_t>(64, std::max<int64_t>(1, (c->n_slot / 4 + 255) / 256));
     c->nbchunk = (c->Bp + 255) / 256;
     if ((rc = dalloc(&c->d_upd_part, (size_t)nslots * c->nchunk * sdca::ResLayout{m}.nupd()))) return rc;
@@ -520,7 +521,7 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     const size_t po = (size_t)s0 * m * c->n_slot * pair_bytes(c);
     void *Sp = (uint8_t *)c->d_S + po, *Yp = (uint8_t *)c->d_Y + po;
     sdca::SlotCmd *dc = c->d_cmds + s0;
-    float *Epart = c->d_Epart + (size_t)g * c->ksplit * c->Mgp * c->Bp;
+    float *Epart = c->d_Epart + (size_t)g * c->ksplit_alloc * c->Mgp * c->Bp;
     hipStream_t st = c->gstream[g];
     hipEvent_t *ev = c->gev[g];
     HIP_TRY(hipMemcpyAsync(dc, hc, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
@@ -531,19 +532,34 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
                        c->d_X0 ? c->d_X0 + vo : nullptr, c->n_slot, (int)c->Kxp, (int)c->Kx, upd_part, c->nchunk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[1], st));
+    // A round with few active rows (the end of a solve) takes proportionally more split-K
+    // slices, packed Mrows apart in the group's partial-energy buffer, so that its energy
+    // launch still fills the GPU.
+    const int Mrows = Mc + Mf;
+    int ksplit = c->ksplit, kchunk = c->kchunk, Mz = c->Mgp;
+    if (2 * Mrows <= c->Mg) {
+        GemmConfig gs;
+        const int64_t cap = std::min<int64_t>(128, (int64_t)c->ksplit_alloc * c->Mgp / Mrows);
+        set_split(c, std::min<int64_t>(cap, (int64_t)c->ksplit * c->Mg / Mrows), gs);
+        if (gs.ksplit <= cap && gs.ksplit > ksplit) {
+            ksplit = gs.ksplit;
+            kchunk = gs.kchunk;
+            Mz = Mrows;
+        }
+    }
     // full rows [0, Mf): hi + lo (or FP32); coarse rows: one MFMA pass over the BF16 high parts
     sdca::kern::gemm_energy(st, c->opts.precision, Mf, c->d_Xtr + ro, split ? c->d_Whi + ro : nullptr,
-                            split ? c->d_Wlo + ro : nullptr, c->d_X, Mc + Mf, c->Mgp, c->Bp, (int)c->Kxp, c->ksplit,
-                            c->kchunk, c->gm_energy, c->tn_energy, Epart);
+                            split ? c->d_Wlo + ro : nullptr, c->d_X, Mrows, Mz, c->Bp, (int)c->Kxp, ksplit, kchunk,
+                            c->gm_energy, c->tn_energy, Epart);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[2], st));
-    sdca::kern::softmax(st, c->opts.precision, Epart, c->ksplit, c->Mgp, c->Bp, c->B, c->q, nc, dc, c->d_statesT,
+    sdca::kern::softmax(st, c->opts.precision, Epart, ksplit, Mz, c->Bp, c->B, c->q, nc, dc, c->d_statesT,
                         c->d_w, split ? nullptr : c->d_G32 + go, split ? c->d_Ghi + go : nullptr,
                         split ? c->d_Glo + go : nullptr, c->d_sm_part + (size_t)s0 * c->nbchunk, c->nbchunk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[3], st));
     sdca::kern::gemm_grad(st, c->opts.precision, Mf, nfull, split ? nullptr : c->d_G32 + go,
-                          split ? c->d_Ghi + go : nullptr, split ? c->d_Glo + go : nullptr, c->d_Xt, Mc + Mf, c->Bp,
+                          split ? c->d_Ghi + go : nullptr, split ? c->d_Glo + go : nullptr, c->d_Xt, Mrows, c->Bp,
                           (int)c->Kxp, (int)c->Kx, c->q, nc, dc, c->d_Xtr + ro, gbuf, c->n_slot,
                           (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), c->gm_grad, c->tn_grad);
     HIP_TRY(hipGetLastError());

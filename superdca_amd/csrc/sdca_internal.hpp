@@ -216,6 +216,7 @@ struct sdca_ctx {
     int64_t xc_off = 0;  // offset of the iterates x_c in the d_Xtr allocation
     int ws_groups = 0;
     int ksplit = 1, kchunk = 0;
+    int ksplit_alloc = 1;            // split-K slices of Mgp rows the partial-energy buffer holds
     int gm_energy = 8, gm_grad = 8;  // XCD super-tile heights (autotuned)
     int tn_energy = 128, tn_grad = 128;  // output tile widths (autotuned)
     int tuned = 0;                   // 1 if the autotuner chose the GEMM configuration
