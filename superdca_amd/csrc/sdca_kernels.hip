@@ -567,26 +567,34 @@ __device__ __forceinline__ double dot4(float4 a, float4 b)
 // The vector passes move 8 consecutive elements per thread and iteration: two float4 of
 // an FP32 vector, or one 16-byte load of a BF16 curvature pair.  Pair loads widen to
 // float; stores round and hand back the rounded value (every product uses the stored
-// vector).
+// vector).  Every slot vector is streamed once per pass and is far larger than the
+// caches, so loads and stores are non-temporal.
 struct f8 {
     float4 a, b;
 };
 __device__ __forceinline__ f8 zero8() { return {make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)}; }
+typedef float fx4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ f8 ld8(const float *v, int64_t e)
 {
-    return {*reinterpret_cast<const float4 *>(v + e), *reinterpret_cast<const float4 *>(v + e + 4)};
+    const fx4 a = __builtin_nontemporal_load(reinterpret_cast<const fx4 *>(v + e));
+    const fx4 b = __builtin_nontemporal_load(reinterpret_cast<const fx4 *>(v + e + 4));
+    return {make_float4(a.x, a.y, a.z, a.w), make_float4(b.x, b.y, b.z, b.w)};
 }
-__device__ __forceinline__ f8 ld8(const uint16_t *v, int64_t e)
+__device__ __forceinline__ f8 unpack8(const u32x4 &u)
 {
-    const u32x4 u = *reinterpret_cast<const u32x4 *>(v + e);
     auto lo = [](uint32_t w) { return __uint_as_float(w << 16); };
     auto hi = [](uint32_t w) { return __uint_as_float(w & 0xffff0000u); };
     return {make_float4(lo(u.x), hi(u.x), lo(u.y), hi(u.y)), make_float4(lo(u.z), hi(u.z), lo(u.w), hi(u.w))};
 }
+__device__ __forceinline__ f8 ld8(const uint16_t *v, int64_t e)
+{
+    return unpack8(__builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(v + e)));
+}
 __device__ __forceinline__ void st8(float *v, int64_t e, const f8 &x)
 {
-    *reinterpret_cast<float4 *>(v + e) = x.a;
-    *reinterpret_cast<float4 *>(v + e + 4) = x.b;
+    const fx4 a = {x.a.x, x.a.y, x.a.z, x.a.w}, b = {x.b.x, x.b.y, x.b.z, x.b.w};
+    __builtin_nontemporal_store(a, reinterpret_cast<fx4 *>(v + e));
+    __builtin_nontemporal_store(b, reinterpret_cast<fx4 *>(v + e + 4));
 }
 __device__ __forceinline__ void st8(float *v, int64_t e, f8 &x, int) { st8(v, e, x); }
 __device__ __forceinline__ void st8(uint16_t *v, int64_t e, f8 &x, int)
@@ -600,8 +608,8 @@ __device__ __forceinline__ void st8(uint16_t *v, int64_t e, f8 &x, int)
     u.y = w[1];
     u.z = w[2];
     u.w = w[3];
-    *reinterpret_cast<u32x4 *>(v + e) = u;
-    x = ld8(reinterpret_cast<const uint16_t *>(&u), 0);
+    __builtin_nontemporal_store(u, reinterpret_cast<u32x4 *>(v + e));
+    x = unpack8(u);
 }
 __device__ __forceinline__ f8 lin(float a, const f8 &x, float b, const f8 &y)
 {
@@ -747,8 +755,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
                 u32x4 hv, lv;
                 hv.x = h[0], hv.y = h[1], hv.z = h[2], hv.w = h[3];
                 lv.x = l[0], lv.y = l[1], lv.z = l[2], lv.w = l[3];
-                *reinterpret_cast<u32x4 *>(Whi + xo + e) = hv;
-                if (!coarse) *reinterpret_cast<u32x4 *>(Wlo + xo + e) = lv;
+                __builtin_nontemporal_store(hv, reinterpret_cast<u32x4 *>(Whi + xo + e));
+                if (!coarse) __builtin_nontemporal_store(lv, reinterpret_cast<u32x4 *>(Wlo + xo + e));
             }
             acc[0] += dot8(xn, xn);
             // the h entry of row s is column Kx (K_x is padded to a multiple of 8, so an
