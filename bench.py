@@ -60,14 +60,16 @@ def run_steps(ctx, order, pos, per_step, steps, label, solve):
 
 
 def roofline(tm, split):
+    # one launch of each kernel per round (the energy and gradient launches are counted
+    # separately by the library)
     groups = {
-        "k_gemm_nt<EpiGrad>": (tm["gemm_grad_ms"], tm["flops_grad"], "mfma"),
-        "k_gemm_nt<EpiEnergy>": (tm["gemm_energy_ms"], tm["flops_energy"], "mfma"),
-        "k_update": (tm["update_ms"], tm["bytes_update"], "hbm"),
-        "k_dots": (tm["dots_ms"], tm["bytes_dots"], "hbm"),
+        "k_gemm_nt<EpiGrad>": (tm["gemm_grad_ms"], tm["flops_grad"], "mfma", tm["n_grad_launches"]),
+        "k_gemm_nt<EpiEnergy>": (tm["gemm_energy_ms"], tm["flops_energy"], "mfma", tm["n_energy_launches"]),
+        "k_update": (tm["update_ms"], tm["bytes_update"], "hbm", tm["n_evals"]),
+        "k_dots": (tm["dots_ms"], tm["bytes_dots"], "hbm", tm["n_evals"]),
     }
     dom = max(groups, key=lambda k: groups[k][0])
-    ms, work, bound = groups[dom]
+    ms, work, bound, n = groups[dom]
     if bound == "mfma":
         peak = PEAK["bf16_tflops"] if split else PEAK["fp32_tflops"]
         achieved, unit = work / (ms * 1e-3) / 1e12, "TFLOP/s"
@@ -76,7 +78,9 @@ def roofline(tm, split):
         achieved, unit = work / (ms * 1e-3) / 1e9, "GB/s"
     rates = {k: round(v[1] / (v[0] * 1e-3) / (1e12 if v[2] == "mfma" else 1e9), 2) for k, v in groups.items() if v[0]}
     return ({"bound": bound, "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-             "frac": round(achieved / peak, 5), "traffic": None}, rates)
+             "frac": round(achieved / peak, 5), "traffic": None, "launches": int(n),
+             "avg_launch_us": round(ms * 1e3 / max(n, 1), 1),
+             "work_per_launch": round(work / max(n, 1) / (1e12 if bound == "mfma" else 1e9), 4)}, rates)
 
 
 def device_ms(tm):
