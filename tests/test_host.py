@@ -34,6 +34,17 @@ def test_default_opts_match_reference_defaults():
     assert o.max_iter == 2000        # plmDCA.hpp:219
     assert o.history == 10
     assert o.precision == sd.PREC_FP32
+    assert o.coarse_factor == 4  # split precision: BF16 high parts alone while ||g||_inf >= 4 * threshold
+
+
+@pytest.mark.parametrize("bad", [{"coarse_factor": -1}, {"pipeline_groups": 3}, {"history": 7},
+                                 {"gemm_tn_energy": 192}, {"gemm_gm_grad": 65}, {"precision": 9}])
+def test_invalid_options_are_rejected_before_any_device_call(bad):
+    # option checks precede device selection, so they hold without a GPU
+    st = np.zeros((4, 3), dtype=np.uint8)
+    with pytest.raises(sd.SdcaError) as e:
+        sd.Context(st, **bad)
+    assert e.value.code == sd.SDCA_E_ARG, bad
 
 
 def test_prepare_alignment_matches_column_stats():
