@@ -517,7 +517,7 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     const size_t go = (size_t)g * c->Mgp * c->Bp;          // its packed node-belief rows
 
     float *gbuf = c->d_gbuf + (size_t)s0 * 2 * c->n_slot;
-    float *dbuf = c->d_dbuf + vo;
+    void *dbuf = split ? (void *)(reinterpret_cast<uint16_t *>(c->d_dbuf) + vo) : (void *)(c->d_dbuf + vo);
     const size_t po = (size_t)s0 * m * c->n_slot * pair_bytes(c);
     void *Sp = (uint8_t *)c->d_S + po, *Yp = (uint8_t *)c->d_Y + po;
     sdca::SlotCmd *dc = c->d_cmds + s0;
@@ -621,17 +621,17 @@ int finish_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
         int npairs = 0;
         for (int p = 0; p < m; ++p) npairs += (k.live >> p) & 1u;
         if (md == sdca::MODE_INIT) bu += 4.0 * (2 + (k.init_src != 0.f));
-        else if (md == sdca::MODE_RETRY) bu += 12.0;  // x_c and d read, x_t written
+        else if (md == sdca::MODE_RETRY) bu += 8.0 + pb;  // x_c and d read, x_t written
         else {
             // x, d and g_k read; x_t, d (and x_c) written; the new pair from g_old
-            bu += 4.0 * (5 + (k.accept != 0)) + (k.pnew >= 0 ? 4.0 + 2.0 * pb : 0.0);
+            bu += 4.0 * (3 + (k.accept != 0)) + 2.0 * pb + (k.pnew >= 0 ? 4.0 + 2.0 * pb : 0.0);
             for (int p = 0; p < m; ++p) {
                 if (p == k.pnew) continue;
                 bu += (k.cs[p] != 0.f ? pb : 0.0) + (k.cy[p] != 0.f ? pb : 0.0);
             }
         }
         if (split) bu += k.coarse ? 2.0 : 4.0;  // BF16 copies of x_t: hi (+ lo)
-        bd += 4.0 * (md == sdca::MODE_INIT ? 1 : 3) + 2.0 * pb * npairs;
+        bd += (md == sdca::MODE_INIT ? 4.0 : 8.0 + pb) + 2.0 * pb * npairs;  // g_t (, g_k, d) and the pairs
     }
     t.bytes_update += bu * (double)c->n_slot;
     t.bytes_dots += bd * (double)c->n_slot;
@@ -1488,15 +1488,19 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
         packed.insert(packed.end(), wl_locus[g].begin(), wl_locus[g].end());
         int32_t *dw = d_wl + (size_t)3 * g * gs;
         HIP_TRY(hipMemcpyAsync(dw, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, st));
-        // blocks: into the staged start, or into ring vector m (first search direction)
-        float *XJ = cw->as_direction ? c->d_dbuf : c->d_X0;
+        // blocks: into the staged start, or into the direction buffer (first search
+        // direction, in the pair precision)
+        const bool dbf = cw->as_direction && c->opts.precision == SDCA_PREC_SPLIT_BF16;
+        void *XJ = cw->as_direction ? (void *)c->d_dbuf : (void *)c->d_X0;
+        const size_t eb = dbf ? 2 : 4;  // bytes per element of XJ
         const int64_t jstride = c->n_slot;
         for (int i : wl_slot[g]) {
             HIP_TRY(hipMemsetAsync(c->d_X0 + (size_t)i * c->n_slot, 0, c->n_slot * 4, st));
-            if (cw->as_direction) HIP_TRY(hipMemsetAsync(XJ + (size_t)i * jstride, 0, c->n_slot * 4, st));
+            if (cw->as_direction)
+                HIP_TRY(hipMemsetAsync((uint8_t *)XJ + (size_t)i * jstride * eb, 0, c->n_slot * eb, st));
         }
         sdca::kern::scatter_warm(st, dw, dw + nw, dw + 2 * nw, cw->d_partners, cw->d_J, cw->d_h, nw, cw->P, c->q,
-                                 c->Kxp, (int)c->Kx, c->d_state_col, c->n_slot, c->d_X0, XJ, jstride);
+                                 c->Kxp, (int)c->Kx, c->d_state_col, c->n_slot, c->d_X0, XJ, dbf, jstride);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(st));  // packed is a host temporary
         wl_slot[g].clear();

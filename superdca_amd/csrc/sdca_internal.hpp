@@ -119,13 +119,13 @@ void gemm_grad(hipStream_t st, int prec, int m_lo, int nfull, const void *G32, c
                const uint32_t *Xt, int Mrows, int Bp, int Kxp, int Kx, int q, int ncmd, const SlotCmd *cmds,
                const float *Xtrial, float *gbuf, int64_t n_slot, float lamJ2, float lamh2, int gm, int tn);
 // Slot vectors: gbuf [slot][2][n_slot], dbuf [slot][n_slot], S and Y [slot][m][n_slot]
-// (uint16_t BF16 when pair_bf16, else float); Xtr, Whi and Wlo are packed rows.  One
+// (dbuf, S and Y uint16_t BF16 when pair_bf16, else float); Xtr, Whi and Wlo are packed rows.  One
 // block row per command (ncmd commands).
 void update(hipStream_t st, int m, int q, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
-            float *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0,
+            void *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0,
             int64_t n_slot, int Kxp, int Kx, double *part, int nchunk);
 void dots(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
-          const float *dbuf, const void *S, const void *Y, int64_t n_slot, double *part, int nchunk);
+          const void *dbuf, const void *S, const void *Y, int64_t n_slot, double *part, int nchunk);
 void reduce_results(hipStream_t st, int m, int ncmd, const double *upd_part, const double *dot_part,
                     const double *sm_part, int nchunk, int nbchunk, double *res);
 void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *loci, int nscore, int L,
@@ -152,7 +152,7 @@ void gather_rows(hipStream_t st, const float *src, const int64_t *idx, int64_t n
 void pair_features(hipStream_t st, const int32_t *counts, int64_t npairs, int q, int B, float *feats);
 void scatter_warm(hipStream_t st, const int32_t *slot_of, const int32_t *item_of, const int32_t *locus_of,
                   const int32_t *partners, const float *J, const float *h, int nwarm, int P, int q, int64_t Kxp, int Kx,
-                  const int32_t *state_col, int64_t n_slot, float *X0, float *XJ, int64_t jstride);
+                  const int32_t *state_col, int64_t n_slot, float *X0, void *XJ, bool xj_bf16, int64_t jstride);
 }  // namespace kern
 
 // Warm starts of a solve call in compact device form: for item k (the k-th listed locus)
@@ -222,7 +222,7 @@ struct sdca_ctx {
     int tuned = 0;                   // 1 if the autotuner chose the GEMM configuration
     int nchunk = 0, nbchunk = 0;
     float *d_gbuf = nullptr;    // gradients: slots x 2 x n_slot (current and trial alternate)
-    float *d_dbuf = nullptr;    // search directions: slots x n_slot
+    float *d_dbuf = nullptr;    // search directions: slots x n_slot (BF16 under split precision)
     void *d_S = nullptr, *d_Y = nullptr;  // curvature pairs: slots x m x n_slot (BF16 or float)
     float *d_Xc = nullptr, *d_Xtr = nullptr, *d_X0 = nullptr;
     uint16_t *d_Whi = nullptr, *d_Wlo = nullptr;

@@ -658,7 +658,7 @@ __device__ __forceinline__ float amax8(const f8 &x)
 // (column Kx of each row), d . g_k}.
 template <int M, class PT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 : 4))) void k_update(const SlotCmd *__restrict__ cmds, const float *__restrict__ gbuf,
-                                                float *__restrict__ dbuf, PT *__restrict__ S, PT *__restrict__ Y,
+                                                PT *__restrict__ dbuf, PT *__restrict__ S, PT *__restrict__ Y,
                                                 float *__restrict__ Xc, float *__restrict__ Xtr,
                                                 uint16_t *__restrict__ Whi, uint16_t *__restrict__ Wlo,
                                                 const float *__restrict__ X0, int64_t n_slot, int Kxp, int Kx, int q,
@@ -681,7 +681,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
         const int64_t i = c.slot, xo = (int64_t)c.row * Kxp;
         float *xc = Xc + i * n_slot;
         float *xt = Xtr + xo;
-        float *dv = dbuf + (int64_t)i * n_slot;
+        PT *dv = dbuf + (int64_t)i * n_slot;
         const float *x0 = X0 ? X0 + (int64_t)i * n_slot : nullptr;
         const float *gk = gbuf + ((int64_t)i * 2 + (c.gk == 1)) * n_slot;
         const float *gold = gbuf + ((int64_t)i * 2 + (c.gk != 1)) * n_slot;
@@ -737,7 +737,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
                         axpy8(d, cy[p], yv);
                     }
                 }
-                st8(dv, e, d);
+                st8(dv, e, d, 0);  // rounded to the stored precision: the products use it
                 acc[2] += dot8(d, g);
                 xn = step8(xv, alpha, d);
             }
@@ -776,7 +776,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
 // part[command][chunk][2M + 4] = {g_t.s_p, g_t.y_p, g_t.g_t, g_t.g_k, g_t.d, max |g_t|}.
 template <int M, class PT>
 __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, const float *__restrict__ gbuf,
-                                              const float *__restrict__ dbuf, const PT *__restrict__ S,
+                                              const PT *__restrict__ dbuf, const PT *__restrict__ S,
                                               const PT *__restrict__ Y, int64_t n_slot, double *__restrict__ part,
                                               int nchunk)
 {
@@ -795,7 +795,7 @@ __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, 
         const float *g = gbuf + (i * 2 + c.gt) * n_slot;
         const bool cur = c.mode != MODE_INIT;  // a current gradient and a direction exist
         const float *gk = gbuf + (i * 2 + (1 - c.gt)) * n_slot;
-        const float *dv = dbuf + i * n_slot;
+        const PT *dv = dbuf + i * n_slot;
         const PT *Sb = S + i * M * n_slot;
         const PT *Yb = Y + i * M * n_slot;
         const uint32_t live = c.live;
@@ -1021,30 +1021,31 @@ void gemm_grad(hipStream_t st, int prec, int m_lo, int nfull, const void *G32, c
     }
 
 void update(hipStream_t st, int m, int q, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
-            float *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0,
+            void *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0,
             int64_t n_slot, int Kxp, int Kx, double *part, int nchunk)
 {
 #define SDCA_UPD(MM)                                                                                                \
     if (pair_bf16)                                                                                                  \
-        hipLaunchKernelGGL((k_update<MM, uint16_t>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf, dbuf,    \
-                           (uint16_t *)S, (uint16_t *)Y, Xc, Xtr, Whi, Wlo, X0, n_slot, Kxp, Kx, q, part, nchunk); \
+        hipLaunchKernelGGL((k_update<MM, uint16_t>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf,          \
+                           (uint16_t *)dbuf, (uint16_t *)S, (uint16_t *)Y, Xc, Xtr, Whi, Wlo, X0, n_slot, Kxp, Kx, \
+                           q, part, nchunk);                                                                        \
     else                                                                                                            \
-        hipLaunchKernelGGL((k_update<MM, float>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf, dbuf,       \
+        hipLaunchKernelGGL((k_update<MM, float>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf, (float *)dbuf, \
                            (float *)S, (float *)Y, Xc, Xtr, Whi, Wlo, X0, n_slot, Kxp, Kx, q, part, nchunk)
     SDCA_DISPATCH_M(m, SDCA_UPD)
 #undef SDCA_UPD
 }
 
 void dots(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
-          const float *dbuf, const void *S, const void *Y, int64_t n_slot, double *part, int nchunk)
+          const void *dbuf, const void *S, const void *Y, int64_t n_slot, double *part, int nchunk)
 {
 #define SDCA_DOT(MM)                                                                                                \
     if (pair_bf16)                                                                                                  \
-        hipLaunchKernelGGL((k_dots<MM, uint16_t>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf, dbuf,      \
-                           (const uint16_t *)S, (const uint16_t *)Y, n_slot, part, nchunk);                         \
+        hipLaunchKernelGGL((k_dots<MM, uint16_t>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf,            \
+                           (const uint16_t *)dbuf, (const uint16_t *)S, (const uint16_t *)Y, n_slot, part, nchunk); \
     else                                                                                                            \
-        hipLaunchKernelGGL((k_dots<MM, float>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf, dbuf,         \
-                           (const float *)S, (const float *)Y, n_slot, part, nchunk)
+        hipLaunchKernelGGL((k_dots<MM, float>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf,               \
+                           (const float *)dbuf, (const float *)S, (const float *)Y, n_slot, part, nchunk)
     SDCA_DISPATCH_M(m, SDCA_DOT)
 #undef SDCA_DOT
 }

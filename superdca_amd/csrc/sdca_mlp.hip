@@ -161,13 +161,14 @@ __global__ void k_pair_features(const int32_t *__restrict__ counts, int64_t npai
 // blocks J[item][p][s*q + a] and fields h[item][s]:
 //   x[s][col(j, a)] = J  for observed a at partner j (j != r), x[s][Kx] = h[s].
 // The caller zero-fills the staged vectors first.
-// The blocks go to slot vector XJ + slot * jstride (the staged start, or a ring vector that
-// becomes the first search direction); the fields always go to X0.
+// The blocks go to slot vector XJ + slot * jstride (the staged start, or the direction
+// buffer as the first search direction, BF16 under split precision); the fields always go
+// to X0.
 __global__ void k_scatter_warm(const int32_t *__restrict__ slot_of, const int32_t *__restrict__ item_of,
                                const int32_t *__restrict__ locus_of, const int32_t *__restrict__ partners,
                                const float *__restrict__ J, const float *__restrict__ h, int nwarm, int P, int q,
                                int64_t Kxp, int Kx, const int32_t *__restrict__ state_col, int64_t n_slot,
-                               float *__restrict__ X0, float *__restrict__ XJ, int64_t jstride)
+                               float *__restrict__ X0, void *__restrict__ XJ, bool xj_bf16, int64_t jstride)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t per = (int64_t)P * q * q;
@@ -188,7 +189,10 @@ __global__ void k_scatter_warm(const int32_t *__restrict__ slot_of, const int32_
     if (j < 0 || j == locus_of[k]) return;
     const int col = state_col[(int64_t)j * q + a];
     if (col < 0) return;
-    XJ[(int64_t)slot_of[k] * jstride + (int64_t)s * Kxp + col] = J[(item * P + p) * q * q + sa];
+    const int64_t o = (int64_t)slot_of[k] * jstride + (int64_t)s * Kxp + col;
+    const float v = J[(item * P + p) * q * q + sa];
+    if (xj_bf16) static_cast<uint16_t *>(XJ)[o] = f2bf(v);
+    else static_cast<float *>(XJ)[o] = v;
 }
 
 }  // namespace
@@ -242,12 +246,12 @@ void pair_features(hipStream_t st, const int32_t *counts, int64_t npairs, int q,
 
 void scatter_warm(hipStream_t st, const int32_t *slot_of, const int32_t *item_of, const int32_t *locus_of,
                   const int32_t *partners, const float *J, const float *h, int nwarm, int P, int q, int64_t Kxp, int Kx,
-                  const int32_t *state_col, int64_t n_slot, float *X0, float *XJ, int64_t jstride)
+                  const int32_t *state_col, int64_t n_slot, float *X0, void *XJ, bool xj_bf16, int64_t jstride)
 {
     const int64_t total = (int64_t)nwarm * ((int64_t)P * q * q + q);
     if (total <= 0) return;
     hipLaunchKernelGGL(k_scatter_warm, dim3(cd(total, 256)), dim3(256), 0, st, slot_of, item_of, locus_of, partners, J,
-                       h, nwarm, P, q, Kxp, Kx, state_col, n_slot, X0, XJ, jstride);
+                       h, nwarm, P, q, Kxp, Kx, state_col, n_slot, X0, XJ, xj_bf16, jstride);
 }
 
 }  // namespace kern
