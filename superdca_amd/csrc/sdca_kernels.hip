@@ -451,7 +451,7 @@ struct EpiGrad {
                     for (int ni = 0; ni < NI; ++ni) {
                         const int64_t n = nw + ni * 32 + (lane & 31);
                         const bool use = gt[u] >= 0 && n <= Kx && !(n >= lo[u] && n < hi[u]);
-                        xv[u][ni] = use ? xr[n] : 0.f;
+                        xv[u][ni] = use ? __builtin_nontemporal_load(xr + n) : 0.f;
                     }
                 }
 #pragma unroll
@@ -463,7 +463,7 @@ struct EpiGrad {
                         float val = 0.f;
                         if (n <= Kx && !(n >= lo[u] && n < hi[u]))
                             val = acc[mi][ni][vq + u] + (n == Kx ? lamh2 : lamJ2) * xv[u][ni];
-                        gring[roff[u] + n] = val;
+                        __builtin_nontemporal_store(val, gring + roff[u] + n);
                     }
                 }
             }
