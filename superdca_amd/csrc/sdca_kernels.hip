@@ -332,7 +332,11 @@ __device__ __forceinline__ void gemm_tile(uint8_t *sA0, uint8_t *sA1, uint8_t *s
         }
         __syncthreads();
         if (k0 + BK < ke) load(k0 + BK);
+        // the MFMA block runs at raised wave priority, so that while one workgroup stages
+        // its next slice the other's matrix work keeps issuing
+        __builtin_amdgcn_s_setprio(1);
         mma_tile<Pol, NA, NI>(sA[0], sA[NA - 1], sB, wm, wn, lane, acc);
+        __builtin_amdgcn_s_setprio(0);
     }
     epi.template tile<NI>((int)m0, wm, n0 + wn, lane, tid, acc, z);
 }
