@@ -207,9 +207,11 @@ int sdca_mlp_train(sdca_ctx *ctx, const float *x, const float *t, const float *m
                    int32_t batch, float lr, uint64_t seed, double *epoch_loss);
 
 /* sdca_solve_loci with warm starts built on the device: for each locus its npartners
- * largest-MI partners (sdca_mi_topk) receive the network's blocks, the fields start at
- * the J = 0 optimum, all other couplings at zero.  The optimum is the cold start's
- * (the problem is strictly convex); only the iteration count changes. */
+ * largest-MI partners (sdca_mi_topk) receive the network's blocks, all other couplings
+ * start at zero.  warm_mode (SDCA_WARM_* flags): the blocks are added to the start point
+ * (POINT) or form the first search direction (DIRECTION); the fields start at the J = 0
+ * optimum, or at zero with NO_FIELDS.  The optimum is the cold start's (the problem is
+ * strictly convex); only the iteration count changes. */
 int sdca_solve_loci_mlp(sdca_ctx *ctx, const int32_t *loci, int32_t K, int32_t npartners, int32_t warm_mode,
                         float *score_rows, double *x_out, sdca_locus_stats *st);
 

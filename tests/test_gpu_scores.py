@@ -1,0 +1,94 @@
+"""Score accuracy over whole alignments: every locus solved on the GPU (through the C ABI),
+scored, symmetrised as the coupling writer does (plmDCA.hpp:667-693), and compared with
+the FP64 oracle's optimum -- all 499,500 pairs of C1 (oracle solved here), and the 10,000
+highest-scoring pairs of C2 (oracle solution committed as tests/golden/c2_top10k.npz by
+tests/golden/make_c2_scores.py; about 20 CPU minutes, too slow for a test run).
+
+The GPU solves stop at a gradient threshold near each precision's floor; the optimum is
+unique (strict convexity), so both sides approximate the same scores.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import superdca_amd as sd
+from superdca_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_top10k.npz")
+TOP = 10000
+
+
+def symmetrised(rows: np.ndarray) -> np.ndarray:
+    return (rows + rows.T).astype(np.float32).astype(np.float64) * 0.5
+
+
+def top_set(S: np.ndarray, k: int) -> set:
+    iu, ju = np.triu_indices(S.shape[0], 1)
+    sv = S[iu, ju]
+    order = np.argsort(-sv, kind="stable")[:k]
+    return set(zip(iu[order].tolist(), ju[order].tolist()))
+
+
+def gpu_scores(states, w, precision, gtol):
+    L = states.shape[1]
+    with sd.Context(states, precision=precision, grad_threshold=gtol, batch_loci=256) as ctx:
+        ctx.set_weights(w)
+        rows, stats, _, rc = ctx.solve(np.arange(L, dtype=np.int32))
+    assert rc == sd.SDCA_OK, np.bincount(stats["status"])
+    return symmetrised(rows)
+
+
+@pytest.fixture(scope="module")
+def c1_oracle(c1):
+    st = c1["states"]
+    w, _ = orc.weights(st, 0.9, nthreads=16)
+    _, lh, lJ = orc.lambdas(w)
+    rows, _, stats = orc.solve_loci(st, w, lh, lJ, np.arange(st.shape[1]), gtol=1e-7, max_iter=20000,
+                                    nthreads=16)
+    # a line search that fails at the FP64 rounding floor still leaves the optimum to within
+    # far less than the GPU thresholds below
+    assert np.all((stats["status"] == 0) | ((stats["status"] == 2) & (stats["gnorm"] < 1e-6))), stats["gnorm"].max()
+    return {"w": w, "S": symmetrised(rows)}
+
+
+# bars: FP32 meets the 1e-4 relative score bar on the top pairs; split BF16 carries about
+# 18 significant bits per product, and its gradient floor (~2e-5 at C1) bounds its scores
+@pytest.mark.parametrize("precision,gtol,rtol_top", [(sd.PREC_FP32, 5e-6, 1e-4), (sd.PREC_SPLIT_BF16, 3e-5, 5e-4)])
+def test_c1_all_pairs_against_oracle_optimum(c1, c1_oracle, precision, gtol, rtol_top):
+    S = gpu_scores(c1["states"], c1_oracle["w"], precision, gtol)
+    Sr = c1_oracle["S"]
+    iu, ju = np.triu_indices(S.shape[0], 1)
+    a, b = S[iu, ju], Sr[iu, ju]
+    rel = np.abs(a - b) / np.maximum(b, 1e-12)
+    top = b >= np.sort(b)[-TOP]
+    overlap = len(top_set(S, TOP) & top_set(Sr, TOP)) / TOP
+    print(f"C1 precision {precision}: top-{TOP} overlap {overlap:.5f}, rel err top max {rel[top].max():.3e} "
+          f"median {np.median(rel[top]):.3e}, all pairs median {np.median(rel):.3e} p99 {np.quantile(rel, 0.99):.3e}, "
+          f"max abs {np.abs(a - b).max():.3e} (score max {b.max():.3e})")
+    assert overlap >= 0.999
+    assert rel[top].max() < rtol_top
+
+
+@pytest.mark.parametrize("precision,gtol,rtol_top", [(sd.PREC_FP32, 1e-5, 1e-4), (sd.PREC_SPLIT_BF16, 1e-4, 5e-4)])
+def test_c2_top_pairs_against_oracle_fixture(precision, gtol, rtol_top):
+    fx = np.load(GOLDEN)
+    st, _ = sd.prepare_alignment(synth.workload("C2")["chars"])
+    assert st.shape == (int(fx["B"]), int(fx["L"]))
+    with sd.Context(st) as ctx:
+        w, _ = ctx.reweight(0.9)  # bit-exact with the oracle's weights
+    assert abs(w.sum() - float(fx["w_sum"])) <= 1e-9 * float(fx["w_sum"])
+    S = gpu_scores(st, w, precision, gtol)
+    ref = set(zip(fx["top_i"].tolist(), fx["top_j"].tolist()))
+    overlap = len(top_set(S, TOP) & ref) / TOP
+    rel_top = np.abs(S[fx["top_i"], fx["top_j"]] - fx["top_s"]) / fx["top_s"]
+    rel_smp = np.abs(S[fx["smp_i"], fx["smp_j"]] - fx["smp_s"]) / np.maximum(fx["smp_s"], 1e-12)
+    print(f"C2 precision {precision}: top-{TOP} overlap {overlap:.5f}, rel err top max {rel_top.max():.3e} "
+          f"median {np.median(rel_top):.3e}; random pairs median {np.median(rel_smp):.3e} max {rel_smp.max():.3e}")
+    assert overlap >= 0.999
+    assert rel_top.max() < rtol_top
