@@ -137,8 +137,9 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
-              "see DESIGN.md for how the loci would shard.", file=sys.stderr)
+        print(f"bench.py measures one GPU: there is no multi-GPU path (asked for --gpus {args.gpus}). Loci are "
+              "independent given X, the weights and lambda, so the locus queue would shard across GPUs, with one "
+              "row exchange for the symmetrised scores.", file=sys.stderr)
         return 2
 
     import superdca_amd as sd
