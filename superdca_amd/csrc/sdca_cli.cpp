@@ -163,8 +163,9 @@ Options parse_args(int argc, char **argv)
         std::exit(EXIT_FAILURE);
     }
     if (o.alignments.size() > 1)
-        die("cannot handle more than 1 alignment in this build (got " + std::to_string(o.alignments.size()) + ")");
-    if (o.norm_of_mean) die("--norm-of-mean-scoring is not supported by this build");
+        die("cannot handle more than 1 alignment: the two-alignment scan mode is not supported (got " +
+            std::to_string(o.alignments.size()) + ")");
+    if (o.norm_of_mean) die("--norm-of-mean-scoring is not supported");
     return o;
 }
 
