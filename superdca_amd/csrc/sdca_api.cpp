@@ -515,7 +515,6 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     const size_t vo = (size_t)s0 * c->n_slot;              // slot-vector offset of the group
     const size_t ro = (size_t)g * c->Mgp * c->Kxp;         // its packed trial-point rows
     const size_t go = (size_t)g * c->Mgp * c->Bp;          // its packed node-belief rows
-
     float *gbuf = c->d_gbuf + (size_t)s0 * 2 * c->n_slot;
     void *dbuf = split ? (void *)(reinterpret_cast<uint16_t *>(c->d_dbuf) + vo) : (void *)(c->d_dbuf + vo);
     const size_t po = (size_t)s0 * m * c->n_slot * pair_bytes(c);
