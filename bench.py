@@ -40,8 +40,9 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_steps(ctx, order, pos, per_step, steps, label, solve):
-    """Times `steps` solve calls; returns (loci/s, iterations, evaluations, seconds, new pos)."""
+def run_steps(ctx, order, pos, per_step, steps, label, solve, fvals=None):
+    """Times `steps` solve calls; returns (loci/s, iterations, evaluations, seconds, new pos).
+    The final objective values are appended to `fvals` when given."""
     import superdca_amd as sd
 
     ctx.timing(reset=True)
@@ -54,6 +55,8 @@ def run_steps(ctx, order, pos, per_step, steps, label, solve):
         converged += int((st["status"] == sd.ST_CONVERGED).sum())
         iters.append(st["iter"])
         evals.append(st["nfeval"])
+        if fvals is not None:
+            fvals.append(st["fval"])
         log(f"[bench] {label} step {k}: {converged} converged, {time.perf_counter() - t0:.2f}s")
     elapsed = time.perf_counter() - t0
     return converged / elapsed, np.concatenate(iters), np.concatenate(evals), elapsed, pos
@@ -181,7 +184,10 @@ def main() -> int:
         _, wst, _, _ = ctx.solve(order[pos:pos + per_step])
         assert np.all(wst["nfeval"] > 1), "warmup loci stopped at their start point"
         pos += per_step
-    cold_v, cold_it, cold_ev, cold_s, pos = run_steps(ctx, order, pos, per_step, args.steps, "cold-split", ctx.solve)
+    timed_from = pos  # the warm leg solves the same loci
+    cold_f = []
+    cold_v, cold_it, cold_ev, cold_s, pos = run_steps(ctx, order, pos, per_step, args.steps, "cold-split", ctx.solve,
+                                                      cold_f)
     cold_tm = ctx.timing()
     gemm_cfg = ctx.gemm_config()
     log(f"[bench] split-BF16 GEMM configuration {gemm_cfg}")
@@ -198,8 +204,13 @@ def main() -> int:
         for k in range(args.warmup):
             solve_w(order[pos:pos + per_step // 4])
             pos += per_step // 4
-        warm_v, warm_it, warm_ev, warm_s, pos = run_steps(ctx, order, pos, per_step, args.steps, "warm-split", solve_w)
-        warm = {"loci_s": warm_v, "iter": warm_it, "evals": warm_ev, "tm": ctx.timing(), "mi_s": mi_s}
+        warm_f = []
+        warm_v, warm_it, warm_ev, warm_s, _ = run_steps(ctx, order, timed_from, per_step, args.steps, "warm-split",
+                                                        solve_w, warm_f)
+        fc, fw = np.concatenate(cold_f), np.concatenate(warm_f)
+        frel = np.abs(fw - fc) / np.abs(fc)
+        warm = {"loci_s": warm_v, "iter": warm_it, "evals": warm_ev, "tm": ctx.timing(), "mi_s": mi_s,
+                "f_rel": {"median": float(np.median(frel)), "max": float(frel.max())}}
     ctx.close()
 
     fp32 = None
@@ -243,6 +254,8 @@ def main() -> int:
                         "cold_fp32": float(np.median(fp32["iter"])) if fp32 else None},
         "iter_p95_headline": float(np.percentile(head["iter"], 95)),
         "evals_median_headline": float(np.median(head["evals"])),
+        # same loci in both legs: final f of the warm start relative to the cold start's
+        "f_warm_vs_cold_rel": warm["f_rel"] if warm else None,
         "partners": partners if warm else 0,
         "warm_mode": args.warm_mode if warm else None,
         "mi_table_s": round(warm["mi_s"], 2) if warm else None,
