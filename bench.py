@@ -62,6 +62,12 @@ def run_steps(ctx, order, pos, per_step, steps, label, solve, fvals=None):
     return converged / elapsed, np.concatenate(iters), np.concatenate(evals), elapsed, pos
 
 
+# HBM bytes moved per algorithmic byte, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over
+# the split-precision C4 solve (scaled by a plain device copy: fetch x2.000, write x1.000);
+# profiles/c4_q5_solve_kernels.md holds the commands and the per-dispatch table.
+HBM_PER_ALGORITHMIC_BYTE = {"k_update": 13.34 / 13.10, "k_dots": 10.51 / 10.50}
+
+
 def roofline(tm, split):
     # one launch of each kernel per round (the energy and gradient launches are counted
     # separately by the library)
@@ -80,8 +86,14 @@ def roofline(tm, split):
         peak = PEAK["hbm_gbs"]
         achieved, unit = work / (ms * 1e-3) / 1e9, "GB/s"
     rates = {k: round(v[1] / (v[0] * 1e-3) / (1e12 if v[2] == "mfma" else 1e9), 2) for k, v in groups.items() if v[0]}
+    ratio = HBM_PER_ALGORITHMIC_BYTE.get(dom) if split else None
+    traffic = round(ratio * work / max(n, 1)) if ratio else None
     return ({"bound": bound, "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-             "frac": round(achieved / peak, 5), "traffic": None, "launches": int(n),
+             "frac": round(achieved / peak, 5), "traffic": traffic,
+             "traffic_basis": (f"HBM bytes per launch: algorithmic bytes x {ratio:.4f}, the measured "
+                               "(FETCH_SIZE + WRITE_SIZE) / algorithmic ratio of this kernel"
+                               if ratio else None),
+             "launches": int(n),
              "avg_launch_us": round(ms * 1e3 / max(n, 1), 1),
              "work_per_launch": round(work / max(n, 1) / (1e12 if bound == "mfma" else 1e9), 4)}, rates)
 
