@@ -112,6 +112,26 @@ def c4_states():
     return st
 
 
+GOLDEN_C4W = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c4_weights.npz")
+
+
+# C4 reweighting, bit-exact against the oracle (tests/golden/c4_weights.npz, written by
+# tests/golden/make_c4_weights.py): the default threshold, and a threshold inside the
+# within-clade identity distribution with multiplicities
+@pytest.mark.skipif(not os.path.exists(GOLDEN_C4W), reason="tests/golden/c4_weights.npz not generated")
+def test_c4_reweight_bit_exact(c4_states):
+    fx = np.load(GOLDEN_C4W)
+    st = c4_states
+    assert st.shape == (int(fx["B"]), int(fx["L"]))
+    with sd.Context(st, q=5) as ctx:
+        w, iw = ctx.reweight(0.9)
+        w_m, iw_m = ctx.reweight(float(fx["theta_m"]), fx["mult"])
+    np.testing.assert_array_equal(iw, fx["int_w"])
+    np.testing.assert_array_equal(w, fx["w"])
+    np.testing.assert_array_equal(iw_m, fx["int_w_m"])
+    np.testing.assert_array_equal(w_m, fx["w_m"])
+
+
 # C4 (B = 3000, L = 94,000, q_env = 5): the score rows of 64 random loci at their 200
 # largest and 200 random entries, against the oracle's optimum (tests/golden/c4_rows64.npz,
 # written by tests/golden/make_c4_rows.py)
