@@ -126,10 +126,15 @@ void sdca_ctx::free_workspace()
     dfree(d_rows);
     dfree(d_src_off);
     dfree(d_score_loci);
-    if (h_cmds) (void)hipHostFree(h_cmds);
-    if (h_res) (void)hipHostFree(h_res);
+    for (void *h : {(void *)h_cmds, (void *)h_res, (void *)h_rows, (void *)h_src_off, (void *)h_score_loci,
+                    (void *)h_wl})
+        if (h) (void)hipHostFree(h);
     h_cmds = nullptr;
     h_res = nullptr;
+    h_rows = nullptr;
+    h_src_off = nullptr;
+    h_score_loci = nullptr;
+    h_wl = nullptr;
     ws_slots = 0;
     ws_m = 0;
     ws_groups = 0;
@@ -456,6 +461,10 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     HIP_TRY(hipHostMalloc((void **)&c->h_cmds, (size_t)nslots * sizeof(sdca::SlotCmd), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&c->h_res, (size_t)nslots * sdca::ResLayout{m}.width() * sizeof(double),
                           hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_rows, (size_t)nslots * c->L * sizeof(float), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_src_off, (size_t)nslots * sizeof(int64_t), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_score_loci, (size_t)nslots * sizeof(int32_t), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_wl, (size_t)3 * nslots * sizeof(int32_t), hipHostMallocDefault));
     c->h_xt_off.assign(nslots, -1);
     // the zero fills above must be complete before any kernel of the solve runs
     HIP_TRY(hipDeviceSynchronize());
@@ -1482,11 +1491,14 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
         if (!cw || wl_slot[g].empty()) return SDCA_OK;
         hipStream_t st = c->gstream[g];
         const int nw = (int)wl_slot[g].size();
-        std::vector<int32_t> packed(wl_slot[g]);
-        packed.insert(packed.end(), wl_item[g].begin(), wl_item[g].end());
-        packed.insert(packed.end(), wl_locus[g].begin(), wl_locus[g].end());
+        // the group's pinned list is rewritten only after the group's next finish_round,
+        // which waits for this copy
+        int32_t *packed = c->h_wl + (size_t)3 * g * gs;
+        std::copy(wl_slot[g].begin(), wl_slot[g].end(), packed);
+        std::copy(wl_item[g].begin(), wl_item[g].end(), packed + nw);
+        std::copy(wl_locus[g].begin(), wl_locus[g].end(), packed + 2 * nw);
         int32_t *dw = d_wl + (size_t)3 * g * gs;
-        HIP_TRY(hipMemcpyAsync(dw, packed.data(), packed.size() * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(dw, packed, (size_t)3 * nw * 4, hipMemcpyHostToDevice, st));
         // blocks: into the staged start, or into the direction buffer (first search
         // direction, in the pair precision)
         const bool dbf = cw->as_direction && c->opts.precision == SDCA_PREC_SPLIT_BF16;
@@ -1501,7 +1513,6 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
         sdca::kern::scatter_warm(st, dw, dw + nw, dw + 2 * nw, cw->d_partners, cw->d_J, cw->d_h, nw, cw->P, c->q,
                                  c->Kxp, (int)c->Kx, c->d_state_col, c->n_slot, c->d_X0, XJ, dbf, jstride);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipStreamSynchronize(st));  // packed is a host temporary
         wl_slot[g].clear();
         wl_item[g].clear();
         wl_locus[g].clear();
@@ -1524,8 +1535,23 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
         }
     }
     std::vector<int> finished;
-    std::vector<int64_t> src_off;
-    std::vector<int32_t> sloci;
+    // score rows on their way to the host, per group: the D2H copy into the group's pinned
+    // slice runs on the group's stream ahead of its next round; the rows go on to the
+    // caller's buffer once that round has been launched
+    std::vector<int32_t> row_items[sdca::kMaxGroups];
+    bool scoring[sdca::kMaxGroups] = {};
+    auto flush_rows = [&](int g) -> int {
+        if (!scoring[g]) return SDCA_OK;
+        scoring[g] = false;
+        hipEvent_t *ev = c->gev[g];
+        HIP_TRY(hipEventSynchronize(ev[8]));
+        c->timing.score += elapsed(ev[6], ev[7]);
+        const float *h = c->h_rows + (size_t)g * gs * c->L;
+        for (size_t u = 0; u < row_items[g].size(); ++u)
+            std::memcpy(score_rows + (size_t)row_items[g][u] * c->L, h + u * c->L, (size_t)c->L * 4);
+        row_items[g].clear();
+        return SDCA_OK;
+    };
     int64_t rounds = 0;
     // the groups' rounds alternate: while the host steps one group's line searches, the
     // other group's round runs on the device
@@ -1544,43 +1570,46 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
             }
             if (!finished.empty()) {
                 // score rows of the finished loci, read before their slots are refilled
+                // the score kernel reads the finished slots' points ahead of the next
+                // round on the same stream, so nothing here waits for the device (apart
+                // from x_out)
                 hipStream_t st = c->gstream[g];
                 hipEvent_t *ev = c->gev[g];
-                src_off.clear();
-                sloci.clear();
-                for (int i : finished) {
-                    const Slot &sl = S.slot(i);
-                    src_off.push_back(sl.from_xt ? c->h_xt_off[i] : c->xc_off + (int64_t)i * c->n_slot);
-                    sloci.push_back(sl.r);
-                }
                 const int nf = (int)finished.size();
+                int64_t *h_off = c->h_src_off + (size_t)g * gs;
+                int32_t *h_sl = c->h_score_loci + (size_t)g * gs;
+                for (int u = 0; u < nf; ++u) {
+                    const int i = finished[u];
+                    const Slot &sl = S.slot(i);
+                    h_off[u] = sl.from_xt ? c->h_xt_off[i] : c->xc_off + (int64_t)i * c->n_slot;
+                    h_sl[u] = sl.r;
+                }
                 int64_t *d_off = c->d_src_off + (size_t)g * gs;
                 int32_t *d_sl = c->d_score_loci + (size_t)g * gs;
                 float *d_rows = c->d_rows + (size_t)g * gs * c->L;
-                HIP_TRY(hipMemcpyAsync(d_off, src_off.data(), nf * 8, hipMemcpyHostToDevice, st));
-                HIP_TRY(hipMemcpyAsync(d_sl, sloci.data(), nf * 4, hipMemcpyHostToDevice, st));
+                HIP_TRY(hipMemcpyAsync(d_off, h_off, nf * 8, hipMemcpyHostToDevice, st));
+                HIP_TRY(hipMemcpyAsync(d_sl, h_sl, nf * 4, hipMemcpyHostToDevice, st));
                 HIP_TRY(hipEventRecord(ev[6], st));
                 sdca::kern::score(st, c->d_Xtr, d_off, d_sl, nf, c->L, c->q, c->gap, (int)c->Kxp, c->d_state_col,
                                   d_rows);
                 HIP_TRY(hipGetLastError());
                 HIP_TRY(hipEventRecord(ev[7], st));
                 if (score_rows) {
-                    for (int u = 0; u < nf; ++u)
-                        HIP_TRY(hipMemcpyAsync(score_rows + (size_t)S.slot(finished[u]).item * c->L,
-                                               d_rows + (size_t)u * c->L, (size_t)c->L * 4, hipMemcpyDeviceToHost,
-                                               st));
+                    HIP_TRY(hipMemcpyAsync(c->h_rows + (size_t)g * gs * c->L, d_rows, (size_t)nf * c->L * 4,
+                                           hipMemcpyDeviceToHost, st));
+                    for (int u = 0; u < nf; ++u) row_items[g].push_back(S.slot(finished[u]).item);
                 }
+                HIP_TRY(hipEventRecord(ev[8], st));
+                scoring[g] = true;
                 if (x_out) {
                     for (int u = 0; u < nf; ++u) {
                         const Slot &sl = S.slot(finished[u]);
-                        HIP_TRY(hipMemcpyAsync(xdev.data(), c->d_Xtr + src_off[u], c->n_slot * 4,
+                        HIP_TRY(hipMemcpyAsync(xdev.data(), c->d_Xtr + h_off[u], c->n_slot * 4,
                                                hipMemcpyDeviceToHost, st));
                         HIP_TRY(hipStreamSynchronize(st));
                         dev_to_ref(c, sl.r, xdev.data(), x_out + (size_t)sl.item * n);
                     }
                 }
-                HIP_TRY(hipStreamSynchronize(st));
-                c->timing.score += elapsed(ev[6], ev[7]);
                 for (int u = 0; u < nf; ++u) {
                     const int i = finished[u];
                     const Slot &sl = S.slot(i);
@@ -1607,9 +1636,12 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
                 if ((rc = launch_round(c, g, S.cmds().data() + (size_t)g * gs))) return rc;
                 inflight[g] = true;
             }
+            if ((rc = flush_rows(g))) return rc;  // while the round runs
         }
         if (!any) break;  // every group idle
     }
+    for (int g = 0; g < ng; ++g)
+        if ((rc = flush_rows(g))) return rc;
     c->timing.total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (nonfinite) return fail(SDCA_E_NONFINITE, "%d loci reached a non-finite objective", nonfinite);
     if (noconv) return fail(SDCA_E_NOCONV, "%d loci stopped without reaching the gradient threshold", noconv);

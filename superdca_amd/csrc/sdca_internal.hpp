@@ -239,6 +239,12 @@ struct sdca_ctx {
     float *d_rows = nullptr;
     int64_t *d_src_off = nullptr;
     int32_t *d_score_loci = nullptr;
+    // pinned host staging, per group slices of gslots entries: score rows (L each), their
+    // sources and loci, and the compact warm-start refill lists (3 per slot)
+    float *h_rows = nullptr;
+    int64_t *h_src_off = nullptr;
+    int32_t *h_score_loci = nullptr;
+    int32_t *h_wl = nullptr;
 
     // pair statistics (sdca_pairs.hip), built on first use
     int QP = 0;
@@ -255,7 +261,7 @@ struct sdca_ctx {
 
     hipEvent_t ev[8] = {};
     hipStream_t gstream[sdca::kMaxGroups] = {};  // gstream[0] is stream
-    hipEvent_t gev[sdca::kMaxGroups][8] = {};
+    hipEvent_t gev[sdca::kMaxGroups][9] = {};  // [8]: a group's score rows are in h_rows
     sdca::Timing timing;
 
     ~sdca_ctx();
