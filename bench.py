@@ -1,13 +1,14 @@
 """Throughput of the per-locus plmDCA solve on one MI355X: loci/s to convergence.
 
 Workload: B = 3000 sequences x N = 94,000 loci (seeded synthetic genome alignment, 60
-clades, 2 % mutation, 2000 planted pairs), q = 4 envelope, L-BFGS m = 10, stop at
+clades, 2 % mutation, 2000 planted pairs), q = 5 envelope, L-BFGS m = 10, stop at
 ||g||_inf < 1e-3 (SuperDCA's default rule).  A step is a batch of loci solved to
-convergence with 256 loci in flight on the GPU (finished slots are refilled).  Timing
+convergence with 384 loci in flight on the GPU (finished slots are refilled).  Timing
 starts with the alignment, weights and lambda resident on the device and ends when the
 last score row is in host memory.
 
-Three legs on the same GPU, each on its own loci:
+Three legs on the same GPU (warm and cold solve the same loci, so their final objectives
+are compared locus by locus; fp32 solves its own):
   warm   split-BF16 contractions, warm starts from the network (trained beforehand on
          cold-start solutions of the C1 and C2 workloads, untimed) for the top-2048 MI
          partners of every locus -- the headline value
@@ -137,7 +138,7 @@ def main() -> int:
     ap.add_argument("--q", type=int, default=0,
                     help="state envelope (softmax width): default 5 ({A,C,G,T,-}) at C4, 4 (SuperDCA's) otherwise")
     ap.add_argument("--loci-per-step", type=int, default=0, help="default: 2048 at C4, 1024 otherwise")
-    ap.add_argument("--slots", type=int, default=0, help="loci in flight (default 256 at C4, 128 otherwise)")
+    ap.add_argument("--slots", type=int, default=0, help="loci in flight (default 384 at C4, 128 otherwise)")
     ap.add_argument("--history", type=int, default=10)
     ap.add_argument("--groups", type=int, default=1, help="slot groups pipelined on two streams (1 or 2)")
     ap.add_argument("--partners", type=int, default=2048, help="warm-started partners per locus")
@@ -145,6 +146,8 @@ def main() -> int:
                     help="direction: network blocks as the first search direction from zero; direction-fields: "
                          "the same from the J=0 field optimum; point: start at fields + blocks")
     ap.add_argument("--fp32-loci", type=int, default=1024, help="loci of the FP32 cold leg (0: skip)")
+    ap.add_argument("--fp32-slots", type=int, default=0,
+                    help="loci in flight in the FP32 leg (default 256 at C4, its fastest; 128 otherwise)")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm leg (value = cold split-BF16)")
     ap.add_argument("--cpu-loci", type=int, default=0, help="CPU baseline sample size (default: one per thread)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -163,7 +166,8 @@ def main() -> int:
     wl = args.workload
     q = args.q or (5 if wl == "C4" else 4)
     per_step = args.loci_per_step or (2048 if wl == "C4" else 1024)
-    slots = args.slots or (256 if wl == "C4" else 128)
+    slots = args.slots or (384 if wl == "C4" else 128)
+    slots32 = args.fp32_slots or (256 if wl == "C4" else 128)
 
     t = time.time()
     states, translation = sd.prepare_alignment(synth.workload(wl)["chars"], q=q)
@@ -171,7 +175,7 @@ def main() -> int:
     partners = min(args.partners, L - 1)
     log(f"[bench] workload {wl}: B={B} L={L} generated in {time.time() - t:.1f}s")
     order = np.random.default_rng(1234).permutation(L).astype(np.int32)
-    need = (2 * args.warmup + 2 * args.steps) * per_step + args.fp32_loci + slots
+    need = (2 * args.warmup + 2 * args.steps) * per_step + args.fp32_loci + max(slots, slots32)
     if need > L:
         order = np.resize(order, need)
     pos = 0
@@ -227,11 +231,11 @@ def main() -> int:
 
     fp32 = None
     if args.fp32_loci > 0:
-        ctx32 = sd.Context(states, q=q, precision=sd.PREC_FP32, batch_loci=slots, history=args.history,
+        ctx32 = sd.Context(states, q=q, precision=sd.PREC_FP32, batch_loci=slots32, history=args.history,
                            grad_threshold=1e-3, pipeline_groups=args.groups)
         ctx32.set_weights(w)
-        ctx32.solve(order[pos:pos + slots // 4])  # allocation and first launches
-        pos += slots // 4
+        ctx32.solve(order[pos:pos + slots32 // 4])  # allocation and first launches
+        pos += slots32 // 4
         fv, fit, fev, fs, pos = run_steps(ctx32, order, pos, args.fp32_loci, 1, "cold-fp32", ctx32.solve)
         fp32 = {"loci_s": fv, "iter": fit, "tm": ctx32.timing(), "gemm": ctx32.gemm_config()}
         ctx32.close()
@@ -258,6 +262,7 @@ def main() -> int:
     detail = {
         "cold_split_bf16_loci_s": round(cold_v, 3),
         "cold_fp32_loci_s": round(fp32["loci_s"], 3) if fp32 else None,
+        "fp32_slots": slots32 if fp32 else None,
         "warm_split_bf16_loci_s": round(warm["loci_s"], 3) if warm else None,
         "warm_over_cold_fp32": round(warm["loci_s"] / fp32["loci_s"], 3) if (warm and fp32) else None,
         "warm_over_cold_split": round(warm["loci_s"] / cold_v, 3) if warm else None,
