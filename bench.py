@@ -3,7 +3,7 @@
 Workload: B = 3000 sequences x N = 94,000 loci (seeded synthetic genome alignment, 60
 clades, 2 % mutation, 2000 planted pairs), q = 5 envelope, L-BFGS m = 10, stop at
 ||g||_inf < 1e-3 (SuperDCA's default rule).  A step is a batch of loci solved to
-convergence with 384 loci in flight on the GPU (finished slots are refilled).  Timing
+convergence with 256 loci in flight on the GPU (finished slots are refilled).  Timing
 starts with the alignment, weights and lambda resident on the device and ends when the
 last score row is in host memory.
 
@@ -138,7 +138,7 @@ def main() -> int:
     ap.add_argument("--q", type=int, default=0,
                     help="state envelope (softmax width): default 5 ({A,C,G,T,-}) at C4, 4 (SuperDCA's) otherwise")
     ap.add_argument("--loci-per-step", type=int, default=0, help="default: 2048 at C4, 1024 otherwise")
-    ap.add_argument("--slots", type=int, default=0, help="loci in flight (default 384 at C4, 128 otherwise)")
+    ap.add_argument("--slots", type=int, default=0, help="loci in flight (default 256 at C4, 128 otherwise)")
     ap.add_argument("--history", type=int, default=10)
     ap.add_argument("--groups", type=int, default=1, help="slot groups pipelined on two streams (1 or 2)")
     ap.add_argument("--partners", type=int, default=2048, help="warm-started partners per locus")
@@ -147,7 +147,7 @@ def main() -> int:
                          "the same from the J=0 field optimum; point: start at fields + blocks")
     ap.add_argument("--fp32-loci", type=int, default=1024, help="loci of the FP32 cold leg (0: skip)")
     ap.add_argument("--fp32-slots", type=int, default=0,
-                    help="loci in flight in the FP32 leg (default 256 at C4, its fastest; 128 otherwise)")
+                    help="loci in flight in the FP32 leg (default: as --slots)")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm leg (value = cold split-BF16)")
     ap.add_argument("--cpu-loci", type=int, default=0, help="CPU baseline sample size (default: one per thread)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -166,8 +166,8 @@ def main() -> int:
     wl = args.workload
     q = args.q or (5 if wl == "C4" else 4)
     per_step = args.loci_per_step or (2048 if wl == "C4" else 1024)
-    slots = args.slots or (384 if wl == "C4" else 128)
-    slots32 = args.fp32_slots or (256 if wl == "C4" else 128)
+    slots = args.slots or (256 if wl == "C4" else 128)
+    slots32 = args.fp32_slots or slots
 
     t = time.time()
     states, translation = sd.prepare_alignment(synth.workload(wl)["chars"], q=q)
