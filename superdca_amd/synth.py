@@ -135,6 +135,21 @@ def generate(B: int, N: int, clades: int, pairs: int, seed: int, mut: float = 0.
     return {"chars": chars, "pairs": np.array(planted, dtype=np.int64).reshape(-1, 2), "clade": clade}
 
 
+def c3_workloads():
+    """The uneven batch: 48 independent alignments, seeds 3000-3047, each with
+    B ~ U[100, 3000] sequences, N ~ U[500, 20000] loci, U[5, 50] clades and planted pairs
+    1 % of N (the shapes are drawn from the seed's own stream [seed, 0])."""
+    out = []
+    for k in range(48):
+        seed = 3000 + k
+        r = np.random.default_rng([seed, 0])
+        B = int(r.integers(100, 3001))
+        N = int(r.integers(500, 20001))
+        clades = int(r.integers(5, 51))
+        out.append(dict(B=B, N=N, clades=clades, pairs=max(1, N // 100), seed=seed))
+    return out
+
+
 def workload(name: str, **override):
     cfg = dict(WORKLOADS[name])
     cfg.update(override)

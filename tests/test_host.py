@@ -105,6 +105,19 @@ def test_writer_rounding_matches_printf(tmp_path):
             k += 1
 
 
+def test_uneven_batch_shapes():
+    cfgs = synth.c3_workloads()
+    assert [c["seed"] for c in cfgs] == list(range(3000, 3048))
+    for c in cfgs:
+        assert 100 <= c["B"] <= 3000 and 500 <= c["N"] <= 20000 and 5 <= c["clades"] <= 50
+        assert c["pairs"] == c["N"] // 100
+    assert cfgs == synth.c3_workloads()  # deterministic
+    assert len({(c["B"], c["N"]) for c in cfgs}) == 48
+    small = min(cfgs, key=lambda c: c["B"] * c["N"])
+    d = synth.generate(**small)
+    assert d["chars"].shape == (small["B"], small["N"]) and synth.passes_filter(d["chars"]).all()
+
+
 def test_generator_is_deterministic_and_filter_clean():
     a = synth.generate(B=80, N=300, clades=5, pairs=4, seed=5)
     b = synth.generate(B=80, N=300, clades=5, pairs=4, seed=5)
