@@ -33,6 +33,7 @@
 #include <string>
 #include <vector>
 
+#include "lbfgs_coeffs.hpp"
 #include "linesearch.hpp"
 #include "sdca_internal.hpp"
 
@@ -982,40 +983,14 @@ private:
         return -1;
     }
 
-    // Two-loop recursion (Nocedal & Wright, Algorithm 7.4) on coefficient vectors over
-    // {s_p, y_p, g}: q = g; for pairs newest first a_p = rho_p s_p.q, q -= a_p y_p;
-    // r = gamma q; for pairs oldest first b = rho_p y_p.r, r += (a_p - b) s_p; d = -r.
+    // d = -H g as coefficients over {g, s_p, y_p} (lbfgs_coeffs.hpp)
     void two_loop(int i, double *cs, double *cy, double &cg)
     {
-        Slot &S = slots_[i];
+        const Slot &S = slots_[i];
+        int pairs[sdca::kMaxHistory];
         const int n = (int)S.pairs.size();
-        for (int p = 0; p < m_; ++p) cs[p] = cy[p] = 0.0;
-        cg = 1.0;
-        double a[sdca::kMaxHistory];
-        for (int t = n - 1; t >= 0; --t) {
-            const int p = S.pairs[t];
-            double sq = cg * S.SG[p];  // s_p . q, q = cg g + sum cy y
-            for (int u : S.pairs) sq += cy[u] * S.SY[p][u];
-            a[p] = sq / S.SY[p][p];
-            cy[p] -= a[p];
-        }
-        if (n > 0) {
-            const int p = S.pairs[n - 1];
-            const double gam = S.SY[p][p] / S.YY[p][p];
-            cg *= gam;
-            for (int u = 0; u < m_; ++u) cy[u] *= gam;
-        }
-        for (int t = 0; t < n; ++t) {
-            const int p = S.pairs[t];
-            double yr = cg * S.YG[p];  // y_p . r, r = cg g + sum cy y + sum cs s
-            for (int u : S.pairs) yr += cy[u] * S.YY[p][u] + cs[u] * S.SY[u][p];
-            cs[p] += a[p] - yr / S.SY[p][p];
-        }
-        for (int p = 0; p < m_; ++p) {
-            cs[p] = -cs[p];
-            cy[p] = -cy[p];
-        }
-        cg = -cg;
+        for (int t = 0; t < n; ++t) pairs[t] = S.pairs[t];
+        sdca::two_loop_coeffs(pairs, n, m_, S.SY, S.YY, S.SG, S.YG, cs, cy, cg);
     }
 
     // First direction = the preloaded direction buffer, unit initial step.  If it turns
