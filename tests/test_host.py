@@ -152,3 +152,11 @@ def test_context_without_gpu_fails_loudly():
         pytest.skip("a GPU is present")
     with pytest.raises(sd.SdcaError):
         sd.Context(np.zeros((4, 3), dtype=np.uint8))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import sys
+
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 2 and "one GPU" in (p.stdout + p.stderr)
