@@ -1546,10 +1546,9 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
                 if (S.process(i, c->h_res + (size_t)i * R.width())) finished.push_back(i);
             }
             if (!finished.empty()) {
-                // score rows of the finished loci, read before their slots are refilled
-                // the score kernel reads the finished slots' points ahead of the next
-                // round on the same stream, so nothing here waits for the device (apart
-                // from x_out)
+                // score rows of the finished loci: the score kernel reads their points on
+                // the group's stream ahead of the next round, which refills their slots,
+                // so nothing here waits for the device (apart from x_out)
                 hipStream_t st = c->gstream[g];
                 hipEvent_t *ev = c->gev[g];
                 const int nf = (int)finished.size();
