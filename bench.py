@@ -99,6 +99,30 @@ def roofline(tm, split):
              "work_per_launch": round(work / max(n, 1) / (1e12 if bound == "mfma" else 1e9), 4)}, rates)
 
 
+def phase_fractions(tm, split):
+    """Roofline fraction of each of the four dominant kernels (algorithmic work over its
+    device time, against the MFMA or HBM peak) and their device-time-weighted mean."""
+    groups = [("k_gemm_nt<EpiEnergy>", "gemm_energy_ms", "flops_energy", "mfma"),
+              ("k_gemm_nt<EpiGrad>", "gemm_grad_ms", "flops_grad", "mfma"),
+              ("k_update", "update_ms", "bytes_update", "hbm"),
+              ("k_dots", "dots_ms", "bytes_dots", "hbm")]
+    out, weighted, total = {}, 0.0, 0.0
+    for name, ms_key, work_key, bound in groups:
+        ms, work = tm[ms_key], tm[work_key]
+        if not ms:
+            continue
+        if bound == "mfma":
+            peak, scale, unit = (PEAK["bf16_tflops"] if split else PEAK["fp32_tflops"]), 1e12, "TFLOP/s"
+        else:
+            peak, scale, unit = PEAK["hbm_gbs"], 1e9, "GB/s"
+        rate = work / (ms * 1e-3) / scale
+        out[name] = {"ms": round(ms, 1), "rate": round(rate, 2), "unit": unit, "peak": peak,
+                     "frac": round(rate / peak, 4)}
+        weighted += ms * rate / peak
+        total += ms
+    return out, (round(weighted / total, 4) if total else None)
+
+
 def device_ms(tm):
     return {k: round(tm[k], 1) for k in ("gemm_energy_ms", "gemm_grad_ms", "softmax_ms", "update_ms", "dots_ms",
                                          "score_ms")}
@@ -242,6 +266,7 @@ def main() -> int:
 
     head = warm if warm is not None else {"loci_s": cold_v, "iter": cold_it, "evals": cold_ev, "tm": cold_tm}
     rl, rates = roofline(head["tm"], True)
+    phases, phases_weighted = phase_fractions(head["tm"], True)
 
     cpu = None
     if not args.no_cpu:
@@ -279,6 +304,8 @@ def main() -> int:
         "reweight_ms": round(reweight_s * 1e3, 2), "B_eff": beff,
         "device_ms_headline": device_ms(head["tm"]),
         "kernel_rates_headline": rates,
+        "roofline_phases_headline": phases,
+        "roofline_time_weighted_frac": phases_weighted,
         "device_ms_cold_fp32": device_ms(fp32["tm"]) if fp32 else None,
         "rounds_headline": head["tm"]["n_evals"],
         "gemm_config_split": gemm_cfg,
