@@ -160,3 +160,81 @@ def test_bench_refuses_more_than_one_gpu():
     p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                        timeout=120)
     assert p.returncode == 2 and "one GPU" in (p.stdout + p.stderr)
+
+
+def _cli():
+    exe = os.path.join(ROOT, "superdca_amd", "sdca")
+    if not os.path.exists(exe):
+        from superdca_amd import build
+        build.build_cli()
+    return exe
+
+
+def _toy_alignment():
+    # 50 loci that pass the default filter plus a monomorphic column (input column 10)
+    # and a half-gap column (input column 31) that do not
+    d = synth.generate(B=40, N=50, clades=4, pairs=2, seed=9)
+    ch = d["chars"]
+    mono = np.full((40, 1), ord("A"), np.uint8)
+    gappy = np.where(np.arange(40)[:, None] % 2 == 0, ord("-"), ord("C")).astype(np.uint8)
+    return np.concatenate([ch[:, :10], mono, ch[:, 10:30], gappy, ch[:, 30:]], axis=1)
+
+
+def _write_wrapped_fasta(path, chars, names):
+    # each sequence over two lines; every other record with CRLF line ends
+    with open(path, "wb") as fh:
+        for b, nm in enumerate(names):
+            eol = b"\r\n" if b % 2 else b"\n"
+            s = chars[b].tobytes()
+            fh.write(b">" + nm.encode() + eol + s[:20] + eol + s[20:] + eol)
+
+
+def _read_fasta(path):
+    names, seqs = [], []
+    for line in open(path).read().splitlines():
+        if line.startswith(">"):
+            names.append(line[1:])
+            seqs.append("")
+        elif line:
+            seqs[-1] += line
+    return names, seqs
+
+
+def test_cli_writes_the_filtered_alignment_and_loci_list(tmp_path):
+    # the host front end only: FASTA in, filter, the .fasta/.loci pair out (written before
+    # any device is touched; the run may then stop for want of a GPU)
+    ch = _toy_alignment()
+    names = [f"s{b}" for b in range(ch.shape[0])]
+    _write_wrapped_fasta(tmp_path / "toy.fa", ch, names)
+    subprocess.run([_cli(), "--output-filtered-alignment", "--no-coupling-output", "toy.fa"], cwd=tmp_path,
+                   capture_output=True, text=True, timeout=300)
+    _, kept = sd.prepare_alignment(ch)
+    assert 10 not in kept and 31 not in kept and kept.size == 50
+    got_names, got = _read_fasta(tmp_path / "toy.fasta")
+    assert got_names == names
+    assert got == [ch[b, kept].tobytes().decode() for b in range(ch.shape[0])]
+    loci = (tmp_path / "toy.loci").read_text().strip().split(",")
+    assert [int(v) for v in loci] == (kept + 1).tolist()  # output indexing base 1
+
+
+def test_cli_sample_and_column_lists(tmp_path):
+    ch = _toy_alignment()
+    names = [f"s{b}" for b in range(ch.shape[0])]
+    _write_wrapped_fasta(tmp_path / "toy.fa", ch, names)
+    keep_names = names[::2]
+    (tmp_path / "samples.txt").write_text("\n".join(">" + n if i % 3 == 0 else n for i, n in enumerate(keep_names)) + "\n")
+    cols = list(range(0, 52, 2)) + [10, 31]  # 1-based list below; duplicates removed, sorted
+    (tmp_path / "cols.txt").write_text(",".join(str(c + 1) for c in cols[:10]) + "\n" +
+                                       " ".join(str(c + 1) for c in cols[10:]) + "\n")
+    subprocess.run([_cli(), "--samplelistfile", "samples.txt", "--filterlistfile", "cols.txt",
+                    "--input-indexing-base", "1", "--output-filtered-alignment", "--no-coupling-output", "toy.fa"],
+                   cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    rows = list(range(0, 40, 2))
+    sel = sorted(set(cols))
+    sub = ch[np.ix_(rows, sel)]
+    _, kept = sd.prepare_alignment(sub)
+    got_names, got = _read_fasta(tmp_path / "toy.fasta")
+    assert got_names == keep_names
+    assert got == [sub[i, kept].tobytes().decode() for i in range(len(rows))]
+    loci = [int(v) for v in (tmp_path / "toy.loci").read_text().strip().split(",")]
+    assert loci == [sel[k] + 1 for k in kept]  # original input columns, base 1
