@@ -238,3 +238,55 @@ def test_cli_sample_and_column_lists(tmp_path):
     assert got == [sub[i, kept].tobytes().decode() for i in range(len(rows))]
     loci = [int(v) for v in (tmp_path / "toy.loci").read_text().strip().split(",")]
     assert loci == [sel[k] + 1 for k in kept]  # original input columns, base 1
+
+
+C_PROGRAM = r"""
+#include <stdio.h>
+#include <string.h>
+#include "sdca.h"
+
+int main(void)
+{
+    sdca_opts o;
+    sdca_default_opts(&o);
+    /* 3 sequences x 4 columns: columns 1 and 3 polymorphic, 0 monomorphic, 2 all gaps */
+    const char chars[] = "AC-G" "AT-G" "AC-T";
+    uint8_t states[12];
+    int32_t loci[4], L = 0;
+    int rc = sdca_prepare_alignment(chars, 3, 4, 0.01, 0.15, 0, 4, states, loci, &L);
+    printf("threshold %g max_iter %d rc %d L %d loci", o.grad_threshold, o.max_iter, rc, L);
+    for (int k = 0; k < L; ++k) printf(" %d", loci[k]);
+    printf("\n");
+    sdca_ctx *ctx = NULL;
+    rc = sdca_ctx_create(&ctx, states, 3, L, 4, 3, &o);
+    if (rc == SDCA_OK) {
+        printf("context created\n");
+        sdca_ctx_destroy(ctx);
+    } else {
+        printf("context refused: %d %s\n", rc, strlen(sdca_last_error()) ? "with message" : "without message");
+    }
+    rc = sdca_prepare_alignment(NULL, 3, 4, 0.01, 0.15, 0, 4, states, loci, &L);
+    printf("null input rc %d\n", rc);
+    return 0;
+}
+"""
+
+
+def test_plain_c_program_uses_the_abi(tmp_path):
+    # the boundary is plain C: the header compiles as C99 and a C program links libsdca.so
+    # (host entry points only; context creation reports a status when no GPU is present)
+    import shutil
+
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not found")
+    src = tmp_path / "use_abi.c"
+    src.write_text(C_PROGRAM)
+    exe = str(tmp_path / "use_abi")
+    libdir = os.path.dirname(sd.LIB_PATH)
+    subprocess.run([gcc, "-std=c99", "-Wall", "-Werror", "-pedantic", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-o", exe, "-L", libdir, "-lsdca", "-Wl,-rpath," + libdir], check=True, capture_output=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=120).stdout.splitlines()
+    assert out[0] == "threshold 0.001 max_iter 2000 rc 0 L 2 loci 1 3"
+    assert out[1] == "context created" or out[1] == f"context refused: {sd.SDCA_E_HIP} with message"
+    assert out[2] == f"null input rc {sd.SDCA_E_ARG}"
