@@ -107,17 +107,22 @@ typedef struct sdca_timing {
     double bytes_dots;       /* algorithmic HBM bytes of the dot-product pass */
 } sdca_timing;
 
+/* Message of the calling thread's last failure (SuperDCA prints its errors to
+ * plmDCA_options::err_stream() and exits, SuperDCA.cpp:90,178). */
 const char *sdca_last_error(void);
 
 /* Creates a context on opts->device over a B x L alignment of envelope states
  * states[b*L + j] in [0, q_env), q_env in {2..8}, gap_state in [0, q_env) (or -1 if
  * none).  The one-hot matrix X (B x K_x, K_x = observed states summed over loci)
- * is built on the device. */
+ * is built on the device.  Replaces the set-up of run_plmDCA before its solve
+ * (plmDCA.hpp:436-578: the alignment, block accounting and parameter dimensions of
+ * plmDCA_optimizer_parameters.hpp:131-151); sdca_ctx_destroy ends it. */
 int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L, int32_t q_env,
                     int32_t gap_state, const sdca_opts *opts);
 void sdca_ctx_destroy(sdca_ctx *ctx);
 
-/* Number of observed-state columns K_x of the context's one-hot X. */
+/* Number of observed-state columns K_x of the context's one-hot X (no counterpart in
+ * SuperDCA, whose parameter vector spans all L*q*q + q entries, plmDCA_optimizer_parameters.hpp:219-247). */
 int64_t sdca_num_columns(const sdca_ctx *ctx);
 
 /* calculate_weights (plmDCA_optimizer_parameters.hpp:39-98), bit-exact:
@@ -133,7 +138,8 @@ int sdca_reweight(sdca_ctx *ctx, double theta, const uint32_t *mult, double *w_o
 int sdca_set_weights(sdca_ctx *ctx, const double *w, double lambda_h_user, double lambda_J_user,
                      double *out3);
 
-/* Parity entry point for plmDCA_objective_fval_and_gradient: for each of the K loci
+/* Parity entry point for plmDCA_objective_fval_and_gradient (plmDCA_objective.hpp:38-249,
+ * called through LBFGS_interface_CppNumericalSolvers.hpp:111-117): for each of the K loci
  * loci[k], evaluates f and g at x (K x n, reference layout: n = L*q*q + q, J_r[j](a,s)
  * at 16-locus blocks, Matrix_kernel_access_order.hpp:28-34, h_r at L*q*q).  Rows of
  * states never observed at j contribute only their L2 term, which is added on the host
@@ -141,7 +147,8 @@ int sdca_set_weights(sdca_ctx *ctx, const double *w, double lambda_h_user, doubl
 int sdca_eval_batch(sdca_ctx *ctx, const int32_t *loci, int32_t K, const double *x, double *f_out,
                     double *g_out);
 
-/* Solves the K loci (any K; batched internally, converged slots are refilled) with the
+/* Replaces the tbb::parallel_reduce over plmDCA_solver::operator() (plmDCA.hpp:277-396,
+ * 582-586).  Solves the K loci (any K; batched internally, converged slots are refilled) with the
  * batched L-BFGS and writes each locus' mean-of-norms score row (K x L floats,
  * plmDCA.hpp:366-394; row[k][loci[k]] = 0).  warm (K x n reference layout, may be NULL)
  * gives start points; NULL = cold start from zero (plmDCA.hpp:298).  x_out (K x n,
@@ -151,7 +158,8 @@ int sdca_eval_batch(sdca_ctx *ctx, const int32_t *loci, int32_t K, const double 
 int sdca_solve_loci(sdca_ctx *ctx, const int32_t *loci, int32_t K, const double *warm,
                     float *score_rows, double *x_out, sdca_locus_stats *st);
 
-/* Timings accumulated since the context was created or last reset. */
+/* Timings accumulated since the context was created or last reset (no counterpart in
+ * SuperDCA beyond its per-locus log, plmDCA.hpp:306-325). */
 int sdca_get_timing(sdca_ctx *ctx, sdca_timing *t, int32_t reset);
 
 /* GEMM configuration of the current workspace (no counterpart in SuperDCA):
