@@ -201,6 +201,8 @@ class Context:
         return w, iw
 
     def set_weights(self, w: np.ndarray, lambda_h: float = -1.0, lambda_J: float = -1.0):
+        """Installs the weights; B_eff and lambda as plmDCA_optimizer_parameters.hpp:257-284
+        (negative = automatic) -> (B_eff, lambda_h, lambda_J)."""
         w = np.ascontiguousarray(w, dtype=np.float64)
         out = np.zeros(3, dtype=np.float64)
         _check(lib().sdca_set_weights(self._h, _ptr(w), float(lambda_h), float(lambda_J), _ptr(out)))
@@ -208,7 +210,8 @@ class Context:
         return self.B_eff, self.lambda_h, self.lambda_J
 
     def eval_batch(self, loci, x: np.ndarray):
-        """f and g of plmDCA_objective_fval_and_gradient for each locus (reference layout)."""
+        """f and g of plmDCA_objective_fval_and_gradient (plmDCA_objective.hpp:38-249) for
+        each locus, x and g in the reference layout."""
         loci = np.ascontiguousarray(loci, dtype=np.int32)
         K = loci.size
         n = nparams(self.L, self.q)
@@ -219,7 +222,8 @@ class Context:
         return f, g
 
     def solve(self, loci, warm: np.ndarray | None = None, want_x: bool = False, rows: bool = True):
-        """Batched L-BFGS solve -> (score_rows K x L float32, stats, x or None, rc)."""
+        """The per-locus solves of plmDCA_solver::operator() (plmDCA.hpp:277-396) as one
+        batched L-BFGS -> (score_rows K x L float32, stats, x or None, rc)."""
         loci = np.ascontiguousarray(loci, dtype=np.int32)
         K = loci.size
         n = nparams(self.L, self.q)
