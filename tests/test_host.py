@@ -290,3 +290,38 @@ def test_plain_c_program_uses_the_abi(tmp_path):
     assert out[0] == "threshold 0.001 max_iter 2000 rc 0 L 2 loci 1 3"
     assert out[1] == "context created" or out[1] == f"context refused: {sd.SDCA_E_HIP} with message"
     assert out[2] == f"null input rc {sd.SDCA_E_ARG}"
+
+
+def test_bench_roofline_fields_from_timings():
+    # the benchmark's roofline object from a library timing record (no device needed):
+    # the dominant kernel by device time, achieved = algorithmic work / time, frac against
+    # the MI355X peak, traffic = algorithmic bytes per launch x the measured counter ratio
+    import importlib.util
+    import sys
+
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    argv = sys.argv
+    try:
+        sys.argv = ["bench.py"]
+        spec.loader.exec_module(bench)
+    finally:
+        sys.argv = argv
+    r = 1000
+    tm = dict(gemm_energy_ms=2.2 * r, flops_energy=1.6e15, n_energy_launches=r,
+              gemm_grad_ms=2.8 * r, flops_grad=1.6e15, n_grad_launches=r,
+              update_ms=3.1 * r, bytes_update=15.6e9 * r, dots_ms=2.1 * r, bytes_dots=12.5e9 * r, n_evals=r,
+              softmax_ms=0.1 * r, score_ms=0.05 * r)
+    rl, rates = bench.roofline(tm, True)
+    assert rl["kernel"] == "k_update" and rl["bound"] == "hbm" and rl["unit"] == "GB/s"
+    assert rl["achieved"] == pytest.approx(15.6e9 / 3.1e-3 / 1e9, rel=1e-4)
+    assert rl["frac"] == pytest.approx(rl["achieved"] / 8000.0, rel=1e-4)
+    assert rl["traffic"] == pytest.approx(15.6e9 * 13.34 / 13.10, rel=1e-6)
+    assert rl["launches"] == r and rl["avg_launch_us"] == pytest.approx(3100.0)
+    assert rates["k_gemm_nt<EpiEnergy>"] == pytest.approx(1.6e15 / 2.2 / 1e12, rel=1e-3)
+    phases, weighted = bench.phase_fractions(tm, True)
+    assert set(phases) == {"k_gemm_nt<EpiEnergy>", "k_gemm_nt<EpiGrad>", "k_update", "k_dots"}
+    assert phases["k_gemm_nt<EpiGrad>"]["frac"] == pytest.approx(1.6e15 / 2.8 / 1e12 / 2500.0, abs=1e-4)
+    fr = [p["frac"] for p in phases.values()]
+    ms = [p["ms"] for p in phases.values()]
+    assert weighted == pytest.approx(sum(f * m for f, m in zip(fr, ms)) / sum(ms), abs=1e-3)
