@@ -243,6 +243,14 @@ int sdca_prepare_alignment(const char *chars, int32_t B, int32_t L_in, double ma
 int sdca_write_couplings(const char *path, const float *S, int32_t L, const int32_t *translation,
                          int32_t base);
 
+/* The largest symmetrised scores only, in the line format of sdca_write_couplings: the
+ * pairs with a value >= min_score (NaN never), the top_n largest of them (all when
+ * top_n == 0), largest first and equal values in the order of the full file.  No
+ * counterpart in SuperDCA, whose .all file lists every pair (plmDCA.hpp:667-693); its
+ * users rank that file afterwards (the top pairs of the paper's S1 table). */
+int sdca_write_couplings_top(const char *path, const float *S, int32_t L, const int32_t *translation,
+                             int32_t base, int64_t top_n, double min_score);
+
 #ifdef __cplusplus
 }
 #endif

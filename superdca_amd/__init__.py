@@ -44,7 +44,7 @@ EXPORTED = (
     "sdca_prepare_alignment", "sdca_write_couplings", "sdca_pair_counts", "sdca_pair_count_table",
     "sdca_mi_topk", "sdca_pair_features", "sdca_mlp_init", "sdca_mlp_num_params", "sdca_mlp_get_params",
     "sdca_mlp_set_params", "sdca_mlp_forward", "sdca_mlp_loss_grad", "sdca_mlp_train", "sdca_solve_loci_mlp",
-    "sdca_gemm_config",
+    "sdca_gemm_config", "sdca_write_couplings_top",
 )
 
 
@@ -103,6 +103,8 @@ def lib() -> C.CDLL:
     L.sdca_prepare_alignment.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32,
                                          C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
     L.sdca_write_couplings.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
+    L.sdca_write_couplings_top.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int64,
+                                           C.c_double]
     L.sdca_pair_counts.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
     L.sdca_pair_count_table.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
     L.sdca_mi_topk.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
@@ -124,7 +126,7 @@ def lib() -> C.CDLL:
                  "sdca_get_timing", "sdca_prepare_alignment", "sdca_write_couplings", "sdca_pair_counts",
                  "sdca_pair_count_table", "sdca_mi_topk", "sdca_pair_features", "sdca_mlp_init",
                  "sdca_mlp_get_params", "sdca_mlp_set_params", "sdca_mlp_forward", "sdca_mlp_loss_grad",
-                 "sdca_mlp_train", "sdca_solve_loci_mlp", "sdca_gemm_config"):
+                 "sdca_mlp_train", "sdca_solve_loci_mlp", "sdca_gemm_config", "sdca_write_couplings_top"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -358,3 +360,13 @@ def write_couplings(path: str, S: np.ndarray, translation: np.ndarray, base: int
     S = np.ascontiguousarray(S, dtype=np.float32)
     tr = np.ascontiguousarray(translation, dtype=np.int32)
     _check(lib().sdca_write_couplings(path.encode(), _ptr(S), S.shape[0], _ptr(tr), int(base)))
+
+
+def write_couplings_top(path: str, S: np.ndarray, translation: np.ndarray, base: int = 1, top_n: int = 0,
+                        min_score: float = float("-inf")) -> None:
+    """The largest symmetrised scores, largest first, in the line format of write_couplings
+    (no counterpart in SuperDCA, whose .all file, plmDCA.hpp:667-693, lists every pair)."""
+    S = np.ascontiguousarray(S, dtype=np.float32)
+    tr = np.ascontiguousarray(translation, dtype=np.int32)
+    _check(lib().sdca_write_couplings_top(path.encode(), _ptr(S), S.shape[0], _ptr(tr), int(base), int(top_n),
+                                          float(min_score)))

@@ -8,6 +8,7 @@
 // their own group (--device, --batch-loci, --precision, --history, --q-env).
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -35,6 +36,8 @@ struct Options {
     double gradient_threshold = 1e-3;  // plmDCA_options.cpp:61
     double lambda_h = -1.0, lambda_J = -1.0;
     bool no_dca = false, no_coupling_output = false;
+    long output_top = 0;                // > 0: also write the largest scores (.top)
+    double output_min_score = -HUGE_VAL;  // and/or only scores at least this large
     double maf_threshold = 0.01, gap_threshold = 0.15;  // README.md:32
     bool no_filter = false;
     int output_base = 1, input_base = 1;
@@ -71,6 +74,8 @@ const char *kUsage =
     "  --lambda-J arg (=-1)           (< 0: automatic)\n"
     "  --no-dca                       Stop after the weights.\n"
     "  --no-coupling-output           Do not write coupling scores.\n"
+    "  --output-top arg               Also write the arg largest scores, largest first (.top).\n"
+    "  --output-min-score arg         Also write every score >= arg, largest first (.top).\n"
     "GPU:\n"
     "  --device arg (=0)  --batch-loci arg (=128)  --history arg (=10)  --max-iter arg (=2000)\n"
     "  --precision fp32|split-bf16 (=fp32)  --q-env arg (=4; 5 keeps four alleles plus gap)\n";
@@ -135,6 +140,8 @@ Options parse_args(int argc, char **argv)
         else if (a == "--lambda-J") o.lambda_J = std::atof(need(i).c_str());
         else if (a == "--no-dca") o.no_dca = true;
         else if (a == "--no-coupling-output") o.no_coupling_output = true;
+        else if (a == "--output-top") o.output_top = std::atol(need(i).c_str());
+        else if (a == "--output-min-score") o.output_min_score = std::atof(need(i).c_str());
         else if (a == "--maf-threshold") o.maf_threshold = std::atof(need(i).c_str());
         else if (a == "--gap-threshold") o.gap_threshold = std::atof(need(i).c_str());
         else if (a == "--no-filter-alignment") o.no_filter = true;
@@ -369,7 +376,8 @@ int main(int argc, char **argv)
                 << " nfeval=" << st[k].nfeval << " iter=" << st[k].iter << "\n";
         log << "plmDCA: solved " << K << " loci in " << solve_s << " s (" << K / solve_s << " loci/s)\n";
     }
-    if (!o.no_coupling_output) {
+    const bool want_top = o.output_top > 0 || o.output_min_score > -HUGE_VAL;
+    if (!o.no_coupling_output || want_top) {
         // scores among the listed loci, row = list position (plmDCA.hpp:684-691)
         std::vector<float> S((size_t)K * K);
         std::vector<int32_t> tr(K);
@@ -377,11 +385,20 @@ int main(int argc, char **argv)
             tr[k] = translation[loci[k]];
             for (int u = 0; u < K; ++u) S[(size_t)k * K + u] = rows[(size_t)k * L + loci[u]];
         }
-        const std::string out =
-            unique_name(id + ".SuperDCA_couplings." + std::to_string(o.output_base) + "-based.all");
-        if (o.verbose) log << "\nplmDCA: writing coupling values to file \"" << out << "\"\n";
-        if (sdca_write_couplings(out.c_str(), S.data(), K, tr.data(), o.output_base) != SDCA_OK)
-            die("could not write \"" + out + "\"");
+        const std::string stem = id + ".SuperDCA_couplings." + std::to_string(o.output_base) + "-based";
+        if (!o.no_coupling_output) {
+            const std::string out = unique_name(stem + ".all");
+            if (o.verbose) log << "\nplmDCA: writing coupling values to file \"" << out << "\"\n";
+            if (sdca_write_couplings(out.c_str(), S.data(), K, tr.data(), o.output_base) != SDCA_OK)
+                die("could not write \"" + out + "\"");
+        }
+        if (want_top) {
+            const std::string out = unique_name(stem + ".top");
+            if (o.verbose) log << "plmDCA: writing the largest coupling values to file \"" << out << "\"\n";
+            if (sdca_write_couplings_top(out.c_str(), S.data(), K, tr.data(), o.output_base, o.output_top,
+                                         o.output_min_score) != SDCA_OK)
+                die("could not write \"" + out + "\"");
+        }
     }
     sdca_ctx_destroy(ctx);
     if (o.verbose) {

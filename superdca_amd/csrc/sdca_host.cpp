@@ -9,6 +9,7 @@
 // most frequent allele over the non-gap count; alleles are numbered by descending count
 // with ties broken A<C<G<T; the gap is the last envelope state; loci with more alleles
 // than the envelope holds are dropped.
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdint>
@@ -157,6 +158,58 @@ int sdca_write_couplings(const char *path, const float *S, int32_t L, const int3
     }
     if (rc == SDCA_OK && p > buf.data() && fwrite(buf.data(), 1, p - buf.data(), fp) != (size_t)(p - buf.data()))
         rc = SDCA_E_IO;
+    if (fclose(fp) != 0) rc = SDCA_E_IO;
+    return rc;
+}
+
+// The highest symmetrised scores only: the pairs of sdca_write_couplings with a value of
+// at least min_score, the top_n largest of them (all if top_n == 0), largest first, equal
+// values in the order of the full file.  One pass over the pairs with a bounded min-heap,
+// so a genome-scale matrix needs no sort of its L(L-1)/2 values.
+int sdca_write_couplings_top(const char *path, const float *S, int32_t L, const int32_t *translation, int32_t base,
+                             int64_t top_n, double min_score)
+{
+    if (!path || !S || !translation || L < 0 || top_n < 0 || std::isnan(min_score)) return SDCA_E_ARG;
+    struct Pair {
+        double v;
+        int64_t ord;  // position in the full file
+        int32_t r, n;
+    };
+    // a before b in the output
+    auto first = [](const Pair &a, const Pair &b) { return a.v > b.v || (a.v == b.v && a.ord < b.ord); };
+    std::vector<Pair> keep;
+    int64_t ord = 0;
+    for (int r = 0; r < L; ++r)
+        for (int n = 0; n < r; ++n, ++ord) {
+            const double v = (double)(float)(S[(size_t)r * L + n] + S[(size_t)n * L + r]) * 0.5;
+            if (!(v >= min_score)) continue;  // also drops NaN
+            const Pair x{v, ord, r, n};
+            if (top_n == 0 || (int64_t)keep.size() < top_n) {
+                keep.push_back(x);
+                if (top_n > 0) std::push_heap(keep.begin(), keep.end(), first);  // front: the last in order
+            } else if (first(x, keep.front())) {
+                std::pop_heap(keep.begin(), keep.end(), first);
+                keep.back() = x;
+                std::push_heap(keep.begin(), keep.end(), first);
+            }
+        }
+    std::sort(keep.begin(), keep.end(), first);
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return SDCA_E_IO;
+    int rc = SDCA_OK;
+    char line[128];  // a float score prints in at most 50 characters, each index in 20
+    for (const Pair &x : keep) {
+        char *p = put_fixed8(line, x.v);
+        *p++ = ' ';
+        p = put_uint(p, (uint64_t)((int64_t)translation[x.r] + base));
+        *p++ = ' ';
+        p = put_uint(p, (uint64_t)((int64_t)translation[x.n] + base));
+        *p++ = '\n';
+        if (fwrite(line, 1, p - line, fp) != (size_t)(p - line)) {
+            rc = SDCA_E_IO;
+            break;
+        }
+    }
     if (fclose(fp) != 0) rc = SDCA_E_IO;
     return rc;
 }

@@ -105,6 +105,45 @@ def test_writer_rounding_matches_printf(tmp_path):
             k += 1
 
 
+def _full_file_values(S):
+    # the symmetrised value of every pair, in the order of the full file (plmDCA.hpp:684-691)
+    L = S.shape[0]
+    vals = []
+    for r in range(L):
+        for n in range(r):
+            vals.append(float(np.float32(S[r, n] + S[n, r])) * 0.5)
+    return vals
+
+
+@pytest.mark.parametrize("top_n,min_score", [(1, float("-inf")), (25, float("-inf")), (0, 0.5), (40, 0.9),
+                                             (10 ** 9, float("-inf"))])
+def test_top_writer_lists_the_largest_lines_of_the_full_file(tmp_path, top_n, min_score):
+    rng = np.random.default_rng(8)
+    L = 120
+    S = rng.random((L, L)).astype(np.float32)
+    S[rng.random((L, L)) < 0.02] = np.nan  # never listed
+    S[:10, :10] = 0.25  # equal values: file order
+    tr = np.sort(rng.choice(50000, L, replace=False)).astype(np.int32)
+    sd.write_couplings(str(tmp_path / "f.all"), S, tr, 0)
+    full = (tmp_path / "f.all").read_text().splitlines()
+    vals = _full_file_values(S)
+    assert len(full) == len(vals)
+    order = sorted((k for k, v in enumerate(vals) if v >= min_score), key=lambda k: (-vals[k], k))
+    if top_n:
+        order = order[:top_n]
+    sd.write_couplings_top(str(tmp_path / "f.top"), S, tr, 0, top_n if top_n < 10 ** 9 else 0, min_score)
+    assert (tmp_path / "f.top").read_text().splitlines() == [full[k] for k in order]
+
+
+def test_top_writer_rejects_bad_arguments(tmp_path):
+    S = np.zeros((3, 3), dtype=np.float32)
+    tr = np.arange(3, dtype=np.int32)
+    with pytest.raises(sd.SdcaError):
+        sd.write_couplings_top(str(tmp_path / "x.top"), S, tr, 1, -1)
+    with pytest.raises(sd.SdcaError):
+        sd.write_couplings_top(str(tmp_path / "x.top"), S, tr, 1, 5, float("nan"))
+
+
 def test_uneven_batch_shapes():
     cfgs = synth.c3_workloads()
     assert [c["seed"] for c in cfgs] == list(range(3000, 3048))
