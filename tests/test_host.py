@@ -177,6 +177,7 @@ def test_cli_help_and_errors(tmp_path):
         build.build_cli()
     p = subprocess.run([exe, "--help"], capture_output=True, text=True)
     assert p.returncode == 0 and "--gradient-threshold" in p.stdout and "--maf-threshold" in p.stdout
+    assert "--output-top" in p.stdout and "--output-min-score" in p.stdout
     p = subprocess.run([exe], capture_output=True, text=True)
     assert p.returncode != 0 and "No alignment file" in p.stderr
     p = subprocess.run([exe, str(tmp_path / "missing.fasta")], capture_output=True, text=True)
