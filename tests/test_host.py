@@ -365,3 +365,13 @@ def test_bench_roofline_fields_from_timings():
     fr = [p["frac"] for p in phases.values()]
     ms = [p["ms"] for p in phases.values()]
     assert weighted == pytest.approx(sum(f * m for f, m in zip(fr, ms)) / sum(ms), abs=1e-3)
+
+
+def test_every_hip_compile_disables_packed_fp32_and_targets_gfx950():
+    # packed FP32 instructions are kept out of every kernel build
+    from superdca_amd import build
+
+    flags = build.HIP_FLAGS
+    assert "--offload-arch=gfx950" in flags
+    k = flags.index("-packed-fp32-ops")
+    assert flags[k - 3:k] == ["-Xclang", "-target-feature", "-Xclang"]
