@@ -20,6 +20,8 @@
 
 #include "../../include/sdca.h"
 
+int sdca_fail(int code, const char *fmt, ...);  // sets the message of sdca_last_error (sdca_api.cpp)
+
 namespace sdca {
 
 // Offset of J_r[j](a, s) in SuperDCA's parameter vector: 16-locus blocks of q*q*16
@@ -90,7 +92,8 @@ extern "C" {
 int sdca_prepare_alignment(const char *chars, int32_t B, int32_t L_in, double maf_threshold, double gap_threshold,
                            int32_t no_filter, int32_t q_env, uint8_t *states_out, int32_t *loci_out, int32_t *L_out)
 {
-    if (!chars || !states_out || !loci_out || !L_out || B < 1 || L_in < 0 || q_env < 2 || q_env > 8) return SDCA_E_ARG;
+    if (!chars || !states_out || !loci_out || !L_out || B < 1 || L_in < 0 || q_env < 2 || q_env > 8)
+        return sdca_fail(SDCA_E_ARG, "sdca_prepare_alignment: null buffer, B < 1, L_in < 0 or q_env outside [2, 8]");
     std::vector<int32_t> kept;
     std::vector<std::array<int, 4>> maps;
     kept.reserve(L_in);
@@ -131,9 +134,10 @@ int sdca_prepare_alignment(const char *chars, int32_t B, int32_t L_in, double ma
 
 int sdca_write_couplings(const char *path, const float *S, int32_t L, const int32_t *translation, int32_t base)
 {
-    if (!path || !S || !translation || L < 0) return SDCA_E_ARG;
+    if (!path || !S || !translation || L < 0)
+        return sdca_fail(SDCA_E_ARG, "sdca_write_couplings: null argument or L < 0");
     FILE *fp = fopen(path, "wb");
-    if (!fp) return SDCA_E_IO;
+    if (!fp) return sdca_fail(SDCA_E_IO, "cannot open \"%s\" for writing", path);
     std::vector<char> buf(1 << 22);
     char *p = buf.data();
     char *const end = buf.data() + buf.size() - 128;
@@ -159,6 +163,7 @@ int sdca_write_couplings(const char *path, const float *S, int32_t L, const int3
     if (rc == SDCA_OK && p > buf.data() && fwrite(buf.data(), 1, p - buf.data(), fp) != (size_t)(p - buf.data()))
         rc = SDCA_E_IO;
     if (fclose(fp) != 0) rc = SDCA_E_IO;
+    if (rc != SDCA_OK) sdca_fail(rc, "writing \"%s\" failed", path);
     return rc;
 }
 
@@ -169,7 +174,8 @@ int sdca_write_couplings(const char *path, const float *S, int32_t L, const int3
 int sdca_write_couplings_top(const char *path, const float *S, int32_t L, const int32_t *translation, int32_t base,
                              int64_t top_n, double min_score)
 {
-    if (!path || !S || !translation || L < 0 || top_n < 0 || std::isnan(min_score)) return SDCA_E_ARG;
+    if (!path || !S || !translation || L < 0 || top_n < 0 || std::isnan(min_score))
+        return sdca_fail(SDCA_E_ARG, "sdca_write_couplings_top: null argument, L < 0, top_n < 0 or NaN min_score");
     struct Pair {
         double v;
         int64_t ord;  // position in the full file
@@ -195,7 +201,7 @@ int sdca_write_couplings_top(const char *path, const float *S, int32_t L, const 
         }
     std::sort(keep.begin(), keep.end(), first);
     FILE *fp = fopen(path, "wb");
-    if (!fp) return SDCA_E_IO;
+    if (!fp) return sdca_fail(SDCA_E_IO, "cannot open \"%s\" for writing", path);
     int rc = SDCA_OK;
     char line[128];  // a float score prints in at most 50 characters, each index in 20
     for (const Pair &x : keep) {
@@ -211,6 +217,7 @@ int sdca_write_couplings_top(const char *path, const float *S, int32_t L, const 
         }
     }
     if (fclose(fp) != 0) rc = SDCA_E_IO;
+    if (rc != SDCA_OK) sdca_fail(rc, "writing \"%s\" failed", path);
     return rc;
 }
 

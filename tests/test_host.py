@@ -138,10 +138,14 @@ def test_top_writer_lists_the_largest_lines_of_the_full_file(tmp_path, top_n, mi
 def test_top_writer_rejects_bad_arguments(tmp_path):
     S = np.zeros((3, 3), dtype=np.float32)
     tr = np.arange(3, dtype=np.int32)
-    with pytest.raises(sd.SdcaError):
+    with pytest.raises(sd.SdcaError, match="top_n < 0"):
         sd.write_couplings_top(str(tmp_path / "x.top"), S, tr, 1, -1)
-    with pytest.raises(sd.SdcaError):
+    with pytest.raises(sd.SdcaError, match="NaN min_score"):
         sd.write_couplings_top(str(tmp_path / "x.top"), S, tr, 1, 5, float("nan"))
+    with pytest.raises(sd.SdcaError, match="cannot open"):
+        sd.write_couplings_top(str(tmp_path / "no_such_dir" / "x.top"), S, tr, 1, 5)
+    with pytest.raises(sd.SdcaError, match="cannot open"):
+        sd.write_couplings(str(tmp_path / "no_such_dir" / "x.all"), S, tr, 1)
 
 
 def test_uneven_batch_shapes():
