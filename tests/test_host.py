@@ -148,6 +148,19 @@ def test_top_writer_rejects_bad_arguments(tmp_path):
         sd.write_couplings(str(tmp_path / "no_such_dir" / "x.all"), S, tr, 1)
 
 
+def test_filter_reads_either_case_and_treats_other_letters_as_gaps():
+    # lower case is the same allele; N and IUPAC codes count as gaps (MAF over non-gaps)
+    col_upper = np.frombuffer(b"AAAAAAAACC", np.uint8)
+    col_lower = np.frombuffer(b"aaaaAAAAcC", np.uint8)
+    col_n = np.frombuffer(b"AAAAAAACCN", np.uint8)  # 1 gap of 10
+    col_iupac = np.frombuffer(b"AAAARYCCAA", np.uint8)  # 2 gaps of 10 > 0.15: dropped
+    ch = np.stack([col_upper, col_lower, col_n, col_iupac], axis=1)
+    st, kept = sd.prepare_alignment(ch)
+    assert kept.tolist() == [0, 1, 2]
+    np.testing.assert_array_equal(st[:, 0], st[:, 1])
+    assert st[9, 2] == 3 and st[:, 2].tolist()[:7] == [0] * 7  # N is the gap state (q - 1)
+
+
 def test_uneven_batch_shapes():
     cfgs = synth.c3_workloads()
     assert [c["seed"] for c in cfgs] == list(range(3000, 3048))
