@@ -201,6 +201,19 @@ def test_cli_help_and_errors(tmp_path):
     assert p.returncode != 0 and "Could not get alignment" in p.stderr
     p = subprocess.run([exe, "--bogus-flag", "x.fa"], capture_output=True, text=True)
     assert p.returncode != 0
+    fa = tmp_path / "t.fa"
+    fa.write_text(">a\nACGT\n>b\nACG\n")
+    p = subprocess.run([exe, str(fa)], capture_output=True, text=True)
+    assert p.returncode != 0 and "different lengths" in p.stderr
+    fa.write_text("ACGT\n")
+    p = subprocess.run([exe, str(fa)], capture_output=True, text=True)
+    assert p.returncode != 0 and "not in FASTA format" in p.stderr
+    fa.write_text(">a\nACGT\n>b\nAGGT\n")
+    (tmp_path / "cols.txt").write_text("1,9\n")
+    p = subprocess.run([exe, "--filterlistfile", str(tmp_path / "cols.txt"), str(fa)], capture_output=True, text=True)
+    assert p.returncode != 0 and "filter list index out of range" in p.stderr
+    p = subprocess.run([exe, "--norm-of-mean-scoring", str(fa)], capture_output=True, text=True)
+    assert p.returncode != 0 and "not supported" in p.stderr
 
 
 def test_context_without_gpu_fails_loudly():
