@@ -265,9 +265,10 @@ def test_gauge_hand_example():
     assert G[3, 1] == 13 - 5.5 - 9 + 7.5
 
 
-def test_gauge_symmetric_equals_textbook_and_invariances():
+@pytest.mark.parametrize("q", [3, 4, 5])
+def test_gauge_symmetric_equals_textbook_and_invariances(q):
     rng = np.random.default_rng(10)
-    A = rng.normal(size=(4, 4))
+    A = rng.normal(size=(q, q))
     S = A + A.T
     textbook = S - S.mean(axis=1, keepdims=True) - S.mean(axis=0, keepdims=True) + S.mean()
     np.testing.assert_allclose(orc.ising_gauge(S), textbook, atol=1e-14)
@@ -287,8 +288,10 @@ def test_frobenius_excludes_gap_row_and_column():
     assert orc.frobenius_excl(G, 1) == pytest.approx(np.sqrt((G[np.ix_(keep, keep)] ** 2).sum()), rel=1e-15)
 
 
-def test_score_row_reads_block_of_each_partner():
-    L, q, r = 20, 4, 6
+@pytest.mark.parametrize("q", [3, 4, 5])
+def test_score_row_reads_block_of_each_partner(q):
+    # every envelope width the engine uses (gap = state q-1)
+    L, r = 20, 6
     rng = np.random.default_rng(11)
     x = rng.normal(size=orc.nparams(L, q))
     row = orc.score_row(x, L, q, r)
@@ -296,7 +299,7 @@ def test_score_row_reads_block_of_each_partner():
     for n in (0, 5, 7, 16, 19):
         M = np.array([[x[orc.ref_index(L, q, n, a, s)] for s in range(q)] for a in range(q)])
         G = orc.ising_gauge(M, transpose=n < r)
-        assert row[n] == np.float32(np.sqrt((G[:3, :3] ** 2).sum()))
+        assert row[n] == np.float32(np.sqrt((G[:q - 1, :q - 1] ** 2).sum()))
 
 
 # ------------------------------------------------------- warm-start network
