@@ -122,6 +122,25 @@ def test_objective_finite_differences():
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i]))
 
 
+@pytest.mark.parametrize("q", [3, 5])
+def test_objective_finite_differences_other_envelopes(q):
+    # the softmax spans the whole q-state envelope, unobserved states included
+    rng = np.random.default_rng(30 + q)
+    B, L, r = 40, 19, 16
+    st = rand_states(rng, B, L, q=q)
+    w = rng.uniform(0.1, 1.0, B)
+    x = rng.normal(size=orc.nparams(L, q)) * 0.3
+    f, g = orc.objective(st, r, w, 2.0, 1.5, x, q=q)
+    h = 1e-6
+    for i in rng.choice(x.size, 40, replace=False):
+        e = np.zeros_like(x)
+        e[i] = h
+        fd = (orc.objective(st, r, w, 2.0, 1.5, x + e, q=q)[0] - orc.objective(st, r, w, 2.0, 1.5, x - e, q=q)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i]))
+    own = [orc.ref_index(L, q, r, a, s) for a in range(q) for s in range(q)]
+    assert np.all(g[own] == 0.0)
+
+
 def test_objective_excludes_own_block():
     rng, st, w, x = setup_small()
     r, L, q = 5, st.shape[1], 4
