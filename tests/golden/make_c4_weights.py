@@ -1,4 +1,4 @@
-"""Writes tests/golden/c4_weights.npz (run from the repository root; about a minute on
+"""Writes tests/golden/c4_weights.npz (run from the repository root; about 20 s on
 eight cores).
 
 The C4 workload (B = 3000 sequences, L = 94,000 loci, q_env = 5, seed 1004 of
