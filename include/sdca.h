@@ -158,6 +158,16 @@ int sdca_eval_batch(sdca_ctx *ctx, const int32_t *loci, int32_t K, const double 
 int sdca_solve_loci(sdca_ctx *ctx, const int32_t *loci, int32_t K, const double *warm,
                     float *score_rows, double *x_out, sdca_locus_stats *st);
 
+/* Two-alignment scan (SuperDCA.cpp:155-159, plmDCA.hpp:482-490): the context's alignment is
+ * the first alignment's n_first loci followed by the second's, over the same sequences.
+ * With n_first > 0 a solve takes loci of the first alignment only and fits each on the
+ * second alignment's columns alone (plmDCA_objective.hpp:67: no block of the second
+ * alignment is excluded, the first is not a predictor); its score row is zero over the
+ * first alignment.  Install the second alignment's weights (plmDCA.hpp:538).  Cold start
+ * only; sdca_eval_batch and sdca_solve_loci_mlp refuse a scanning context.  0 turns the
+ * scan off. */
+int sdca_set_scan_split(sdca_ctx *ctx, int32_t n_first);
+
 /* Timings accumulated since the context was created or last reset (no counterpart in
  * SuperDCA beyond its per-locus log, plmDCA.hpp:306-325). */
 int sdca_get_timing(sdca_ctx *ctx, sdca_timing *t, int32_t reset);
@@ -250,6 +260,13 @@ int sdca_write_couplings(const char *path, const float *S, int32_t L, const int3
  * users rank that file afterwards (the top pairs of the paper's S1 table). */
 int sdca_write_couplings_top(const char *path, const float *S, int32_t L, const int32_t *translation,
                              int32_t base, int64_t top_n, double min_score);
+
+/* The scan's output (plmDCA.hpp:625-638,672-681): for each of the K first-alignment loci
+ * (row k of S, K x L2: its scores over the second alignment) and every second-alignment
+ * locus n, the line "score r n" with the float score printed %.8f, r = translation1[k] +
+ * base and n = translation2[n] + base; not symmetrised. */
+int sdca_write_scan_couplings(const char *path, const float *S, int32_t K, int32_t L2,
+                              const int32_t *translation1, const int32_t *translation2, int32_t base);
 
 #ifdef __cplusplus
 }

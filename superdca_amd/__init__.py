@@ -44,7 +44,7 @@ EXPORTED = (
     "sdca_prepare_alignment", "sdca_write_couplings", "sdca_pair_counts", "sdca_pair_count_table",
     "sdca_mi_topk", "sdca_pair_features", "sdca_mlp_init", "sdca_mlp_num_params", "sdca_mlp_get_params",
     "sdca_mlp_set_params", "sdca_mlp_forward", "sdca_mlp_loss_grad", "sdca_mlp_train", "sdca_solve_loci_mlp",
-    "sdca_gemm_config", "sdca_write_couplings_top",
+    "sdca_gemm_config", "sdca_write_couplings_top", "sdca_set_scan_split", "sdca_write_scan_couplings",
 )
 
 
@@ -105,6 +105,9 @@ def lib() -> C.CDLL:
     L.sdca_write_couplings.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
     L.sdca_write_couplings_top.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_int64,
                                            C.c_double]
+    L.sdca_set_scan_split.argtypes = [C.c_void_p, C.c_int32]
+    L.sdca_write_scan_couplings.argtypes = [C.c_char_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                            C.c_int32]
     L.sdca_pair_counts.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
     L.sdca_pair_count_table.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
     L.sdca_mi_topk.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
@@ -126,7 +129,8 @@ def lib() -> C.CDLL:
                  "sdca_get_timing", "sdca_prepare_alignment", "sdca_write_couplings", "sdca_pair_counts",
                  "sdca_pair_count_table", "sdca_mi_topk", "sdca_pair_features", "sdca_mlp_init",
                  "sdca_mlp_get_params", "sdca_mlp_set_params", "sdca_mlp_forward", "sdca_mlp_loss_grad",
-                 "sdca_mlp_train", "sdca_solve_loci_mlp", "sdca_gemm_config", "sdca_write_couplings_top"):
+                 "sdca_mlp_train", "sdca_solve_loci_mlp", "sdca_gemm_config", "sdca_write_couplings_top",
+                 "sdca_set_scan_split", "sdca_write_scan_couplings"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -193,6 +197,11 @@ class Context:
     @property
     def num_columns(self) -> int:
         return int(lib().sdca_num_columns(self._h))
+
+    def set_scan_split(self, n_first: int) -> None:
+        """Two-alignment scan (SuperDCA.cpp:155-159): loci [0, n_first) are the first
+        alignment's and are fitted on the remaining columns only; 0 turns it off."""
+        _check(lib().sdca_set_scan_split(self._h, int(n_first)))
 
     def reweight(self, theta: float = 0.9, mult: np.ndarray | None = None):
         """calculate_weights (plmDCA_optimizer_parameters.hpp:39-98) -> (w, int_w)."""
@@ -360,6 +369,18 @@ def write_couplings(path: str, S: np.ndarray, translation: np.ndarray, base: int
     S = np.ascontiguousarray(S, dtype=np.float32)
     tr = np.ascontiguousarray(translation, dtype=np.int32)
     _check(lib().sdca_write_couplings(path.encode(), _ptr(S), S.shape[0], _ptr(tr), int(base)))
+
+
+def write_scan_couplings(path: str, S: np.ndarray, translation1: np.ndarray, translation2: np.ndarray,
+                         base: int = 1) -> None:
+    """The scan's "score r n" lines (plmDCA.hpp:625-638,672-681): S is K x L2."""
+    S = np.ascontiguousarray(S, dtype=np.float32)
+    t1 = np.ascontiguousarray(translation1, dtype=np.int32)
+    t2 = np.ascontiguousarray(translation2, dtype=np.int32)
+    if S.shape != (t1.size, t2.size):
+        raise ValueError(f"S is {S.shape}, expected ({t1.size}, {t2.size})")
+    _check(lib().sdca_write_scan_couplings(path.encode(), _ptr(S), S.shape[0], S.shape[1], _ptr(t1), _ptr(t2),
+                                           int(base)))
 
 
 def write_couplings_top(path: str, S: np.ndarray, translation: np.ndarray, base: int = 1, top_n: int = 0,

@@ -775,8 +775,10 @@ public:
         k.gt = S.gt;
         k.gk = -1;
         k.pnew = -1;
-        k.self_lo = c_->col_off[r];
-        k.self_hi = c_->col_off[r + 1];
+        // the masked columns: the locus' own, or in a two-alignment scan every column of
+        // the first alignment (plmDCA_objective.hpp:67: no block of the second is excluded)
+        k.self_lo = c_->scan_split > 0 ? 0 : c_->col_off[r];
+        k.self_hi = c_->scan_split > 0 ? c_->col_off[c_->scan_split] : c_->col_off[r + 1];
         k.init_src = warm ? 1.f : 0.f;
     }
 
@@ -1251,6 +1253,16 @@ void sdca_ctx_destroy(sdca_ctx *ctx) { delete ctx; }
 
 int64_t sdca_num_columns(const sdca_ctx *ctx) { return ctx ? ctx->Kx : -1; }
 
+int sdca_set_scan_split(sdca_ctx *c, int32_t n_first)
+{
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (n_first < 0 || n_first >= c->L)
+        return fail(SDCA_E_ARG, "the first alignment's loci must number in [0, L) (got %d of L=%d)", n_first, c->L);
+    c->scan_split = n_first;
+    return SDCA_OK;
+}
+
 int sdca_reweight(sdca_ctx *c, double theta, const uint32_t *mult, double *w_out, uint64_t *int_w_out)
 {
     int rc = check_ctx(c);
@@ -1339,6 +1351,7 @@ int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x
     int rc = check_ctx(c);
     if (rc) return rc;
     if (!c->have_weights) return fail(SDCA_E_STATE, "sdca_set_weights must be called first");
+    if (c->scan_split > 0) return fail(SDCA_E_STATE, "sdca_eval_batch does not evaluate the scan objective");
     if (K < 0 || (K > 0 && (!loci || !x || !f_out || !g_out))) return fail(SDCA_E_ARG, "bad arguments");
     for (int k = 0; k < K; ++k)
         if (loci[k] < 0 || loci[k] >= c->L) return fail(SDCA_E_ARG, "locus %d out of range", loci[k]);
@@ -1417,6 +1430,14 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
     if (K < 0 || (K > 0 && !loci)) return fail(SDCA_E_ARG, "bad arguments");
     for (int k = 0; k < K; ++k)
         if (loci[k] < 0 || loci[k] >= c->L) return fail(SDCA_E_ARG, "locus %d out of range", loci[k]);
+    if (c->scan_split > 0) {
+        // the masked first alignment must stay at zero: no start points there
+        if (warm || cw) return fail(SDCA_E_STATE, "a two-alignment scan solves from a cold start only");
+        for (int k = 0; k < K; ++k)
+            if (loci[k] >= c->scan_split)
+                return fail(SDCA_E_ARG, "locus %d is not in the first alignment (loci [0, %d))", loci[k],
+                            c->scan_split);
+    }
     if (K == 0) return SDCA_OK;
     const auto t0 = std::chrono::steady_clock::now();
     const int nslots = std::min(c->opts.batch_loci, K);

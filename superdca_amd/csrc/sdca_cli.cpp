@@ -47,7 +47,8 @@ struct Options {
 };
 
 const char *kUsage =
-    "Usage: sdca [options] <alignmentfile>\n"
+    "Usage: sdca [options] <alignmentfile> [<second alignmentfile>]\n"
+    "  (with two alignments: scan every locus of the first against the loci of the second)\n"
     "General:\n"
     "  -h [ --help ]                  Print this help message.\n"
     "  -v [ --verbose ]               Be verbose.\n"
@@ -169,9 +170,8 @@ Options parse_args(int argc, char **argv)
         std::cerr << "SuperDCA ERROR: No alignment file specified!\n" << kUsage;
         std::exit(EXIT_FAILURE);
     }
-    if (o.alignments.size() > 1)
-        die("cannot handle more than 1 alignment: the two-alignment scan mode is not supported (got " +
-            std::to_string(o.alignments.size()) + ")");
+    if (o.alignments.size() > 2)  // SuperDCA.cpp:155-159
+        die("cannot handle more than 2 alignments (got " + std::to_string(o.alignments.size()) + " filenames)");
     if (o.norm_of_mean) die("--norm-of-mean-scoring is not supported");
     return o;
 }
@@ -228,10 +228,133 @@ void write_fasta(const std::string &path, const Fasta &f)
 
 }  // namespace
 
+sdca_opts solver_opts(const Options &o)
+{
+    sdca_opts so;
+    sdca_default_opts(&so);
+    so.device = o.device;
+    so.precision = o.precision;
+    so.batch_loci = o.batch_loci;
+    so.history = o.history;
+    so.max_iter = o.max_iter;
+    so.grad_threshold = o.gradient_threshold;
+    so.verbose = o.verbose;
+    return so;
+}
+
+// Two-alignment scan (SuperDCA.cpp:155-159, plmDCA.hpp:482-490): every locus of the first
+// alignment against the loci of the second, weights from the second (plmDCA.hpp:538),
+// "score r n" lines, not symmetrised, in <first>_scan.SuperDCA_couplings.<base>-based.all
+// (plmDCA.hpp:598,672-681).
+int run_scan(const Options &o)
+{
+    std::ostream &log = std::cout;
+    if (!o.filterlist.empty() || !o.samplelist.empty())
+        die("--filterlistfile and --samplelistfile are not supported with two alignments");
+    Fasta fa[2] = {read_fasta(o.alignments[0]), read_fasta(o.alignments[1])};
+    const int B = (int)fa[0].seqs.size();
+    if ((int)fa[1].seqs.size() != B) die("the two alignments hold different numbers of sequences");
+    std::vector<uint8_t> st[2];
+    std::vector<int32_t> tr[2];
+    int32_t Ls[2] = {0, 0};
+    for (int a = 0; a < 2; ++a) {
+        const int L_in = (int)fa[a].seqs[0].size();
+        std::vector<char> chars((size_t)B * L_in);
+        for (int b = 0; b < B; ++b) std::memcpy(chars.data() + (size_t)b * L_in, fa[a].seqs[b].data(), L_in);
+        st[a].resize((size_t)B * L_in);
+        tr[a].resize(L_in);
+        if (sdca_prepare_alignment(chars.data(), B, L_in, o.maf_threshold, o.gap_threshold, o.no_filter, o.q_env,
+                                   st[a].data(), tr[a].data(), &Ls[a]) != SDCA_OK)
+            die("alignment filtering failed");
+        st[a].resize((size_t)B * Ls[a]);
+        tr[a].resize(Ls[a]);
+        if (o.verbose)
+            log << "SuperDCA: alignment " << id_string(o.alignments[a]) << ": " << B << " sequences, " << Ls[a]
+                << " of " << L_in << " loci pass the filter\n";
+    }
+    const int L1 = Ls[0], L2 = Ls[1];
+    if (L1 < 1 || L2 < 2) {
+        if (o.verbose) log << "plmDCA: there are not enough loci for plmDCA analysis\n";
+        return EXIT_FAILURE;
+    }
+    std::vector<int32_t> loci;
+    if (!o.locilist.empty()) {
+        for (long v : read_index_list(o.locilist, o.input_base)) {
+            if (v < 0 || v >= L1) die("loci list index out of range");
+            loci.push_back((int32_t)v);
+        }
+    } else {
+        const long end = o.end_locus < 0 ? L1 : std::min<long>(L1, o.end_locus + 1);
+        for (long k = o.begin_locus; k < end; ++k) loci.push_back((int32_t)k);
+    }
+    if (loci.empty()) {
+        if (o.verbose) log << "plmDCA: there are not enough loci for plmDCA analysis\n";
+        return EXIT_FAILURE;
+    }
+    const sdca_opts so = solver_opts(o);
+    std::vector<double> w(B, 1.0);
+    if (!o.no_reweighting) {  // the second alignment's weights
+        sdca_ctx *c2 = nullptr;
+        if (sdca_ctx_create(&c2, st[1].data(), B, L2, o.q_env, o.q_env - 1, &so) != SDCA_OK)
+            die(std::string("could not set up the GPU solve: ") + sdca_last_error());
+        const int rc = sdca_reweight(c2, o.reweighting_threshold, nullptr, w.data(), nullptr);
+        sdca_ctx_destroy(c2);
+        if (rc != SDCA_OK) die(std::string("reweighting failed: ") + sdca_last_error());
+    }
+    const std::string id = id_string(o.alignments[0]);
+    if (o.output_weights) {
+        FILE *wf = fopen(unique_name(id + "_scan.weights").c_str(), "w");
+        if (!wf) die("cannot write weights");
+        for (double v : w) fprintf(wf, "%.8e\n", v);
+        fclose(wf);
+    }
+    if (o.no_dca) return EXIT_SUCCESS;
+    // the first alignment's loci, then the second's: loci [0, L1) are fitted on [L1, L1 + L2)
+    const int L = L1 + L2;
+    std::vector<uint8_t> both((size_t)B * L);
+    for (int b = 0; b < B; ++b) {
+        std::memcpy(both.data() + (size_t)b * L, st[0].data() + (size_t)b * L1, L1);
+        std::memcpy(both.data() + (size_t)b * L + L1, st[1].data() + (size_t)b * L2, L2);
+    }
+    sdca_ctx *ctx = nullptr;
+    if (sdca_ctx_create(&ctx, both.data(), B, L, o.q_env, o.q_env - 1, &so) != SDCA_OK)
+        die(std::string("could not set up the GPU solve: ") + sdca_last_error());
+    double lam[3];
+    if (sdca_set_weights(ctx, w.data(), o.lambda_h, o.lambda_J, lam) != SDCA_OK ||
+        sdca_set_scan_split(ctx, L1) != SDCA_OK)
+        die(std::string("scan set-up failed: ") + sdca_last_error());
+    if (o.verbose)
+        log << "plmDCA: scan of " << loci.size() << " loci against " << L2 << " loci, n(effective)=" << lam[0]
+            << "\n";
+    const int K = (int)loci.size();
+    std::vector<float> rows((size_t)K * L);
+    std::vector<sdca_locus_stats> stats(K);
+    const int rc = sdca_solve_loci(ctx, loci.data(), K, nullptr, rows.data(), nullptr, stats.data());
+    if (rc != SDCA_OK && rc != SDCA_E_NOCONV) die(std::string("solve failed: ") + sdca_last_error());
+    if (rc == SDCA_E_NOCONV) std::cerr << "plmDCA warning: " << sdca_last_error() << "\n";
+    sdca_ctx_destroy(ctx);
+    if (!o.no_coupling_output) {
+        std::vector<float> S((size_t)K * L2);
+        std::vector<int32_t> t1(K);
+        for (int k = 0; k < K; ++k) {
+            t1[k] = tr[0][loci[k]];
+            std::memcpy(S.data() + (size_t)k * L2, rows.data() + (size_t)k * L + L1, (size_t)L2 * 4);
+        }
+        const std::string out =
+            unique_name(id + "_scan.SuperDCA_couplings." + std::to_string(o.output_base) + "-based.all");
+        if (o.verbose) log << "\nplmDCA: writing coupling values to file \"" << out << "\"\n";
+        if (sdca_write_scan_couplings(out.c_str(), S.data(), K, L2, t1.data(), tr[1].data(), o.output_base) !=
+            SDCA_OK)
+            die("could not write \"" + out + "\"");
+    }
+    return EXIT_SUCCESS;
+}
+
 int main(int argc, char **argv)
 {
     const auto t_start = std::chrono::steady_clock::now();
     Options o = parse_args(argc, argv);
+    if (o.alignments.size() == 2) return run_scan(o);
     std::ostream &log = std::cout;
     const std::string &path = o.alignments[0];
     if (o.verbose) log << "SuperDCA: get alignment from file \"" << path << "\"\n";
@@ -325,15 +448,7 @@ int main(int argc, char **argv)
         return EXIT_FAILURE;
     }
 
-    sdca_opts so;
-    sdca_default_opts(&so);
-    so.device = o.device;
-    so.precision = o.precision;
-    so.batch_loci = o.batch_loci;
-    so.history = o.history;
-    so.max_iter = o.max_iter;
-    so.grad_threshold = o.gradient_threshold;
-    so.verbose = o.verbose;
+    sdca_opts so = solver_opts(o);
     sdca_ctx *ctx = nullptr;
     if (sdca_ctx_create(&ctx, states.data(), B, L, o.q_env, o.q_env - 1, &so) != SDCA_OK)
         die(std::string("could not set up the GPU solve: ") + sdca_last_error());

@@ -167,6 +167,41 @@ int sdca_write_couplings(const char *path, const float *S, int32_t L, const int3
     return rc;
 }
 
+// Two-alignment scan output: row k of S against every locus of the second alignment,
+// the float score as printed by the reference's fixed 8-digit stream.
+int sdca_write_scan_couplings(const char *path, const float *S, int32_t K, int32_t L2, const int32_t *translation1,
+                              const int32_t *translation2, int32_t base)
+{
+    if (!path || !S || !translation1 || !translation2 || K < 0 || L2 < 0)
+        return sdca_fail(SDCA_E_ARG, "sdca_write_scan_couplings: null argument or negative size");
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return sdca_fail(SDCA_E_IO, "cannot open \"%s\" for writing", path);
+    std::vector<char> buf(1 << 22);
+    char *p = buf.data();
+    char *const end = buf.data() + buf.size() - 128;
+    int rc = SDCA_OK;
+    for (int k = 0; k < K && rc == SDCA_OK; ++k) {
+        const uint64_t ri = (uint64_t)((int64_t)translation1[k] + base);
+        for (int n = 0; n < L2; ++n) {
+            p = put_fixed8(p, (double)S[(size_t)k * L2 + n]);
+            *p++ = ' ';
+            p = put_uint(p, ri);
+            *p++ = ' ';
+            p = put_uint(p, (uint64_t)((int64_t)translation2[n] + base));
+            *p++ = '\n';
+            if (p >= end) {
+                if (fwrite(buf.data(), 1, p - buf.data(), fp) != (size_t)(p - buf.data())) rc = SDCA_E_IO;
+                p = buf.data();
+            }
+        }
+    }
+    if (rc == SDCA_OK && p > buf.data() && fwrite(buf.data(), 1, p - buf.data(), fp) != (size_t)(p - buf.data()))
+        rc = SDCA_E_IO;
+    if (fclose(fp) != 0) rc = SDCA_E_IO;
+    if (rc != SDCA_OK) sdca_fail(rc, "writing \"%s\" failed", path);
+    return rc;
+}
+
 // The highest symmetrised scores only: the pairs of sdca_write_couplings with a value of
 // at least min_score, the top_n largest of them (all if top_n == 0), largest first, equal
 // values in the order of the full file.  One pass over the pairs with a bounded min-heap,

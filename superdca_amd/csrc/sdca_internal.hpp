@@ -195,6 +195,7 @@ struct sdca_ctx {
     int64_t Kx = 0;   // observed-state columns
     int64_t Kxp = 0;  // Kx + 1 padded to kTile
     std::vector<int32_t> col_off;    // L + 1
+    int32_t scan_split = 0;          // > 0: two-alignment scan, loci [0, scan_split) are the first alignment's
     std::vector<int32_t> state_col;  // L x q: column of (j, a) or -1
     std::vector<uint8_t> h_states;   // B x L copy (host)
     hipStream_t stream = nullptr;

@@ -412,6 +412,7 @@ int sdca_solve_loci_mlp(sdca_ctx *c, const int32_t *loci, int32_t K, int32_t npa
     if (rc || (rc = need_mlp(c))) return rc;
     if (warm_mode < 0 || warm_mode > 3) return sdca_fail(SDCA_E_ARG, "unknown warm_mode %d", warm_mode);
     if (!c->have_weights) return sdca_fail(SDCA_E_STATE, "sdca_set_weights must be called first");
+    if (c->scan_split > 0) return sdca_fail(SDCA_E_STATE, "a two-alignment scan solves from a cold start only");
     if (K < 0 || (K > 0 && !loci)) return sdca_fail(SDCA_E_ARG, "bad arguments");
     if (npartners < 1 || npartners > c->L - 1) return sdca_fail(SDCA_E_ARG, "npartners must be in [1, L-1]");
     for (int k = 0; k < K; ++k)

@@ -1,0 +1,92 @@
+"""Two-alignment scan (SuperDCA.cpp:155-159; objective plmDCA_objective.hpp:67, output
+plmDCA.hpp:672-681) through the C ABI: a locus of the first alignment fitted on the second
+alignment's columns alone is the ordinary per-locus problem of the alignment made of that
+locus followed by the second alignment, which the FP64 oracle solves here."""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as orc
+import superdca_amd as sd
+from superdca_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+L1 = 20
+
+
+@pytest.fixture(scope="module")
+def scan_case():
+    d = synth.generate(B=200, N=90, clades=8, pairs=10, seed=505)
+    st, _ = sd.prepare_alignment(d["chars"])
+    st2 = np.ascontiguousarray(st[:, L1:])
+    w2, _ = orc.weights(st2, 0.9)  # the second alignment's weights (plmDCA.hpp:538)
+    _, lh, lJ = orc.lambdas(w2)
+    return st, st2, w2, lh, lJ
+
+
+def test_scan_matches_the_oracle_on_the_reduced_alignment(scan_case):
+    st, st2, w2, lh, lJ = scan_case
+    loci = np.array([0, 7, 19], dtype=np.int32)
+    with sd.Context(st, grad_threshold=1e-6, batch_loci=3) as ctx:
+        ctx.set_weights(w2)
+        ctx.set_scan_split(L1)
+        rows, stats, _, rc = ctx.solve(loci)
+    assert rc in (sd.SDCA_OK, sd.SDCA_E_NOCONV)
+    assert np.all(stats["gnorm"] < 1e-4)
+    for k, r in enumerate(loci):
+        red = np.ascontiguousarray(np.concatenate([st[:, r:r + 1], st2], axis=1))
+        ref, _, so = orc.solve_loci(red, w2, lh, lJ, np.array([0], dtype=np.int32), gtol=1e-9, max_iter=10000)
+        assert stats["fval"][k] == pytest.approx(so["fval"][0], rel=1e-6)
+        assert np.all(rows[k, :L1] == 0.0)  # the first alignment is no predictor
+        got, want = rows[k, L1:], ref[0, 1:]
+        big = want > np.quantile(want, 0.9)
+        rel = np.abs(got - want)[big] / want[big]
+        print(f"scan locus {r}: f {stats['fval'][k]:.8f} vs {so['fval'][0]:.8f}, top-decile rel err {rel.max():.2e}")
+        assert rel.max() < 1e-3
+
+
+def test_scan_refusals(scan_case):
+    st, _, w2, _, _ = scan_case
+    with sd.Context(st, batch_loci=2) as ctx:
+        ctx.set_weights(w2)
+        with pytest.raises(sd.SdcaError):
+            ctx.set_scan_split(st.shape[1])
+        ctx.set_scan_split(L1)
+        with pytest.raises(sd.SdcaError, match="not in the first alignment"):
+            ctx.solve(np.array([L1], dtype=np.int32))
+        with pytest.raises(sd.SdcaError, match="scan"):
+            ctx.eval_batch([0], np.zeros((1, sd.nparams(st.shape[1], 4))))
+        with pytest.raises(sd.SdcaError, match="cold start"):
+            ctx.solve(np.array([0], dtype=np.int32), warm=np.zeros((1, sd.nparams(st.shape[1], 4))))
+        ctx.set_scan_split(0)  # off again: the ordinary solve
+        _, stats, _, rc = ctx.solve(np.array([L1], dtype=np.int32))
+        assert rc == sd.SDCA_OK and stats["status"][0] == sd.ST_CONVERGED
+
+
+def test_cli_scan_end_to_end(tmp_path):
+    # sdca with two alignment files: the same numbers as the library driven directly
+    d = synth.generate(B=120, N=50, clades=6, pairs=5, seed=606)
+    a, b = d["chars"][:, :15], d["chars"][:, 15:]
+    for name, part in (("first.fa", a), ("second.fa", b)):
+        synth.write_fasta(str(tmp_path / name), part)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "superdca_amd", "sdca")
+    p = subprocess.run([exe, "first.fa", "second.fa"], capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    assert p.returncode == 0, p.stderr
+    lines = (tmp_path / "first_scan.SuperDCA_couplings.1-based.all").read_text().splitlines()
+    st1, tr1 = sd.prepare_alignment(a)
+    st2, tr2 = sd.prepare_alignment(b)
+    L1, L2 = st1.shape[1], st2.shape[1]
+    assert len(lines) == L1 * L2
+    with sd.Context(st2) as ctx:
+        w2, _ = ctx.reweight(0.9)
+    with sd.Context(np.ascontiguousarray(np.concatenate([st1, st2], axis=1))) as ctx:
+        ctx.set_weights(w2)
+        ctx.set_scan_split(L1)
+        rows, _, _, _ = ctx.solve(np.arange(L1, dtype=np.int32))
+    sd.write_scan_couplings(str(tmp_path / "direct.all"), np.ascontiguousarray(rows[:, L1:]), tr1, tr2, 1)
+    assert (tmp_path / "direct.all").read_text().splitlines() == lines

@@ -161,6 +161,21 @@ def test_filter_reads_either_case_and_treats_other_letters_as_gaps():
     assert st[9, 2] == 3 and st[:, 2].tolist()[:7] == [0] * 7  # N is the gap state (q - 1)
 
 
+def test_scan_writer_lines(tmp_path):
+    # plmDCA.hpp:672-681: every first-alignment locus against every second-alignment locus,
+    # the float score printed with 8 fixed digits, no symmetrisation
+    rng = np.random.default_rng(12)
+    K, L2 = 7, 33
+    S = (rng.random((K, L2)) ** 2 * 3).astype(np.float32)
+    t1 = np.sort(rng.choice(1000, K, replace=False)).astype(np.int32)
+    t2 = np.sort(rng.choice(5000, L2, replace=False)).astype(np.int32)
+    sd.write_scan_couplings(str(tmp_path / "s.all"), S, t1, t2, 1)
+    lines = (tmp_path / "s.all").read_text().splitlines()
+    assert lines == ["%.8f %d %d" % (float(S[k, n]), t1[k] + 1, t2[n] + 1) for k in range(K) for n in range(L2)]
+    with pytest.raises(ValueError):
+        sd.write_scan_couplings(str(tmp_path / "x.all"), S, t1[:3], t2, 1)
+
+
 def test_uneven_batch_shapes():
     cfgs = synth.c3_workloads()
     assert [c["seed"] for c in cfgs] == list(range(3000, 3048))
@@ -340,6 +355,22 @@ int main(void)
     return 0;
 }
 """
+
+
+def test_cli_two_alignment_argument_checks(tmp_path):
+    exe = _cli()
+    a = tmp_path / "a.fa"
+    b = tmp_path / "b.fa"
+    a.write_text(">s0\nACGT\n>s1\nAGGT\n")
+    b.write_text(">s0\nACGT\n")
+    p = subprocess.run([exe, str(a), str(b), str(a)], capture_output=True, text=True)
+    assert p.returncode != 0 and "more than 2 alignments" in p.stderr
+    p = subprocess.run([exe, str(a), str(b)], capture_output=True, text=True)
+    assert p.returncode != 0 and "different numbers of sequences" in p.stderr
+    (tmp_path / "cols.txt").write_text("1\n")
+    p = subprocess.run([exe, "--filterlistfile", str(tmp_path / "cols.txt"), str(a), str(a)], capture_output=True,
+                       text=True)
+    assert p.returncode != 0 and "not supported with two alignments" in p.stderr
 
 
 def test_plain_c_program_uses_the_abi(tmp_path):
