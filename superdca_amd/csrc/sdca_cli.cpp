@@ -249,8 +249,10 @@ sdca_opts solver_opts(const Options &o)
 int run_scan(const Options &o)
 {
     std::ostream &log = std::cout;
-    if (!o.filterlist.empty() || !o.samplelist.empty())
-        die("--filterlistfile and --samplelistfile are not supported with two alignments");
+    if (!o.filterlist.empty() || !o.samplelist.empty() || o.out_filtered || o.out_filterlist || o.out_samplelist ||
+        o.output_top > 0 || o.output_min_score > -HUGE_VAL)
+        die("--filterlistfile, --samplelistfile, the --output-*-alignment options and the top-score output are "
+            "not supported with two alignments");
     Fasta fa[2] = {read_fasta(o.alignments[0]), read_fasta(o.alignments[1])};
     const int B = (int)fa[0].seqs.size();
     if ((int)fa[1].seqs.size() != B) die("the two alignments hold different numbers of sequences");

@@ -371,6 +371,8 @@ def test_cli_two_alignment_argument_checks(tmp_path):
     p = subprocess.run([exe, "--filterlistfile", str(tmp_path / "cols.txt"), str(a), str(a)], capture_output=True,
                        text=True)
     assert p.returncode != 0 and "not supported with two alignments" in p.stderr
+    p = subprocess.run([exe, "--output-filtered-alignment", str(a), str(a)], capture_output=True, text=True)
+    assert p.returncode != 0 and "not supported with two alignments" in p.stderr
 
 
 def test_plain_c_program_uses_the_abi(tmp_path):
