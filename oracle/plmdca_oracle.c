@@ -581,7 +581,7 @@ int32_t orc_lbfgs(orc_objfn fn, void *ctx, int64_t n, double *x, int32_t m, doub
         ms.stpmin = 1e-20;
         ms.stpmax = 1e20;
         mt_start(&ms, f, dg, stp);
-        int ls = 0;
+        int ls = 0, approx = 0;
         for (int fe = 0; fe < 20; ++fe) {
             for (int64_t e = 0; e < n; ++e) x[e] = xo[e] + stp * d[e];
             f = fn(ctx, x, g);
@@ -590,14 +590,25 @@ int32_t orc_lbfgs(orc_objfn fn, void *ctx, int64_t n, double *x, int32_t m, doub
                 ls = 3;
                 break;
             }
-            ls = mt_step(&ms, f, dotv(g, d, n), &stp);
+            const double dgt = dotv(g, d, n);
+            /* Approximate Wolfe conditions (W. W. Hager and H. Zhang, SIAM J. Optim. 16
+             * (2005) 170-192, eq. (23)): near the optimum a step lowers f by less than the
+             * rounding of its FP64 sum over sequences, while the directional derivative is
+             * still exact to many digits; the step is then accepted on the derivative, with
+             * f allowed to exceed f(0) by that rounding level. */
+            if (dgt >= ms.gtol * dg && dgt <= (2.0 * ms.ftol - 1.0) * dg && f <= fo + 1e-12 * fabs(fo)) {
+                ls = 1;
+                approx = 1;
+                break;
+            }
+            ls = mt_step(&ms, f, dgt, &stp);
             if (ls != 0) break;
         }
         if (ls == 3) {
             status = 3;
             break;
         }
-        if (!(f < fo)) { /* no progress: keep the previous iterate */
+        if (!(f < fo) && !approx) { /* no progress: keep the previous iterate */
             memcpy(x, xo, n * sizeof(double));
             memcpy(g, go, n * sizeof(double));
             f = fo;

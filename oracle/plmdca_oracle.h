@@ -81,7 +81,9 @@ typedef double (*orc_objfn)(void *ctx, const double *x, double *g);
 
 /* FP64 L-BFGS (history m, two-loop recursion) with the More-Thuente line search,
  * stopping when ||g||_inf < gtol or after max_iter iterations
- * (plmDCA.hpp:218-221,298,304).  x holds the start point on entry. */
+ * (plmDCA.hpp:218-221,298,304).  Below the rounding level of f's FP64 sum a step is
+ * accepted on the approximate Wolfe conditions of Hager and Zhang, so the solve
+ * reaches gradients far below 1e-6.  x holds the start point on entry. */
 int32_t orc_lbfgs(orc_objfn fn, void *ctx, int64_t n, double *x, int32_t m, double gtol,
                   int32_t max_iter, orc_stats *st);
 

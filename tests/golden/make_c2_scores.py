@@ -3,8 +3,7 @@ eight cores).
 
 The C2 workload (B = 1000 sequences, L = 5000 loci, q_env = 4, seed 1002 of
 superdca_amd.synth) solved for every locus by the FP64 oracle to its optimum
-(||g||_inf < 1e-7, or below 1e-5 where the line search stops at the FP64 rounding floor,
-which at C2 lies near 1e-6), scored with the reference gauge and Frobenius norm, symmetrised as the
+(||g||_inf < 1e-8), scored with the reference gauge and Frobenius norm, symmetrised as the
 coupling writer does (double(float(S(r, n) + S(n, r))) * 0.5, plmDCA.hpp:667-693), and
 reduced to what the GPU comparison needs:
 
@@ -42,11 +41,9 @@ def main() -> None:
     w, _ = orc.weights(st, 0.9, nthreads=os.cpu_count() or 8)
     _, lh, lJ = orc.lambdas(w)
     t = time.time()
-    rows, _, stats = orc.solve_loci(st, w, lh, lJ, np.arange(L), gtol=1e-7, max_iter=20000,
+    rows, _, stats = orc.solve_loci(st, w, lh, lJ, np.arange(L), gtol=1e-8, max_iter=20000,
                                     nthreads=os.cpu_count() or 8)
-    # a line search that fails at the FP64 rounding floor (about 1e-6 here) still leaves
-    # the optimum to within the GPU's own thresholds; the largest final gradient is kept
-    ok = (stats["status"] == 0) | ((stats["status"] == 2) & (stats["gnorm"] < 1e-5))
+    ok = stats["status"] == 0
     print(f"solved {L} loci in {time.time() - t:.0f} s, status counts {np.bincount(stats['status'])}, "
           f"max ||g||_inf {stats['gnorm'].max():.2e}", flush=True)
     assert np.all(ok), stats["gnorm"][~ok]
