@@ -65,7 +65,19 @@ typedef struct sdca_opts {
                                 pass) while its ||g||_inf >= coarse_factor * grad_threshold, then
                                 its iterate is evaluated again in full and it finishes at split
                                 precision (default 4; 0: full precision throughout) */
-    int32_t reserved[4];
+    int32_t fine_factor;     /* with j_threshold > 0 under split-BF16: a locus moves to the fine
+                                level (W and G as three BF16 parts, about 24 bits, i.e. FP32
+                                contractions) once ||g||_inf < fine_factor * j_threshold, or when a
+                                line search at split precision fails (default 10); 0: no fine
+                                level, the coupling threshold is tested at split precision */
+    double j_threshold;      /* > 0: a locus is converged only when, besides ||g||_inf <
+                                grad_threshold, the couplings' gradient ||g_J||_inf < j_threshold,
+                                evaluated at the highest level (the fine one unless fine_factor is
+                                0).  The scores depend on J alone; the fields' own
+                                gradient converges more slowly (the field column is the sum of every
+                                locus' one-hot columns) without moving the scores.  0 (default):
+                                SuperDCA's rule alone (plmDCA.hpp:218-221) */
+    int32_t reserved[2];
 } sdca_opts;
 
 /* How sdca_solve_loci_mlp uses the network's coupling blocks (flags). */
@@ -84,6 +96,9 @@ typedef struct sdca_locus_stats {
     int32_t iter;
     int32_t status;
     int32_t nfeval_first;  /* evaluations up to the first accepted step (start point included) */
+    double gnorm_J;        /* ||g||_inf over the coupling entries J_r (the fields h_r excluded) */
+    int32_t nfeval_fine;   /* evaluations at the fine level (FP32-equivalent contractions) */
+    int32_t reserved;
 } sdca_locus_stats;
 
 /* Cumulative device timings of a context (milliseconds, HIP events). */
@@ -141,7 +156,8 @@ int sdca_set_weights(sdca_ctx *ctx, const double *w, double lambda_h_user, doubl
 /* Parity entry point for plmDCA_objective_fval_and_gradient (plmDCA_objective.hpp:38-249,
  * called through LBFGS_interface_CppNumericalSolvers.hpp:111-117): for each of the K loci
  * loci[k], evaluates f and g at x (K x n, reference layout: n = L*q*q + q, J_r[j](a,s)
- * at 16-locus blocks, Matrix_kernel_access_order.hpp:28-34, h_r at L*q*q).  Rows of
+ * at 16-locus blocks, Matrix_kernel_access_order.hpp:28-34, h_r at L*q*q), at the fine
+ * level when opts.j_threshold > 0, else at the context's precision.  Rows of
  * states never observed at j contribute only their L2 term, which is added on the host
  * in FP64.  g (K x n) is overwritten; the block of locus r itself stays 0. */
 int sdca_eval_batch(sdca_ctx *ctx, const int32_t *loci, int32_t K, const double *x, double *f_out,

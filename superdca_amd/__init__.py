@@ -60,12 +60,14 @@ class Opts(C.Structure):
                 ("grad_threshold", C.c_double), ("verbose", C.c_int32), ("autotune", C.c_int32),
                 ("gemm_ksplit", C.c_int32), ("gemm_gm_energy", C.c_int32), ("gemm_gm_grad", C.c_int32),
                 ("gemm_tn_energy", C.c_int32), ("gemm_tn_grad", C.c_int32), ("pipeline_groups", C.c_int32),
-                ("coarse_factor", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("coarse_factor", C.c_int32), ("fine_factor", C.c_int32), ("j_threshold", C.c_double),
+                ("reserved", C.c_int32 * 2)]
 
 
 class LocusStats(C.Structure):
     _fields_ = [("fval", C.c_double), ("gnorm", C.c_double), ("nfeval", C.c_int32), ("iter", C.c_int32),
-                ("status", C.c_int32), ("nfeval_first", C.c_int32)]
+                ("status", C.c_int32), ("nfeval_first", C.c_int32), ("gnorm_J", C.c_double),
+                ("nfeval_fine", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Timing(C.Structure):
@@ -251,6 +253,8 @@ class Context:
             "iter": np.array([st[k].iter for k in range(K)], dtype=np.int64),
             "status": np.array([st[k].status for k in range(K)], dtype=np.int64),
             "nfeval_first": np.array([st[k].nfeval_first for k in range(K)], dtype=np.int64),
+            "gnorm_J": np.array([st[k].gnorm_J for k in range(K)]),
+            "nfeval_fine": np.array([st[k].nfeval_fine for k in range(K)], dtype=np.int64),
         }
         return S, stats, X, rc
 
@@ -334,7 +338,8 @@ class Context:
         rc = _check(lib().sdca_solve_loci_mlp(self._h, _ptr(loci), K, int(npartners), int(mode), _ptr(S), _ptr(X),
                                               C.cast(st, C.c_void_p)), allow=(SDCA_E_NOCONV,))
         stats = {k: np.array([getattr(st[i], k) for i in range(K)]) for k in ("fval", "gnorm", "nfeval", "iter",
-                                                                               "status", "nfeval_first")}
+                                                                               "status", "nfeval_first", "gnorm_J",
+                                                                               "nfeval_fine")}
         return S, stats, X, rc
 
     def timing(self, reset: bool = False) -> dict:

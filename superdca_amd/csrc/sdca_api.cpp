@@ -115,9 +115,12 @@ void sdca_ctx::free_workspace()
     dfree(d_X0);
     dfree(d_Whi);
     dfree(d_Wlo);
+    dfree(d_Wl3);
     dfree(d_G32);
     dfree(d_Ghi);
     dfree(d_Glo);
+    dfree(d_Gl3);
+    dfree(d_hpart);
     dfree(d_Epart);
     dfree(d_upd_part);
     dfree(d_dot_part);
@@ -307,7 +310,10 @@ int autotune(sdca_ctx *c, int nslots, const std::vector<GemmConfig> &cand, GemmC
         cmds[i].pnew = -1;
         cmds[i].slot = i;
         cmds[i].row = i * c->q;
+        cmds[i].level = sdca::LV_SPLIT;
     }
+    // every command at the split level: the fine part is empty, the coarse part starts at Mg
+    const sdca::Parts parts{{0, 0, c->Mg}, {0, 0, nslots, nslots}};
     HIP_TRY(hipMemcpyAsync(c->d_cmds, cmds.data(), nslots * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     const size_t nw = (size_t)c->Mpad * c->Kxp;
     HIP_TRY(hipMemsetD32Async((hipDeviceptr_t)c->d_Xtr, 0x3c23d70au, nw, st));  // 0.01f
@@ -317,15 +323,16 @@ int autotune(sdca_ctx *c, int nslots, const std::vector<GemmConfig> &cand, GemmC
     }
     const int reps = 3;
     auto energy = [&](const GemmConfig &g) {
-        sdca::kern::gemm_energy(st, c->opts.precision, c->Mg, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_X, c->Mg, c->Mg, c->Bp,
-                                (int)c->Kxp, g.ksplit, g.kchunk, g.gm_e, g.tn_e, c->d_Epart);
+        sdca::kern::gemm_energy(st, c->opts.precision, parts, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_Wl3, c->d_X, c->Mg,
+                                c->Mg, c->Bp, (int)c->Kxp, g.ksplit, g.kchunk, g.gm_e, g.tn_e, c->d_Epart);
         sdca::kern::softmax(st, c->opts.precision, c->d_Epart, g.ksplit, c->Mg, c->Bp, c->B, c->q, nslots,
-                            c->d_cmds, c->d_statesT, c->d_w, c->d_G32, c->d_Ghi, c->d_Glo, c->d_sm_part, c->nbchunk);
+                            c->d_cmds, c->d_statesT, c->d_w, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Gl3, c->d_sm_part,
+                            c->d_hpart, c->nbchunk);
     };
     auto grad = [&](int gm, int tn) {
-        sdca::kern::gemm_grad(st, c->opts.precision, c->Mg, nslots, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Xt, c->Mg, c->Bp,
-                              (int)c->Kxp, (int)c->Kx, c->q, nslots, c->d_cmds, c->d_Xtr, c->d_gbuf, c->n_slot,
-                              (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), gm, tn);
+        sdca::kern::gemm_grad(st, c->opts.precision, parts, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Gl3, c->d_Xt, c->Mg,
+                              c->Bp, (int)c->Kxp, (int)c->Kx, c->q, c->d_cmds, c->d_Xtr, c->d_gbuf, c->n_slot,
+                              2.0 * c->lambda_J, 2.0 * c->lambda_h, c->d_hpart, c->nbchunk, gm, tn);
     };
     auto time = [&](auto &&launch) -> float {
         launch();
@@ -405,7 +412,7 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     c->gslots = nslots / ng;
     c->Mg = (int)round_up((int64_t)c->gslots * c->q, sdca::kTile);
     c->Mpad = ng * c->Mg;
-    c->Mgp = c->Mg + sdca::kTile;
+    c->Mgp = c->Mg + 2 * sdca::kTile;
     const size_t prow = (size_t)ng * c->Mgp;  // packed rows of all groups
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
     int rc;
@@ -422,6 +429,11 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
         if ((rc = dalloc(&c->d_Wlo, prow * c->Kxp))) return rc;
         if ((rc = dalloc(&c->d_Ghi, prow * c->Bp))) return rc;
         if ((rc = dalloc(&c->d_Glo, prow * c->Bp))) return rc;
+        if ((rc = dalloc(&c->d_Gl3, prow * c->Bp))) return rc;
+        // the third parts are read only for fine commands: allocated when a coupling
+        // threshold can bring a slot to the fine level
+        if (c->opts.j_threshold > 0.0 && c->opts.fine_factor > 0 && (rc = dalloc(&c->d_Wl3, prow * c->Kxp)))
+            return rc;
     } else {
         if ((rc = dalloc(&c->d_G32, prow * c->Bp))) return rc;
     }
@@ -453,6 +465,7 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     // 3.15 to 3.03 ms per round against 64; 32 was slower)
     c->nchunk = (int)std::min<int64_t>(128, std::max<int64_t>(1, (c->n_slot / 4 + 255) / 256));
     c->nbchunk = (c->Bp + 255) / 256;
+    if ((rc = dalloc(&c->d_hpart, (size_t)nslots * c->nbchunk * c->q))) return rc;
     if ((rc = dalloc(&c->d_upd_part, (size_t)nslots * c->nchunk * sdca::ResLayout{m}.nupd()))) return rc;
     if ((rc = dalloc(&c->d_dot_part, (size_t)nslots * c->nchunk * sdca::ResLayout{m}.ndots()))) return rc;
     if ((rc = dalloc(&c->d_sm_part, (size_t)nslots * c->nbchunk))) return rc;
@@ -502,28 +515,31 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     const int m = c->ws_m, W = sdca::ResLayout{m}.width();
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
     const int ns = c->gslots, s0 = g * ns;
-    // command list: the active slots, full-precision ones first, each part in slot order;
-    // the coarse part starts on a tile boundary of the packed rows
+    // command list: the active slots by level, finest first, each part in slot order and
+    // starting on a tile boundary of the packed rows
     std::vector<int> &lst = c->gcmd_slot[g];
     lst.clear();
     sdca::SlotCmd *hc = c->h_cmds + s0;
-    int ncoarse = 0, row = 0;
-    for (int part = 0; part <= 1; ++part) {
-        if (part == 1) row = (int)round_up(row, sdca::kTile);
+    sdca::Parts parts;
+    int row = 0;
+    for (int part = 0; part < 3; ++part) {
+        const int level = sdca::LV_FINE - part;
+        row = (int)round_up(row, sdca::kTile);
+        parts.row[part] = row;
+        parts.cmd[part] = (int)lst.size();
         for (int i = 0; i < ns; ++i) {
-            if (cmds[i].mode == sdca::MODE_IDLE || (cmds[i].coarse != 0) != (part == 1)) continue;
+            if (cmds[i].mode == sdca::MODE_IDLE || cmds[i].level != level) continue;
             hc[lst.size()] = cmds[i];
             hc[lst.size()].slot = i;
             hc[lst.size()].row = row;
             row += c->q;
             lst.push_back(i);
-            ncoarse += part;
         }
     }
-    const int nc = (int)lst.size(), nfull = nc - ncoarse;
+    const int nc = (int)lst.size();
+    parts.cmd[3] = nc;
     if (nc == 0) return SDCA_OK;
-    const int Mc = (int)round_up((int64_t)ncoarse * c->q, sdca::kTile);  // packed rows of each part
-    const int Mf = (int)round_up((int64_t)nfull * c->q, sdca::kTile);
+    const int Mrows = (int)round_up(row, sdca::kTile);  // packed rows of the three parts
     const size_t vo = (size_t)s0 * c->n_slot;              // slot-vector offset of the group
     const size_t ro = (size_t)g * c->Mgp * c->Kxp;         // its packed trial-point rows
     const size_t go = (size_t)g * c->Mgp * c->Bp;          // its packed node-belief rows
@@ -540,13 +556,13 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     double *upd_part = c->d_upd_part + (size_t)s0 * c->nchunk * sdca::ResLayout{m}.nupd();
     sdca::kern::update(st, m, c->q, split, dc, nc, gbuf, dbuf, Sp, Yp, c->d_Xc + vo, c->d_Xtr + ro,
                        split ? c->d_Whi + ro : nullptr, split ? c->d_Wlo + ro : nullptr,
-                       c->d_X0 ? c->d_X0 + vo : nullptr, c->n_slot, (int)c->Kxp, (int)c->Kx, upd_part, c->nchunk);
+                       split ? c->d_Wl3 + ro : nullptr, c->d_X0 ? c->d_X0 + vo : nullptr, c->n_slot, (int)c->Kxp,
+                       (int)c->Kx, upd_part, c->nchunk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[1], st));
     // A round with few active rows (the end of a solve) takes proportionally more split-K
     // slices, packed Mrows apart in the group's partial-energy buffer, so that its energy
     // launch still fills the GPU.
-    const int Mrows = Mc + Mf;
     int ksplit = c->ksplit, kchunk = c->kchunk, Mz = c->Mgp;
     if (2 * Mrows <= c->Mg) {
         GemmConfig gs;
@@ -558,26 +574,29 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
             Mz = Mrows;
         }
     }
-    // full rows [0, Mf): hi + lo (or FP32); coarse rows: one MFMA pass over the BF16 high parts
-    sdca::kern::gemm_energy(st, c->opts.precision, Mf, c->d_Xtr + ro, split ? c->d_Whi + ro : nullptr,
-                            split ? c->d_Wlo + ro : nullptr, c->d_X, Mrows, Mz, c->Bp, (int)c->Kxp, ksplit, kchunk,
-                            c->gm_energy, c->tn_energy, Epart);
+    // fine rows: hi + mid + lo, split rows: hi + lo, coarse rows: the high parts (or FP32 throughout)
+    sdca::kern::gemm_energy(st, c->opts.precision, parts, c->d_Xtr + ro, split ? c->d_Whi + ro : nullptr,
+                            split ? c->d_Wlo + ro : nullptr, split ? c->d_Wl3 + ro : nullptr, c->d_X, Mrows, Mz,
+                            c->Bp, (int)c->Kxp, ksplit, kchunk, c->gm_energy, c->tn_energy, Epart);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[2], st));
+    double *hpart = c->d_hpart + (size_t)s0 * c->nbchunk * c->q;
     sdca::kern::softmax(st, c->opts.precision, Epart, ksplit, Mz, c->Bp, c->B, c->q, nc, dc, c->d_statesT,
                         c->d_w, split ? nullptr : c->d_G32 + go, split ? c->d_Ghi + go : nullptr,
-                        split ? c->d_Glo + go : nullptr, c->d_sm_part + (size_t)s0 * c->nbchunk, c->nbchunk);
+                        split ? c->d_Glo + go : nullptr, split ? c->d_Gl3 + go : nullptr,
+                        c->d_sm_part + (size_t)s0 * c->nbchunk, hpart, c->nbchunk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[3], st));
-    sdca::kern::gemm_grad(st, c->opts.precision, Mf, nfull, split ? nullptr : c->d_G32 + go,
-                          split ? c->d_Ghi + go : nullptr, split ? c->d_Glo + go : nullptr, c->d_Xt, Mrows, c->Bp,
-                          (int)c->Kxp, (int)c->Kx, c->q, nc, dc, c->d_Xtr + ro, gbuf, c->n_slot,
-                          (float)(2.0 * c->lambda_J), (float)(2.0 * c->lambda_h), c->gm_grad, c->tn_grad);
-    HIP_TRY(hipGetLastError());
+    sdca::kern::gemm_grad(st, c->opts.precision, parts, split ? nullptr : c->d_G32 + go,
+                          split ? c->d_Ghi + go : nullptr, split ? c->d_Glo + go : nullptr,
+                          split ? c->d_Gl3 + go : nullptr, c->d_Xt, Mrows, c->Bp, (int)c->Kxp, (int)c->Kx, c->q, dc,
+                          c->d_Xtr + ro, gbuf, c->n_slot, 2.0 * c->lambda_J, 2.0 * c->lambda_h, hpart, c->nbchunk,
+                          c->gm_grad, c->tn_grad);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[4], st));
     double *dot_part = c->d_dot_part + (size_t)s0 * c->nchunk * sdca::ResLayout{m}.ndots();
-    sdca::kern::dots(st, m, split, dc, nc, gbuf, dbuf, Sp, Yp, c->n_slot, dot_part, c->nchunk);
+    sdca::kern::dots(st, m, split, dc, nc, gbuf, dbuf, Sp, Yp, c->n_slot, (int)c->Kxp, (int)c->Kx, dot_part,
+                     c->nchunk);
     HIP_TRY(hipGetLastError());
     sdca::kern::reduce_results(st, m, nc, upd_part, dot_part,
                                c->d_sm_part + (size_t)s0 * c->nbchunk, c->nchunk, c->nbchunk,
@@ -641,7 +660,7 @@ int finish_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
                 bu += (k.cs[p] != 0.f ? pb : 0.0) + (k.cy[p] != 0.f ? pb : 0.0);
             }
         }
-        if (split) bu += k.coarse ? 2.0 : 4.0;  // BF16 copies of x_t: hi (+ lo)
+        if (split) bu += 2.0 * (k.level + 1);  // BF16 parts of x_t: hi (+ lo (+ l3))
         bd += (md == sdca::MODE_INIT ? 4.0 : 8.0 + pb) + 2.0 * pb * npairs;  // g_t (, g_k, d) and the pairs
     }
     t.bytes_update += bu * (double)c->n_slot;
@@ -698,12 +717,13 @@ sdca::SlotCmd idle_cmd()
     std::memset(&k, 0, sizeof(k));
     k.mode = sdca::MODE_IDLE;
     k.locus = -1;
+    k.level = sdca::LV_SPLIT;
     return k;
 }
 
 // ------------------------------------------------------- per-slot solver state
 
-// PH_REBASE: the iterate is evaluated again at full precision after the coarse phase.
+// PH_REBASE: the iterate is evaluated again at the next precision level.
 enum Phase { PH_IDLE, PH_INIT, PH_LS, PH_REBASE, PH_DONE };
 
 // Host side of one slot: line-search state and the inner products of the stored vectors
@@ -728,9 +748,11 @@ struct Slot {
     double gnorm = 0.0;
     bool from_xt = true;
     bool preloaded = false;  // the direction buffer holds a first search direction
-    bool coarse = false;     // evaluations use the BF16 high parts only (coarse phase)
+    int level = sdca::LV_SPLIT;  // precision level of the slot's evaluations (sdca::Level)
     int restarts = 0;        // line searches restarted from steepest descent
     int nfeval_first = 0;    // evaluations up to the first accepted step
+    int nfeval_fine = 0;     // evaluations at the fine level
+    double gnormJ = 0.0;     // ||g||_inf over the coupling entries at the accepted iterate
     double SY[sdca::kMaxHistory][sdca::kMaxHistory], YY[sdca::kMaxHistory][sdca::kMaxHistory];
     double SG[sdca::kMaxHistory], YG[sdca::kMaxHistory], DY[sdca::kMaxHistory];
     double TS[sdca::kMaxHistory], TY[sdca::kMaxHistory];
@@ -753,17 +775,39 @@ class Solver {
 public:
     Solver(sdca_ctx *c, int nslots)
         : c_(c), n_(nslots), m_(c->ws_m), slots_(nslots), cmds_(nslots, idle_cmd()),
-          f_noise_(c->opts.precision == SDCA_PREC_SPLIT_BF16 ? 1e-5 : 1e-6),
-          coarse_gnorm_(c->opts.precision == SDCA_PREC_SPLIT_BF16 && c->opts.coarse_factor > 0
-                            ? c->opts.coarse_factor * c->opts.grad_threshold
-                            : 0.0) {}
+          split_(c->opts.precision == SDCA_PREC_SPLIT_BF16),
+          coarse_gnorm_(split_ && c->opts.coarse_factor > 0 ? c->opts.coarse_factor * c->opts.grad_threshold : 0.0),
+          jthr_(c->opts.j_threshold),
+          fine_gnorm_(split_ && jthr_ > 0.0 ? c->opts.fine_factor * jthr_ : 0.0),
+          max_level_(jthr_ > 0.0 && (!split_ || c->opts.fine_factor > 0) ? sdca::LV_FINE : sdca::LV_SPLIT)
+    {
+    }
+
+    // Level a slot at `level` moves to once its ||g||_inf is ginf: coarse until
+    // coarse_gnorm_, then split, then (with a coupling threshold) fine below fine_gnorm_.
+    // FP32 contractions have no coarse or split level.
+    int target_level(int level, double ginf) const
+    {
+        if (!split_) return max_level_;
+        if (level == sdca::LV_COARSE && !(ginf >= coarse_gnorm_)) level = sdca::LV_SPLIT;
+        if (level == sdca::LV_SPLIT && max_level_ == sdca::LV_FINE && ginf < fine_gnorm_) level = sdca::LV_FINE;
+        return level;
+    }
+
+    // SuperDCA's rule (plmDCA.hpp:218-221) at full precision, plus the coupling threshold
+    // at the highest level when one is set.
+    bool converged(const Slot &S, double ginf, double gJ) const
+    {
+        if (!(ginf < c_->opts.grad_threshold) || S.level < sdca::LV_SPLIT) return false;
+        return jthr_ <= 0.0 || (S.level == max_level_ && gJ < jthr_);
+    }
 
     void assign(int i, int item, int r, bool warm, bool preloaded = false)
     {
         Slot &S = slots_[i];
         S = Slot();
         S.preloaded = preloaded;
-        S.coarse = coarse_gnorm_ > 0.0;
+        S.level = !split_ ? max_level_ : (coarse_gnorm_ > 0.0 ? sdca::LV_COARSE : sdca::LV_SPLIT);
         S.item = item;
         S.r = r;
         S.phase = PH_INIT;
@@ -805,15 +849,17 @@ public:
         S.TG = res[R.tg()];
         S.TD = res[R.td()];
         const double f = res[R.fdata()] + c_->lambda_J * res[R.xsqJ()] + c_->lambda_h * res[R.xsqh()];
-        const double ginf = res[R.ginf()];
+        const double ginf = res[R.ginf()], gJ = res[R.gJinf()];
         S.nfeval++;
+        if (k.level == sdca::LV_FINE) S.nfeval_fine++;
         if (!std::isfinite(f) || !std::isfinite(ginf)) return finish(i, SDCA_ST_NONFINITE, S.phase == PH_INIT);
-        if (S.phase == PH_INIT && S.coarse && ginf < coarse_gnorm_) {
-            S.coarse = false;  // near the end already: evaluate the start point again in full
+        if (S.phase == PH_INIT && target_level(S.level, ginf) > S.level) {
+            // near the end already: evaluate the start point again at the higher level
+            S.level = target_level(S.level, ginf);
             return false;
         }
         if (S.phase == PH_REBASE) {
-            // the full-precision gradient at x_c becomes the current one
+            // the gradient at x_c at the new level becomes the current one
             std::swap(S.gk, S.gt);
             S.GG = S.TT;
             for (int p : S.pairs) {
@@ -822,7 +868,9 @@ public:
             }
             S.f = f;
             S.gnorm = ginf;
-            if (ginf < c_->opts.grad_threshold) return finish(i, SDCA_ST_CONVERGED, true);  // x_t = x_c
+            S.gnormJ = gJ;
+            if (converged(S, ginf, gJ)) return finish(i, SDCA_ST_CONVERGED, true);  // x_t = x_c
+            if (target_level(S.level, ginf) > S.level) return rebase(i);
             newdir(i, false, -1, 0.0);
             return false;
         }
@@ -832,7 +880,8 @@ public:
             S.GG = S.TT;
             S.f = f;
             S.gnorm = ginf;
-            if (ginf < c_->opts.grad_threshold) return finish(i, SDCA_ST_CONVERGED, true);
+            S.gnormJ = gJ;
+            if (converged(S, ginf, gJ)) return finish(i, SDCA_ST_CONVERGED, true);
             if (S.preloaded) {
                 S.preloaded = false;
                 newdir_preloaded(i);
@@ -846,7 +895,7 @@ public:
             const double dg0 = S.DG;
             if (!(dg0 < 0.0)) {  // rounding destroyed descent: restart from steepest descent
                 S.pairs.clear();
-                if (S.coarse) return rebase(i);
+                if (S.level < max_level_) return rebase(i);
                 newdir(i, false, -1, 0.0);  // the trial point along d is not accepted
                 return false;
             }
@@ -859,9 +908,10 @@ public:
         // 170-192): near the optimum the objective, summed from FP32 energies, changes by
         // less than its rounding noise while the directional derivative is still accurate;
         // a step is then accepted on the derivative alone, with f allowed to exceed f(0)
-        // by at most the noise level.
+        // by at most the noise level of the slot's precision.
+        const double f_noise = split_ && S.level < sdca::LV_FINE ? 1e-5 : 1e-6;
         const bool approx_wolfe = dg >= S.ms.gtol * S.dg0 && dg <= (2.0 * S.ms.ftol - 1.0) * S.dg0 &&
-                                  f <= S.f + f_noise_ * std::fabs(S.f);
+                                  f <= S.f + f_noise * std::fabs(S.f);
         sdca::MoreThuente::Result act = sdca::MoreThuente::Converged;
         if (!approx_wolfe) {
             act = S.ms.step(f, dg, S.stp);
@@ -870,7 +920,7 @@ public:
                 // a quasi-Newton step rejected twice in the coarse phase: the steps have
                 // become small against the BF16 rounding of x, where the coarse objective
                 // is too rough for the line search
-                if (S.coarse && !S.pairs.empty() && S.ls_evals >= 2) return rebase(i);
+                if (S.level == sdca::LV_COARSE && !S.pairs.empty() && S.ls_evals >= 2) return rebase(i);
                 k.mode = sdca::MODE_RETRY;
                 k.gt = S.gt;
                 k.gk = S.gk;
@@ -878,7 +928,8 @@ public:
                 return false;
             }
             if (!(f < S.f)) {
-                if (S.coarse) return rebase(i);  // continue at full precision from x_c
+                // continue at the next level from x_c: the search is below this level's noise
+                if (S.level < max_level_) return rebase(i);
                 if (S.pairs.empty() || S.restarts >= kMaxRestarts) return finish(i, SDCA_ST_LS_FAIL, false);
                 S.restarts++;
                 S.pairs.clear();
@@ -918,28 +969,30 @@ public:
         std::swap(S.gk, S.gt);
         S.f = f;
         S.gnorm = ginf;
+        S.gnormJ = gJ;
         S.iter++;
         if (S.iter == 1) S.nfeval_first = S.nfeval;
-        if (S.coarse && ginf < coarse_gnorm_) {
-            // end of the coarse phase: make the trial point the iterate (storing its pair)
-            // and evaluate it again at full precision
-            S.coarse = false;
+        const int lv = target_level(S.level, ginf);
+        if (lv > S.level) {
+            // end of a precision level: make the trial point the iterate (storing its pair)
+            // and evaluate it again at the next level
+            S.level = lv;
             newdir_identity(i, pnew, stp_trial);
             S.phase = PH_REBASE;
             return false;
         }
-        if (ginf < c_->opts.grad_threshold) return finish(i, SDCA_ST_CONVERGED, true);
+        if (converged(S, ginf, gJ)) return finish(i, SDCA_ST_CONVERGED, true);
         if (S.iter >= c_->opts.max_iter) return finish(i, SDCA_ST_ITER_CAP, true);
         newdir(i, true, pnew, stp_trial);
         return false;
     }
 
-    // End of the coarse phase without an accepted step: x_c is evaluated again at full
-    // precision (a trial step of zero along the current direction).
+    // End of a precision level without an accepted step: x_c is evaluated again at the
+    // next level (a trial step of zero along the current direction).
     bool rebase(int i)
     {
         Slot &S = slots_[i];
-        S.coarse = false;
+        S.level = std::min<int>(max_level_, std::max(S.level + 1, target_level(S.level, S.gnorm)));
         S.phase = PH_REBASE;
         sdca::SlotCmd &k = cmds_[i];
         k.mode = sdca::MODE_RETRY;
@@ -956,7 +1009,7 @@ public:
             const Slot &S = slots_[i];
             sdca::SlotCmd &k = cmds_[i];
             k.live = 0u;
-            k.coarse = S.coarse ? 1 : 0;
+            k.level = S.level;
             if (S.phase == PH_IDLE || k.mode == sdca::MODE_IDLE) continue;
             for (int p : S.pairs) k.live |= 1u << p;
         }
@@ -1093,9 +1146,12 @@ private:
     int n_, m_;
     std::vector<Slot> slots_;
     std::vector<sdca::SlotCmd> cmds_;
-    double f_noise_;  // relative rounding noise of the objective (FP32 or split-BF16 energies)
+    bool split_;  // split-BF16 contractions (else FP32)
     // split precision: coarse phase (BF16 high parts only) while ||g||_inf >= this; 0: none
     double coarse_gnorm_;
+    double jthr_;        // coupling-gradient threshold (0: none)
+    double fine_gnorm_;  // split precision with jthr_: the fine level starts below this ||g||_inf
+    int max_level_;      // LV_FINE with a coupling threshold, else LV_SPLIT
 };
 
 // One-hot int8 X8 (L*QP rows, b contiguous) and the marginal counts, built on first use.
@@ -1145,6 +1201,8 @@ void sdca_default_opts(sdca_opts *o)
     o->verbose = 0;
     o->autotune = 1;
     o->coarse_factor = 4;
+    o->fine_factor = 10;
+    o->j_threshold = 0.0;
 }
 
 int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L, int32_t q, int32_t gap,
@@ -1165,6 +1223,9 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
         if (tn != 0 && tn != 128 && tn != 256) return fail(SDCA_E_ARG, "gemm_tn_* must be 0, 128 or 256");
     if (o.coarse_factor < 0 || o.coarse_factor > 1000000)
         return fail(SDCA_E_ARG, "coarse_factor must be 0 (off) or a factor >= 1 (got %d)", o.coarse_factor);
+    if (o.fine_factor < 0 || o.fine_factor > 1000000)
+        return fail(SDCA_E_ARG, "fine_factor must be 0 (no fine level) or a factor >= 1 (got %d)", o.fine_factor);
+    if (!(o.j_threshold >= 0.0)) return fail(SDCA_E_ARG, "j_threshold must be >= 0");
     if (o.pipeline_groups < 0 || o.pipeline_groups > sdca::kMaxGroups)
         return fail(SDCA_E_ARG, "pipeline_groups must be 0, 1 or 2");
     if (o.precision != SDCA_PREC_FP32 && o.precision != SDCA_PREC_SPLIT_BF16)
@@ -1382,6 +1443,9 @@ int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x
             cmds[i].self_lo = c->col_off[r];
             cmds[i].self_hi = c->col_off[r + 1];
             cmds[i].init_src = 1.f;
+            cmds[i].level = c->opts.j_threshold > 0.0 && (c->opts.precision == SDCA_PREC_FP32 || c->opts.fine_factor > 0)
+                                ? sdca::LV_FINE
+                                : sdca::LV_SPLIT;
         }
         HIP_TRY(hipMemcpy(c->d_X0, stage.data(), (size_t)cnt * c->n_slot * 4, hipMemcpyHostToDevice));
         for (int g = 0; g < c->ngroups; ++g)
@@ -1618,6 +1682,9 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
                         o.iter = sl.iter;
                         o.status = sl.status;
                         o.nfeval_first = sl.nfeval_first;
+                        o.gnorm_J = sl.gnormJ;
+                        o.nfeval_fine = sl.nfeval_fine;
+                        o.reserved = 0;
                     }
                     noconv += (sl.status == SDCA_ST_ITER_CAP || sl.status == SDCA_ST_LS_FAIL);
                     nonfinite += (sl.status == SDCA_ST_NONFINITE);

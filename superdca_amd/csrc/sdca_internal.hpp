@@ -51,6 +51,14 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 // x_c + pstep d_old by the next NEWDIR (the same fused multiply-add, hence the same value).
 enum SlotMode : int32_t { MODE_IDLE = 0, MODE_INIT = 1, MODE_RETRY = 2, MODE_NEWDIR = 3 };
 
+// Precision level of a command's contractions under split precision: the trial point W
+// and the node beliefs G enter the MFMA as one BF16 part (coarse: the high part, ~8 bits),
+// two (split: hi + lo, ~16 bits) or three (fine: hi + mid + lo, ~24 bits, the FP32
+// contraction).  The fine level also takes exp and log at full accuracy in the softmax.
+// Under FP32 precision the contractions are FP32 at every level; LV_FINE still selects
+// the accurate exp and log.
+enum Level : int32_t { LV_COARSE = 0, LV_SPLIT = 1, LV_FINE = 2 };
+
 struct SlotCmd {
     int32_t mode;     // SlotMode
     int32_t locus;    // locus r of the slot, -1 if idle
@@ -58,7 +66,7 @@ struct SlotCmd {
     int32_t self_hi;
     int32_t slot;     // slot of the command within its group
     int32_t row;      // first of its q rows in the round's packed layout
-    int32_t coarse;   // 1: the round's GEMMs use the BF16 high parts only (split precision)
+    int32_t level;    // contraction precision of the round (split precision): LV_COARSE, LV_SPLIT or LV_FINE
     int32_t gt;       // gradient buffer (0 or 1) that receives the trial gradient
     int32_t gk;       // gradient buffer of the current iterate (after NEWDIR's accept); -1 at INIT
     int32_t accept;   // NEWDIR: 1 = the last trial point becomes the iterate first (x_c += pstep d_old)
@@ -80,12 +88,13 @@ struct ResLayout {
     int tg() const { return 2 * m + 1; }       // g_t . g_k
     int td() const { return 2 * m + 2; }       // g_t . d
     int ginf() const { return 2 * m + 3; }     // max |g_t|
-    int xsqJ() const { return 2 * m + 4; }     // sum of squares of x_t, J part
-    int xsqh() const { return 2 * m + 5; }     // sum of squares of x_t, h part
-    int dg() const { return 2 * m + 6; }       // NEWDIR: d . g_k
-    int fdata() const { return 2 * m + 7; }    // sum_b w_b (LSE_b - E_b(x_br))
-    int width() const { return 2 * m + 8; }
-    int ndots() const { return 2 * m + 4; }    // entries produced by the dot-product pass
+    int gJinf() const { return 2 * m + 4; }    // max |g_t| over the coupling entries (column K_x excluded)
+    int xsqJ() const { return 2 * m + 5; }     // sum of squares of x_t, J part
+    int xsqh() const { return 2 * m + 6; }     // sum of squares of x_t, h part
+    int dg() const { return 2 * m + 7; }       // NEWDIR: d . g_k
+    int fdata() const { return 2 * m + 8; }    // sum_b w_b (LSE_b - E_b(x_br))
+    int width() const { return 2 * m + 9; }
+    int ndots() const { return 2 * m + 5; }    // entries produced by the dot-product pass
     int nupd() const { return 3; }             // entries produced by the trial-point pass
 };
 
@@ -94,6 +103,16 @@ struct Timing {
     int64_t evals = 0, energy_launches = 0, grad_launches = 0;
     double flops_energy = 0, flops_grad = 0, bytes_lbfgs = 0;
     double update = 0, dots = 0, bytes_update = 0, bytes_dots = 0;
+};
+
+// The commands of a round are listed by level, finest first, each level's packed rows
+// starting on a tile boundary: row[l] is the first packed row and cmd[l] the first command
+// of the l-th part (fine, split, coarse), cmd[3] the number of commands.  Under split
+// precision the fine part's GEMM tiles take three MFMA passes (hi, mid, lo), the split
+// part's two and the coarse part's one.
+struct Parts {
+    int row[3];
+    int cmd[4];
 };
 
 // Kernel launchers (sdca_kernels.hip).
@@ -105,27 +124,29 @@ void hamming_neighbours(hipStream_t st, const uint64_t *planes, int B, int q, in
                         const uint32_t *d_mult, unsigned long long *d_iw);
 int energy_superiles(int Mpad, int Bp, int gm, int tn);  // super-tiles per split-K slice
 // Packed rows (trial points W, energies, beliefs): Mrows rows computed (a multiple of
-// kTile), Mz rows between split-K slices of Epart.  Command k owns rows [row, row + q).
-// Split precision: rows from m_lo on (coarse commands; m_lo a multiple of kTile) take one
-// MFMA pass over the BF16 high parts, the rows before them two (hi and lo).
-void gemm_energy(hipStream_t st, int prec, int m_lo, const void *W32, const void *Whi, const void *Wlo,
-                 const uint32_t *X, int Mrows, int Mz, int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn,
-                 float *Epart);
+// kTile), Mz rows between split-K slices of Epart.  Command k owns rows [row, row + q);
+// the Parts of the round (above) say which tiles take how many MFMA passes.
+void gemm_energy(hipStream_t st, int prec, const Parts &parts, const void *W32, const void *Whi, const void *Wlo,
+                 const void *Wl3, const uint32_t *X, int Mrows, int Mz, int Bp, int Kxp, int ksplit, int kchunk,
+                 int gm, int tn, float *Epart);
+// hpart: per command and block of 256 sequences, the FP64 sums over b of G_b(s) (q each),
+// from which the gradient epilogue forms the fields' gradient.
 void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mz, int Bp, int B, int q, int ncmd,
              const SlotCmd *cmds, const uint8_t *statesT, const float *wts, float *G32, uint16_t *Ghi,
-             uint16_t *Glo, double *fpart, int nbchunk);
-// The first nfull commands own rows [0, nfull q), the coarse ones start at row m_lo.
-void gemm_grad(hipStream_t st, int prec, int m_lo, int nfull, const void *G32, const void *Ghi, const void *Glo,
-               const uint32_t *Xt, int Mrows, int Bp, int Kxp, int Kx, int q, int ncmd, const SlotCmd *cmds,
-               const float *Xtrial, float *gbuf, int64_t n_slot, float lamJ2, float lamh2, int gm, int tn);
+             uint16_t *Glo, uint16_t *Gl3, double *fpart, double *hpart, int nbchunk);
+void gemm_grad(hipStream_t st, int prec, const Parts &parts, const void *G32, const void *Ghi, const void *Glo,
+               const void *Gl3, const uint32_t *Xt, int Mrows, int Bp, int Kxp, int Kx, int q, const SlotCmd *cmds,
+               const float *Xtrial, float *gbuf, int64_t n_slot, double lamJ2, double lamh2, const double *hpart,
+               int nbchunk, int gm, int tn);
 // Slot vectors: gbuf [slot][2][n_slot], dbuf [slot][n_slot], S and Y [slot][m][n_slot]
 // (dbuf, S and Y uint16_t BF16 when pair_bf16, else float); Xtr, Whi and Wlo are packed rows.  One
 // block row per command (ncmd commands).
 void update(hipStream_t st, int m, int q, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
-            void *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0,
-            int64_t n_slot, int Kxp, int Kx, double *part, int nchunk);
+            void *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, uint16_t *Wl3,
+            const float *X0, int64_t n_slot, int Kxp, int Kx, double *part, int nchunk);
 void dots(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
-          const void *dbuf, const void *S, const void *Y, int64_t n_slot, double *part, int nchunk);
+          const void *dbuf, const void *S, const void *Y, int64_t n_slot, int Kxp, int Kx, double *part,
+          int nchunk);
 void reduce_results(hipStream_t st, int m, int ncmd, const double *upd_part, const double *dot_part,
                     const double *sm_part, int nchunk, int nbchunk, double *res);
 void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *loci, int nscore, int L,
@@ -213,7 +234,7 @@ struct sdca_ctx {
     int64_t n_slot = 0;
     int Mpad = 0;
     int ngroups = 1, gslots = 0, Mg = 0;  // slot groups: slots and padded GEMM rows per group
-    int Mgp = 0;        // packed rows per group: Mg plus one tile (coarse and full parts pad separately)
+    int Mgp = 0;        // packed rows per group: Mg plus two tiles (the three levels' parts pad separately)
     int64_t xc_off = 0;  // offset of the iterates x_c in the d_Xtr allocation
     int ws_groups = 0;
     int ksplit = 1, kchunk = 0;
@@ -226,9 +247,10 @@ struct sdca_ctx {
     float *d_dbuf = nullptr;    // search directions: slots x n_slot (BF16 under split precision)
     void *d_S = nullptr, *d_Y = nullptr;  // curvature pairs: slots x m x n_slot (BF16 or float)
     float *d_Xc = nullptr, *d_Xtr = nullptr, *d_X0 = nullptr;
-    uint16_t *d_Whi = nullptr, *d_Wlo = nullptr;
+    uint16_t *d_Whi = nullptr, *d_Wlo = nullptr, *d_Wl3 = nullptr;  // BF16 parts of the trial points
     float *d_G32 = nullptr;
-    uint16_t *d_Ghi = nullptr, *d_Glo = nullptr;
+    uint16_t *d_Ghi = nullptr, *d_Glo = nullptr, *d_Gl3 = nullptr;  // BF16 parts of the node beliefs
+    double *d_hpart = nullptr;  // FP64 field sums per command and block of 256 sequences
     float *d_Epart = nullptr;
     double *d_upd_part = nullptr, *d_dot_part = nullptr, *d_sm_part = nullptr, *d_res = nullptr;
     sdca::SlotCmd *d_cmds = nullptr;

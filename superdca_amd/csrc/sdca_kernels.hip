@@ -244,14 +244,14 @@ __device__ __forceinline__ bool map_tile(const TileMap &tm, int bid, int &mt, in
 
 // Two workgroups per CU (eight waves): the register budget of a wave is 256 (VGPRs +
 // AGPRs), which holds the wide tile's accumulators and one prefetch stage.
-// One 128 x 64 NI output tile, split-K slice z, of C = A B^T with A = A0 (+ A1): the body
-// of k_gemm_nt for NA operand parts.
-template <class Pol, int NA, int NI, class Epi>
-__device__ __forceinline__ void gemm_tile(uint8_t *sA0, uint8_t *sA1, uint8_t *sB,
-                                          const typename Pol::T *__restrict__ A0,
-                                          const typename Pol::T *__restrict__ A1, int64_t lda,
-                                          const uint32_t *__restrict__ Bbits, int64_t ldbw, int K, int kchunk,
-                                          int mt, int nt, int z, const Epi &epi)
+// Adds the 128 x 64 NI output tile (mt, nt) of split-K slice z of A B^T, A = A0 (+ A1),
+// to acc: the K loop of k_gemm_nt for NA operand parts.
+template <class Pol, int NA, int NI>
+__device__ __forceinline__ void gemm_accum(uint8_t *sA0, uint8_t *sA1, uint8_t *sB,
+                                           const typename Pol::T *__restrict__ A0,
+                                           const typename Pol::T *__restrict__ A1, int64_t lda,
+                                           const uint32_t *__restrict__ Bbits, int64_t ldbw, int K, int kchunk,
+                                           int mt, int nt, int z, floatx16 (&acc)[2][NI])
 {
     constexpr int TN = 64 * NI;  // B rows (output columns) per tile
     uint8_t *sA[2] = {sA0, sA1};
@@ -260,13 +260,6 @@ __device__ __forceinline__ void gemm_tile(uint8_t *sA0, uint8_t *sA1, uint8_t *s
     const int kb = z * kchunk;
     const int ke = min(K, kb + kchunk);
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * (TN / 2);
-    floatx16 acc[2][NI];
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int ni = 0; ni < NI; ++ni)
-#pragma unroll
-            for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0.f;
     const int lrow = tid >> 3;         // 0..31
     const int lcol = (tid & 7) * 16;   // byte offset in the 128-byte row
     const uint8_t *pa0 = reinterpret_cast<const uint8_t *>(A0 + (m0 + lrow) * lda) + lcol;
@@ -338,32 +331,47 @@ __device__ __forceinline__ void gemm_tile(uint8_t *sA0, uint8_t *sA1, uint8_t *s
         mma_tile<Pol, NA, NI>(sA[0], sA[NA - 1], sB, wm, wn, lane, acc);
         __builtin_amdgcn_s_setprio(0);
     }
-    epi.template tile<NI>((int)m0, wm, n0 + wn, lane, tid, acc, z);
 }
 
-// Tiles of rows from m_lo on (a multiple of kTile) use the first operand part alone: under
-// split precision those are the rows of coarse commands, multiplied with the BF16 high
-// parts in one MFMA pass; the rows before them take both passes.  One launch serves both
-// kinds; the dearer tiles have the lower row indices and are dispatched first, so the
-// cheaper ones fill the end of the launch.
+// The packed rows come in three parts by precision level (each starting on a tile
+// boundary): tiles below m_split belong to fine commands and take three MFMA passes under
+// split precision (hi and mid together, then lo), tiles below m_coarse two (hi and lo),
+// the rest one pass over the BF16 high parts.  One launch serves all three; the dearer
+// tiles have the lower row indices and are dispatched first, so the cheaper ones fill the
+// end of the launch.
 template <class Pol, int NA, int NI, class Epi>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_nt(const typename Pol::T *__restrict__ A0,
-                                                    const typename Pol::T *__restrict__ A1, int64_t lda,
+                                                    const typename Pol::T *__restrict__ A1,
+                                                    const typename Pol::T *__restrict__ A2, int64_t lda,
                                                     const uint32_t *__restrict__ Bbits, int64_t ldbw, int K,
-                                                    int kchunk, int m_lo, TileMap tm, Epi epi)
+                                                    int kchunk, int m_split, int m_coarse, TileMap tm, Epi epi)
 {
     constexpr int TN = 64 * NI;
     __shared__ __attribute__((aligned(16))) uint8_t sA[NA][kTile * ROWB];
     __shared__ __attribute__((aligned(16))) uint8_t sB[TN * ROWB];
     int mt, nt, z;
     if (!map_tile(tm, blockIdx.x, mt, nt, z)) return;
+    floatx16 acc[2][NI];
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int ni = 0; ni < NI; ++ni)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0.f;
+    const int m0 = mt * kTile;
     if constexpr (NA == 2) {
-        if (mt * kTile >= m_lo) {
-            gemm_tile<Pol, 1, NI>(sA[0], sA[0], sB, A0, A0, lda, Bbits, ldbw, K, kchunk, mt, nt, z, epi);
-            return;
+        if (m0 >= m_coarse) {
+            gemm_accum<Pol, 1, NI>(sA[0], sA[0], sB, A0, A0, lda, Bbits, ldbw, K, kchunk, mt, nt, z, acc);
+        } else {
+            gemm_accum<Pol, 2, NI>(sA[0], sA[1], sB, A0, A1, lda, Bbits, ldbw, K, kchunk, mt, nt, z, acc);
+            if (m0 < m_split)
+                gemm_accum<Pol, 1, NI>(sA[0], sA[0], sB, A2, A2, lda, Bbits, ldbw, K, kchunk, mt, nt, z, acc);
         }
+    } else {
+        gemm_accum<Pol, 1, NI>(sA[0], sA[0], sB, A0, A0, lda, Bbits, ldbw, K, kchunk, mt, nt, z, acc);
     }
-    gemm_tile<Pol, NA, NI>(sA[0], sA[NA - 1], sB, A0, A1, lda, Bbits, ldbw, K, kchunk, mt, nt, z, epi);
+    const int tid = threadIdx.x, wave = tid >> 6;
+    epi.template tile<NI>(m0, (wave >> 1) * 64, (int64_t)nt * TN + (wave & 1) * (TN / 2), tid & 63, tid, acc, z);
 }
 
 // 32x32 MFMA result layout: register v of lane l holds row 8*(v/4) + 4*(l/32) + v%4,
@@ -393,34 +401,44 @@ struct EpiEnergy {
     }
 };
 
-// Gradient epilogue: packed row m = row_of(k) + s of command k, column n.  Adds 2*lambda*x (lambda_h on
-// the bias column, lambda_J elsewhere: plmDCA_objective.hpp:192-244), zeroes the locus'
-// own columns (the j != r exclusions of :95,111,161,176) and pad columns, and writes
-// the trial gradient into gradient buffer gt of the command's slot.  The commands under
-// the tile's 128 rows are staged in LDS once; each group of four rows first loads its
-// trial-point values (packed rows, like the beliefs), then stores.
+// Gradient epilogue: packed row m = row_of(k) + s of command k, column n.  Adds 2*lambda*x
+// (plmDCA_objective.hpp:192-244), zeroes the locus' own columns (the j != r exclusions of
+// :95,111,161,176) and pad columns, and writes the trial gradient into gradient buffer gt
+// of the command's slot.  The field column K_x takes its data term from the FP64 sums of
+// the softmax (sum_b G_b(s), hpart) instead of the FP32 contraction, and adds
+// 2 lambda_h h in FP64: that sum runs over every sequence, and its FP32 partial sums grow
+// to about B_eff while the result is small.  The commands under the tile's 128 rows are
+// staged in LDS once; each group of four rows first loads its trial-point values (packed
+// rows, like the beliefs), then stores.
 struct EpiGrad {
     float *gring;
     const float *Xtrial;
     const SlotCmd *cmds;
+    const double *hpart;
     int64_t n_slot, Kxp;
-    int ng, g0, q, Kx, ncmd;  // gradients per slot, ring index of the first, commands
-    int mcoarse, nfull;       // the coarse commands start at row mcoarse, after nfull others
-    float lamJ2, lamh2;
-    // command owning packed row m (ncmd for the pad rows after the full-precision part),
-    // and its first row
+    int ng, g0, q, Kx, nbchunk;  // gradients per slot, ring index of the first
+    Parts parts;
+    float lamJ2;
+    double lamh2;
+    // command owning packed row m (the number of commands for a pad row) and its first row
     __device__ __forceinline__ int cmd_of(int m) const
     {
-        if (m >= mcoarse) return nfull + (m - mcoarse) / q;
-        return m < nfull * q ? m / q : ncmd;
+        const int p = m >= parts.row[2] ? 2 : (m >= parts.row[1] ? 1 : 0);
+        const int i = (m - parts.row[p]) / q;
+        return i < parts.cmd[p + 1] - parts.cmd[p] ? parts.cmd[p] + i : parts.cmd[3];
     }
-    __device__ __forceinline__ int row_of(int i) const { return i < nfull ? i * q : mcoarse + (i - nfull) * q; }
+    __device__ __forceinline__ int row_of(int i) const
+    {
+        const int p = i >= parts.cmd[2] ? 2 : (i >= parts.cmd[1] ? 1 : 0);
+        return parts.row[p] + (i - parts.cmd[p]) * q;
+    }
     template <int NI>
     __device__ __forceinline__ void tile(int m0, int wm, int64_t nw, int lane, int tid, floatx16 (&acc)[2][NI],
                                          int) const
     {
         // s_gt < 0: no gradient to write
         __shared__ int s_gt[kTile + 2], s_lo[kTile + 2], s_hi[kTile + 2], s_slot[kTile + 2];
+        const int ncmd = parts.cmd[3];
         const int i0 = cmd_of(m0);
         const int ns = min(ncmd - i0, (kTile - 1) / q + 2);  // commands under rows [m0, m0 + 128)
         if (tid < ns) {
@@ -435,7 +453,7 @@ struct EpiGrad {
         for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
             for (int vq = 0; vq < 16; vq += 4) {  // four rows at a time
-                int gt[4], lo[4], hi[4];
+                int gt[4], lo[4], hi[4], cm[4], srow[4];
                 int64_t roff[4];
                 float xv[4][NI];
 #pragma unroll
@@ -447,9 +465,10 @@ struct EpiGrad {
                     gt[u] = ok ? s_gt[il] : -1;
                     lo[u] = ok ? s_lo[il] : 0;
                     hi[u] = ok ? s_hi[il] : 0;
+                    cm[u] = i;
+                    srow[u] = ok ? m - row_of(i) : 0;
                     const int slot = ok ? s_slot[il] : 0;
-                    roff[u] = ((int64_t)slot * ng + (gt[u] < 0 ? 0 : gt[u] - g0)) * n_slot +
-                              (int64_t)(ok ? m - row_of(i) : 0) * Kxp;
+                    roff[u] = ((int64_t)slot * ng + (gt[u] < 0 ? 0 : gt[u] - g0)) * n_slot + (int64_t)srow[u] * Kxp;
                     const float *xr = Xtrial + (int64_t)m * Kxp;
 #pragma unroll
                     for (int ni = 0; ni < NI; ++ni) {
@@ -465,8 +484,14 @@ struct EpiGrad {
                     for (int ni = 0; ni < NI; ++ni) {
                         const int64_t n = nw + ni * 32 + (lane & 31);
                         float val = 0.f;
-                        if (n <= Kx && !(n >= lo[u] && n < hi[u]))
-                            val = acc[mi][ni][vq + u] + (n == Kx ? lamh2 : lamJ2) * xv[u][ni];
+                        if (n == Kx) {
+                            const double *hp = hpart + (int64_t)cm[u] * nbchunk * q + srow[u];
+                            double hs = 0.0;
+                            for (int ch = 0; ch < nbchunk; ++ch) hs += hp[(int64_t)ch * q];
+                            val = (float)(hs + lamh2 * (double)xv[u][ni]);
+                        } else if (n < Kx && !(n >= lo[u] && n < hi[u])) {
+                            val = acc[mi][ni][vq + u] + lamJ2 * xv[u][ni];
+                        }
                         __builtin_nontemporal_store(val, gring + roff[u] + n);
                     }
                 }
@@ -480,62 +505,117 @@ struct EpiGrad {
 // log-partition lz = log sum_s exp E_b(s) (max-shifted; the same value as the
 // unshifted plmDCA_objective.hpp:140), the data term w_b (lz - E_b(x_br)) (:143) and the
 // node beliefs G_b(s) = w_b P_b(s) - w_b [s == x_br] (:147-148), with 1 - P(x_br)
-// formed as the sum of the other probabilities.  The data term is reduced in FP64.
+// formed as the sum of the other probabilities.  Coarse and split commands take the
+// hardware exp2/log2 approximations; fine ones the accurate expf/logf and P = e_s / z.
+// The data term and, per state, sum_b G_b(s) (the fields' data gradient, :151) are
+// reduced in FP64 per block of 256 sequences.
 __global__ __launch_bounds__(256) void k_softmax(const float *__restrict__ Epart, int ksplit, int64_t zstride, int Bp,
                                                  int B, int q, const SlotCmd *__restrict__ cmds,
                                                  const uint8_t *__restrict__ statesT, const float *__restrict__ wts,
                                                  float *__restrict__ G32, uint16_t *__restrict__ Ghi,
-                                                 uint16_t *__restrict__ Glo, double *__restrict__ fpart, int nbchunk)
+                                                 uint16_t *__restrict__ Glo, uint16_t *__restrict__ Gl3,
+                                                 double *__restrict__ fpart, double *__restrict__ hpart, int nbchunk)
 {
     const int i = blockIdx.y;
     const int b = blockIdx.x * 256 + threadIdx.x;
     const int r = cmds[i].mode == MODE_IDLE ? -1 : cmds[i].locus;
     const int64_t row0 = cmds[i].row;
-    const bool coarse = cmds[i].coarse != 0;  // BF16 high parts only
+    const int level = cmds[i].level;
     double fb = 0.0;
+    double hs[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (b < Bp) {
         float e[8], g[8];
         if (r >= 0) {
-            for (int s = 0; s < q; ++s) {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                if (s >= q) break;
                 float acc = 0.f;
                 for (int z = 0; z < ksplit; ++z) acc += Epart[(int64_t)z * zstride + (row0 + s) * Bp + b];
                 e[s] = acc;
             }
             const float w = wts[b];
             const int xr = b < B ? statesT[(int64_t)r * B + b] : 0;
-            float mx = e[0];
-            for (int s = 1; s < q; ++s) mx = fmaxf(mx, e[s]);
-            float z = 0.f;
-            for (int s = 0; s < q; ++s) z += __expf(e[s] - mx);
-            const float lz = mx + __logf(z);
-            fb = (double)w * ((double)lz - (double)e[xr]);
-            float other = 0.f;
-            for (int s = 0; s < q; ++s) {
-                const float p = __expf(e[s] - lz);
-                g[s] = w * p;
-                if (s != xr) other += p;
+            float mx = e[0], exr = e[0];
+#pragma unroll
+            for (int s = 1; s < 8; ++s) {
+                if (s >= q) break;
+                mx = fmaxf(mx, e[s]);
+                if (s == xr) exr = e[s];
             }
-            g[xr] = -w * other;
+            float p[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            float lz;
+            if (level == LV_FINE) {
+                float z = 0.f;
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    if (s >= q) break;
+                    p[s] = expf(e[s] - mx);
+                    z += p[s];
+                }
+                lz = mx + logf(z);
+                const float iz = 1.f / z;
+#pragma unroll
+                for (int s = 0; s < 8; ++s) p[s] *= iz;
+            } else {
+                float z = 0.f;
+#pragma unroll
+                for (int s = 0; s < 8; ++s)
+                    if (s < q) z += __expf(e[s] - mx);
+                lz = mx + __logf(z);
+#pragma unroll
+                for (int s = 0; s < 8; ++s) p[s] = s < q ? __expf(e[s] - lz) : 0.f;
+            }
+            fb = (double)w * ((double)lz - (double)exr);
+            float other = 0.f;
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+                if (s < q && s != xr) other += p[s];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                g[s] = s == xr ? -w * other : w * p[s];
+                hs[s] = (double)g[s];
+            }
         } else {
-            for (int s = 0; s < q; ++s) g[s] = 0.f;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) g[s] = 0.f;
         }
-        for (int s = 0; s < q; ++s) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            if (s >= q) break;
             const int64_t o = (row0 + s) * Bp + b;
             if (G32) {
                 G32[o] = g[s];
             } else {
                 const uint16_t h = f2bf(g[s]);
                 Ghi[o] = h;
-                if (!coarse) Glo[o] = f2bf(g[s] - bf2f(h));
+                if (level != LV_COARSE) {
+                    const float rest = g[s] - bf2f(h);
+                    const uint16_t l = f2bf(rest);
+                    Glo[o] = l;
+                    if (level == LV_FINE) Gl3[o] = f2bf(rest - bf2f(l));
+                }
             }
         }
     }
-    // block reduction of fb
-    __shared__ double red[4];
-    for (int off = 32; off > 0; off >>= 1) fb += __shfl_down(fb, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = fb;
+    // block reductions of fb and of the field sums
+    __shared__ double red[4][9];
+    for (int off = 32; off > 0; off >>= 1) {
+        fb += __shfl_down(fb, off, 64);
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            if (s < q) hs[s] += __shfl_down(hs[s], off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6][8] = fb;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) red[threadIdx.x >> 6][s] = hs[s];
+    }
     __syncthreads();
-    if (threadIdx.x == 0) fpart[(int64_t)i * nbchunk + blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+    if (threadIdx.x == 0) fpart[(int64_t)i * nbchunk + blockIdx.x] = red[0][8] + red[1][8] + red[2][8] + red[3][8];
+    if (threadIdx.x < q) {
+        const int s = threadIdx.x;
+        hpart[((int64_t)i * nbchunk + blockIdx.x) * q + s] = red[0][s] + red[1][s] + red[2][s] + red[3][s];
+    }
 }
 
 // ------------------------------------------------------ L-BFGS vector passes
@@ -656,7 +736,8 @@ __device__ __forceinline__ float amax8(const f8 &x)
 //           x_t = x_c + alpha d; and the product d . g_k of the stored d.  Pairs with a
 //           zero coefficient are not read.  (The products d . y_p follow on the host from
 //           the coefficients and the pair products.)
-// x_t (and its BF16 hi/lo copies; hi only for a coarse command) is written at the
+// x_t (and its BF16 parts: hi for a coarse command, hi and lo for a split one, hi, mid and
+// lo for a fine one) is written at the
 // command's packed rows.  Plus the squared norms of x_t for the L2 terms:
 // part[command][chunk][3] = {sum x^2 over the whole vector, sum over the h entries
 // (column Kx of each row), d . g_k}.
@@ -665,6 +746,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
                                                 PT *__restrict__ dbuf, PT *__restrict__ S, PT *__restrict__ Y,
                                                 float *__restrict__ Xc, float *__restrict__ Xtr,
                                                 uint16_t *__restrict__ Whi, uint16_t *__restrict__ Wlo,
+                                                uint16_t *__restrict__ Wl3,
                                                 const float *__restrict__ X0, int64_t n_slot, int Kxp, int Kx, int q,
                                                 double *__restrict__ part, int nchunk)
 {
@@ -695,7 +777,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
         const int pnew = c.pnew;
         const int64_t n8 = n_slot >> 3;
         const bool accept = c.accept != 0;
-        const bool coarse = c.coarse != 0;  // the GEMMs read the BF16 high part only
+        const int level = c.level;  // BF16 parts the GEMMs read: 1 (coarse), 2 (split) or 3 (fine)
         const bool from_x0 = c.init_src != 0.f && x0;
         float cs[M], cy[M];
 #pragma unroll
@@ -748,25 +830,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
             st8(xt, e, xn);
             const float xs[8] = {xn.a.x, xn.a.y, xn.a.z, xn.a.w, xn.b.x, xn.b.y, xn.b.z, xn.b.w};
             if (Whi) {
-                uint32_t h[4], l[4];
+                // x = hi + lo (+ l3): each part is the rounded remainder of the previous ones,
+                // and x - hi and (x - hi) - lo are exact in FP32
+                uint32_t h[4], l[4], t[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
                     const uint16_t h0 = f2bf(xs[2 * k]), h1 = f2bf(xs[2 * k + 1]);
-                    const uint16_t l0 = f2bf(xs[2 * k] - bf2f(h0)), l1 = f2bf(xs[2 * k + 1] - bf2f(h1));
+                    const float r0 = xs[2 * k] - bf2f(h0), r1 = xs[2 * k + 1] - bf2f(h1);
+                    const uint16_t l0 = f2bf(r0), l1 = f2bf(r1);
                     h[k] = (uint32_t)h0 | ((uint32_t)h1 << 16);
                     l[k] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+                    t[k] = (uint32_t)f2bf(r0 - bf2f(l0)) | ((uint32_t)f2bf(r1 - bf2f(l1)) << 16);
                 }
-                u32x4 hv, lv;
+                u32x4 hv, lv, tv;
                 hv.x = h[0], hv.y = h[1], hv.z = h[2], hv.w = h[3];
                 lv.x = l[0], lv.y = l[1], lv.z = l[2], lv.w = l[3];
+                tv.x = t[0], tv.y = t[1], tv.z = t[2], tv.w = t[3];
                 __builtin_nontemporal_store(hv, reinterpret_cast<u32x4 *>(Whi + xo + e));
-                if (!coarse) __builtin_nontemporal_store(lv, reinterpret_cast<u32x4 *>(Wlo + xo + e));
+                if (level != LV_COARSE) __builtin_nontemporal_store(lv, reinterpret_cast<u32x4 *>(Wlo + xo + e));
+                if (level == LV_FINE) __builtin_nontemporal_store(tv, reinterpret_cast<u32x4 *>(Wl3 + xo + e));
             }
             acc[0] += dot8(xn, xn);
             // the h entry of row s is column Kx (K_x is padded to a multiple of 8, so an
-            // 8-element chunk lies in one row)
-            const int64_t row = e / Kxp, col = e - row * Kxp;
-            if (row < q && Kx >= col && Kx < col + 8) {
+            // 8-element chunk lies in one row; a slot vector has fewer than 2^32 elements)
+            const uint32_t col = (uint32_t)e % (uint32_t)Kxp;
+            if ((uint32_t)Kx >= col && (uint32_t)Kx < col + 8) {
                 const float hv = xs[Kx - col];
                 acc[1] += (double)hv * hv;
             }
@@ -776,15 +864,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
 }
 
 // Products of each command slot's trial gradient g_t with the stored pairs, itself, the
-// current gradient and the direction, and its infinity norm:
-// part[command][chunk][2M + 4] = {g_t.s_p, g_t.y_p, g_t.g_t, g_t.g_k, g_t.d, max |g_t|}.
+// current gradient and the direction, and its infinity norms over all entries and over
+// the coupling entries (the field column K_x of each row excluded):
+// part[command][chunk][2M + 5] = {g_t.s_p, g_t.y_p, g_t.g_t, g_t.g_k, g_t.d, max |g_t|, max |g_t,J|}.
 template <int M, class PT>
 __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, const float *__restrict__ gbuf,
                                               const PT *__restrict__ dbuf, const PT *__restrict__ S,
-                                              const PT *__restrict__ Y, int64_t n_slot, double *__restrict__ part,
-                                              int nchunk)
+                                              const PT *__restrict__ Y, int64_t n_slot, int Kxp, int Kx,
+                                              double *__restrict__ part, int nchunk)
 {
-    constexpr int NV = 2 * M + 4;
+    constexpr int NV = 2 * M + 5;
     const int j = blockIdx.y;
     const SlotCmd &c = cmds[j];
     double acc[NV];
@@ -792,7 +881,7 @@ __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, 
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
         acc[k] = 0.0;
-        is_max[k] = (k == 2 * M + 3);
+        is_max[k] = (k >= 2 * M + 3);
     }
     if (c.mode != MODE_IDLE && (c.gt == 0 || c.gt == 1)) {
         const int64_t i = c.slot;
@@ -807,7 +896,20 @@ __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, 
         for (int64_t e8 = (int64_t)blockIdx.x * 256 + threadIdx.x; e8 < n8; e8 += (int64_t)nchunk * 256) {
             const int64_t e = e8 * 8;
             const f8 gv = ld8(g, e);
-            acc[2 * M + 3] = fmax(acc[2 * M + 3], (double)amax8(gv));
+            const float ga = amax8(gv);
+            acc[2 * M + 3] = fmax(acc[2 * M + 3], (double)ga);
+            // the coupling entries: the chunk that holds a row's field entry leaves it out
+            const uint32_t col = (uint32_t)e % (uint32_t)Kxp;
+            float gJ = ga;
+            if ((uint32_t)Kx >= col && (uint32_t)Kx < col + 8) {
+                const float gs[8] = {gv.a.x, gv.a.y, gv.a.z, gv.a.w, gv.b.x, gv.b.y, gv.b.z, gv.b.w};
+                const uint32_t hpos = (uint32_t)Kx - col;
+                gJ = 0.f;
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if ((uint32_t)u != hpos) gJ = fmaxf(gJ, fabsf(gs[u]));
+            }
+            acc[2 * M + 4] = fmax(acc[2 * M + 4], (double)gJ);
             acc[2 * M] += dot8(gv, gv);
             if (cur) {
                 acc[2 * M + 1] += dot8(gv, ld8(gk, e));
@@ -825,17 +927,18 @@ __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, 
 }
 
 // Per-command reduction of the chunk partials into the result row (ResLayout): the dots
-// pass fills [0, 2m+4), the trial-point pass [2m+4, 2m+7) (two squares and d.g_k; the
-// first square is the whole vector, stored as the J part), the softmax the data term.
+// pass fills [0, 2m+5) (its last two entries are maxima), the trial-point pass
+// [2m+5, 2m+8) (two squares and d.g_k; the first square is the whole vector, stored as
+// the J part), the softmax the data term.
 __global__ void k_reduce(int m, const double *__restrict__ upd_part, const double *__restrict__ dot_part,
                          const double *__restrict__ sm_part, int nchunk, int nbchunk, double *__restrict__ res)
 {
     const int i = blockIdx.x;
     const int t = threadIdx.x;
-    const int ND = 2 * m + 4, NU = 3, W = 2 * m + 8;
+    const int ND = 2 * m + 5, NU = 3, W = 2 * m + 9;
     double *out = res + (int64_t)i * W;
     if (t < ND) {
-        const bool mx = t == ND - 1;
+        const bool mx = t >= ND - 2;
         double a = 0.0;
         for (int ch = 0; ch < nchunk; ++ch) {
             const double v = dot_part[((int64_t)i * nchunk + ch) * ND + t];
@@ -968,49 +1071,53 @@ int energy_superiles(int Mpad, int Bp, int gm, int tn)
 
 // Launches k_gemm_nt with output tiles 128 x tn (tn = 128 or 256; N must be a multiple).
 template <class Pol, int NA, class Epi>
-static void launch_gemm(hipStream_t st, int tn, const TileMap &tm, const void *A0, const void *A1, int64_t lda,
-                        const uint32_t *Bbits, int64_t ldbw, int K, int kchunk, int m_lo, const Epi &epi)
+static void launch_gemm(hipStream_t st, int tn, const TileMap &tm, const void *A0, const void *A1, const void *A2,
+                        int64_t lda, const uint32_t *Bbits, int64_t ldbw, int K, int kchunk, int m_split,
+                        int m_coarse, const Epi &epi)
 {
     using T = typename Pol::T;
     const dim3 grid(map_grid(tm));
     if (tn == 256)
-        hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 4, Epi>), grid, dim3(256), 0, st, (const T *)A0, (const T *)A1, lda,
-                           Bbits, ldbw, K, kchunk, m_lo, tm, epi);
+        hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 4, Epi>), grid, dim3(256), 0, st, (const T *)A0, (const T *)A1,
+                           (const T *)A2, lda, Bbits, ldbw, K, kchunk, m_split, m_coarse, tm, epi);
     else
-        hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 2, Epi>), grid, dim3(256), 0, st, (const T *)A0, (const T *)A1, lda,
-                           Bbits, ldbw, K, kchunk, m_lo, tm, epi);
+        hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 2, Epi>), grid, dim3(256), 0, st, (const T *)A0, (const T *)A1,
+                           (const T *)A2, lda, Bbits, ldbw, K, kchunk, m_split, m_coarse, tm, epi);
 }
 
-void gemm_energy(hipStream_t st, int prec, int m_lo, const void *W32, const void *Whi, const void *Wlo,
-                 const uint32_t *X, int Mrows, int Mz, int Bp, int Kxp, int ksplit, int kchunk, int gm, int tn,
-                 float *Epart)
+void gemm_energy(hipStream_t st, int prec, const Parts &parts, const void *W32, const void *Whi, const void *Wlo,
+                 const void *Wl3, const uint32_t *X, int Mrows, int Mz, int Bp, int Kxp, int ksplit, int kchunk,
+                 int gm, int tn, float *Epart)
 {
     EpiEnergy epi{Epart, (int64_t)Bp, (int64_t)Mz * Bp};
     const TileMap tm = make_map(Mrows / kTile, Bp / tn, ksplit, gm);
     if (prec == SDCA_PREC_FP32)
-        launch_gemm<PolF32, 1>(st, tn, tm, W32, W32, Kxp, X, Kxp / 32, Kxp, kchunk, Mrows, epi);
+        launch_gemm<PolF32, 1>(st, tn, tm, W32, W32, W32, Kxp, X, Kxp / 32, Kxp, kchunk, 0, Mrows, epi);
     else
-        launch_gemm<PolBF16, 2>(st, tn, tm, Whi, Wlo, Kxp, X, Kxp / 32, Kxp, kchunk, m_lo, epi);
+        launch_gemm<PolBF16, 2>(st, tn, tm, Whi, Wlo, Wl3, Kxp, X, Kxp / 32, Kxp, kchunk, parts.row[1],
+                                parts.row[2], epi);
 }
 
 void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mz, int Bp, int B, int q, int ncmd,
              const SlotCmd *cmds, const uint8_t *statesT, const float *wts, float *G32, uint16_t *Ghi,
-             uint16_t *Glo, double *fpart, int nbchunk)
+             uint16_t *Glo, uint16_t *Gl3, double *fpart, double *hpart, int nbchunk)
 {
     hipLaunchKernelGGL(k_softmax, dim3(nbchunk, ncmd), dim3(256), 0, st, Epart, ksplit, (int64_t)Mz * Bp, Bp, B, q,
-                       cmds, statesT, wts, prec == SDCA_PREC_FP32 ? G32 : nullptr, Ghi, Glo, fpart, nbchunk);
+                       cmds, statesT, wts, prec == SDCA_PREC_FP32 ? G32 : nullptr, Ghi, Glo, Gl3, fpart, hpart,
+                       nbchunk);
 }
 
-void gemm_grad(hipStream_t st, int prec, int m_lo, int nfull, const void *G32, const void *Ghi, const void *Glo,
-               const uint32_t *Xt, int Mrows, int Bp, int Kxp, int Kx, int q, int ncmd, const SlotCmd *cmds,
-               const float *Xtrial, float *gbuf, int64_t n_slot, float lamJ2, float lamh2, int gm, int tn)
+void gemm_grad(hipStream_t st, int prec, const Parts &parts, const void *G32, const void *Ghi, const void *Glo,
+               const void *Gl3, const uint32_t *Xt, int Mrows, int Bp, int Kxp, int Kx, int q, const SlotCmd *cmds,
+               const float *Xtrial, float *gbuf, int64_t n_slot, double lamJ2, double lamh2, const double *hpart,
+               int nbchunk, int gm, int tn)
 {
-    EpiGrad epi{gbuf, Xtrial, cmds, n_slot, (int64_t)Kxp, 2, 0, q, Kx, ncmd, m_lo, nfull, lamJ2, lamh2};
+    EpiGrad epi{gbuf, Xtrial, cmds, hpart, n_slot, (int64_t)Kxp, 2, 0, q, Kx, nbchunk, parts, (float)lamJ2, lamh2};
     const TileMap tm = make_map(Mrows / kTile, Kxp / tn, 1, gm);
     if (prec == SDCA_PREC_FP32)
-        launch_gemm<PolF32, 1>(st, tn, tm, G32, G32, Bp, Xt, Bp / 32, Bp, Bp, Mrows, epi);
+        launch_gemm<PolF32, 1>(st, tn, tm, G32, G32, G32, Bp, Xt, Bp / 32, Bp, Bp, 0, Mrows, epi);
     else
-        launch_gemm<PolBF16, 2>(st, tn, tm, Ghi, Glo, Bp, Xt, Bp / 32, Bp, Bp, m_lo, epi);
+        launch_gemm<PolBF16, 2>(st, tn, tm, Ghi, Glo, Gl3, Bp, Xt, Bp / 32, Bp, Bp, parts.row[1], parts.row[2], epi);
 }
 
 #define SDCA_DISPATCH_M(m, CALL) \
@@ -1025,31 +1132,34 @@ void gemm_grad(hipStream_t st, int prec, int m_lo, int nfull, const void *G32, c
     }
 
 void update(hipStream_t st, int m, int q, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
-            void *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, const float *X0,
-            int64_t n_slot, int Kxp, int Kx, double *part, int nchunk)
+            void *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, uint16_t *Wl3,
+            const float *X0, int64_t n_slot, int Kxp, int Kx, double *part, int nchunk)
 {
 #define SDCA_UPD(MM)                                                                                                \
     if (pair_bf16)                                                                                                  \
         hipLaunchKernelGGL((k_update<MM, uint16_t>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf,          \
-                           (uint16_t *)dbuf, (uint16_t *)S, (uint16_t *)Y, Xc, Xtr, Whi, Wlo, X0, n_slot, Kxp, Kx, \
-                           q, part, nchunk);                                                                        \
+                           (uint16_t *)dbuf, (uint16_t *)S, (uint16_t *)Y, Xc, Xtr, Whi, Wlo, Wl3, X0, n_slot, Kxp, \
+                           Kx, q, part, nchunk);                                                                    \
     else                                                                                                            \
         hipLaunchKernelGGL((k_update<MM, float>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf, (float *)dbuf, \
-                           (float *)S, (float *)Y, Xc, Xtr, Whi, Wlo, X0, n_slot, Kxp, Kx, q, part, nchunk)
+                           (float *)S, (float *)Y, Xc, Xtr, Whi, Wlo, Wl3, X0, n_slot, Kxp, Kx, q, part, \
+                           nchunk)
     SDCA_DISPATCH_M(m, SDCA_UPD)
 #undef SDCA_UPD
 }
 
 void dots(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
-          const void *dbuf, const void *S, const void *Y, int64_t n_slot, double *part, int nchunk)
+          const void *dbuf, const void *S, const void *Y, int64_t n_slot, int Kxp, int Kx, double *part,
+          int nchunk)
 {
 #define SDCA_DOT(MM)                                                                                                \
     if (pair_bf16)                                                                                                  \
         hipLaunchKernelGGL((k_dots<MM, uint16_t>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf,            \
-                           (const uint16_t *)dbuf, (const uint16_t *)S, (const uint16_t *)Y, n_slot, part, nchunk); \
+                           (const uint16_t *)dbuf, (const uint16_t *)S, (const uint16_t *)Y, n_slot, Kxp, Kx, part, \
+                           nchunk);                                                                                 \
     else                                                                                                            \
         hipLaunchKernelGGL((k_dots<MM, float>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf,               \
-                           (const float *)dbuf, (const float *)S, (const float *)Y, n_slot, part, nchunk)
+                           (const float *)dbuf, (const float *)S, (const float *)Y, n_slot, Kxp, Kx, part, nchunk)
     SDCA_DISPATCH_M(m, SDCA_DOT)
 #undef SDCA_DOT
 }

@@ -3,7 +3,8 @@ eight cores).
 
 The C2 workload (B = 1000 sequences, L = 5000 loci, q_env = 4, seed 1002 of
 superdca_amd.synth) solved for every locus by the FP64 oracle to its optimum
-(||g||_inf < 1e-8), scored with the reference gauge and Frobenius norm, symmetrised as the
+(||g||_inf < 1e-8, or below 1e-6 where its line search stops at the FP64 rounding
+floor), scored with the reference gauge and Frobenius norm, symmetrised as the
 coupling writer does (double(float(S(r, n) + S(n, r))) * 0.5, plmDCA.hpp:667-693), and
 reduced to what the GPU comparison needs:
 
@@ -43,7 +44,7 @@ def main() -> None:
     t = time.time()
     rows, _, stats = orc.solve_loci(st, w, lh, lJ, np.arange(L), gtol=1e-8, max_iter=20000,
                                     nthreads=os.cpu_count() or 8)
-    ok = stats["status"] == 0
+    ok = (stats["status"] == 0) | ((stats["status"] == 2) & (stats["gnorm"] < 1e-6))
     print(f"solved {L} loci in {time.time() - t:.0f} s, status counts {np.bincount(stats['status'])}, "
           f"max ||g||_inf {stats['gnorm'].max():.2e}", flush=True)
     assert np.all(ok), stats["gnorm"][~ok]

@@ -3,7 +3,8 @@ eight cores).
 
 The C4 workload (B = 3000 sequences, L = 94,000 loci, q_env = 5, seed 1004 of
 superdca_amd.synth): 64 loci drawn at random (seed 64) solved by the FP64 oracle to its
-optimum (||g||_inf < 1e-8) and scored with the reference gauge and Frobenius norm (score rows S(r, .),
+optimum (||g||_inf < 1e-8, or below 1e-6 where its line search stops at the FP64
+rounding floor) and scored with the reference gauge and Frobenius norm (score rows S(r, .),
 plmDCA.hpp:366-394).  Kept per locus: the 200 largest entries of its row and 200 entries
 drawn at random (seed 65), with their indices.
 
@@ -38,7 +39,7 @@ def main() -> None:
     loci = np.sort(np.random.default_rng(64).choice(L, 64, replace=False)).astype(np.int32)
     t = time.time()
     rows, _, stats = orc.solve_loci(st, w, lh, lJ, loci, q=5, gtol=1e-8, max_iter=20000, nthreads=nt)
-    ok = stats["status"] == 0
+    ok = (stats["status"] == 0) | ((stats["status"] == 2) & (stats["gnorm"] < 1e-6))
     print(f"solved {loci.size} loci in {time.time() - t:.0f} s, status counts {np.bincount(stats['status'])}, "
           f"max ||g||_inf {stats['gnorm'].max():.2e}", flush=True)
     assert np.all(ok), stats["gnorm"][~ok]

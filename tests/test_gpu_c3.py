@@ -26,7 +26,7 @@ def c3_cases():
         w, iw = orc.weights(st, 0.9, nthreads=16)
         _, lh, lJ = orc.lambdas(w)
         loci = np.linspace(0, L - 1, NLOCI).astype(np.int32)
-        rows, _, stats = orc.solve_loci(st, w, lh, lJ, loci, q=q, gtol=1e-7, max_iter=20000, nthreads=16)
+        rows, _, stats = orc.solve_loci(st, w, lh, lJ, loci, q=q, gtol=1e-9, max_iter=20000, nthreads=16)
         # a line search that fails at the FP64 rounding floor leaves the optimum to within far
         # less than the GPU thresholds below
         assert np.all((stats["status"] == 0) | ((stats["status"] == 2) & (stats["gnorm"] < 1e-6)))
@@ -43,19 +43,19 @@ def test_c3_reweight_bit_exact(c3_cases):
         np.testing.assert_array_equal(w, c["w"])
 
 
-# measured on MI355X (largest over the three alignments): FP32 at 1e-5: f 8.8e-7,
-# top-decile scores 8.3e-5 (median over loci 9e-6..2.4e-5); split BF16 at 1e-4: f 1.1e-6,
-# scores 4.9e-4 (median 4.9e-5..3.2e-4) -- the largest errors on B = 103, L = 16,890
-@pytest.mark.parametrize("precision,gtol,frtol,rtol_top", [(sd.PREC_FP32, 1e-5, 3e-6, 2e-4),
-                                                           (sd.PREC_SPLIT_BF16, 1e-4, 5e-6, 1.5e-3)])
-def test_c3_loci_against_oracle_optimum(c3_cases, precision, gtol, frtol, rtol_top):
+# SuperDCA's rule plus the coupling threshold ||g_J||_inf < 3e-6; bars: f 1e-6 relative,
+# every locus' top-decile scores 1e-4 relative
+@pytest.mark.parametrize("kw", [pytest.param(dict(precision=sd.PREC_FP32), id="fp32"),
+                                pytest.param(dict(precision=sd.PREC_SPLIT_BF16, fine_factor=0), id="split"),
+                                pytest.param(dict(precision=sd.PREC_SPLIT_BF16, fine_factor=10), id="split+fine")])
+def test_c3_loci_against_oracle_optimum(c3_cases, kw):
     for c in c3_cases:
         st, loci, ref = c["states"], c["loci"], c["rows"]
-        with sd.Context(st, q=c["q"], precision=precision, grad_threshold=gtol, batch_loci=NLOCI) as ctx:
+        with sd.Context(st, q=c["q"], j_threshold=3e-6, batch_loci=NLOCI, **kw) as ctx:
             ctx.set_weights(c["w"])
             rows, stats, _, rc = ctx.solve(loci)
         assert rc in (sd.SDCA_OK, sd.SDCA_E_NOCONV), rc
-        assert np.all((stats["status"] == sd.ST_CONVERGED) | (stats["gnorm"] < 1e-3))
+        assert np.all((stats["status"] == sd.ST_CONVERGED) | ((stats["status"] == sd.ST_LS_FAIL) & (stats["gnorm"] < 1e-3)))
         frel = np.abs(stats["fval"] - c["f"]) / np.abs(c["f"])
         # per locus: the relative error of the row's top decile (locus itself excluded)
         per_locus = []
@@ -66,8 +66,8 @@ def test_c3_loci_against_oracle_optimum(c3_cases, precision, gtol, frtol, rtol_t
             per_locus.append((np.abs(rows[k] - ref[k])[big] / ref[k][big]).max())
         per_locus = np.array(per_locus)
         B, L = st.shape
-        print(f"C3 seed {c['cfg']['seed']} B={B} L={L} q={c['q']} precision {precision} ||g|| < {gtol:g}: "
+        print(f"C3 seed {c['cfg']['seed']} B={B} L={L} q={c['q']} {kw}: "
               f"f rel err max {frel.max():.2e}; per-locus top-decile score rel err "
               f"{np.array2string(per_locus, precision=1, separator=' ')}")
-        assert frel.max() < frtol
-        assert per_locus.max() < rtol_top
+        assert frel.max() < 1e-6
+        assert per_locus.max() < 1e-4

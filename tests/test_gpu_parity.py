@@ -1,8 +1,8 @@
 """GPU parity: the HIP path (through the C ABI of libsdca.so) against the FP64 oracle.
 
 Bars: reweighting bit-exact; objective value and gradient within the stated FP32 /
-split-BF16 tolerances; solved scores against the oracle's optimum within the stated
-relative tolerance.
+split-BF16 / fine-level tolerances; solved scores against the oracle's optimum within
+1e-4 relative.
 """
 import os
 import subprocess
@@ -61,15 +61,19 @@ def test_reweight_bit_exact_c2(c2):
 EVAL_LOCI = [0, 1, 17, 500, 991, 999]  # 1000 % 16 = 8: 992..999 form the short last block
 
 
-@pytest.mark.parametrize("precision,ftol,gtol", [(sd.PREC_FP32, 2e-6, 2e-5), (sd.PREC_SPLIT_BF16, 2e-5, 2e-4)])
-def test_eval_batch_matches_oracle(c1, precision, ftol, gtol):
+# FP32, split BF16 (hi + lo) and the fine level (three BF16 parts, accurate exp and log:
+# the evaluation of a context with a coupling threshold)
+@pytest.mark.parametrize("precision,jthr,ftol,gtol", [(sd.PREC_FP32, 0.0, 2e-6, 2e-5),
+                                                      (sd.PREC_SPLIT_BF16, 0.0, 2e-5, 2e-4),
+                                                      (sd.PREC_SPLIT_BF16, 1e-6, 2e-6, 2e-5)])
+def test_eval_batch_matches_oracle(c1, precision, jthr, ftol, gtol):
     st = c1["states"]
     B, L = st.shape
     w, _ = orc.weights(st, 0.9, nthreads=8)
     rng = np.random.default_rng(12)
     n = orc.nparams(L, 4)
     x = rng.normal(size=(len(EVAL_LOCI), n)) * 0.05
-    with sd.Context(st, precision=precision, batch_loci=4) as ctx:
+    with sd.Context(st, precision=precision, batch_loci=4, j_threshold=jthr) as ctx:
         beff, lh, lJ = ctx.set_weights(w)
         f, g = ctx.eval_batch(EVAL_LOCI, x)
     for k, r in enumerate(EVAL_LOCI):
@@ -102,7 +106,8 @@ def c1_optimum(c1):
     st = c1["states"]
     w, _ = orc.weights(st, 0.9, nthreads=8)
     beff, lh, lJ = orc.lambdas(w)
-    rows, X, stats = orc.solve_loci(st, w, lh, lJ, SOLVE_LOCI, gtol=1e-7, max_iter=10000, nthreads=8, want_x=True)
+    rows, X, stats = orc.solve_loci(st, w, lh, lJ, SOLVE_LOCI, gtol=1e-9, max_iter=10000, nthreads=8, want_x=True)
+    assert np.all((stats["status"] == 0) | ((stats["status"] == 2) & (stats["gnorm"] < 1e-7)))
     return {"w": w, "rows": rows, "x": X, "stats": stats, "lam": (lh, lJ)}
 
 
@@ -115,36 +120,49 @@ def score_errors(rows, ref):
     return np.abs(rows - ref)[mask].max(), rel.max()
 
 
-def test_solve_fp32_matches_oracle_optimum(c1, c1_optimum):
-    # 5e-6 is near the FP32 floor: a locus may stop on a line-search stall there, but only
-    # with its gradient already below 1e-5
+def test_gradient_at_the_optimum_is_below_the_coupling_threshold(c1, c1_optimum):
+    # at the oracle's optimum (||g||_inf < 1e-7) the GPU gradient is its own error: it must
+    # lie well below the coupling threshold 3e-6 the accurate solves stop at
     st = c1["states"]
-    with sd.Context(st, grad_threshold=5e-6, batch_loci=64) as ctx:
+    L, q = st.shape[1], 4
+    hidx = L * q * q + np.arange(q)
+    for precision, jthr in ((sd.PREC_SPLIT_BF16, 1e-6), (sd.PREC_SPLIT_BF16, 0.0), (sd.PREC_FP32, 1e-6)):
+        with sd.Context(st, precision=precision, batch_loci=8, j_threshold=jthr) as ctx:
+            ctx.set_weights(c1_optimum["w"])
+            f, g = ctx.eval_batch(SOLVE_LOCI[:8], c1_optimum["x"][:8])
+        jm = np.ones(g.shape[1], dtype=bool)
+        jm[hidx] = False
+        gJ, gh = np.abs(g[:, jm]).max(), np.abs(g[:, hidx]).max()
+        frel = np.abs(f - c1_optimum["stats"]["fval"][:8]) / np.abs(c1_optimum["stats"]["fval"][:8])
+        print(f"precision {precision} j {jthr:g}: |g_J| {gJ:.2e}, |g_h| {gh:.2e} at the optimum, f rel {frel.max():.2e}")
+        assert gJ < 1e-6 and gh < 1e-6
+        assert frel.max() < 1e-6
+
+
+# accurate solves: SuperDCA's rule (||g||_inf < 1e-3) plus the coupling threshold
+# ||g_J||_inf < 3e-6, in FP32, at split precision alone and with the fine level; a locus may
+# end on a line-search stall below the threshold's neighbourhood
+ACCURATE = [pytest.param(dict(precision=sd.PREC_FP32), id="fp32"),
+            pytest.param(dict(precision=sd.PREC_SPLIT_BF16, fine_factor=0), id="split"),
+            pytest.param(dict(precision=sd.PREC_SPLIT_BF16, fine_factor=10), id="split+fine")]
+
+
+@pytest.mark.parametrize("kw", ACCURATE)
+def test_solve_accurate_matches_oracle_optimum(c1, c1_optimum, kw):
+    st = c1["states"]
+    with sd.Context(st, j_threshold=3e-6, batch_loci=64, **kw) as ctx:
         ctx.set_weights(c1_optimum["w"])
         rows, stats, X, rc = ctx.solve(SOLVE_LOCI, want_x=True)
     assert rc in (sd.SDCA_OK, sd.SDCA_E_NOCONV)
-    ok = (stats["status"] == sd.ST_CONVERGED) | ((stats["status"] == sd.ST_LS_FAIL) & (stats["gnorm"] < 1e-5))
+    ok = (stats["status"] == sd.ST_CONVERGED) | ((stats["status"] == sd.ST_LS_FAIL) & (stats["gnorm_J"] < 3e-5))
     assert ok.all(), stats
     np.testing.assert_allclose(stats["fval"], c1_optimum["stats"]["fval"], rtol=1e-6)
     for k, r in enumerate(SOLVE_LOCI):
         assert rows[k, r] == 0.0
     abs_err, rel_top = score_errors(rows, c1_optimum["rows"])
-    print(f"fp32 scores: max abs err {abs_err:.3e}, max rel err (top decile) {rel_top:.3e}")
+    print(f"{kw}: {int((stats['status'] != 0).sum())} line-search stops, scores max abs err {abs_err:.3e}, "
+          f"max rel err (top decile) {rel_top:.3e}")
     assert rel_top < 1e-4
-
-
-def test_solve_split_bf16_matches_oracle_optimum(c1, c1_optimum):
-    # hi + lo BF16 carries about 18 significant bits per product (|x - hi - lo| <= 2^-18 |x|):
-    # the gradient floor is ~2e-5 here and the top-decile score error ~3e-4
-    st = c1["states"]
-    with sd.Context(st, precision=sd.PREC_SPLIT_BF16, grad_threshold=3e-5, batch_loci=64) as ctx:
-        ctx.set_weights(c1_optimum["w"])
-        rows, stats, X, rc = ctx.solve(SOLVE_LOCI)
-    assert rc == sd.SDCA_OK
-    np.testing.assert_allclose(stats["fval"], c1_optimum["stats"]["fval"], rtol=1e-6)
-    abs_err, rel_top = score_errors(rows, c1_optimum["rows"])
-    print(f"split-bf16 scores: max abs err {abs_err:.3e}, max rel err (top decile) {rel_top:.3e}")
-    assert rel_top < 5e-4
 
 
 def test_reference_mode_threshold_and_refill(c1, c1_optimum):
@@ -205,16 +223,19 @@ def test_cli_end_to_end(tmp_path):
     dict(gemm_ksplit=3, gemm_gm_energy=2, gemm_gm_grad=4, gemm_tn_energy=256, gemm_tn_grad=256),
     dict(gemm_ksplit=7, gemm_gm_energy=64, gemm_gm_grad=3, gemm_tn_energy=256, gemm_tn_grad=128),
     dict(gemm_ksplit=2, gemm_gm_energy=1, gemm_gm_grad=8, gemm_tn_energy=128, gemm_tn_grad=256)])
-@pytest.mark.parametrize("precision,ftol,gtol", [(sd.PREC_FP32, 2e-6, 2e-5), (sd.PREC_SPLIT_BF16, 2e-5, 2e-4)])
-def test_gemm_configurations_match_oracle(c1, cfg, precision, ftol, gtol):
-    # every tile walk / split-K the autotuner may choose computes the same objective;
-    # 20 slots x q = 80 rows pad to one 128-row tile, 48 slots to two
+@pytest.mark.parametrize("precision,jthr,ftol,gtol", [(sd.PREC_FP32, 0.0, 2e-6, 2e-5),
+                                                      (sd.PREC_SPLIT_BF16, 0.0, 2e-5, 2e-4),
+                                                      (sd.PREC_SPLIT_BF16, 1e-6, 2e-6, 2e-5)])
+def test_gemm_configurations_match_oracle(c1, cfg, precision, jthr, ftol, gtol):
+    # every tile walk / split-K the autotuner may choose computes the same objective, with
+    # two MFMA passes (split) or three (the fine level of a context with a coupling
+    # threshold); 20 slots x q = 80 rows pad to one 128-row tile, 48 slots to two
     st = c1["states"]
     w, _ = orc.weights(st, 0.9, nthreads=8)
     n = orc.nparams(st.shape[1], 4)
     loci = np.arange(3, 1000, 21)[:48]
     x = np.random.default_rng(31).normal(size=(loci.size, n)) * 0.05
-    with sd.Context(st, precision=precision, batch_loci=48, **cfg) as ctx:
+    with sd.Context(st, precision=precision, batch_loci=48, j_threshold=jthr, **cfg) as ctx:
         beff, lh, lJ = ctx.set_weights(w)
         f, g = ctx.eval_batch(loci, x)
         got = ctx.gemm_config()

@@ -35,10 +35,13 @@ def test_default_opts_match_reference_defaults():
     assert o.history == 10
     assert o.precision == sd.PREC_FP32
     assert o.coarse_factor == 4  # split precision: BF16 high parts alone while ||g||_inf >= 4 * threshold
+    assert o.j_threshold == 0.0  # SuperDCA's stopping rule alone unless a coupling threshold is asked for
+    assert o.fine_factor == 10
 
 
 @pytest.mark.parametrize("bad", [{"coarse_factor": -1}, {"pipeline_groups": 3}, {"history": 7},
-                                 {"gemm_tn_energy": 192}, {"gemm_gm_grad": 65}, {"precision": 9}])
+                                 {"gemm_tn_energy": 192}, {"gemm_gm_grad": 65}, {"precision": 9},
+                                 {"fine_factor": -1}, {"j_threshold": -1e-6}])
 def test_invalid_options_are_rejected_before_any_device_call(bad):
     # option checks precede device selection, so they hold without a GPU
     st = np.zeros((4, 3), dtype=np.uint8)
