@@ -912,9 +912,14 @@ public:
         const double f_noise = split_ && S.level < sdca::LV_FINE ? 1e-5 : 1e-6;
         const bool approx_wolfe = dg >= S.ms.gtol * S.dg0 && dg <= (2.0 * S.ms.ftol - 1.0) * S.dg0 &&
                                   f <= S.f + f_noise * std::fabs(S.f);
+        // Within the noise level the measured f no longer orders the trial steps: the search
+        // then runs on phi(0) + stp (phi'(0) + phi'(stp)) / 2, the trapezoid rule over the
+        // accurate directional derivatives (exact for a quadratic), so rounding in f cannot
+        // fake a bracket.
+        const double f_ls = std::fabs(f - S.f) <= f_noise * std::fabs(S.f) ? S.f + 0.5 * stp_trial * (S.dg0 + dg) : f;
         sdca::MoreThuente::Result act = sdca::MoreThuente::Converged;
         if (!approx_wolfe) {
-            act = S.ms.step(f, dg, S.stp);
+            act = S.ms.step(f_ls, dg, S.stp);
             S.ls_evals++;
             if (act == sdca::MoreThuente::Continue && S.ls_evals < c_->opts.max_ls) {
                 // a quasi-Newton step rejected twice in the coarse phase: the steps have
@@ -927,7 +932,7 @@ public:
                 k.alpha = (float)S.stp;
                 return false;
             }
-            if (!(f < S.f)) {
+            if (!(f_ls < S.f)) {
                 // continue at the next level from x_c: the search is below this level's noise
                 if (S.level < max_level_) return rebase(i);
                 if (S.pairs.empty() || S.restarts >= kMaxRestarts) return finish(i, SDCA_ST_LS_FAIL, false);
@@ -1331,14 +1336,7 @@ int sdca_reweight(sdca_ctx *c, double theta, const uint32_t *mult, double *w_out
     if (!w_out) return fail(SDCA_E_ARG, "w_out is null");
     const int B = c->B, L = c->L, q = c->q;
     const int W = (L + 63) / 64;
-    // largest mismatch count d with double(L - d)/double(L) >= theta
-    // (plmDCA_optimizer_parameters.hpp:67-72: size_t difference, then the FP64 division)
-    int64_t dmax = -1;
-    for (int64_t d = 0; d <= L; ++d) {
-        const double identity = (double)((size_t)L - (size_t)d) / (double)(size_t)L;
-        if (identity >= theta) dmax = d;
-        else break;  // identity is non-increasing in d
-    }
+    const int64_t dmax = sdca::identity_dmax(L, theta);
     uint64_t *d_planes = nullptr;
     unsigned long long *d_iw = nullptr;
     uint32_t *d_mult = nullptr;

@@ -115,6 +115,21 @@ struct Parts {
     int cmd[4];
 };
 
+// Largest mismatch count d with double(L - d) / double(L) >= theta, evaluated with the
+// reference's own expression (plmDCA_optimizer_parameters.hpp:67-72: a size_t difference,
+// then the FP64 division), or -1 if there is none; the reweighting kernels then compare
+// integers.
+inline int64_t identity_dmax(int L, double theta)
+{
+    int64_t dmax = -1;
+    for (int64_t d = 0; d <= L; ++d) {
+        const double identity = (double)((size_t)L - (size_t)d) / (double)(size_t)L;
+        if (identity >= theta) dmax = d;
+        else break;  // identity is non-increasing in d
+    }
+    return dmax;
+}
+
 // Kernel launchers (sdca_kernels.hip).
 namespace kern {
 void build_x(hipStream_t st, const uint8_t *d_states, int B, int L, int q, const int32_t *d_state_col,

@@ -205,14 +205,14 @@ def test_cli_end_to_end(tmp_path):
     assert len(lines) == L * (L - 1) // 2
     first = [int(t.split()[1]) > int(t.split()[2]) for t in lines]
     assert all(first)
-    # same numbers as the library called directly
+    # the numbers of the library called directly (to the last digits an autotuned split-K moves)
     with sd.Context(states) as ctx:
         w, _ = ctx.reweight(0.9)
         ctx.set_weights(w)
         rows, _, _, _ = ctx.solve(np.arange(L))
     ref = tmp_path / "direct.all"
     sd.write_couplings(str(ref), rows, loci, 1)
-    assert ref.read_text().splitlines() == lines
+    assert_same_couplings(lines, ref.read_text().splitlines())
 
 
 # ------------------------------------------------------------ GEMM configurations
@@ -324,3 +324,16 @@ def test_eval_and_solve_q5_envelope(c1, precision, ftol, gtol):
     assert rc == sd.SDCA_OK and np.all(stats["gnorm"] < 1e-3)
     ro, _, so = orc.solve_loci(st5, w, lh, lJ, np.array(loci), q=5, gtol=1e-7, max_iter=10000, nthreads=4)
     np.testing.assert_allclose(stats["fval"], so["fval"], rtol=1e-5)
+
+
+def assert_same_couplings(got: list, ref: list, atol: float = 2e-7):
+    """Two coupling files with the same pairs in the same order and scores within atol: each
+    process autotunes its own split-K, which changes the FP32 summation order of the
+    energies and so the last digits of a solve stopped at SuperDCA's rule."""
+    assert len(got) == len(ref)
+    a = [t.split() for t in got]
+    b = [t.split() for t in ref]
+    assert [t[1:] for t in a] == [t[1:] for t in b]
+    va = np.array([float(t[0]) for t in a])
+    vb = np.array([float(t[0]) for t in b])
+    assert np.abs(va - vb).max() <= atol, np.abs(va - vb).max()

@@ -69,7 +69,7 @@ def test_scan_refusals(scan_case):
 
 
 def test_cli_scan_end_to_end(tmp_path):
-    # sdca with two alignment files: the same numbers as the library driven directly
+    # sdca with two alignment files: the numbers of the library driven directly
     d = synth.generate(B=120, N=50, clades=6, pairs=5, seed=606)
     a, b = d["chars"][:, :15], d["chars"][:, 15:]
     for name, part in (("first.fa", a), ("second.fa", b)):
@@ -89,4 +89,17 @@ def test_cli_scan_end_to_end(tmp_path):
         ctx.set_scan_split(L1)
         rows, _, _, _ = ctx.solve(np.arange(L1, dtype=np.int32))
     sd.write_scan_couplings(str(tmp_path / "direct.all"), np.ascontiguousarray(rows[:, L1:]), tr1, tr2, 1)
-    assert (tmp_path / "direct.all").read_text().splitlines() == lines
+    assert_same_couplings(lines, (tmp_path / "direct.all").read_text().splitlines())
+
+
+def assert_same_couplings(got: list, ref: list, atol: float = 2e-7):
+    """Two coupling files with the same pairs in the same order and scores within atol: each
+    process autotunes its own split-K, which changes the FP32 summation order of the
+    energies and so the last digits of a solve stopped at SuperDCA's rule."""
+    assert len(got) == len(ref)
+    a = [t.split() for t in got]
+    b = [t.split() for t in ref]
+    assert [t[1:] for t in a] == [t[1:] for t in b]
+    va = np.array([float(t[0]) for t in a])
+    vb = np.array([float(t[0]) for t in b])
+    assert np.abs(va - vb).max() <= atol, np.abs(va - vb).max()
