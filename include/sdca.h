@@ -147,6 +147,21 @@ int64_t sdca_num_columns(const sdca_ctx *ctx);
 int sdca_reweight(sdca_ctx *ctx, double theta, const uint32_t *mult, double *w_out,
                   uint64_t *int_w_out);
 
+/* sdca_reweight without device allocation.  No counterpart in SuperDCA: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Inputs and outputs are device pointers: d_states
+ * (B x L envelope states, row-major, values in [0, q)), d_mult (B multiplicities, may be
+ * NULL), d_int_w (B neighbour counts, may be NULL) and d_w (B weights, FP64, the values
+ * of sdca_reweight bit for bit).  The kernels run on `stream` (a hipStream_t, NULL for
+ * the default stream) of the current device and the call returns without waiting; no
+ * hipMalloc, hipFree or memset is issued, so the call may be captured in a HIP graph.
+ * The bit-planes and counters live in the library's fixed 64 MiB device scratch array,
+ * one per device: B * (8 + 8 * q * ceil(L/64)) bytes (rounded) must fit, else SDCA_E_ARG.
+ * Every call on a device shares that device's scratch: calls that overlap on one device
+ * (other streams, other threads) race. */
+int sdca_reweight_noalloc(const uint8_t *d_states, int32_t B, int32_t L, int32_t q, double theta,
+                          const uint32_t *d_mult, uint64_t *d_int_w, double *d_w, void *stream);
+
 /* Installs sequence weights and derives B_eff and lambda_h, lambda_J as in
  * plmDCA_optimizer_parameters.hpp:257-284 (negative user value = automatic rule).
  * out3 (may be NULL) receives {B_eff, lambda_h, lambda_J}. */

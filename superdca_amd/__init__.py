@@ -45,6 +45,7 @@ EXPORTED = (
     "sdca_mi_topk", "sdca_pair_features", "sdca_mlp_init", "sdca_mlp_num_params", "sdca_mlp_get_params",
     "sdca_mlp_set_params", "sdca_mlp_forward", "sdca_mlp_loss_grad", "sdca_mlp_train", "sdca_solve_loci_mlp",
     "sdca_gemm_config", "sdca_write_couplings_top", "sdca_set_scan_split", "sdca_write_scan_couplings",
+    "sdca_reweight_noalloc",
 )
 
 
@@ -127,12 +128,14 @@ def lib() -> C.CDLL:
     L.sdca_solve_loci_mlp.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                                       C.c_void_p, C.c_void_p]
     L.sdca_gemm_config.argtypes = [C.c_void_p, C.c_void_p]
+    L.sdca_reweight_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p]
     for name in ("sdca_ctx_create", "sdca_reweight", "sdca_set_weights", "sdca_eval_batch", "sdca_solve_loci",
                  "sdca_get_timing", "sdca_prepare_alignment", "sdca_write_couplings", "sdca_pair_counts",
                  "sdca_pair_count_table", "sdca_mi_topk", "sdca_pair_features", "sdca_mlp_init",
                  "sdca_mlp_get_params", "sdca_mlp_set_params", "sdca_mlp_forward", "sdca_mlp_loss_grad",
                  "sdca_mlp_train", "sdca_solve_loci_mlp", "sdca_gemm_config", "sdca_write_couplings_top",
-                 "sdca_set_scan_split", "sdca_write_scan_couplings"):
+                 "sdca_set_scan_split", "sdca_write_scan_couplings", "sdca_reweight_noalloc"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -353,6 +356,16 @@ class Context:
         _check(lib().sdca_gemm_config(self._h, cfg))
         return {"tuned": bool(cfg[0]), "ksplit": cfg[1], "kchunk": cfg[2], "gm_energy": cfg[3], "gm_grad": cfg[4],
                 "tn_energy": cfg[5], "tn_grad": cfg[6]}
+
+
+def reweight_noalloc(d_states: int, B: int, L: int, q: int, theta: float, d_mult: int | None, d_int_w: int | None,
+                     d_w: int, stream: int | None = None) -> None:
+    """sdca_reweight_noalloc on device buffers given by address (no device allocation; the
+    kernels are queued on `stream` and the call returns without waiting)."""
+    _check(lib().sdca_reweight_noalloc(C.c_void_p(d_states), int(B), int(L), int(q), float(theta),
+                                       C.c_void_p(d_mult) if d_mult else None,
+                                       C.c_void_p(d_int_w) if d_int_w else None, C.c_void_p(d_w),
+                                       C.c_void_p(stream) if stream else None))
 
 
 def prepare_alignment(chars: np.ndarray, maf_threshold: float = 0.01, gap_threshold: float = 0.15,

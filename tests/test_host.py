@@ -441,3 +441,15 @@ def test_every_hip_compile_disables_packed_fp32_and_targets_gfx950():
     assert "--offload-arch=gfx950" in flags
     k = flags.index("-packed-fp32-ops")
     assert flags[k - 3:k] == ["-Xclang", "-target-feature", "-Xclang"]
+
+
+def test_noalloc_reweight_refuses_bad_arguments_before_any_device_call():
+    # null buffers, shapes outside the envelope and alignments beyond the 64 MiB scratch are
+    # refused before a device is touched (the addresses below are never dereferenced)
+    fake = 4096
+    cases = [(None, 10, 10, 4, fake), (fake, 0, 10, 4, fake), (fake, 10, 10, 9, fake), (fake, 10, 10, 4, None),
+             (fake, 3000, 94000, 5, fake)]
+    for d_states, B, L, q, d_w in cases:
+        with pytest.raises(sd.SdcaError) as e:
+            sd.reweight_noalloc(d_states, B, L, q, 0.9, None, None, d_w)
+        assert e.value.code == sd.SDCA_E_ARG, (B, L, q)
