@@ -191,6 +191,10 @@ int sdca_solve_loci(sdca_ctx *ctx, const int32_t *loci, int32_t K, const double 
 
 /* Two-alignment scan (SuperDCA.cpp:155-159, plmDCA.hpp:482-490): the context's alignment is
  * the first alignment's n_first loci followed by the second's, over the same sequences.
+ * Parameters (x_out) keep the layout of that combined alignment, n = (n_first + L2) q q + q,
+ * with the blocks over the first alignment zero; the reference's scan parameter vector
+ * covers the L2 loci of the second alignment only (plmDCA.hpp:482-490), i.e. the blocks
+ * from locus n_first on.
  * With n_first > 0 a solve takes loci of the first alignment only and fits each on the
  * second alignment's columns alone (plmDCA_objective.hpp:67: no block of the second
  * alignment is excluded, the first is not a predictor); its score row is zero over the

@@ -1215,7 +1215,7 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
 {
     if (!out) return fail(SDCA_E_ARG, "out is null");
     *out = nullptr;
-    if (!states || B < 1 || L < 2) return fail(SDCA_E_ARG, "need states, B >= 1 and L >= 2 (got B=%d L=%d)", B, L);
+    if (!states || B < 1 || L < 1) return fail(SDCA_E_ARG, "need states, B >= 1 and L >= 1 (got B=%d L=%d)", B, L);
     if (q < 2 || q > 8) return fail(SDCA_E_ARG, "q_env must be in [2, 8] (got %d)", q);
     if (gap >= q) return fail(SDCA_E_ARG, "gap_state %d outside [0, %d)", gap, q);
     sdca_opts o;

@@ -34,6 +34,7 @@ struct Options {
     bool output_weights = false;
     bool norm_of_mean = false;
     double gradient_threshold = 1e-3;  // plmDCA_options.cpp:61
+    double coupling_threshold = 0.0;
     double lambda_h = -1.0, lambda_J = -1.0;
     bool no_dca = false, no_coupling_output = false;
     long output_top = 0;                // > 0: also write the largest scores (.top)
@@ -71,6 +72,9 @@ const char *kUsage =
     "  --reweighting-threshold arg (=0.9)\n"
     "  --output-weights               Write sample weights to file.\n"
     "  --gradient-threshold arg (=0.001)\n"
+    "  --coupling-gradient-threshold arg (=0)\n"
+    "                                 Also require ||g||_inf < arg over the couplings J (0: off);\n"
+    "                                 3e-6 puts the scores within 1e-4 of the exact optimum.\n"
     "  --lambda-h arg (=-1)           (< 0: automatic)\n"
     "  --lambda-J arg (=-1)           (< 0: automatic)\n"
     "  --no-dca                       Stop after the weights.\n"
@@ -137,6 +141,7 @@ Options parse_args(int argc, char **argv)
         else if (a == "--output-weights") o.output_weights = true;
         else if (a == "--norm-of-mean-scoring") o.norm_of_mean = true;
         else if (a == "--gradient-threshold") o.gradient_threshold = std::atof(need(i).c_str());
+        else if (a == "--coupling-gradient-threshold") o.coupling_threshold = std::atof(need(i).c_str());
         else if (a == "--lambda-h") o.lambda_h = std::atof(need(i).c_str());
         else if (a == "--lambda-J") o.lambda_J = std::atof(need(i).c_str());
         else if (a == "--no-dca") o.no_dca = true;
@@ -238,6 +243,7 @@ sdca_opts solver_opts(const Options &o)
     so.history = o.history;
     so.max_iter = o.max_iter;
     so.grad_threshold = o.gradient_threshold;
+    so.j_threshold = o.coupling_threshold;
     so.verbose = o.verbose;
     return so;
 }
@@ -275,7 +281,7 @@ int run_scan(const Options &o)
                 << " of " << L_in << " loci pass the filter\n";
     }
     const int L1 = Ls[0], L2 = Ls[1];
-    if (L1 < 1 || L2 < 2) {
+    if (L1 < 1 || L2 < 1) {  // any non-empty second alignment (plmDCA.hpp:491)
         if (o.verbose) log << "plmDCA: there are not enough loci for plmDCA analysis\n";
         return EXIT_FAILURE;
     }
@@ -305,7 +311,7 @@ int run_scan(const Options &o)
     }
     const std::string id = id_string(o.alignments[0]);
     if (o.output_weights) {
-        FILE *wf = fopen(unique_name(id + "_scan.weights").c_str(), "w");
+        FILE *wf = fopen(unique_name(id + ".weights").c_str(), "w");  // plmDCA.hpp:550, also when scanning
         if (!wf) die("cannot write weights");
         for (double v : w) fprintf(wf, "%.8e\n", v);
         fclose(wf);

@@ -41,13 +41,13 @@ def test_scan_matches_the_oracle_on_the_reduced_alignment(scan_case):
     for k, r in enumerate(loci):
         red = np.ascontiguousarray(np.concatenate([st[:, r:r + 1], st2], axis=1))
         ref, _, so = orc.solve_loci(red, w2, lh, lJ, np.array([0], dtype=np.int32), gtol=1e-9, max_iter=10000)
-        assert stats["fval"][k] == pytest.approx(so["fval"][0], rel=1e-6)
+        assert stats["fval"][k] == pytest.approx(so["fval"][0], rel=1e-7)
         assert np.all(rows[k, :L1] == 0.0)  # the first alignment is no predictor
         got, want = rows[k, L1:], ref[0, 1:]
         big = want > np.quantile(want, 0.9)
         rel = np.abs(got - want)[big] / want[big]
         print(f"scan locus {r}: f {stats['fval'][k]:.8f} vs {so['fval'][0]:.8f}, top-decile rel err {rel.max():.2e}")
-        assert rel.max() < 1e-3
+        assert rel.max() < 1e-5
 
 
 def test_scan_refusals(scan_case):
@@ -75,8 +75,11 @@ def test_cli_scan_end_to_end(tmp_path):
     for name, part in (("first.fa", a), ("second.fa", b)):
         synth.write_fasta(str(tmp_path / name), part)
     exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "superdca_amd", "sdca")
-    p = subprocess.run([exe, "first.fa", "second.fa"], capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    p = subprocess.run([exe, "--output-weights", "first.fa", "second.fa"], capture_output=True, text=True,
+                       cwd=tmp_path, timeout=300)
     assert p.returncode == 0, p.stderr
+    # the weights file keeps the reference's name when scanning (plmDCA.hpp:550)
+    assert (tmp_path / "first.weights").exists() and not (tmp_path / "first_scan.weights").exists()
     lines = (tmp_path / "first_scan.SuperDCA_couplings.1-based.all").read_text().splitlines()
     st1, tr1 = sd.prepare_alignment(a)
     st2, tr2 = sd.prepare_alignment(b)
@@ -90,6 +93,23 @@ def test_cli_scan_end_to_end(tmp_path):
         rows, _, _, _ = ctx.solve(np.arange(L1, dtype=np.int32))
     sd.write_scan_couplings(str(tmp_path / "direct.all"), np.ascontiguousarray(rows[:, L1:]), tr1, tr2, 1)
     assert_same_couplings(lines, (tmp_path / "direct.all").read_text().splitlines())
+
+
+def test_cli_scan_against_a_one_locus_second_alignment(tmp_path):
+    # any non-empty second alignment is scanned (plmDCA.hpp:491), one locus included
+    d = synth.generate(B=120, N=30, clades=6, pairs=3, seed=607)
+    a, b = d["chars"][:, :20], d["chars"][:, 20:]
+    _, kept = sd.prepare_alignment(b)
+    one = np.ascontiguousarray(b[:, kept[0]:kept[0] + 1])  # a column that passes the filter
+    for name, part in (("first.fa", a), ("second.fa", one)):
+        synth.write_fasta(str(tmp_path / name), part)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "superdca_amd", "sdca")
+    p = subprocess.run([exe, "first.fa", "second.fa"], capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    assert p.returncode == 0, p.stderr
+    L1 = sd.prepare_alignment(a)[0].shape[1]
+    lines = (tmp_path / "first_scan.SuperDCA_couplings.1-based.all").read_text().splitlines()
+    assert len(lines) == L1
+    assert all(t.split()[2] == "1" for t in lines)
 
 
 def assert_same_couplings(got: list, ref: list, atol: float = 2e-7):
