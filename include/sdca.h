@@ -303,6 +303,49 @@ int sdca_write_couplings_top(const char *path, const float *S, int32_t L, const 
 int sdca_write_scan_couplings(const char *path, const float *S, int32_t K, int32_t L2,
                               const int32_t *translation1, const int32_t *translation2, int32_t base);
 
+/* ---- file-backed stores and streaming writers (genome-scale output) ----
+ * SuperDCA holds every score, or with --norm-of-mean-scoring every q x q coupling block,
+ * of a run in memory and writes its output from there (plmDCA.hpp:86-190,589-697): 35 GB
+ * of scores at 94,000 loci, and far more blocks.  A store keeps them in an unlinked file
+ * instead, rows x cols entries of `entry` floats; the writers read it in tiles of rows, so
+ * host memory stays at two bands of about 256 MB (or `tile` rows of each). */
+typedef struct sdca_store sdca_store;
+
+/* Creates the store's file at path (which must not exist; it is unlinked at once and
+ * its space returned by sdca_store_destroy or the end of the process). */
+int sdca_store_create(sdca_store **out, const char *path, int64_t rows, int64_t cols, int32_t entry);
+void sdca_store_destroy(sdca_store *s);
+
+/* Rows [r0, r0 + n) of the store: n x cols x entry floats, row-major. */
+int sdca_store_put_rows(sdca_store *s, int64_t r0, int64_t n, const float *rows);
+
+/* Row `row` of a store with entry q*q: for each column u the coupling block of locus r
+ * with locus col_loci[u] from the solution x (reference layout, L*q*q + q doubles), as
+ * float and transposed when col_loci[u] < r, so that both blocks of a pair run [state at
+ * the larger locus][state at the smaller] (the storage of plmDCA.hpp:335-363, without the
+ * row aliasing of its get_Ji_matrices at :133-137); the block of r itself is zero. */
+int sdca_store_put_blocks(sdca_store *s, int64_t row, const double *x, int32_t L, int32_t q, int32_t r,
+                          const int32_t *col_loci);
+
+/* sdca_write_couplings and sdca_write_couplings_top over a square store of scores (entry
+ * 1, row and column k = list position k), byte for byte the same files; either path may
+ * be NULL.  tile: rows per tile (0: automatic). */
+int sdca_store_write_couplings(const sdca_store *s, const char *path_all, const char *path_top,
+                               const int32_t *translation, int32_t base, int64_t top_n, double min_score,
+                               int32_t tile);
+
+/* The --norm-of-mean-scoring output (plmDCA.hpp:617-666) from a square store of q*q
+ * blocks: for r over the list and n before r, "J r n Jij Jji (Jij+Jji)/2" with Jij, Jji
+ * the norms (gap row and column excluded) of the two gauged blocks and J the norm of their
+ * mean, in the reference's float arithmetic; values %.8f. */
+int sdca_store_write_norm_of_mean(const sdca_store *s, const char *path, const int32_t *translation, int32_t base,
+                                  int32_t q, int32_t gap, int32_t tile);
+
+/* The two-alignment scan's --norm-of-mean-scoring output (plmDCA.hpp:625-638): one line
+ * "Jij r n" per block of the store (rows: first-alignment loci, columns: second). */
+int sdca_store_write_norm_of_mean_scan(const sdca_store *s, const char *path, const int32_t *translation1,
+                                       const int32_t *translation2, int32_t base, int32_t q, int32_t gap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,11 @@ def lib() -> C.CDLL:
         L.orc_ising_gauge.argtypes = [vp, i32, i32, vp]
         L.orc_frobenius_excl.argtypes = [vp, i32, i32]
         L.orc_frobenius_excl.restype = dbl
+        L.orc_nom_block.argtypes = [vp, i32, i32, i32, i32, vp]
+        L.orc_ising_gauge_float.argtypes = [vp, i32, vp]
+        L.orc_frobenius_excl_float.argtypes = [vp, i32, i32]
+        L.orc_frobenius_excl_float.restype = C.c_float
+        L.orc_nom_pair.argtypes = [vp, vp, i32, i32, vp]
         L.orc_solve_loci.argtypes = [vp, i32, i32, i32, i32, vp, dbl, dbl, vp, i32, dbl, i32, i32, i32, i32,
                                      vp, vp, vp, vp]
         L.orc_solve_loci.restype = i32
@@ -145,6 +150,29 @@ def ising_gauge(M, transpose=False):
     q = M.shape[0]
     out = np.zeros((q, q))
     lib().orc_ising_gauge(_p(M), q, int(transpose), _p(out))
+    return out
+
+
+def nom_block(x, L, q, r, n):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    M = np.zeros((q, q), dtype=np.float32)
+    lib().orc_nom_block(_p(x), int(L), int(q), int(r), int(n), _p(M))
+    return M
+
+
+def ising_gauge_float(M):
+    M = np.ascontiguousarray(M, dtype=np.float32)
+    out = np.zeros_like(M)
+    lib().orc_ising_gauge_float(_p(M), M.shape[0], _p(out))
+    return out
+
+
+def nom_pair(Mrn, Mnr, gap=None):
+    a = np.ascontiguousarray(Mrn, dtype=np.float32)
+    b = np.ascontiguousarray(Mnr, dtype=np.float32)
+    q = a.shape[0]
+    out = np.zeros(3, dtype=np.float32)
+    lib().orc_nom_pair(_p(a), _p(b), q, q - 1 if gap is None else int(gap), _p(out))
     return out
 
 

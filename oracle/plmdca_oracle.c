@@ -601,7 +601,10 @@ int32_t orc_lbfgs(orc_objfn fn, void *ctx, int64_t n, double *x, int32_t m, doub
                 approx = 1;
                 break;
             }
-            ls = mt_step(&ms, f, dgt, &stp);
+            /* within that rounding level the search runs on f(0) + stp (g0.d + g.d) / 2, the
+             * trapezoid rule over the directional derivatives (exact for a quadratic) */
+            const double f_ls = fabs(f - fo) <= 1e-12 * fabs(fo) ? fo + 0.5 * stp * (dg + dgt) : f;
+            ls = mt_step(&ms, f_ls, dgt, &stp);
             if (ls != 0) break;
         }
         if (ls == 3) {
@@ -695,6 +698,71 @@ double orc_frobenius_excl(const double *G, int32_t q, int32_t exclude)
 
 /* plmDCA.hpp:378-392: rows n<r use the transposed gauge, n>r the plain one;
  * the result is stored as float (CouplingStorage internal_real_t, :80). */
+/* --norm-of-mean-scoring (plmDCA.hpp:335-363 storage, 617-666 output).  The stored block
+ * of (r, n) is the float copy of J_r[n](a, s) (a: state at n, s: state at r), transposed
+ * when n < r (plmDCA.hpp:345-357: copy(..., transpose = n < r)). */
+void orc_nom_block(const double *x, int32_t L, int32_t q, int32_t r, int32_t n, float *M)
+{
+    for (int32_t a = 0; a < q; ++a)
+        for (int32_t s = 0; s < q; ++s) {
+            const float v = (float)x[orc_ref_index(L, q, n, a, s)];
+            if (n < r) M[s * q + a] = v;
+            else M[a * q + s] = v;
+        }
+}
+
+/* ising_gauge on a float matrix view (plmDCA_utility.hpp:31-54) with the float mean,
+ * repmat and element operators of Matrix_math.hpp: row means m1, column means m2 (mean of
+ * the transpose), mm the mean of m1; G[i][k] = ((M[i][k] - m1[k]) - m2[i]) + mm in float. */
+void orc_ising_gauge_float(const float *M, int32_t q, float *out)
+{
+    float m1[16], m2[16];
+    for (int32_t i = 0; i < q; ++i) {
+        float a = 0.0f;
+        for (int32_t j = 0; j < q; ++j) a = a + M[i * q + j];
+        m1[i] = a / (float)q;
+    }
+    float mm = 0.0f;
+    for (int32_t i = 0; i < q; ++i) mm += m1[i];
+    mm = mm / (float)q;
+    for (int32_t i = 0; i < q; ++i) {
+        float a = 0.0f;
+        for (int32_t j = 0; j < q; ++j) a = a + M[j * q + i];
+        m2[i] = a / (float)q;
+    }
+    for (int32_t i = 0; i < q; ++i)
+        for (int32_t k = 0; k < q; ++k) out[i * q + k] = ((M[i * q + k] - m1[k]) - m2[i]) + mm;
+}
+
+/* frobenius_norm(a, exclude) on float (Matrix_math.hpp:132-178): sum is float and takes
+ * std::pow(a, 2) (a double for a float argument) in each step: sum = float(sum + a*a). */
+float orc_frobenius_excl_float(const float *G, int32_t q, int32_t exclude)
+{
+    float sum = 0.0f;
+    for (int32_t i = 0; i < q; ++i) {
+        if (i == exclude) continue;
+        for (int32_t k = 0; k < q; ++k) {
+            if (k == exclude) continue;
+            const double a = (double)G[i * q + k];
+            sum = (float)((double)sum + a * a);
+        }
+    }
+    return sqrtf(sum);
+}
+
+/* One output line of plmDCA.hpp:646-663 for the stored blocks Mrn, Mnr:
+ * out = {|G(Mrn) + G(Mnr)| * 0.5 as the norm of the float mean, |G(Mrn)|, |G(Mnr)|}. */
+void orc_nom_pair(const float *Mrn, const float *Mnr, int32_t q, int32_t gap, float *out3)
+{
+    float Gij[256], Gji[256], Gm[256];
+    orc_ising_gauge_float(Mrn, q, Gij);
+    orc_ising_gauge_float(Mnr, q, Gji);
+    for (int32_t e = 0; e < q * q; ++e) Gm[e] = (Gij[e] + Gji[e]) * 0.5f;
+    out3[0] = orc_frobenius_excl_float(Gm, q, gap);
+    out3[1] = orc_frobenius_excl_float(Gij, q, gap);
+    out3[2] = orc_frobenius_excl_float(Gji, q, gap);
+}
+
 void orc_score_row(const double *x, int32_t L, int32_t q, int32_t r, int32_t gap, float *row)
 {
     double M[16 * 16], G[16 * 16];

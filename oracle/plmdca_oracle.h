@@ -18,6 +18,10 @@
  *                        of CppNumericalSolvers (not vendored) is parity unpinned
  *   orc_score_row        pinned: hand-computed 4x4 blocks, textbook gauge for symmetric
  *                        blocks, transpose and constant-shift invariance
+ *   orc_nom_*            pinned: a hand-computed block in exactly representable values,
+ *                        agreement with the FP64 gauge to float rounding, the mean of a block
+ *                        with itself, the triangle inequality, and the block orientation
+ *                        against the reference layout
  */
 #ifndef PLMDCA_ORACLE_H
 #define PLMDCA_ORACLE_H
@@ -96,6 +100,14 @@ void orc_score_row(const double *x, int32_t L, int32_t q, int32_t r, int32_t gap
 /* The gauged block itself (for tests): out[i*q+k]. */
 void orc_ising_gauge(const double *M, int32_t q, int32_t transpose, double *out);
 double orc_frobenius_excl(const double *G, int32_t q, int32_t exclude);
+
+/* --norm-of-mean-scoring (plmDCA.hpp:335-363, 617-666): the stored float block of the pair
+ * (r, n) from r's solution x (transposed when n < r); the float Ising gauge and norm; and
+ * the line values {norm of the mean of the two gauged blocks, norm of G(Mrn), of G(Mnr)}. */
+void orc_nom_block(const double *x, int32_t L, int32_t q, int32_t r, int32_t n, float *M);
+void orc_ising_gauge_float(const float *M, int32_t q, float *out);
+float orc_frobenius_excl_float(const float *G, int32_t q, int32_t exclude);
+void orc_nom_pair(const float *Mrn, const float *Mnr, int32_t q, int32_t gap, float *out3);
 
 /* Solve a list of loci, one locus per thread (the TBB grain-1 scheme of
  * plmDCA.hpp:582-586), cold start from zero unless x0 (K x n) is given.

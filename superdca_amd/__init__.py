@@ -45,7 +45,8 @@ EXPORTED = (
     "sdca_mi_topk", "sdca_pair_features", "sdca_mlp_init", "sdca_mlp_num_params", "sdca_mlp_get_params",
     "sdca_mlp_set_params", "sdca_mlp_forward", "sdca_mlp_loss_grad", "sdca_mlp_train", "sdca_solve_loci_mlp",
     "sdca_gemm_config", "sdca_write_couplings_top", "sdca_set_scan_split", "sdca_write_scan_couplings",
-    "sdca_reweight_noalloc",
+    "sdca_reweight_noalloc", "sdca_store_create", "sdca_store_destroy", "sdca_store_put_rows", "sdca_store_put_blocks",
+    "sdca_store_write_couplings", "sdca_store_write_norm_of_mean", "sdca_store_write_norm_of_mean_scan",
 )
 
 
@@ -128,6 +129,16 @@ def lib() -> C.CDLL:
     L.sdca_solve_loci_mlp.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
                                       C.c_void_p, C.c_void_p]
     L.sdca_gemm_config.argtypes = [C.c_void_p, C.c_void_p]
+    L.sdca_store_create.argtypes = [P(C.c_void_p), C.c_char_p, C.c_int64, C.c_int64, C.c_int32]
+    L.sdca_store_destroy.argtypes = [C.c_void_p]
+    L.sdca_store_put_rows.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]
+    L.sdca_store_put_blocks.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p]
+    L.sdca_store_write_couplings.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_void_p, C.c_int32, C.c_int64,
+                                             C.c_double, C.c_int32]
+    L.sdca_store_write_norm_of_mean.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                                C.c_int32]
+    L.sdca_store_write_norm_of_mean_scan.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                                     C.c_int32, C.c_int32]
     L.sdca_reweight_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_void_p]
     for name in ("sdca_ctx_create", "sdca_reweight", "sdca_set_weights", "sdca_eval_batch", "sdca_solve_loci",
@@ -135,7 +146,9 @@ def lib() -> C.CDLL:
                  "sdca_pair_count_table", "sdca_mi_topk", "sdca_pair_features", "sdca_mlp_init",
                  "sdca_mlp_get_params", "sdca_mlp_set_params", "sdca_mlp_forward", "sdca_mlp_loss_grad",
                  "sdca_mlp_train", "sdca_solve_loci_mlp", "sdca_gemm_config", "sdca_write_couplings_top",
-                 "sdca_set_scan_split", "sdca_write_scan_couplings", "sdca_reweight_noalloc"):
+                 "sdca_set_scan_split", "sdca_write_scan_couplings", "sdca_reweight_noalloc", "sdca_store_create",
+                 "sdca_store_put_rows", "sdca_store_put_blocks", "sdca_store_write_couplings",
+                 "sdca_store_write_norm_of_mean", "sdca_store_write_norm_of_mean_scan"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -356,6 +369,59 @@ class Context:
         _check(lib().sdca_gemm_config(self._h, cfg))
         return {"tuned": bool(cfg[0]), "ksplit": cfg[1], "kchunk": cfg[2], "gm_energy": cfg[3], "gm_grad": cfg[4],
                 "tn_energy": cfg[5], "tn_grad": cfg[6]}
+
+
+class Store:
+    """File-backed rows x cols entries of `entry` floats (sdca_store_*): the scores, or the
+    q x q coupling blocks of --norm-of-mean-scoring, streamed to the writers in tiles."""
+
+    def __init__(self, path: str, rows: int, cols: int, entry: int = 1):
+        h = C.c_void_p()
+        _check(lib().sdca_store_create(C.byref(h), path.encode(), int(rows), int(cols), int(entry)))
+        self._h, self.rows, self.cols, self.entry = h, rows, cols, entry
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sdca_store_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def put_rows(self, r0: int, rows: np.ndarray) -> None:
+        rows = np.ascontiguousarray(rows, dtype=np.float32)
+        _check(lib().sdca_store_put_rows(self._h, int(r0), rows.shape[0], _ptr(rows)))
+
+    def put_blocks(self, row: int, x: np.ndarray, L: int, q: int, r: int, col_loci) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        cl = np.ascontiguousarray(col_loci, dtype=np.int32)
+        _check(lib().sdca_store_put_blocks(self._h, int(row), _ptr(x), int(L), int(q), int(r), _ptr(cl)))
+
+    def write_couplings(self, path_all, path_top, translation, base=1, top_n=0, min_score=-np.inf, tile=0):
+        tr = np.ascontiguousarray(translation, dtype=np.int32)
+        _check(lib().sdca_store_write_couplings(self._h, None if path_all is None else str(path_all).encode(),
+                                                None if path_top is None else str(path_top).encode(), _ptr(tr),
+                                                int(base), int(top_n), float(min_score), int(tile)))
+
+    def write_norm_of_mean(self, path, translation, base=1, q=4, gap=None, tile=0):
+        tr = np.ascontiguousarray(translation, dtype=np.int32)
+        _check(lib().sdca_store_write_norm_of_mean(self._h, str(path).encode(), _ptr(tr), int(base), int(q),
+                                                   q - 1 if gap is None else int(gap), int(tile)))
+
+    def write_norm_of_mean_scan(self, path, translation1, translation2, base=1, q=4, gap=None):
+        t1 = np.ascontiguousarray(translation1, dtype=np.int32)
+        t2 = np.ascontiguousarray(translation2, dtype=np.int32)
+        _check(lib().sdca_store_write_norm_of_mean_scan(self._h, str(path).encode(), _ptr(t1), _ptr(t2), int(base),
+                                                        int(q), q - 1 if gap is None else int(gap)))
 
 
 def reweight_noalloc(d_states: int, B: int, L: int, q: int, theta: float, d_mult: int | None, d_int_w: int | None,

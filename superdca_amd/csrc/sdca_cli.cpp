@@ -45,6 +45,7 @@ struct Options {
     long begin_locus = 0, end_locus = -1;
     int device = 0, batch_loci = 128, history = 10, q_env = 4, max_iter = 2000;
     int precision = SDCA_PREC_FP32;
+    int chunk_loci = 0;  // loci per solve call (0: 4096, or fewer with --norm-of-mean-scoring)
 };
 
 const char *kUsage =
@@ -71,6 +72,8 @@ const char *kUsage =
     "  --no-reweighting               Do not reweight samples.\n"
     "  --reweighting-threshold arg (=0.9)\n"
     "  --output-weights               Write sample weights to file.\n"
+    "  --norm-of-mean-scoring         Score the mean of the two gauged couplings (six columns; stores\n"
+    "                                 every q x q block on disk: loci^2 x q^2 x 4 bytes).\n"
     "  --gradient-threshold arg (=0.001)\n"
     "  --coupling-gradient-threshold arg (=0)\n"
     "                                 Also require ||g||_inf < arg over the couplings J (0: off);\n"
@@ -83,6 +86,8 @@ const char *kUsage =
     "  --output-min-score arg         Also write every score >= arg, largest first (.top).\n"
     "GPU:\n"
     "  --device arg (=0)  --batch-loci arg (=128)  --history arg (=10)  --max-iter arg (=2000)\n"
+    "  --chunk-loci arg (=4096)       Loci per solve call; the scores stream through a file next to\n"
+    "                                 the output, so host memory is O(chunk x loci).\n"
     "  --precision fp32|split-bf16 (=fp32)  --q-env arg (=4; 5 keeps four alleles plus gap)\n";
 
 [[noreturn]] void die(const std::string &msg)
@@ -157,6 +162,7 @@ Options parse_args(int argc, char **argv)
         else if (a == "--end-locus") o.end_locus = std::atol(need(i).c_str());
         else if (a == "--device") o.device = std::atoi(need(i).c_str());
         else if (a == "--batch-loci") o.batch_loci = std::atoi(need(i).c_str());
+        else if (a == "--chunk-loci") o.chunk_loci = std::atoi(need(i).c_str());
         else if (a == "--history") o.history = std::atoi(need(i).c_str());
         else if (a == "--max-iter") o.max_iter = std::atoi(need(i).c_str());
         else if (a == "--q-env") o.q_env = std::atoi(need(i).c_str());
@@ -177,7 +183,10 @@ Options parse_args(int argc, char **argv)
     }
     if (o.alignments.size() > 2)  // SuperDCA.cpp:155-159
         die("cannot handle more than 2 alignments (got " + std::to_string(o.alignments.size()) + " filenames)");
-    if (o.norm_of_mean) die("--norm-of-mean-scoring is not supported");
+    if (o.norm_of_mean && (o.output_top > 0 || o.output_min_score > -HUGE_VAL))
+        die("--norm-of-mean-scoring writes its own six-column output; --output-top and --output-min-score rank "
+            "the default mean-of-norms scores only");
+    if (o.chunk_loci < 0) die("--chunk-loci must be >= 0");
     return o;
 }
 
@@ -335,6 +344,45 @@ int run_scan(const Options &o)
         log << "plmDCA: scan of " << loci.size() << " loci against " << L2 << " loci, n(effective)=" << lam[0]
             << "\n";
     const int K = (int)loci.size();
+    std::vector<int32_t> t1(K);
+    for (int k = 0; k < K; ++k) t1[k] = tr[0][loci[k]];
+    const std::string out =
+        unique_name(id + "_scan.SuperDCA_couplings." + std::to_string(o.output_base) + "-based.all");
+    if (o.norm_of_mean) {
+        // the q x q blocks of every (r, n), n over the second alignment, through a file-backed
+        // store (plmDCA.hpp:340-345,625-638)
+        const int q = o.q_env;
+        const size_t n = (size_t)L * q * q + q;
+        std::vector<int32_t> cols(L2);
+        for (int u = 0; u < L2; ++u) cols[u] = L1 + u;
+        sdca_store *store = nullptr;
+        if (!o.no_coupling_output &&
+            sdca_store_create(&store, unique_name(out + ".store").c_str(), K, L2, q * q) != SDCA_OK)
+            die(std::string("cannot create the block store: ") + sdca_last_error());
+        const int chunk = o.chunk_loci > 0 ? o.chunk_loci
+                                           : (int)std::max<size_t>(1, std::min<size_t>(1024, (size_t(256) << 20) / (n * 8)));
+        std::vector<double> xs((size_t)std::min(chunk, K) * n);
+        std::vector<sdca_locus_stats> stats(K);
+        for (int k0 = 0; k0 < K; k0 += chunk) {
+            const int c = std::min(chunk, K - k0);
+            const int rc = sdca_solve_loci(ctx, loci.data() + k0, c, nullptr, nullptr, xs.data(), stats.data() + k0);
+            if (rc != SDCA_OK && rc != SDCA_E_NOCONV) die(std::string("solve failed: ") + sdca_last_error());
+            if (rc == SDCA_E_NOCONV) std::cerr << "plmDCA warning: " << sdca_last_error() << "\n";
+            for (int k = 0; k < c && store; ++k)
+                if (sdca_store_put_blocks(store, k0 + k, xs.data() + (size_t)k * n, L, q, loci[k0 + k], cols.data()) !=
+                    SDCA_OK)
+                    die(std::string("cannot store coupling blocks: ") + sdca_last_error());
+        }
+        sdca_ctx_destroy(ctx);
+        if (store) {
+            if (o.verbose) log << "\nplmDCA: writing coupling values to file \"" << out << "\"\n";
+            if (sdca_store_write_norm_of_mean_scan(store, out.c_str(), t1.data(), tr[1].data(), o.output_base, q,
+                                                   q - 1) != SDCA_OK)
+                die(std::string("could not write the couplings: ") + sdca_last_error());
+            sdca_store_destroy(store);
+        }
+        return EXIT_SUCCESS;
+    }
     std::vector<float> rows((size_t)K * L);
     std::vector<sdca_locus_stats> stats(K);
     const int rc = sdca_solve_loci(ctx, loci.data(), K, nullptr, rows.data(), nullptr, stats.data());
@@ -343,13 +391,8 @@ int run_scan(const Options &o)
     sdca_ctx_destroy(ctx);
     if (!o.no_coupling_output) {
         std::vector<float> S((size_t)K * L2);
-        std::vector<int32_t> t1(K);
-        for (int k = 0; k < K; ++k) {
-            t1[k] = tr[0][loci[k]];
+        for (int k = 0; k < K; ++k)
             std::memcpy(S.data() + (size_t)k * L2, rows.data() + (size_t)k * L + L1, (size_t)L2 * 4);
-        }
-        const std::string out =
-            unique_name(id + "_scan.SuperDCA_couplings." + std::to_string(o.output_base) + "-based.all");
         if (o.verbose) log << "\nplmDCA: writing coupling values to file \"" << out << "\"\n";
         if (sdca_write_scan_couplings(out.c_str(), S.data(), K, L2, t1.data(), tr[1].data(), o.output_base) !=
             SDCA_OK)
@@ -485,13 +528,55 @@ int main(int argc, char **argv)
     }
     if (o.verbose) log << "\nplmDCA: let's learn!\n";
     const int K = (int)loci.size();
-    std::vector<float> rows((size_t)K * L);
+    const int q = o.q_env;
+    const size_t n = (size_t)L * q * q + q;  // parameters per locus (reference layout)
+    const bool want_top = o.output_top > 0 || o.output_min_score > -HUGE_VAL;
+    const bool want_all = !o.no_coupling_output;
+    const bool nom = o.norm_of_mean;
+    const bool keep = want_all || want_top;
+    const std::string stem = id + ".SuperDCA_couplings." + std::to_string(o.output_base) + "-based";
+    // the scores (or the coupling blocks) of the listed loci, row and column = list position
+    // (plmDCA.hpp:684-691), go to a file-backed store chunk by chunk
+    sdca_store *store = nullptr;
+    if (keep) {
+        const std::string sp = unique_name(stem + ".store");
+        if (sdca_store_create(&store, sp.c_str(), K, K, nom ? q * q : 1) != SDCA_OK)
+            die(std::string("cannot create the score store: ") + sdca_last_error());
+    }
+    const int chunk = o.chunk_loci > 0 ? o.chunk_loci
+                                       : (nom ? (int)std::max<size_t>(1, std::min<size_t>(1024, (size_t(256) << 20) / (n * 8)))
+                                              : 4096);
+    std::vector<float> rows(keep && !nom ? (size_t)std::min(chunk, K) * L : 0);
+    std::vector<float> band(keep && !nom ? (size_t)std::min(chunk, K) * K : 0);
+    std::vector<double> xs(keep && nom ? (size_t)std::min(chunk, K) * n : 0);
     std::vector<sdca_locus_stats> st(K);
     const auto t_solve = std::chrono::steady_clock::now();
-    const int rc = sdca_solve_loci(ctx, loci.data(), K, nullptr, rows.data(), nullptr, st.data());
+    int nwarn = 0;
+    for (int k0 = 0; k0 < K; k0 += chunk) {
+        const int c = std::min(chunk, K - k0);
+        const int rc = sdca_solve_loci(ctx, loci.data() + k0, c, nullptr, rows.empty() ? nullptr : rows.data(),
+                                       xs.empty() ? nullptr : xs.data(), st.data() + k0);
+        if (rc != SDCA_OK && rc != SDCA_E_NOCONV) die(std::string("solve failed: ") + sdca_last_error());
+        if (rc == SDCA_E_NOCONV) ++nwarn;
+        if (!rows.empty()) {
+            for (int k = 0; k < c; ++k)
+                for (int u = 0; u < K; ++u) band[(size_t)k * K + u] = rows[(size_t)k * L + loci[u]];
+            if (sdca_store_put_rows(store, k0, c, band.data()) != SDCA_OK)
+                die(std::string("cannot store scores: ") + sdca_last_error());
+        }
+        if (!xs.empty())
+            for (int k = 0; k < c; ++k)
+                if (sdca_store_put_blocks(store, k0 + k, xs.data() + (size_t)k * n, L, q, loci[k0 + k], loci.data()) !=
+                    SDCA_OK)
+                    die(std::string("cannot store coupling blocks: ") + sdca_last_error());
+        if (o.verbose && K > chunk) log << "plmDCA: " << k0 + c << " / " << K << " loci solved\n";
+    }
     const double solve_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_solve).count();
-    if (rc != SDCA_OK && rc != SDCA_E_NOCONV) die(std::string("solve failed: ") + sdca_last_error());
-    if (rc == SDCA_E_NOCONV) std::cerr << "plmDCA warning: " << sdca_last_error() << "\n";
+    if (nwarn) {
+        int nc = 0;
+        for (const auto &x : st) nc += x.status == SDCA_ST_ITER_CAP || x.status == SDCA_ST_LS_FAIL;
+        std::cerr << "plmDCA warning: " << nc << " loci stopped without reaching the gradient threshold\n";
+    }
     if (o.verbose) {
         for (int k = 0; k < K; ++k)
             log << "  " << k + 1 << " / " << K << " (out of " << L << ") locus=" << translation[loci[k]] + 1
@@ -499,29 +584,19 @@ int main(int argc, char **argv)
                 << " nfeval=" << st[k].nfeval << " iter=" << st[k].iter << "\n";
         log << "plmDCA: solved " << K << " loci in " << solve_s << " s (" << K / solve_s << " loci/s)\n";
     }
-    const bool want_top = o.output_top > 0 || o.output_min_score > -HUGE_VAL;
-    if (!o.no_coupling_output || want_top) {
-        // scores among the listed loci, row = list position (plmDCA.hpp:684-691)
-        std::vector<float> S((size_t)K * K);
+    if (store) {
         std::vector<int32_t> tr(K);
-        for (int k = 0; k < K; ++k) {
-            tr[k] = translation[loci[k]];
-            for (int u = 0; u < K; ++u) S[(size_t)k * K + u] = rows[(size_t)k * L + loci[u]];
-        }
-        const std::string stem = id + ".SuperDCA_couplings." + std::to_string(o.output_base) + "-based";
-        if (!o.no_coupling_output) {
-            const std::string out = unique_name(stem + ".all");
-            if (o.verbose) log << "\nplmDCA: writing coupling values to file \"" << out << "\"\n";
-            if (sdca_write_couplings(out.c_str(), S.data(), K, tr.data(), o.output_base) != SDCA_OK)
-                die("could not write \"" + out + "\"");
-        }
-        if (want_top) {
-            const std::string out = unique_name(stem + ".top");
-            if (o.verbose) log << "plmDCA: writing the largest coupling values to file \"" << out << "\"\n";
-            if (sdca_write_couplings_top(out.c_str(), S.data(), K, tr.data(), o.output_base, o.output_top,
-                                         o.output_min_score) != SDCA_OK)
-                die("could not write \"" + out + "\"");
-        }
+        for (int k = 0; k < K; ++k) tr[k] = translation[loci[k]];
+        const std::string out_all = want_all ? unique_name(stem + ".all") : std::string();
+        const std::string out_top = want_top ? unique_name(stem + ".top") : std::string();
+        if (o.verbose && want_all) log << "\nplmDCA: writing coupling values to file \"" << out_all << "\"\n";
+        if (o.verbose && want_top) log << "plmDCA: writing the largest coupling values to file \"" << out_top << "\"\n";
+        const int rc = nom ? sdca_store_write_norm_of_mean(store, out_all.c_str(), tr.data(), o.output_base, q, q - 1, 0)
+                           : sdca_store_write_couplings(store, want_all ? out_all.c_str() : nullptr,
+                                                        want_top ? out_top.c_str() : nullptr, tr.data(), o.output_base,
+                                                        o.output_top, o.output_min_score, 0);
+        if (rc != SDCA_OK) die(std::string("could not write the couplings: ") + sdca_last_error());
+        sdca_store_destroy(store);
     }
     sdca_ctx_destroy(ctx);
     if (o.verbose) {

@@ -196,7 +196,9 @@ def test_cli_end_to_end(tmp_path):
     fa = tmp_path / "toy.fasta"
     synth.write_fasta(str(fa), d["chars"])
     exe = os.path.join(ROOT, "superdca_amd", "sdca")
-    p = subprocess.run([exe, "-v", str(fa)], capture_output=True, text=True, cwd=tmp_path, timeout=300)
+    # chunks of 37 loci: the scores reach the file-backed store from several solve calls
+    p = subprocess.run([exe, "-v", "--chunk-loci", "37", str(fa)], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=300)
     assert p.returncode == 0, p.stderr
     out = tmp_path / "toy.SuperDCA_couplings.1-based.all"
     lines = out.read_text().splitlines()

@@ -230,8 +230,13 @@ def test_cli_help_and_errors(tmp_path):
     (tmp_path / "cols.txt").write_text("1,9\n")
     p = subprocess.run([exe, "--filterlistfile", str(tmp_path / "cols.txt"), str(fa)], capture_output=True, text=True)
     assert p.returncode != 0 and "filter list index out of range" in p.stderr
-    p = subprocess.run([exe, "--norm-of-mean-scoring", str(fa)], capture_output=True, text=True)
-    assert p.returncode != 0 and "not supported" in p.stderr
+    # --norm-of-mean-scoring writes its own six columns: the top-score output does not apply
+    p = subprocess.run([exe, "--norm-of-mean-scoring", "--output-top", "10", str(fa)], capture_output=True, text=True)
+    assert p.returncode != 0 and "six-column" in p.stderr
+    p = subprocess.run([exe, "--chunk-loci", "-3", str(fa)], capture_output=True, text=True)
+    assert p.returncode != 0 and "--chunk-loci" in p.stderr
+    p = subprocess.run([exe, "--help"], capture_output=True, text=True)
+    assert "--norm-of-mean-scoring" in p.stdout and "--coupling-gradient-threshold" in p.stdout
 
 
 def test_context_without_gpu_fails_loudly():

@@ -344,3 +344,53 @@ def test_mlp_reference_gradient_finite_differences():
     # the loss is the masked mean squared error
     y, _ = mlp_ref.forward(p, x, din, dh, dout)
     assert loss == pytest.approx((((y - t) * mask) ** 2).sum() / mask.sum(), rel=1e-14)
+
+
+# ---------------------------------------------------------- norm of mean (float path)
+
+def test_float_gauge_hand_computed_block():
+    # quarters are exact in float: row means (0.75, 0.25, 0, 1), column means
+    # (0.25, 0.75, 0, 1), their mean 0.5; G[i][k] = M[i][k] - rowmean[k] - colmean[i] + 0.5
+    M = np.array([[1, 2, 0, 0], [0, 1, 0, 0], [0, 0, 0, 0], [0, 0, 0, 4]], dtype=np.float32)
+    want = np.array([[0.5, 2.0, 0.25, -0.75],
+                     [-1.0, 0.5, -0.25, -1.25],
+                     [-0.25, 0.25, 0.5, -0.5],
+                     [-1.25, -0.75, -0.5, 2.5]], dtype=np.float32)
+    np.testing.assert_array_equal(orc.ising_gauge_float(M), want)
+    # gap (state 3) excluded: (0.25 + 4 + 0.0625) + (1 + 0.25 + 0.0625) + (0.0625 + 0.0625 + 0.25) = 6
+    out = orc.nom_pair(M, M)
+    assert out[1] == np.float32(np.sqrt(np.float32(6.0)))
+
+
+@pytest.mark.parametrize("q", [3, 4, 5])
+def test_float_gauge_agrees_with_the_fp64_gauge(q):
+    rng = np.random.default_rng(q)
+    for _ in range(50):
+        M = rng.normal(scale=0.1, size=(q, q)).astype(np.float32)
+        g32 = orc.ising_gauge_float(M).astype(np.float64)
+        g64 = orc.ising_gauge(M.astype(np.float64))
+        assert np.abs(g32 - g64).max() <= 8 * np.finfo(np.float32).eps * max(1.0, np.abs(M).max())
+
+
+def test_norm_of_mean_invariants():
+    rng = np.random.default_rng(2)
+    for _ in range(100):
+        A = rng.normal(scale=0.1, size=(4, 4)).astype(np.float32)
+        B = rng.normal(scale=0.1, size=(4, 4)).astype(np.float32)
+        same = orc.nom_pair(A, A)
+        assert same[0] == same[1] == same[2]  # the mean of a block with itself is the block
+        jm, jij, jji = (float(v) for v in orc.nom_pair(A, B))
+        assert jm <= 0.5 * (jij + jji) * (1 + 1e-6)  # the norm of a mean is at most the mean norm
+        assert (jij, jji) == tuple(float(v) for v in orc.nom_pair(B, A)[[2, 1]])
+
+
+def test_nom_block_orientation():
+    # J_r[n](a=1, s=2) = 7 (a: state at n, s: state at r) lands at [state at the larger
+    # locus][state at the smaller] of the stored block, for n above and below r
+    L, q = 20, 4
+    for r, n in ((3, 17), (17, 3)):
+        x = np.zeros(orc.nparams(L, q))
+        x[orc.ref_index(L, q, n, 1, 2)] = 7.0
+        M = orc.nom_block(x, L, q, r, n)
+        big, small = (1, 2) if n > r else (2, 1)  # (state at the larger, at the smaller)
+        assert M[big, small] == 7.0 and np.count_nonzero(M) == 1
