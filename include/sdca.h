@@ -203,6 +203,25 @@ int sdca_solve_loci(sdca_ctx *ctx, const int32_t *loci, int32_t K, const double 
  * scan off. */
 int sdca_set_scan_split(sdca_ctx *ctx, int32_t n_first);
 
+/* ---- several alignments in one batched solve ----
+ * SuperDCA solves the loci of one alignment per run (plmDCA.hpp:582-586).  A multi-solve
+ * takes loci of several contexts (alignments of any B, L and q_env, each with its weights
+ * installed, on one device) into the same slots: every round's kernels cover the loci of
+ * all the alignments in flight, each GEMM launch with one group of tiles per alignment.
+ * The contexts must outlive the multi-solve.  opts: the solver options (precision,
+ * batch_loci, history, thresholds, coarse and fine levels); NULL = defaults. */
+typedef struct sdca_multi sdca_multi;
+int sdca_multi_create(sdca_multi **out, sdca_ctx *const *ctxs, int32_t n, const sdca_opts *opts);
+void sdca_multi_destroy(sdca_multi *m);
+
+/* Solves item k = locus loci[k] of context alns[k] (any order, alignments mixed) with the
+ * batched L-BFGS of sdca_solve_loci, cold start.  Score row k (the L of its alignment)
+ * goes to score_rows + row_off[k] (row_off NULL: rows packed in item order); score_rows
+ * and st may be NULL.  SDCA_E_NOCONV as sdca_solve_loci. */
+int sdca_multi_solve(sdca_multi *m, const int32_t *alns, const int32_t *loci, int32_t K, float *score_rows,
+                     const int64_t *row_off, sdca_locus_stats *st);
+int sdca_multi_get_timing(sdca_multi *m, sdca_timing *t, int32_t reset);
+
 /* Timings accumulated since the context was created or last reset (no counterpart in
  * SuperDCA beyond its per-locus log, plmDCA.hpp:306-325). */
 int sdca_get_timing(sdca_ctx *ctx, sdca_timing *t, int32_t reset);

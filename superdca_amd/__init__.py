@@ -47,6 +47,7 @@ EXPORTED = (
     "sdca_gemm_config", "sdca_write_couplings_top", "sdca_set_scan_split", "sdca_write_scan_couplings",
     "sdca_reweight_noalloc", "sdca_store_create", "sdca_store_destroy", "sdca_store_put_rows", "sdca_store_put_blocks",
     "sdca_store_write_couplings", "sdca_store_write_norm_of_mean", "sdca_store_write_norm_of_mean_scan",
+    "sdca_multi_create", "sdca_multi_destroy", "sdca_multi_solve", "sdca_multi_get_timing",
 )
 
 
@@ -139,6 +140,10 @@ def lib() -> C.CDLL:
                                                 C.c_int32]
     L.sdca_store_write_norm_of_mean_scan.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_void_p, C.c_int32,
                                                      C.c_int32, C.c_int32]
+    L.sdca_multi_create.argtypes = [P(C.c_void_p), C.c_void_p, C.c_int32, P(Opts)]
+    L.sdca_multi_destroy.argtypes = [C.c_void_p]
+    L.sdca_multi_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.sdca_multi_get_timing.argtypes = [C.c_void_p, P(Timing), C.c_int32]
     L.sdca_reweight_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_void_p]
     for name in ("sdca_ctx_create", "sdca_reweight", "sdca_set_weights", "sdca_eval_batch", "sdca_solve_loci",
@@ -148,7 +153,8 @@ def lib() -> C.CDLL:
                  "sdca_mlp_train", "sdca_solve_loci_mlp", "sdca_gemm_config", "sdca_write_couplings_top",
                  "sdca_set_scan_split", "sdca_write_scan_couplings", "sdca_reweight_noalloc", "sdca_store_create",
                  "sdca_store_put_rows", "sdca_store_put_blocks", "sdca_store_write_couplings",
-                 "sdca_store_write_norm_of_mean", "sdca_store_write_norm_of_mean_scan"):
+                 "sdca_store_write_norm_of_mean", "sdca_store_write_norm_of_mean_scan", "sdca_multi_create",
+                 "sdca_multi_solve", "sdca_multi_get_timing"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -369,6 +375,57 @@ class Context:
         _check(lib().sdca_gemm_config(self._h, cfg))
         return {"tuned": bool(cfg[0]), "ksplit": cfg[1], "kchunk": cfg[2], "gm_energy": cfg[3], "gm_grad": cfg[4],
                 "tn_energy": cfg[5], "tn_grad": cfg[6]}
+
+
+class Multi:
+    """Several alignments (Contexts with their weights installed) solved in one batched
+    solve (sdca_multi_*): every round's kernels cover the loci of all alignments in flight."""
+
+    def __init__(self, contexts, **opts):
+        self.contexts = list(contexts)  # kept alive: the solve reads their device data
+        self.opts = default_opts(**opts)
+        arr = (C.c_void_p * len(self.contexts))(*[c._h.value for c in self.contexts])
+        h = C.c_void_p()
+        _check(lib().sdca_multi_create(C.byref(h), C.cast(arr, C.c_void_p), len(self.contexts), C.byref(self.opts)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sdca_multi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def solve(self, alns, loci):
+        """-> (list of score rows, one per item, each the L of its alignment; stats; rc)."""
+        alns = np.ascontiguousarray(alns, dtype=np.int32)
+        loci = np.ascontiguousarray(loci, dtype=np.int32)
+        K = loci.size
+        Ls = np.array([self.contexts[a].L for a in alns], dtype=np.int64)
+        off = np.concatenate([[0], np.cumsum(Ls)]).astype(np.int64)
+        S = np.zeros(int(off[-1]), dtype=np.float32)
+        st = (LocusStats * max(K, 1))()
+        rc = _check(lib().sdca_multi_solve(self._h, _ptr(alns), _ptr(loci), K, _ptr(S), _ptr(off[:-1].copy()),
+                                           C.cast(st, C.c_void_p)), allow=(SDCA_E_NOCONV,))
+        rows = [S[off[k]:off[k + 1]] for k in range(K)]
+        stats = {k: np.array([getattr(st[i], k) for i in range(K)]) for k in ("fval", "gnorm", "nfeval", "iter",
+                                                                               "status", "gnorm_J", "nfeval_fine")}
+        return rows, stats, rc
+
+    def timing(self, reset: bool = False) -> dict:
+        t = Timing()
+        _check(lib().sdca_multi_get_timing(self._h, C.byref(t), int(reset)))
+        return {name: getattr(t, name) for name, _ in Timing._fields_}
 
 
 class Store:

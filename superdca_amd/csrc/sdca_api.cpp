@@ -130,8 +130,12 @@ void sdca_ctx::free_workspace()
     dfree(d_rows);
     dfree(d_src_off);
     dfree(d_score_loci);
+    dfree(d_score_ld);
+    dfree(d_score_aln);
+    dfree(d_aln);
+    dfree(d_groups);
     for (void *h : {(void *)h_cmds, (void *)h_res, (void *)h_rows, (void *)h_src_off, (void *)h_score_loci,
-                    (void *)h_wl})
+                    (void *)h_wl, (void *)h_score_ld, (void *)h_score_aln, (void *)h_groups})
         if (h) (void)hipHostFree(h);
     h_cmds = nullptr;
     h_res = nullptr;
@@ -139,6 +143,10 @@ void sdca_ctx::free_workspace()
     h_src_off = nullptr;
     h_score_loci = nullptr;
     h_wl = nullptr;
+    h_score_ld = nullptr;
+    h_score_aln = nullptr;
+    h_groups = nullptr;
+    max_groups = 0;
     ws_slots = 0;
     ws_m = 0;
     ws_groups = 0;
@@ -246,14 +254,16 @@ void store_config(const GemmKey &k, const GemmConfig &g)
     }
 }
 
-// Split-K chunk for a requested number of slices (whole BK steps per slice).
-void set_split(const sdca_ctx *c, int64_t ks, GemmConfig &g)
+// Split-K chunk of a reduction of length K for a requested number of slices (whole BK
+// steps per slice).
+void set_split(const sdca_ctx *c, int64_t K, int64_t ks, GemmConfig &g)
 {
     const int BK = c->opts.precision == SDCA_PREC_SPLIT_BF16 ? 64 : 32;
-    ks = std::max<int64_t>(1, std::min<int64_t>(ks, c->Kxp / BK));
-    g.kchunk = (int)round_up((c->Kxp + ks - 1) / ks, BK);
-    g.ksplit = (int)((c->Kxp + g.kchunk - 1) / g.kchunk);
+    ks = std::max<int64_t>(1, std::min<int64_t>(ks, K / BK));
+    g.kchunk = (int)round_up((K + ks - 1) / ks, BK);
+    g.ksplit = (int)((K + g.kchunk - 1) / g.kchunk);
 }
+void set_split(const sdca_ctx *c, int64_t ks, GemmConfig &g) { set_split(c, c->Kxp, ks, g); }
 
 // Default rule: >= 2048 energy workgroups (128-wide tiles) with the number of
 // super-tiles a multiple of the eight XCDs, super-tiles gm tiles high.
@@ -264,7 +274,8 @@ GemmConfig default_config(const sdca_ctx *c, int gm, int tn_e = 128)
     g.tn_e = tn_e;
     const int64_t tiles = (int64_t)(c->Mg / sdca::kTile) * (c->Bp / sdca::kTile);
     int64_t ks = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
-    const int64_t sup = sdca::kern::energy_superiles(c->Mg, c->Bp, gm, tn_e);
+    const sdca::TileMap tm1 = sdca::make_map(c->Mg / sdca::kTile, c->Bp / tn_e, 1, gm);
+    const int64_t sup = (int64_t)tm1.MB * tm1.NB;
     while ((ks * sup) % 8 != 0 && ks < 64) ++ks;
     set_split(c, ks, g);
     return g;
@@ -293,9 +304,87 @@ std::vector<GemmConfig> tuning_candidates(const sdca_ctx *c)
 
 float elapsed(hipEvent_t a, hipEvent_t b);
 
+// The GEMM group of member a over the commands of parts P (packed rows [P.row[0], rend)):
+// the energy contraction against its one-hot X (K = Kxp, split-K ksplit x kchunk) or the
+// gradient contraction against X^T (K = Bp).
+sdca::GemmGroup make_group(const sdca_ctx *c, int a, const sdca::Parts &P, int rend, bool energy, int ksplit,
+                           int kchunk, int gm, int tn)
+{
+    const sdca_ctx *m = c->members[a];
+    sdca::GemmGroup G;
+    std::memset(&G, 0, sizeof(G));
+    G.aln = a;
+    G.q = m->q;
+    G.parts = P;
+    const int Mt = (rend - P.row[0]) / sdca::kTile;
+    if (energy) {
+        G.K = (int)m->Kxp;
+        G.kchunk = kchunk;
+        G.Bbits = m->d_X;
+        G.ldbw = m->Kxp / 32;
+        G.tm = sdca::make_map(Mt, m->Bp / tn, ksplit, gm);
+    } else {
+        G.K = m->Bp;
+        G.kchunk = m->Bp;
+        G.Bbits = m->d_Xt;
+        G.ldbw = m->Bp / 32;
+        G.tm = sdca::make_map(Mt, (int)(m->Kxp / tn), 1, gm);
+    }
+    return G;
+}
+
+// Lays the groups of a launch out one after the other; returns the launch's workgroups.
+int place_groups(sdca::GemmGroup *gs, int ng)
+{
+    int grid = 0;
+    for (int k = 0; k < ng; ++k) {
+        gs[k].tile0 = grid;
+        grid += sdca::map_grid(gs[k].tm);
+    }
+    return grid;
+}
+
+// Uploads the energy groups (ne) and gradient groups (ngr) of slot group g's round; they
+// land at d_groups + g * 2 * max_groups (energy) and + max_groups (gradient).
+int upload_groups(sdca_ctx *c, int g, int ne, int ngr, hipStream_t st)
+{
+    sdca::GemmGroup *h = c->h_groups + (size_t)g * 2 * c->max_groups;
+    sdca::GemmGroup *d = c->d_groups + (size_t)g * 2 * c->max_groups;
+    HIP_TRY(hipMemcpyAsync(d, h, (size_t)ne * sizeof(sdca::GemmGroup), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(d + c->max_groups, h + c->max_groups, (size_t)ngr * sizeof(sdca::GemmGroup),
+                           hipMemcpyHostToDevice, st));
+    return SDCA_OK;
+}
+
+// The members' descriptors, uploaded before a solve (weights and lambda may have changed).
+int upload_alns(sdca_ctx *c)
+{
+    std::vector<sdca::AlnDev> v(c->members.size());
+    for (size_t a = 0; a < c->members.size(); ++a) {
+        const sdca_ctx *m = c->members[a];
+        sdca::AlnDev &d = v[a];
+        d.X = m->d_X;
+        d.Xt = m->d_Xt;
+        d.statesT = m->d_statesT;
+        d.w = m->d_w;
+        d.state_col = m->d_state_col;
+        d.B = m->B;
+        d.Bp = m->Bp;
+        d.L = m->L;
+        d.q = m->q;
+        d.gap = m->gap;
+        d.Kx = (int)m->Kx;
+        d.Kxp = (int)m->Kxp;
+        d.lamJ2 = (float)(2.0 * m->lambda_J);
+        d.lamh2 = 2.0 * m->lambda_h;
+    }
+    HIP_TRY(hipMemcpy(c->d_aln, v.data(), v.size() * sizeof(sdca::AlnDev), hipMemcpyHostToDevice));
+    return SDCA_OK;
+}
+
 // Times the candidates on the workspace's own buffers (the first slot group, nslots
 // slots) with synthetic slot commands (every slot active, writing its first gradient) and
-// leaves the buffers zeroed again.
+// leaves the buffers zeroed again.  A context solving its own alignment only.
 int autotune(sdca_ctx *c, int nslots, const std::vector<GemmConfig> &cand, GemmConfig &best)
 {
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
@@ -311,10 +400,13 @@ int autotune(sdca_ctx *c, int nslots, const std::vector<GemmConfig> &cand, GemmC
         cmds[i].slot = i;
         cmds[i].row = i * c->q;
         cmds[i].level = sdca::LV_SPLIT;
+        cmds[i].aln = 0;
+        cmds[i].q = c->q;
+        cmds[i].Kx = (int)c->Kx;
+        cmds[i].Kxp = (int)c->Kxp;
     }
     // every command at the split level: the fine part is empty, the coarse part starts at Mg
     const sdca::Parts parts{{0, 0, c->Mg}, {0, 0, nslots, nslots}};
-    HIP_TRY(hipMemcpyAsync(c->d_cmds, cmds.data(), nslots * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     const size_t nw = (size_t)c->Mpad * c->Kxp;
     HIP_TRY(hipMemsetD32Async((hipDeviceptr_t)c->d_Xtr, 0x3c23d70au, nw, st));  // 0.01f
     if (split) {
@@ -322,17 +414,33 @@ int autotune(sdca_ctx *c, int nslots, const std::vector<GemmConfig> &cand, GemmC
         HIP_TRY(hipMemsetD16Async((hipDeviceptr_t)c->d_Wlo, 0x3700u, nw, st));
     }
     const int reps = 3;
-    auto energy = [&](const GemmConfig &g) {
-        sdca::kern::gemm_energy(st, c->opts.precision, parts, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_Wl3, c->d_X, c->Mg,
-                                c->Mg, c->Bp, (int)c->Kxp, g.ksplit, g.kchunk, g.gm_e, g.tn_e, c->d_Epart);
-        sdca::kern::softmax(st, c->opts.precision, c->d_Epart, g.ksplit, c->Mg, c->Bp, c->B, c->q, nslots,
-                            c->d_cmds, c->d_statesT, c->d_w, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Gl3, c->d_sm_part,
-                            c->d_hpart, c->nbchunk);
+    sdca::GemmGroup *hg = c->h_groups, *dg = c->d_groups;
+    int grid_e = 0, grid_g = 0;
+    auto set_energy = [&](const GemmConfig &g) -> int {
+        for (int i = 0; i < nslots; ++i) cmds[i].ksplit = g.ksplit;
+        HIP_TRY(hipStreamSynchronize(st));  // the pinned staging is rewritten
+        HIP_TRY(hipMemcpyAsync(c->d_cmds, cmds.data(), nslots * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
+        hg[0] = make_group(c, 0, parts, c->Mg, true, g.ksplit, g.kchunk, g.gm_e, g.tn_e);
+        grid_e = place_groups(hg, 1);
+        return upload_groups(c, 0, 1, 0, st);
     };
-    auto grad = [&](int gm, int tn) {
-        sdca::kern::gemm_grad(st, c->opts.precision, parts, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Gl3, c->d_Xt, c->Mg,
-                              c->Bp, (int)c->Kxp, (int)c->Kx, c->q, c->d_cmds, c->d_Xtr, c->d_gbuf, c->n_slot,
-                              2.0 * c->lambda_J, 2.0 * c->lambda_h, c->d_hpart, c->nbchunk, gm, tn);
+    auto set_grad = [&](int gm, int tn) -> int {
+        HIP_TRY(hipStreamSynchronize(st));
+        hg[c->max_groups] = make_group(c, 0, parts, c->Mg, false, 1, 0, gm, tn);
+        grid_g = place_groups(hg + c->max_groups, 1);
+        return upload_groups(c, 0, 0, 1, st);
+    };
+    const int64_t ldx = c->ws_ldx, ldb = c->ws_ldb;
+    auto energy = [&](const GemmConfig &g) {
+        sdca::kern::gemm_energy(st, c->opts.precision, dg, 1, grid_e, g.tn_e, c->d_Xtr, c->d_Whi, c->d_Wlo, c->d_Wl3,
+                                ldx, ldb, (int64_t)c->Mg * ldb, c->d_Epart);
+        sdca::kern::softmax(st, c->opts.precision, c->d_Epart, (int64_t)c->Mg * ldb, ldb, nslots, c->nbchunk,
+                            c->d_cmds, c->d_aln, c->d_G32, c->d_Ghi, c->d_Glo, c->d_Gl3, c->d_sm_part, c->d_hpart);
+    };
+    auto grad = [&](int tn) {
+        sdca::kern::gemm_grad(st, c->opts.precision, dg + c->max_groups, 1, grid_g, tn, c->d_G32, c->d_Ghi,
+                              c->d_Glo, c->d_Gl3, ldb, c->d_aln, c->d_cmds, c->d_Xtr, ldx, c->d_gbuf, c->n_slot,
+                              c->d_hpart, c->nbchunk);
     };
     auto time = [&](auto &&launch) -> float {
         launch();
@@ -343,8 +451,10 @@ int autotune(sdca_ctx *c, int nslots, const std::vector<GemmConfig> &cand, GemmC
         HIP_TRY(hipEventSynchronize(c->ev[7]));
         return elapsed(c->ev[6], c->ev[7]) / reps;
     };
+    int rc;
     float bt = 1e30f;
     for (const auto &g : cand) {
+        if ((rc = set_energy(g))) return rc;
         const float t = time([&] { energy(g); });
         if (t < 0) return SDCA_E_HIP;
         if (c->opts.verbose)
@@ -358,13 +468,15 @@ int autotune(sdca_ctx *c, int nslots, const std::vector<GemmConfig> &cand, GemmC
             best.tn_e = g.tn_e;
         }
     }
+    if ((rc = set_energy(best))) return rc;
     energy(best);  // node beliefs G for the gradient candidates
     bt = 1e30f;
     const int Mt = c->Mg / sdca::kTile;
     for (int tn : {128, 256})
         for (int gm = 1; gm <= 8; gm *= 2) {
             if (gm > 1 && gm / 2 >= Mt) break;
-            const float t = time([&] { grad(gm, tn); });
+            if ((rc = set_grad(gm, tn))) return rc;
+            const float t = time([&] { grad(tn); });
             if (c->opts.verbose) std::fprintf(stderr, "sdca tune: grad tn %d gm %d: %.3f ms\n", tn, gm, t);
             if (t < bt) {
                 bt = t;
@@ -396,23 +508,47 @@ inline size_t pair_bytes(const sdca_ctx *c) { return c->opts.precision == SDCA_P
 // kernels compete for the same CUs, and at the C4 shape one group is faster.
 int choose_groups(const sdca_ctx *c, int nslots)
 {
-    if (c->opts.pipeline_groups != 2) return 1;
+    if (c->opts.pipeline_groups != 2 || c->members.size() != 1) return 1;
     const bool even = nslots % 2 == 0 && ((int64_t)(nslots / 2) * c->q) % sdca::kTile == 0;
     return even ? 2 : 1;
 }
 
-// Allocates the batch workspace for nslots concurrent loci and history m.
+// A context solving its own alignment only (not the internal context of a multi-alignment
+// solve): its workspace dimensions are its own, its GEMM shapes can be tuned.
+bool solves_itself(const sdca_ctx *c) { return c->members.size() == 1 && c->members[0] == c; }
+
+// Allocates the batch workspace for nslots concurrent loci and history m, sized for the
+// largest member alignment.
 int ensure_workspace(sdca_ctx *c, int nslots, int m)
 {
     const int ng = choose_groups(c, nslots);
-    if (c->ws_slots == nslots && c->ws_m == m && c->ws_groups == ng) return SDCA_OK;
+    int wq = 0, wldb = 0, wL = 0;
+    int64_t wldx = 0, nsl = 0;
+    for (const sdca_ctx *a : c->members) {
+        wq = std::max(wq, a->q);
+        wldb = std::max(wldb, a->Bp);
+        wL = std::max(wL, a->L);
+        wldx = std::max(wldx, a->Kxp);
+        nsl = std::max(nsl, (int64_t)a->q * a->Kxp);
+    }
+    const int nmem = (int)c->members.size();
+    if (c->ws_slots == nslots && c->ws_m == m && c->ws_groups == ng && c->n_slot == nsl && c->ws_q == wq &&
+        c->ws_ldx == wldx && c->ws_ldb == wldb && c->ws_L == wL && c->max_groups == nmem)
+        return SDCA_OK;
     c->free_workspace();
-    c->n_slot = (int64_t)c->q * c->Kxp;
+    const bool own = solves_itself(c);
+    c->n_slot = nsl;
+    c->ws_q = wq;
+    c->ws_ldx = wldx;
+    c->ws_ldb = wldb;
+    c->ws_L = wL;
+    c->max_groups = nmem;
     c->ngroups = ng;
     c->gslots = nslots / ng;
-    c->Mg = (int)round_up((int64_t)c->gslots * c->q, sdca::kTile);
+    c->Mg = (int)round_up((int64_t)c->gslots * wq, sdca::kTile);
     c->Mpad = ng * c->Mg;
-    c->Mgp = c->Mg + 2 * sdca::kTile;
+    // each (alignment, level) part of a round's packed rows starts on a tile boundary
+    c->Mgp = c->Mg + (own ? 2 : 3 * std::min(nmem, c->gslots)) * sdca::kTile;
     const size_t prow = (size_t)ng * c->Mgp;  // packed rows of all groups
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
     int rc;
@@ -421,37 +557,45 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
     if ((rc = dalloc(&c->d_dbuf, (size_t)nslots * c->n_slot))) return rc;
     if ((rc = dalloc((uint8_t **)&c->d_S, (size_t)nslots * m * c->n_slot * pair_bytes(c)))) return rc;
     if ((rc = dalloc((uint8_t **)&c->d_Y, (size_t)nslots * m * c->n_slot * pair_bytes(c)))) return rc;
-    c->xc_off = (int64_t)prow * c->Kxp;
-    if ((rc = dalloc(&c->d_Xtr, (prow + c->Mpad) * c->Kxp))) return rc;
+    c->xc_off = (int64_t)prow * wldx;
+    if ((rc = dalloc(&c->d_Xtr, prow * wldx + (size_t)nslots * c->n_slot))) return rc;
     c->d_Xc = c->d_Xtr + c->xc_off;
     if (split) {
-        if ((rc = dalloc(&c->d_Whi, prow * c->Kxp))) return rc;
-        if ((rc = dalloc(&c->d_Wlo, prow * c->Kxp))) return rc;
-        if ((rc = dalloc(&c->d_Ghi, prow * c->Bp))) return rc;
-        if ((rc = dalloc(&c->d_Glo, prow * c->Bp))) return rc;
-        if ((rc = dalloc(&c->d_Gl3, prow * c->Bp))) return rc;
+        if ((rc = dalloc(&c->d_Whi, prow * wldx))) return rc;
+        if ((rc = dalloc(&c->d_Wlo, prow * wldx))) return rc;
+        if ((rc = dalloc(&c->d_Ghi, prow * wldb))) return rc;
+        if ((rc = dalloc(&c->d_Glo, prow * wldb))) return rc;
+        if ((rc = dalloc(&c->d_Gl3, prow * wldb))) return rc;
         // the third parts are read only for fine commands: allocated when a coupling
         // threshold can bring a slot to the fine level
-        if (c->opts.j_threshold > 0.0 && c->opts.fine_factor > 0 && (rc = dalloc(&c->d_Wl3, prow * c->Kxp)))
+        if (c->opts.j_threshold > 0.0 && c->opts.fine_factor > 0 && (rc = dalloc(&c->d_Wl3, prow * wldx)))
             return rc;
     } else {
-        if ((rc = dalloc(&c->d_G32, prow * c->Bp))) return rc;
+        if ((rc = dalloc(&c->d_G32, prow * wldb))) return rc;
     }
-    // GEMM configuration: the cached/tuned one for this shape, else the default rule
+    // GEMM configuration: the cached/tuned one for this shape, else the default rule; the
+    // groups of a multi-alignment solve take 256-wide tiles and a split-K chosen per round
     const GemmKey key{c->opts.precision, c->Mg, c->Bp, c->Kxp};
     const sdca_opts &o = c->opts;
     const bool forced = o.gemm_ksplit > 0 || o.gemm_gm_energy > 0 || o.gemm_gm_grad > 0 || o.gemm_tn_energy > 0 ||
                         o.gemm_tn_grad > 0;
     GemmConfig cfg;
-    bool have = !forced && lookup_config(key, cfg);
-    const bool tune = !forced && !have && o.autotune;
-    if (!have)
-        cfg = default_config(c, o.gemm_gm_energy > 0 ? o.gemm_gm_energy : 8,
-                             o.gemm_tn_energy > 0 ? o.gemm_tn_energy : 128);
-    if (forced) {
-        if (o.gemm_ksplit > 0) set_split(c, o.gemm_ksplit, cfg);
-        cfg.gm_g = o.gemm_gm_grad > 0 ? o.gemm_gm_grad : 8;
-        cfg.tn_g = o.gemm_tn_grad > 0 ? o.gemm_tn_grad : 128;
+    bool have = own && !forced && lookup_config(key, cfg);
+    const bool tune = own && !forced && !have && o.autotune;
+    if (!own) {
+        cfg.ksplit = 16;
+        cfg.kchunk = 0;
+        cfg.gm_e = cfg.gm_g = 8;
+        cfg.tn_e = cfg.tn_g = 256;
+    } else {
+        if (!have)
+            cfg = default_config(c, o.gemm_gm_energy > 0 ? o.gemm_gm_energy : 8,
+                                 o.gemm_tn_energy > 0 ? o.gemm_tn_energy : 128);
+        if (forced) {
+            if (o.gemm_ksplit > 0) set_split(c, o.gemm_ksplit, cfg);
+            cfg.gm_g = o.gemm_gm_grad > 0 ? o.gemm_gm_grad : 8;
+            cfg.tn_g = o.gemm_tn_grad > 0 ? o.gemm_tn_grad : 128;
+        }
     }
     int max_ksplit = cfg.ksplit;
     std::vector<GemmConfig> cand;
@@ -459,31 +603,40 @@ int ensure_workspace(sdca_ctx *c, int nslots, int m)
         cand = tuning_candidates(c);
         for (const auto &k : cand) max_ksplit = std::max(max_ksplit, k.ksplit);
     }
-    if ((rc = dalloc(&c->d_Epart, (size_t)max_ksplit * prow * c->Bp))) return rc;  // per group: ksplit x Mgp x Bp
+    if ((rc = dalloc(&c->d_Epart, (size_t)max_ksplit * prow * wldb))) return rc;  // per group: ksplit x Mgp x ldb
     c->ksplit_alloc = max_ksplit;
     // blocks per slot of the vector passes: up to 128 (at C4, 128 took k_update from
     // 3.15 to 3.03 ms per round against 64; 32 was slower)
     c->nchunk = (int)std::min<int64_t>(128, std::max<int64_t>(1, (c->n_slot / 4 + 255) / 256));
-    c->nbchunk = (c->Bp + 255) / 256;
-    if ((rc = dalloc(&c->d_hpart, (size_t)nslots * c->nbchunk * c->q))) return rc;
+    c->nbchunk = (wldb + 255) / 256;
+    if ((rc = dalloc(&c->d_hpart, (size_t)nslots * c->nbchunk * 8))) return rc;
     if ((rc = dalloc(&c->d_upd_part, (size_t)nslots * c->nchunk * sdca::ResLayout{m}.nupd()))) return rc;
     if ((rc = dalloc(&c->d_dot_part, (size_t)nslots * c->nchunk * sdca::ResLayout{m}.ndots()))) return rc;
     if ((rc = dalloc(&c->d_sm_part, (size_t)nslots * c->nbchunk))) return rc;
     if ((rc = dalloc(&c->d_res, (size_t)nslots * sdca::ResLayout{m}.width()))) return rc;
     if ((rc = dalloc(&c->d_cmds, (size_t)nslots))) return rc;
-    if ((rc = dalloc(&c->d_rows, (size_t)nslots * c->L))) return rc;
+    if ((rc = dalloc(&c->d_rows, (size_t)nslots * wL))) return rc;
     if ((rc = dalloc(&c->d_src_off, (size_t)nslots))) return rc;
     if ((rc = dalloc(&c->d_score_loci, (size_t)nslots))) return rc;
+    if ((rc = dalloc(&c->d_score_ld, (size_t)nslots))) return rc;
+    if ((rc = dalloc(&c->d_score_aln, (size_t)nslots))) return rc;
+    if ((rc = dalloc(&c->d_aln, (size_t)nmem))) return rc;
+    if ((rc = dalloc(&c->d_groups, (size_t)ng * 2 * nmem))) return rc;
     HIP_TRY(hipHostMalloc((void **)&c->h_cmds, (size_t)nslots * sizeof(sdca::SlotCmd), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&c->h_res, (size_t)nslots * sdca::ResLayout{m}.width() * sizeof(double),
                           hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void **)&c->h_rows, (size_t)nslots * c->L * sizeof(float), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_rows, (size_t)nslots * wL * sizeof(float), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&c->h_src_off, (size_t)nslots * sizeof(int64_t), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&c->h_score_loci, (size_t)nslots * sizeof(int32_t), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_score_ld, (size_t)nslots * sizeof(int32_t), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_score_aln, (size_t)nslots * sizeof(int32_t), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&c->h_wl, (size_t)3 * nslots * sizeof(int32_t), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&c->h_groups, (size_t)ng * 2 * nmem * sizeof(sdca::GemmGroup),
+                          hipHostMallocDefault));
     c->h_xt_off.assign(nslots, -1);
     // the zero fills above must be complete before any kernel of the solve runs
     HIP_TRY(hipDeviceSynchronize());
+    if ((rc = upload_alns(c))) return rc;
     c->tuned = 0;
     if (tune && (rc = autotune(c, c->gslots, cand, cfg))) return rc;
     c->ksplit = cfg.ksplit;
@@ -515,88 +668,133 @@ int launch_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     const int m = c->ws_m, W = sdca::ResLayout{m}.width();
     const bool split = c->opts.precision == SDCA_PREC_SPLIT_BF16;
     const int ns = c->gslots, s0 = g * ns;
-    // command list: the active slots by level, finest first, each part in slot order and
-    // starting on a tile boundary of the packed rows
+    const int nmem = (int)c->members.size();
+    const bool own = solves_itself(c);
+    // command list: the active slots by alignment (member order), within an alignment by
+    // level, finest first; each (alignment, level) part in slot order and starting on a tile
+    // boundary of the packed rows.  One GEMM group per alignment with commands.
     std::vector<int> &lst = c->gcmd_slot[g];
     lst.clear();
     sdca::SlotCmd *hc = c->h_cmds + s0;
-    sdca::Parts parts;
+    std::vector<sdca::Parts> gparts;
+    std::vector<int> galn, gend;
     int row = 0;
-    for (int part = 0; part < 3; ++part) {
-        const int level = sdca::LV_FINE - part;
-        row = (int)round_up(row, sdca::kTile);
-        parts.row[part] = row;
-        parts.cmd[part] = (int)lst.size();
-        for (int i = 0; i < ns; ++i) {
-            if (cmds[i].mode == sdca::MODE_IDLE || cmds[i].level != level) continue;
-            hc[lst.size()] = cmds[i];
-            hc[lst.size()].slot = i;
-            hc[lst.size()].row = row;
-            row += c->q;
-            lst.push_back(i);
+    for (int a = 0; a < nmem; ++a) {
+        const sdca_ctx *A = c->members[a];
+        bool any = false;
+        for (int i = 0; i < ns && !any; ++i) any = cmds[i].mode != sdca::MODE_IDLE && cmds[i].aln == a;
+        if (!any) continue;
+        sdca::Parts parts;
+        for (int part = 0; part < 3; ++part) {
+            const int level = sdca::LV_FINE - part;
+            row = (int)round_up(row, sdca::kTile);
+            parts.row[part] = row;
+            parts.cmd[part] = (int)lst.size();
+            for (int i = 0; i < ns; ++i) {
+                if (cmds[i].mode == sdca::MODE_IDLE || cmds[i].aln != a || cmds[i].level != level) continue;
+                sdca::SlotCmd &k = hc[lst.size()];
+                k = cmds[i];
+                k.slot = i;
+                k.row = row;
+                k.q = A->q;
+                k.Kx = (int)A->Kx;
+                k.Kxp = (int)A->Kxp;
+                row += A->q;
+                lst.push_back(i);
+            }
         }
+        parts.cmd[3] = (int)lst.size();
+        row = (int)round_up(row, sdca::kTile);
+        gparts.push_back(parts);
+        galn.push_back(a);
+        gend.push_back(row);
     }
     const int nc = (int)lst.size();
-    parts.cmd[3] = nc;
     if (nc == 0) return SDCA_OK;
-    const int Mrows = (int)round_up(row, sdca::kTile);  // packed rows of the three parts
+    const int Mrows = row;  // packed rows of all parts
+    const int ngr = (int)gparts.size();
+    // split-K of the energy contractions.  One alignment: the tuned slices, and a round with
+    // few active rows (the end of a solve) takes proportionally more, packed Mrows apart in
+    // the group's partial-energy buffer, so that its energy launch still fills the GPU.
+    // Several alignments: the same count of slices for every group, about 2048 workgroups.
+    int ksplit = c->ksplit, kchunk = c->kchunk, Mz = c->Mgp;
+    const int64_t cap = std::min<int64_t>(128, (int64_t)c->ksplit_alloc * c->Mgp / Mrows);
+    if (own) {
+        if (2 * Mrows <= c->Mg) {
+            GemmConfig gs;
+            set_split(c, std::min<int64_t>(cap, (int64_t)c->ksplit * c->Mg / Mrows), gs);
+            if (gs.ksplit <= cap && gs.ksplit > ksplit) {
+                ksplit = gs.ksplit;
+                kchunk = gs.kchunk;
+                Mz = Mrows;
+            }
+        }
+    } else {
+        int64_t tiles = 0;
+        for (int k = 0; k < ngr; ++k)
+            tiles += (int64_t)(gend[k] - gparts[k].row[0]) / sdca::kTile * (c->members[galn[k]]->Bp / c->tn_energy);
+        ksplit = (int)std::max<int64_t>(1, std::min<int64_t>(cap, (2048 + tiles - 1) / tiles));
+        Mz = Mrows;
+    }
+    sdca::GemmGroup *hge = c->h_groups + (size_t)g * 2 * c->max_groups, *hgg = hge + c->max_groups;
+    for (int k = 0; k < ngr; ++k) {
+        int ks = ksplit, kc = kchunk;
+        if (!own) {
+            GemmConfig gs;
+            set_split(c, c->members[galn[k]]->Kxp, ksplit, gs);
+            ks = gs.ksplit;
+            kc = gs.kchunk;
+        }
+        for (int i = gparts[k].cmd[0]; i < gparts[k].cmd[3]; ++i) hc[i].ksplit = ks;
+        hge[k] = make_group(c, galn[k], gparts[k], gend[k], true, ks, kc, c->gm_energy, c->tn_energy);
+        hgg[k] = make_group(c, galn[k], gparts[k], gend[k], false, 1, 0, c->gm_grad, c->tn_grad);
+    }
+    const int grid_e = place_groups(hge, ngr), grid_g = place_groups(hgg, ngr);
     const size_t vo = (size_t)s0 * c->n_slot;              // slot-vector offset of the group
-    const size_t ro = (size_t)g * c->Mgp * c->Kxp;         // its packed trial-point rows
-    const size_t go = (size_t)g * c->Mgp * c->Bp;          // its packed node-belief rows
+    const size_t ro = (size_t)g * c->Mgp * c->ws_ldx;      // its packed trial-point rows
+    const size_t go = (size_t)g * c->Mgp * c->ws_ldb;      // its packed node-belief rows
     float *gbuf = c->d_gbuf + (size_t)s0 * 2 * c->n_slot;
     void *dbuf = split ? (void *)(reinterpret_cast<uint16_t *>(c->d_dbuf) + vo) : (void *)(c->d_dbuf + vo);
     const size_t po = (size_t)s0 * m * c->n_slot * pair_bytes(c);
     void *Sp = (uint8_t *)c->d_S + po, *Yp = (uint8_t *)c->d_Y + po;
     sdca::SlotCmd *dc = c->d_cmds + s0;
-    float *Epart = c->d_Epart + (size_t)g * c->ksplit_alloc * c->Mgp * c->Bp;
+    float *Epart = c->d_Epart + (size_t)g * c->ksplit_alloc * c->Mgp * c->ws_ldb;
+    const sdca::GemmGroup *dge = c->d_groups + (size_t)g * 2 * c->max_groups, *dgg = dge + c->max_groups;
     hipStream_t st = c->gstream[g];
     hipEvent_t *ev = c->gev[g];
     HIP_TRY(hipMemcpyAsync(dc, hc, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
+    int rc;
+    if ((rc = upload_groups(c, g, ngr, ngr, st))) return rc;
     HIP_TRY(hipEventRecord(ev[0], st));
     double *upd_part = c->d_upd_part + (size_t)s0 * c->nchunk * sdca::ResLayout{m}.nupd();
-    sdca::kern::update(st, m, c->q, split, dc, nc, gbuf, dbuf, Sp, Yp, c->d_Xc + vo, c->d_Xtr + ro,
+    sdca::kern::update(st, m, split, dc, nc, gbuf, dbuf, Sp, Yp, c->d_Xc + vo, c->d_Xtr + ro,
                        split ? c->d_Whi + ro : nullptr, split ? c->d_Wlo + ro : nullptr,
-                       split ? c->d_Wl3 + ro : nullptr, c->d_X0 ? c->d_X0 + vo : nullptr, c->n_slot, (int)c->Kxp,
-                       (int)c->Kx, upd_part, c->nchunk);
+                       split ? c->d_Wl3 + ro : nullptr, c->d_X0 ? c->d_X0 + vo : nullptr, c->n_slot, c->ws_ldx,
+                       upd_part, c->nchunk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[1], st));
-    // A round with few active rows (the end of a solve) takes proportionally more split-K
-    // slices, packed Mrows apart in the group's partial-energy buffer, so that its energy
-    // launch still fills the GPU.
-    int ksplit = c->ksplit, kchunk = c->kchunk, Mz = c->Mgp;
-    if (2 * Mrows <= c->Mg) {
-        GemmConfig gs;
-        const int64_t cap = std::min<int64_t>(128, (int64_t)c->ksplit_alloc * c->Mgp / Mrows);
-        set_split(c, std::min<int64_t>(cap, (int64_t)c->ksplit * c->Mg / Mrows), gs);
-        if (gs.ksplit <= cap && gs.ksplit > ksplit) {
-            ksplit = gs.ksplit;
-            kchunk = gs.kchunk;
-            Mz = Mrows;
-        }
-    }
     // fine rows: hi + mid + lo, split rows: hi + lo, coarse rows: the high parts (or FP32 throughout)
-    sdca::kern::gemm_energy(st, c->opts.precision, parts, c->d_Xtr + ro, split ? c->d_Whi + ro : nullptr,
-                            split ? c->d_Wlo + ro : nullptr, split ? c->d_Wl3 + ro : nullptr, c->d_X, Mrows, Mz,
-                            c->Bp, (int)c->Kxp, ksplit, kchunk, c->gm_energy, c->tn_energy, Epart);
+    const int64_t zstride = (int64_t)Mz * c->ws_ldb;
+    sdca::kern::gemm_energy(st, c->opts.precision, dge, ngr, grid_e, c->tn_energy, c->d_Xtr + ro,
+                            split ? c->d_Whi + ro : nullptr, split ? c->d_Wlo + ro : nullptr,
+                            split ? c->d_Wl3 + ro : nullptr, c->ws_ldx, c->ws_ldb, zstride, Epart);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[2], st));
-    double *hpart = c->d_hpart + (size_t)s0 * c->nbchunk * c->q;
-    sdca::kern::softmax(st, c->opts.precision, Epart, ksplit, Mz, c->Bp, c->B, c->q, nc, dc, c->d_statesT,
-                        c->d_w, split ? nullptr : c->d_G32 + go, split ? c->d_Ghi + go : nullptr,
+    double *hpart = c->d_hpart + (size_t)s0 * c->nbchunk * 8;
+    sdca::kern::softmax(st, c->opts.precision, Epart, zstride, c->ws_ldb, nc, c->nbchunk, dc, c->d_aln,
+                        split ? nullptr : c->d_G32 + go, split ? c->d_Ghi + go : nullptr,
                         split ? c->d_Glo + go : nullptr, split ? c->d_Gl3 + go : nullptr,
-                        c->d_sm_part + (size_t)s0 * c->nbchunk, hpart, c->nbchunk);
+                        c->d_sm_part + (size_t)s0 * c->nbchunk, hpart);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[3], st));
-    sdca::kern::gemm_grad(st, c->opts.precision, parts, split ? nullptr : c->d_G32 + go,
+    sdca::kern::gemm_grad(st, c->opts.precision, dgg, ngr, grid_g, c->tn_grad, split ? nullptr : c->d_G32 + go,
                           split ? c->d_Ghi + go : nullptr, split ? c->d_Glo + go : nullptr,
-                          split ? c->d_Gl3 + go : nullptr, c->d_Xt, Mrows, c->Bp, (int)c->Kxp, (int)c->Kx, c->q, dc,
-                          c->d_Xtr + ro, gbuf, c->n_slot, 2.0 * c->lambda_J, 2.0 * c->lambda_h, hpart, c->nbchunk,
-                          c->gm_grad, c->tn_grad);
+                          split ? c->d_Gl3 + go : nullptr, c->ws_ldb, c->d_aln, dc, c->d_Xtr + ro, c->ws_ldx, gbuf,
+                          c->n_slot, hpart, c->nbchunk);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[4], st));
     double *dot_part = c->d_dot_part + (size_t)s0 * c->nchunk * sdca::ResLayout{m}.ndots();
-    sdca::kern::dots(st, m, split, dc, nc, gbuf, dbuf, Sp, Yp, c->n_slot, (int)c->Kxp, (int)c->Kx, dot_part,
-                     c->nchunk);
+    sdca::kern::dots(st, m, split, dc, nc, gbuf, dbuf, Sp, Yp, c->n_slot, dot_part, c->nchunk);
     HIP_TRY(hipGetLastError());
     sdca::kern::reduce_results(st, m, nc, upd_part, dot_part,
                                c->d_sm_part + (size_t)s0 * c->nbchunk, c->nchunk, c->nbchunk,
@@ -625,7 +823,7 @@ int finish_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
         tmp.assign(res, res + lst.size() * W);
         for (size_t k = 0; k < lst.size(); ++k) {
             std::memcpy(res + (size_t)lst[k] * W, tmp.data() + k * W, W * sizeof(double));
-            c->h_xt_off[s0 + lst[k]] = (int64_t)((size_t)g * c->Mgp + (size_t)c->h_cmds[s0 + k].row) * c->Kxp;
+            c->h_xt_off[s0 + lst[k]] = (int64_t)((size_t)g * c->Mgp + (size_t)c->h_cmds[s0 + k].row) * c->ws_ldx;
         }
     }
     hipEvent_t *ev = c->gev[g];
@@ -640,34 +838,39 @@ int finish_round(sdca_ctx *c, int g, const sdca::SlotCmd *cmds)
     t.evals += 1;
     t.energy_launches += 1;
     t.grad_launches += 1;
-    int active = 0;
     const double pb = (double)pair_bytes(c);
-    double bu = 0.0, bd = 0.0;  // bytes per element of the update and dot-product passes
+    double bu = 0.0, bd = 0.0;  // bytes of the update and dot-product passes
+    double flops = 0.0;         // of one contraction
     for (int i = 0; i < c->gslots; ++i) {
         const sdca::SlotCmd &k = cmds[i];
         const int md = k.mode;
         if (md == sdca::MODE_IDLE) continue;
-        ++active;
+        const sdca_ctx *A = c->members[k.aln];
+        const double nel = (double)A->q * (double)A->Kxp;  // elements of the command's vectors
+        flops += 2.0 * A->B * (double)A->Kx * A->q;
+        double u = 0.0, d = 0.0;  // bytes per element
         int npairs = 0;
         for (int p = 0; p < m; ++p) npairs += (k.live >> p) & 1u;
-        if (md == sdca::MODE_INIT) bu += 4.0 * (2 + (k.init_src != 0.f));
-        else if (md == sdca::MODE_RETRY) bu += 8.0 + pb;  // x_c and d read, x_t written
+        if (md == sdca::MODE_INIT) u += 4.0 * (2 + (k.init_src != 0.f));
+        else if (md == sdca::MODE_RETRY) u += 8.0 + pb;  // x_c and d read, x_t written
         else {
             // x, d and g_k read; x_t, d (and x_c) written; the new pair from g_old
-            bu += 4.0 * (3 + (k.accept != 0)) + 2.0 * pb + (k.pnew >= 0 ? 4.0 + 2.0 * pb : 0.0);
+            u += 4.0 * (3 + (k.accept != 0)) + 2.0 * pb + (k.pnew >= 0 ? 4.0 + 2.0 * pb : 0.0);
             for (int p = 0; p < m; ++p) {
                 if (p == k.pnew) continue;
-                bu += (k.cs[p] != 0.f ? pb : 0.0) + (k.cy[p] != 0.f ? pb : 0.0);
+                u += (k.cs[p] != 0.f ? pb : 0.0) + (k.cy[p] != 0.f ? pb : 0.0);
             }
         }
-        if (split) bu += 2.0 * (k.level + 1);  // BF16 parts of x_t: hi (+ lo (+ l3))
-        bd += (md == sdca::MODE_INIT ? 4.0 : 8.0 + pb) + 2.0 * pb * npairs;  // g_t (, g_k, d) and the pairs
+        if (split) u += 2.0 * (k.level + 1);  // BF16 parts of x_t: hi (+ lo (+ l3))
+        d += (md == sdca::MODE_INIT ? 4.0 : 8.0 + pb) + 2.0 * pb * npairs;  // g_t (, g_k, d) and the pairs
+        bu += u * nel;
+        bd += d * nel;
     }
-    t.bytes_update += bu * (double)c->n_slot;
-    t.bytes_dots += bd * (double)c->n_slot;
-    t.bytes_lbfgs += (bu + bd) * (double)c->n_slot;
-    t.flops_energy += 2.0 * c->B * (double)c->Kx * c->q * active;
-    t.flops_grad += 2.0 * c->B * (double)c->Kx * c->q * active;
+    t.bytes_update += bu;
+    t.bytes_dots += bd;
+    t.bytes_lbfgs += bu + bd;
+    t.flops_energy += flops;
+    t.flops_grad += flops;
     return SDCA_OK;
 }
 
@@ -734,6 +937,7 @@ enum Phase { PH_IDLE, PH_INIT, PH_LS, PH_REBASE, PH_DONE };
 //   TS[p], TY[p], TT, TG = t.g, TD = t.d.
 struct Slot {
     int item = -1;  // index into the caller's loci list
+    int aln = 0;    // alignment of the locus (member of the solving context)
     int r = -1;
     Phase phase = PH_IDLE;
     std::deque<int> pairs;  // pair slots, oldest first
@@ -802,10 +1006,11 @@ public:
         return jthr_ <= 0.0 || (S.level == max_level_ && gJ < jthr_);
     }
 
-    void assign(int i, int item, int r, bool warm, bool preloaded = false)
+    void assign(int i, int item, int aln, int r, bool warm, bool preloaded = false)
     {
         Slot &S = slots_[i];
         S = Slot();
+        S.aln = aln;
         S.preloaded = preloaded;
         S.level = !split_ ? max_level_ : (coarse_gnorm_ > 0.0 ? sdca::LV_COARSE : sdca::LV_SPLIT);
         S.item = item;
@@ -821,8 +1026,10 @@ public:
         k.pnew = -1;
         // the masked columns: the locus' own, or in a two-alignment scan every column of
         // the first alignment (plmDCA_objective.hpp:67: no block of the second is excluded)
-        k.self_lo = c_->scan_split > 0 ? 0 : c_->col_off[r];
-        k.self_hi = c_->scan_split > 0 ? c_->col_off[c_->scan_split] : c_->col_off[r + 1];
+        const sdca_ctx *A = c_->members[aln];
+        k.aln = aln;
+        k.self_lo = A->scan_split > 0 ? 0 : A->col_off[r];
+        k.self_hi = A->scan_split > 0 ? A->col_off[A->scan_split] : A->col_off[r + 1];
         k.init_src = warm ? 1.f : 0.f;
     }
 
@@ -848,7 +1055,8 @@ public:
         S.TT = res[R.tt()];
         S.TG = res[R.tg()];
         S.TD = res[R.td()];
-        const double f = res[R.fdata()] + c_->lambda_J * res[R.xsqJ()] + c_->lambda_h * res[R.xsqh()];
+        const sdca_ctx *A = c_->members[S.aln];
+        const double f = res[R.fdata()] + A->lambda_J * res[R.xsqJ()] + A->lambda_h * res[R.xsqh()];
         const double ginf = res[R.ginf()], gJ = res[R.gJinf()];
         S.nfeval++;
         if (k.level == sdca::LV_FINE) S.nfeval_fine++;
@@ -1178,6 +1386,24 @@ int ensure_x8(sdca_ctx *c)
     return SDCA_OK;
 }
 
+// The streams and events of a context on its device (default priority; blocking streams,
+// so they are ordered after the null-stream zero fills of dalloc()).  One stream per slot
+// group; group 0 uses c->stream.
+int create_streams(sdca_ctx *c)
+{
+    if (hipSetDevice(c->device) != hipSuccess) return fail(SDCA_E_HIP, "hipSetDevice(%d) failed", c->device);
+    if (hipStreamCreate(&c->stream) != hipSuccess) return fail(SDCA_E_HIP, "hipStreamCreate failed");
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(SDCA_E_HIP, "hipEventCreate failed");
+    c->gstream[0] = c->stream;
+    for (int g = 1; g < sdca::kMaxGroups; ++g)
+        if (hipStreamCreate(&c->gstream[g]) != hipSuccess) return fail(SDCA_E_HIP, "hipStreamCreate failed");
+    for (auto &g : c->gev)
+        for (auto &e : g)
+            if (hipEventCreate(&e) != hipSuccess) return fail(SDCA_E_HIP, "hipEventCreate failed");
+    return SDCA_OK;
+}
+
 int check_ctx(sdca_ctx *c)
 {
     if (!c) return fail(SDCA_E_ARG, "null context");
@@ -1280,19 +1506,8 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
         delete c;
         return code;
     };
-    if (hipSetDevice(c->device) != hipSuccess) return bail(fail(SDCA_E_HIP, "hipSetDevice(%d) failed", c->device));
-    // a blocking stream: it is ordered after the null-stream zero fills of dalloc()
-    if (hipStreamCreate(&c->stream) != hipSuccess)
-        return bail(fail(SDCA_E_HIP, "hipStreamCreate failed"));
-    for (auto &e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) return bail(fail(SDCA_E_HIP, "hipEventCreate failed"));
-    // one blocking stream per slot group (default priority); group 0 uses c->stream
-    c->gstream[0] = c->stream;
-    for (int g = 1; g < sdca::kMaxGroups; ++g)
-        if (hipStreamCreate(&c->gstream[g]) != hipSuccess) return bail(fail(SDCA_E_HIP, "hipStreamCreate failed"));
-    for (auto &g : c->gev)
-        for (auto &e : g)
-            if (hipEventCreate(&e) != hipSuccess) return bail(fail(SDCA_E_HIP, "hipEventCreate failed"));
+    if ((rc = create_streams(c))) return bail(rc);
+    c->members.assign(1, c);
     std::vector<uint8_t> sT((size_t)B * L);
     for (int b = 0; b < B; ++b)
         for (int j = 0; j < L; ++j) sT[(size_t)j * B + b] = states[(size_t)b * L + j];
@@ -1417,6 +1632,7 @@ int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x
     if (K == 0) return SDCA_OK;
     const int nslots = std::min(c->opts.batch_loci, K);
     if ((rc = ensure_workspace(c, nslots, c->opts.history))) return rc;
+    if ((rc = upload_alns(c))) return rc;
     if (!c->d_X0) {
         if ((rc = dalloc(&c->d_X0, (size_t)nslots * c->n_slot))) return rc;
         HIP_TRY(hipDeviceSynchronize());
@@ -1488,22 +1704,37 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
 {
     int rc = check_ctx(c);
     if (rc) return rc;
-    if (!c->have_weights) return fail(SDCA_E_STATE, "sdca_set_weights must be called first");
+    return sdca_solve_batch(c, nullptr, loci, K, warm, cw, score_rows, nullptr, x_out, st_out);
+}
+
+int sdca_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int32_t K, const double *warm,
+                     const sdca::CompactWarm *cw, float *score_rows, const int64_t *row_off, double *x_out,
+                     sdca_locus_stats *st_out)
+{
+    int rc;
+    const int nmem = (int)c->members.size();
+    for (const sdca_ctx *A : c->members)
+        if (!A->have_weights) return fail(SDCA_E_STATE, "sdca_set_weights must be called first");
     if (K < 0 || (K > 0 && !loci)) return fail(SDCA_E_ARG, "bad arguments");
-    for (int k = 0; k < K; ++k)
-        if (loci[k] < 0 || loci[k] >= c->L) return fail(SDCA_E_ARG, "locus %d out of range", loci[k]);
-    if (c->scan_split > 0) {
-        // the masked first alignment must stay at zero: no start points there
-        if (warm || cw) return fail(SDCA_E_STATE, "a two-alignment scan solves from a cold start only");
-        for (int k = 0; k < K; ++k)
-            if (loci[k] >= c->scan_split)
+    if (nmem != 1 && (warm || cw || x_out)) return fail(SDCA_E_ARG, "warm starts and x_out need a single alignment");
+    for (int k = 0; k < K; ++k) {
+        const int a = alns ? alns[k] : 0;
+        if (a < 0 || a >= nmem) return fail(SDCA_E_ARG, "alignment %d of item %d out of range", a, k);
+        const sdca_ctx *A = c->members[a];
+        if (loci[k] < 0 || loci[k] >= A->L) return fail(SDCA_E_ARG, "locus %d out of range", loci[k]);
+        if (A->scan_split > 0) {
+            // the masked first alignment must stay at zero: no start points there
+            if (warm || cw) return fail(SDCA_E_STATE, "a two-alignment scan solves from a cold start only");
+            if (loci[k] >= A->scan_split)
                 return fail(SDCA_E_ARG, "locus %d is not in the first alignment (loci [0, %d))", loci[k],
-                            c->scan_split);
+                            A->scan_split);
+        }
     }
     if (K == 0) return SDCA_OK;
     const auto t0 = std::chrono::steady_clock::now();
     const int nslots = std::min(c->opts.batch_loci, K);
     if ((rc = ensure_workspace(c, nslots, c->opts.history))) return rc;
+    if ((rc = upload_alns(c))) return rc;
     const bool any_warm = warm || cw;
     if (any_warm && !c->d_X0) {
         if ((rc = dalloc(&c->d_X0, (size_t)nslots * c->n_slot))) return rc;
@@ -1533,6 +1764,7 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
         }
         const int item = next++;
         const int r = loci[item];
+        const int a = alns ? alns[item] : 0;
         if (warm) {
             (void)ref_to_dev(c, r, warm + (size_t)item * n, stage.data());
             HIP_TRY(hipMemcpy(c->d_X0 + (size_t)i * c->n_slot, stage.data(), c->n_slot * 4, hipMemcpyHostToDevice));
@@ -1543,7 +1775,7 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
             wl_item[g].push_back(item);
             wl_locus[g].push_back(r);
         }
-        S.assign(i, item, r, any_warm, cw && cw->as_direction);
+        S.assign(i, item, a, r, any_warm, cw && cw->as_direction);
         return SDCA_OK;
     };
     // stage the compact warm starts of group g's refills into their slots' X0 vectors
@@ -1606,9 +1838,14 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
         hipEvent_t *ev = c->gev[g];
         HIP_TRY(hipEventSynchronize(ev[8]));
         c->timing.score += elapsed(ev[6], ev[7]);
-        const float *h = c->h_rows + (size_t)g * gs * c->L;
-        for (size_t u = 0; u < row_items[g].size(); ++u)
-            std::memcpy(score_rows + (size_t)row_items[g][u] * c->L, h + u * c->L, (size_t)c->L * 4);
+        const float *h = c->h_rows + (size_t)g * gs * c->ws_L;
+        for (size_t u = 0; u < row_items[g].size(); ++u) {
+            // row k of the caller's buffer: at k * L (one alignment) or at row_off[k]
+            const int item = row_items[g][u];
+            const int La = c->members[alns ? alns[item] : 0]->L;
+            const int64_t off = row_off ? row_off[item] : (int64_t)item * La;
+            std::memcpy(score_rows + off, h + u * c->ws_L, (size_t)La * 4);
+        }
         row_items[g].clear();
         return SDCA_OK;
     };
@@ -1637,24 +1874,31 @@ int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *w
                 const int nf = (int)finished.size();
                 int64_t *h_off = c->h_src_off + (size_t)g * gs;
                 int32_t *h_sl = c->h_score_loci + (size_t)g * gs;
+                int32_t *h_ld = c->h_score_ld + (size_t)g * gs, *h_al = c->h_score_aln + (size_t)g * gs;
                 for (int u = 0; u < nf; ++u) {
                     const int i = finished[u];
                     const Slot &sl = S.slot(i);
+                    // the solution: the last trial point (packed rows of ws_ldx) or x_c (the
+                    // slot vector, rows of its own Kxp)
                     h_off[u] = sl.from_xt ? c->h_xt_off[i] : c->xc_off + (int64_t)i * c->n_slot;
+                    h_ld[u] = sl.from_xt ? (int32_t)c->ws_ldx : (int32_t)c->members[sl.aln]->Kxp;
                     h_sl[u] = sl.r;
+                    h_al[u] = sl.aln;
                 }
                 int64_t *d_off = c->d_src_off + (size_t)g * gs;
                 int32_t *d_sl = c->d_score_loci + (size_t)g * gs;
-                float *d_rows = c->d_rows + (size_t)g * gs * c->L;
+                int32_t *d_ld = c->d_score_ld + (size_t)g * gs, *d_al = c->d_score_aln + (size_t)g * gs;
+                float *d_rows = c->d_rows + (size_t)g * gs * c->ws_L;
                 HIP_TRY(hipMemcpyAsync(d_off, h_off, nf * 8, hipMemcpyHostToDevice, st));
                 HIP_TRY(hipMemcpyAsync(d_sl, h_sl, nf * 4, hipMemcpyHostToDevice, st));
+                HIP_TRY(hipMemcpyAsync(d_ld, h_ld, nf * 4, hipMemcpyHostToDevice, st));
+                HIP_TRY(hipMemcpyAsync(d_al, h_al, nf * 4, hipMemcpyHostToDevice, st));
                 HIP_TRY(hipEventRecord(ev[6], st));
-                sdca::kern::score(st, c->d_Xtr, d_off, d_sl, nf, c->L, c->q, c->gap, (int)c->Kxp, c->d_state_col,
-                                  d_rows);
+                sdca::kern::score(st, c->d_Xtr, d_off, d_ld, d_sl, d_al, nf, c->ws_L, c->ws_L, c->d_aln, d_rows);
                 HIP_TRY(hipGetLastError());
                 HIP_TRY(hipEventRecord(ev[7], st));
                 if (score_rows) {
-                    HIP_TRY(hipMemcpyAsync(c->h_rows + (size_t)g * gs * c->L, d_rows, (size_t)nf * c->L * 4,
+                    HIP_TRY(hipMemcpyAsync(c->h_rows + (size_t)g * gs * c->ws_L, d_rows, (size_t)nf * c->ws_L * 4,
                                            hipMemcpyDeviceToHost, st));
                     for (int u = 0; u < nf; ++u) row_items[g].push_back(S.slot(finished[u]).item);
                 }
@@ -1855,3 +2099,83 @@ int sdca_get_timing(sdca_ctx *c, sdca_timing *t, int32_t reset)
 }
 
 }  // extern "C"
+
+// ====================================================== multi-alignment solves
+
+// A solve over several alignments at once: an internal context that holds the workspace,
+// streams and events, whose members are the caller's contexts (their one-hot X, weights
+// and lambda); the caller's contexts must outlive it.
+struct sdca_multi {
+    sdca_ctx *w = nullptr;
+    ~sdca_multi() { delete w; }
+};
+
+extern "C" {
+
+int sdca_multi_create(sdca_multi **out, sdca_ctx *const *ctxs, int32_t n, const sdca_opts *opts)
+{
+    if (!out) return fail(SDCA_E_ARG, "out is null");
+    *out = nullptr;
+    if (!ctxs || n < 1 || n > 4096) return fail(SDCA_E_ARG, "need 1 to 4096 contexts (got %d)", n);
+    sdca_opts o;
+    if (opts) o = *opts;
+    else sdca_default_opts(&o);
+    if (o.precision != SDCA_PREC_FP32 && o.precision != SDCA_PREC_SPLIT_BF16)
+        return fail(SDCA_E_ARG, "unknown precision %d", o.precision);
+    if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 8, 10, 16 (got %d)", o.history);
+    if (o.batch_loci < 1 || o.max_iter < 1 || o.max_ls < 1 || !(o.grad_threshold >= 0.0) || !(o.j_threshold >= 0.0) ||
+        o.coarse_factor < 0 || o.fine_factor < 0)
+        return fail(SDCA_E_ARG, "invalid solver options");
+    for (int a = 0; a < n; ++a) {
+        if (!ctxs[a]) return fail(SDCA_E_ARG, "context %d is null", a);
+        if (ctxs[a]->device != o.device)
+            return fail(SDCA_E_ARG, "context %d is on device %d, the solve on %d", a, ctxs[a]->device, o.device);
+    }
+    auto *m = new sdca_multi();
+    m->w = new sdca_ctx();
+    sdca_ctx *w = m->w;
+    w->opts = o;
+    w->opts.pipeline_groups = 1;
+    w->device = o.device;
+    int rc = create_streams(w);
+    if (rc) {
+        delete m;
+        return rc;
+    }
+    w->members.assign(ctxs, ctxs + n);
+    *out = m;
+    return SDCA_OK;
+}
+
+void sdca_multi_destroy(sdca_multi *m) { delete m; }
+
+int sdca_multi_solve(sdca_multi *m, const int32_t *alns, const int32_t *loci, int32_t K, float *score_rows,
+                     const int64_t *row_off, sdca_locus_stats *st)
+{
+    if (!m || !m->w) return fail(SDCA_E_ARG, "null multi-alignment solve");
+    if (K > 0 && !alns) return fail(SDCA_E_ARG, "alns is null");
+    int rc = check_ctx(m->w);
+    if (rc) return rc;
+    std::vector<int64_t> packed;
+    if (score_rows && !row_off) {  // rows one after the other, each the L of its alignment
+        packed.resize(K);
+        int64_t off = 0;
+        for (int k = 0; k < K; ++k) {
+            if (alns[k] < 0 || alns[k] >= (int)m->w->members.size())
+                return fail(SDCA_E_ARG, "alignment %d of item %d out of range", alns[k], k);
+            packed[k] = off;
+            off += m->w->members[alns[k]]->L;
+        }
+        row_off = packed.data();
+    }
+    return sdca_solve_batch(m->w, alns, loci, K, nullptr, nullptr, score_rows, row_off, nullptr, st);
+}
+
+int sdca_multi_get_timing(sdca_multi *m, sdca_timing *t, int32_t reset)
+{
+    if (!m) return fail(SDCA_E_ARG, "null multi-alignment solve");
+    return sdca_get_timing(m->w, t, reset);
+}
+
+}  // extern "C"
+

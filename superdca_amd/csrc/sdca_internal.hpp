@@ -67,6 +67,10 @@ struct SlotCmd {
     int32_t slot;     // slot of the command within its group
     int32_t row;      // first of its q rows in the round's packed layout
     int32_t level;    // contraction precision of the round (split precision): LV_COARSE, LV_SPLIT or LV_FINE
+    int32_t aln;      // alignment of the locus (index into the solve's alignment table)
+    int32_t q;        // its envelope: the command owns q packed rows, its slot vectors q x Kxp floats
+    int32_t Kx, Kxp;  // its one-hot columns and their padded count (the slot vectors' row length)
+    int32_t ksplit;   // split-K slices of its energy contraction in this round
     int32_t gt;       // gradient buffer (0 or 1) that receives the trial gradient
     int32_t gk;       // gradient buffer of the current iterate (after NEWDIR's accept); -1 at INIT
     int32_t accept;   // NEWDIR: 1 = the last trial point becomes the iterate first (x_c += pstep d_old)
@@ -105,6 +109,51 @@ struct Timing {
     double update = 0, dots = 0, bytes_update = 0, bytes_dots = 0;
 };
 
+// One alignment of a solve, as the kernels see it.  A solve covers one alignment (a
+// context) or several (sdca_multi): every command names its alignment.
+struct AlnDev {
+    const uint32_t *X;         // Bp x Kxp one-hot bits
+    const uint32_t *Xt;        // Kxp x Bp, its transpose
+    const uint8_t *statesT;    // L x B envelope states, the target columns contiguous
+    const float *w;            // Bp sequence weights (pads 0)
+    const int32_t *state_col;  // L x q: column of (j, a) or -1
+    int B, Bp, L, q, gap, Kx, Kxp;
+    float lamJ2;               // 2 lambda_J
+    double lamh2;              // 2 lambda_h
+};
+
+// XCD-aware tile order of a GEMM launch.  Workgroups are dispatched to the eight XCDs
+// round-robin (workgroup id % 8), so consecutive ids on one XCD are id, id+8, id+16, ...
+// Tiles are grouped into super-tiles of GM x GN output tiles at one split-K index z; a
+// super-tile runs on a single XCD, whose L2 then serves each A rows-tile to GN workgroups
+// and each B rows-tile to GM workgroups.  Super-tiles are dealt to XCDs round-robin in
+// the order N fastest, then split-K, then M: the row tiles start in increasing order, so
+// the rows of a launch that take more MFMA passes (placed first) are not left for its end.
+struct TileMap {
+    int Mt, Nt, Z;   // output tiles along M and N, split-K slices
+    int GM, GN;      // super-tile shape
+    int MB, NB, S;   // super-tiles along M and N, total super-tiles
+};
+
+inline TileMap make_map(int Mt, int Nt, int Z, int gm)
+{
+    TileMap tm;
+    tm.Mt = Mt;
+    tm.Nt = Nt;
+    tm.Z = Z;
+    if (gm < 1) gm = 8;
+    if (gm > 64) gm = 64;
+    tm.GM = Mt < gm ? Mt : gm;
+    tm.GN = Nt < 64 / tm.GM ? Nt : 64 / tm.GM;
+    tm.MB = (Mt + tm.GM - 1) / tm.GM;
+    tm.NB = (Nt + tm.GN - 1) / tm.GN;
+    tm.S = Z * tm.MB * tm.NB;
+    return tm;
+}
+
+// Workgroups of a tile map: whole rounds of eight super-tiles (one per XCD).
+inline int map_grid(const TileMap &tm) { return ((tm.S + 7) / 8) * 8 * tm.GM * tm.GN; }
+
 // The commands of a round are listed by level, finest first, each level's packed rows
 // starting on a tile boundary: row[l] is the first packed row and cmd[l] the first command
 // of the l-th part (fine, split, coarse), cmd[3] the number of commands.  Under split
@@ -113,6 +162,20 @@ struct Timing {
 struct Parts {
     int row[3];
     int cmd[4];
+};
+
+// One alignment's share of a round's GEMM launch: its commands' packed rows (parts, in
+// absolute rows and command indices; the rows from parts.row[0] on, tile by tile), the
+// B operand (its one-hot bits, ldbw words per row), the reduction length and split-K chunk,
+// and its tile map; workgroups [tile0, tile0 + map_grid(tm)) of the launch compute it.
+struct GemmGroup {
+    int tile0;
+    int aln, q;
+    Parts parts;
+    int K, kchunk;
+    const uint32_t *Bbits;
+    int64_t ldbw;
+    TileMap tm;
 };
 
 // Largest mismatch count d with double(L - d) / double(L) >= theta, evaluated with the
@@ -137,35 +200,38 @@ void build_x(hipStream_t st, const uint8_t *d_states, int B, int L, int q, const
 void bitplanes(hipStream_t st, const uint8_t *d_states, int B, int L, int q, int W, uint64_t *planes);
 void hamming_neighbours(hipStream_t st, const uint64_t *planes, int B, int q, int W, int L, int64_t dmax,
                         const uint32_t *d_mult, unsigned long long *d_iw);
-int energy_superiles(int Mpad, int Bp, int gm, int tn);  // super-tiles per split-K slice
-// Packed rows (trial points W, energies, beliefs): Mrows rows computed (a multiple of
-// kTile), Mz rows between split-K slices of Epart.  Command k owns rows [row, row + q);
-// the Parts of the round (above) say which tiles take how many MFMA passes.
-void gemm_energy(hipStream_t st, int prec, const Parts &parts, const void *W32, const void *Whi, const void *Wlo,
-                 const void *Wl3, const uint32_t *X, int Mrows, int Mz, int Bp, int Kxp, int ksplit, int kchunk,
-                 int gm, int tn, float *Epart);
-// hpart: per command and block of 256 sequences, the FP64 sums over b of G_b(s) (q each),
-// from which the gradient epilogue forms the fields' gradient.
-void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mz, int Bp, int B, int q, int ncmd,
-             const SlotCmd *cmds, const uint8_t *statesT, const float *wts, float *G32, uint16_t *Ghi,
-             uint16_t *Glo, uint16_t *Gl3, double *fpart, double *hpart, int nbchunk);
-void gemm_grad(hipStream_t st, int prec, const Parts &parts, const void *G32, const void *Ghi, const void *Glo,
-               const void *Gl3, const uint32_t *Xt, int Mrows, int Bp, int Kxp, int Kx, int q, const SlotCmd *cmds,
-               const float *Xtrial, float *gbuf, int64_t n_slot, double lamJ2, double lamh2, const double *hpart,
-               int nbchunk, int gm, int tn);
+// GEMMs over the groups of a round (d_groups on the device, ngroups of them, grid
+// workgroups in all; tn the output tile width of every group).  Packed rows have ldx
+// floats (trial points, W parts) or ldb (energies, beliefs); split-K slices of the energy
+// partials are zstride floats apart.  Under split precision a group's fine part takes
+// three MFMA passes, its split part two and its coarse part one.
+void gemm_energy(hipStream_t st, int prec, const GemmGroup *d_groups, int ngroups, int grid, int tn, const void *W32,
+                 const void *Whi, const void *Wlo, const void *Wl3, int64_t ldx, int64_t ldb, int64_t zstride,
+                 float *Epart);
+// hpart: per command and block of 256 sequences, the FP64 sums over b of G_b(s) (8 per
+// block), from which the gradient epilogue forms the fields' gradient.
+void softmax(hipStream_t st, int prec, const float *Epart, int64_t zstride, int64_t ldb, int ncmd, int nbchunk,
+             const SlotCmd *cmds, const AlnDev *d_aln, float *G32, uint16_t *Ghi, uint16_t *Glo, uint16_t *Gl3,
+             double *fpart, double *hpart);
+void gemm_grad(hipStream_t st, int prec, const GemmGroup *d_groups, int ngroups, int grid, int tn, const void *G32,
+               const void *Ghi, const void *Glo, const void *Gl3, int64_t ldb, const AlnDev *d_aln, const SlotCmd *cmds,
+               const float *Xtrial, int64_t ldx, float *gbuf, int64_t n_slot, const double *hpart, int nbchunk);
 // Slot vectors: gbuf [slot][2][n_slot], dbuf [slot][n_slot], S and Y [slot][m][n_slot]
-// (dbuf, S and Y uint16_t BF16 when pair_bf16, else float); Xtr, Whi and Wlo are packed rows.  One
-// block row per command (ncmd commands).
-void update(hipStream_t st, int m, int q, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
-            void *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, uint16_t *Wl3,
-            const float *X0, int64_t n_slot, int Kxp, int Kx, double *part, int nchunk);
+// (dbuf, S and Y uint16_t BF16 when pair_bf16, else float); a command uses the first
+// q x Kxp of its slot's.  Xtr, Whi, Wlo and Wl3 are packed rows of ldx.  One block row per
+// command (ncmd commands).
+void update(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf, void *dbuf,
+            void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, uint16_t *Wl3, const float *X0,
+            int64_t n_slot, int64_t ldx, double *part, int nchunk);
 void dots(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
-          const void *dbuf, const void *S, const void *Y, int64_t n_slot, int Kxp, int Kx, double *part,
-          int nchunk);
+          const void *dbuf, const void *S, const void *Y, int64_t n_slot, double *part, int nchunk);
 void reduce_results(hipStream_t st, int m, int ncmd, const double *upd_part, const double *dot_part,
                     const double *sm_part, int nchunk, int nbchunk, double *res);
-void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *loci, int nscore, int L,
-           int q, int gap, int Kxp, const int32_t *state_col, float *rows);
+// Score rows of nscore finished loci: item k reads its solution from xsrc + src_off[k]
+// (rows src_ld[k] floats apart), locus loci[k] of alignment alns[k], and writes its row of
+// that alignment's L entries at rows + k * ldrow.
+void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *src_ld, const int32_t *loci,
+           const int32_t *alns, int nscore, int Lmax, int64_t ldrow, const AlnDev *d_aln, float *rows);
 // sdca_pairs.hip
 void build_x8(hipStream_t st, const uint8_t *states, int B, int L, int QP, int Bp, int8_t *X8);
 void marginals(hipStream_t st, const uint8_t *states, int B, int L, int QP, int32_t *cnt);
@@ -222,6 +288,11 @@ struct Mlp {
 // layout on the host, or compact on the device).
 int sdca_solve_impl(sdca_ctx *c, const int32_t *loci, int32_t K, const double *warm_dense,
                     const sdca::CompactWarm *cw, float *score_rows, double *x_out, sdca_locus_stats *st);
+// The same over the member alignments of c (alns[k]: member of item k, NULL: all 0); the
+// score row of item k goes to score_rows + row_off[k] (NULL: k * L).
+int sdca_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int32_t K, const double *warm_dense,
+                     const sdca::CompactWarm *cw, float *score_rows, const int64_t *row_off, double *x_out,
+                     sdca_locus_stats *st);
 
 struct sdca_ctx {
     sdca_opts opts{};
@@ -258,6 +329,17 @@ struct sdca_ctx {
     int tn_energy = 128, tn_grad = 128;  // output tile widths (autotuned)
     int tuned = 0;                   // 1 if the autotuner chose the GEMM configuration
     int nchunk = 0, nbchunk = 0;
+    // The alignments the workspace serves: this context alone, or (for the internal
+    // context of an sdca_multi) the member contexts of a multi-alignment solve.  The
+    // workspace is sized for the largest: ws_q rows per command, packed rows of ws_ldx
+    // (trial points) and ws_ldb (energies, beliefs) floats, score rows of ws_L, slot vectors
+    // of n_slot floats.
+    std::vector<sdca_ctx *> members;
+    int ws_q = 0, ws_ldb = 0, ws_L = 0;
+    int64_t ws_ldx = 0;
+    sdca::AlnDev *d_aln = nullptr;                        // members' descriptors (device)
+    sdca::GemmGroup *d_groups = nullptr, *h_groups = nullptr;  // per slot group: energy, then gradient groups
+    int max_groups = 0;                                   // groups per launch (one per member)
     float *d_gbuf = nullptr;    // gradients: slots x 2 x n_slot (current and trial alternate)
     float *d_dbuf = nullptr;    // search directions: slots x n_slot (BF16 under split precision)
     void *d_S = nullptr, *d_Y = nullptr;  // curvature pairs: slots x m x n_slot (BF16 or float)
@@ -282,6 +364,8 @@ struct sdca_ctx {
     float *h_rows = nullptr;
     int64_t *h_src_off = nullptr;
     int32_t *h_score_loci = nullptr;
+    int32_t *h_score_ld = nullptr, *h_score_aln = nullptr;  // row length and alignment of each scored solution
+    int32_t *d_score_ld = nullptr, *d_score_aln = nullptr;
     int32_t *h_wl = nullptr;
 
     // pair statistics (sdca_pairs.hip), built on first use

@@ -214,19 +214,7 @@ __device__ __forceinline__ void mma_tile(const uint8_t *sA0, const uint8_t *sA1,
     }
 }
 
-// XCD-aware tile order.  Workgroups are dispatched to the eight XCDs round-robin
-// (workgroup id % 8), so consecutive ids on one XCD are id, id+8, id+16, ...  Tiles are
-// grouped into super-tiles of GM x GN output tiles at one split-K index z; a super-tile
-// runs on a single XCD, whose L2 then serves each A rows-tile to GN workgroups and each
-// B rows-tile to GM workgroups.  Super-tiles are dealt to XCDs round-robin in the order
-// N fastest, then split-K, then M: the row tiles start in increasing order, so the rows
-// of a launch that take two MFMA passes (placed first) are not left for its end.
-struct TileMap {
-    int Mt, Nt, Z;   // output tiles along M and N, split-K slices
-    int GM, GN;      // super-tile shape
-    int MB, NB, S;   // super-tiles along M and N, total super-tiles
-};
-
+// Tile (mt, nt) and split-K slice z of workgroup bid of a tile map (TileMap, sdca_internal.hpp).
 __device__ __forceinline__ bool map_tile(const TileMap &tm, int bid, int &mt, int &nt, int &z)
 {
     const int xcd = bid & 7, local = bid >> 3;
@@ -333,24 +321,29 @@ __device__ __forceinline__ void gemm_accum(uint8_t *sA0, uint8_t *sA1, uint8_t *
     }
 }
 
-// The packed rows come in three parts by precision level (each starting on a tile
-// boundary): tiles below m_split belong to fine commands and take three MFMA passes under
-// split precision (hi and mid together, then lo), tiles below m_coarse two (hi and lo),
-// the rest one pass over the BF16 high parts.  One launch serves all three; the dearer
-// tiles have the lower row indices and are dispatched first, so the cheaper ones fill the
-// end of the launch.
+// A round's GEMM launch covers the groups of its commands (one per alignment, GemmGroup);
+// workgroup bid belongs to the last group with tile0 <= bid.  Within a group the packed
+// rows come in three parts by precision level (each starting on a tile boundary): tiles
+// below parts.row[1] belong to fine commands and take three MFMA passes under split
+// precision (hi and mid together, then lo), tiles below parts.row[2] two (hi and lo), the
+// rest one pass over the BF16 high parts.  The dearer tiles have the lower row indices and
+// are dispatched first, so the cheaper ones fill the end of the launch.
 template <class Pol, int NA, int NI, class Epi>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gemm_nt(const typename Pol::T *__restrict__ A0,
                                                     const typename Pol::T *__restrict__ A1,
                                                     const typename Pol::T *__restrict__ A2, int64_t lda,
-                                                    const uint32_t *__restrict__ Bbits, int64_t ldbw, int K,
-                                                    int kchunk, int m_split, int m_coarse, TileMap tm, Epi epi)
+                                                    const GemmGroup *__restrict__ groups, int ngroups, Epi epi)
 {
     constexpr int TN = 64 * NI;
     __shared__ __attribute__((aligned(16))) uint8_t sA[NA][kTile * ROWB];
     __shared__ __attribute__((aligned(16))) uint8_t sB[TN * ROWB];
+    const int bid = blockIdx.x;
+    int g = 0;
+    while (g + 1 < ngroups && groups[g + 1].tile0 <= bid) ++g;
+    const GemmGroup &G = groups[g];
     int mt, nt, z;
-    if (!map_tile(tm, blockIdx.x, mt, nt, z)) return;
+    if (!map_tile(G.tm, bid - G.tile0, mt, nt, z)) return;
+    mt += G.parts.row[0] / kTile;  // absolute packed row tile
     floatx16 acc[2][NI];
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
@@ -359,19 +352,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[mi][ni][v] = 0.f;
     const int m0 = mt * kTile;
+    const uint32_t *Bbits = G.Bbits;
+    const int64_t ldbw = G.ldbw;
+    const int K = G.K, kchunk = G.kchunk;
     if constexpr (NA == 2) {
-        if (m0 >= m_coarse) {
+        if (m0 >= G.parts.row[2]) {
             gemm_accum<Pol, 1, NI>(sA[0], sA[0], sB, A0, A0, lda, Bbits, ldbw, K, kchunk, mt, nt, z, acc);
         } else {
             gemm_accum<Pol, 2, NI>(sA[0], sA[1], sB, A0, A1, lda, Bbits, ldbw, K, kchunk, mt, nt, z, acc);
-            if (m0 < m_split)
+            if (m0 < G.parts.row[1])
                 gemm_accum<Pol, 1, NI>(sA[0], sA[0], sB, A2, A2, lda, Bbits, ldbw, K, kchunk, mt, nt, z, acc);
         }
     } else {
         gemm_accum<Pol, 1, NI>(sA[0], sA[0], sB, A0, A0, lda, Bbits, ldbw, K, kchunk, mt, nt, z, acc);
     }
     const int tid = threadIdx.x, wave = tid >> 6;
-    epi.template tile<NI>(m0, (wave >> 1) * 64, (int64_t)nt * TN + (wave & 1) * (TN / 2), tid & 63, tid, acc, z);
+    epi.template tile<NI>(G, m0, (wave >> 1) * 64, (int64_t)nt * TN + (wave & 1) * (TN / 2), tid & 63, tid, acc, z);
 }
 
 // 32x32 MFMA result layout: register v of lane l holds row 8*(v/4) + 4*(l/32) + v%4,
@@ -386,8 +382,8 @@ struct EpiEnergy {
     float *E;
     int64_t ld, zstride;
     template <int NI>
-    __device__ __forceinline__ void tile(int m0, int wm, int64_t nw, int lane, int, floatx16 (&acc)[2][NI],
-                                         int z) const
+    __device__ __forceinline__ void tile(const GemmGroup &, int m0, int wm, int64_t nw, int lane, int,
+                                         floatx16 (&acc)[2][NI], int z) const
     {
         float *base = E + (int64_t)z * zstride + nw + (lane & 31);
 #pragma unroll
@@ -415,32 +411,32 @@ struct EpiGrad {
     const float *Xtrial;
     const SlotCmd *cmds;
     const double *hpart;
-    int64_t n_slot, Kxp;
-    int ng, g0, q, Kx, nbchunk;  // gradients per slot, ring index of the first
-    Parts parts;
-    float lamJ2;
-    double lamh2;
-    // command owning packed row m (the number of commands for a pad row) and its first row
-    __device__ __forceinline__ int cmd_of(int m) const
+    const AlnDev *aln;
+    int64_t n_slot, ldx;
+    int ng, g0, nbchunk;  // gradients per slot, ring index of the first; softmax blocks per command
+    // command owning packed row m of group G (G's end command for a pad row) and its first row
+    __device__ __forceinline__ static int cmd_of(const GemmGroup &G, int m)
     {
-        const int p = m >= parts.row[2] ? 2 : (m >= parts.row[1] ? 1 : 0);
-        const int i = (m - parts.row[p]) / q;
-        return i < parts.cmd[p + 1] - parts.cmd[p] ? parts.cmd[p] + i : parts.cmd[3];
+        const Parts &P = G.parts;
+        const int p = m >= P.row[2] ? 2 : (m >= P.row[1] ? 1 : 0);
+        const int i = (m - P.row[p]) / G.q;
+        return i < P.cmd[p + 1] - P.cmd[p] ? P.cmd[p] + i : P.cmd[3];
     }
-    __device__ __forceinline__ int row_of(int i) const
+    __device__ __forceinline__ static int row_of(const GemmGroup &G, int i)
     {
-        const int p = i >= parts.cmd[2] ? 2 : (i >= parts.cmd[1] ? 1 : 0);
-        return parts.row[p] + (i - parts.cmd[p]) * q;
+        const Parts &P = G.parts;
+        const int p = i >= P.cmd[2] ? 2 : (i >= P.cmd[1] ? 1 : 0);
+        return P.row[p] + (i - P.cmd[p]) * G.q;
     }
     template <int NI>
-    __device__ __forceinline__ void tile(int m0, int wm, int64_t nw, int lane, int tid, floatx16 (&acc)[2][NI],
-                                         int) const
+    __device__ __forceinline__ void tile(const GemmGroup &G, int m0, int wm, int64_t nw, int lane, int tid,
+                                         floatx16 (&acc)[2][NI], int) const
     {
         // s_gt < 0: no gradient to write
         __shared__ int s_gt[kTile + 2], s_lo[kTile + 2], s_hi[kTile + 2], s_slot[kTile + 2];
-        const int ncmd = parts.cmd[3];
-        const int i0 = cmd_of(m0);
-        const int ns = min(ncmd - i0, (kTile - 1) / q + 2);  // commands under rows [m0, m0 + 128)
+        const int cend = G.parts.cmd[3];
+        const int i0 = cmd_of(G, m0);
+        const int ns = min(cend - i0, (kTile - 1) / G.q + 2);  // commands under rows [m0, m0 + 128)
         if (tid < ns) {
             const SlotCmd &c = cmds[i0 + tid];
             s_gt[tid] = (c.mode == MODE_IDLE || c.gt < g0 || c.gt >= g0 + ng) ? -1 : c.gt;
@@ -449,6 +445,10 @@ struct EpiGrad {
             s_slot[tid] = c.slot;
         }
         __syncthreads();
+        const AlnDev &A = aln[G.aln];
+        const int64_t Kx = A.Kx, Kxp = A.Kxp;
+        const float lamJ2 = A.lamJ2;
+        const double lamh2 = A.lamh2;
 #pragma unroll
         for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
@@ -459,17 +459,17 @@ struct EpiGrad {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int m = m0 + wm + acc_row(mi, vq + u, lane);
-                    const int i = cmd_of(m);
+                    const int i = cmd_of(G, m);
                     const int il = i - i0;
                     const bool ok = il < ns;
                     gt[u] = ok ? s_gt[il] : -1;
                     lo[u] = ok ? s_lo[il] : 0;
                     hi[u] = ok ? s_hi[il] : 0;
                     cm[u] = i;
-                    srow[u] = ok ? m - row_of(i) : 0;
+                    srow[u] = ok ? m - row_of(G, i) : 0;
                     const int slot = ok ? s_slot[il] : 0;
                     roff[u] = ((int64_t)slot * ng + (gt[u] < 0 ? 0 : gt[u] - g0)) * n_slot + (int64_t)srow[u] * Kxp;
-                    const float *xr = Xtrial + (int64_t)m * Kxp;
+                    const float *xr = Xtrial + (int64_t)m * ldx;
 #pragma unroll
                     for (int ni = 0; ni < NI; ++ni) {
                         const int64_t n = nw + ni * 32 + (lane & 31);
@@ -483,11 +483,12 @@ struct EpiGrad {
 #pragma unroll
                     for (int ni = 0; ni < NI; ++ni) {
                         const int64_t n = nw + ni * 32 + (lane & 31);
+                        if (n >= Kxp) continue;
                         float val = 0.f;
                         if (n == Kx) {
-                            const double *hp = hpart + (int64_t)cm[u] * nbchunk * q + srow[u];
+                            const double *hp = hpart + (int64_t)cm[u] * nbchunk * 8 + srow[u];
                             double hs = 0.0;
-                            for (int ch = 0; ch < nbchunk; ++ch) hs += hp[(int64_t)ch * q];
+                            for (int ch = 0; ch < nbchunk; ++ch) hs += hp[(int64_t)ch * 8];
                             val = (float)(hs + lamh2 * (double)xv[u][ni]);
                         } else if (n < Kx && !(n >= lo[u] && n < hi[u])) {
                             val = acc[mi][ni][vq + u] + lamJ2 * xv[u][ni];
@@ -509,18 +510,22 @@ struct EpiGrad {
 // hardware exp2/log2 approximations; fine ones the accurate expf/logf and P = e_s / z.
 // The data term and, per state, sum_b G_b(s) (the fields' data gradient, :151) are
 // reduced in FP64 per block of 256 sequences.
-__global__ __launch_bounds__(256) void k_softmax(const float *__restrict__ Epart, int ksplit, int64_t zstride, int Bp,
-                                                 int B, int q, const SlotCmd *__restrict__ cmds,
-                                                 const uint8_t *__restrict__ statesT, const float *__restrict__ wts,
+__global__ __launch_bounds__(256) void k_softmax(const float *__restrict__ Epart, int64_t zstride, int64_t ldb,
+                                                 const SlotCmd *__restrict__ cmds, const AlnDev *__restrict__ aln,
                                                  float *__restrict__ G32, uint16_t *__restrict__ Ghi,
                                                  uint16_t *__restrict__ Glo, uint16_t *__restrict__ Gl3,
                                                  double *__restrict__ fpart, double *__restrict__ hpart, int nbchunk)
 {
     const int i = blockIdx.y;
     const int b = blockIdx.x * 256 + threadIdx.x;
-    const int r = cmds[i].mode == MODE_IDLE ? -1 : cmds[i].locus;
-    const int64_t row0 = cmds[i].row;
-    const int level = cmds[i].level;
+    const SlotCmd &c = cmds[i];
+    const int r = c.mode == MODE_IDLE ? -1 : c.locus;
+    const int64_t row0 = c.row;
+    const int level = c.level, q = c.q, ksplit = c.ksplit;
+    const AlnDev &A = aln[c.aln];
+    const int B = A.B, Bp = A.Bp;
+    const uint8_t *statesT = A.statesT;
+    const float *wts = A.w;
     double fb = 0.0;
     double hs[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (b < Bp) {
@@ -530,7 +535,7 @@ __global__ __launch_bounds__(256) void k_softmax(const float *__restrict__ Epart
             for (int s = 0; s < 8; ++s) {
                 if (s >= q) break;
                 float acc = 0.f;
-                for (int z = 0; z < ksplit; ++z) acc += Epart[(int64_t)z * zstride + (row0 + s) * Bp + b];
+                for (int z = 0; z < ksplit; ++z) acc += Epart[(int64_t)z * zstride + (row0 + s) * ldb + b];
                 e[s] = acc;
             }
             const float w = wts[b];
@@ -582,7 +587,7 @@ __global__ __launch_bounds__(256) void k_softmax(const float *__restrict__ Epart
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             if (s >= q) break;
-            const int64_t o = (row0 + s) * Bp + b;
+            const int64_t o = (row0 + s) * ldb + b;
             if (G32) {
                 G32[o] = g[s];
             } else {
@@ -612,9 +617,9 @@ __global__ __launch_bounds__(256) void k_softmax(const float *__restrict__ Epart
     }
     __syncthreads();
     if (threadIdx.x == 0) fpart[(int64_t)i * nbchunk + blockIdx.x] = red[0][8] + red[1][8] + red[2][8] + red[3][8];
-    if (threadIdx.x < q) {
+    if (threadIdx.x < 8) {
         const int s = threadIdx.x;
-        hpart[((int64_t)i * nbchunk + blockIdx.x) * q + s] = red[0][s] + red[1][s] + red[2][s] + red[3][s];
+        hpart[((int64_t)i * nbchunk + blockIdx.x) * 8 + s] = red[0][s] + red[1][s] + red[2][s] + red[3][s];
     }
 }
 
@@ -747,7 +752,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
                                                 float *__restrict__ Xc, float *__restrict__ Xtr,
                                                 uint16_t *__restrict__ Whi, uint16_t *__restrict__ Wlo,
                                                 uint16_t *__restrict__ Wl3,
-                                                const float *__restrict__ X0, int64_t n_slot, int Kxp, int Kx, int q,
+                                                const float *__restrict__ X0, int64_t n_slot, int64_t ldx,
                                                 double *__restrict__ part, int nchunk)
 {
     constexpr int NV = 3;
@@ -764,7 +769,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
     const bool ok = mode == MODE_INIT || mode == MODE_RETRY ||
                     (mode == MODE_NEWDIR && (c.gk == 0 || c.gk == 1) && c.pnew < M);
     if (mode != MODE_IDLE && ok) {
-        const int64_t i = c.slot, xo = (int64_t)c.row * Kxp;
+        const int64_t i = c.slot, xo = (int64_t)c.row * ldx;
+        const uint32_t Kxp = (uint32_t)c.Kxp, Kx = (uint32_t)c.Kx;
         float *xc = Xc + i * n_slot;
         float *xt = Xtr + xo;
         PT *dv = dbuf + (int64_t)i * n_slot;
@@ -775,7 +781,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
         PT *Yb = Y + (int64_t)i * M * n_slot;
         const float alpha = c.alpha, pstep = c.pstep, cd = c.cd, cg = c.cg;
         const int pnew = c.pnew;
-        const int64_t n8 = n_slot >> 3;
+        const int64_t n8 = ((int64_t)c.q * Kxp) >> 3;  // the command's q x Kxp elements
         const bool accept = c.accept != 0;
         const int level = c.level;  // BF16 parts the GEMMs read: 1 (coarse), 2 (split) or 3 (fine)
         const bool from_x0 = c.init_src != 0.f && x0;
@@ -787,6 +793,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
         }
         for (int64_t e8 = (int64_t)blockIdx.x * 256 + threadIdx.x; e8 < n8; e8 += (int64_t)nchunk * 256) {
             const int64_t e = e8 * 8;
+            // row s and column of the element in the command's q x Kxp vector, and its place
+            // in the packed trial-point rows (K_x is padded to a multiple of 8, so an
+            // 8-element chunk lies in one row; a slot vector has fewer than 2^32 elements)
+            const uint32_t srow = (uint32_t)e / Kxp, col = (uint32_t)e - srow * Kxp;
+            const int64_t pe = (int64_t)srow * ldx + col;
             f8 xn;
             if (mode == MODE_INIT) {
                 xn = from_x0 ? ld8(x0, e) : zero8();
@@ -827,7 +838,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
                 acc[2] += dot8(d, g);
                 xn = step8(xv, alpha, d);
             }
-            st8(xt, e, xn);
+            st8(xt, pe, xn);
             const float xs[8] = {xn.a.x, xn.a.y, xn.a.z, xn.a.w, xn.b.x, xn.b.y, xn.b.z, xn.b.w};
             if (Whi) {
                 // x = hi + lo (+ l3): each part is the rounded remainder of the previous ones,
@@ -846,15 +857,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
                 hv.x = h[0], hv.y = h[1], hv.z = h[2], hv.w = h[3];
                 lv.x = l[0], lv.y = l[1], lv.z = l[2], lv.w = l[3];
                 tv.x = t[0], tv.y = t[1], tv.z = t[2], tv.w = t[3];
-                __builtin_nontemporal_store(hv, reinterpret_cast<u32x4 *>(Whi + xo + e));
-                if (level != LV_COARSE) __builtin_nontemporal_store(lv, reinterpret_cast<u32x4 *>(Wlo + xo + e));
-                if (level == LV_FINE) __builtin_nontemporal_store(tv, reinterpret_cast<u32x4 *>(Wl3 + xo + e));
+                __builtin_nontemporal_store(hv, reinterpret_cast<u32x4 *>(Whi + xo + pe));
+                if (level != LV_COARSE) __builtin_nontemporal_store(lv, reinterpret_cast<u32x4 *>(Wlo + xo + pe));
+                if (level == LV_FINE) __builtin_nontemporal_store(tv, reinterpret_cast<u32x4 *>(Wl3 + xo + pe));
             }
             acc[0] += dot8(xn, xn);
-            // the h entry of row s is column Kx (K_x is padded to a multiple of 8, so an
-            // 8-element chunk lies in one row; a slot vector has fewer than 2^32 elements)
-            const uint32_t col = (uint32_t)e % (uint32_t)Kxp;
-            if ((uint32_t)Kx >= col && (uint32_t)Kx < col + 8) {
+            // the h entry of row s is column Kx
+            if (Kx >= col && Kx < col + 8) {
                 const float hv = xs[Kx - col];
                 acc[1] += (double)hv * hv;
             }
@@ -870,8 +879,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M > 10 ? 1 
 template <int M, class PT>
 __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, const float *__restrict__ gbuf,
                                               const PT *__restrict__ dbuf, const PT *__restrict__ S,
-                                              const PT *__restrict__ Y, int64_t n_slot, int Kxp, int Kx,
-                                              double *__restrict__ part, int nchunk)
+                                              const PT *__restrict__ Y, int64_t n_slot, double *__restrict__ part,
+                                              int nchunk)
 {
     constexpr int NV = 2 * M + 5;
     const int j = blockIdx.y;
@@ -892,18 +901,19 @@ __global__ __launch_bounds__(256) void k_dots(const SlotCmd *__restrict__ cmds, 
         const PT *Sb = S + i * M * n_slot;
         const PT *Yb = Y + i * M * n_slot;
         const uint32_t live = c.live;
-        const int64_t n8 = n_slot >> 3;
+        const uint32_t Kxp = (uint32_t)c.Kxp, Kx = (uint32_t)c.Kx;
+        const int64_t n8 = ((int64_t)c.q * Kxp) >> 3;  // the command's q x Kxp elements
         for (int64_t e8 = (int64_t)blockIdx.x * 256 + threadIdx.x; e8 < n8; e8 += (int64_t)nchunk * 256) {
             const int64_t e = e8 * 8;
             const f8 gv = ld8(g, e);
             const float ga = amax8(gv);
             acc[2 * M + 3] = fmax(acc[2 * M + 3], (double)ga);
             // the coupling entries: the chunk that holds a row's field entry leaves it out
-            const uint32_t col = (uint32_t)e % (uint32_t)Kxp;
+            const uint32_t col = (uint32_t)e % Kxp;
             float gJ = ga;
-            if ((uint32_t)Kx >= col && (uint32_t)Kx < col + 8) {
+            if (Kx >= col && Kx < col + 8) {
                 const float gs[8] = {gv.a.x, gv.a.y, gv.a.z, gv.a.w, gv.b.x, gv.b.y, gv.b.z, gv.b.w};
-                const uint32_t hpos = (uint32_t)Kx - col;
+                const uint32_t hpos = Kx - col;
                 gJ = 0.f;
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
@@ -967,14 +977,19 @@ __global__ void k_reduce(int m, const double *__restrict__ upd_part, const doubl
 // plmDCA_utility.hpp:57-83 (on M^T for n < r), and the Frobenius norm over
 // i, k != gap (Matrix_math.hpp:134-178), accumulated in FP64 and stored as float.
 __global__ __launch_bounds__(256) void k_score(const float *__restrict__ xsrc, const int64_t *__restrict__ src_off,
-                                               const int32_t *__restrict__ loci, int L, int q, int gap, int Kxp,
-                                               const int32_t *__restrict__ state_col, float *__restrict__ rows)
+                                               const int32_t *__restrict__ src_ld, const int32_t *__restrict__ loci,
+                                               const int32_t *__restrict__ alns, int64_t ldrow,
+                                               const AlnDev *__restrict__ aln, float *__restrict__ rows)
 {
     const int k = blockIdx.y;
     const int n = blockIdx.x * 256 + threadIdx.x;
+    const AlnDev &Ad = aln[alns[k]];
+    const int L = Ad.L, q = Ad.q, gap = Ad.gap;
+    const int32_t *state_col = Ad.state_col;
+    const int64_t Kxp = src_ld[k];  // row length of the solution's q rows
     if (n >= L) return;
     const int r = loci[k];
-    float *row = rows + (int64_t)k * L;
+    float *row = rows + (int64_t)k * ldrow;
     if (n == r) {
         row[n] = 0.f;
         return;
@@ -1045,79 +1060,49 @@ void hamming_neighbours(hipStream_t st, const uint64_t *planes, int B, int q, in
     hipLaunchKernelGGL(k_hamming, dim3((unsigned)ntiles), dim3(256), 0, st, planes, B, q, W, L, dmax, d_mult, d_iw);
 }
 
-static TileMap make_map(int Mt, int Nt, int Z, int gm)
-{
-    TileMap tm;
-    tm.Mt = Mt;
-    tm.Nt = Nt;
-    tm.Z = Z;
-    if (gm < 1) gm = 8;
-    if (gm > 64) gm = 64;
-    tm.GM = Mt < gm ? Mt : gm;
-    tm.GN = Nt < 64 / tm.GM ? Nt : 64 / tm.GM;
-    tm.MB = (Mt + tm.GM - 1) / tm.GM;
-    tm.NB = (Nt + tm.GN - 1) / tm.GN;
-    tm.S = Z * tm.MB * tm.NB;
-    return tm;
-}
-
-static unsigned map_grid(const TileMap &tm) { return (unsigned)(((tm.S + 7) / 8) * 8 * tm.GM * tm.GN); }
-
-int energy_superiles(int Mpad, int Bp, int gm, int tn)
-{
-    const TileMap tm = make_map(Mpad / kTile, Bp / tn, 1, gm);
-    return tm.MB * tm.NB;
-}
-
-// Launches k_gemm_nt with output tiles 128 x tn (tn = 128 or 256; N must be a multiple).
+// Launches k_gemm_nt with output tiles 128 x tn (tn = 128 or 256; every group's N a multiple).
 template <class Pol, int NA, class Epi>
-static void launch_gemm(hipStream_t st, int tn, const TileMap &tm, const void *A0, const void *A1, const void *A2,
-                        int64_t lda, const uint32_t *Bbits, int64_t ldbw, int K, int kchunk, int m_split,
-                        int m_coarse, const Epi &epi)
+static void launch_gemm(hipStream_t st, int tn, int grid, const void *A0, const void *A1, const void *A2, int64_t lda,
+                        const GemmGroup *groups, int ngroups, const Epi &epi)
 {
     using T = typename Pol::T;
-    const dim3 grid(map_grid(tm));
+    if (grid <= 0) return;
     if (tn == 256)
-        hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 4, Epi>), grid, dim3(256), 0, st, (const T *)A0, (const T *)A1,
-                           (const T *)A2, lda, Bbits, ldbw, K, kchunk, m_split, m_coarse, tm, epi);
+        hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 4, Epi>), dim3(grid), dim3(256), 0, st, (const T *)A0, (const T *)A1,
+                           (const T *)A2, lda, groups, ngroups, epi);
     else
-        hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 2, Epi>), grid, dim3(256), 0, st, (const T *)A0, (const T *)A1,
-                           (const T *)A2, lda, Bbits, ldbw, K, kchunk, m_split, m_coarse, tm, epi);
+        hipLaunchKernelGGL((k_gemm_nt<Pol, NA, 2, Epi>), dim3(grid), dim3(256), 0, st, (const T *)A0, (const T *)A1,
+                           (const T *)A2, lda, groups, ngroups, epi);
 }
 
-void gemm_energy(hipStream_t st, int prec, const Parts &parts, const void *W32, const void *Whi, const void *Wlo,
-                 const void *Wl3, const uint32_t *X, int Mrows, int Mz, int Bp, int Kxp, int ksplit, int kchunk,
-                 int gm, int tn, float *Epart)
+void gemm_energy(hipStream_t st, int prec, const GemmGroup *d_groups, int ngroups, int grid, int tn, const void *W32,
+                 const void *Whi, const void *Wlo, const void *Wl3, int64_t ldx, int64_t ldb, int64_t zstride,
+                 float *Epart)
 {
-    EpiEnergy epi{Epart, (int64_t)Bp, (int64_t)Mz * Bp};
-    const TileMap tm = make_map(Mrows / kTile, Bp / tn, ksplit, gm);
+    EpiEnergy epi{Epart, ldb, zstride};
     if (prec == SDCA_PREC_FP32)
-        launch_gemm<PolF32, 1>(st, tn, tm, W32, W32, W32, Kxp, X, Kxp / 32, Kxp, kchunk, 0, Mrows, epi);
+        launch_gemm<PolF32, 1>(st, tn, grid, W32, W32, W32, ldx, d_groups, ngroups, epi);
     else
-        launch_gemm<PolBF16, 2>(st, tn, tm, Whi, Wlo, Wl3, Kxp, X, Kxp / 32, Kxp, kchunk, parts.row[1],
-                                parts.row[2], epi);
+        launch_gemm<PolBF16, 2>(st, tn, grid, Whi, Wlo, Wl3, ldx, d_groups, ngroups, epi);
 }
 
-void softmax(hipStream_t st, int prec, const float *Epart, int ksplit, int Mz, int Bp, int B, int q, int ncmd,
-             const SlotCmd *cmds, const uint8_t *statesT, const float *wts, float *G32, uint16_t *Ghi,
-             uint16_t *Glo, uint16_t *Gl3, double *fpart, double *hpart, int nbchunk)
+void softmax(hipStream_t st, int prec, const float *Epart, int64_t zstride, int64_t ldb, int ncmd, int nbchunk,
+             const SlotCmd *cmds, const AlnDev *d_aln, float *G32, uint16_t *Ghi, uint16_t *Glo, uint16_t *Gl3,
+             double *fpart, double *hpart)
 {
-    hipLaunchKernelGGL(k_softmax, dim3(nbchunk, ncmd), dim3(256), 0, st, Epart, ksplit, (int64_t)Mz * Bp, Bp, B, q,
-                       cmds, statesT, wts, prec == SDCA_PREC_FP32 ? G32 : nullptr, Ghi, Glo, Gl3, fpart, hpart,
-                       nbchunk);
+    hipLaunchKernelGGL(k_softmax, dim3(nbchunk, ncmd), dim3(256), 0, st, Epart, zstride, ldb, cmds, d_aln,
+                       prec == SDCA_PREC_FP32 ? G32 : nullptr, Ghi, Glo, Gl3, fpart, hpart, nbchunk);
 }
 
-void gemm_grad(hipStream_t st, int prec, const Parts &parts, const void *G32, const void *Ghi, const void *Glo,
-               const void *Gl3, const uint32_t *Xt, int Mrows, int Bp, int Kxp, int Kx, int q, const SlotCmd *cmds,
-               const float *Xtrial, float *gbuf, int64_t n_slot, double lamJ2, double lamh2, const double *hpart,
-               int nbchunk, int gm, int tn)
+void gemm_grad(hipStream_t st, int prec, const GemmGroup *d_groups, int ngroups, int grid, int tn, const void *G32,
+               const void *Ghi, const void *Glo, const void *Gl3, int64_t ldb, const AlnDev *d_aln, const SlotCmd *cmds,
+               const float *Xtrial, int64_t ldx, float *gbuf, int64_t n_slot, const double *hpart, int nbchunk)
 {
-    EpiGrad epi{gbuf, Xtrial, cmds, hpart, n_slot, (int64_t)Kxp, 2, 0, q, Kx, nbchunk, parts, (float)lamJ2, lamh2};
-    const TileMap tm = make_map(Mrows / kTile, Kxp / tn, 1, gm);
+    EpiGrad epi{gbuf, Xtrial, cmds, hpart, d_aln, n_slot, ldx, 2, 0, nbchunk};
     if (prec == SDCA_PREC_FP32)
-        launch_gemm<PolF32, 1>(st, tn, tm, G32, G32, G32, Bp, Xt, Bp / 32, Bp, Bp, 0, Mrows, epi);
+        launch_gemm<PolF32, 1>(st, tn, grid, G32, G32, G32, ldb, d_groups, ngroups, epi);
     else
-        launch_gemm<PolBF16, 2>(st, tn, tm, Ghi, Glo, Gl3, Bp, Xt, Bp / 32, Bp, Bp, parts.row[1], parts.row[2], epi);
+        launch_gemm<PolBF16, 2>(st, tn, grid, Ghi, Glo, Gl3, ldb, d_groups, ngroups, epi);
 }
 
 #define SDCA_DISPATCH_M(m, CALL) \
@@ -1131,35 +1116,32 @@ void gemm_grad(hipStream_t st, int prec, const Parts &parts, const void *G32, co
     default: break;              \
     }
 
-void update(hipStream_t st, int m, int q, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
-            void *dbuf, void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, uint16_t *Wl3,
-            const float *X0, int64_t n_slot, int Kxp, int Kx, double *part, int nchunk)
+void update(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf, void *dbuf,
+            void *S, void *Y, float *Xc, float *Xtr, uint16_t *Whi, uint16_t *Wlo, uint16_t *Wl3, const float *X0,
+            int64_t n_slot, int64_t ldx, double *part, int nchunk)
 {
 #define SDCA_UPD(MM)                                                                                                \
     if (pair_bf16)                                                                                                  \
         hipLaunchKernelGGL((k_update<MM, uint16_t>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf,          \
-                           (uint16_t *)dbuf, (uint16_t *)S, (uint16_t *)Y, Xc, Xtr, Whi, Wlo, Wl3, X0, n_slot, Kxp, \
-                           Kx, q, part, nchunk);                                                                    \
+                           (uint16_t *)dbuf, (uint16_t *)S, (uint16_t *)Y, Xc, Xtr, Whi, Wlo, Wl3, X0, n_slot, ldx, \
+                           part, nchunk);                                                                           \
     else                                                                                                            \
         hipLaunchKernelGGL((k_update<MM, float>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf, (float *)dbuf, \
-                           (float *)S, (float *)Y, Xc, Xtr, Whi, Wlo, Wl3, X0, n_slot, Kxp, Kx, q, part, \
-                           nchunk)
+                           (float *)S, (float *)Y, Xc, Xtr, Whi, Wlo, Wl3, X0, n_slot, ldx, part, nchunk)
     SDCA_DISPATCH_M(m, SDCA_UPD)
 #undef SDCA_UPD
 }
 
 void dots(hipStream_t st, int m, bool pair_bf16, const SlotCmd *cmds, int ncmd, const float *gbuf,
-          const void *dbuf, const void *S, const void *Y, int64_t n_slot, int Kxp, int Kx, double *part,
-          int nchunk)
+          const void *dbuf, const void *S, const void *Y, int64_t n_slot, double *part, int nchunk)
 {
 #define SDCA_DOT(MM)                                                                                                \
     if (pair_bf16)                                                                                                  \
         hipLaunchKernelGGL((k_dots<MM, uint16_t>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf,            \
-                           (const uint16_t *)dbuf, (const uint16_t *)S, (const uint16_t *)Y, n_slot, Kxp, Kx, part, \
-                           nchunk);                                                                                 \
+                           (const uint16_t *)dbuf, (const uint16_t *)S, (const uint16_t *)Y, n_slot, part, nchunk); \
     else                                                                                                            \
         hipLaunchKernelGGL((k_dots<MM, float>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, gbuf,               \
-                           (const float *)dbuf, (const float *)S, (const float *)Y, n_slot, Kxp, Kx, part, nchunk)
+                           (const float *)dbuf, (const float *)S, (const float *)Y, n_slot, part, nchunk)
     SDCA_DISPATCH_M(m, SDCA_DOT)
 #undef SDCA_DOT
 }
@@ -1172,11 +1154,11 @@ void reduce_results(hipStream_t st, int m, int ncmd, const double *upd_part, con
                        nbchunk, res);
 }
 
-void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *loci, int nscore, int L, int q,
-           int gap, int Kxp, const int32_t *state_col, float *rows)
+void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *src_ld, const int32_t *loci,
+           const int32_t *alns, int nscore, int Lmax, int64_t ldrow, const AlnDev *d_aln, float *rows)
 {
-    hipLaunchKernelGGL(k_score, dim3(cdiv(L, 256), nscore), dim3(256), 0, st, xsrc, src_off, loci, L, q, gap, Kxp,
-                       state_col, rows);
+    hipLaunchKernelGGL(k_score, dim3(cdiv(Lmax, 256), nscore), dim3(256), 0, st, xsrc, src_off, src_ld, loci, alns,
+                       ldrow, d_aln, rows);
 }
 
 }  // namespace kern
