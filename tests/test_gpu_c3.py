@@ -104,7 +104,8 @@ def gapless_two_state(B=300, L=700, clades=12, seed=77):
 def test_c3_all_alignments_in_one_solve(kw):
     # the 48 alignments of the uneven batch (envelopes 3, 4 and 5) and a gapless two-state
     # alignment, their loci interleaved in one multi-alignment solve: every round's GEMM
-    # launches carry one tile group per alignment in flight
+    # launches carry one tile group per alignment in flight.  The coupling threshold is 1e-6:
+    # rows of up to 20,000 entries reach down to small scores in their top 200
     fx = np.load(FIXTURE)
     cfgs = synth.c3_workloads()
     ctxs, items = [], []
@@ -129,7 +130,7 @@ def test_c3_all_alignments_in_one_solve(kw):
     alns = np.array([items[i][0] for i in order], dtype=np.int32)
     loci = np.array([items[i][1] for i in order], dtype=np.int32)
     print(f"{len(items)} loci of {len(ctxs)} alignments, q_env {sorted(set(c.q for c in ctxs))}", flush=True)
-    with sd.Multi(ctxs, j_threshold=3e-6, batch_loci=256, **kw) as mu:
+    with sd.Multi(ctxs, j_threshold=1e-6, batch_loci=256, **kw) as mu:
         rows, stats, rc = mu.solve(alns, loci)
         tm = mu.timing()
     assert rc in (sd.SDCA_OK, sd.SDCA_E_NOCONV)
@@ -144,6 +145,9 @@ def test_c3_all_alignments_in_one_solve(kw):
             got = rows[u][fx[f"top_idx_{k}"][j]]
             rel = np.abs(got - fx[f"top_val_{k}"][j]) / fx[f"top_val_{k}"][j]
             worst_f, worst_s = max(worst_f, frel), max(worst_s, rel.max())
+            if rel.max() >= worst_s:
+                where_worst = (k, int(r), float(fx[f"top_val_{k}"][j].min()), float(fx[f"top_val_{k}"][j].max()))
+    print(f"  worst score error at alignment, locus, row top-200 range {where_worst}")
     # the two-state alignment against the oracle in the test (no gap state: nothing excluded)
     ref, _, so = orc.solve_loci(st2, w2, lh2, lJ2, loci2, q=2, gap=-1, gtol=1e-9, max_iter=20000, nthreads=16)
     for j, r in enumerate(loci2):
