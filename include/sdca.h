@@ -77,8 +77,28 @@ typedef struct sdca_opts {
                                 gradient converges more slowly (the field column is the sum of every
                                 locus' one-hot columns) without moving the scores.  0 (default):
                                 SuperDCA's rule alone (plmDCA.hpp:218-221) */
-    int32_t reserved[2];
+    int32_t formulation;     /* SDCA_FORM_*: how a locus' couplings are represented during its
+                                solve (default SDCA_FORM_AUTO) */
+    int32_t reserved;
 } sdca_opts;
+
+/* Formulations of the per-locus solve (sdca_opts.formulation).
+ *   PRIMAL  the parameter vector itself (q x K_x couplings and fields), energies and
+ *           gradients as GEMMs against the one-hot alignment in opts->precision.
+ *   GRAM    the couplings as J = X_r^T c with c the q x B coefficients of the sequences: the
+ *           L2 term makes 2 lambda_J J = -X_r^T G at the optimum, and every L-BFGS iterate
+ *           from zero lies in the same span, so the iterates are those of the primal solve
+ *           while each evaluation costs a B x B x q product with the matrix of identical-locus
+ *           counts instead of two B x K_x x q ones.  FP64 throughout; cold starts only; the
+ *           stopping rule tests ||g||_2 and ||g_J||_2, which bound the infinity norms of the
+ *           primal rule from above (sdca_locus_stats.gnorm and gnorm_J report them).  The
+ *           couplings of a finished locus are formed once, by one contraction with FP32
+ *           products (three BF16 parts of its coefficients).
+ *   AUTO    GRAM when no warm start is given and B <= K_x for every alignment of the solve,
+ *           else PRIMAL. */
+#define SDCA_FORM_AUTO 0
+#define SDCA_FORM_PRIMAL 1
+#define SDCA_FORM_GRAM 2
 
 /* How sdca_solve_loci_mlp uses the network's coupling blocks (flags). */
 #define SDCA_WARM_DIRECTION 0 /* start at the fields; the blocks are the first search direction */
@@ -96,7 +116,8 @@ typedef struct sdca_locus_stats {
     int32_t iter;
     int32_t status;
     int32_t nfeval_first;  /* evaluations up to the first accepted step (start point included) */
-    double gnorm_J;        /* ||g||_inf over the coupling entries J_r (the fields h_r excluded) */
+    double gnorm_J;        /* ||g||_inf over the coupling entries J_r (the fields h_r excluded);
+                              under SDCA_FORM_GRAM gnorm and gnorm_J are 2-norms */
     int32_t nfeval_fine;   /* evaluations at the fine level (FP32-equivalent contractions) */
     int32_t reserved;
 } sdca_locus_stats;

@@ -27,6 +27,10 @@ SDCA_E_IO = -6
 
 PREC_FP32 = 0
 PREC_SPLIT_BF16 = 1
+# sdca_opts.formulation: AUTO picks GRAM when there is no warm start and B <= K_x
+FORM_AUTO = 0
+FORM_PRIMAL = 1
+FORM_GRAM = 2
 
 ST_CONVERGED = 0
 ST_ITER_CAP = 1
@@ -64,7 +68,7 @@ class Opts(C.Structure):
                 ("gemm_ksplit", C.c_int32), ("gemm_gm_energy", C.c_int32), ("gemm_gm_grad", C.c_int32),
                 ("gemm_tn_energy", C.c_int32), ("gemm_tn_grad", C.c_int32), ("pipeline_groups", C.c_int32),
                 ("coarse_factor", C.c_int32), ("fine_factor", C.c_int32), ("j_threshold", C.c_double),
-                ("reserved", C.c_int32 * 2)]
+                ("formulation", C.c_int32), ("reserved", C.c_int32)]
 
 
 class LocusStats(C.Structure):

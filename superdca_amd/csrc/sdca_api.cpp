@@ -168,6 +168,9 @@ sdca_ctx::~sdca_ctx()
     dfree(d_SP);
     dfree(d_topk);
     delete mlp;
+    delete gram;
+    dfree(d_gram);
+    dfree(d_w64);
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &g : gev)
@@ -176,6 +179,40 @@ sdca_ctx::~sdca_ctx()
     for (int g = 0; g < sdca::kMaxGroups; ++g)
         if (gstream[g] && gstream[g] != stream) (void)hipStreamDestroy(gstream[g]);
     if (stream) (void)hipStreamDestroy(stream);
+}
+
+sdca::GramWs::~GramWs()
+{
+    dfree(V);
+    dfree(Gp);
+    dfree(Zp);
+    dfree(upd_part);
+    dfree(sm_part);
+    dfree(dot_part);
+    dfree(res);
+    dfree(d_cmds);
+    dfree(d_galn);
+    dfree(d_aln);
+    dfree(M32);
+    dfree(Mout);
+    dfree(Mhi);
+    dfree(Mlo);
+    dfree(Ml3);
+    dfree(mhpart);
+    dfree(d_mcmds);
+    dfree(d_mgroups);
+    dfree(d_moff);
+    dfree(d_soff);
+    dfree(d_mrow);
+    dfree(d_maln);
+    dfree(d_sloc);
+    dfree(d_sld);
+    dfree(d_rows);
+    if (h_cmds) (void)hipHostFree(h_cmds);
+    if (h_res) (void)hipHostFree(h_res);
+    if (h_rows) (void)hipHostFree(h_rows);
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
 }
 
 namespace {
@@ -357,7 +394,7 @@ int upload_groups(sdca_ctx *c, int g, int ne, int ngr, hipStream_t st)
 }
 
 // The members' descriptors, uploaded before a solve (weights and lambda may have changed).
-int upload_alns(sdca_ctx *c)
+int upload_alns(sdca_ctx *c, sdca::AlnDev *dst)
 {
     std::vector<sdca::AlnDev> v(c->members.size());
     for (size_t a = 0; a < c->members.size(); ++a) {
@@ -378,9 +415,10 @@ int upload_alns(sdca_ctx *c)
         d.lamJ2 = (float)(2.0 * m->lambda_J);
         d.lamh2 = 2.0 * m->lambda_h;
     }
-    HIP_TRY(hipMemcpy(c->d_aln, v.data(), v.size() * sizeof(sdca::AlnDev), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dst, v.data(), v.size() * sizeof(sdca::AlnDev), hipMemcpyHostToDevice));
     return SDCA_OK;
 }
+int upload_alns(sdca_ctx *c) { return upload_alns(c, c->d_aln); }
 
 // Times the candidates on the workspace's own buffers (the first slot group, nslots
 // slots) with synthetic slot commands (every slot active, writing its first gradient) and
@@ -977,9 +1015,10 @@ constexpr double kFirstStep = 1.0 / 16.0;
 
 class Solver {
 public:
-    Solver(sdca_ctx *c, int nslots)
-        : c_(c), n_(nslots), m_(c->ws_m), slots_(nslots), cmds_(nslots, idle_cmd()),
-          split_(c->opts.precision == SDCA_PREC_SPLIT_BF16),
+    // gram: FP64 evaluations of the Gram formulation (one precision level, FP64 rounding of f)
+    Solver(sdca_ctx *c, int nslots, int m, bool gram)
+        : c_(c), n_(nslots), m_(m), slots_(nslots), cmds_(nslots, idle_cmd()),
+          split_(!gram && c->opts.precision == SDCA_PREC_SPLIT_BF16), gram_(gram),
           coarse_gnorm_(split_ && c->opts.coarse_factor > 0 ? c->opts.coarse_factor * c->opts.grad_threshold : 0.0),
           jthr_(c->opts.j_threshold),
           fine_gnorm_(split_ && jthr_ > 0.0 ? c->opts.fine_factor * jthr_ : 0.0),
@@ -1117,7 +1156,7 @@ public:
         // less than its rounding noise while the directional derivative is still accurate;
         // a step is then accepted on the derivative alone, with f allowed to exceed f(0)
         // by at most the noise level of the slot's precision.
-        const double f_noise = split_ && S.level < sdca::LV_FINE ? 1e-5 : 1e-6;
+        const double f_noise = gram_ ? 1e-13 : (split_ && S.level < sdca::LV_FINE ? 1e-5 : 1e-6);
         const bool approx_wolfe = dg >= S.ms.gtol * S.dg0 && dg <= (2.0 * S.ms.ftol - 1.0) * S.dg0 &&
                                   f <= S.f + f_noise * std::fabs(S.f);
         // Within the noise level the measured f no longer orders the trial steps: the search
@@ -1360,12 +1399,425 @@ private:
     std::vector<Slot> slots_;
     std::vector<sdca::SlotCmd> cmds_;
     bool split_;  // split-BF16 contractions (else FP32)
+    bool gram_;   // Gram formulation: FP64 evaluations
     // split precision: coarse phase (BF16 high parts only) while ||g||_inf >= this; 0: none
     double coarse_gnorm_;
     double jthr_;        // coupling-gradient threshold (0: none)
     double fine_gnorm_;  // split precision with jthr_: the fine level starts below this ||g||_inf
     int max_level_;      // LV_FINE with a coupling threshold, else LV_SPLIT
 };
+
+// ------------------------------------------------------------ Gram formulation
+//
+// The solve in the representation of sdca_gram.hip: FP64 slot vectors of coefficients,
+// their images under K_r and the fields; per round one trial-point/softmax pass, one
+// B x B x q GEMM per alignment, one gradient-image/products pass, and the same host Solver
+// as the primal formulation.  A finished locus' couplings J = X_r^T c are formed once, by
+// the gradient contraction of the primal path without its L2 term, and scored.
+
+bool gram_wanted(const sdca_ctx *c, bool warm)
+{
+    const int f = c->opts.formulation;
+    if (f == SDCA_FORM_PRIMAL || warm) return false;
+    if (f == SDCA_FORM_GRAM) return true;
+    for (const sdca_ctx *A : c->members)
+        if ((int64_t)A->B > A->Kx) return false;
+    return true;
+}
+
+// Identical-locus counts of alignment A over the loci its energies use: all of them (a
+// command removes its own locus), or in a two-alignment scan those of the second alignment
+// (plmDCA_objective.hpp:67: the first alignment's blocks are excluded).
+int ensure_gram_matrix(sdca_ctx *A, hipStream_t st)
+{
+    const int j0 = A->scan_split > 0 ? A->scan_split : 0;
+    if (A->d_gram && A->gram_split == j0) return SDCA_OK;
+    dfree(A->d_gram);
+    A->gram_split = -1;
+    int rc;
+    if ((rc = dalloc(&A->d_gram, (size_t)A->Bp * A->Bp))) return rc;
+    const int W = (A->L + 63) / 64;
+    uint64_t *planes = nullptr;
+    if ((rc = dalloc(&planes, (size_t)A->B * A->q * W))) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    sdca::kern::bitplanes(st, A->d_states, A->B, A->L, A->q, W, planes);
+    sdca::gram::gram_matrix(st, planes, A->B, A->q, W, j0, A->Bp, A->d_gram);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    dfree(planes);
+    if (e != hipSuccess) return fail(SDCA_E_HIP, "identical-locus counts failed: %s", hipGetErrorString(e));
+    A->gram_split = j0;
+    return SDCA_OK;
+}
+
+int ensure_gram_ws(sdca_ctx *c, int nslots, int m)
+{
+    int gq = 0, gBp = 0, gL = 0;
+    int64_t nitem = 0;
+    for (const sdca_ctx *A : c->members) {
+        gq = std::max(gq, A->q);
+        gBp = std::max(gBp, A->Bp);
+        gL = std::max(gL, A->L);
+        nitem = std::max(nitem, (int64_t)A->q * A->Kxp);
+    }
+    const int nmem = (int)c->members.size();
+    sdca::GramWs *G = c->gram;
+    if (G && G->slots == nslots && G->m == m && G->gq == gq && G->gBp == gBp && G->gL == gL && G->nmem == nmem &&
+        G->n_item == nitem)
+        return SDCA_OK;
+    delete c->gram;
+    c->gram = G = new sdca::GramWs();
+    G->slots = nslots;
+    G->m = m;
+    G->gq = gq;
+    G->gBp = gBp;
+    G->gL = gL;
+    G->nmem = nmem;
+    G->n_item = nitem;
+    G->nvs = sdca::gram::vec_len(gq, gBp);
+    G->vstride = (sdca::gram::kVecPairs + 2 * m) * G->nvs;
+    G->ldG = gBp;
+    G->nchunk = gBp / 256;
+    G->nmat = std::min(nslots, 64);
+    // each alignment's rows of a batch of finished loci start on a tile boundary
+    G->mrows = (int)round_up((int64_t)G->nmat * gq, sdca::kTile) + std::min(nmem, G->nmat) * sdca::kTile;
+    const int W = sdca::ResLayout{m}.width();
+    const int64_t ncols = round_up((int64_t)nslots * gq, 128) + 128;  // the GEMM reads whole 128-column tiles
+    int rc;
+    if ((rc = dalloc(&G->V, (size_t)nslots * G->vstride)) || (rc = dalloc(&G->Gp, (size_t)ncols * G->ldG)) ||
+        (rc = dalloc(&G->Zp, (size_t)ncols * G->ldG)) ||
+        (rc = dalloc(&G->upd_part, (size_t)nslots * (G->nchunk + 1) * 3)) ||
+        (rc = dalloc(&G->sm_part, (size_t)nslots * G->nchunk * 65)) ||
+        (rc = dalloc(&G->dot_part, (size_t)nslots * G->nchunk * sdca::ResLayout{m}.ndots())) ||
+        (rc = dalloc(&G->res, (size_t)nslots * W)) || (rc = dalloc(&G->d_cmds, (size_t)nslots)) ||
+        (rc = dalloc(&G->d_galn, (size_t)nmem)) || (rc = dalloc(&G->d_aln, (size_t)nmem)))
+        return rc;
+    // the coupling contraction of finished loci: three BF16 parts (hi + mid + lo, the FP32
+    // product) at either precision setting
+    const size_t mel = (size_t)G->mrows * gBp;
+    if ((rc = dalloc(&G->Mhi, mel)) || (rc = dalloc(&G->Mlo, mel)) || (rc = dalloc(&G->Ml3, mel))) return rc;
+    if ((rc = dalloc(&G->Mout, (size_t)G->nmat * nitem)) || (rc = dalloc(&G->mhpart, (size_t)G->nmat * 8)) ||
+        (rc = dalloc(&G->d_mcmds, (size_t)G->nmat)) || (rc = dalloc(&G->d_mgroups, (size_t)nmem)) ||
+        (rc = dalloc(&G->d_moff, (size_t)G->nmat)) || (rc = dalloc(&G->d_soff, (size_t)G->nmat)) ||
+        (rc = dalloc(&G->d_mrow, (size_t)G->nmat)) || (rc = dalloc(&G->d_maln, (size_t)G->nmat)) ||
+        (rc = dalloc(&G->d_sloc, (size_t)G->nmat)) || (rc = dalloc(&G->d_sld, (size_t)G->nmat)) ||
+        (rc = dalloc(&G->d_rows, (size_t)G->nmat * gL)))
+        return rc;
+    HIP_TRY(hipHostMalloc((void **)&G->h_cmds, (size_t)nslots * sizeof(sdca::SlotCmd), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&G->h_res, (size_t)nslots * W * sizeof(double), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&G->h_rows, (size_t)G->nmat * gL * sizeof(float), hipHostMallocDefault));
+    for (auto &e : G->ev) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipDeviceSynchronize());
+    return SDCA_OK;
+}
+
+// Per solve: the members' FP64 weights and descriptors (weights and lambda may change
+// between solves).
+int upload_gram_alns(sdca_ctx *c)
+{
+    sdca::GramWs *G = c->gram;
+    std::vector<sdca::GramAln> v(c->members.size());
+    int rc;
+    for (size_t a = 0; a < c->members.size(); ++a) {
+        sdca_ctx *A = c->members[a];
+        if (!A->d_w64 && (rc = dalloc(&A->d_w64, (size_t)A->Bp))) return rc;
+        HIP_TRY(hipDeviceSynchronize());
+        HIP_TRY(hipMemcpy(A->d_w64, A->h_w.data(), (size_t)A->B * sizeof(double), hipMemcpyHostToDevice));
+        sdca::GramAln &g = v[a];
+        g.K = A->d_gram;
+        g.statesT = A->d_statesT;
+        g.w = A->d_w64;
+        g.B = A->B;
+        g.Bp = A->Bp;
+        g.q = A->q;
+        g.own = A->scan_split > 0 ? 0 : 1;
+        g.lamJ2 = 2.0 * A->lambda_J;
+        g.lamh2 = 2.0 * A->lambda_h;
+    }
+    HIP_TRY(hipMemcpy(G->d_galn, v.data(), v.size() * sizeof(sdca::GramAln), hipMemcpyHostToDevice));
+    return upload_alns(c, G->d_aln);
+}
+
+// One round: the commands of the active slots (by alignment, in slot order), each owning q
+// consecutive columns of the packed GEMM operand.
+int gram_launch_round(sdca_ctx *c, const std::vector<sdca::SlotCmd> &cmds)
+{
+    sdca::GramWs *G = c->gram;
+    const int m = G->m, W = sdca::ResLayout{m}.width();
+    const int nmem = (int)c->members.size();
+    std::vector<int> &lst = G->cmd_slot;
+    lst.clear();
+    std::vector<int> col0(nmem, 0), ncol(nmem, 0);
+    int col = 0;
+    for (int a = 0; a < nmem; ++a) {
+        const sdca_ctx *A = c->members[a];
+        col0[a] = col;
+        for (size_t i = 0; i < cmds.size(); ++i) {
+            if (cmds[i].mode == sdca::MODE_IDLE || cmds[i].aln != a) continue;
+            sdca::SlotCmd &k = G->h_cmds[lst.size()];
+            k = cmds[i];
+            k.slot = (int)i;
+            k.row = col;
+            k.q = A->q;
+            k.Kx = (int)A->Kx;
+            k.Kxp = (int)A->Kxp;
+            lst.push_back((int)i);
+            col += A->q;
+        }
+        ncol[a] = col - col0[a];
+    }
+    const int nc = (int)lst.size();
+    if (nc == 0) return SDCA_OK;
+    hipStream_t st = c->stream;
+    HIP_TRY(hipMemcpyAsync(G->d_cmds, G->h_cmds, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(G->ev[0], st));
+    sdca::gram::step(st, m, G->d_cmds, nc, G->d_galn, G->V, G->vstride, G->nvs, G->Gp, G->ldG, G->upd_part,
+                     G->sm_part, G->nchunk);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(G->ev[1], st));
+    for (int a = 0; a < nmem; ++a) {
+        if (!ncol[a]) continue;
+        const sdca_ctx *A = c->members[a];
+        sdca::gram::gemm(st, A->d_gram, A->Bp, G->Gp + (size_t)col0[a] * G->ldG, G->ldG,
+                         G->Zp + (size_t)col0[a] * G->ldG, G->ldG, A->Bp, ncol[a], A->Bp);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(G->ev[2], st));
+    sdca::gram::dots(st, m, G->d_cmds, nc, G->d_galn, G->V, G->vstride, G->nvs, G->Zp, G->ldG, G->sm_part,
+                     G->dot_part, G->nchunk);
+    sdca::gram::reduce(st, m, nc, G->upd_part, G->dot_part, G->sm_part, G->nchunk, G->res);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(G->ev[3], st));
+    HIP_TRY(hipMemcpyAsync(G->h_res, G->res, (size_t)nc * W * sizeof(double), hipMemcpyDeviceToHost, st));
+    return SDCA_OK;
+}
+
+// Waits for the round, moves its result rows to their slots (the norms: square roots of
+// the squared 2-norms the device summed) and accounts its time, flops and bytes.
+int gram_finish_round(sdca_ctx *c, const std::vector<sdca::SlotCmd> &cmds, std::vector<double> &res)
+{
+    sdca::GramWs *G = c->gram;
+    const std::vector<int> &lst = G->cmd_slot;
+    if (lst.empty()) return SDCA_OK;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const sdca::ResLayout R{G->m};
+    const int W = R.width();
+    for (size_t k = 0; k < lst.size(); ++k) {
+        double *o = res.data() + (size_t)lst[k] * W;
+        std::memcpy(o, G->h_res + k * W, W * sizeof(double));
+        o[R.ginf()] = std::sqrt(std::max(0.0, o[R.ginf()]));
+        o[R.gJinf()] = std::sqrt(std::max(0.0, o[R.gJinf()]));
+    }
+    auto &t = c->timing;
+    const double upd = elapsed(G->ev[0], G->ev[1]), gm = elapsed(G->ev[1], G->ev[2]), dts = elapsed(G->ev[2], G->ev[3]);
+    t.update += upd;
+    t.grad += gm;
+    t.dots += dts;
+    t.lbfgs += upd + dts;
+    t.evals += 1;
+    t.grad_launches += 1;
+    double flops = 0.0, bu = 0.0, bd = 0.0;
+    for (int i : lst) {
+        const sdca::SlotCmd &k = cmds[i];
+        const sdca_ctx *A = c->members[k.aln];
+        const double half = (double)A->q * A->Bp;  // coefficients (or image entries) of a vector
+        flops += 2.0 * A->B * (double)A->B * A->q;
+        int npairs = 0;
+        for (int p = 0; p < G->m; ++p) npairs += (k.live >> p) & 1u;
+        // vector accesses (coefficients and image) of the trial-point pass, 8 bytes each
+        double acc;
+        if (k.mode == sdca::MODE_INIT) acc = 2;
+        else if (k.mode == sdca::MODE_RETRY) acc = 3;
+        else {
+            acc = 3 + (k.accept != 0) + (k.pnew >= 0 ? 3 : 0) + 2;
+            for (int p = 0; p < G->m; ++p) {
+                if (p == k.pnew) continue;
+                acc += (k.cs[p] != 0.f) + (k.cy[p] != 0.f);
+            }
+        }
+        bu += 8.0 * half * (2.0 * acc + 2.0);  // + the beliefs and the gradient's coefficients
+        // products pass: Z, the trial image and gradient coefficients read, its image written,
+        // and the images of g_k, d and the live pairs
+        bd += 8.0 * half * (4.0 + (k.mode == sdca::MODE_INIT ? 0.0 : 2.0 + 2.0 * npairs));
+    }
+    t.flops_grad += flops;
+    t.bytes_update += bu;
+    t.bytes_dots += bd;
+    t.bytes_lbfgs += bu + bd;
+    return SDCA_OK;
+}
+
+// Couplings, score rows, x_out and statistics of the finished slots fin (their solutions
+// from x_t or x_c), nmat at a time.
+int gram_finish_loci(sdca_ctx *c, const Solver &S, const std::vector<int> &fin, const int32_t *alns,
+                     float *score_rows, const int64_t *row_off, double *x_out, sdca_locus_stats *st_out)
+{
+    sdca::GramWs *G = c->gram;
+    hipStream_t st = c->stream;
+    const int nmem = (int)c->members.size();
+    const int gm = c->opts.gemm_gm_grad > 0 ? c->opts.gemm_gm_grad : 2;
+    const int tn = c->opts.gemm_tn_grad > 0 ? c->opts.gemm_tn_grad : 256;
+    const size_t n = (size_t)c->L * c->q * c->q + c->q;
+    std::vector<float> xdev(x_out ? G->n_item : 0);
+    for (size_t u0 = 0; u0 < fin.size(); u0 += G->nmat) {
+        const int nb = (int)std::min<size_t>(G->nmat, fin.size() - u0);
+        // items by alignment
+        std::vector<int> order(nb);
+        for (int u = 0; u < nb; ++u) order[u] = fin[u0 + u];
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return S.slot(x).aln < S.slot(y).aln; });
+        std::vector<int64_t> moff(nb), soff(nb);
+        std::vector<int32_t> mrow(nb), maln(nb), sloc(nb), sld(nb);
+        std::vector<sdca::SlotCmd> mc(nb);
+        std::vector<sdca::GemmGroup> groups;
+        int row = 0;
+        for (int u = 0; u < nb;) {
+            const int a = S.slot(order[u]).aln;
+            const sdca_ctx *A = c->members[a];
+            row = (int)round_up(row, sdca::kTile);
+            sdca::Parts P;
+            P.row[0] = row;
+            P.cmd[0] = u;
+            int v = u;
+            for (; v < nb && S.slot(order[v]).aln == a; ++v) {
+                const Slot &sl = S.slot(order[v]);
+                moff[v] = (int64_t)order[v] * G->vstride +
+                          (sl.from_xt ? sdca::gram::kVecXt : sdca::gram::kVecXc) * G->nvs;
+                mrow[v] = row;
+                maln[v] = a;
+                sloc[v] = sl.r;
+                sld[v] = (int32_t)A->Kxp;
+                soff[v] = (int64_t)v * G->n_item;
+                sdca::SlotCmd &k = mc[v];
+                k = idle_cmd();
+                k.mode = sdca::MODE_INIT;
+                k.locus = sl.r;
+                k.self_lo = A->scan_split > 0 ? 0 : A->col_off[sl.r];
+                k.self_hi = A->scan_split > 0 ? A->col_off[A->scan_split] : A->col_off[sl.r + 1];
+                k.slot = v;
+                k.row = row;
+                k.level = sdca::LV_FINE;
+                k.aln = a;
+                k.q = A->q;
+                k.Kx = (int)A->Kx;
+                k.Kxp = (int)A->Kxp;
+                k.gt = 0;
+                row += A->q;
+            }
+            const int rend = (int)round_up(row, sdca::kTile);
+            P.row[1] = P.row[2] = rend;
+            P.cmd[1] = P.cmd[2] = P.cmd[3] = v;
+            groups.push_back(make_group(c, a, P, rend, false, 1, 0, gm, tn));
+            row = rend;
+            u = v;
+        }
+        if (row > G->mrows || (int)groups.size() > nmem) return fail(SDCA_E_STATE, "coupling batch exceeds its buffers");
+        const int grid = place_groups(groups.data(), (int)groups.size());
+        HIP_TRY(hipMemcpyAsync(G->d_moff, moff.data(), nb * 8, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(G->d_soff, soff.data(), nb * 8, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(G->d_mrow, mrow.data(), nb * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(G->d_maln, maln.data(), nb * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(G->d_sloc, sloc.data(), nb * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(G->d_sld, sld.data(), nb * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(G->d_mcmds, mc.data(), nb * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(G->d_mgroups, groups.data(), groups.size() * sizeof(sdca::GemmGroup),
+                               hipMemcpyHostToDevice, st));
+        HIP_TRY(hipEventRecord(G->ev[4], st));
+        sdca::gram::split(st, G->d_moff, G->d_mrow, G->d_maln, nb, G->nchunk, G->d_galn, G->V, G->gBp, nullptr,
+                          G->Mhi, G->Mlo, G->Ml3, G->mhpart);
+        sdca::kern::gemm_materialize(st, SDCA_PREC_SPLIT_BF16, G->d_mgroups, (int)groups.size(), grid, tn, G->M32,
+                                     G->Mhi, G->Mlo, G->Ml3, G->gBp, G->d_aln, G->d_mcmds, G->Mout, G->n_item,
+                                     G->mhpart);
+        sdca::kern::score(st, G->Mout, G->d_soff, G->d_sld, G->d_sloc, G->d_maln, nb, G->gL, G->gL, G->d_aln,
+                          G->d_rows);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(G->ev[5], st));
+        if (score_rows)
+            HIP_TRY(hipMemcpyAsync(G->h_rows, G->d_rows, (size_t)nb * G->gL * 4, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        c->timing.score += elapsed(G->ev[4], G->ev[5]);
+        for (int v = 0; v < nb; ++v) {
+            const Slot &sl = S.slot(order[v]);
+            const int La = c->members[sl.aln]->L;
+            if (score_rows) {
+                const int64_t off = row_off ? row_off[sl.item] : (int64_t)sl.item * La;
+                std::memcpy(score_rows + off, G->h_rows + (size_t)v * G->gL, (size_t)La * 4);
+            }
+            if (x_out) {
+                HIP_TRY(hipMemcpy(xdev.data(), G->Mout + (size_t)v * G->n_item, G->n_item * 4, hipMemcpyDeviceToHost));
+                dev_to_ref(c, sl.r, xdev.data(), x_out + (size_t)sl.item * n);
+            }
+            if (st_out) {
+                sdca_locus_stats &o = st_out[sl.item];
+                o.fval = sl.f;
+                o.gnorm = sl.gnorm;
+                o.nfeval = sl.nfeval;
+                o.iter = sl.iter;
+                o.status = sl.status;
+                o.nfeval_first = sl.nfeval_first;
+                o.gnorm_J = sl.gnormJ;
+                o.nfeval_fine = 0;
+                o.reserved = 0;
+            }
+        }
+    }
+    return SDCA_OK;
+}
+
+int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int32_t K, float *score_rows,
+                     const int64_t *row_off, double *x_out, sdca_locus_stats *st_out)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    for (sdca_ctx *A : c->members)
+        if ((rc = ensure_gram_matrix(A, c->stream))) return rc;
+    const int nslots = std::min(c->opts.batch_loci, K);
+    const int m = c->opts.history;
+    if ((rc = ensure_gram_ws(c, nslots, m))) return rc;
+    if ((rc = upload_gram_alns(c))) return rc;
+    const sdca::ResLayout R{m};
+    Solver S(c, nslots, m, true);
+    std::vector<double> res((size_t)nslots * R.width(), 0.0);
+    int next = 0, done = 0, noconv = 0, nonfinite = 0;
+    auto fill = [&](int i) {
+        if (next >= K) {
+            S.set_idle(i);
+            return;
+        }
+        const int item = next++;
+        S.assign(i, item, alns ? alns[item] : 0, loci[item], false);
+    };
+    for (int i = 0; i < nslots; ++i) fill(i);
+    int64_t rounds = 0;
+    std::vector<int> finished;
+    while (done < K) {
+        S.mark_live(0, nslots);
+        if ((rc = gram_launch_round(c, S.cmds()))) return rc;
+        if (c->gram->cmd_slot.empty()) break;
+        if ((rc = gram_finish_round(c, S.cmds(), res))) return rc;
+        ++rounds;
+        finished.clear();
+        for (int i = 0; i < nslots; ++i) {
+            if (S.slot(i).phase == PH_IDLE) continue;
+            if (S.process(i, res.data() + (size_t)i * R.width())) finished.push_back(i);
+        }
+        if (finished.empty()) continue;
+        if ((rc = gram_finish_loci(c, S, finished, alns, score_rows, row_off, x_out, st_out))) return rc;
+        for (int i : finished) {
+            const Slot &sl = S.slot(i);
+            noconv += (sl.status == SDCA_ST_ITER_CAP || sl.status == SDCA_ST_LS_FAIL);
+            nonfinite += (sl.status == SDCA_ST_NONFINITE);
+            ++done;
+            fill(i);
+        }
+        if (c->opts.verbose)
+            fprintf(stderr, "sdca: %d / %d loci solved after %lld rounds\n", done, K, (long long)rounds);
+    }
+    c->timing.total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (nonfinite) return fail(SDCA_E_NONFINITE, "%d loci reached a non-finite objective", nonfinite);
+    if (noconv) return fail(SDCA_E_NOCONV, "%d loci stopped without reaching the gradient threshold", noconv);
+    return SDCA_OK;
+}
 
 // One-hot int8 X8 (L*QP rows, b contiguous) and the marginal counts, built on first use.
 int ensure_x8(sdca_ctx *c)
@@ -1461,6 +1913,8 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
         return fail(SDCA_E_ARG, "pipeline_groups must be 0, 1 or 2");
     if (o.precision != SDCA_PREC_FP32 && o.precision != SDCA_PREC_SPLIT_BF16)
         return fail(SDCA_E_ARG, "unknown precision %d", o.precision);
+    if (o.formulation < SDCA_FORM_AUTO || o.formulation > SDCA_FORM_GRAM)
+        return fail(SDCA_E_ARG, "unknown formulation %d", o.formulation);
     if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 8, 10, 16 (got %d)", o.history);
     if (o.batch_loci < 1) return fail(SDCA_E_ARG, "batch_loci must be >= 1");
     if (o.max_iter < 1 || o.max_ls < 1) return fail(SDCA_E_ARG, "max_iter and max_ls must be >= 1");
@@ -1731,6 +2185,9 @@ int sdca_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
         }
     }
     if (K == 0) return SDCA_OK;
+    if (c->opts.formulation == SDCA_FORM_GRAM && (warm || cw))
+        return fail(SDCA_E_ARG, "the Gram formulation solves from a cold start only");
+    if (gram_wanted(c, warm || cw)) return gram_solve_batch(c, alns, loci, K, score_rows, row_off, x_out, st_out);
     const auto t0 = std::chrono::steady_clock::now();
     const int nslots = std::min(c->opts.batch_loci, K);
     if ((rc = ensure_workspace(c, nslots, c->opts.history))) return rc;
@@ -1753,7 +2210,7 @@ int sdca_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
     const size_t n = (size_t)c->L * c->q * c->q + c->q;
     const sdca::ResLayout R{m};
     const int ng = c->ngroups, gs = c->gslots;
-    Solver S(c, nslots);
+    Solver S(c, nslots, c->ws_m, false);
     std::vector<float> stage(warm ? c->n_slot : 0), xdev(x_out ? c->n_slot : 0);
     std::vector<int32_t> wl_slot[sdca::kMaxGroups], wl_item[sdca::kMaxGroups], wl_locus[sdca::kMaxGroups];
     int next = 0, done = 0, noconv = 0, nonfinite = 0;
@@ -2122,6 +2579,8 @@ int sdca_multi_create(sdca_multi **out, sdca_ctx *const *ctxs, int32_t n, const 
     else sdca_default_opts(&o);
     if (o.precision != SDCA_PREC_FP32 && o.precision != SDCA_PREC_SPLIT_BF16)
         return fail(SDCA_E_ARG, "unknown precision %d", o.precision);
+    if (o.formulation < SDCA_FORM_AUTO || o.formulation > SDCA_FORM_GRAM)
+        return fail(SDCA_E_ARG, "unknown formulation %d", o.formulation);
     if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 8, 10, 16 (got %d)", o.history);
     if (o.batch_loci < 1 || o.max_iter < 1 || o.max_ls < 1 || !(o.grad_threshold >= 0.0) || !(o.j_threshold >= 0.0) ||
         o.coarse_factor < 0 || o.fine_factor < 0)

@@ -1,0 +1,685 @@
+// sdca_gram.hip -- the Gram formulation of the per-locus solve (kernels).
+//
+// The objective of locus r (plmDCA_objective.hpp:86-244) is a weighted multinomial
+// log-loss plus lambda_h |h|^2 + lambda_J |J|^2.  Its stationarity condition in J reads
+// 2 lambda_J J = -X_r^T G, with G_b(s) = w_b (P_b(s) - [x_br == s]) and X_r the one-hot
+// alignment without locus r: the optimum lies in the span of the rows of X_r.  So does
+// every L-BFGS iterate started from zero (plmDCA.hpp:298), since steps and gradients are
+// combinations of gradients.  Each coupling vector v of the solve is therefore held as
+// coefficients c_v (q x B, J_v = X_r^T c_v) together with its image K_r c_v, where
+//   K_r[b][b'] = #{ j != r : x_bj == x_b'j } = K[b][b'] - [x_br == x_b'r]
+// is the Gram matrix of X_r.  Then
+//   energies     E_b(s) = h(s) + (K_r c_x)(s, b)                 (no product over K_x)
+//   gradient     g = (sum_b G_b + 2 lambda_h h,  G + 2 lambda_J c_x)  in coefficients
+//   its image    K_r G + 2 lambda_J K_r c_x                      (one B x B x q GEMM)
+//   products     <u, v> = u_h . v_h + c_u . (K_r c_v)             (the primal dot product)
+// so the L-BFGS recursion runs unchanged on the host, and a locus costs B^2 q instead of
+// B K_x q per evaluation.  K is shared by every locus of the alignment; K_r differs from
+// it by the locus' own column, T(s, a) = sum_{b: x_br = a} G_b(s), subtracted after the GEMM.
+// All vectors and reductions are FP64.
+//
+// Slot vectors (FP64, a command uses the first 2 q Bp + 8 entries of each):
+//   [0, q Bp)          coefficients c(s, b) at s * Bp + b (pads b >= B stay 0)
+//   [q Bp, 2 q Bp)     the image (K_r c)(s, b), same layout
+//   [2 q Bp, + q)      the fields h(s)
+// in the order x_c, x_t, g[0], g[1], d, s_0..s_{m-1}, y_0..y_{m-1}.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "sdca_internal.hpp"
+
+namespace sdca {
+namespace {
+
+using gram::kVecD;
+using gram::kVecG;
+using gram::kVecPairs;
+using gram::kVecXc;
+using gram::kVecXt;
+
+constexpr int kSm = 65;  // softmax partials per block: T (8 x 8) and the data term
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    return x;  // lane 0
+}
+
+// Sums NV values over a block of 256 threads into out[0..NV).
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&v)[NV], double *out)
+{
+    __shared__ double red[4][NV];
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        const double x = wave_sum(v[k]);
+        if ((threadIdx.x & 63) == 0) red[wave][k] = x;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < NV; k += blockDim.x) out[k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+}
+
+// ------------------------------------------------------------------ Gram matrix
+
+// K[i][j] = sum_v popcount(P_i^v & P_j^v) over the loci [j0, L) (bits of loci below j0
+// masked off): 64 x 64 lower-triangular tiles of sequence pairs, each thread 4 x 4 pairs,
+// both triangles written.  planes[b][v][w] bit t set iff x_b(64 w + t) == v.
+constexpr int GT = 64;
+constexpr int GWC = 4;
+__global__ __launch_bounds__(256) void k_gram(const uint64_t *__restrict__ planes, int B, int q, int W, int j0,
+                                              int64_t ldk, double *__restrict__ K)
+{
+    __shared__ uint64_t sI[GT][8][GWC];
+    __shared__ uint64_t sJ[GT][8][GWC];
+    const int64_t t = blockIdx.x;
+    int ti = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(ti + 1) * (ti + 2) / 2 <= t) ++ti;
+    while ((int64_t)ti * (ti + 1) / 2 > t) --ti;
+    const int tj = (int)(t - (int64_t)ti * (ti + 1) / 2);
+    const int i0 = ti * GT, jb0 = tj * GT;
+    const int tid = threadIdx.x;
+    const int tx = tid & 15, ty = tid >> 4;
+    const int wfirst = j0 >> 6;
+    const uint64_t first_mask = ~0ull << (j0 & 63);
+    int matches[4][4];
+    for (int a = 0; a < 4; ++a)
+        for (int c = 0; c < 4; ++c) matches[a][c] = 0;
+    for (int w0 = wfirst; w0 < W; w0 += GWC) {
+        __syncthreads();
+        for (int e = tid; e < GT * q * GWC; e += 256) {
+            const int s = e / (q * GWC);
+            const int rem = e - s * q * GWC;
+            const int v = rem / GWC;
+            const int ww = rem - v * GWC;
+            const int w = w0 + ww;
+            const uint64_t mask = w == wfirst ? first_mask : ~0ull;
+            const int gi = i0 + s, gj = jb0 + s;
+            sI[s][v][ww] = (gi < B && w < W) ? planes[((int64_t)gi * q + v) * W + w] & mask : 0ull;
+            sJ[s][v][ww] = (gj < B && w < W) ? planes[((int64_t)gj * q + v) * W + w] & mask : 0ull;
+        }
+        __syncthreads();
+        for (int v = 0; v < q; ++v)
+            for (int ww = 0; ww < GWC; ++ww) {
+                uint64_t pi[4], pj[4];
+                for (int a = 0; a < 4; ++a) pi[a] = sI[ty * 4 + a][v][ww];
+                for (int c = 0; c < 4; ++c) pj[c] = sJ[tx * 4 + c][v][ww];
+                for (int a = 0; a < 4; ++a)
+                    for (int c = 0; c < 4; ++c) matches[a][c] += __popcll(pi[a] & pj[c]);
+            }
+    }
+    for (int a = 0; a < 4; ++a)
+        for (int c = 0; c < 4; ++c) {
+            const int gi = i0 + ty * 4 + a, gj = jb0 + tx * 4 + c;
+            if (gi < B && gj < B && gi >= gj) {
+                const double v = (double)matches[a][c];
+                K[(int64_t)gi * ldk + gj] = v;
+                K[(int64_t)gj * ldk + gi] = v;
+            }
+        }
+}
+
+// ---------------------------------------------------------------- FP64 GEMM
+
+// C[n][m] = sum_k Bm[n][k] A[m][k] (both operands K-contiguous; with A = K symmetric this
+// is Z = K G per column n of the packed beliefs).  128 x 128 output tiles, four waves of
+// 64 x 64 (4 x 4 tiles of v_mfma_f64_16x16x4f64), K in steps of 16 through LDS (k-major,
+// double-buffered: one barrier per step).  m < M (a multiple of 128), k < Kd (a multiple
+// of 16); Bm rows [0, round_up(N, 128)) must be readable, only n < N is stored.
+typedef double dx4 __attribute__((ext_vector_type(4)));
+constexpr int DT = 128, DK = 16, DPAD = 16;
+
+__device__ __forceinline__ void ld8d(const double *p, double (&r)[8])
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const d2 v = *reinterpret_cast<const d2 *>(p + 2 * u);
+        r[2 * u] = v.x;
+        r[2 * u + 1] = v.y;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dgemm(const double *__restrict__ A, int64_t lda, const double *__restrict__ Bm,
+                                               int64_t ldb, double *__restrict__ C, int64_t ldc, int N, int Kd)
+{
+    __shared__ double sA[2][DK][DT + DPAD];
+    __shared__ double sB[2][DK][DT + DPAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * DT, n0 = blockIdx.x * DT;
+    const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+    // staging: thread -> tile row r, k half kh (eight doubles of each operand per step)
+    const int r = tid >> 1, kh = (tid & 1) * 8;
+    const double *ga = A + (int64_t)(m0 + r) * lda + kh;
+    const double *gb = Bm + (int64_t)(n0 + r) * ldb + kh;
+    double ra[8], rb[8];
+    ld8d(ga, ra);
+    ld8d(gb, rb);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        sA[0][kh + u][r] = ra[u];
+        sB[0][kh + u][r] = rb[u];
+    }
+    __syncthreads();
+    dx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
+    // D of a 16 x 16 tile holds row lk + 4 v of column li in lane (li + 16 lk), element v;
+    // A rows enter permuted (lane li supplies row 4 (li % 4) + li / 4), so that the lane's
+    // four values are rows 4 lk .. 4 lk + 3, contiguous in C
+    const int li = lane & 15, lk = lane >> 4;
+    const int ai = 4 * (li & 3) + (li >> 2);
+    for (int k0 = 0; k0 < Kd; k0 += DK) {
+        const int cur = (k0 / DK) & 1;
+        const bool more = k0 + DK < Kd;
+        if (more) {
+            ga += DK;
+            gb += DK;
+            ld8d(ga, ra);
+            ld8d(gb, rb);
+        }
+#pragma unroll
+        for (int kk = 0; kk < DK; kk += 4) {
+            double af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) af[i] = sA[cur][kk + lk][wm + i * 16 + ai];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bf[j] = sB[cur][kk + lk][wn + j * 16 + li];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                sA[cur ^ 1][kh + u][r] = ra[u];
+                sB[cur ^ 1][kh + u][r] = rb[u];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int n = n0 + wn + j * 16 + li;
+        if (n >= N) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int m = m0 + wm + i * 16 + 4 * lk;
+            *reinterpret_cast<dx4 *>(C + (int64_t)n * ldc + m) = acc[i][j];
+        }
+    }
+}
+
+// ------------------------------------------------------------- trial points
+
+struct StepArgs {
+    int mode, pnew;
+    bool accept;
+    double alpha, pstep, cd, cg;
+};
+
+// One element e of a slot's vectors (a coefficient, an image entry or a field), as
+// k_update does it for the primal vectors:
+//   INIT    x_c = 0, x_t = 0
+//   RETRY   x_t = x_c + alpha d
+//   NEWDIR  x_c += pstep d_old (accept); s_pnew = pstep d_old, y_pnew = g[gk] - g[1-gk];
+//           d = cd d_old + cg g_k + sum_p (cs[p] s_p + cy[p] y_p); x_t = x_c + alpha d
+// Returns x_t(e); dn and gv receive d(e) and g_k(e) (NEWDIR, else 0).  The operations are
+// linear, so an image entry follows its coefficients exactly.
+template <int M>
+__device__ __forceinline__ double step_elem(const StepArgs &a, const double (&cs)[M], const double (&cy)[M],
+                                            double *__restrict__ xc, double *__restrict__ xt,
+                                            double *__restrict__ dv, const double *__restrict__ gk,
+                                            const double *__restrict__ gold, double *__restrict__ Sb,
+                                            double *__restrict__ Yb, int64_t nvs, int64_t e, double &dn, double &gv)
+{
+    double xn;
+    dn = 0.0;
+    gv = 0.0;
+    if (a.mode == MODE_INIT) {
+        xn = 0.0;
+        xc[e] = 0.0;
+    } else if (a.mode == MODE_RETRY) {
+        xn = fma(a.alpha, dv[e], xc[e]);
+    } else {
+        const double dold = dv[e];
+        double xv = xc[e];
+        if (a.accept) {
+            xv = fma(a.pstep, dold, xv);
+            xc[e] = xv;
+        }
+        const double g = gk[e];
+        double sn = 0.0, yn = 0.0;
+        if (a.pnew >= 0) {
+            sn = a.pstep * dold;
+            yn = g - gold[e];
+            Sb[(int64_t)a.pnew * nvs + e] = sn;
+            Yb[(int64_t)a.pnew * nvs + e] = yn;
+        }
+        double d = a.cd * dold + a.cg * g;
+#pragma unroll
+        for (int p = 0; p < M; ++p) {
+            const bool mine = p == a.pnew;
+            if (cs[p] != 0.0) d += cs[p] * (mine ? sn : Sb[(int64_t)p * nvs + e]);
+            if (cy[p] != 0.0) d += cy[p] * (mine ? yn : Yb[(int64_t)p * nvs + e]);
+        }
+        dv[e] = d;
+        xn = fma(a.alpha, d, xv);
+        dn = d;
+        gv = g;
+    }
+    xt[e] = xn;
+    return xn;
+}
+
+template <int M>
+__device__ __forceinline__ void load_step(const SlotCmd &c, StepArgs &a, double (&cs)[M], double (&cy)[M])
+{
+    a.mode = c.mode;
+    a.pnew = c.pnew < M ? c.pnew : -1;
+    a.accept = c.accept != 0;
+    a.alpha = c.alpha;
+    a.pstep = c.pstep;
+    a.cd = c.cd;
+    a.cg = c.cg;
+#pragma unroll
+    for (int p = 0; p < M; ++p) {
+        cs[p] = c.cs[p];
+        cy[p] = c.cy[p];
+    }
+}
+
+struct SlotPtrs {
+    double *xc, *xt, *dv, *gt, *Sb, *Yb;
+    const double *gk, *gold;
+};
+
+__device__ __forceinline__ SlotPtrs slot_ptrs(const SlotCmd &c, double *V, int64_t vstride, int64_t nvs, int M)
+{
+    double *base = V + (int64_t)c.slot * vstride;
+    SlotPtrs p;
+    p.xc = base + kVecXc * nvs;
+    p.xt = base + kVecXt * nvs;
+    p.dv = base + kVecD * nvs;
+    p.gt = base + (kVecG + (c.gt == 1)) * nvs;
+    p.gk = base + (kVecG + (c.gk == 1)) * nvs;     // NEWDIR: the current gradient
+    p.gold = base + (kVecG + (c.gk != 1)) * nvs;   // NEWDIR: the previous one
+    p.Sb = base + kVecPairs * nvs;
+    p.Yb = p.Sb + (int64_t)M * nvs;
+    return p;
+}
+
+// The fields of every command's trial point (one wave per command): the field entries
+// step first, so that the softmax blocks of k_gstep all read the final h of x_t.
+// upd_part[command][nchunk] = {0, |h_t|^2, d_h . g_k,h}.
+template <int M>
+__global__ __launch_bounds__(64) void k_ghstep(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                               double *__restrict__ V, int64_t vstride, int64_t nvs,
+                                               double *__restrict__ upd_part, int nchunk)
+{
+    const int j = blockIdx.x;
+    const SlotCmd &c = cmds[j];
+    const int s = threadIdx.x;
+    double xsqh = 0.0, dgh = 0.0;
+    if (c.mode != MODE_IDLE) {
+        const GramAln &A = aln[c.aln];
+        if (s < A.q) {
+            StepArgs a;
+            double cs[M], cy[M];
+            load_step<M>(c, a, cs, cy);
+            const SlotPtrs P = slot_ptrs(c, V, vstride, nvs, M);
+            const int64_t e = 2 * (int64_t)A.q * A.Bp + s;
+            double dn, g;
+            const double xn = step_elem<M>(a, cs, cy, P.xc, P.xt, P.dv, P.gk, P.gold, P.Sb, P.Yb, nvs, e, dn, g);
+            xsqh = xn * xn;
+            dgh = dn * g;
+        }
+    }
+    xsqh = wave_sum(xsqh);
+    dgh = wave_sum(dgh);
+    if (s == 0) {
+        double *o = upd_part + ((int64_t)j * (nchunk + 1) + nchunk) * 3;
+        o[0] = 0.0;
+        o[1] = xsqh;
+        o[2] = dgh;
+    }
+}
+
+// Trial point (coefficients and image) and softmax, one thread per sequence b of a
+// command (blocks of 256 sequences):
+//   E_b(s) = h_t(s) + (K_r c_t)(s, b), lz = log sum_s exp E_b(s) (max-shifted),
+//   f_b = w_b (lz - E_b(x_br)), G_b(s) = w_b (P_b(s) - [s == x_br])
+// (plmDCA_objective.hpp:86-151), G into the packed GEMM operand (column row + s, b
+// contiguous) and the trial gradient's coefficients G + 2 lambda_J c_t into g[gt].
+// Per block: upd_part {c_t . K_r c_t, 0, d . g_k} and sm_part {T(s, a) at 8 s + a, sum f_b}.
+template <int M>
+__global__ __launch_bounds__(256) void k_gstep(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                               double *__restrict__ V, int64_t vstride, int64_t nvs,
+                                               double *__restrict__ Gp, int64_t ldG, double *__restrict__ upd_part,
+                                               double *__restrict__ sm_part, int nchunk)
+{
+    const int j = blockIdx.y;
+    const SlotCmd &c = cmds[j];
+    __shared__ double sW[4][kSm];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double *up = upd_part + ((int64_t)j * (nchunk + 1) + blockIdx.x) * 3;
+    double *sp = sm_part + ((int64_t)j * nchunk + blockIdx.x) * kSm;
+    if (c.mode == MODE_IDLE) {
+        if (tid < 3) up[tid] = 0.0;
+        if (tid < kSm) sp[tid] = 0.0;
+        return;
+    }
+    const GramAln &A = aln[c.aln];
+    const int q = A.q, Bp = A.Bp, B = A.B;
+    const int64_t qB = (int64_t)q * Bp;
+    StepArgs a;
+    double cs[M], cy[M];
+    load_step<M>(c, a, cs, cy);
+    const SlotPtrs P = slot_ptrs(c, V, vstride, nvs, M);
+    double hs[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) hs[s] = s < q ? P.xt[2 * qB + s] : 0.0;  // from k_ghstep
+    const int b = blockIdx.x * 256 + tid;
+    const bool valid = b < Bp;
+    double xtc[8], xti[8];
+    double xsqJ = 0.0, dg = 0.0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        xtc[s] = 0.0;
+        xti[s] = 0.0;
+        if (s < q && valid) {
+            const int64_t ec = (int64_t)s * Bp + b, ei = ec + qB;
+            double dnc, gc, dni, gi;
+            xtc[s] = step_elem<M>(a, cs, cy, P.xc, P.xt, P.dv, P.gk, P.gold, P.Sb, P.Yb, nvs, ec, dnc, gc);
+            xti[s] = step_elem<M>(a, cs, cy, P.xc, P.xt, P.dv, P.gk, P.gold, P.Sb, P.Yb, nvs, ei, dni, gi);
+            xsqJ += xtc[s] * xti[s];
+            dg += dnc * gi;  // d . g_k = c_d . (K_r c_gk)
+        }
+    }
+    double G[8];
+    double fb = 0.0;
+    int y = -1;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) G[s] = 0.0;
+    if (valid && b < B) {
+        y = A.statesT[(int64_t)c.locus * B + b];
+        const double wb = A.w[b];
+        double E[8], mx = -INFINITY;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            E[s] = hs[s] + xti[s];
+            if (s < q) mx = fmax(mx, E[s]);
+        }
+        double ex[8], z = 0.0, Ey = 0.0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            ex[s] = s < q ? exp(E[s] - mx) : 0.0;
+            z += ex[s];
+            if (s == y) Ey = E[s];
+        }
+        const double lz = mx + log(z);
+        fb = wb * (lz - Ey);
+        const double iz = 1.0 / z;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            if (s >= q) continue;
+            G[s] = wb * (ex[s] * iz - (s == y ? 1.0 : 0.0));
+            Gp[(int64_t)(c.row + s) * ldG + b] = G[s];
+            P.gt[(int64_t)s * Bp + b] = G[s] + A.lamJ2 * xtc[s];
+        }
+    } else if (valid) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            if (s < q) {
+                P.gt[(int64_t)s * Bp + b] = 0.0;  // pad sequences (the GEMM reads them: zero)
+                Gp[(int64_t)(c.row + s) * ldG + b] = 0.0;
+            }
+    }
+    // T(s, a) = sum over the block's sequences with x_br = a of G_b(s), and the data term
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (s >= q || u >= q) continue;
+            const double v = wave_sum(y == u ? G[s] : 0.0);
+            if (lane == 0) sW[wave][s * 8 + u] = v;
+        }
+    fb = wave_sum(fb);
+    if (lane == 0) sW[wave][64] = fb;
+    double red[3] = {xsqJ, 0.0, dg};
+    block_sum_store<3>(red, up);  // synchronises the block: sW is complete below
+    if (tid < kSm) {
+        const int s = tid >> 3, u = tid & 7;
+        const bool ok = tid == 64 || (s < q && u < q);
+        sp[tid] = ok ? (sW[0][tid] + sW[1][tid]) + (sW[2][tid] + sW[3][tid]) : 0.0;
+    }
+}
+
+// The trial gradient's image and its products, one thread per sequence b of a command:
+//   (K_r g_t)(s, b) = Z(s, b) - T(s, x_br) + 2 lambda_J (K_r c_t)(s, b), Z = K G (the GEMM),
+// stored into g[gt]; the fields' gradient sum_b G_b(s) + 2 lambda_h h_t(s) (block 0).
+// part[command][chunk][2M + 5] = {g_t.s_p, g_t.y_p, g_t.g_t, g_t.g_k, g_t.d, 0, |g_t,J|^2}
+// in the primal metric (coefficients of one against the image of the other).
+template <int M>
+__global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                               double *__restrict__ V, int64_t vstride, int64_t nvs,
+                                               const double *__restrict__ Zp, int64_t ldG,
+                                               const double *__restrict__ sm_part, double *__restrict__ part,
+                                               int nchunk)
+{
+    constexpr int NV = 2 * M + 5;
+    const int j = blockIdx.y;
+    const SlotCmd &c = cmds[j];
+    __shared__ double sT[64];
+    const int tid = threadIdx.x;
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    double *out = part + ((int64_t)j * nchunk + blockIdx.x) * NV;
+    if (c.mode == MODE_IDLE || c.gt < 0 || c.gt > 1) {
+        for (int k = tid; k < NV; k += 256) out[k] = 0.0;
+        return;
+    }
+    const GramAln &A = aln[c.aln];
+    const int q = A.q, Bp = A.Bp, B = A.B;
+    const int64_t qB = (int64_t)q * Bp;
+    double *base = V + (int64_t)c.slot * vstride;
+    const double *xt = base + kVecXt * nvs;
+    double *gt = base + (kVecG + c.gt) * nvs;
+    const double *gk = base + (kVecG + (1 - c.gt)) * nvs;
+    const double *dv = base + kVecD * nvs;
+    const double *Sb = base + kVecPairs * nvs;
+    const double *Yb = Sb + (int64_t)M * nvs;
+    if (tid < 64) {
+        const int s = tid >> 3, u = tid & 7;
+        double t = 0.0;
+        if (s < q && u < q)
+            for (int ch = 0; ch < nchunk; ++ch) t += sm_part[((int64_t)j * nchunk + ch) * kSm + tid];
+        sT[tid] = t;
+    }
+    __syncthreads();
+    const bool cur = c.mode != MODE_INIT;  // a current gradient and a direction exist
+    const uint32_t live = c.live;
+    const int b = blockIdx.x * 256 + tid;
+    if (b < Bp) {
+        const int y = b < B ? A.statesT[(int64_t)c.locus * B + b] : 0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            if (s >= q) continue;
+            const int64_t ec = (int64_t)s * Bp + b, ei = ec + qB;
+            double kg = 0.0;
+            if (b < B)
+                kg = Zp[(int64_t)(c.row + s) * ldG + b] - (A.own ? sT[s * 8 + y] : 0.0) + A.lamJ2 * xt[ei];
+            gt[ei] = kg;
+            const double gc = gt[ec];
+            acc[2 * M] += gc * kg;
+            acc[2 * M + 4] += gc * kg;
+            if (cur) {
+                acc[2 * M + 1] += gc * gk[ei];
+                acc[2 * M + 2] += gc * dv[ei];
+#pragma unroll
+                for (int p = 0; p < M; ++p) {
+                    if (!((live >> p) & 1u)) continue;
+                    acc[p] += gc * Sb[(int64_t)p * nvs + ei];
+                    acc[M + p] += gc * Yb[(int64_t)p * nvs + ei];
+                }
+            }
+        }
+    }
+    if (blockIdx.x == 0 && tid < q) {
+        // the fields: g_h(s) = sum_a T(s, a) + 2 lambda_h h_t(s)
+        const int s = tid;
+        double gh = 0.0;
+        for (int u = 0; u < q; ++u) gh += sT[s * 8 + u];
+        const int64_t eh = 2 * qB + s;
+        gh += A.lamh2 * xt[eh];
+        gt[eh] = gh;
+        acc[2 * M] += gh * gh;
+        if (cur) {
+            acc[2 * M + 1] += gh * gk[eh];
+            acc[2 * M + 2] += gh * dv[eh];
+#pragma unroll
+            for (int p = 0; p < M; ++p) {
+                if (!((live >> p) & 1u)) continue;
+                acc[p] += gh * Sb[(int64_t)p * nvs + eh];
+                acc[M + p] += gh * Yb[(int64_t)p * nvs + eh];
+            }
+        }
+    }
+    block_sum_store<NV>(acc, out);
+}
+
+// Result rows (ResLayout): the products, {c_t . K_r c_t, |h_t|^2, d . g_k} and the data
+// term.  The entry of max |g| holds |g_t|^2 (the host takes the square roots).
+__global__ void k_greduce(int m, const double *__restrict__ upd_part, const double *__restrict__ dot_part,
+                          const double *__restrict__ sm_part, int nchunk, double *__restrict__ res)
+{
+    const int i = blockIdx.x;
+    const int t = threadIdx.x;
+    const int ND = 2 * m + 5, W = 2 * m + 9;
+    double *out = res + (int64_t)i * W;
+    if (t < ND) {
+        double a = 0.0;
+        for (int ch = 0; ch < nchunk; ++ch) a += dot_part[((int64_t)i * nchunk + ch) * ND + t];
+        out[t] = a;
+    } else if (t < ND + 3) {
+        const int u = t - ND;
+        double a = 0.0;
+        for (int ch = 0; ch <= nchunk; ++ch) a += upd_part[((int64_t)i * (nchunk + 1) + ch) * 3 + u];
+        out[t] = a;
+    } else if (t == W - 1) {
+        double a = 0.0;
+        for (int ch = 0; ch < nchunk; ++ch) a += sm_part[((int64_t)i * nchunk + ch) * kSm + 64];
+        out[t] = a;
+    }
+    __syncthreads();
+    if (t == 0) out[2 * m + 3] = out[2 * m];
+}
+
+// Coefficients of finished loci as the A operand of the coupling contraction J = X_r^T c:
+// item u's q rows from packed row rows[u] (FP32, or three BF16 parts hi + mid + lo), and
+// its fields into hpart[u][0..8) (the contraction's field column).
+__global__ __launch_bounds__(256) void k_gsplit(const double *__restrict__ V, const int64_t *__restrict__ src_off,
+                                                const int32_t *__restrict__ rows, const int32_t *__restrict__ alns,
+                                                const GramAln *__restrict__ aln, int64_t ldm, float *__restrict__ G32,
+                                                uint16_t *__restrict__ Ghi, uint16_t *__restrict__ Glo,
+                                                uint16_t *__restrict__ Gl3, double *__restrict__ hpart)
+{
+    const int u = blockIdx.y;
+    const GramAln &A = aln[alns[u]];
+    const int q = A.q, Bp = A.Bp;
+    const double *src = V + src_off[u];
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b < Bp) {
+        for (int s = 0; s < q; ++s) {
+            const double v = src[(int64_t)s * Bp + b];
+            const int64_t o = (int64_t)(rows[u] + s) * ldm + b;
+            const float f = (float)v;
+            if (G32) {
+                G32[o] = f;
+            } else {
+                const uint16_t h = f2bf(f);
+                const float r1 = f - bf2f(h);
+                const uint16_t l = f2bf(r1);
+                Ghi[o] = h;
+                Glo[o] = l;
+                Gl3[o] = f2bf(r1 - bf2f(l));
+            }
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < 8)
+        hpart[(int64_t)u * 8 + threadIdx.x] = threadIdx.x < q ? src[2 * (int64_t)q * Bp + threadIdx.x] : 0.0;
+}
+
+}  // namespace
+
+namespace gram {
+
+void gram_matrix(hipStream_t st, const uint64_t *planes, int B, int q, int W, int j0, int64_t ldk, double *K)
+{
+    const int nt = (B + GT - 1) / GT;
+    const int64_t tiles = (int64_t)nt * (nt + 1) / 2;
+    hipLaunchKernelGGL(k_gram, dim3((unsigned)tiles), dim3(256), 0, st, planes, B, q, W, j0, ldk, K);
+}
+
+void gemm(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_t ldb, double *C, int64_t ldc, int M,
+          int N, int Kd)
+{
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_dgemm, dim3((N + DT - 1) / DT, M / DT), dim3(256), 0, st, A, lda, Bm, ldb, C, ldc, N, Kd);
+}
+
+#define SDCA_GRAM_DISPATCH(m, CALL) \
+    switch (m) {                    \
+    case 1: CALL(1); break;         \
+    case 3: CALL(3); break;         \
+    case 5: CALL(5); break;         \
+    case 8: CALL(8); break;         \
+    case 10: CALL(10); break;       \
+    case 16: CALL(16); break;       \
+    default: break;                 \
+    }
+
+void step(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
+          int64_t nvs, double *Gp, int64_t ldG, double *upd_part, double *sm_part, int nchunk)
+{
+#define SDCA_GSTEP(MM)                                                                                            \
+    hipLaunchKernelGGL((k_ghstep<MM>), dim3(ncmd), dim3(64), 0, st, cmds, aln, V, vstride, nvs, upd_part, nchunk); \
+    hipLaunchKernelGGL((k_gstep<MM>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Gp, ldG,  \
+                       upd_part, sm_part, nchunk)
+    SDCA_GRAM_DISPATCH(m, SDCA_GSTEP)
+#undef SDCA_GSTEP
+}
+
+void dots(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
+          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, double *dot_part, int nchunk)
+{
+#define SDCA_GDOTS(MM)                                                                                            \
+    hipLaunchKernelGGL((k_gdots<MM>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Zp, ldG, \
+                       sm_part, dot_part, nchunk)
+    SDCA_GRAM_DISPATCH(m, SDCA_GDOTS)
+#undef SDCA_GDOTS
+}
+
+void reduce(hipStream_t st, int m, int ncmd, const double *upd_part, const double *dot_part, const double *sm_part,
+            int nchunk, double *res)
+{
+    const int W = 2 * m + 9;
+    hipLaunchKernelGGL(k_greduce, dim3(ncmd), dim3((W + 63) / 64 * 64), 0, st, m, upd_part, dot_part, sm_part,
+                       nchunk, res);
+}
+
+void split(hipStream_t st, const int64_t *src_off, const int32_t *rows, const int32_t *alns, int n, int nchunk,
+           const GramAln *aln, const double *V, int64_t ldm, float *G32, uint16_t *Ghi, uint16_t *Glo,
+           uint16_t *Gl3, double *hpart)
+{
+    hipLaunchKernelGGL(k_gsplit, dim3(nchunk, n), dim3(256), 0, st, V, src_off, rows, alns, aln, ldm, G32, Ghi, Glo,
+                       Gl3, hpart);
+}
+
+}  // namespace gram
+}  // namespace sdca

@@ -122,6 +122,16 @@ struct AlnDev {
     double lamh2;              // 2 lambda_h
 };
 
+// One alignment of a Gram-formulation solve (sdca_gram.hip).
+struct GramAln {
+    const double *K;          // Gram matrix of identical-locus counts, Bp x Bp (pads 0)
+    const uint8_t *statesT;   // L x B envelope states
+    const double *w;          // Bp sequence weights in FP64 (pads 0)
+    int B, Bp, q;
+    int own;                  // 1: K_r = K minus the locus' own column; 0: K already excludes it (scan)
+    double lamJ2, lamh2;      // 2 lambda_J, 2 lambda_h
+};
+
 // XCD-aware tile order of a GEMM launch.  Workgroups are dispatched to the eight XCDs
 // round-robin (workgroup id % 8), so consecutive ids on one XCD are id, id+8, id+16, ...
 // Tiles are grouped into super-tiles of GM x GN output tiles at one split-K index z; a
@@ -232,6 +242,12 @@ void reduce_results(hipStream_t st, int m, int ncmd, const double *upd_part, con
 // that alignment's L entries at rows + k * ldrow.
 void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *src_ld, const int32_t *loci,
            const int32_t *alns, int nscore, int Lmax, int64_t ldrow, const AlnDev *d_aln, float *rows);
+// Couplings J = X_r^T c of finished Gram-formulation loci (the gradient contraction
+// without its L2 term): item u's q x Kxp device vector at out + u * n_item, its field
+// column from hpart[u][0..8).
+void gemm_materialize(hipStream_t st, int prec, const GemmGroup *d_groups, int ngroups, int grid, int tn,
+                      const void *G32, const void *Ghi, const void *Glo, const void *Gl3, int64_t ldb,
+                      const AlnDev *d_aln, const SlotCmd *cmds, float *out, int64_t n_item, const double *hpart);
 // sdca_pairs.hip
 void build_x8(hipStream_t st, const uint8_t *states, int B, int L, int QP, int Bp, int8_t *X8);
 void marginals(hipStream_t st, const uint8_t *states, int B, int L, int QP, int32_t *cnt);
@@ -256,6 +272,61 @@ void scatter_warm(hipStream_t st, const int32_t *slot_of, const int32_t *item_of
                   const int32_t *partners, const float *J, const float *h, int nwarm, int P, int q, int64_t Kxp, int Kx,
                   const int32_t *state_col, int64_t n_slot, float *X0, void *XJ, bool xj_bf16, int64_t jstride);
 }  // namespace kern
+
+// The Gram formulation (sdca_gram.hip): slot vectors x_c, x_t, g[0], g[1], d, then the m
+// pairs s_p and the m pairs y_p, each nvs doubles apart (coefficients, image, fields).
+namespace gram {
+constexpr int kVecXc = 0, kVecXt = 1, kVecG = 2, kVecD = 4, kVecPairs = 5;
+inline int64_t vec_len(int q, int Bp) { return 2 * (int64_t)q * Bp + 8; }
+// Identical-locus counts over loci [j0, L) from bit-planes (k_bitplanes layout).
+void gram_matrix(hipStream_t st, const uint64_t *planes, int B, int q, int W, int j0, int64_t ldk, double *K);
+// Trial points and softmax of a round's commands: G into Gp (column row + s of ldG),
+// partials upd_part [cmd][nchunk + 1][3] and sm_part [cmd][nchunk][65].
+void step(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
+          int64_t nvs, double *Gp, int64_t ldG, double *upd_part, double *sm_part, int nchunk);
+// C[n][m] = sum_k Bm[n][k] A[m][k], FP64 MFMA: m < M (multiple of 128), k < Kd (multiple
+// of 16), n < N stored; Bm rows up to round_up(N, 128) readable.
+void gemm(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_t ldb, double *C, int64_t ldc, int M,
+          int N, int Kd);
+// Gradient images from Z = K G and the products: dot_part [cmd][nchunk][2m + 5].
+void dots(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
+          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, double *dot_part, int nchunk);
+void reduce(hipStream_t st, int m, int ncmd, const double *upd_part, const double *dot_part, const double *sm_part,
+            int nchunk, double *res);
+// Coefficients of n finished loci (V + src_off[u]) as contraction operands (rows[u]).
+void split(hipStream_t st, const int64_t *src_off, const int32_t *rows, const int32_t *alns, int n, int nchunk,
+           const GramAln *aln, const double *V, int64_t ldm, float *G32, uint16_t *Ghi, uint16_t *Glo,
+           uint16_t *Gl3, double *hpart);
+}  // namespace gram
+
+// Workspace of Gram-formulation solves (sdca_api.cpp), sized for `slots` loci in flight,
+// history m and the largest member alignment (q gq, padded depth gBp, gL loci, gKxp columns).
+struct GramWs {
+    int slots = 0, m = 0, gq = 0, gBp = 0, gL = 0, nchunk = 0, nmem = 0;
+    int64_t nvs = 0, vstride = 0, ldG = 0;
+    double *V = nullptr;                  // slot vectors
+    double *Gp = nullptr, *Zp = nullptr;  // packed GEMM operand G and product K G, ldG per column
+    double *upd_part = nullptr, *sm_part = nullptr, *dot_part = nullptr, *res = nullptr;
+    SlotCmd *d_cmds = nullptr, *h_cmds = nullptr;  // h_cmds, h_res: pinned
+    double *h_res = nullptr;
+    GramAln *d_galn = nullptr;
+    AlnDev *d_aln = nullptr;
+    std::vector<int> cmd_slot;  // slot of each listed command of the round in flight
+    // finished loci, up to nmat per batch: contraction operands (mrows packed rows of gBp),
+    // their couplings (n_item floats each: q x Kxp), score rows (gL each)
+    int nmat = 0, mrows = 0;
+    int64_t n_item = 0;
+    float *M32 = nullptr, *Mout = nullptr;
+    uint16_t *Mhi = nullptr, *Mlo = nullptr, *Ml3 = nullptr;
+    double *mhpart = nullptr;
+    SlotCmd *d_mcmds = nullptr;
+    GemmGroup *d_mgroups = nullptr;
+    int64_t *d_moff = nullptr, *d_soff = nullptr;
+    int32_t *d_mrow = nullptr, *d_maln = nullptr, *d_sloc = nullptr, *d_sld = nullptr;
+    float *d_rows = nullptr, *h_rows = nullptr;  // h_rows: pinned
+    hipEvent_t ev[8] = {};
+    ~GramWs();
+};
 
 // Warm starts of a solve call in compact device form: for item k (the k-th listed locus)
 // P partner loci, the network's q x q blocks for them and the fields h.
@@ -380,6 +451,13 @@ struct sdca_ctx {
     int topk_K = 0;
     std::vector<double> h_w;  // installed weights (host copy)
     sdca::Mlp *mlp = nullptr;
+
+    // Gram formulation: this alignment's identical-locus counts (computed for the scan
+    // split gram_split) and FP64 weights, and the workspace of the solves it runs
+    double *d_gram = nullptr;
+    int gram_split = -1;
+    double *d_w64 = nullptr;
+    sdca::GramWs *gram = nullptr;
 
     hipEvent_t ev[8] = {};
     hipStream_t gstream[sdca::kMaxGroups] = {};  // gstream[0] is stream

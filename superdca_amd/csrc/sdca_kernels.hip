@@ -406,6 +406,8 @@ struct EpiEnergy {
 // to about B_eff while the result is small.  The commands under the tile's 128 rows are
 // staged in LDS once; each group of four rows first loads its trial-point values (packed
 // rows, like the beliefs), then stores.
+// Without Xtrial (null) the epilogue stores the contraction itself: the couplings
+// J = X_r^T c of coefficients c (the Gram formulation's solutions), with column K_x from hpart.
 struct EpiGrad {
     float *gring;
     const float *Xtrial;
@@ -469,11 +471,11 @@ struct EpiGrad {
                     srow[u] = ok ? m - row_of(G, i) : 0;
                     const int slot = ok ? s_slot[il] : 0;
                     roff[u] = ((int64_t)slot * ng + (gt[u] < 0 ? 0 : gt[u] - g0)) * n_slot + (int64_t)srow[u] * Kxp;
-                    const float *xr = Xtrial + (int64_t)m * ldx;
+                    const float *xr = Xtrial ? Xtrial + (int64_t)m * ldx : nullptr;
 #pragma unroll
                     for (int ni = 0; ni < NI; ++ni) {
                         const int64_t n = nw + ni * 32 + (lane & 31);
-                        const bool use = gt[u] >= 0 && n <= Kx && !(n >= lo[u] && n < hi[u]);
+                        const bool use = Xtrial && gt[u] >= 0 && n <= Kx && !(n >= lo[u] && n < hi[u]);
                         xv[u][ni] = use ? __builtin_nontemporal_load(xr + n) : 0.f;
                     }
                 }
@@ -1099,6 +1101,17 @@ void gemm_grad(hipStream_t st, int prec, const GemmGroup *d_groups, int ngroups,
                const float *Xtrial, int64_t ldx, float *gbuf, int64_t n_slot, const double *hpart, int nbchunk)
 {
     EpiGrad epi{gbuf, Xtrial, cmds, hpart, d_aln, n_slot, ldx, 2, 0, nbchunk};
+    if (prec == SDCA_PREC_FP32)
+        launch_gemm<PolF32, 1>(st, tn, grid, G32, G32, G32, ldb, d_groups, ngroups, epi);
+    else
+        launch_gemm<PolBF16, 2>(st, tn, grid, Ghi, Glo, Gl3, ldb, d_groups, ngroups, epi);
+}
+
+void gemm_materialize(hipStream_t st, int prec, const GemmGroup *d_groups, int ngroups, int grid, int tn,
+                      const void *G32, const void *Ghi, const void *Glo, const void *Gl3, int64_t ldb,
+                      const AlnDev *d_aln, const SlotCmd *cmds, float *out, int64_t n_item, const double *hpart)
+{
+    EpiGrad epi{out, nullptr, cmds, hpart, d_aln, n_item, 0, 1, 0, 1};
     if (prec == SDCA_PREC_FP32)
         launch_gemm<PolF32, 1>(st, tn, grid, G32, G32, G32, ldb, d_groups, ngroups, epi);
     else

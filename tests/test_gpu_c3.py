@@ -51,13 +51,15 @@ def test_c3_reweight_bit_exact(c3_cases):
 # SuperDCA's rule plus the coupling threshold ||g_J||_inf < 3e-6; bars: f 3e-6 relative
 # (the FP32-summed energies of the B = 103, L = 16,890 alignment carry about 1e-6), every
 # locus' top-decile scores 1e-4 relative
-@pytest.mark.parametrize("kw", [pytest.param(dict(precision=sd.PREC_FP32), id="fp32"),
-                                pytest.param(dict(precision=sd.PREC_SPLIT_BF16, fine_factor=0), id="split"),
-                                pytest.param(dict(precision=sd.PREC_SPLIT_BF16, fine_factor=10), id="split+fine")])
+@pytest.mark.parametrize("kw", [
+    pytest.param(dict(formulation=sd.FORM_PRIMAL, precision=sd.PREC_FP32), id="fp32"),
+    pytest.param(dict(formulation=sd.FORM_PRIMAL, precision=sd.PREC_SPLIT_BF16, fine_factor=0), id="split"),
+    pytest.param(dict(formulation=sd.FORM_PRIMAL, precision=sd.PREC_SPLIT_BF16, fine_factor=10), id="split+fine"),
+    pytest.param(dict(formulation=sd.FORM_GRAM, j_threshold=1e-5), id="gram")])
 def test_c3_loci_against_oracle_optimum(c3_cases, kw):
     for c in c3_cases:
         st, loci, ref = c["states"], c["loci"], c["rows"]
-        with sd.Context(st, q=c["q"], j_threshold=3e-6, batch_loci=NLOCI, **kw) as ctx:
+        with sd.Context(st, q=c["q"], batch_loci=NLOCI, **{"j_threshold": 3e-6, **kw}) as ctx:
             ctx.set_weights(c["w"])
             rows, stats, _, rc = ctx.solve(loci)
         assert rc in (sd.SDCA_OK, sd.SDCA_E_NOCONV), rc
@@ -99,8 +101,10 @@ def gapless_two_state(B=300, L=700, clades=12, seed=77):
 
 
 @pytest.mark.skipif(not os.path.exists(FIXTURE), reason="tests/golden/c3_fixture.npz not generated")
-@pytest.mark.parametrize("kw", [pytest.param(dict(precision=sd.PREC_SPLIT_BF16, fine_factor=0), id="split"),
-                                pytest.param(dict(precision=sd.PREC_FP32), id="fp32")])
+@pytest.mark.parametrize("kw", [
+    pytest.param(dict(formulation=sd.FORM_PRIMAL, precision=sd.PREC_SPLIT_BF16, fine_factor=0), id="split"),
+    pytest.param(dict(formulation=sd.FORM_PRIMAL, precision=sd.PREC_FP32), id="fp32"),
+    pytest.param(dict(formulation=sd.FORM_GRAM, j_threshold=1e-5), id="gram")])
 def test_c3_all_alignments_in_one_solve(kw):
     # the 48 alignments of the uneven batch (envelopes 3, 4 and 5) and a gapless two-state
     # alignment, their loci interleaved in one multi-alignment solve: every round's GEMM
@@ -130,7 +134,7 @@ def test_c3_all_alignments_in_one_solve(kw):
     alns = np.array([items[i][0] for i in order], dtype=np.int32)
     loci = np.array([items[i][1] for i in order], dtype=np.int32)
     print(f"{len(items)} loci of {len(ctxs)} alignments, q_env {sorted(set(c.q for c in ctxs))}", flush=True)
-    with sd.Multi(ctxs, j_threshold=1e-6, batch_loci=256, **kw) as mu:
+    with sd.Multi(ctxs, batch_loci=256, **{"j_threshold": 1e-6, **kw}) as mu:
         rows, stats, rc = mu.solve(alns, loci)
         tm = mu.timing()
     assert rc in (sd.SDCA_OK, sd.SDCA_E_NOCONV)

@@ -142,15 +142,17 @@ def test_gradient_at_the_optimum_is_below_the_coupling_threshold(c1, c1_optimum)
 # accurate solves: SuperDCA's rule (||g||_inf < 1e-3) plus the coupling threshold
 # ||g_J||_inf < 3e-6, in FP32, at split precision alone and with the fine level; a locus may
 # end on a line-search stall below the threshold's neighbourhood
-ACCURATE = [pytest.param(dict(precision=sd.PREC_FP32), id="fp32"),
-            pytest.param(dict(precision=sd.PREC_SPLIT_BF16, fine_factor=0), id="split"),
-            pytest.param(dict(precision=sd.PREC_SPLIT_BF16, fine_factor=10), id="split+fine")]
+ACCURATE = [pytest.param(dict(formulation=sd.FORM_PRIMAL, precision=sd.PREC_FP32), id="fp32"),
+            pytest.param(dict(formulation=sd.FORM_PRIMAL, precision=sd.PREC_SPLIT_BF16, fine_factor=0), id="split"),
+            pytest.param(dict(formulation=sd.FORM_PRIMAL, precision=sd.PREC_SPLIT_BF16, fine_factor=10),
+                         id="split+fine"),
+            pytest.param(dict(formulation=sd.FORM_GRAM, j_threshold=1e-5), id="gram")]
 
 
 @pytest.mark.parametrize("kw", ACCURATE)
 def test_solve_accurate_matches_oracle_optimum(c1, c1_optimum, kw):
     st = c1["states"]
-    with sd.Context(st, j_threshold=3e-6, batch_loci=64, **kw) as ctx:
+    with sd.Context(st, batch_loci=64, **{"j_threshold": 3e-6, **kw}) as ctx:
         ctx.set_weights(c1_optimum["w"])
         rows, stats, X, rc = ctx.solve(SOLVE_LOCI, want_x=True)
     assert rc in (sd.SDCA_OK, sd.SDCA_E_NOCONV)
@@ -160,9 +162,15 @@ def test_solve_accurate_matches_oracle_optimum(c1, c1_optimum, kw):
     for k, r in enumerate(SOLVE_LOCI):
         assert rows[k, r] == 0.0
     abs_err, rel_top = score_errors(rows, c1_optimum["rows"])
+    xerr = np.abs(X - c1_optimum["x"]).max() / np.abs(c1_optimum["x"]).max()
     print(f"{kw}: {int((stats['status'] != 0).sum())} line-search stops, scores max abs err {abs_err:.3e}, "
-          f"max rel err (top decile) {rel_top:.3e}")
+          f"max rel err (top decile) {rel_top:.3e}, parameters max err {xerr:.3e} of max |x|")
     assert rel_top < 1e-4
+    if kw.get("formulation") == sd.FORM_GRAM:
+        # FP64 evaluations: f at the optimum to its rounding, parameters (formed once, with
+        # FP32 products) to FP32 accuracy
+        np.testing.assert_allclose(stats["fval"], c1_optimum["stats"]["fval"], rtol=1e-10)
+        assert xerr < 1e-5
 
 
 def test_reference_mode_threshold_and_refill(c1, c1_optimum):
@@ -182,7 +190,7 @@ def test_warm_start_reaches_cold_optimum(c1, c1_optimum):
     rng = np.random.default_rng(5)
     # a start near the optimum (10 % multiplicative perturbation of every parameter)
     warm = c1_optimum["x"] * (1.0 + 0.1 * rng.normal(size=c1_optimum["x"].shape))
-    with sd.Context(st, grad_threshold=1e-5, batch_loci=64) as ctx:
+    with sd.Context(st, grad_threshold=1e-5, batch_loci=64, formulation=sd.FORM_PRIMAL) as ctx:
         ctx.set_weights(c1_optimum["w"])
         _, sw, _, rc = ctx.solve(SOLVE_LOCI, warm=warm)
         _, sc, _, rc2 = ctx.solve(SOLVE_LOCI)
@@ -267,7 +275,8 @@ def test_pipelined_slot_groups_match_single_group(c1, precision):
     loci = np.arange(0, 1000, 12)[:80]
     out = {}
     for groups in (1, 2):
-        with sd.Context(st, precision=precision, grad_threshold=1e-4, batch_loci=64, pipeline_groups=groups) as ctx:
+        with sd.Context(st, precision=precision, grad_threshold=1e-4, batch_loci=64, pipeline_groups=groups,
+                        formulation=sd.FORM_PRIMAL) as ctx:
             ctx.set_weights(w)
             rows, stats, _, rc = ctx.solve(loci)
         assert rc == sd.SDCA_OK, groups
@@ -290,7 +299,7 @@ def test_coarse_phase_reaches_the_full_precision_optimum(c1):
     out = {}
     for factor in (0, 4, 1000000):
         with sd.Context(st, precision=sd.PREC_SPLIT_BF16, grad_threshold=1e-4, batch_loci=64,
-                        coarse_factor=factor) as ctx:
+                        coarse_factor=factor, formulation=sd.FORM_PRIMAL) as ctx:
             ctx.set_weights(w)
             rows, stats, _, rc = ctx.solve(loci)
         assert rc == sd.SDCA_OK, factor
@@ -315,7 +324,7 @@ def test_eval_and_solve_q5_envelope(c1, precision, ftol, gtol):
     n = orc.nparams(st5.shape[1], 5)
     loci = [0, 17, 500, 999]
     x = np.random.default_rng(41).normal(size=(len(loci), n)) * 0.05
-    with sd.Context(st5, q=5, precision=precision, batch_loci=4) as ctx:
+    with sd.Context(st5, q=5, precision=precision, batch_loci=4, formulation=sd.FORM_PRIMAL) as ctx:
         beff, lh, lJ = ctx.set_weights(w)
         f, g = ctx.eval_batch(loci, x)
         rows, stats, _, rc = ctx.solve(loci)

@@ -5,9 +5,9 @@ highest-scoring pairs of C2 (oracle solution committed as tests/golden/c2_top10k
 tests/golden/make_c2_scores.py; about 20 CPU minutes, too slow for a test run).
 
 The GPU solves stop at SuperDCA's rule (||g||_inf < 1e-3) plus the coupling threshold
-||g_J||_inf < 3e-6, in FP32, at split BF16 alone and with the fine level; the optimum is
-unique (strict convexity), so both sides approximate the same scores.  Bar: 1e-4 relative
-on every top pair.
+||g_J||_inf < 3e-6, in FP32, at split BF16 alone and with the fine level, or in the Gram
+formulation (FP64) at ||g||_2 < 1e-3 and ||g_J||_2 < 1e-5; the optimum is unique (strict
+convexity), so both sides approximate the same scores.  Bar: 1e-4 relative on every top pair.
 """
 from __future__ import annotations
 
@@ -38,9 +38,18 @@ def top_set(S: np.ndarray, k: int) -> set:
 
 
 JTHR = 3e-6
-ACCURATE = [pytest.param(dict(precision=sd.PREC_FP32), id="fp32"),
-            pytest.param(dict(precision=sd.PREC_SPLIT_BF16, fine_factor=0), id="split"),
-            pytest.param(dict(precision=sd.PREC_SPLIT_BF16, fine_factor=10), id="split+fine")]
+# the Gram formulation tests 2-norms (an upper bound on the infinity norms): its coupling
+# threshold is on ||g_J||_2
+JTHR_GRAM = 1e-5
+P = sd.FORM_PRIMAL
+ACCURATE = [pytest.param(dict(formulation=P, precision=sd.PREC_FP32), id="fp32"),
+            pytest.param(dict(formulation=P, precision=sd.PREC_SPLIT_BF16, fine_factor=0), id="split"),
+            pytest.param(dict(formulation=P, precision=sd.PREC_SPLIT_BF16, fine_factor=10), id="split+fine"),
+            pytest.param(dict(formulation=sd.FORM_GRAM, j_threshold=JTHR_GRAM), id="gram")]
+
+
+def with_jthr(kw):
+    return {"j_threshold": JTHR, **kw}
 
 
 def check_stats(stats):
@@ -55,7 +64,7 @@ def check_stats(stats):
 
 def gpu_scores(states, w, kw):
     L = states.shape[1]
-    with sd.Context(states, j_threshold=JTHR, batch_loci=256, **kw) as ctx:
+    with sd.Context(states, batch_loci=256, **with_jthr(kw)) as ctx:
         ctx.set_weights(w)
         rows, stats, _, rc = ctx.solve(np.arange(L, dtype=np.int32))
     assert rc in (sd.SDCA_OK, sd.SDCA_E_NOCONV), rc
@@ -157,7 +166,7 @@ def test_c4_rows_against_oracle_fixture(c4_states, kw):
         w, _ = ctx.reweight(0.9)
     assert abs(w.sum() - float(fx["w_sum"])) <= 1e-9 * float(fx["w_sum"])
     loci = fx["loci"]
-    with sd.Context(st, q=5, j_threshold=JTHR, batch_loci=loci.size, **kw) as ctx:
+    with sd.Context(st, q=5, batch_loci=loci.size, **with_jthr(kw)) as ctx:
         ctx.set_weights(w)
         rows, stats, _, rc = ctx.solve(loci)
     assert rc in (sd.SDCA_OK, sd.SDCA_E_NOCONV), rc
