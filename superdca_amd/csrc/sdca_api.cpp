@@ -211,7 +211,11 @@ sdca::GramWs::~GramWs()
     if (h_cmds) (void)hipHostFree(h_cmds);
     if (h_res) (void)hipHostFree(h_res);
     if (h_rows) (void)hipHostFree(h_rows);
-    for (auto &e : ev)
+    dfree(Pend);
+    for (auto &g : ev)
+        for (auto &e : g)
+            if (e) (void)hipEventDestroy(e);
+    for (auto &e : stage_ev)
         if (e) (void)hipEventDestroy(e);
 }
 
@@ -1461,9 +1465,12 @@ int ensure_gram_ws(sdca_ctx *c, int nslots, int m)
         nitem = std::max(nitem, (int64_t)A->q * A->Kxp);
     }
     const int nmem = (int)c->members.size();
+    // two slot groups (their rounds overlap: one group's GEMM with the other's vector passes
+    // and host steps) unless pipeline_groups is 1 or the batch is small
+    const int ng = (c->opts.pipeline_groups == 1 || nslots < 64) ? 1 : 2;
     sdca::GramWs *G = c->gram;
     if (G && G->slots == nslots && G->m == m && G->gq == gq && G->gBp == gBp && G->gL == gL && G->nmem == nmem &&
-        G->n_item == nitem)
+        G->n_item == nitem && G->ngroups == ng)
         return SDCA_OK;
     delete c->gram;
     c->gram = G = new sdca::GramWs();
@@ -1478,24 +1485,31 @@ int ensure_gram_ws(sdca_ctx *c, int nslots, int m)
     G->vstride = (sdca::gram::kVecPairs + 2 * m) * G->nvs;
     G->ldG = gBp;
     G->nchunk = gBp / 256;
-    G->nmat = std::min(nslots, 64);
+    G->nchq = gq * gBp / 256;
+    G->nmat = std::min(nslots, 128);
+    G->ngroups = ng;
+    G->gs = (nslots + ng - 1) / ng;
+    G->colcap = round_up((int64_t)G->gs * gq, 128) + 128;  // the GEMM reads whole 128-column tiles
     // each alignment's rows of a batch of finished loci start on a tile boundary
     G->mrows = (int)round_up((int64_t)G->nmat * gq, sdca::kTile) + std::min(nmem, G->nmat) * sdca::kTile;
     const int W = sdca::ResLayout{m}.width();
-    const int64_t ncols = round_up((int64_t)nslots * gq, 128) + 128;  // the GEMM reads whole 128-column tiles
+    const int64_t ncols = G->colcap * ng;
+    const int npad = G->gs * ng;  // per-command buffers, a group's from its first slot on
     int rc;
     if ((rc = dalloc(&G->V, (size_t)nslots * G->vstride)) || (rc = dalloc(&G->Gp, (size_t)ncols * G->ldG)) ||
         (rc = dalloc(&G->Zp, (size_t)ncols * G->ldG)) ||
-        (rc = dalloc(&G->upd_part, (size_t)nslots * (G->nchunk + 1) * 3)) ||
-        (rc = dalloc(&G->sm_part, (size_t)nslots * G->nchunk * 65)) ||
-        (rc = dalloc(&G->dot_part, (size_t)nslots * G->nchunk * sdca::ResLayout{m}.ndots())) ||
-        (rc = dalloc(&G->res, (size_t)nslots * W)) || (rc = dalloc(&G->d_cmds, (size_t)nslots)) ||
+        (rc = dalloc(&G->upd_part, (size_t)npad * (G->nchq + 1) * 3)) ||
+        (rc = dalloc(&G->sm_part, (size_t)npad * G->nchunk * 65)) ||
+        (rc = dalloc(&G->dot_part, (size_t)npad * G->nchq * sdca::ResLayout{m}.ndots())) ||
+        (rc = dalloc(&G->res, (size_t)npad * W)) || (rc = dalloc(&G->d_cmds, (size_t)npad)) ||
         (rc = dalloc(&G->d_galn, (size_t)nmem)) || (rc = dalloc(&G->d_aln, (size_t)nmem)))
         return rc;
     // the coupling contraction of finished loci: three BF16 parts (hi + mid + lo, the FP32
     // product) at either precision setting
     const size_t mel = (size_t)G->mrows * gBp;
     if ((rc = dalloc(&G->Mhi, mel)) || (rc = dalloc(&G->Mlo, mel)) || (rc = dalloc(&G->Ml3, mel))) return rc;
+    G->pcap = G->nmat + G->gs;
+    if ((rc = dalloc(&G->Pend, (size_t)G->pcap * G->nvs))) return rc;
     if ((rc = dalloc(&G->Mout, (size_t)G->nmat * nitem)) || (rc = dalloc(&G->mhpart, (size_t)G->nmat * 8)) ||
         (rc = dalloc(&G->d_mcmds, (size_t)G->nmat)) || (rc = dalloc(&G->d_mgroups, (size_t)nmem)) ||
         (rc = dalloc(&G->d_moff, (size_t)G->nmat)) || (rc = dalloc(&G->d_soff, (size_t)G->nmat)) ||
@@ -1503,10 +1517,14 @@ int ensure_gram_ws(sdca_ctx *c, int nslots, int m)
         (rc = dalloc(&G->d_sloc, (size_t)G->nmat)) || (rc = dalloc(&G->d_sld, (size_t)G->nmat)) ||
         (rc = dalloc(&G->d_rows, (size_t)G->nmat * gL)))
         return rc;
-    HIP_TRY(hipHostMalloc((void **)&G->h_cmds, (size_t)nslots * sizeof(sdca::SlotCmd), hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void **)&G->h_res, (size_t)nslots * W * sizeof(double), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&G->h_cmds, (size_t)npad * sizeof(sdca::SlotCmd), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&G->h_res, (size_t)npad * W * sizeof(double), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&G->h_rows, (size_t)G->nmat * gL * sizeof(float), hipHostMallocDefault));
-    for (auto &e : G->ev) HIP_TRY(hipEventCreate(&e));
+    for (auto &g : G->ev)
+        for (auto &e : g) HIP_TRY(hipEventCreate(&e));
+    for (auto &e : G->stage_ev) HIP_TRY(hipEventCreate(&e));
+    G->pend.clear();
+    G->pend.reserve(G->pcap);
     HIP_TRY(hipDeviceSynchronize());
     return SDCA_OK;
 }
@@ -1538,78 +1556,87 @@ int upload_gram_alns(sdca_ctx *c)
     return upload_alns(c, G->d_aln);
 }
 
-// One round: the commands of the active slots (by alignment, in slot order), each owning q
-// consecutive columns of the packed GEMM operand.
-int gram_launch_round(sdca_ctx *c, const std::vector<sdca::SlotCmd> &cmds)
+// One round of slot group g (slots [g gs, (g + 1) gs)) on its stream: the commands of its
+// active slots (by alignment, in slot order), each owning q consecutive columns of the
+// group's share of the packed GEMM operand.
+int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds)
 {
     sdca::GramWs *G = c->gram;
     const int m = G->m, W = sdca::ResLayout{m}.width();
     const int nmem = (int)c->members.size();
-    std::vector<int> &lst = G->cmd_slot;
+    const int s0 = g * G->gs, s1 = std::min<int>((int)cmds.size(), s0 + G->gs);
+    std::vector<int> &lst = G->cmd_slot[g];
     lst.clear();
+    sdca::SlotCmd *hc = G->h_cmds + s0;
     std::vector<int> col0(nmem, 0), ncol(nmem, 0);
     int col = 0;
     for (int a = 0; a < nmem; ++a) {
         const sdca_ctx *A = c->members[a];
         col0[a] = col;
-        for (size_t i = 0; i < cmds.size(); ++i) {
+        for (int i = s0; i < s1; ++i) {
             if (cmds[i].mode == sdca::MODE_IDLE || cmds[i].aln != a) continue;
-            sdca::SlotCmd &k = G->h_cmds[lst.size()];
+            sdca::SlotCmd &k = hc[lst.size()];
             k = cmds[i];
-            k.slot = (int)i;
+            k.slot = i;
             k.row = col;
             k.q = A->q;
             k.Kx = (int)A->Kx;
             k.Kxp = (int)A->Kxp;
-            lst.push_back((int)i);
+            lst.push_back(i);
             col += A->q;
         }
         ncol[a] = col - col0[a];
     }
     const int nc = (int)lst.size();
     if (nc == 0) return SDCA_OK;
-    hipStream_t st = c->stream;
-    HIP_TRY(hipMemcpyAsync(G->d_cmds, G->h_cmds, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipEventRecord(G->ev[0], st));
-    sdca::gram::step(st, m, G->d_cmds, nc, G->d_galn, G->V, G->vstride, G->nvs, G->Gp, G->ldG, G->upd_part,
-                     G->sm_part, G->nchunk);
+    hipStream_t st = c->gstream[g];
+    hipEvent_t *ev = G->ev[g];
+    sdca::SlotCmd *dc = G->d_cmds + s0;
+    double *Gp = G->Gp + (size_t)g * G->colcap * G->ldG, *Zp = G->Zp + (size_t)g * G->colcap * G->ldG;
+    double *upd = G->upd_part + (size_t)s0 * (G->nchq + 1) * 3, *sm = G->sm_part + (size_t)s0 * G->nchunk * 65;
+    double *dot = G->dot_part + (size_t)s0 * G->nchq * sdca::ResLayout{m}.ndots(), *res = G->res + (size_t)s0 * W;
+    HIP_TRY(hipMemcpyAsync(dc, hc, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(ev[0], st));
+    sdca::gram::step(st, m, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, Gp, G->ldG, upd, sm, G->nchunk, G->nchq);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(G->ev[1], st));
+    HIP_TRY(hipEventRecord(ev[1], st));
     for (int a = 0; a < nmem; ++a) {
         if (!ncol[a]) continue;
         const sdca_ctx *A = c->members[a];
-        sdca::gram::gemm(st, A->d_gram, A->Bp, G->Gp + (size_t)col0[a] * G->ldG, G->ldG,
-                         G->Zp + (size_t)col0[a] * G->ldG, G->ldG, A->Bp, ncol[a], A->Bp);
+        sdca::gram::gemm(st, A->d_gram, A->Bp, Gp + (size_t)col0[a] * G->ldG, G->ldG, Zp + (size_t)col0[a] * G->ldG,
+                         G->ldG, A->Bp, ncol[a], A->Bp);
     }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(G->ev[2], st));
-    sdca::gram::dots(st, m, G->d_cmds, nc, G->d_galn, G->V, G->vstride, G->nvs, G->Zp, G->ldG, G->sm_part,
-                     G->dot_part, G->nchunk);
-    sdca::gram::reduce(st, m, nc, G->upd_part, G->dot_part, G->sm_part, G->nchunk, G->res);
+    HIP_TRY(hipEventRecord(ev[2], st));
+    sdca::gram::dots(st, m, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, Zp, G->ldG, sm, G->nchunk, dot, G->nchq);
+    sdca::gram::reduce(st, m, nc, upd, dot, sm, G->nchunk, G->nchq, res);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(G->ev[3], st));
-    HIP_TRY(hipMemcpyAsync(G->h_res, G->res, (size_t)nc * W * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(ev[3], st));
+    HIP_TRY(hipMemcpyAsync(G->h_res + (size_t)s0 * W, res, (size_t)nc * W * sizeof(double), hipMemcpyDeviceToHost,
+                           st));
     return SDCA_OK;
 }
 
-// Waits for the round, moves its result rows to their slots (the norms: square roots of
-// the squared 2-norms the device summed) and accounts its time, flops and bytes.
-int gram_finish_round(sdca_ctx *c, const std::vector<sdca::SlotCmd> &cmds, std::vector<double> &res)
+// Waits for group g's round, moves its result rows to their slots (the norms: square roots
+// of the squared 2-norms the device summed) and accounts its time, flops and bytes.
+int gram_finish_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds, std::vector<double> &res)
 {
     sdca::GramWs *G = c->gram;
-    const std::vector<int> &lst = G->cmd_slot;
+    const std::vector<int> &lst = G->cmd_slot[g];
     if (lst.empty()) return SDCA_OK;
-    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipStreamSynchronize(c->gstream[g]));
     const sdca::ResLayout R{G->m};
     const int W = R.width();
+    const double *hr = G->h_res + (size_t)g * G->gs * W;
     for (size_t k = 0; k < lst.size(); ++k) {
         double *o = res.data() + (size_t)lst[k] * W;
-        std::memcpy(o, G->h_res + k * W, W * sizeof(double));
+        std::memcpy(o, hr + k * W, W * sizeof(double));
         o[R.ginf()] = std::sqrt(std::max(0.0, o[R.ginf()]));
         o[R.gJinf()] = std::sqrt(std::max(0.0, o[R.gJinf()]));
     }
+    hipEvent_t *ev = G->ev[g];
     auto &t = c->timing;
-    const double upd = elapsed(G->ev[0], G->ev[1]), gm = elapsed(G->ev[1], G->ev[2]), dts = elapsed(G->ev[2], G->ev[3]);
+    const double upd = elapsed(ev[0], ev[1]), gm = elapsed(ev[1], ev[2]), dts = elapsed(ev[2], ev[3]);
     t.update += upd;
     t.grad += gm;
     t.dots += dts;
@@ -1635,7 +1662,8 @@ int gram_finish_round(sdca_ctx *c, const std::vector<sdca::SlotCmd> &cmds, std::
                 acc += (k.cs[p] != 0.f) + (k.cy[p] != 0.f);
             }
         }
-        bu += 8.0 * half * (2.0 * acc + 2.0);  // + the beliefs and the gradient's coefficients
+        // + the softmax: x_t read, the beliefs and the gradient's coefficients written
+        bu += 8.0 * half * (2.0 * acc + 4.0);
         // products pass: Z, the trial image and gradient coefficients read, its image written,
         // and the images of g_k, d and the live pairs
         bd += 8.0 * half * (4.0 + (k.mode == sdca::MODE_INIT ? 0.0 : 2.0 + 2.0 * npairs));
@@ -1647,53 +1675,91 @@ int gram_finish_round(sdca_ctx *c, const std::vector<sdca::SlotCmd> &cmds, std::
     return SDCA_OK;
 }
 
-// Couplings, score rows, x_out and statistics of the finished slots fin (their solutions
-// from x_t or x_c), nmat at a time.
-int gram_finish_loci(sdca_ctx *c, const Solver &S, const std::vector<int> &fin, const int32_t *alns,
-                     float *score_rows, const int64_t *row_off, double *x_out, sdca_locus_stats *st_out)
+// Finished slots fin of group g: their statistics, and their solutions (x_t or x_c:
+// coefficients and fields) copied to the pending buffer on the group's stream, so that the
+// slots can take new loci at once.
+int gram_stage(sdca_ctx *c, int g, const Solver &S, const std::vector<int> &fin, sdca_locus_stats *st_out)
 {
     sdca::GramWs *G = c->gram;
-    hipStream_t st = c->stream;
+    hipStream_t st = c->gstream[g];
+    for (int i : fin) {
+        const Slot &sl = S.slot(i);
+        const sdca_ctx *A = c->members[sl.aln];
+        const int64_t qB = (int64_t)A->q * A->Bp;
+        const size_t u = G->pend.size();
+        if ((int)u >= G->pcap) return fail(SDCA_E_STATE, "pending buffer full");
+        const double *src = G->V + (size_t)i * G->vstride + (sl.from_xt ? sdca::gram::kVecXt : sdca::gram::kVecXc) * G->nvs;
+        double *dst = G->Pend + u * G->nvs;
+        HIP_TRY(hipMemcpyAsync(dst, src, (size_t)qB * sizeof(double), hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(dst + 2 * qB, src + 2 * qB, 8 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        G->pend.push_back({sl.item, sl.r, sl.aln});
+        if (st_out) {
+            sdca_locus_stats &o = st_out[sl.item];
+            o.fval = sl.f;
+            o.gnorm = sl.gnorm;
+            o.nfeval = sl.nfeval;
+            o.iter = sl.iter;
+            o.status = sl.status;
+            o.nfeval_first = sl.nfeval_first;
+            o.gnorm_J = sl.gnormJ;
+            o.nfeval_fine = 0;
+            o.reserved = 0;
+        }
+    }
+    HIP_TRY(hipEventRecord(G->stage_ev[g], st));
+    return SDCA_OK;
+}
+
+// Couplings, score rows and x_out of the pending loci, nmat per batch on group g's stream
+// (after the other groups' staging copies); all of them once nmat have gathered, or when
+// force is set.
+int gram_flush(sdca_ctx *c, int g, bool force, float *score_rows, const int64_t *row_off, double *x_out)
+{
+    sdca::GramWs *G = c->gram;
+    if (G->pend.empty() || (!force && (int)G->pend.size() < G->nmat)) return SDCA_OK;
+    hipStream_t st = c->gstream[g];
+    hipEvent_t *ev = G->ev[g];
+    for (int h = 0; h < G->ngroups; ++h)
+        if (h != g) HIP_TRY(hipStreamWaitEvent(st, G->stage_ev[h], 0));
     const int nmem = (int)c->members.size();
     const int gm = c->opts.gemm_gm_grad > 0 ? c->opts.gemm_gm_grad : 2;
     const int tn = c->opts.gemm_tn_grad > 0 ? c->opts.gemm_tn_grad : 256;
     const size_t n = (size_t)c->L * c->q * c->q + c->q;
     std::vector<float> xdev(x_out ? G->n_item : 0);
-    for (size_t u0 = 0; u0 < fin.size(); u0 += G->nmat) {
-        const int nb = (int)std::min<size_t>(G->nmat, fin.size() - u0);
-        // items by alignment
+    const std::vector<sdca::GramWs::Pending> &P = G->pend;
+    for (size_t u0 = 0; u0 < P.size(); u0 += G->nmat) {
+        const int nb = (int)std::min<size_t>(G->nmat, P.size() - u0);
+        // pending entries of the batch by alignment
         std::vector<int> order(nb);
-        for (int u = 0; u < nb; ++u) order[u] = fin[u0 + u];
-        std::stable_sort(order.begin(), order.end(),
-                         [&](int x, int y) { return S.slot(x).aln < S.slot(y).aln; });
+        for (int u = 0; u < nb; ++u) order[u] = (int)u0 + u;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return P[x].aln < P[y].aln; });
         std::vector<int64_t> moff(nb), soff(nb);
         std::vector<int32_t> mrow(nb), maln(nb), sloc(nb), sld(nb);
         std::vector<sdca::SlotCmd> mc(nb);
         std::vector<sdca::GemmGroup> groups;
         int row = 0;
         for (int u = 0; u < nb;) {
-            const int a = S.slot(order[u]).aln;
+            const int a = P[order[u]].aln;
             const sdca_ctx *A = c->members[a];
             row = (int)round_up(row, sdca::kTile);
-            sdca::Parts P;
-            P.row[0] = row;
-            P.cmd[0] = u;
+            sdca::Parts parts;
+            parts.row[0] = row;
+            parts.cmd[0] = u;
             int v = u;
-            for (; v < nb && S.slot(order[v]).aln == a; ++v) {
-                const Slot &sl = S.slot(order[v]);
-                moff[v] = (int64_t)order[v] * G->vstride +
-                          (sl.from_xt ? sdca::gram::kVecXt : sdca::gram::kVecXc) * G->nvs;
+            for (; v < nb && P[order[v]].aln == a; ++v) {
+                const sdca::GramWs::Pending &pe = P[order[v]];
+                moff[v] = (int64_t)order[v] * G->nvs;
                 mrow[v] = row;
                 maln[v] = a;
-                sloc[v] = sl.r;
+                sloc[v] = pe.r;
                 sld[v] = (int32_t)A->Kxp;
                 soff[v] = (int64_t)v * G->n_item;
                 sdca::SlotCmd &k = mc[v];
                 k = idle_cmd();
                 k.mode = sdca::MODE_INIT;
-                k.locus = sl.r;
-                k.self_lo = A->scan_split > 0 ? 0 : A->col_off[sl.r];
-                k.self_hi = A->scan_split > 0 ? A->col_off[A->scan_split] : A->col_off[sl.r + 1];
+                k.locus = pe.r;
+                k.self_lo = A->scan_split > 0 ? 0 : A->col_off[pe.r];
+                k.self_hi = A->scan_split > 0 ? A->col_off[A->scan_split] : A->col_off[pe.r + 1];
                 k.slot = v;
                 k.row = row;
                 k.level = sdca::LV_FINE;
@@ -1705,9 +1771,9 @@ int gram_finish_loci(sdca_ctx *c, const Solver &S, const std::vector<int> &fin, 
                 row += A->q;
             }
             const int rend = (int)round_up(row, sdca::kTile);
-            P.row[1] = P.row[2] = rend;
-            P.cmd[1] = P.cmd[2] = P.cmd[3] = v;
-            groups.push_back(make_group(c, a, P, rend, false, 1, 0, gm, tn));
+            parts.row[1] = parts.row[2] = rend;
+            parts.cmd[1] = parts.cmd[2] = parts.cmd[3] = v;
+            groups.push_back(make_group(c, a, parts, rend, false, 1, 0, gm, tn));
             row = rend;
             u = v;
         }
@@ -1722,8 +1788,8 @@ int gram_finish_loci(sdca_ctx *c, const Solver &S, const std::vector<int> &fin, 
         HIP_TRY(hipMemcpyAsync(G->d_mcmds, mc.data(), nb * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(G->d_mgroups, groups.data(), groups.size() * sizeof(sdca::GemmGroup),
                                hipMemcpyHostToDevice, st));
-        HIP_TRY(hipEventRecord(G->ev[4], st));
-        sdca::gram::split(st, G->d_moff, G->d_mrow, G->d_maln, nb, G->nchunk, G->d_galn, G->V, G->gBp, nullptr,
+        HIP_TRY(hipEventRecord(ev[4], st));
+        sdca::gram::split(st, G->d_moff, G->d_mrow, G->d_maln, nb, G->nchunk, G->d_galn, G->Pend, G->gBp, nullptr,
                           G->Mhi, G->Mlo, G->Ml3, G->mhpart);
         sdca::kern::gemm_materialize(st, SDCA_PREC_SPLIT_BF16, G->d_mgroups, (int)groups.size(), grid, tn, G->M32,
                                      G->Mhi, G->Mlo, G->Ml3, G->gBp, G->d_aln, G->d_mcmds, G->Mout, G->n_item,
@@ -1731,36 +1797,25 @@ int gram_finish_loci(sdca_ctx *c, const Solver &S, const std::vector<int> &fin, 
         sdca::kern::score(st, G->Mout, G->d_soff, G->d_sld, G->d_sloc, G->d_maln, nb, G->gL, G->gL, G->d_aln,
                           G->d_rows);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(G->ev[5], st));
+        HIP_TRY(hipEventRecord(ev[5], st));
         if (score_rows)
             HIP_TRY(hipMemcpyAsync(G->h_rows, G->d_rows, (size_t)nb * G->gL * 4, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        c->timing.score += elapsed(G->ev[4], G->ev[5]);
+        c->timing.score += elapsed(ev[4], ev[5]);
         for (int v = 0; v < nb; ++v) {
-            const Slot &sl = S.slot(order[v]);
-            const int La = c->members[sl.aln]->L;
+            const sdca::GramWs::Pending &pe = P[order[v]];
+            const int La = c->members[pe.aln]->L;
             if (score_rows) {
-                const int64_t off = row_off ? row_off[sl.item] : (int64_t)sl.item * La;
+                const int64_t off = row_off ? row_off[pe.item] : (int64_t)pe.item * La;
                 std::memcpy(score_rows + off, G->h_rows + (size_t)v * G->gL, (size_t)La * 4);
             }
             if (x_out) {
                 HIP_TRY(hipMemcpy(xdev.data(), G->Mout + (size_t)v * G->n_item, G->n_item * 4, hipMemcpyDeviceToHost));
-                dev_to_ref(c, sl.r, xdev.data(), x_out + (size_t)sl.item * n);
-            }
-            if (st_out) {
-                sdca_locus_stats &o = st_out[sl.item];
-                o.fval = sl.f;
-                o.gnorm = sl.gnorm;
-                o.nfeval = sl.nfeval;
-                o.iter = sl.iter;
-                o.status = sl.status;
-                o.nfeval_first = sl.nfeval_first;
-                o.gnorm_J = sl.gnormJ;
-                o.nfeval_fine = 0;
-                o.reserved = 0;
+                dev_to_ref(c, pe.r, xdev.data(), x_out + (size_t)pe.item * n);
             }
         }
     }
+    G->pend.clear();
     return SDCA_OK;
 }
 
@@ -1777,6 +1832,8 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
     if ((rc = upload_gram_alns(c))) return rc;
     const sdca::ResLayout R{m};
     Solver S(c, nslots, m, true);
+    sdca::GramWs *G = c->gram;
+    const int ng = G->ngroups, gs = G->gs;
     std::vector<double> res((size_t)nslots * R.width(), 0.0);
     int next = 0, done = 0, noconv = 0, nonfinite = 0;
     auto fill = [&](int i) {
@@ -1787,32 +1844,61 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
         const int item = next++;
         S.assign(i, item, alns ? alns[item] : 0, loci[item], false);
     };
+    auto group_end = [&](int g) { return std::min(nslots, (g + 1) * gs); };
+    auto group_active = [&](int g) {
+        for (int i = g * gs; i < group_end(g); ++i)
+            if (S.slot(i).phase != PH_IDLE) return true;
+        return false;
+    };
     for (int i = 0; i < nslots; ++i) fill(i);
+    bool inflight[sdca::kMaxGroups] = {};
+    for (int g = 0; g < ng; ++g) {
+        if (!group_active(g)) continue;
+        S.mark_live(g * gs, group_end(g));
+        if ((rc = gram_launch_round(c, g, S.cmds()))) return rc;
+        inflight[g] = true;
+    }
     int64_t rounds = 0;
     std::vector<int> finished;
+    // the groups' rounds alternate: while the host steps one group's line searches (and
+    // forms its finished loci's couplings), the other group's round runs on the device
     while (done < K) {
-        S.mark_live(0, nslots);
-        if ((rc = gram_launch_round(c, S.cmds()))) return rc;
-        if (c->gram->cmd_slot.empty()) break;
-        if ((rc = gram_finish_round(c, S.cmds(), res))) return rc;
-        ++rounds;
-        finished.clear();
-        for (int i = 0; i < nslots; ++i) {
-            if (S.slot(i).phase == PH_IDLE) continue;
-            if (S.process(i, res.data() + (size_t)i * R.width())) finished.push_back(i);
+        bool any = false;
+        for (int g = 0; g < ng; ++g) {
+            if (!inflight[g]) continue;
+            any = true;
+            inflight[g] = false;
+            if ((rc = gram_finish_round(c, g, S.cmds(), res))) return rc;
+            ++rounds;
+            finished.clear();
+            for (int i = g * gs; i < group_end(g); ++i) {
+                if (S.slot(i).phase == PH_IDLE) continue;
+                if (S.process(i, res.data() + (size_t)i * R.width())) finished.push_back(i);
+            }
+            if (!finished.empty()) {
+                if ((rc = gram_stage(c, g, S, finished, st_out))) return rc;
+                for (int i : finished) {
+                    const Slot &sl = S.slot(i);
+                    noconv += (sl.status == SDCA_ST_ITER_CAP || sl.status == SDCA_ST_LS_FAIL);
+                    nonfinite += (sl.status == SDCA_ST_NONFINITE);
+                    ++done;
+                    fill(i);
+                }
+                if (c->opts.verbose)
+                    fprintf(stderr, "sdca: %d / %d loci solved after %lld rounds\n", done, K, (long long)rounds);
+            }
+            // a full batch of finished loci: couplings and score rows (while the other group's
+            // round runs)
+            if ((rc = gram_flush(c, g, false, score_rows, row_off, x_out))) return rc;
+            if (group_active(g)) {
+                S.mark_live(g * gs, group_end(g));
+                if ((rc = gram_launch_round(c, g, S.cmds()))) return rc;
+                inflight[g] = true;
+            }
         }
-        if (finished.empty()) continue;
-        if ((rc = gram_finish_loci(c, S, finished, alns, score_rows, row_off, x_out, st_out))) return rc;
-        for (int i : finished) {
-            const Slot &sl = S.slot(i);
-            noconv += (sl.status == SDCA_ST_ITER_CAP || sl.status == SDCA_ST_LS_FAIL);
-            nonfinite += (sl.status == SDCA_ST_NONFINITE);
-            ++done;
-            fill(i);
-        }
-        if (c->opts.verbose)
-            fprintf(stderr, "sdca: %d / %d loci solved after %lld rounds\n", done, K, (long long)rounds);
+        if (!any) break;
     }
+    if ((rc = gram_flush(c, 0, true, score_rows, row_off, x_out))) return rc;
     c->timing.total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (nonfinite) return fail(SDCA_E_NONFINITE, "%d loci reached a non-finite objective", nonfinite);
     if (noconv) return fail(SDCA_E_NOCONV, "%d loci stopped without reaching the gradient threshold", noconv);

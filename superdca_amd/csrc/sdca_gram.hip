@@ -216,65 +216,31 @@ __global__ __launch_bounds__(256) void k_dgemm(const double *__restrict__ A, int
 
 // ------------------------------------------------------------- trial points
 
+struct SlotPtrs {
+    double *xc, *xt, *dv, *gt, *Sb, *Yb;
+    const double *gk, *gold;
+};
+
+__device__ __forceinline__ SlotPtrs slot_ptrs(const SlotCmd &c, double *V, int64_t vstride, int64_t nvs, int M)
+{
+    double *base = V + (int64_t)c.slot * vstride;
+    SlotPtrs p;
+    p.xc = base + kVecXc * nvs;
+    p.xt = base + kVecXt * nvs;
+    p.dv = base + kVecD * nvs;
+    p.gt = base + (kVecG + (c.gt == 1)) * nvs;
+    p.gk = base + (kVecG + (c.gk == 1)) * nvs;    // NEWDIR: the current gradient
+    p.gold = base + (kVecG + (c.gk != 1)) * nvs;  // NEWDIR: the previous one
+    p.Sb = base + kVecPairs * nvs;
+    p.Yb = p.Sb + (int64_t)M * nvs;
+    return p;
+}
+
 struct StepArgs {
     int mode, pnew;
     bool accept;
     double alpha, pstep, cd, cg;
 };
-
-// One element e of a slot's vectors (a coefficient, an image entry or a field), as
-// k_update does it for the primal vectors:
-//   INIT    x_c = 0, x_t = 0
-//   RETRY   x_t = x_c + alpha d
-//   NEWDIR  x_c += pstep d_old (accept); s_pnew = pstep d_old, y_pnew = g[gk] - g[1-gk];
-//           d = cd d_old + cg g_k + sum_p (cs[p] s_p + cy[p] y_p); x_t = x_c + alpha d
-// Returns x_t(e); dn and gv receive d(e) and g_k(e) (NEWDIR, else 0).  The operations are
-// linear, so an image entry follows its coefficients exactly.
-template <int M>
-__device__ __forceinline__ double step_elem(const StepArgs &a, const double (&cs)[M], const double (&cy)[M],
-                                            double *__restrict__ xc, double *__restrict__ xt,
-                                            double *__restrict__ dv, const double *__restrict__ gk,
-                                            const double *__restrict__ gold, double *__restrict__ Sb,
-                                            double *__restrict__ Yb, int64_t nvs, int64_t e, double &dn, double &gv)
-{
-    double xn;
-    dn = 0.0;
-    gv = 0.0;
-    if (a.mode == MODE_INIT) {
-        xn = 0.0;
-        xc[e] = 0.0;
-    } else if (a.mode == MODE_RETRY) {
-        xn = fma(a.alpha, dv[e], xc[e]);
-    } else {
-        const double dold = dv[e];
-        double xv = xc[e];
-        if (a.accept) {
-            xv = fma(a.pstep, dold, xv);
-            xc[e] = xv;
-        }
-        const double g = gk[e];
-        double sn = 0.0, yn = 0.0;
-        if (a.pnew >= 0) {
-            sn = a.pstep * dold;
-            yn = g - gold[e];
-            Sb[(int64_t)a.pnew * nvs + e] = sn;
-            Yb[(int64_t)a.pnew * nvs + e] = yn;
-        }
-        double d = a.cd * dold + a.cg * g;
-#pragma unroll
-        for (int p = 0; p < M; ++p) {
-            const bool mine = p == a.pnew;
-            if (cs[p] != 0.0) d += cs[p] * (mine ? sn : Sb[(int64_t)p * nvs + e]);
-            if (cy[p] != 0.0) d += cy[p] * (mine ? yn : Yb[(int64_t)p * nvs + e]);
-        }
-        dv[e] = d;
-        xn = fma(a.alpha, d, xv);
-        dn = d;
-        gv = g;
-    }
-    xt[e] = xn;
-    return xn;
-}
 
 template <int M>
 __device__ __forceinline__ void load_step(const SlotCmd &c, StepArgs &a, double (&cs)[M], double (&cy)[M])
@@ -293,33 +259,95 @@ __device__ __forceinline__ void load_step(const SlotCmd &c, StepArgs &a, double 
     }
 }
 
-struct SlotPtrs {
-    double *xc, *xt, *dv, *gt, *Sb, *Yb;
-    const double *gk, *gold;
-};
-
-__device__ __forceinline__ SlotPtrs slot_ptrs(const SlotCmd &c, double *V, int64_t vstride, int64_t nvs, int M)
+// NE elements e[k] of a slot's vectors (coefficients, image entries or fields), as k_update
+// does it for the primal vectors:
+//   INIT    x_c = 0, x_t = 0
+//   RETRY   x_t = x_c + alpha d
+//   NEWDIR  x_c += pstep d_old (accept); s_pnew = pstep d_old, y_pnew = g[gk] - g[1-gk];
+//           d = cd d_old + cg g_k + sum_p (cs[p] s_p + cy[p] y_p); x_t = x_c + alpha d
+// xn receives x_t(e), dn and gv d(e) and g_k(e) (NEWDIR, else 0).  Every load comes before
+// the arithmetic, so that a thread has all of its element's vectors in flight at once.
+// The operations are linear, so an image entry follows its coefficients exactly.
+template <int M, int NE>
+__device__ __forceinline__ void step_elems(const StepArgs &a, const double (&cs)[M], const double (&cy)[M],
+                                           const SlotPtrs &P, int64_t nvs, const int64_t (&e)[NE],
+                                           double (&xn)[NE], double (&dn)[NE], double (&gv)[NE])
 {
-    double *base = V + (int64_t)c.slot * vstride;
-    SlotPtrs p;
-    p.xc = base + kVecXc * nvs;
-    p.xt = base + kVecXt * nvs;
-    p.dv = base + kVecD * nvs;
-    p.gt = base + (kVecG + (c.gt == 1)) * nvs;
-    p.gk = base + (kVecG + (c.gk == 1)) * nvs;     // NEWDIR: the current gradient
-    p.gold = base + (kVecG + (c.gk != 1)) * nvs;   // NEWDIR: the previous one
-    p.Sb = base + kVecPairs * nvs;
-    p.Yb = p.Sb + (int64_t)M * nvs;
-    return p;
+#pragma unroll
+    for (int k = 0; k < NE; ++k) dn[k] = gv[k] = 0.0;
+    if (a.mode == MODE_INIT) {
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            xn[k] = 0.0;
+            P.xc[e[k]] = 0.0;
+            P.xt[e[k]] = 0.0;
+        }
+        return;
+    }
+    if (a.mode == MODE_RETRY) {
+        double x[NE], d[NE];
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            x[k] = P.xc[e[k]];
+            d[k] = P.dv[e[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            xn[k] = fma(a.alpha, d[k], x[k]);
+            P.xt[e[k]] = xn[k];
+        }
+        return;
+    }
+    double dold[NE], xv[NE], g[NE], go[NE], sv[M][NE], yv[M][NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        dold[k] = P.dv[e[k]];
+        xv[k] = P.xc[e[k]];
+        g[k] = P.gk[e[k]];
+        go[k] = a.pnew >= 0 ? P.gold[e[k]] : 0.0;
+    }
+#pragma unroll
+    for (int p = 0; p < M; ++p)
+#pragma unroll
+        for (int k = 0; k < NE; ++k) {
+            sv[p][k] = (cs[p] != 0.0 && p != a.pnew) ? P.Sb[(int64_t)p * nvs + e[k]] : 0.0;
+            yv[p][k] = (cy[p] != 0.0 && p != a.pnew) ? P.Yb[(int64_t)p * nvs + e[k]] : 0.0;
+        }
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        if (a.accept) {
+            xv[k] = fma(a.pstep, dold[k], xv[k]);
+            P.xc[e[k]] = xv[k];
+        }
+        double sn = 0.0, yn = 0.0;
+        if (a.pnew >= 0) {
+            sn = a.pstep * dold[k];
+            yn = g[k] - go[k];
+            P.Sb[(int64_t)a.pnew * nvs + e[k]] = sn;
+            P.Yb[(int64_t)a.pnew * nvs + e[k]] = yn;
+        }
+        double d = a.cd * dold[k] + a.cg * g[k];
+#pragma unroll
+        for (int p = 0; p < M; ++p) {
+            const bool mine = p == a.pnew;
+            d += cs[p] * (mine ? sn : sv[p][k]);
+            d += cy[p] * (mine ? yn : yv[p][k]);
+        }
+        P.dv[e[k]] = d;
+        xn[k] = fma(a.alpha, d, xv[k]);
+        P.xt[e[k]] = xn[k];
+        dn[k] = d;
+        gv[k] = g[k];
+    }
 }
 
 // The fields of every command's trial point (one wave per command): the field entries
-// step first, so that the softmax blocks of k_gstep all read the final h of x_t.
-// upd_part[command][nchunk] = {0, |h_t|^2, d_h . g_k,h}.
+// step first, so that the softmax of k_gsoftmax reads the final h of x_t.
+// upd_part[command][nchq] = {0, |h_t|^2, d_h . g_k,h}.
 template <int M>
 __global__ __launch_bounds__(64) void k_ghstep(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
                                                double *__restrict__ V, int64_t vstride, int64_t nvs,
-                                               double *__restrict__ upd_part, int nchunk)
+                                               double *__restrict__ upd_part, int nchq)
 {
     const int j = blockIdx.x;
     const SlotCmd &c = cmds[j];
@@ -332,73 +360,91 @@ __global__ __launch_bounds__(64) void k_ghstep(const SlotCmd *__restrict__ cmds,
             double cs[M], cy[M];
             load_step<M>(c, a, cs, cy);
             const SlotPtrs P = slot_ptrs(c, V, vstride, nvs, M);
-            const int64_t e = 2 * (int64_t)A.q * A.Bp + s;
-            double dn, g;
-            const double xn = step_elem<M>(a, cs, cy, P.xc, P.xt, P.dv, P.gk, P.gold, P.Sb, P.Yb, nvs, e, dn, g);
-            xsqh = xn * xn;
-            dgh = dn * g;
+            const int64_t e[1] = {2 * (int64_t)A.q * A.Bp + s};
+            double xn[1], dn[1], g[1];
+            step_elems<M, 1>(a, cs, cy, P, nvs, e, xn, dn, g);
+            xsqh = xn[0] * xn[0];
+            dgh = dn[0] * g[0];
         }
     }
     xsqh = wave_sum(xsqh);
     dgh = wave_sum(dgh);
     if (s == 0) {
-        double *o = upd_part + ((int64_t)j * (nchunk + 1) + nchunk) * 3;
+        double *o = upd_part + ((int64_t)j * (nchq + 1) + nchq) * 3;
         o[0] = 0.0;
         o[1] = xsqh;
         o[2] = dgh;
     }
 }
 
-// Trial point (coefficients and image) and softmax, one thread per sequence b of a
-// command (blocks of 256 sequences):
+// Trial point of every command, one thread per coefficient (s, b) and its image entry
+// (flat over the q Bp coefficients; blocks of 256).  upd_part[command][chunk] =
+// {c_t . K_r c_t, 0, d . g_k} with d . g_k = c_d . (K_r c_gk).
+template <int M>
+__global__ __launch_bounds__(256) void k_gstep(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                               double *__restrict__ V, int64_t vstride, int64_t nvs,
+                                               double *__restrict__ upd_part, int nchq)
+{
+    const int j = blockIdx.y;
+    const SlotCmd &c = cmds[j];
+    const int tid = threadIdx.x;
+    double *up = upd_part + ((int64_t)j * (nchq + 1) + blockIdx.x) * 3;
+    if (c.mode == MODE_IDLE) {
+        if (tid < 3) up[tid] = 0.0;
+        return;
+    }
+    const GramAln &A = aln[c.aln];
+    const int64_t qB = (int64_t)A.q * A.Bp;
+    StepArgs a;
+    double cs[M], cy[M];
+    load_step<M>(c, a, cs, cy);
+    const SlotPtrs P = slot_ptrs(c, V, vstride, nvs, M);
+    const int64_t idx = (int64_t)blockIdx.x * 256 + tid;
+    double red[3] = {0.0, 0.0, 0.0};
+    if (idx < qB) {
+        const int64_t e[2] = {idx, idx + qB};
+        double xn[2], dn[2], gv[2];
+        step_elems<M, 2>(a, cs, cy, P, nvs, e, xn, dn, gv);
+        red[0] = xn[0] * xn[1];
+        red[2] = dn[0] * gv[1];
+    }
+    block_sum_store<3>(red, up);
+}
+
+// Softmax of every command, one thread per sequence b (blocks of 256 sequences):
 //   E_b(s) = h_t(s) + (K_r c_t)(s, b), lz = log sum_s exp E_b(s) (max-shifted),
 //   f_b = w_b (lz - E_b(x_br)), G_b(s) = w_b (P_b(s) - [s == x_br])
 // (plmDCA_objective.hpp:86-151), G into the packed GEMM operand (column row + s, b
 // contiguous) and the trial gradient's coefficients G + 2 lambda_J c_t into g[gt].
-// Per block: upd_part {c_t . K_r c_t, 0, d . g_k} and sm_part {T(s, a) at 8 s + a, sum f_b}.
-template <int M>
-__global__ __launch_bounds__(256) void k_gstep(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
-                                               double *__restrict__ V, int64_t vstride, int64_t nvs,
-                                               double *__restrict__ Gp, int64_t ldG, double *__restrict__ upd_part,
-                                               double *__restrict__ sm_part, int nchunk)
+// sm_part[command][chunk] = {T(s, a) at 8 s + a, sum f_b}.
+__global__ __launch_bounds__(256) void k_gsoftmax(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                                  double *__restrict__ V, int64_t vstride, int64_t nvs,
+                                                  double *__restrict__ Gp, int64_t ldG, double *__restrict__ sm_part,
+                                                  int nchunk)
 {
     const int j = blockIdx.y;
     const SlotCmd &c = cmds[j];
     __shared__ double sW[4][kSm];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    double *up = upd_part + ((int64_t)j * (nchunk + 1) + blockIdx.x) * 3;
     double *sp = sm_part + ((int64_t)j * nchunk + blockIdx.x) * kSm;
     if (c.mode == MODE_IDLE) {
-        if (tid < 3) up[tid] = 0.0;
         if (tid < kSm) sp[tid] = 0.0;
         return;
     }
     const GramAln &A = aln[c.aln];
     const int q = A.q, Bp = A.Bp, B = A.B;
     const int64_t qB = (int64_t)q * Bp;
-    StepArgs a;
-    double cs[M], cy[M];
-    load_step<M>(c, a, cs, cy);
-    const SlotPtrs P = slot_ptrs(c, V, vstride, nvs, M);
-    double hs[8];
-#pragma unroll
-    for (int s = 0; s < 8; ++s) hs[s] = s < q ? P.xt[2 * qB + s] : 0.0;  // from k_ghstep
+    double *base = V + (int64_t)c.slot * vstride;
+    const double *xt = base + kVecXt * nvs;
+    double *gt = base + (kVecG + (c.gt == 1)) * nvs;
+    double hs[8], xtc[8], xti[8];
     const int b = blockIdx.x * 256 + tid;
     const bool valid = b < Bp;
-    double xtc[8], xti[8];
-    double xsqJ = 0.0, dg = 0.0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        xtc[s] = 0.0;
-        xti[s] = 0.0;
-        if (s < q && valid) {
-            const int64_t ec = (int64_t)s * Bp + b, ei = ec + qB;
-            double dnc, gc, dni, gi;
-            xtc[s] = step_elem<M>(a, cs, cy, P.xc, P.xt, P.dv, P.gk, P.gold, P.Sb, P.Yb, nvs, ec, dnc, gc);
-            xti[s] = step_elem<M>(a, cs, cy, P.xc, P.xt, P.dv, P.gk, P.gold, P.Sb, P.Yb, nvs, ei, dni, gi);
-            xsqJ += xtc[s] * xti[s];
-            dg += dnc * gi;  // d . g_k = c_d . (K_r c_gk)
-        }
+        hs[s] = s < q ? xt[2 * qB + s] : 0.0;  // from k_ghstep
+        xtc[s] = (s < q && valid) ? xt[(int64_t)s * Bp + b] : 0.0;
+        xti[s] = (s < q && valid) ? xt[qB + (int64_t)s * Bp + b] : 0.0;
     }
     double G[8];
     double fb = 0.0;
@@ -429,13 +475,13 @@ __global__ __launch_bounds__(256) void k_gstep(const SlotCmd *__restrict__ cmds,
             if (s >= q) continue;
             G[s] = wb * (ex[s] * iz - (s == y ? 1.0 : 0.0));
             Gp[(int64_t)(c.row + s) * ldG + b] = G[s];
-            P.gt[(int64_t)s * Bp + b] = G[s] + A.lamJ2 * xtc[s];
+            gt[(int64_t)s * Bp + b] = G[s] + A.lamJ2 * xtc[s];
         }
     } else if (valid) {
 #pragma unroll
         for (int s = 0; s < 8; ++s)
             if (s < q) {
-                P.gt[(int64_t)s * Bp + b] = 0.0;  // pad sequences (the GEMM reads them: zero)
+                gt[(int64_t)s * Bp + b] = 0.0;  // pad sequences (the GEMM reads them: zero)
                 Gp[(int64_t)(c.row + s) * ldG + b] = 0.0;
             }
     }
@@ -450,8 +496,7 @@ __global__ __launch_bounds__(256) void k_gstep(const SlotCmd *__restrict__ cmds,
         }
     fb = wave_sum(fb);
     if (lane == 0) sW[wave][64] = fb;
-    double red[3] = {xsqJ, 0.0, dg};
-    block_sum_store<3>(red, up);  // synchronises the block: sW is complete below
+    __syncthreads();
     if (tid < kSm) {
         const int s = tid >> 3, u = tid & 7;
         const bool ok = tid == 64 || (s < q && u < q);
@@ -459,7 +504,8 @@ __global__ __launch_bounds__(256) void k_gstep(const SlotCmd *__restrict__ cmds,
     }
 }
 
-// The trial gradient's image and its products, one thread per sequence b of a command:
+// The trial gradient's image and its products, one thread per coefficient (s, b) (flat
+// over the q Bp coefficients):
 //   (K_r g_t)(s, b) = Z(s, b) - T(s, x_br) + 2 lambda_J (K_r c_t)(s, b), Z = K G (the GEMM),
 // stored into g[gt]; the fields' gradient sum_b G_b(s) + 2 lambda_h h_t(s) (block 0).
 // part[command][chunk][2M + 5] = {g_t.s_p, g_t.y_p, g_t.g_t, g_t.g_k, g_t.d, 0, |g_t,J|^2}
@@ -468,8 +514,8 @@ template <int M>
 __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
                                                double *__restrict__ V, int64_t vstride, int64_t nvs,
                                                const double *__restrict__ Zp, int64_t ldG,
-                                               const double *__restrict__ sm_part, double *__restrict__ part,
-                                               int nchunk)
+                                               const double *__restrict__ sm_part, int nchunk,
+                                               double *__restrict__ part, int nchq)
 {
     constexpr int NV = 2 * M + 5;
     const int j = blockIdx.y;
@@ -479,7 +525,7 @@ __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds,
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    double *out = part + ((int64_t)j * nchunk + blockIdx.x) * NV;
+    double *out = part + ((int64_t)j * nchq + blockIdx.x) * NV;
     if (c.mode == MODE_IDLE || c.gt < 0 || c.gt > 1) {
         for (int k = tid; k < NV; k += 256) out[k] = 0.0;
         return;
@@ -503,31 +549,34 @@ __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds,
     }
     __syncthreads();
     const bool cur = c.mode != MODE_INIT;  // a current gradient and a direction exist
-    const uint32_t live = c.live;
-    const int b = blockIdx.x * 256 + tid;
-    if (b < Bp) {
-        const int y = b < B ? A.statesT[(int64_t)c.locus * B + b] : 0;
+    const uint32_t live = cur ? c.live : 0u;
+    const int64_t idx = (int64_t)blockIdx.x * 256 + tid;
+    if (idx < qB) {
+        const int s = (int)(idx / Bp), b = (int)(idx - (int64_t)s * Bp);
+        const int64_t e0 = idx, e1 = idx + qB;
+        const bool seq = b < B;
+        const double z = seq ? Zp[(int64_t)(c.row + s) * ldG + b] : 0.0;
+        const int y = seq ? A.statesT[(int64_t)c.locus * B + b] : 0;
+        const double xi = xt[e1];
+        const double gc = gt[e0];
+        const double gki = cur ? gk[e1] : 0.0, di = cur ? dv[e1] : 0.0;
+        double sp[M], yp[M];
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            if (s >= q) continue;
-            const int64_t ec = (int64_t)s * Bp + b, ei = ec + qB;
-            double kg = 0.0;
-            if (b < B)
-                kg = Zp[(int64_t)(c.row + s) * ldG + b] - (A.own ? sT[s * 8 + y] : 0.0) + A.lamJ2 * xt[ei];
-            gt[ei] = kg;
-            const double gc = gt[ec];
-            acc[2 * M] += gc * kg;
-            acc[2 * M + 4] += gc * kg;
-            if (cur) {
-                acc[2 * M + 1] += gc * gk[ei];
-                acc[2 * M + 2] += gc * dv[ei];
+        for (int p = 0; p < M; ++p) {
+            const bool on = (live >> p) & 1u;
+            sp[p] = on ? Sb[(int64_t)p * nvs + e1] : 0.0;
+            yp[p] = on ? Yb[(int64_t)p * nvs + e1] : 0.0;
+        }
+        const double kg = seq ? z - (A.own ? sT[s * 8 + y] : 0.0) + A.lamJ2 * xi : 0.0;
+        gt[e1] = kg;
+        acc[2 * M] = gc * kg;
+        acc[2 * M + 4] = gc * kg;
+        acc[2 * M + 1] = gc * gki;
+        acc[2 * M + 2] = gc * di;
 #pragma unroll
-                for (int p = 0; p < M; ++p) {
-                    if (!((live >> p) & 1u)) continue;
-                    acc[p] += gc * Sb[(int64_t)p * nvs + ei];
-                    acc[M + p] += gc * Yb[(int64_t)p * nvs + ei];
-                }
-            }
+        for (int p = 0; p < M; ++p) {
+            acc[p] = gc * sp[p];
+            acc[M + p] = gc * yp[p];
         }
     }
     if (blockIdx.x == 0 && tid < q) {
@@ -556,7 +605,7 @@ __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds,
 // Result rows (ResLayout): the products, {c_t . K_r c_t, |h_t|^2, d . g_k} and the data
 // term.  The entry of max |g| holds |g_t|^2 (the host takes the square roots).
 __global__ void k_greduce(int m, const double *__restrict__ upd_part, const double *__restrict__ dot_part,
-                          const double *__restrict__ sm_part, int nchunk, double *__restrict__ res)
+                          const double *__restrict__ sm_part, int nchunk, int nchq, double *__restrict__ res)
 {
     const int i = blockIdx.x;
     const int t = threadIdx.x;
@@ -564,12 +613,12 @@ __global__ void k_greduce(int m, const double *__restrict__ upd_part, const doub
     double *out = res + (int64_t)i * W;
     if (t < ND) {
         double a = 0.0;
-        for (int ch = 0; ch < nchunk; ++ch) a += dot_part[((int64_t)i * nchunk + ch) * ND + t];
+        for (int ch = 0; ch < nchq; ++ch) a += dot_part[((int64_t)i * nchq + ch) * ND + t];
         out[t] = a;
     } else if (t < ND + 3) {
         const int u = t - ND;
         double a = 0.0;
-        for (int ch = 0; ch <= nchunk; ++ch) a += upd_part[((int64_t)i * (nchunk + 1) + ch) * 3 + u];
+        for (int ch = 0; ch <= nchq; ++ch) a += upd_part[((int64_t)i * (nchq + 1) + ch) * 3 + u];
         out[t] = a;
     } else if (t == W - 1) {
         double a = 0.0;
@@ -645,32 +694,33 @@ void gemm(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_
     }
 
 void step(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
-          int64_t nvs, double *Gp, int64_t ldG, double *upd_part, double *sm_part, int nchunk)
+          int64_t nvs, double *Gp, int64_t ldG, double *upd_part, double *sm_part, int nchunk, int nchq)
 {
-#define SDCA_GSTEP(MM)                                                                                            \
-    hipLaunchKernelGGL((k_ghstep<MM>), dim3(ncmd), dim3(64), 0, st, cmds, aln, V, vstride, nvs, upd_part, nchunk); \
-    hipLaunchKernelGGL((k_gstep<MM>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Gp, ldG,  \
-                       upd_part, sm_part, nchunk)
+#define SDCA_GSTEP(MM)                                                                                               \
+    hipLaunchKernelGGL((k_ghstep<MM>), dim3(ncmd), dim3(64), 0, st, cmds, aln, V, vstride, nvs, upd_part, nchq);     \
+    hipLaunchKernelGGL((k_gstep<MM>), dim3(nchq, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, upd_part, nchq)
     SDCA_GRAM_DISPATCH(m, SDCA_GSTEP)
 #undef SDCA_GSTEP
+    hipLaunchKernelGGL(k_gsoftmax, dim3(nchunk, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Gp, ldG, sm_part,
+                       nchunk);
 }
 
 void dots(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
-          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, double *dot_part, int nchunk)
+          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, int nchunk, double *dot_part, int nchq)
 {
 #define SDCA_GDOTS(MM)                                                                                            \
-    hipLaunchKernelGGL((k_gdots<MM>), dim3(nchunk, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Zp, ldG, \
-                       sm_part, dot_part, nchunk)
+    hipLaunchKernelGGL((k_gdots<MM>), dim3(nchq, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Zp, ldG,   \
+                       sm_part, nchunk, dot_part, nchq)
     SDCA_GRAM_DISPATCH(m, SDCA_GDOTS)
 #undef SDCA_GDOTS
 }
 
 void reduce(hipStream_t st, int m, int ncmd, const double *upd_part, const double *dot_part, const double *sm_part,
-            int nchunk, double *res)
+            int nchunk, int nchq, double *res)
 {
     const int W = 2 * m + 9;
     hipLaunchKernelGGL(k_greduce, dim3(ncmd), dim3((W + 63) / 64 * 64), 0, st, m, upd_part, dot_part, sm_part,
-                       nchunk, res);
+                       nchunk, nchq, res);
 }
 
 void split(hipStream_t st, const int64_t *src_off, const int32_t *rows, const int32_t *alns, int n, int nchunk,

@@ -281,18 +281,19 @@ inline int64_t vec_len(int q, int Bp) { return 2 * (int64_t)q * Bp + 8; }
 // Identical-locus counts over loci [j0, L) from bit-planes (k_bitplanes layout).
 void gram_matrix(hipStream_t st, const uint64_t *planes, int B, int q, int W, int j0, int64_t ldk, double *K);
 // Trial points and softmax of a round's commands: G into Gp (column row + s of ldG),
-// partials upd_part [cmd][nchunk + 1][3] and sm_part [cmd][nchunk][65].
+// partials upd_part [cmd][nchq + 1][3] (blocks of 256 coefficients) and sm_part
+// [cmd][nchunk][65] (blocks of 256 sequences).
 void step(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
-          int64_t nvs, double *Gp, int64_t ldG, double *upd_part, double *sm_part, int nchunk);
+          int64_t nvs, double *Gp, int64_t ldG, double *upd_part, double *sm_part, int nchunk, int nchq);
 // C[n][m] = sum_k Bm[n][k] A[m][k], FP64 MFMA: m < M (multiple of 128), k < Kd (multiple
 // of 16), n < N stored; Bm rows up to round_up(N, 128) readable.
 void gemm(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_t ldb, double *C, int64_t ldc, int M,
           int N, int Kd);
-// Gradient images from Z = K G and the products: dot_part [cmd][nchunk][2m + 5].
+// Gradient images from Z = K G and the products: dot_part [cmd][nchq][2m + 5].
 void dots(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
-          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, double *dot_part, int nchunk);
+          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, int nchunk, double *dot_part, int nchq);
 void reduce(hipStream_t st, int m, int ncmd, const double *upd_part, const double *dot_part, const double *sm_part,
-            int nchunk, double *res);
+            int nchunk, int nchq, double *res);
 // Coefficients of n finished loci (V + src_off[u]) as contraction operands (rows[u]).
 void split(hipStream_t st, const int64_t *src_off, const int32_t *rows, const int32_t *alns, int n, int nchunk,
            const GramAln *aln, const double *V, int64_t ldm, float *G32, uint16_t *Ghi, uint16_t *Glo,
@@ -302,7 +303,13 @@ void split(hipStream_t st, const int64_t *src_off, const int32_t *rows, const in
 // Workspace of Gram-formulation solves (sdca_api.cpp), sized for `slots` loci in flight,
 // history m and the largest member alignment (q gq, padded depth gBp, gL loci, gKxp columns).
 struct GramWs {
-    int slots = 0, m = 0, gq = 0, gBp = 0, gL = 0, nchunk = 0, nmem = 0;
+    int slots = 0, m = 0, gq = 0, gBp = 0, gL = 0, nmem = 0;
+    int nchunk = 0, nchq = 0;  // blocks of 256 sequences, of 256 coefficients (q x Bp) per command
+    // slot groups whose rounds run on their own streams (c->gstream): gs slots each, colcap
+    // columns of the packed GEMM operand each; a group's command lists, partials, result
+    // rows and columns start at its first slot (or column) times the per-slot (per-column) size
+    int ngroups = 1, gs = 0;
+    int64_t colcap = 0;
     int64_t nvs = 0, vstride = 0, ldG = 0;
     double *V = nullptr;                  // slot vectors
     double *Gp = nullptr, *Zp = nullptr;  // packed GEMM operand G and product K G, ldG per column
@@ -311,9 +318,18 @@ struct GramWs {
     double *h_res = nullptr;
     GramAln *d_galn = nullptr;
     AlnDev *d_aln = nullptr;
-    std::vector<int> cmd_slot;  // slot of each listed command of the round in flight
-    // finished loci, up to nmat per batch: contraction operands (mrows packed rows of gBp),
-    // their couplings (n_item floats each: q x Kxp), score rows (gL each)
+    std::vector<int> cmd_slot[kMaxGroups];  // per group: slot of each listed command of its round in flight
+    // finished loci wait in Pend (their coefficients and fields, at nvs doubles per locus,
+    // up to pcap of them) until nmat have gathered; then, nmat per batch: contraction
+    // operands (mrows packed rows of gBp), couplings (n_item floats each: q x Kxp), score rows
+    // (gL each)
+    struct Pending {
+        int item, r, aln;
+    };
+    double *Pend = nullptr;
+    int pcap = 0;
+    std::vector<Pending> pend;
+    hipEvent_t stage_ev[kMaxGroups] = {};
     int nmat = 0, mrows = 0;
     int64_t n_item = 0;
     float *M32 = nullptr, *Mout = nullptr;
@@ -324,7 +340,7 @@ struct GramWs {
     int64_t *d_moff = nullptr, *d_soff = nullptr;
     int32_t *d_mrow = nullptr, *d_maln = nullptr, *d_sloc = nullptr, *d_sld = nullptr;
     float *d_rows = nullptr, *h_rows = nullptr;  // h_rows: pinned
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[kMaxGroups][8] = {};
     ~GramWs();
 };
 
