@@ -1486,6 +1486,7 @@ int ensure_gram_ws(sdca_ctx *c, int nslots, int m)
     G->ldG = gBp;
     G->nchunk = gBp / 256;
     G->nchq = gq * gBp / 256;
+    G->nchd = sdca::gram::dots_chunks((int64_t)gq * gBp);
     G->nmat = std::min(nslots, 128);
     G->ngroups = ng;
     G->gs = (nslots + ng - 1) / ng;
@@ -1500,7 +1501,7 @@ int ensure_gram_ws(sdca_ctx *c, int nslots, int m)
         (rc = dalloc(&G->Zp, (size_t)ncols * G->ldG)) ||
         (rc = dalloc(&G->upd_part, (size_t)npad * (G->nchq + 1) * 3)) ||
         (rc = dalloc(&G->sm_part, (size_t)npad * G->nchunk * 65)) ||
-        (rc = dalloc(&G->dot_part, (size_t)npad * G->nchq * sdca::ResLayout{m}.ndots())) ||
+        (rc = dalloc(&G->dot_part, (size_t)npad * G->nchd * sdca::ResLayout{m}.ndots())) ||
         (rc = dalloc(&G->res, (size_t)npad * W)) || (rc = dalloc(&G->d_cmds, (size_t)npad)) ||
         (rc = dalloc(&G->d_galn, (size_t)nmem)) || (rc = dalloc(&G->d_aln, (size_t)nmem)))
         return rc;
@@ -1594,7 +1595,7 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     sdca::SlotCmd *dc = G->d_cmds + s0;
     double *Gp = G->Gp + (size_t)g * G->colcap * G->ldG, *Zp = G->Zp + (size_t)g * G->colcap * G->ldG;
     double *upd = G->upd_part + (size_t)s0 * (G->nchq + 1) * 3, *sm = G->sm_part + (size_t)s0 * G->nchunk * 65;
-    double *dot = G->dot_part + (size_t)s0 * G->nchq * sdca::ResLayout{m}.ndots(), *res = G->res + (size_t)s0 * W;
+    double *dot = G->dot_part + (size_t)s0 * G->nchd * sdca::ResLayout{m}.ndots(), *res = G->res + (size_t)s0 * W;
     HIP_TRY(hipMemcpyAsync(dc, hc, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(ev[0], st));
     sdca::gram::step(st, m, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, Gp, G->ldG, upd, sm, G->nchunk, G->nchq);
@@ -1608,8 +1609,8 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[2], st));
-    sdca::gram::dots(st, m, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, Zp, G->ldG, sm, G->nchunk, dot, G->nchq);
-    sdca::gram::reduce(st, m, nc, upd, dot, sm, G->nchunk, G->nchq, res);
+    sdca::gram::dots(st, m, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, Zp, G->ldG, sm, G->nchunk, dot, G->nchd);
+    sdca::gram::reduce(st, m, nc, upd, dot, sm, G->nchunk, G->nchq, G->nchd, res);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[3], st));
     HIP_TRY(hipMemcpyAsync(G->h_res + (size_t)s0 * W, res, (size_t)nc * W * sizeof(double), hipMemcpyDeviceToHost,

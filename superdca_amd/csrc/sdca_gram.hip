@@ -125,10 +125,14 @@ __global__ __launch_bounds__(256) void k_gram(const uint64_t *__restrict__ plane
 // C[n][m] = sum_k Bm[n][k] A[m][k] (both operands K-contiguous; with A = K symmetric this
 // is Z = K G per column n of the packed beliefs).  128 x 128 output tiles, four waves of
 // 64 x 64 (4 x 4 tiles of v_mfma_f64_16x16x4f64), K in steps of 16 through LDS (k-major,
-// double-buffered: one barrier per step).  m < M (a multiple of 128), k < Kd (a multiple
-// of 16); Bm rows [0, round_up(N, 128)) must be readable, only n < N is stored.
+// double-buffered: one barrier per step).  One block per CU: the FP64 MFMA pipes stay busy
+// with one wave per SIMD, and the other slot group's memory-bound passes share the CU (at
+// C4 this gains more than the second block's latency hiding).  m < M (a multiple of 128),
+// k < Kd (a multiple of 16); Bm rows [0, round_up(N, 128)) must be readable, only n < N is
+// stored.
 typedef double dx4 __attribute__((ext_vector_type(4)));
-constexpr int DT = 128, DK = 16, DPAD = 16;
+// DPAD 48: 90 KB of LDS per block, so one block (four waves) per CU
+constexpr int DT = 128, DK = 16, DPAD = 48;
 
 __device__ __forceinline__ void ld8d(const double *p, double (&r)[8])
 {
@@ -149,15 +153,28 @@ __global__ __launch_bounds__(256) void k_dgemm(const double *__restrict__ A, int
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * DT, n0 = blockIdx.x * DT;
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
-    // staging: thread -> tile row r, k half kh (eight doubles of each operand per step)
-    const int r = tid >> 1, kh = (tid & 1) * 8;
+    // staging: thread -> tile row r, k half kh (DK / 2 doubles of each operand per step)
+    constexpr int KH = DK / 2;
+    const int r = tid >> 1, kh = (tid & 1) * KH;
     const double *ga = A + (int64_t)(m0 + r) * lda + kh;
     const double *gb = Bm + (int64_t)(n0 + r) * ldb + kh;
-    double ra[8], rb[8];
-    ld8d(ga, ra);
-    ld8d(gb, rb);
+    double ra[KH], rb[KH];
+    auto load = [&]() {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+        for (int h = 0; h < KH; h += 8) {
+            double ta[8], tb[8];
+            ld8d(ga + h, ta);
+            ld8d(gb + h, tb);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                ra[h + u] = ta[u];
+                rb[h + u] = tb[u];
+            }
+        }
+    };
+    load();
+#pragma unroll
+    for (int u = 0; u < KH; ++u) {
         sA[0][kh + u][r] = ra[u];
         sB[0][kh + u][r] = rb[u];
     }
@@ -178,8 +195,7 @@ __global__ __launch_bounds__(256) void k_dgemm(const double *__restrict__ A, int
         if (more) {
             ga += DK;
             gb += DK;
-            ld8d(ga, ra);
-            ld8d(gb, rb);
+            load();
         }
 #pragma unroll
         for (int kk = 0; kk < DK; kk += 4) {
@@ -195,7 +211,7 @@ __global__ __launch_bounds__(256) void k_dgemm(const double *__restrict__ A, int
         }
         if (more) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
+            for (int u = 0; u < KH; ++u) {
                 sA[cur ^ 1][kh + u][r] = ra[u];
                 sB[cur ^ 1][kh + u][r] = rb[u];
             }
@@ -402,11 +418,14 @@ __global__ __launch_bounds__(256) void k_gstep(const SlotCmd *__restrict__ cmds,
     const int64_t idx = (int64_t)blockIdx.x * 256 + tid;
     double red[3] = {0.0, 0.0, 0.0};
     if (idx < qB) {
-        const int64_t e[2] = {idx, idx + qB};
-        double xn[2], dn[2], gv[2];
-        step_elems<M, 2>(a, cs, cy, P, nvs, e, xn, dn, gv);
-        red[0] = xn[0] * xn[1];
-        red[2] = dn[0] * gv[1];
+        // the coefficient, then its image entry (one load phase each: fewer registers, so
+        // that the pass can share a CU with the other slot group's GEMM)
+        const int64_t ec[1] = {idx}, ei[1] = {idx + qB};
+        double xc[1], dc[1], gc[1], xi[1], di[1], gi[1];
+        step_elems<M, 1>(a, cs, cy, P, nvs, ec, xc, dc, gc);
+        step_elems<M, 1>(a, cs, cy, P, nvs, ei, xi, di, gi);
+        red[0] = xc[0] * xi[0];
+        red[2] = dc[0] * gi[0];
     }
     block_sum_store<3>(red, up);
 }
@@ -504,18 +523,19 @@ __global__ __launch_bounds__(256) void k_gsoftmax(const SlotCmd *__restrict__ cm
     }
 }
 
-// The trial gradient's image and its products, one thread per coefficient (s, b) (flat
-// over the q Bp coefficients):
+// The trial gradient's image and its products, flat over the q Bp coefficients (s, b) of a
+// command, blocks of 1024 (four per thread, 256 apart):
 //   (K_r g_t)(s, b) = Z(s, b) - T(s, x_br) + 2 lambda_J (K_r c_t)(s, b), Z = K G (the GEMM),
 // stored into g[gt]; the fields' gradient sum_b G_b(s) + 2 lambda_h h_t(s) (block 0).
 // part[command][chunk][2M + 5] = {g_t.s_p, g_t.y_p, g_t.g_t, g_t.g_k, g_t.d, 0, |g_t,J|^2}
 // in the primal metric (coefficients of one against the image of the other).
+constexpr int kDotsPerThread = 4;
 template <int M>
 __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
                                                double *__restrict__ V, int64_t vstride, int64_t nvs,
                                                const double *__restrict__ Zp, int64_t ldG,
                                                const double *__restrict__ sm_part, int nchunk,
-                                               double *__restrict__ part, int nchq)
+                                               double *__restrict__ part, int nchd)
 {
     constexpr int NV = 2 * M + 5;
     const int j = blockIdx.y;
@@ -525,7 +545,7 @@ __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds,
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    double *out = part + ((int64_t)j * nchq + blockIdx.x) * NV;
+    double *out = part + ((int64_t)j * nchd + blockIdx.x) * NV;
     if (c.mode == MODE_IDLE || c.gt < 0 || c.gt > 1) {
         for (int k = tid; k < NV; k += 256) out[k] = 0.0;
         return;
@@ -550,8 +570,10 @@ __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds,
     __syncthreads();
     const bool cur = c.mode != MODE_INIT;  // a current gradient and a direction exist
     const uint32_t live = cur ? c.live : 0u;
-    const int64_t idx = (int64_t)blockIdx.x * 256 + tid;
-    if (idx < qB) {
+#pragma unroll
+    for (int k = 0; k < kDotsPerThread; ++k) {
+        const int64_t idx = (int64_t)blockIdx.x * (256 * kDotsPerThread) + k * 256 + tid;
+        if (idx >= qB) break;
         const int s = (int)(idx / Bp), b = (int)(idx - (int64_t)s * Bp);
         const int64_t e0 = idx, e1 = idx + qB;
         const bool seq = b < B;
@@ -569,14 +591,14 @@ __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds,
         }
         const double kg = seq ? z - (A.own ? sT[s * 8 + y] : 0.0) + A.lamJ2 * xi : 0.0;
         gt[e1] = kg;
-        acc[2 * M] = gc * kg;
-        acc[2 * M + 4] = gc * kg;
-        acc[2 * M + 1] = gc * gki;
-        acc[2 * M + 2] = gc * di;
+        acc[2 * M] += gc * kg;
+        acc[2 * M + 4] += gc * kg;
+        acc[2 * M + 1] += gc * gki;
+        acc[2 * M + 2] += gc * di;
 #pragma unroll
         for (int p = 0; p < M; ++p) {
-            acc[p] = gc * sp[p];
-            acc[M + p] = gc * yp[p];
+            acc[p] += gc * sp[p];
+            acc[M + p] += gc * yp[p];
         }
     }
     if (blockIdx.x == 0 && tid < q) {
@@ -603,30 +625,34 @@ __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds,
 }
 
 // Result rows (ResLayout): the products, {c_t . K_r c_t, |h_t|^2, d . g_k} and the data
-// term.  The entry of max |g| holds |g_t|^2 (the host takes the square roots).
-__global__ void k_greduce(int m, const double *__restrict__ upd_part, const double *__restrict__ dot_part,
-                          const double *__restrict__ sm_part, int nchunk, int nchq, double *__restrict__ res)
+// term.  The entry of max |g| holds |g_t|^2 (the host takes the square roots).  One block
+// of 256 per command: eight threads per entry sum its chunk partials.
+__global__ __launch_bounds__(256) void k_greduce(int m, const double *__restrict__ upd_part,
+                                                 const double *__restrict__ dot_part,
+                                                 const double *__restrict__ sm_part, int nchunk, int nchq, int nchd,
+                                                 double *__restrict__ res)
 {
     const int i = blockIdx.x;
-    const int t = threadIdx.x;
     const int ND = 2 * m + 5, W = 2 * m + 9;
     double *out = res + (int64_t)i * W;
-    if (t < ND) {
+    const int sub = threadIdx.x & 7;
+    for (int it = 0; it < (W + 31) / 32; ++it) {  // the same trip count for every lane
+        const int t = it * 32 + (threadIdx.x >> 3);
         double a = 0.0;
-        for (int ch = 0; ch < nchq; ++ch) a += dot_part[((int64_t)i * nchq + ch) * ND + t];
-        out[t] = a;
-    } else if (t < ND + 3) {
-        const int u = t - ND;
-        double a = 0.0;
-        for (int ch = 0; ch <= nchq; ++ch) a += upd_part[((int64_t)i * (nchq + 1) + ch) * 3 + u];
-        out[t] = a;
-    } else if (t == W - 1) {
-        double a = 0.0;
-        for (int ch = 0; ch < nchunk; ++ch) a += sm_part[((int64_t)i * nchunk + ch) * kSm + 64];
-        out[t] = a;
+        if (t < ND) {
+            for (int ch = sub; ch < nchd; ch += 8) a += dot_part[((int64_t)i * nchd + ch) * ND + t];
+        } else if (t < ND + 3) {
+            for (int ch = sub; ch <= nchq; ch += 8) a += upd_part[((int64_t)i * (nchq + 1) + ch) * 3 + (t - ND)];
+        } else if (t == W - 1) {
+            for (int ch = sub; ch < nchunk; ch += 8) a += sm_part[((int64_t)i * nchunk + ch) * kSm + 64];
+        }
+        a += __shfl_xor(a, 1, 64);
+        a += __shfl_xor(a, 2, 64);
+        a += __shfl_xor(a, 4, 64);
+        if (sub == 0 && t < W) out[t] = a;
     }
     __syncthreads();
-    if (t == 0) out[2 * m + 3] = out[2 * m];
+    if (threadIdx.x == 0) out[2 * m + 3] = out[2 * m];
 }
 
 // Coefficients of finished loci as the A operand of the coupling contraction J = X_r^T c:
@@ -705,22 +731,23 @@ void step(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *a
                        nchunk);
 }
 
+int dots_chunks(int64_t qB) { return (int)((qB + 256 * kDotsPerThread - 1) / (256 * kDotsPerThread)); }
+
 void dots(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
-          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, int nchunk, double *dot_part, int nchq)
+          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, int nchunk, double *dot_part, int nchd)
 {
 #define SDCA_GDOTS(MM)                                                                                            \
-    hipLaunchKernelGGL((k_gdots<MM>), dim3(nchq, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Zp, ldG,   \
-                       sm_part, nchunk, dot_part, nchq)
+    hipLaunchKernelGGL((k_gdots<MM>), dim3(nchd, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Zp, ldG,   \
+                       sm_part, nchunk, dot_part, nchd)
     SDCA_GRAM_DISPATCH(m, SDCA_GDOTS)
 #undef SDCA_GDOTS
 }
 
 void reduce(hipStream_t st, int m, int ncmd, const double *upd_part, const double *dot_part, const double *sm_part,
-            int nchunk, int nchq, double *res)
+            int nchunk, int nchq, int nchd, double *res)
 {
-    const int W = 2 * m + 9;
-    hipLaunchKernelGGL(k_greduce, dim3(ncmd), dim3((W + 63) / 64 * 64), 0, st, m, upd_part, dot_part, sm_part,
-                       nchunk, nchq, res);
+    hipLaunchKernelGGL(k_greduce, dim3(ncmd), dim3(256), 0, st, m, upd_part, dot_part, sm_part, nchunk, nchq, nchd,
+                       res);
 }
 
 void split(hipStream_t st, const int64_t *src_off, const int32_t *rows, const int32_t *alns, int n, int nchunk,

@@ -289,11 +289,13 @@ void step(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *a
 // of 16), n < N stored; Bm rows up to round_up(N, 128) readable.
 void gemm(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_t ldb, double *C, int64_t ldc, int M,
           int N, int Kd);
-// Gradient images from Z = K G and the products: dot_part [cmd][nchq][2m + 5].
+// Gradient images from Z = K G and the products: dot_part [cmd][nchd][2m + 5], nchd =
+// dots_chunks(q Bp) blocks per command.
+int dots_chunks(int64_t qB);
 void dots(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
-          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, int nchunk, double *dot_part, int nchq);
+          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, int nchunk, double *dot_part, int nchd);
 void reduce(hipStream_t st, int m, int ncmd, const double *upd_part, const double *dot_part, const double *sm_part,
-            int nchunk, int nchq, double *res);
+            int nchunk, int nchq, int nchd, double *res);
 // Coefficients of n finished loci (V + src_off[u]) as contraction operands (rows[u]).
 void split(hipStream_t st, const int64_t *src_off, const int32_t *rows, const int32_t *alns, int n, int nchunk,
            const GramAln *aln, const double *V, int64_t ldm, float *G32, uint16_t *Ghi, uint16_t *Glo,
@@ -304,7 +306,7 @@ void split(hipStream_t st, const int64_t *src_off, const int32_t *rows, const in
 // history m and the largest member alignment (q gq, padded depth gBp, gL loci, gKxp columns).
 struct GramWs {
     int slots = 0, m = 0, gq = 0, gBp = 0, gL = 0, nmem = 0;
-    int nchunk = 0, nchq = 0;  // blocks of 256 sequences, of 256 coefficients (q x Bp) per command
+    int nchunk = 0, nchq = 0, nchd = 0;  // blocks per command: of 256 sequences, of 256 and of 1024 coefficients
     // slot groups whose rounds run on their own streams (c->gstream): gs slots each, colcap
     // columns of the packed GEMM operand each; a group's command lists, partials, result
     // rows and columns start at its first slot (or column) times the per-slot (per-column) size
