@@ -2,27 +2,30 @@
 
 Workload: B = 3000 sequences x N = 94,000 loci (seeded synthetic genome alignment, 60
 clades, 2 % mutation, 2000 planted pairs), q = 5 envelope, L-BFGS m = 10.  A step is a
-batch of loci solved to convergence with 256 loci in flight on the GPU (finished slots
-are refilled).  Timing starts with the alignment, weights and lambda resident on the
-device and ends when the last score row is in host memory.
+batch of loci solved to convergence with the slots in flight refilled as loci finish.
+Timing starts with the alignment, weights and lambda resident on the device (and, for the
+Gram formulation, the alignment's identical-locus counts built once: reported) and ends
+when the last score row is in host memory.
 
-Convergence (headline): SuperDCA's rule ||g||_inf < 1e-3 plus the coupling threshold
-||g_J||_inf < 3e-6, at which the coupling scores lie within 1e-4 relative of the FP64
-optimum; the bench measures that error itself on the 64 loci of the committed C4
-fixture, solved with the headline settings.  SuperDCA's rule alone is reported beside it.
+Headline (value): the Gram formulation -- each locus' couplings held as J = X_r^T c in
+the span of the data, FP64 evaluations with one B x B x q GEMM per round, cold start --
+stopping at ||g||_2 < 1e-3 and ||g_J||_2 < 1e-5 (2-norms: bounds on the infinity norms of
+SuperDCA's rule from above).  Its score error against the FP64 oracle is measured on the
+64 loci of the committed C4 fixture, solved with the headline settings.
 
-Legs on the same GPU (warm and cold solve the same loci, so their final objectives are
-compared locus by locus):
-  warm   split-BF16 contractions, warm starts from the network (trained beforehand on
-         cold-start solutions of the C1 and C2 workloads, untimed) for the top-2048 MI
-         partners of every locus -- the headline value
-  cold   split-BF16, cold start
-  fp32   the plain path: FP32 MFMA contractions, cold start, the same convergence rule
-  rule   split-BF16 warm and cold at SuperDCA's rule alone
+Secondary legs on the same GPU:
+  gram-rule  the Gram formulation at ||g||_2 < 1e-3 alone
+  primal     split-BF16 contractions over the K_x one-hot columns, ||g||_inf < 1e-3 and
+             ||g_J||_inf < 3e-6, cold and warm-started from the network (trained
+             beforehand on solutions of the C1 and C2 workloads, untimed) for the top-2048
+             MI partners of every locus, on the same loci
+  fp32       the plain path: primal FP32 MFMA contractions, cold start, the same rule
 The MI partner table of the alignment is built once (reported, not in the loci/s).
 
-Prints one JSON line: value (warm loci/s), the roofline of the dominant kernel of the
-headline leg (HIP events on the library's stream), and a CPU baseline: the FP64 oracle
+Prints one JSON line: value (headline loci/s), the roofline of the headline's dominant
+kernel (HIP events on the library's streams over the timed region; with two slot groups
+on two streams a kernel's launch time includes the other group's kernels sharing the GPU,
+so single-group rates are reported beside it), and a CPU baseline: the FP64 oracle
 (block-compressed, one locus per thread) on a bounded sample of the same workload at
 SuperDCA's rule, on the box's CPU share and on one thread.
 """
@@ -39,7 +42,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"bf16_tflops": 2500.0, "fp32_tflops": 157.3, "hbm_gbs": 8000.0}
+PEAK = {"bf16_tflops": 2500.0, "fp32_tflops": 157.3, "fp64_tflops": 78.6, "hbm_gbs": 8000.0}
 METRIC = "loci/s to converged plmDCA at B=3000,N=94k; warm vs cold start, same GPU"
 
 
@@ -69,61 +72,61 @@ def run_steps(ctx, order, pos, per_step, steps, label, solve, fvals=None):
     return converged / elapsed, np.concatenate(iters), np.concatenate(evals), elapsed, pos
 
 
-# HBM bytes moved per algorithmic byte, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes over
-# the split-precision C4 solve (scaled by a plain device copy: fetch x2.000, write x1.000);
-# profiles/c4_q5_solve_kernels.md holds the commands and the per-dispatch table.
+# HBM bytes moved per algorithmic byte of a kernel, from rocprofv3 FETCH_SIZE and WRITE_SIZE
+# passes (scaled by a plain device copy) over the C4 solve; the profiles hold the commands
+# and the per-dispatch tables.
 HBM_PER_ALGORITHMIC_BYTE = {"k_update": 13.34 / 13.10, "k_dots": 10.51 / 10.50}
+# HBM bytes (FETCH_SIZE + WRITE_SIZE) per algorithmic flop of the Gram GEMM k_dgemm at C4
+GRAM_TRAFFIC_PER_FLOP = None
 
 
-def roofline(tm, split):
-    # one launch of each kernel per round (the energy and gradient launches are counted
-    # separately by the library)
-    groups = {
-        "k_gemm_nt<EpiGrad>": (tm["gemm_grad_ms"], tm["flops_grad"], "mfma", tm["n_grad_launches"]),
-        "k_gemm_nt<EpiEnergy>": (tm["gemm_energy_ms"], tm["flops_energy"], "mfma", tm["n_energy_launches"]),
-        "k_update": (tm["update_ms"], tm["bytes_update"], "hbm", tm["n_evals"]),
-        "k_dots": (tm["dots_ms"], tm["bytes_dots"], "hbm", tm["n_evals"]),
+def kernel_groups(tm, gram, split):
+    """(device ms, algorithmic work, bound, launches, peak) of each dominant kernel of a leg.
+    Gram: k_dgemm is the B x B x q FP64 GEMM (one launch per round and group), the step pass
+    (k_ghstep, k_gstep, k_gsoftmax) and the products pass (k_gdots, k_greduce) move the
+    slot vectors.  Primal: the two split-BF16 (or FP32) contractions and the vector passes."""
+    if gram:
+        return {
+            "k_dgemm": (tm["gemm_grad_ms"], tm["flops_grad"], "mfma", tm["n_grad_launches"], PEAK["fp64_tflops"]),
+            "k_gstep": (tm["update_ms"], tm["bytes_update"], "hbm", tm["n_evals"], PEAK["hbm_gbs"]),
+            "k_gdots": (tm["dots_ms"], tm["bytes_dots"], "hbm", tm["n_evals"], PEAK["hbm_gbs"]),
+        }
+    mp = PEAK["bf16_tflops"] if split else PEAK["fp32_tflops"]
+    return {
+        "k_gemm_nt<EpiGrad>": (tm["gemm_grad_ms"], tm["flops_grad"], "mfma", tm["n_grad_launches"], mp),
+        "k_gemm_nt<EpiEnergy>": (tm["gemm_energy_ms"], tm["flops_energy"], "mfma", tm["n_energy_launches"], mp),
+        "k_update": (tm["update_ms"], tm["bytes_update"], "hbm", tm["n_evals"], PEAK["hbm_gbs"]),
+        "k_dots": (tm["dots_ms"], tm["bytes_dots"], "hbm", tm["n_evals"], PEAK["hbm_gbs"]),
     }
+
+
+def roofline(tm, gram, split, traffic_ratio=None):
+    groups = kernel_groups(tm, gram, split)
     dom = max(groups, key=lambda k: groups[k][0])
-    ms, work, bound, n = groups[dom]
-    if bound == "mfma":
-        peak = PEAK["bf16_tflops"] if split else PEAK["fp32_tflops"]
-        achieved, unit = work / (ms * 1e-3) / 1e12, "TFLOP/s"
-    else:
-        peak = PEAK["hbm_gbs"]
-        achieved, unit = work / (ms * 1e-3) / 1e9, "GB/s"
-    rates = {k: round(v[1] / (v[0] * 1e-3) / (1e12 if v[2] == "mfma" else 1e9), 2) for k, v in groups.items() if v[0]}
-    ratio = HBM_PER_ALGORITHMIC_BYTE.get(dom) if split else None
-    traffic = round(ratio * work / max(n, 1)) if ratio else None
-    return ({"bound": bound, "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-             "frac": round(achieved / peak, 5), "traffic": traffic,
-             "traffic_basis": (f"HBM bytes per launch: algorithmic bytes x {ratio:.4f}, the measured "
-                               "(FETCH_SIZE + WRITE_SIZE) / algorithmic ratio of this kernel"
-                               if ratio else None),
-             "launches": int(n),
-             "avg_launch_us": round(ms * 1e3 / max(n, 1), 1),
-             "work_per_launch": round(work / max(n, 1) / (1e12 if bound == "mfma" else 1e9), 4)}, rates)
+    ms, work, bound, n, peak = groups[dom]
+    scale, unit = (1e12, "TFLOP/s") if bound == "mfma" else (1e9, "GB/s")
+    achieved = work / (ms * 1e-3) / scale
+    ratio = traffic_ratio if traffic_ratio is not None else (HBM_PER_ALGORITHMIC_BYTE.get(dom) if not gram else None)
+    traffic = round(ratio * work / max(n, 1)) if (ratio and bound == "hbm") else None
+    return {"bound": bound, "kernel": dom, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+            "frac": round(achieved / peak, 5), "traffic": traffic,
+            "traffic_basis": (f"HBM bytes per launch: algorithmic bytes x {ratio:.4f}, the measured "
+                              "(FETCH_SIZE + WRITE_SIZE) / algorithmic ratio of this kernel" if traffic else None),
+            "launches": int(n), "avg_launch_us": round(ms * 1e3 / max(n, 1), 1),
+            "work_per_launch": round(work / max(n, 1) / scale, 4)}
 
 
-def phase_fractions(tm, split):
-    """Roofline fraction of each of the four dominant kernels (algorithmic work over its
-    device time, against the MFMA or HBM peak) and their device-time-weighted mean."""
-    groups = [("k_gemm_nt<EpiEnergy>", "gemm_energy_ms", "flops_energy", "mfma"),
-              ("k_gemm_nt<EpiGrad>", "gemm_grad_ms", "flops_grad", "mfma"),
-              ("k_update", "update_ms", "bytes_update", "hbm"),
-              ("k_dots", "dots_ms", "bytes_dots", "hbm")]
+def phase_fractions(tm, gram, split):
+    """Roofline fraction of each dominant kernel (algorithmic work over its device time,
+    against the MFMA or HBM peak) and their device-time-weighted mean."""
     out, weighted, total = {}, 0.0, 0.0
-    for name, ms_key, work_key, bound in groups:
-        ms, work = tm[ms_key], tm[work_key]
+    for name, (ms, work, bound, n, peak) in kernel_groups(tm, gram, split).items():
         if not ms:
             continue
-        if bound == "mfma":
-            peak, scale, unit = (PEAK["bf16_tflops"] if split else PEAK["fp32_tflops"]), 1e12, "TFLOP/s"
-        else:
-            peak, scale, unit = PEAK["hbm_gbs"], 1e9, "GB/s"
+        scale, unit = (1e12, "TFLOP/s") if bound == "mfma" else (1e9, "GB/s")
         rate = work / (ms * 1e-3) / scale
         out[name] = {"ms": round(ms, 1), "rate": round(rate, 2), "unit": unit, "peak": peak,
-                     "frac": round(rate / peak, 4)}
+                     "frac": round(rate / peak, 4), "launches": int(n)}
         weighted += ms * rate / peak
         total += ms
     return out, (round(weighted / total, 4) if total else None)
@@ -195,24 +198,28 @@ def main() -> int:
     ap.add_argument("--workload", default="C4", choices=["C1", "C2", "C4"])
     ap.add_argument("--q", type=int, default=0,
                     help="state envelope (softmax width): default 5 ({A,C,G,T,-}) at C4, 4 (SuperDCA's) otherwise")
-    ap.add_argument("--loci-per-step", type=int, default=0, help="default: 2048 at C4, 1024 otherwise")
-    ap.add_argument("--slots", type=int, default=0, help="loci in flight (default 256 at C4, 128 otherwise)")
+    ap.add_argument("--loci-per-step", type=int, default=0, help="default: 16384 at C4, 1024 otherwise")
+    ap.add_argument("--slots", type=int, default=0, help="headline loci in flight (default 4096 at C4, 512 otherwise)")
     ap.add_argument("--history", type=int, default=10)
-    ap.add_argument("--groups", type=int, default=1, help="slot groups pipelined on two streams (1 or 2)")
-    ap.add_argument("--j-threshold", type=float, default=3e-6,
-                    help="coupling-gradient threshold of the headline convergence rule (0: SuperDCA's rule alone)")
-    ap.add_argument("--fine-factor", type=int, default=0,
-                    help="split precision: fine level below fine_factor * j_threshold (0: none)")
-    ap.add_argument("--partners", type=int, default=2048, help="warm-started partners per locus")
+    ap.add_argument("--groups", type=int, default=0,
+                    help="headline slot groups: 1, or 2 pipelined on two streams (0: the library's default, 2)")
+    ap.add_argument("--j-threshold", type=float, default=1e-5,
+                    help="headline coupling threshold on ||g_J||_2 (0: ||g||_2 < 1e-3 alone)")
+    ap.add_argument("--rule-loci", type=int, default=4096,
+                    help="loci of the Gram leg at ||g||_2 < 1e-3 alone (0: skip)")
+    ap.add_argument("--single-group-loci", type=int, default=2048,
+                    help="Gram loci solved with one slot group after the timed legs, for the kernels' rates without "
+                         "the other group sharing the GPU (0: skip)")
+    ap.add_argument("--primal-loci", type=int, default=1024, help="loci of the primal split-BF16 legs (0: skip)")
+    ap.add_argument("--primal-slots", type=int, default=256)
+    ap.add_argument("--primal-j-threshold", type=float, default=3e-6,
+                    help="primal legs' coupling threshold on ||g_J||_inf")
+    ap.add_argument("--partners", type=int, default=2048, help="warm-started partners per locus (primal)")
     ap.add_argument("--warm-mode", default="direction", choices=list(WARM_MODES),
                     help="direction: network blocks as the first search direction from zero; direction-fields: "
                          "the same from the J=0 field optimum; point: start at fields + blocks")
-    ap.add_argument("--fp32-loci", type=int, default=1024, help="loci of the FP32 cold leg (0: skip)")
-    ap.add_argument("--fp32-slots", type=int, default=0,
-                    help="loci in flight in the FP32 leg (default: as --slots)")
-    ap.add_argument("--rule-loci", type=int, default=2048,
-                    help="loci of each leg at SuperDCA's rule alone (0: skip)")
-    ap.add_argument("--no-warm", action="store_true", help="skip the warm legs (value = cold split-BF16)")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm-started primal leg")
+    ap.add_argument("--fp32-loci", type=int, default=512, help="loci of the primal FP32 cold leg (0: skip)")
     ap.add_argument("--cpu-loci", type=int, default=0, help="CPU baseline sample (default: 2 loci per thread)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="default: the usable host cores, at most 16")
     ap.add_argument("--no-cpu", action="store_true")
@@ -229,9 +236,8 @@ def main() -> int:
 
     wl = args.workload
     q = args.q or (5 if wl == "C4" else 4)
-    per_step = args.loci_per_step or (2048 if wl == "C4" else 1024)
-    slots = args.slots or (256 if wl == "C4" else 128)
-    slots32 = args.fp32_slots or slots
+    per_step = args.loci_per_step or (16384 if wl == "C4" else 1024)
+    slots = args.slots or (4096 if wl == "C4" else 512)
     jthr = args.j_threshold
 
     t = time.time()
@@ -240,111 +246,131 @@ def main() -> int:
     partners = min(args.partners, L - 1)
     log(f"[bench] workload {wl}: B={B} L={L} generated in {time.time() - t:.1f}s")
     order = np.random.default_rng(1234).permutation(L).astype(np.int32)
-    need = (2 * args.warmup + 2 * args.steps) * per_step + args.fp32_loci + 3 * args.rule_loci + 2 * max(slots, slots32)
+    need = (args.warmup * per_step // 4 + args.steps * per_step + args.rule_loci + args.single_group_loci +
+            args.primal_loci + args.fp32_loci + 3 * args.primal_slots)
     if need > L:
         order = np.resize(order, need)
     pos = 0
     fx_path = os.path.join(ROOT, "tests", "golden", "c4_rows64.npz")
     fx = np.load(fx_path) if wl == "C4" and q == 5 and os.path.exists(fx_path) else None
 
-    net = None
-    if not args.no_warm:
-        t = time.time()
-        net = train_network(q)
-        log(f"[bench] network trained on {net[2]} pairs in {time.time() - t:.1f}s, loss {net[1][0]:.3e} -> "
-            f"{net[1][-1]:.3e}")
-
-    rule = dict(grad_threshold=1e-3)
-    accurate = dict(grad_threshold=1e-3, j_threshold=jthr, fine_factor=args.fine_factor)
-    ctx = sd.Context(states, q=q, precision=sd.PREC_SPLIT_BF16, batch_loci=slots, history=args.history,
-                     pipeline_groups=args.groups, **accurate)
+    # ---- headline: the Gram formulation, cold start
+    gram = dict(formulation=sd.FORM_GRAM, precision=sd.PREC_SPLIT_BF16, history=args.history, grad_threshold=1e-3)
+    ctx = sd.Context(states, q=q, batch_loci=slots, pipeline_groups=args.groups, j_threshold=jthr, **gram)
     t = time.time()
     w, _ = ctx.reweight(0.9)
     reweight_s = time.time() - t
     beff, lh, lJ = ctx.set_weights(w)
     log(f"[bench] K_x={ctx.num_columns}, reweighting {reweight_s * 1e3:.1f} ms, B_eff={beff:.2f}")
-
-    # cold split-BF16
+    t = time.time()
+    ctx.solve(order[pos:pos + 8])  # the identical-locus counts and the workspace
+    first_s = time.time() - t
+    pos += 8
     for k in range(args.warmup):
-        _, wst, _, _ = ctx.solve(order[pos:pos + per_step])
+        _, wst, _, _ = ctx.solve(order[pos:pos + per_step // 4])
         assert np.all(wst["nfeval"] > 1), "warmup loci stopped at their start point"
-        pos += per_step
-    timed_from = pos  # the warm leg solves the same loci
-    cold_f = []
-    cold_v, cold_it, cold_ev, cold_s, pos = run_steps(ctx, order, pos, per_step, args.steps, "cold-split", ctx.solve,
-                                                      cold_f)
-    cold_tm = ctx.timing()
-    gemm_cfg = ctx.gemm_config()
-    log(f"[bench] split-BF16 GEMM configuration {gemm_cfg}")
-
-    warm = None
-    solve_head = ctx.solve
-    if net is not None:
-        ctx.mlp_set_params(net[0], 128)
-        t = time.time()
-        ctx.mi_topk(partners)
-        mi_s = time.time() - t
-        log(f"[bench] MI top-{partners} partner table in {mi_s:.2f}s")
-        mode = WARM_MODES[args.warm_mode](sd)
-        solve_w = lambda loci: ctx.solve_mlp(loci, partners, mode)
-        for k in range(args.warmup):
-            solve_w(order[pos:pos + per_step // 4])
-            pos += per_step // 4
-        warm_f = []
-        warm_v, warm_it, warm_ev, warm_s, _ = run_steps(ctx, order, timed_from, per_step, args.steps, "warm-split",
-                                                        solve_w, warm_f)
-        fc, fw = np.concatenate(cold_f), np.concatenate(warm_f)
-        frel = np.abs(fw - fc) / np.abs(fc)
-        warm = {"loci_s": warm_v, "iter": warm_it, "evals": warm_ev, "tm": ctx.timing(), "mi_s": mi_s,
-                "f_rel": {"median": float(np.median(frel)), "max": float(frel.max())}}
-        solve_head = solve_w
-    # score error of the headline settings: the fixture's loci, solved like the timed ones
+        pos += per_step // 4
+    head_v, head_it, head_ev, head_s, pos = run_steps(ctx, order, pos, per_step, args.steps, "gram", ctx.solve)
+    head_tm = ctx.timing()
     accuracy = None
     if fx is not None:
-        rows_fx, st_fx, _, _ = solve_head(fx["loci"])
+        rows_fx, st_fx, _, _ = ctx.solve(fx["loci"])
         accuracy = fixture_error(rows_fx, fx)
         accuracy["converged"] = int((st_fx["status"] == sd.ST_CONVERGED).sum())
         log(f"[bench] headline score error on the fixture: {accuracy}")
     ctx.close()
 
-    # SuperDCA's rule alone (secondary figures)
-    rule_legs = None
+    # ---- the same with one slot group: the kernels' rates with the GPU to themselves
+    single = None
+    if args.single_group_loci > 0:
+        with sd.Context(states, q=q, batch_loci=min(slots, args.single_group_loci), pipeline_groups=1,
+                        j_threshold=jthr, **gram) as c1:
+            c1.set_weights(w)
+            c1.solve(order[pos:pos + 8])
+            pos += 8
+            v1, _, _, _, pos = run_steps(c1, order, pos, args.single_group_loci, 1, "gram-one-group", c1.solve)
+            tm1 = c1.timing()
+        ph1, ph1w = phase_fractions(tm1, True, True)
+        single = {"loci_s": round(v1, 3), "slots": min(slots, args.single_group_loci), "phases": ph1,
+                  "time_weighted_frac": ph1w, "device_ms": device_ms(tm1)}
+
+    # ---- the Gram formulation at ||g||_2 < 1e-3 alone
+    gram_rule = None
     if args.rule_loci > 0:
-        ctxr = sd.Context(states, q=q, precision=sd.PREC_SPLIT_BF16, batch_loci=slots, history=args.history,
-                          pipeline_groups=args.groups, **rule)
-        ctxr.set_weights(w)
-        ctxr.solve(order[pos:pos + slots])  # allocation and first launches
-        pos += slots
-        rule_from = pos
-        rc_v, rc_it, _, _, pos = run_steps(ctxr, order, pos, args.rule_loci, 1, "cold-split-rule", ctxr.solve)
-        rule_legs = {"cold_split_loci_s": round(rc_v, 3), "cold_split_iter_median": float(np.median(rc_it))}
-        if net is not None:
-            ctxr.mlp_set_params(net[0], 128)
-            ctxr.mi_topk(partners)
-            mode = WARM_MODES[args.warm_mode](sd)
-            rw_v, rw_it, _, _, _ = run_steps(ctxr, order, rule_from, args.rule_loci, 1, "warm-split-rule",
-                                             lambda loci: ctxr.solve_mlp(loci, partners, mode))
-            rule_legs.update({"warm_split_loci_s": round(rw_v, 3), "warm_split_iter_median": float(np.median(rw_it)),
-                              "warm_over_cold_split": round(rw_v / rc_v, 3)})
-        if fx is not None:
-            rows_fx, _, _, _ = ctxr.solve(fx["loci"])
-            rule_legs["score_error"] = fixture_error(rows_fx, fx)
-        ctxr.close()
+        with sd.Context(states, q=q, batch_loci=min(slots, args.rule_loci), pipeline_groups=args.groups,
+                        j_threshold=0.0, **gram) as cr:
+            cr.set_weights(w)
+            cr.solve(order[pos:pos + 8])
+            pos += 8
+            rv, rit, rev, _, pos = run_steps(cr, order, pos, args.rule_loci, 1, "gram-rule", cr.solve)
+            gram_rule = {"loci_s": round(rv, 3), "evals_median": float(np.median(rev)),
+                         "convergence": "||g||_2<1e-3"}
+            if fx is not None:
+                rows_fx, _, _, _ = cr.solve(fx["loci"])
+                gram_rule["score_error"] = fixture_error(rows_fx, fx)
+
+    # ---- the primal formulation: split-BF16, cold and warm-started on the same loci
+    primal = None
+    pj = args.primal_j_threshold
+    prim = dict(formulation=sd.FORM_PRIMAL, precision=sd.PREC_SPLIT_BF16, history=args.history,
+                grad_threshold=1e-3, j_threshold=pj, fine_factor=0, batch_loci=args.primal_slots)
+    if args.primal_loci > 0:
+        net = None
+        if not args.no_warm:
+            t = time.time()
+            net = train_network(q)
+            log(f"[bench] network trained on {net[2]} pairs in {time.time() - t:.1f}s, loss {net[1][0]:.3e} -> "
+                f"{net[1][-1]:.3e}")
+        with sd.Context(states, q=q, **prim) as cp:
+            cp.set_weights(w)
+            cp.solve(order[pos:pos + args.primal_slots])  # allocation, autotuning, first launches
+            pos += args.primal_slots
+            from_ = pos
+            cold_f = []
+            pc_v, pc_it, pc_ev, _, pos = run_steps(cp, order, pos, args.primal_loci, 1, "primal-cold", cp.solve, cold_f)
+            pc_tm = cp.timing()
+            primal = {"convergence": f"||g||_inf<1e-3 and ||g_J||_inf<{pj:g}", "slots": args.primal_slots,
+                      "cold_split_loci_s": round(pc_v, 3), "iter_median_cold": float(np.median(pc_it)),
+                      "evals_median_cold": float(np.median(pc_ev)), "device_ms_cold": device_ms(pc_tm),
+                      "gemm_config": cp.gemm_config()}
+            if fx is not None:
+                rows_fx, _, _, _ = cp.solve(fx["loci"])
+                primal["score_error_cold"] = fixture_error(rows_fx, fx)
+            if net is not None:
+                cp.mlp_set_params(net[0], 128)
+                t = time.time()
+                cp.mi_topk(partners)
+                mi_s = time.time() - t
+                mode = WARM_MODES[args.warm_mode](sd)
+                solve_w = lambda loci: cp.solve_mlp(loci, partners, mode)
+                solve_w(order[pos:pos + args.primal_slots // 4])
+                pos += args.primal_slots // 4
+                warm_f = []
+                pw_v, pw_it, pw_ev, _, _ = run_steps(cp, order, from_, args.primal_loci, 1, "primal-warm", solve_w,
+                                                     warm_f)
+                fc, fw = np.concatenate(cold_f), np.concatenate(warm_f)
+                frel = np.abs(fw - fc) / np.abs(fc)
+                primal.update({"warm_split_loci_s": round(pw_v, 3), "warm_over_cold_split": round(pw_v / pc_v, 3),
+                               "iter_median_warm": float(np.median(pw_it)),
+                               "evals_median_warm": float(np.median(pw_ev)),
+                               "f_warm_vs_cold_rel": {"median": float(np.median(frel)), "max": float(frel.max())},
+                               "partners": partners, "warm_mode": args.warm_mode, "mi_table_s": round(mi_s, 2)})
 
     fp32 = None
     if args.fp32_loci > 0:
-        ctx32 = sd.Context(states, q=q, precision=sd.PREC_FP32, batch_loci=slots32, history=args.history,
-                           pipeline_groups=args.groups, **accurate)
-        ctx32.set_weights(w)
-        ctx32.solve(order[pos:pos + slots32 // 4])  # allocation and first launches
-        pos += slots32 // 4
-        fv, fit, fev, fs, pos = run_steps(ctx32, order, pos, args.fp32_loci, 1, "cold-fp32", ctx32.solve)
-        fp32 = {"loci_s": fv, "iter": fit, "tm": ctx32.timing(), "gemm": ctx32.gemm_config()}
-        ctx32.close()
+        with sd.Context(states, q=q, **{**prim, "precision": sd.PREC_FP32}) as c32:
+            c32.set_weights(w)
+            c32.solve(order[pos:pos + args.primal_slots // 4])
+            pos += args.primal_slots // 4
+            fv, fit, _, _, pos = run_steps(c32, order, pos, args.fp32_loci, 1, "primal-fp32", c32.solve)
+            fp32 = {"loci_s": round(fv, 3), "iter_median": float(np.median(fit)), "device_ms": device_ms(c32.timing())}
 
-    head = warm if warm is not None else {"loci_s": cold_v, "iter": cold_it, "evals": cold_ev, "tm": cold_tm}
-    rl, rates = roofline(head["tm"], True)
-    phases, phases_weighted = phase_fractions(head["tm"], True)
+    rl = roofline(head_tm, True, True, traffic_ratio=GRAM_TRAFFIC_PER_FLOP)
+    if rl["bound"] == "mfma" and GRAM_TRAFFIC_PER_FLOP:
+        rl["traffic"] = round(GRAM_TRAFFIC_PER_FLOP * rl["work_per_launch"] * 1e12)
+        rl["traffic_basis"] = (f"HBM bytes per launch: {GRAM_TRAFFIC_PER_FLOP:.4f} bytes per algorithmic flop, the "
+                               "measured (FETCH_SIZE + WRITE_SIZE) per flop of this kernel")
+    phases, phases_weighted = phase_fractions(head_tm, True, True)
 
     cpu = None
     if not args.no_cpu:
@@ -372,55 +398,45 @@ def main() -> int:
                "evals_median": float(np.median(cst["nfeval"]))}
         log(f"[bench] cpu baseline {cpu}")
 
+    conv = f"||g||_2<1e-3 and ||g_J||_2<{jthr:g}" if jthr > 0 else "||g||_2<1e-3"
     detail = {
-        "convergence": f"||g||_inf<1e-3 and ||g_J||_inf<{jthr:g}" if jthr > 0 else "||g||_inf<1e-3",
-        "fine_factor": args.fine_factor,
+        "formulation": "gram",
+        "convergence": conv,
         "score_error_headline": accuracy,
-        "cold_split_bf16_loci_s": round(cold_v, 3),
-        "cold_fp32_loci_s": round(fp32["loci_s"], 3) if fp32 else None,
-        "fp32_slots": slots32 if fp32 else None,
-        "warm_split_bf16_loci_s": round(warm["loci_s"], 3) if warm else None,
-        "warm_over_cold_fp32": round(warm["loci_s"] / fp32["loci_s"], 3) if (warm and fp32) else None,
-        "warm_over_cold_split": round(warm["loci_s"] / cold_v, 3) if warm else None,
-        "superdca_rule": rule_legs,
-        "iter_median": {"cold_split": float(np.median(cold_it)),
-                        "warm_split": float(np.median(warm["iter"])) if warm else None,
-                        "cold_fp32": float(np.median(fp32["iter"])) if fp32 else None},
-        "iter_p95_headline": float(np.percentile(head["iter"], 95)),
-        "evals_median_headline": float(np.median(head["evals"])),
-        # same loci in both legs: final f of the warm start relative to the cold start's
-        "f_warm_vs_cold_rel": warm["f_rel"] if warm else None,
-        "partners": partners if warm else 0,
-        "warm_mode": args.warm_mode if warm else None,
-        "mi_table_s": round(warm["mi_s"], 2) if warm else None,
-        "reweight_ms": round(reweight_s * 1e3, 2), "B_eff": beff,
-        "device_ms_headline": device_ms(head["tm"]),
-        "kernel_rates_headline": rates,
+        "first_call_s": round(first_s, 3),
+        "evals_median_headline": float(np.median(head_ev)),
+        "iter_median_headline": float(np.median(head_it)),
+        "iter_p95_headline": float(np.percentile(head_it, 95)),
+        "rounds_headline": head_tm["n_evals"],
+        "device_ms_headline": device_ms(head_tm),
         "roofline_phases_headline": phases,
         "roofline_time_weighted_frac": phases_weighted,
-        "device_ms_cold_fp32": device_ms(fp32["tm"]) if fp32 else None,
-        "rounds_headline": head["tm"]["n_evals"],
-        "gemm_config_split": gemm_cfg,
-        "gemm_config_fp32": fp32["gemm"] if fp32 else None,
+        "single_group": single,
+        "gram_rule": gram_rule,
+        "primal_split": primal,
+        "cold_fp32_loci_s": fp32["loci_s"] if fp32 else None,
+        "cold_fp32": fp32,
+        "headline_over_cold_fp32": round(head_v / fp32["loci_s"], 3) if fp32 else None,
+        "headline_over_primal_cold_split": round(head_v / primal["cold_split_loci_s"], 3) if primal else None,
+        "reweight_ms": round(reweight_s * 1e3, 2), "B_eff": beff,
     }
-    conv = f"||g||_inf<1e-3 and ||g_J||_inf<{jthr:g}" if jthr > 0 else "||g||_inf<1e-3"
     out = {
         "metric": METRIC,
-        "value": round(head["loci_s"], 4),
+        "value": round(head_v, 4),
         "unit": "loci/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(per_step / head["loci_s"] * 1e3, 3),
+        "ms_per_step": round(per_step / head_v * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x2+fp32",
+        "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{wl}: B={B} sequences x N={L} loci, q_env={q}, {'warm' if warm else 'cold'} start, "
+        "config": {"workload": f"{wl}: B={B} sequences x N={L} loci, q_env={q}, cold start, Gram formulation (FP64), "
                                f"{per_step} loci per step ({slots} in flight), L-BFGS m={args.history}, {conv}",
-                   "B": B, "N": L, "q_env": q, "loci_per_step": per_step, "slots": slots,
-                   "precision": "split-bf16", "start": "warm" if warm else "cold", "convergence": conv,
+                   "B": B, "N": L, "q_env": q, "loci_per_step": per_step, "slots": slots, "formulation": "gram",
+                   "precision": "fp64", "start": "cold", "convergence": conv,
                    "parallelism": "loci batched on 1 GPU"},
         "roofline": rl,
         "cpu_baseline": cpu,

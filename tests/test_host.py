@@ -423,19 +423,29 @@ def test_bench_roofline_fields_from_timings():
               gemm_grad_ms=2.8 * r, flops_grad=1.6e15, n_grad_launches=r,
               update_ms=3.1 * r, bytes_update=15.6e9 * r, dots_ms=2.1 * r, bytes_dots=12.5e9 * r, n_evals=r,
               softmax_ms=0.1 * r, score_ms=0.05 * r)
-    rl, rates = bench.roofline(tm, True)
+    # the primal formulation (split BF16)
+    rl = bench.roofline(tm, False, True)
     assert rl["kernel"] == "k_update" and rl["bound"] == "hbm" and rl["unit"] == "GB/s"
     assert rl["achieved"] == pytest.approx(15.6e9 / 3.1e-3 / 1e9, rel=1e-4)
     assert rl["frac"] == pytest.approx(rl["achieved"] / 8000.0, rel=1e-4)
     assert rl["traffic"] == pytest.approx(15.6e9 * 13.34 / 13.10, rel=1e-6)
     assert rl["launches"] == r and rl["avg_launch_us"] == pytest.approx(3100.0)
-    assert rates["k_gemm_nt<EpiEnergy>"] == pytest.approx(1.6e15 / 2.2 / 1e12, rel=1e-3)
-    phases, weighted = bench.phase_fractions(tm, True)
+    phases, weighted = bench.phase_fractions(tm, False, True)
     assert set(phases) == {"k_gemm_nt<EpiEnergy>", "k_gemm_nt<EpiGrad>", "k_update", "k_dots"}
+    assert phases["k_gemm_nt<EpiEnergy>"]["rate"] == pytest.approx(1.6e15 / 2.2 / 1e12, rel=1e-3)
     assert phases["k_gemm_nt<EpiGrad>"]["frac"] == pytest.approx(1.6e15 / 2.8 / 1e12 / 2500.0, abs=1e-4)
     fr = [p["frac"] for p in phases.values()]
     ms = [p["ms"] for p in phases.values()]
     assert weighted == pytest.approx(sum(f * m for f, m in zip(fr, ms)) / sum(ms), abs=1e-3)
+    # the Gram formulation: the FP64 GEMM against the FP64 MFMA peak, traffic per flop
+    tg = dict(tm, gemm_energy_ms=0.0, flops_energy=0.0, n_energy_launches=0, gemm_grad_ms=4.0 * r,
+              flops_grad=2.0e11 * r)
+    rl = bench.roofline(tg, True, True, traffic_ratio=0.25)
+    assert rl["kernel"] == "k_dgemm" and rl["bound"] == "mfma" and rl["peak"] == 78.6
+    assert rl["achieved"] == pytest.approx(2.0e11 / 4.0e-3 / 1e12, rel=1e-4)
+    assert rl["frac"] == pytest.approx(50.0 / 78.6, rel=1e-4)
+    phases, _ = bench.phase_fractions(tg, True, True)
+    assert set(phases) == {"k_dgemm", "k_gstep", "k_gdots"}
 
 
 def test_every_hip_compile_disables_packed_fp32_and_targets_gfx950():
