@@ -1490,7 +1490,8 @@ int ensure_gram_ws(sdca_ctx *c, int nslots, int m)
     G->nmat = std::min(nslots, 128);
     G->ngroups = ng;
     G->gs = (nslots + ng - 1) / ng;
-    G->colcap = round_up((int64_t)G->gs * gq, 128) + 128;  // the GEMM reads whole 128-column tiles
+    // the GEMM reads whole tiles of kGemmCols columns
+    G->colcap = round_up((int64_t)G->gs * gq, sdca::gram::kGemmCols) + sdca::gram::kGemmCols;
     // each alignment's rows of a batch of finished loci start on a tile boundary
     G->mrows = (int)round_up((int64_t)G->nmat * gq, sdca::kTile) + std::min(nmem, G->nmat) * sdca::kTile;
     const int W = sdca::ResLayout{m}.width();

@@ -133,6 +133,7 @@ __global__ __launch_bounds__(256) void k_gram(const uint64_t *__restrict__ plane
 typedef double dx4 __attribute__((ext_vector_type(4)));
 // DPAD 48: 90 KB of LDS per block, so one block (four waves) per CU
 constexpr int DT = 128, DK = 16, DPAD = 48;
+static_assert(DT == gram::kGemmCols, "the packed operand is padded for whole column tiles");
 
 __device__ __forceinline__ void ld8d(const double *p, double (&r)[8])
 {

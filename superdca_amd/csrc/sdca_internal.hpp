@@ -286,7 +286,8 @@ void gram_matrix(hipStream_t st, const uint64_t *planes, int B, int q, int W, in
 void step(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
           int64_t nvs, double *Gp, int64_t ldG, double *upd_part, double *sm_part, int nchunk, int nchq);
 // C[n][m] = sum_k Bm[n][k] A[m][k], FP64 MFMA: m < M (multiple of 128), k < Kd (multiple
-// of 16), n < N stored; Bm rows up to round_up(N, 128) readable.
+// of 16), n < N stored; Bm rows up to round_up(N, kGemmCols) readable.
+constexpr int kGemmCols = 128;
 void gemm(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_t ldb, double *C, int64_t ldc, int M,
           int N, int Kd);
 // Gradient images from Z = K G and the products: dot_part [cmd][nchd][2m + 5], nchd =

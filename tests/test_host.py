@@ -440,8 +440,9 @@ def test_bench_roofline_fields_from_timings():
     # the Gram formulation: the FP64 GEMM against the FP64 MFMA peak, traffic per flop
     tg = dict(tm, gemm_energy_ms=0.0, flops_energy=0.0, n_energy_launches=0, gemm_grad_ms=4.0 * r,
               flops_grad=2.0e11 * r)
-    rl = bench.roofline(tg, True, True, traffic_ratio=0.25)
+    rl = bench.roofline(tg, True, True)
     assert rl["kernel"] == "k_dgemm" and rl["bound"] == "mfma" and rl["peak"] == 78.6
+    assert rl["traffic"] == pytest.approx(bench.HBM_BYTES_PER_FLOP["k_dgemm"] * 2.0e11, rel=1e-6)
     assert rl["achieved"] == pytest.approx(2.0e11 / 4.0e-3 / 1e12, rel=1e-4)
     assert rl["frac"] == pytest.approx(50.0 / 78.6, rel=1e-4)
     phases, _ = bench.phase_fractions(tg, True, True)
