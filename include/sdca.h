@@ -48,7 +48,8 @@ typedef struct sdca_ctx sdca_ctx;
 typedef struct sdca_opts {
     int32_t device;          /* HIP device ordinal (default 0) */
     int32_t precision;       /* SDCA_PREC_* (default SDCA_PREC_FP32) */
-    int32_t batch_loci;      /* loci solved concurrently on the GPU (default 128) */
+    int32_t batch_loci;      /* loci solved concurrently on the GPU (default 128; 0: 4096 for the
+                                Gram formulation, 128 for the primal one) */
     int32_t history;         /* L-BFGS history m: 1, 3, 5, 8, 10 (default) or 16 */
     int32_t max_iter;        /* iteration cap per locus (default 2000, plmDCA.hpp:219) */
     int32_t max_ls;          /* function evaluations per line search (default 20) */

@@ -102,6 +102,14 @@ inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
 bool valid_history(int m) { return m == 1 || m == 3 || m == 5 || m == 8 || m == 10 || m == 16; }
 
+// Loci in flight: opts.batch_loci, or with 0 the formulation's default (Gram slots are
+// small -- 6.1 MB at B = 3000, q = 5 -- so 4096 of them; primal slots hold q x K_x vectors).
+int slots_for(const sdca_opts &o, bool gram, int K)
+{
+    const int b = o.batch_loci > 0 ? o.batch_loci : (gram ? 4096 : 128);
+    return std::min(b, K);
+}
+
 }  // namespace
 
 void sdca_ctx::free_workspace()
@@ -1828,7 +1836,7 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
     int rc;
     for (sdca_ctx *A : c->members)
         if ((rc = ensure_gram_matrix(A, c->stream))) return rc;
-    const int nslots = std::min(c->opts.batch_loci, K);
+    const int nslots = slots_for(c->opts, true, K);
     const int m = c->opts.history;
     if ((rc = ensure_gram_ws(c, nslots, m))) return rc;
     if ((rc = upload_gram_alns(c))) return rc;
@@ -2004,7 +2012,7 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
     if (o.formulation < SDCA_FORM_AUTO || o.formulation > SDCA_FORM_GRAM)
         return fail(SDCA_E_ARG, "unknown formulation %d", o.formulation);
     if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 8, 10, 16 (got %d)", o.history);
-    if (o.batch_loci < 1) return fail(SDCA_E_ARG, "batch_loci must be >= 1");
+    if (o.batch_loci < 0) return fail(SDCA_E_ARG, "batch_loci must be >= 1 (or 0: the formulation's default)");
     if (o.max_iter < 1 || o.max_ls < 1) return fail(SDCA_E_ARG, "max_iter and max_ls must be >= 1");
     if (!(o.grad_threshold >= 0.0)) return fail(SDCA_E_ARG, "grad_threshold must be >= 0");
     for (int64_t e = 0; e < (int64_t)B * L; ++e)
@@ -2172,7 +2180,7 @@ int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x
     for (int k = 0; k < K; ++k)
         if (loci[k] < 0 || loci[k] >= c->L) return fail(SDCA_E_ARG, "locus %d out of range", loci[k]);
     if (K == 0) return SDCA_OK;
-    const int nslots = std::min(c->opts.batch_loci, K);
+    const int nslots = slots_for(c->opts, false, K);
     if ((rc = ensure_workspace(c, nslots, c->opts.history))) return rc;
     if ((rc = upload_alns(c))) return rc;
     if (!c->d_X0) {
@@ -2277,7 +2285,7 @@ int sdca_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
         return fail(SDCA_E_ARG, "the Gram formulation solves from a cold start only");
     if (gram_wanted(c, warm || cw)) return gram_solve_batch(c, alns, loci, K, score_rows, row_off, x_out, st_out);
     const auto t0 = std::chrono::steady_clock::now();
-    const int nslots = std::min(c->opts.batch_loci, K);
+    const int nslots = slots_for(c->opts, false, K);
     if ((rc = ensure_workspace(c, nslots, c->opts.history))) return rc;
     if ((rc = upload_alns(c))) return rc;
     const bool any_warm = warm || cw;
@@ -2670,7 +2678,7 @@ int sdca_multi_create(sdca_multi **out, sdca_ctx *const *ctxs, int32_t n, const 
     if (o.formulation < SDCA_FORM_AUTO || o.formulation > SDCA_FORM_GRAM)
         return fail(SDCA_E_ARG, "unknown formulation %d", o.formulation);
     if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 8, 10, 16 (got %d)", o.history);
-    if (o.batch_loci < 1 || o.max_iter < 1 || o.max_ls < 1 || !(o.grad_threshold >= 0.0) || !(o.j_threshold >= 0.0) ||
+    if (o.batch_loci < 0 || o.max_iter < 1 || o.max_ls < 1 || !(o.grad_threshold >= 0.0) || !(o.j_threshold >= 0.0) ||
         o.coarse_factor < 0 || o.fine_factor < 0)
         return fail(SDCA_E_ARG, "invalid solver options");
     for (int a = 0; a < n; ++a) {

@@ -43,8 +43,9 @@ struct Options {
     bool no_filter = false;
     int output_base = 1, input_base = 1;
     long begin_locus = 0, end_locus = -1;
-    int device = 0, batch_loci = 128, history = 10, q_env = 4, max_iter = 2000;
+    int device = 0, batch_loci = 0, history = 10, q_env = 4, max_iter = 2000;
     int precision = SDCA_PREC_FP32;
+    int formulation = SDCA_FORM_AUTO;
     int chunk_loci = 0;  // loci per solve call (0: 4096, or fewer with --norm-of-mean-scoring)
 };
 
@@ -76,8 +77,9 @@ const char *kUsage =
     "                                 every q x q block on disk: loci^2 x q^2 x 4 bytes).\n"
     "  --gradient-threshold arg (=0.001)\n"
     "  --coupling-gradient-threshold arg (=0)\n"
-    "                                 Also require ||g||_inf < arg over the couplings J (0: off);\n"
-    "                                 3e-6 puts the scores within 1e-4 of the exact optimum.\n"
+    "                                 Also require the couplings' gradient below arg (0: off):\n"
+    "                                 ||g_J||_inf (primal) or ||g_J||_2 (gram); 3e-6 or 1e-5\n"
+    "                                 puts the scores within 1e-4 of the exact optimum.\n"
     "  --lambda-h arg (=-1)           (< 0: automatic)\n"
     "  --lambda-J arg (=-1)           (< 0: automatic)\n"
     "  --no-dca                       Stop after the weights.\n"
@@ -85,7 +87,12 @@ const char *kUsage =
     "  --output-top arg               Also write the arg largest scores, largest first (.top).\n"
     "  --output-min-score arg         Also write every score >= arg, largest first (.top).\n"
     "GPU:\n"
-    "  --device arg (=0)  --batch-loci arg (=128)  --history arg (=10)  --max-iter arg (=2000)\n"
+    "  --device arg (=0)  --history arg (=10)  --max-iter arg (=2000)\n"
+    "  --batch-loci arg (=0)          Loci in flight (0: 4096 for the Gram formulation, 128 otherwise).\n"
+    "  --formulation auto|gram|primal (=auto)\n"
+    "                                 gram: couplings in the span of the sequences, FP64, one B x B\n"
+    "                                 GEMM per evaluation, stopping on 2-norms; primal: the q x K_x\n"
+    "                                 parameters (--precision); auto: gram when B <= K_x.\n"
     "  --chunk-loci arg (=4096)       Loci per solve call; the scores stream through a file next to\n"
     "                                 the output, so host memory is O(chunk x loci).\n"
     "  --precision fp32|split-bf16 (=fp32)  --q-env arg (=4; 5 keeps four alleles plus gap)\n";
@@ -166,7 +173,13 @@ Options parse_args(int argc, char **argv)
         else if (a == "--history") o.history = std::atoi(need(i).c_str());
         else if (a == "--max-iter") o.max_iter = std::atoi(need(i).c_str());
         else if (a == "--q-env") o.q_env = std::atoi(need(i).c_str());
-        else if (a == "--precision") {
+        else if (a == "--formulation") {
+            const std::string v = need(i);
+            if (v == "auto") o.formulation = SDCA_FORM_AUTO;
+            else if (v == "gram") o.formulation = SDCA_FORM_GRAM;
+            else if (v == "primal") o.formulation = SDCA_FORM_PRIMAL;
+            else die("unknown formulation \"" + v + "\"");
+        } else if (a == "--precision") {
             const std::string v = need(i);
             if (v == "fp32") o.precision = SDCA_PREC_FP32;
             else if (v == "split-bf16") o.precision = SDCA_PREC_SPLIT_BF16;
@@ -248,7 +261,8 @@ sdca_opts solver_opts(const Options &o)
     sdca_default_opts(&so);
     so.device = o.device;
     so.precision = o.precision;
-    so.batch_loci = o.batch_loci;
+    so.formulation = o.formulation;
+    so.batch_loci = o.batch_loci;  // 0: the library's default for the formulation it picks
     so.history = o.history;
     so.max_iter = o.max_iter;
     so.grad_threshold = o.gradient_threshold;
