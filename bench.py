@@ -209,7 +209,7 @@ def main() -> int:
     ap.add_argument("--history", type=int, default=10)
     ap.add_argument("--groups", type=int, default=0,
                     help="headline slot groups: 1, or 2 pipelined on two streams (0: the library's default, 2)")
-    ap.add_argument("--j-threshold", type=float, default=1e-5,
+    ap.add_argument("--j-threshold", type=float, default=3e-5,
                     help="headline coupling threshold on ||g_J||_2 (0: ||g||_2 < 1e-3 alone)")
     ap.add_argument("--rule-loci", type=int, default=4096,
                     help="loci of the Gram leg at ||g||_2 < 1e-3 alone (0: skip)")

@@ -80,7 +80,12 @@ typedef struct sdca_opts {
                                 SuperDCA's rule alone (plmDCA.hpp:218-221) */
     int32_t formulation;     /* SDCA_FORM_*: how a locus' couplings are represented during its
                                 solve (default SDCA_FORM_AUTO) */
-    int32_t reserved;
+    int32_t graph_rounds;    /* 1: a Gram-formulation solve over one alignment captures each slot
+                                group's round (command upload, kernels, result copy) as a HIP graph
+                                once and replays it every round, with fixed grids over all of the
+                                group's slots (idle slots return at once inside the kernels); the
+                                round's device time is then reported whole, in lbfgs_ms.  0 (default):
+                                the kernels are launched each round over the active slots */
 } sdca_opts;
 
 /* Formulations of the per-locus solve (sdca_opts.formulation).

@@ -68,7 +68,7 @@ class Opts(C.Structure):
                 ("gemm_ksplit", C.c_int32), ("gemm_gm_energy", C.c_int32), ("gemm_gm_grad", C.c_int32),
                 ("gemm_tn_energy", C.c_int32), ("gemm_tn_grad", C.c_int32), ("pipeline_groups", C.c_int32),
                 ("coarse_factor", C.c_int32), ("fine_factor", C.c_int32), ("j_threshold", C.c_double),
-                ("formulation", C.c_int32), ("reserved", C.c_int32)]
+                ("formulation", C.c_int32), ("graph_rounds", C.c_int32)]
 
 
 class LocusStats(C.Structure):

@@ -220,6 +220,10 @@ sdca::GramWs::~GramWs()
     if (h_res) (void)hipHostFree(h_res);
     if (h_rows) (void)hipHostFree(h_rows);
     dfree(Pend);
+    for (int g = 0; g < sdca::kMaxGroups; ++g) {
+        if (gexec[g]) (void)hipGraphExecDestroy(gexec[g]);
+        if (graph[g]) (void)hipGraphDestroy(graph[g]);
+    }
     for (auto &g : ev)
         for (auto &e : g)
             if (e) (void)hipEventDestroy(e);
@@ -1569,11 +1573,67 @@ int upload_gram_alns(sdca_ctx *c)
 // One round of slot group g (slots [g gs, (g + 1) gs)) on its stream: the commands of its
 // active slots (by alignment, in slot order), each owning q consecutive columns of the
 // group's share of the packed GEMM operand.
+// With graph_rounds (one alignment): every slot of the group gets a command (idle ones
+// stay IDLE), command k at columns k q; the round was captured once as a HIP graph with
+// these fixed grids and is replayed.
+int gram_launch_graph(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds)
+{
+    sdca::GramWs *G = c->gram;
+    const int m = G->m, W = sdca::ResLayout{m}.width();
+    const sdca_ctx *A = c->members[0];
+    const int s0 = g * G->gs, s1 = std::min<int>((int)cmds.size(), s0 + G->gs), n = s1 - s0;
+    std::vector<int> &lst = G->cmd_slot[g];
+    lst.clear();
+    sdca::SlotCmd *hc = G->h_cmds + s0;
+    for (int i = s0; i < s1; ++i) {
+        sdca::SlotCmd &k = hc[i - s0];
+        k = cmds[i];
+        k.slot = i;
+        k.row = (i - s0) * A->q;
+        k.aln = 0;
+        k.q = A->q;
+        k.Kx = (int)A->Kx;
+        k.Kxp = (int)A->Kxp;
+        lst.push_back(i);
+    }
+    hipStream_t st = c->gstream[g];
+    hipEvent_t *ev = G->ev[g];
+    if (!G->gexec[g] || G->graph_K[g] != A->d_gram) {
+        if (G->gexec[g]) (void)hipGraphExecDestroy(G->gexec[g]);
+        if (G->graph[g]) (void)hipGraphDestroy(G->graph[g]);
+        G->gexec[g] = nullptr;
+        G->graph[g] = nullptr;
+        sdca::SlotCmd *dc = G->d_cmds + s0;
+        double *Gp = G->Gp + (size_t)g * G->colcap * G->ldG, *Zp = G->Zp + (size_t)g * G->colcap * G->ldG;
+        double *upd = G->upd_part + (size_t)s0 * (G->nchq + 1) * 3, *sm = G->sm_part + (size_t)s0 * G->nchunk * 65;
+        double *dot = G->dot_part + (size_t)s0 * G->nchd * sdca::ResLayout{m}.ndots(), *res = G->res + (size_t)s0 * W;
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        (void)hipMemcpyAsync(dc, hc, n * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st);
+        sdca::gram::step(st, m, dc, n, G->d_galn, G->V, G->vstride, G->nvs, Gp, G->ldG, upd, sm, G->nchunk, G->nchq);
+        sdca::gram::gemm(st, A->d_gram, A->Bp, Gp, G->ldG, Zp, G->ldG, A->Bp, n * A->q, A->Bp);
+        sdca::gram::dots(st, m, dc, n, G->d_galn, G->V, G->vstride, G->nvs, Zp, G->ldG, sm, G->nchunk, dot, G->nchd);
+        sdca::gram::reduce(st, m, n, upd, dot, sm, G->nchunk, G->nchq, G->nchd, res);
+        (void)hipMemcpyAsync(G->h_res + (size_t)s0 * W, res, (size_t)n * W * sizeof(double), hipMemcpyDeviceToHost,
+                             st);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &graph);
+        if (e != hipSuccess) return fail(SDCA_E_HIP, "capturing a round failed: %s", hipGetErrorString(e));
+        G->graph[g] = graph;
+        HIP_TRY(hipGraphInstantiate(&G->gexec[g], graph, nullptr, nullptr, 0));
+        G->graph_K[g] = A->d_gram;
+    }
+    HIP_TRY(hipEventRecord(ev[0], st));
+    HIP_TRY(hipGraphLaunch(G->gexec[g], st));
+    HIP_TRY(hipEventRecord(ev[3], st));
+    return SDCA_OK;
+}
+
 int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds)
 {
     sdca::GramWs *G = c->gram;
     const int m = G->m, W = sdca::ResLayout{m}.width();
     const int nmem = (int)c->members.size();
+    if (c->opts.graph_rounds && nmem == 1) return gram_launch_graph(c, g, cmds);
     const int s0 = g * G->gs, s1 = std::min<int>((int)cmds.size(), s0 + G->gs);
     std::vector<int> &lst = G->cmd_slot[g];
     lst.clear();
@@ -1646,16 +1706,22 @@ int gram_finish_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     }
     hipEvent_t *ev = G->ev[g];
     auto &t = c->timing;
-    const double upd = elapsed(ev[0], ev[1]), gm = elapsed(ev[1], ev[2]), dts = elapsed(ev[2], ev[3]);
-    t.update += upd;
-    t.grad += gm;
-    t.dots += dts;
-    t.lbfgs += upd + dts;
+    const bool graphed = c->opts.graph_rounds && c->members.size() == 1;
+    if (graphed) {
+        t.lbfgs += elapsed(ev[0], ev[3]);  // the whole replayed round
+    } else {
+        const double upd = elapsed(ev[0], ev[1]), gm = elapsed(ev[1], ev[2]), dts = elapsed(ev[2], ev[3]);
+        t.update += upd;
+        t.grad += gm;
+        t.dots += dts;
+        t.lbfgs += upd + dts;
+        t.grad_launches += 1;
+    }
     t.evals += 1;
-    t.grad_launches += 1;
     double flops = 0.0, bu = 0.0, bd = 0.0;
     for (int i : lst) {
         const sdca::SlotCmd &k = cmds[i];
+        if (k.mode == sdca::MODE_IDLE) continue;
         const sdca_ctx *A = c->members[k.aln];
         const double half = (double)A->q * A->Bp;  // coefficients (or image entries) of a vector
         flops += 2.0 * A->B * (double)A->B * A->q;

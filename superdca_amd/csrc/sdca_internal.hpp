@@ -344,6 +344,10 @@ struct GramWs {
     int32_t *d_mrow = nullptr, *d_maln = nullptr, *d_sloc = nullptr, *d_sld = nullptr;
     float *d_rows = nullptr, *h_rows = nullptr;  // h_rows: pinned
     hipEvent_t ev[kMaxGroups][8] = {};
+    // graph_rounds: per group, the captured round and the identical-locus counts it reads
+    hipGraph_t graph[kMaxGroups] = {};
+    hipGraphExec_t gexec[kMaxGroups] = {};
+    const double *graph_K[kMaxGroups] = {};
     ~GramWs();
 };
 

@@ -199,6 +199,34 @@ def test_warm_start_reaches_cold_optimum(c1, c1_optimum):
     assert np.median(sw["iter"]) < np.median(sc["iter"])
 
 
+def test_gram_graph_rounds_match_launches(c1, c1_optimum):
+    # each slot group's round replayed from a captured HIP graph (fixed grids over all of the
+    # group's slots, idle ones skipped inside the kernels) against rounds launched over the
+    # active slots: 80 loci through 64 slots (two groups, refills in both, idle slots at
+    # the end), then 80 more through the same workspace, so the graphs are replayed across
+    # solves with the host's own work and synchronisations between the replays; every
+    # result must be identical
+    st = c1["states"]
+    first, second = np.arange(1, 1000, 12)[:80], np.arange(5, 1000, 12)[:80]
+    out = {}
+    for graph in (0, 1):
+        with sd.Context(st, formulation=sd.FORM_GRAM, grad_threshold=1e-4, j_threshold=1e-5, batch_loci=64,
+                        graph_rounds=graph) as ctx:
+            ctx.set_weights(c1_optimum["w"])
+            r1, s1, _, rc1 = ctx.solve(first)
+            r2, s2, _, rc2 = ctx.solve(second)
+            tm = ctx.timing()
+        assert rc1 == sd.SDCA_OK and rc2 == sd.SDCA_OK, graph
+        out[graph] = (r1, s1, r2, s2, tm)
+    for k in (0, 2):
+        np.testing.assert_array_equal(out[1][k], out[0][k])
+    for k in (1, 3):
+        for f in ("fval", "gnorm", "nfeval", "iter", "status"):
+            np.testing.assert_array_equal(out[1][k][f], out[0][k][f])
+    assert out[1][4]["n_evals"] == out[0][4]["n_evals"]
+    assert out[1][4]["gemm_grad_ms"] == 0.0 and out[1][4]["lbfgs_ms"] > 0.0  # a replayed round is timed whole
+
+
 def test_cli_end_to_end(tmp_path):
     d = synth.generate(B=150, N=120, clades=6, pairs=4, seed=77)
     fa = tmp_path / "toy.fasta"
