@@ -9,7 +9,7 @@ when the last score row is in host memory.
 
 Headline (value): the Gram formulation -- each locus' couplings held as J = X_r^T c in
 the span of the data, FP64 evaluations with one B x B x q GEMM per round, cold start --
-stopping at ||g||_2 < 1e-3 and ||g_J||_2 < 1e-5 (2-norms: bounds on the infinity norms of
+stopping at ||g||_2 < 1e-3 and ||g_J||_2 < 3e-5 (2-norms: bounds on the infinity norms of
 SuperDCA's rule from above).  Its score error against the FP64 oracle is measured on the
 64 loci of the committed C4 fixture, solved with the headline settings.
 
