@@ -281,12 +281,17 @@ int sdca_mi_topk(sdca_ctx *ctx, int32_t K, int32_t *partners_out, float *mi_out)
 /* ---- warm starts: pair features and the learned network (no counterpart in SuperDCA) ---- */
 
 /* Features of locus pairs (r, j) = (pairs[2p], pairs[2p+1]): the joint frequencies
- * f_rj(s, a) at [s*q + a], the marginals f_r(s), f_j(a) and the pair's mutual information,
- * q*q + 2q + 1 floats per pair (unweighted frequencies from exact counts). */
+ * f_rj(s, a) at [s*q + a], the marginals f_r(s), f_j(a) and the pair's mean-field coupling
+ * estimate J_mf(s, a) at [q*q + 2q + s*q + a]: 2q*q + 2q floats per pair (unweighted
+ * frequencies from exact counts).  J_mf = -(C^-1)_rj of the pair's connected correlations
+ * over the first q-1 states of each locus, with pseudocount weight 1/2; 0 for the last
+ * state. */
 int sdca_pair_features(sdca_ctx *ctx, const int32_t *pairs, int64_t npairs, float *feats);
 
 /* The warm-start network: features -> hidden -> hidden (ReLU) -> q*q outputs, where output
- * s*q + a is the start value of J_r[j](a, s).  FP32 parameters on the device.
+ * s*q + a is the start value of J_r[j](a, s).  hidden is a multiple of 32 in [32, 1024].
+ * FP32 master parameters on the device; the layers run as BF16 products on MFMA with FP32
+ * accumulation (weights, activations and backward output gradients rounded to BF16).
  * sdca_mlp_init draws He-uniform weights from seed; sdca_mlp_set_params installs given
  * weights (the layout W1 [H][Din], b1, W2 [H][H], b2, W3 [Dout][H], b3). */
 int sdca_mlp_init(sdca_ctx *ctx, int32_t hidden, uint64_t seed);

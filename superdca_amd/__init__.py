@@ -309,7 +309,7 @@ class Context:
 
     def pair_features(self, pairs) -> np.ndarray:
         pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
-        D = self.q * self.q + 2 * self.q + 1
+        D = 2 * self.q * self.q + 2 * self.q
         out = np.zeros((pairs.shape[0], D), dtype=np.float32)
         _check(lib().sdca_pair_features(self._h, _ptr(pairs), pairs.shape[0], _ptr(out)))
         return out

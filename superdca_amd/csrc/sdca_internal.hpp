@@ -258,12 +258,33 @@ void seq_planes(hipStream_t st, const uint8_t *states, int B, int L, int q, int 
 void pair_counts(hipStream_t st, const uint64_t *SP, int q, int W, const int32_t *pairs, int64_t npairs,
                  int32_t *counts);
 // sdca_mlp.hip
-void linear_fwd(hipStream_t st, bool relu, const float *X, const float *W, const float *b, float *Y, int64_t N, int I,
-                int O);
-void linear_bwd_dx(hipStream_t st, const float *dY, const float *W, const float *H, float *dX, int64_t N, int I, int O);
-void linear_bwd_dw(hipStream_t st, const float *dY, const float *X, float *dW, float *db, int64_t N, int I, int O);
-int mse(hipStream_t st, const float *Y, const float *T, const float *M, float *dY, int64_t total, float scale,
-        double *part, double *loss_acc);
+// Epilogue of the network's BF16 products C = A B^T (mlp_gemm): bias, ReLU or the
+// backward ReLU mask, and FP32 rows, BF16 rows and BF16 transposed rows as outputs.
+struct MlpEpi {
+    const float *bias = nullptr;     // [N]
+    int relu = 0;
+    const uint16_t *mask = nullptr;  // BF16 [M][ldm]: x = 0 where mask <= 0
+    int64_t ldm = 0;
+    float *f32 = nullptr;            // [M][ldf]
+    int64_t ldf = 0;
+    uint16_t *bf = nullptr;          // [Mt*32][ldbf]; rows >= M written as 0
+    int64_t ldbf = 0;
+    uint16_t *bt = nullptr;          // [N][ldbt]; columns >= M written as 0
+    int64_t ldbt = 0;
+    int64_t M = 0;                   // valid rows
+    int N = 0;                       // valid columns
+};
+// Mt x Nt blocks of 32 x 32; A rows up to Mt*32 and B rows up to Nt*32 must be readable,
+// K a multiple of 16, lda and ldb multiples of 8.
+void mlp_gemm(hipStream_t st, const uint16_t *A, int64_t lda, const uint16_t *B, int64_t ldb, int K, int64_t Mt, int Nt,
+              const MlpEpi &ep);
+void rows_bf16(hipStream_t st, const float *X, int64_t n, int D, int64_t npad, int DP, uint16_t *Xb, uint16_t *Xt,
+               int64_t ldt);
+void pack_weights(hipStream_t st, const float *p, int din, int H, int dout, int DP, int DO, int64_t oW1, int64_t oW2,
+                  int64_t oW3, uint16_t *W1b, uint16_t *W2b, uint16_t *W2t, uint16_t *W3b, uint16_t *W3t);
+void colsum(hipStream_t st, const float *Z, int64_t ld, int64_t n, int O, float *out);
+int mse(hipStream_t st, const float *Y, const float *T, const float *M, float *dY, uint16_t *dYb, uint16_t *dYt,
+        int64_t n, int64_t npad, int dout, int DO, int64_t ldt, float scale, double *part, double *loss_acc);
 void adam(hipStream_t st, float *p, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2,
           float eps, float c1, float c2);
 void gather_rows(hipStream_t st, const float *src, const int64_t *idx, int64_t nrows, int D, float *out);
@@ -366,11 +387,24 @@ struct CompactWarm {
 // The warm-start network (sdca_mlp.hip) and its training state.
 struct Mlp {
     int din = 0, dh = 0, dout = 0;
+    int DP = 0, DO = 0;  // din and dout padded to 32: widths of the BF16 operands
     int64_t np = 0, step = 0;
-    float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;
-    int64_t cap = 0;  // rows of the scratch buffers below
-    float *d_x = nullptr, *d_t = nullptr, *d_msk = nullptr, *d_h1 = nullptr, *d_h2 = nullptr, *d_y = nullptr;
-    float *d_dy = nullptr, *d_dz2 = nullptr, *d_dz1 = nullptr;
+    float *d_p = nullptr, *d_g = nullptr, *d_m = nullptr, *d_v = nullptr;  // FP32 masters, gradient, Adam moments
+    // BF16 operands of the masters: W1 [dh][DP], W2 [dh][dh] and its transpose, W3 [DO][dh]
+    // and its transpose [dh][DO]; packed says they match d_p
+    uint16_t *d_W1b = nullptr, *d_W2b = nullptr, *d_W2t = nullptr, *d_W3b = nullptr, *d_W3t = nullptr;
+    bool packed = false;
+    // forward buffers, cap rows (a multiple of 32): FP32 input staging [cap][din], output
+    // [cap][dout], BF16 input and activations
+    int64_t cap = 0;
+    float *d_x = nullptr, *d_y = nullptr;
+    uint16_t *d_xb = nullptr, *d_h1b = nullptr, *d_h2b = nullptr;
+    // training buffers, tcap rows: targets, mask, output gradients (FP32 [tcap][width]), BF16
+    // copies and the transposes [width][tcap] that the weight-gradient products read
+    int64_t tcap = 0;
+    float *d_t = nullptr, *d_msk = nullptr, *d_dy = nullptr, *d_dz2 = nullptr, *d_dz1 = nullptr;
+    uint16_t *d_xt = nullptr, *d_h1t = nullptr, *d_h2t = nullptr, *d_dyb = nullptr, *d_dyt = nullptr,
+             *d_dz2b = nullptr, *d_dz2t = nullptr, *d_dz1t = nullptr;
     double *d_part = nullptr, *d_loss = nullptr;
     int64_t *d_idx = nullptr;
     ~Mlp();

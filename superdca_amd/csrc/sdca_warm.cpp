@@ -64,59 +64,174 @@ Offs offsets(int din, int dh, int dout)
     return o;
 }
 
-void free_scratch(sdca::Mlp *m)
+int64_t round32(int64_t n) { return (n + 31) / 32 * 32; }
+
+void free_forward(sdca::Mlp *m)
 {
     wfree(m->d_x);
-    wfree(m->d_t);
-    wfree(m->d_msk);
-    wfree(m->d_h1);
-    wfree(m->d_h2);
     wfree(m->d_y);
-    wfree(m->d_dy);
-    wfree(m->d_dz2);
-    wfree(m->d_dz1);
-    wfree(m->d_part);
-    wfree(m->d_loss);
-    wfree(m->d_idx);
+    wfree(m->d_xb);
+    wfree(m->d_h1b);
+    wfree(m->d_h2b);
     m->cap = 0;
 }
 
+void free_training(sdca::Mlp *m)
+{
+    wfree(m->d_t);
+    wfree(m->d_msk);
+    wfree(m->d_dy);
+    wfree(m->d_dz2);
+    wfree(m->d_dz1);
+    wfree(m->d_xt);
+    wfree(m->d_h1t);
+    wfree(m->d_h2t);
+    wfree(m->d_dyb);
+    wfree(m->d_dyt);
+    wfree(m->d_dz2b);
+    wfree(m->d_dz2t);
+    wfree(m->d_dz1t);
+    wfree(m->d_part);
+    wfree(m->d_loss);
+    wfree(m->d_idx);
+    m->tcap = 0;
+}
+
+// forward buffers for at least `rows` rows
 int ensure_scratch(sdca::Mlp *m, int64_t rows)
 {
+    rows = round32(std::max<int64_t>(rows, 1));
     if (m->cap >= rows) return SDCA_OK;
-    free_scratch(m);
-    const int64_t wide = std::max({m->din, m->dh, m->dout});
+    free_forward(m);
     int rc;
-    if ((rc = walloc(&m->d_x, rows * m->din)) || (rc = walloc(&m->d_t, rows * m->dout)) ||
-        (rc = walloc(&m->d_msk, rows * m->dout)) || (rc = walloc(&m->d_h1, rows * m->dh)) ||
-        (rc = walloc(&m->d_h2, rows * m->dh)) || (rc = walloc(&m->d_y, rows * m->dout)) ||
-        (rc = walloc(&m->d_dy, rows * wide)) || (rc = walloc(&m->d_dz2, rows * m->dh)) ||
-        (rc = walloc(&m->d_dz1, rows * m->dh)) || (rc = walloc(&m->d_part, (size_t)(rows * m->dout / 256 + 2))) ||
-        (rc = walloc(&m->d_loss, 1)) || (rc = walloc(&m->d_idx, rows)))
+    if ((rc = walloc(&m->d_x, rows * m->din)) || (rc = walloc(&m->d_y, rows * m->dout)) ||
+        (rc = walloc(&m->d_xb, rows * m->DP)) || (rc = walloc(&m->d_h1b, rows * m->dh)) ||
+        (rc = walloc(&m->d_h2b, rows * m->dh)))
         return rc;
     m->cap = rows;
     return SDCA_OK;
 }
 
-// forward of n rows already in m->d_x: d_h1, d_h2, d_y
-void forward(hipStream_t st, sdca::Mlp *m, const float *x, int64_t n)
+// forward and training buffers for mini-batches of at most `rows` rows
+int ensure_training(sdca::Mlp *m, int64_t rows)
 {
-    const Offs o = offsets(m->din, m->dh, m->dout);
-    sdca::kern::linear_fwd(st, true, x, m->d_p + o.W1, m->d_p + o.b1, m->d_h1, n, m->din, m->dh);
-    sdca::kern::linear_fwd(st, true, m->d_h1, m->d_p + o.W2, m->d_p + o.b2, m->d_h2, n, m->dh, m->dh);
-    sdca::kern::linear_fwd(st, false, m->d_h2, m->d_p + o.W3, m->d_p + o.b3, m->d_y, n, m->dh, m->dout);
+    rows = round32(std::max<int64_t>(rows, 1));
+    int rc = ensure_scratch(m, rows);
+    if (rc || m->tcap >= rows) return rc;
+    free_training(m);
+    const int64_t nblocks = rows * m->DO / 256 + 2;
+    if ((rc = walloc(&m->d_t, rows * m->dout)) || (rc = walloc(&m->d_msk, rows * m->dout)) ||
+        (rc = walloc(&m->d_dy, rows * m->dout)) || (rc = walloc(&m->d_dz2, rows * m->dh)) ||
+        (rc = walloc(&m->d_dz1, rows * m->dh)) || (rc = walloc(&m->d_xt, rows * m->DP)) ||
+        (rc = walloc(&m->d_h1t, rows * m->dh)) || (rc = walloc(&m->d_h2t, rows * m->dh)) ||
+        (rc = walloc(&m->d_dyb, rows * m->DO)) || (rc = walloc(&m->d_dyt, rows * m->DO)) ||
+        (rc = walloc(&m->d_dz2b, rows * m->dh)) || (rc = walloc(&m->d_dz2t, rows * m->dh)) ||
+        (rc = walloc(&m->d_dz1t, rows * m->dh)) || (rc = walloc(&m->d_part, (size_t)nblocks)) ||
+        (rc = walloc(&m->d_loss, 1)) || (rc = walloc(&m->d_idx, rows)))
+        return rc;
+    m->tcap = rows;
+    return SDCA_OK;
 }
 
-// masked MSE gradient of n rows (x, t, mask on the device) into m->d_g; loss accumulated in d_loss
-void backward(hipStream_t st, sdca::Mlp *m, const float *x, const float *t, const float *mask, int64_t n, float count)
+// the BF16 operands from the FP32 masters (after a change of d_p)
+void pack(hipStream_t st, sdca::Mlp *m)
+{
+    if (m->packed) return;
+    const Offs o = offsets(m->din, m->dh, m->dout);
+    sdca::kern::pack_weights(st, m->d_p, m->din, m->dh, m->dout, m->DP, m->DO, o.W1, o.W2, o.W3, m->d_W1b, m->d_W2b,
+                             m->d_W2t, m->d_W3b, m->d_W3t);
+    m->packed = true;
+}
+
+// Forward of n rows of FP32 features x (on the device, [n][din]) into m->d_y [n][dout];
+// with train, the transposed BF16 input and activations that backward() reads as well
+// (n <= tcap).  Three BF16 products: H1 = relu(X W1^T + b1), H2 = relu(H1 W2^T + b2),
+// Y = H2 W3^T + b3.
+void forward(hipStream_t st, sdca::Mlp *m, const float *x, int64_t n, bool train)
+{
+    if (n <= 0) return;
+    pack(st, m);
+    const Offs o = offsets(m->din, m->dh, m->dout);
+    const int64_t npad = round32(n), Mt = npad / 32, ldt = m->tcap;
+    const int Ht = m->dh / 32;
+    sdca::kern::rows_bf16(st, x, n, m->din, npad, m->DP, m->d_xb, train ? m->d_xt : nullptr, ldt);
+    sdca::kern::MlpEpi e1;
+    e1.bias = m->d_p + o.b1;
+    e1.relu = 1;
+    e1.bf = m->d_h1b;
+    e1.ldbf = m->dh;
+    e1.bt = train ? m->d_h1t : nullptr;
+    e1.ldbt = ldt;
+    e1.M = n;
+    e1.N = m->dh;
+    sdca::kern::mlp_gemm(st, m->d_xb, m->DP, m->d_W1b, m->DP, m->DP, Mt, Ht, e1);
+    sdca::kern::MlpEpi e2 = e1;
+    e2.bias = m->d_p + o.b2;
+    e2.bf = m->d_h2b;
+    e2.bt = train ? m->d_h2t : nullptr;
+    sdca::kern::mlp_gemm(st, m->d_h1b, m->dh, m->d_W2b, m->dh, m->dh, Mt, Ht, e2);
+    sdca::kern::MlpEpi e3;
+    e3.bias = m->d_p + o.b3;
+    e3.f32 = m->d_y;
+    e3.ldf = m->dout;
+    e3.M = n;
+    e3.N = m->dout;
+    sdca::kern::mlp_gemm(st, m->d_h2b, m->dh, m->d_W3b, m->dh, m->dh, Mt, m->DO / 32, e3);
+}
+
+// Masked MSE gradient of the n rows of the last training forward (targets t and mask on
+// the device) into m->d_g; the loss is accumulated in d_loss.  Weight gradients are BF16
+// products over the batch (dW = dZ^T H); bias gradients FP32 column sums.
+void backward(hipStream_t st, sdca::Mlp *m, const float *t, const float *mask, int64_t n, float count)
 {
     const Offs o = offsets(m->din, m->dh, m->dout);
-    sdca::kern::mse(st, m->d_y, t, mask, m->d_dy, n * m->dout, 2.0f / count, m->d_part, m->d_loss);
-    sdca::kern::linear_bwd_dw(st, m->d_dy, m->d_h2, m->d_g + o.W3, m->d_g + o.b3, n, m->dh, m->dout);
-    sdca::kern::linear_bwd_dx(st, m->d_dy, m->d_p + o.W3, m->d_h2, m->d_dz2, n, m->dh, m->dout);
-    sdca::kern::linear_bwd_dw(st, m->d_dz2, m->d_h1, m->d_g + o.W2, m->d_g + o.b2, n, m->dh, m->dh);
-    sdca::kern::linear_bwd_dx(st, m->d_dz2, m->d_p + o.W2, m->d_h1, m->d_dz1, n, m->dh, m->dh);
-    sdca::kern::linear_bwd_dw(st, m->d_dz1, x, m->d_g + o.W1, m->d_g + o.b1, n, m->din, m->dh);
+    const int64_t npad = round32(n), Mt = npad / 32, ldt = m->tcap;
+    const int Ht = m->dh / 32;
+    sdca::kern::mse(st, m->d_y, t, mask, m->d_dy, m->d_dyb, m->d_dyt, n, npad, m->dout, m->DO, ldt, 2.0f / count,
+                    m->d_part, m->d_loss);
+    sdca::kern::MlpEpi w3;  // dW3 [dout][dh] = dY^T H2
+    w3.f32 = m->d_g + o.W3;
+    w3.ldf = m->dh;
+    w3.M = m->dout;
+    w3.N = m->dh;
+    sdca::kern::mlp_gemm(st, m->d_dyt, ldt, m->d_h2t, ldt, (int)npad, m->DO / 32, Ht, w3);
+    sdca::kern::colsum(st, m->d_dy, m->dout, n, m->dout, m->d_g + o.b3);
+    sdca::kern::MlpEpi z2;  // dZ2 = (dY W3) * [H2 > 0]
+    z2.mask = m->d_h2b;
+    z2.ldm = m->dh;
+    z2.f32 = m->d_dz2;
+    z2.ldf = m->dh;
+    z2.bf = m->d_dz2b;
+    z2.ldbf = m->dh;
+    z2.bt = m->d_dz2t;
+    z2.ldbt = ldt;
+    z2.M = n;
+    z2.N = m->dh;
+    sdca::kern::mlp_gemm(st, m->d_dyb, m->DO, m->d_W3t, m->DO, m->DO, Mt, Ht, z2);
+    sdca::kern::MlpEpi w2;  // dW2 [dh][dh] = dZ2^T H1
+    w2.f32 = m->d_g + o.W2;
+    w2.ldf = m->dh;
+    w2.M = m->dh;
+    w2.N = m->dh;
+    sdca::kern::mlp_gemm(st, m->d_dz2t, ldt, m->d_h1t, ldt, (int)npad, Ht, Ht, w2);
+    sdca::kern::colsum(st, m->d_dz2, m->dh, n, m->dh, m->d_g + o.b2);
+    sdca::kern::MlpEpi z1;  // dZ1 = (dZ2 W2) * [H1 > 0]
+    z1.mask = m->d_h1b;
+    z1.ldm = m->dh;
+    z1.f32 = m->d_dz1;
+    z1.ldf = m->dh;
+    z1.bt = m->d_dz1t;
+    z1.ldbt = ldt;
+    z1.M = n;
+    z1.N = m->dh;
+    sdca::kern::mlp_gemm(st, m->d_dz2b, m->dh, m->d_W2t, m->dh, m->dh, Mt, Ht, z1);
+    sdca::kern::MlpEpi w1;  // dW1 [dh][din] = dZ1^T X
+    w1.f32 = m->d_g + o.W1;
+    w1.ldf = m->din;
+    w1.M = m->dh;
+    w1.N = m->din;
+    sdca::kern::mlp_gemm(st, m->d_dz1t, ldt, m->d_xt, ldt, (int)npad, Ht, m->DP / 32, w1);
+    sdca::kern::colsum(st, m->d_dz1, m->dh, n, m->dh, m->d_g + o.b1);
 }
 
 int check(sdca_ctx *c)
@@ -134,17 +249,22 @@ int need_mlp(sdca_ctx *c)
 
 int create_mlp(sdca_ctx *c, int hidden)
 {
-    if (hidden < 1 || hidden > 4096) return sdca_fail(SDCA_E_ARG, "hidden width %d out of range", hidden);
+    if (hidden < 32 || hidden > 1024 || hidden % 32 != 0)
+        return sdca_fail(SDCA_E_ARG, "hidden width %d: a multiple of 32 in [32, 1024] expected", hidden);
     delete c->mlp;
     c->mlp = new sdca::Mlp();
     sdca::Mlp *m = c->mlp;
-    m->din = c->q * c->q + 2 * c->q + 1;
+    m->din = 2 * c->q * c->q + 2 * c->q;
     m->dh = hidden;
     m->dout = c->q * c->q;
+    m->DP = (int)round32(m->din);
+    m->DO = (int)round32(m->dout);
     m->np = offsets(m->din, m->dh, m->dout).n;
     int rc;
     if ((rc = walloc(&m->d_p, m->np)) || (rc = walloc(&m->d_g, m->np)) || (rc = walloc(&m->d_m, m->np)) ||
-        (rc = walloc(&m->d_v, m->np)))
+        (rc = walloc(&m->d_v, m->np)) || (rc = walloc(&m->d_W1b, (size_t)m->dh * m->DP)) ||
+        (rc = walloc(&m->d_W2b, (size_t)m->dh * m->dh)) || (rc = walloc(&m->d_W2t, (size_t)m->dh * m->dh)) ||
+        (rc = walloc(&m->d_W3b, (size_t)m->DO * m->dh)) || (rc = walloc(&m->d_W3t, (size_t)m->dh * m->DO)))
         return rc;
     return SDCA_OK;
 }
@@ -202,11 +322,17 @@ void fields_j0(const sdca_ctx *c, int r, double *h)
 
 sdca::Mlp::~Mlp()
 {
-    free_scratch(this);
+    free_forward(this);
+    free_training(this);
     wfree(d_p);
     wfree(d_g);
     wfree(d_m);
     wfree(d_v);
+    wfree(d_W1b);
+    wfree(d_W2b);
+    wfree(d_W2t);
+    wfree(d_W3b);
+    wfree(d_W3t);
 }
 
 extern "C" {
@@ -263,7 +389,7 @@ int sdca_mlp_forward(sdca_ctx *c, const float *x, int64_t n, float *y)
     for (int64_t r0 = 0; r0 < n; r0 += chunk) {
         const int64_t cnt = std::min(chunk, n - r0);
         WTRY(hipMemcpyAsync(m->d_x, x + r0 * m->din, cnt * m->din * 4, hipMemcpyHostToDevice, c->stream));
-        forward(c->stream, m, m->d_x, cnt);
+        forward(c->stream, m, m->d_x, cnt, false);
         WTRY(hipGetLastError());
         WTRY(hipMemcpyAsync(y + r0 * m->dout, m->d_y, cnt * m->dout * 4, hipMemcpyDeviceToHost, c->stream));
         WTRY(hipStreamSynchronize(c->stream));
@@ -278,7 +404,7 @@ int sdca_mlp_loss_grad(sdca_ctx *c, const float *x, const float *t, const float 
     if (rc || (rc = need_mlp(c))) return rc;
     if (n < 1 || !x || !t || !loss || !grad) return sdca_fail(SDCA_E_ARG, "bad arguments");
     sdca::Mlp *m = c->mlp;
-    if ((rc = ensure_scratch(m, n))) return rc;
+    if ((rc = ensure_training(m, n))) return rc;
     WTRY(hipMemcpyAsync(m->d_x, x, n * m->din * 4, hipMemcpyHostToDevice, c->stream));
     WTRY(hipMemcpyAsync(m->d_t, t, n * m->dout * 4, hipMemcpyHostToDevice, c->stream));
     double count = (double)n * m->dout;
@@ -289,8 +415,8 @@ int sdca_mlp_loss_grad(sdca_ctx *c, const float *x, const float *t, const float 
     }
     if (count <= 0.0) return sdca_fail(SDCA_E_ARG, "mask selects no outputs");
     WTRY(hipMemsetAsync(m->d_loss, 0, 8, c->stream));
-    forward(c->stream, m, m->d_x, n);
-    backward(c->stream, m, m->d_x, m->d_t, mask ? m->d_msk : nullptr, n, (float)count);
+    forward(c->stream, m, m->d_x, n, true);
+    backward(c->stream, m, m->d_t, mask ? m->d_msk : nullptr, n, (float)count);
     WTRY(hipGetLastError());
     double l = 0.0;
     WTRY(hipMemcpyAsync(&l, m->d_loss, 8, hipMemcpyDeviceToHost, c->stream));
@@ -308,7 +434,7 @@ int sdca_mlp_train(sdca_ctx *c, const float *x, const float *t, const float *mas
     if (n < 1 || !x || !t || epochs < 1 || batch < 1 || !(lr > 0.f)) return sdca_fail(SDCA_E_ARG, "bad arguments");
     sdca::Mlp *m = c->mlp;
     const int64_t nb = std::min<int64_t>(batch, n);
-    if ((rc = ensure_scratch(m, nb))) return rc;
+    if ((rc = ensure_training(m, nb))) return rc;
     float *dX = nullptr, *dT = nullptr, *dM = nullptr;
     auto cleanup = [&]() {
         wfree(dX);
@@ -356,11 +482,12 @@ int sdca_mlp_train(sdca_ctx *c, const float *x, const float *t, const float *mas
             else count = (double)cnt * m->dout;
             if (count <= 0.0) continue;
             total += count;
-            forward(c->stream, m, m->d_x, cnt);
-            backward(c->stream, m, m->d_x, m->d_t, m->d_msk, cnt, (float)count);
+            forward(c->stream, m, m->d_x, cnt, true);
+            backward(c->stream, m, m->d_t, m->d_msk, cnt, (float)count);
             m->step += 1;
             const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
             sdca::kern::adam(c->stream, m->d_p, m->d_g, m->d_m, m->d_v, m->np, lr, b1, b2, eps, c1, c2);
+            m->packed = false;
             e = hipGetLastError();
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // perm slice is read asynchronously
         }
@@ -381,7 +508,7 @@ int sdca_pair_features(sdca_ctx *c, const int32_t *pairs, int64_t npairs, float 
     if (rc) return rc;
     if (npairs < 0 || (npairs > 0 && (!pairs || !feats))) return sdca_fail(SDCA_E_ARG, "bad arguments");
     if (npairs == 0) return SDCA_OK;
-    const int q = c->q, D = q * q + 2 * q + 1;
+    const int q = c->q, D = 2 * q * q + 2 * q;
     std::vector<int32_t> counts((size_t)npairs * q * q);
     if ((rc = sdca_pair_counts(c, pairs, npairs, counts.data()))) return rc;
     int32_t *d_c = nullptr;
@@ -420,7 +547,7 @@ int sdca_solve_loci_mlp(sdca_ctx *c, const int32_t *loci, int32_t K, int32_t npa
     if (K == 0) return SDCA_OK;
     const int q = c->q, P = npartners;
     sdca::Mlp *m = c->mlp;
-    if (m->din != q * q + 2 * q + 1 || m->dout != q * q) return sdca_fail(SDCA_E_STATE, "network shape does not match q");
+    if (m->din != 2 * q * q + 2 * q || m->dout != q * q) return sdca_fail(SDCA_E_STATE, "network shape does not match q");
     // partners: the context's MI top-K table (computed here if missing or too short)
     if (!c->d_topk || c->topk_K < P) {
         std::vector<int32_t> tmp((size_t)c->L * P);
@@ -482,7 +609,7 @@ int sdca_solve_loci_mlp(sdca_ctx *c, const int32_t *loci, int32_t K, int32_t npa
         if (e != hipSuccess) break;
         sdca::kern::pair_counts(c->stream, c->d_SP, q, W, d_pairs, cnt, d_cnt);
         sdca::kern::pair_features(c->stream, d_cnt, cnt, q, c->B, d_feat);
-        forward(c->stream, m, d_feat, cnt);
+        forward(c->stream, m, d_feat, cnt, false);
         e = hipMemcpyAsync(d_J + p0 * q * q, m->d_y, cnt * q * q * 4, hipMemcpyDeviceToDevice, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);  // pairs is reused
     }

@@ -35,3 +35,13 @@ def c1():
     d = synth.workload("C1")
     states, loci = sd.prepare_alignment(d["chars"])
     return {"chars": d["chars"], "states": states, "loci": loci, "pairs": d["pairs"]}
+
+
+@pytest.fixture(scope="session")
+def c4_states():
+    """The genome-scale C4 workload (B=3000, N=94,000) as q_env = 5 states."""
+    import superdca_amd as sd
+    from superdca_amd import synth
+
+    st, _ = sd.prepare_alignment(synth.workload("C4")["chars"], q=5)
+    return st

@@ -124,12 +124,6 @@ def test_c2_top_pairs_against_oracle_fixture(kw):
 GOLDEN_C4 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c4_rows64.npz")
 
 
-@pytest.fixture(scope="module")
-def c4_states():
-    st, _ = sd.prepare_alignment(synth.workload("C4")["chars"], q=5)
-    return st
-
-
 GOLDEN_C4W = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c4_weights.npz")
 
 

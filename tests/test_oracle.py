@@ -346,6 +346,80 @@ def test_mlp_reference_gradient_finite_differences():
     assert loss == pytest.approx((((y - t) * mask) ** 2).sum() / mask.sum(), rel=1e-14)
 
 
+def test_bf16_rounding_matches_torch():
+    import torch
+    from oracle import mlp_ref
+
+    rng = np.random.default_rng(21)
+    x = np.concatenate([rng.normal(size=4000).astype(np.float32) * 10.0 ** rng.integers(-30, 30, 4000),
+                        np.array([0.0, -0.0, 1.0, np.inf, -np.inf, 3.4e38, 1e-40], dtype=np.float32)])
+    # halfway cases: low 16 bits exactly 0x8000 round to the even neighbour
+    half = (np.arange(1, 200, dtype=np.uint32) << 16 | 0x8000).view(np.float32)
+    x = np.concatenate([x, half, -half]).astype(np.float32)
+    want = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    np.testing.assert_array_equal(mlp_ref.bf16(x), want)
+    assert np.isnan(mlp_ref.bf16(np.array([np.nan], dtype=np.float32))[0])
+
+
+def test_mlp_bf16_gradient_close_to_fp64():
+    """The BF16 rounding points change the loss and gradient by BF16's relative precision,
+    not more: a dropped rounding point or a transposed operand would not stay this close."""
+    from oracle import mlp_ref
+
+    rng = np.random.default_rng(17)
+    din, dh, dout, n = 40, 64, 16, 300
+    npar = dh * din + dh + dh * dh + dh + dout * dh + dout
+    p = rng.normal(size=npar) * 0.15
+    x = rng.random((n, din))
+    t = rng.normal(size=(n, dout)) * 0.1
+    mask = (rng.random((n, dout)) < 0.8).astype(float)
+    l64, g64 = mlp_ref.loss_grad(p, x, t, mask, din, dh, dout)
+    l16, g16 = mlp_ref.loss_grad_bf16(p, x, t, mask, din, dh, dout)
+    assert l16 == pytest.approx(l64, rel=2e-2)
+    assert np.linalg.norm(g16 - g64) < 2e-2 * np.linalg.norm(g64)
+    # every weight matrix separately, so a wrong block cannot hide behind the others
+    for a, b in ((0, dh * din), (dh * din + dh, dh * din + dh + dh * dh),
+                 (npar - dout - dout * dh, npar - dout)):
+        assert np.linalg.norm(g16[a:b] - g64[a:b]) < 3e-2 * np.linalg.norm(g64[a:b])
+    y16, _ = mlp_ref.forward_bf16(p, x, din, dh, dout)
+    y64, _ = mlp_ref.forward(p, x, din, dh, dout)
+    assert np.abs(y16 - y64).max() < 2e-2 * np.abs(y64).max()
+
+
+def test_mean_field_block_closed_form_and_invariants():
+    from oracle import mlp_ref
+
+    # q = 2: one state per locus, C = [[c11, c12], [c12, c22]], J = c12 / (c11 c22 - c12^2)
+    N = np.array([[30, 10], [5, 55]])
+    B, lam = 100, 0.5
+    f = N / B
+    fij = (1 - lam) * f + lam / 4
+    fr = (1 - lam) * f.sum(1) + lam / 2
+    fj = (1 - lam) * f.sum(0) + lam / 2
+    c11, c22 = fr[0] * (1 - fr[0]), fj[0] * (1 - fj[0])
+    c12 = fij[0, 0] - fr[0] * fj[0]
+    J = mlp_ref.mean_field_block(N, B, 2)
+    assert J[0, 0] == pytest.approx(c12 / (c11 * c22 - c12 ** 2), rel=1e-12)
+    assert J[1, :].tolist() == [0.0, 0.0] and J[:, 1].tolist() == [0.0, 0.0]
+    # independent pair (joint = product of marginals) without pseudocount: no coupling
+    # (the pseudocount mixes in a uniform table, which is not a product of these marginals)
+    rng = np.random.default_rng(3)
+    for q in (3, 4, 5):
+        a = rng.integers(1, 9, q)
+        b = rng.integers(1, 9, q)
+        Nind = np.outer(a, b)
+        assert np.abs(mlp_ref.mean_field_block(Nind, int(Nind.sum()), q, pseudocount=0.0)).max() < 1e-9
+        assert np.abs(mlp_ref.mean_field_block(Nind, int(Nind.sum()), q)).max() > 1e-3
+        # swapping the pair transposes the block
+        Nr = rng.integers(0, 40, (q, q))
+        Jrj = mlp_ref.mean_field_block(Nr, int(Nr.sum()), q)
+        Jjr = mlp_ref.mean_field_block(Nr.T, int(Nr.sum()), q)
+        np.testing.assert_allclose(Jjr, Jrj.T, rtol=1e-10, atol=1e-12)
+    # a positively associated pair gets a positive coupling on the shared state
+    Npos = np.array([[40, 2, 1], [2, 40, 1], [1, 1, 12]])
+    assert mlp_ref.mean_field_block(Npos, int(Npos.sum()), 3)[0, 0] > 0
+
+
 # ---------------------------------------------------------- norm of mean (float path)
 
 def test_float_gauge_hand_computed_block():
