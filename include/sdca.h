@@ -86,6 +86,16 @@ typedef struct sdca_opts {
                                 group's slots (idle slots return at once inside the kernels); the
                                 round's device time is then reported whole, in lbfgs_ms.  0 (default):
                                 the kernels are launched each round over the active slots */
+    int32_t subspace_rank;   /* Gram formulation, one alignment, no scan: > 0 (128, the only rank
+                                supported) warm-starts every locus from its solve restricted to
+                                c = Q a, Q an orthonormal basis of the dominant 128-dimensional
+                                eigen-subspace of the identical-locus counts (power iteration,
+                                once per alignment); the restricted solve costs two B x 128 x q
+                                products per evaluation instead of one B x B x q product.  The
+                                optimum is unchanged (strictly convex); the full solve's iterations
+                                drop.  0 (default): cold start from zero */
+    double subspace_threshold; /* the restricted solve stops at ||g||_2 < this in its own metric
+                                (default 3e-2), after at most max_iter iterations */
 } sdca_opts;
 
 /* Formulations of the per-locus solve (sdca_opts.formulation).
@@ -125,7 +135,7 @@ typedef struct sdca_locus_stats {
     double gnorm_J;        /* ||g||_inf over the coupling entries J_r (the fields h_r excluded);
                               under SDCA_FORM_GRAM gnorm and gnorm_J are 2-norms */
     int32_t nfeval_fine;   /* evaluations at the fine level (FP32-equivalent contractions) */
-    int32_t reserved;
+    int32_t nfeval_sub;    /* evaluations of the subspace warm start (sdca_opts.subspace_rank) */
 } sdca_locus_stats;
 
 /* Cumulative device timings of a context (milliseconds, HIP events). */
@@ -147,6 +157,9 @@ typedef struct sdca_timing {
     double dots_ms;          /* gradient dot products + reduction (part of lbfgs_ms) */
     double bytes_update;     /* algorithmic HBM bytes of the update pass */
     double bytes_dots;       /* algorithmic HBM bytes of the dot-product pass */
+    double subspace_ms;      /* rounds of the subspace warm start (sdca_opts.subspace_rank) */
+    int64_t n_sub_evals;     /* their batched evaluations (rounds) */
+    double flops_sub;        /* their algorithmic flops: 4 * B * 128 * q per locus-evaluation */
 } sdca_timing;
 
 /* Message of the calling thread's last failure (SuperDCA prints its errors to

@@ -68,13 +68,14 @@ class Opts(C.Structure):
                 ("gemm_ksplit", C.c_int32), ("gemm_gm_energy", C.c_int32), ("gemm_gm_grad", C.c_int32),
                 ("gemm_tn_energy", C.c_int32), ("gemm_tn_grad", C.c_int32), ("pipeline_groups", C.c_int32),
                 ("coarse_factor", C.c_int32), ("fine_factor", C.c_int32), ("j_threshold", C.c_double),
-                ("formulation", C.c_int32), ("graph_rounds", C.c_int32)]
+                ("formulation", C.c_int32), ("graph_rounds", C.c_int32), ("subspace_rank", C.c_int32),
+                ("subspace_threshold", C.c_double)]
 
 
 class LocusStats(C.Structure):
     _fields_ = [("fval", C.c_double), ("gnorm", C.c_double), ("nfeval", C.c_int32), ("iter", C.c_int32),
                 ("status", C.c_int32), ("nfeval_first", C.c_int32), ("gnorm_J", C.c_double),
-                ("nfeval_fine", C.c_int32), ("reserved", C.c_int32)]
+                ("nfeval_fine", C.c_int32), ("nfeval_sub", C.c_int32)]
 
 
 class Timing(C.Structure):
@@ -83,7 +84,8 @@ class Timing(C.Structure):
                 ("total_ms", C.c_double), ("n_evals", C.c_int64), ("n_energy_launches", C.c_int64),
                 ("n_grad_launches", C.c_int64), ("flops_energy", C.c_double), ("flops_grad", C.c_double),
                 ("bytes_lbfgs", C.c_double), ("update_ms", C.c_double), ("dots_ms", C.c_double),
-                ("bytes_update", C.c_double), ("bytes_dots", C.c_double)]
+                ("bytes_update", C.c_double), ("bytes_dots", C.c_double), ("subspace_ms", C.c_double),
+                ("n_sub_evals", C.c_int64), ("flops_sub", C.c_double)]
 
 
 _lib = None
@@ -281,6 +283,7 @@ class Context:
             "nfeval_first": np.array([st[k].nfeval_first for k in range(K)], dtype=np.int64),
             "gnorm_J": np.array([st[k].gnorm_J for k in range(K)]),
             "nfeval_fine": np.array([st[k].nfeval_fine for k in range(K)], dtype=np.int64),
+            "nfeval_sub": np.array([st[k].nfeval_sub for k in range(K)], dtype=np.int64),
         }
         return S, stats, X, rc
 

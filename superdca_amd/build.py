@@ -23,7 +23,7 @@ ARCH = "gfx950"
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-LIB_SOURCES = ["sdca_kernels.hip", "sdca_gram.hip", "sdca_pairs.hip", "sdca_mlp.hip", "sdca_noalloc.hip", "sdca_api.cpp", "sdca_warm.cpp",
+LIB_SOURCES = ["sdca_kernels.hip", "sdca_gram.hip", "sdca_subspace.hip", "sdca_pairs.hip", "sdca_mlp.hip", "sdca_noalloc.hip", "sdca_api.cpp", "sdca_warm.cpp",
                "sdca_host.cpp", "sdca_store.cpp"]
 
 

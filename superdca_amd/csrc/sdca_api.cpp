@@ -30,6 +30,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -179,6 +180,11 @@ sdca_ctx::~sdca_ctx()
     delete gram;
     dfree(d_gram);
     dfree(d_w64);
+    dfree(d_subQ);
+    dfree(d_subQt);
+    dfree(d_subKQ);
+    dfree(d_subKQt);
+    dfree(d_subMinv);
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &g : gev)
@@ -191,6 +197,13 @@ sdca_ctx::~sdca_ctx()
 
 sdca::GramWs::~GramWs()
 {
+    dfree(Vr);
+    dfree(aux);
+    dfree(Ap);
+    dfree(Zr);
+    dfree(Wc);
+    dfree(Wst);
+    dfree(d_galn_r);
     dfree(V);
     dfree(Gp);
     dfree(Zp);
@@ -975,6 +988,7 @@ sdca::SlotCmd idle_cmd()
     k.mode = sdca::MODE_IDLE;
     k.locus = -1;
     k.level = sdca::LV_SPLIT;
+    k.item = -1;
     return k;
 }
 
@@ -1029,14 +1043,22 @@ constexpr int kMaxRestarts = 2;
 // evaluations to two (C4: median 62 instead of 63 evaluations per locus).
 constexpr double kFirstStep = 1.0 / 16.0;
 
+// Stopping rule of a Solver other than the context's options (the subspace warm start).
+struct SolverLimits {
+    double grad_threshold, j_threshold;
+    int max_iter;
+};
+
 class Solver {
 public:
     // gram: FP64 evaluations of the Gram formulation (one precision level, FP64 rounding of f)
-    Solver(sdca_ctx *c, int nslots, int m, bool gram)
+    Solver(sdca_ctx *c, int nslots, int m, bool gram, const SolverLimits *lim = nullptr)
         : c_(c), n_(nslots), m_(m), slots_(nslots), cmds_(nslots, idle_cmd()),
           split_(!gram && c->opts.precision == SDCA_PREC_SPLIT_BF16), gram_(gram),
+          gthr_(lim ? lim->grad_threshold : c->opts.grad_threshold),
+          max_iter_(lim ? lim->max_iter : c->opts.max_iter),
           coarse_gnorm_(split_ && c->opts.coarse_factor > 0 ? c->opts.coarse_factor * c->opts.grad_threshold : 0.0),
-          jthr_(c->opts.j_threshold),
+          jthr_(lim ? lim->j_threshold : c->opts.j_threshold),
           fine_gnorm_(split_ && jthr_ > 0.0 ? c->opts.fine_factor * jthr_ : 0.0),
           max_level_(jthr_ > 0.0 && (!split_ || c->opts.fine_factor > 0) ? sdca::LV_FINE : sdca::LV_SPLIT)
     {
@@ -1057,7 +1079,7 @@ public:
     // at the highest level when one is set.
     bool converged(const Slot &S, double ginf, double gJ) const
     {
-        if (!(ginf < c_->opts.grad_threshold) || S.level < sdca::LV_SPLIT) return false;
+        if (!(ginf < gthr_) || S.level < sdca::LV_SPLIT) return false;
         return jthr_ <= 0.0 || (S.level == max_level_ && gJ < jthr_);
     }
 
@@ -1086,6 +1108,7 @@ public:
         k.self_lo = A->scan_split > 0 ? 0 : A->col_off[r];
         k.self_hi = A->scan_split > 0 ? A->col_off[A->scan_split] : A->col_off[r + 1];
         k.init_src = warm ? 1.f : 0.f;
+        k.item = item;
     }
 
     void set_idle(int i)
@@ -1250,7 +1273,7 @@ public:
             return false;
         }
         if (converged(S, ginf, gJ)) return finish(i, SDCA_ST_CONVERGED, true);
-        if (S.iter >= c_->opts.max_iter) return finish(i, SDCA_ST_ITER_CAP, true);
+        if (S.iter >= max_iter_) return finish(i, SDCA_ST_ITER_CAP, true);
         newdir(i, true, pnew, stp_trial);
         return false;
     }
@@ -1416,6 +1439,8 @@ private:
     std::vector<sdca::SlotCmd> cmds_;
     bool split_;  // split-BF16 contractions (else FP32)
     bool gram_;   // Gram formulation: FP64 evaluations
+    double gthr_;   // ||g|| threshold of the stopping rule
+    int max_iter_;  // iteration cap per locus
     // split precision: coarse phase (BF16 high parts only) while ||g||_inf >= this; 0: none
     double coarse_gnorm_;
     double jthr_;        // coupling-gradient threshold (0: none)
@@ -1450,6 +1475,7 @@ int ensure_gram_matrix(sdca_ctx *A, hipStream_t st)
     if (A->d_gram && A->gram_split == j0) return SDCA_OK;
     dfree(A->d_gram);
     A->gram_split = -1;
+    A->sub_rank = 0;  // the subspace basis belongs to the old counts
     int rc;
     if ((rc = dalloc(&A->d_gram, (size_t)A->Bp * A->Bp))) return rc;
     const int W = (A->L + 63) / 64;
@@ -1633,7 +1659,7 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     sdca::GramWs *G = c->gram;
     const int m = G->m, W = sdca::ResLayout{m}.width();
     const int nmem = (int)c->members.size();
-    if (c->opts.graph_rounds && nmem == 1) return gram_launch_graph(c, g, cmds);
+    if (c->opts.graph_rounds && nmem == 1 && !G->warm_rounds) return gram_launch_graph(c, g, cmds);
     const int s0 = g * G->gs, s1 = std::min<int>((int)cmds.size(), s0 + G->gs);
     std::vector<int> &lst = G->cmd_slot[g];
     lst.clear();
@@ -1667,6 +1693,11 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     double *dot = G->dot_part + (size_t)s0 * G->nchd * sdca::ResLayout{m}.ndots(), *res = G->res + (size_t)s0 * W;
     HIP_TRY(hipMemcpyAsync(dc, hc, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(ev[0], st));
+    if (G->warm_rounds) {  // staged starts of the round's new loci into their x_c
+        const sdca_ctx *A = c->members[0];
+        sdca::sub::warm(st, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, A->d_subQt, A->d_subKQt, A->d_subQ,
+                        sdca::kSubRank, G->Wst, G->wstride);
+    }
     sdca::gram::step(st, m, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, Gp, G->ldG, upd, sm, G->nchunk, G->nchq);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[1], st));
@@ -1706,7 +1737,7 @@ int gram_finish_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     }
     hipEvent_t *ev = G->ev[g];
     auto &t = c->timing;
-    const bool graphed = c->opts.graph_rounds && c->members.size() == 1;
+    const bool graphed = c->opts.graph_rounds && c->members.size() == 1 && !G->warm_rounds;
     if (graphed) {
         t.lbfgs += elapsed(ev[0], ev[3]);  // the whole replayed round
     } else {
@@ -1779,7 +1810,7 @@ int gram_stage(sdca_ctx *c, int g, const Solver &S, const std::vector<int> &fin,
             o.nfeval_first = sl.nfeval_first;
             o.gnorm_J = sl.gnormJ;
             o.nfeval_fine = 0;
-            o.reserved = 0;
+            o.nfeval_sub = G->sub_nev ? G->sub_nev[sl.item] : 0;
         }
     }
     HIP_TRY(hipEventRecord(G->stage_ev[g], st));
@@ -1895,6 +1926,361 @@ int gram_flush(sdca_ctx *c, int g, bool force, float *score_rows, const int64_t 
     return SDCA_OK;
 }
 
+// ------------------------------------------------ subspace warm start (Gram formulation)
+//
+// Each locus is first solved with its coefficients restricted to c = Q a, Q an orthonormal
+// basis of the dominant kSubRank-dimensional eigen-subspace of K (sdca_subspace.hip): the
+// same host Solver on slot vectors of q x kSubRank coefficients, their images under
+// M_r = Q^T K_r Q and the fields, stopping at ||g||_2 < subspace_threshold.  The restricted
+// optima (a*, h*) are staged per locus; the full solve then starts each locus at
+// c = Q a*, K_r c = K Q a* - U V_r a* (k_gwarm) instead of zero.
+
+bool subspace_wanted(const sdca_ctx *c)
+{
+    if (c->opts.subspace_rank <= 0 || c->members.size() != 1 || c->scan_split > 0) return false;
+    const sdca_ctx *A = c->members[0];
+    return A->scan_split == 0 && A->B >= 2 * sdca::kSubRank;
+}
+
+// Orthonormalises the kp rows of Q (row stride ld, n entries each) in place: modified
+// Gram-Schmidt, twice.
+void mgs_rows(std::vector<double> &Q, int kp, int n, int64_t ld)
+{
+    for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < kp; ++i) {
+            double *qi = Q.data() + (size_t)i * ld;
+            for (int j = 0; j < i; ++j) {
+                const double *qj = Q.data() + (size_t)j * ld;
+                double r = 0.0;
+                for (int b = 0; b < n; ++b) r += qi[b] * qj[b];
+                for (int b = 0; b < n; ++b) qi[b] -= r * qj[b];
+            }
+            double nn = 0.0;
+            for (int b = 0; b < n; ++b) nn += qi[b] * qi[b];
+            const double inv = nn > 0.0 ? 1.0 / std::sqrt(nn) : 0.0;
+            for (int b = 0; b < n; ++b) qi[b] *= inv;
+        }
+}
+
+// The basis of A's identical-locus counts: power iteration Q <- orth(K Q) (three products
+// on the device from a seeded Gaussian start), then K Q and (Q^T K Q)^-1 (Cholesky on the
+// host).  Returns SDCA_OK, or SDCA_E_STATE when Q^T K Q is not positive definite.
+int ensure_subspace(sdca_ctx *A, hipStream_t st)
+{
+    constexpr int kp = sdca::kSubRank;
+    if (A->sub_rank == kp) return SDCA_OK;
+    const int B = A->B, Bp = A->Bp;
+    std::vector<double> Qt((size_t)kp * Bp, 0.0), KQt((size_t)kp * Bp, 0.0);
+    std::mt19937_64 gen(0x5d5ca11);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for (int i = 0; i < kp; ++i)
+        for (int b = 0; b < B; ++b) Qt[(size_t)i * Bp + b] = nd(gen);
+    mgs_rows(Qt, kp, B, Bp);
+    double *dQt = nullptr, *dY = nullptr;
+    int rc;
+    if ((rc = dalloc(&dQt, (size_t)kp * Bp)) || (rc = dalloc(&dY, (size_t)kp * Bp))) {
+        dfree(dQt);
+        return rc;
+    }
+    auto product = [&](std::vector<double> &out) -> hipError_t {
+        hipError_t e = hipMemcpyAsync(dQt, Qt.data(), Qt.size() * sizeof(double), hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return e;
+        // Y^T[i][b] = sum_b' Qt[i][b'] K[b][b']
+        sdca::gram::gemm(st, A->d_gram, Bp, dQt, Bp, dY, Bp, Bp, kp, Bp);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(out.data(), dY, out.size() * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        return e;
+    };
+    hipError_t e = hipSuccess;
+    for (int it = 0; it < 3 && e == hipSuccess; ++it) {
+        e = product(KQt);
+        if (e == hipSuccess) {
+            Qt.swap(KQt);
+            mgs_rows(Qt, kp, B, Bp);
+        }
+    }
+    if (e == hipSuccess) e = product(KQt);
+    dfree(dQt);
+    dfree(dY);
+    if (e != hipSuccess) return fail(SDCA_E_HIP, "subspace basis failed: %s", hipGetErrorString(e));
+    // M = Q^T K Q (symmetrised), its Cholesky factor and inverse
+    std::vector<double> M((size_t)kp * kp), Lc((size_t)kp * kp, 0.0), Mi((size_t)kp * kp, 0.0);
+    for (int i = 0; i < kp; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double v = 0.0, w = 0.0;
+            for (int b = 0; b < B; ++b) {
+                v += Qt[(size_t)i * Bp + b] * KQt[(size_t)j * Bp + b];
+                w += Qt[(size_t)j * Bp + b] * KQt[(size_t)i * Bp + b];
+            }
+            M[(size_t)i * kp + j] = M[(size_t)j * kp + i] = 0.5 * (v + w);
+        }
+    for (int j = 0; j < kp; ++j) {
+        double d = M[(size_t)j * kp + j];
+        for (int k = 0; k < j; ++k) d -= Lc[(size_t)j * kp + k] * Lc[(size_t)j * kp + k];
+        if (!(d > 0.0)) return fail(SDCA_E_STATE, "Q^T K Q is not positive definite");
+        Lc[(size_t)j * kp + j] = std::sqrt(d);
+        for (int i = j + 1; i < kp; ++i) {
+            double v = M[(size_t)i * kp + j];
+            for (int k = 0; k < j; ++k) v -= Lc[(size_t)i * kp + k] * Lc[(size_t)j * kp + k];
+            Lc[(size_t)i * kp + j] = v / Lc[(size_t)j * kp + j];
+        }
+    }
+    for (int col = 0; col < kp; ++col) {  // M^-1 e_col: forward then back substitution
+        std::vector<double> y(kp, 0.0), x(kp, 0.0);
+        for (int i = 0; i < kp; ++i) {
+            double v = i == col ? 1.0 : 0.0;
+            for (int k = 0; k < i; ++k) v -= Lc[(size_t)i * kp + k] * y[k];
+            y[i] = v / Lc[(size_t)i * kp + i];
+        }
+        for (int i = kp - 1; i >= 0; --i) {
+            double v = y[i];
+            for (int k = i + 1; k < kp; ++k) v -= Lc[(size_t)k * kp + i] * x[k];
+            x[i] = v / Lc[(size_t)i * kp + i];
+        }
+        for (int i = 0; i < kp; ++i) Mi[(size_t)i * kp + col] = x[i];
+    }
+    for (int i = 0; i < kp; ++i)  // exactly symmetric (the kernels read it both ways)
+        for (int j = 0; j < i; ++j) Mi[(size_t)i * kp + j] = Mi[(size_t)j * kp + i] =
+            0.5 * (Mi[(size_t)i * kp + j] + Mi[(size_t)j * kp + i]);
+    std::vector<double> Q((size_t)Bp * kp), KQ((size_t)Bp * kp);
+    for (int i = 0; i < kp; ++i)
+        for (int b = 0; b < Bp; ++b) {
+            Q[(size_t)b * kp + i] = Qt[(size_t)i * Bp + b];
+            KQ[(size_t)b * kp + i] = KQt[(size_t)i * Bp + b];
+        }
+    // the gradient product's operand: rows [0, kp) (K Q)^T, rows [kp, 2 kp) (K Q M^-1)^T =
+    // M^-1 (K Q)^T (M^-1 symmetric)
+    KQt.resize((size_t)2 * kp * Bp, 0.0);
+    for (int i = 0; i < kp; ++i) {
+        double *dst = KQt.data() + (size_t)(kp + i) * Bp;
+        for (int j = 0; j < kp; ++j) {
+            const double mij = Mi[(size_t)i * kp + j];
+            const double *src = KQt.data() + (size_t)j * Bp;
+            for (int b = 0; b < B; ++b) dst[b] += mij * src[b];
+        }
+    }
+    dfree(A->d_subQ);
+    dfree(A->d_subQt);
+    dfree(A->d_subKQ);
+    dfree(A->d_subKQt);
+    dfree(A->d_subMinv);
+    if ((rc = dalloc(&A->d_subQ, Q.size())) || (rc = dalloc(&A->d_subQt, Qt.size())) ||
+        (rc = dalloc(&A->d_subKQ, KQ.size())) || (rc = dalloc(&A->d_subKQt, KQt.size())) ||
+        (rc = dalloc(&A->d_subMinv, Mi.size())))
+        return rc;
+    HIP_TRY(hipMemcpy(A->d_subQ, Q.data(), Q.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(A->d_subQt, Qt.data(), Qt.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(A->d_subKQ, KQ.data(), KQ.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(A->d_subKQt, KQt.data(), KQt.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(A->d_subMinv, Mi.data(), Mi.size() * sizeof(double), hipMemcpyHostToDevice));
+    A->sub_rank = kp;
+    return SDCA_OK;
+}
+
+// Restricted workspace of the Gram workspace (slot vectors, per-slot constants, packed
+// operands) and room for K staged optima.
+int ensure_sub_ws(sdca_ctx *c, int32_t K)
+{
+    constexpr int kp = sdca::kSubRank;
+    sdca::GramWs *G = c->gram;
+    const sdca_ctx *A = c->members[0];
+    int rc;
+    if (!G->Vr) {
+        G->nvs_r = sdca::gram::vec_len(G->gq, kp);
+        G->vstride_r = (sdca::gram::kVecPairs + 2 * G->m) * G->nvs_r;
+        G->auxstride = 16 * kp + 64;
+        G->nchq_r = (int)((G->gq * kp + 255) / 256);
+        G->nchd_r = sdca::gram::dots_chunks((int64_t)G->gq * kp);
+        const int64_t ncols = G->colcap * G->ngroups;
+        const int npad = G->gs * G->ngroups;
+        if ((rc = dalloc(&G->Vr, (size_t)G->slots * G->vstride_r)) || (rc = dalloc(&G->aux, (size_t)G->slots * G->auxstride)) ||
+            (rc = dalloc(&G->Ap, (size_t)ncols * kp)) ||
+            (rc = dalloc(&G->Zr, (size_t)ncols * 2 * kp * ((G->gBp + sdca::kSubKc - 1) / sdca::kSubKc))) ||
+            (rc = dalloc(&G->Wc, (size_t)npad * 64)) || (rc = dalloc(&G->d_galn_r, 1)))
+            return rc;
+    }
+    // the restricted descriptor: vectors of q x kp coefficients, every "sequence" valid, no
+    // own-column term in the products pass (the gradient kernel subtracts it)
+    sdca::GramAln g{};
+    g.K = nullptr;
+    g.statesT = A->d_statesT;
+    g.w = A->d_w64;
+    g.B = kp;
+    g.Bp = kp;
+    g.q = A->q;
+    g.own = 0;
+    g.lamJ2 = 2.0 * A->lambda_J;
+    g.lamh2 = 2.0 * A->lambda_h;
+    HIP_TRY(hipMemcpy(G->d_galn_r, &g, sizeof(g), hipMemcpyHostToDevice));
+    G->wstride = (int64_t)G->gq * kp + 8;
+    if (G->wcap < K) {
+        dfree(G->Wst);
+        if ((rc = dalloc(&G->Wst, (size_t)std::max(K, 1) * G->wstride))) return rc;
+        G->wcap = K;
+    }
+    return SDCA_OK;
+}
+
+// One restricted round of slot group g (the layout of gram_launch_round; the packed
+// energies and beliefs reuse the Gram workspace's Zp and Gp).
+int sub_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds)
+{
+    constexpr int kp = sdca::kSubRank;
+    sdca::GramWs *G = c->gram;
+    const sdca_ctx *A = c->members[0];
+    const int m = G->m, W = sdca::ResLayout{m}.width();
+    const int s0 = g * G->gs, s1 = std::min<int>((int)cmds.size(), s0 + G->gs);
+    std::vector<int> &lst = G->cmd_slot[g];
+    lst.clear();
+    sdca::SlotCmd *hc = G->h_cmds + s0;
+    int col = 0;
+    for (int i = s0; i < s1; ++i) {
+        if (cmds[i].mode == sdca::MODE_IDLE) continue;
+        sdca::SlotCmd &k = hc[lst.size()];
+        k = cmds[i];
+        k.slot = i;
+        k.row = col;
+        k.q = A->q;
+        lst.push_back(i);
+        col += A->q;
+    }
+    const int nc = (int)lst.size();
+    if (nc == 0) return SDCA_OK;
+    hipStream_t st = c->gstream[g];
+    hipEvent_t *ev = G->ev[g];
+    sdca::SlotCmd *dc = G->d_cmds + s0;
+    double *Gp = G->Gp + (size_t)g * G->colcap * G->ldG, *Ep = G->Zp + (size_t)g * G->colcap * G->ldG;
+    double *Ap = G->Ap + (size_t)g * G->colcap * kp, *Zr = G->Zr + (size_t)g * G->colcap * 2 * kp;
+    double *Wc = G->Wc + (size_t)s0 * 64;
+    double *upd = G->upd_part + (size_t)s0 * (G->nchq + 1) * 3, *sm = G->sm_part + (size_t)s0 * G->nchunk * 65;
+    double *dot = G->dot_part + (size_t)s0 * G->nchd * sdca::ResLayout{m}.ndots(), *res = G->res + (size_t)s0 * W;
+    HIP_TRY(hipMemcpyAsync(dc, hc, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipEventRecord(ev[4], st));
+    sdca::sub::init(st, dc, nc, G->d_galn, A->d_subQ, A->d_subMinv, kp, G->aux, G->auxstride);
+    sdca::gram::step_vectors(st, m, dc, nc, G->d_galn_r, G->Vr, G->vstride_r, G->nvs_r, upd, G->nchq_r);
+    sdca::sub::pack(st, dc, nc, G->d_galn, G->Vr, G->vstride_r, G->nvs_r, kp, G->aux, G->auxstride, Ap, Wc);
+    sdca::gram::gemm(st, A->d_subKQ, kp, Ap, kp, Ep, G->ldG, A->Bp, col, kp);
+    sdca::sub::softmax(st, dc, nc, G->d_galn, G->Vr, G->vstride_r, G->nvs_r, kp, Ep, Wc, Gp, G->ldG, sm, G->nchunk);
+    // split-K over the sequences: slice z of the product at Zs + z * zs (Zs after the slot
+    // groups' first slices), summed by the gradient kernel into Zr
+    const int nz = (A->Bp + sdca::kSubKc - 1) / sdca::kSubKc;
+    const int64_t zs = G->colcap * G->ngroups * (int64_t)(2 * kp);
+    sdca::gram::gemm_splitk(st, A->d_subKQt, A->Bp, Gp, G->ldG, Zr, 2 * kp, 2 * kp, col, A->Bp, sdca::kSubKc, zs);
+    sdca::sub::grad(st, dc, nc, G->d_galn, G->Vr, G->vstride_r, G->nvs_r, kp, G->aux, G->auxstride, sm, G->nchunk, Zr,
+                    nz, zs, 2.0 * A->lambda_J);
+    sdca::gram::dots(st, m, dc, nc, G->d_galn_r, G->Vr, G->vstride_r, G->nvs_r, Zr, 2 * kp, sm, G->nchunk, dot,
+                     G->nchd_r);
+    sdca::gram::reduce(st, m, nc, upd, dot, sm, G->nchunk, G->nchq_r, G->nchd_r, res);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(ev[5], st));
+    HIP_TRY(hipMemcpyAsync(G->h_res + (size_t)s0 * W, res, (size_t)nc * W * sizeof(double), hipMemcpyDeviceToHost,
+                           st));
+    return SDCA_OK;
+}
+
+int sub_finish_round(sdca_ctx *c, int g, std::vector<double> &res)
+{
+    sdca::GramWs *G = c->gram;
+    const std::vector<int> &lst = G->cmd_slot[g];
+    if (lst.empty()) return SDCA_OK;
+    HIP_TRY(hipStreamSynchronize(c->gstream[g]));
+    const sdca::ResLayout R{G->m};
+    const int W = R.width();
+    const double *hr = G->h_res + (size_t)g * G->gs * W;
+    for (size_t k = 0; k < lst.size(); ++k) {
+        double *o = res.data() + (size_t)lst[k] * W;
+        std::memcpy(o, hr + k * W, W * sizeof(double));
+        o[R.ginf()] = std::sqrt(std::max(0.0, o[R.ginf()]));
+        o[R.gJinf()] = std::sqrt(std::max(0.0, o[R.gJinf()]));
+    }
+    const sdca_ctx *A = c->members[0];
+    c->timing.sub += elapsed(G->ev[g][4], G->ev[g][5]);
+    c->timing.sub_evals += 1;
+    c->timing.flops_sub += 4.0 * A->B * (double)sdca::kSubRank * A->q * (double)lst.size();
+    return SDCA_OK;
+}
+
+// The restricted solves of items [0, K): their optima into G->Wst (item * wstride), their
+// evaluations into nev, and whether each may start the full solve into ok.
+int sub_solve(sdca_ctx *c, const int32_t *loci, int32_t K, std::vector<int> &nev, std::vector<char> &ok)
+{
+    constexpr int kp = sdca::kSubRank;
+    sdca::GramWs *G = c->gram;
+    const int nslots = G->slots, m = G->m;
+    const sdca::ResLayout R{m};
+    const SolverLimits lim{c->opts.subspace_threshold, 0.0, c->opts.max_iter};
+    Solver S(c, nslots, m, true, &lim);
+    const int ng = G->ngroups, gs = G->gs;
+    const int q = c->members[0]->q;
+    std::vector<double> res((size_t)nslots * R.width(), 0.0);
+    nev.assign(K, 0);
+    ok.assign(K, 0);
+    int next = 0, done = 0;
+    auto fill = [&](int i) {
+        if (next >= K) {
+            S.set_idle(i);
+            return;
+        }
+        const int item = next++;
+        S.assign(i, item, 0, loci[item], false);
+    };
+    auto group_end = [&](int g) { return std::min(nslots, (g + 1) * gs); };
+    auto group_active = [&](int g) {
+        for (int i = g * gs; i < group_end(g); ++i)
+            if (S.slot(i).phase != PH_IDLE) return true;
+        return false;
+    };
+    int rc;
+    for (int i = 0; i < nslots; ++i) fill(i);
+    bool inflight[sdca::kMaxGroups] = {};
+    for (int g = 0; g < ng; ++g) {
+        if (!group_active(g)) continue;
+        S.mark_live(g * gs, group_end(g));
+        if ((rc = sub_launch_round(c, g, S.cmds()))) return rc;
+        inflight[g] = true;
+    }
+    std::vector<int> finished;
+    while (done < K) {
+        bool any = false;
+        for (int g = 0; g < ng; ++g) {
+            if (!inflight[g]) continue;
+            any = true;
+            inflight[g] = false;
+            if ((rc = sub_finish_round(c, g, res))) return rc;
+            finished.clear();
+            for (int i = g * gs; i < group_end(g); ++i) {
+                if (S.slot(i).phase == PH_IDLE) continue;
+                if (S.process(i, res.data() + (size_t)i * R.width())) finished.push_back(i);
+            }
+            hipStream_t st = c->gstream[g];
+            for (int i : finished) {
+                const Slot &sl = S.slot(i);
+                nev[sl.item] = sl.nfeval;
+                if (sl.status != SDCA_ST_NONFINITE) {
+                    const double *src = G->Vr + (size_t)i * G->vstride_r +
+                                        (sl.from_xt ? sdca::gram::kVecXt : sdca::gram::kVecXc) * G->nvs_r;
+                    double *dst = G->Wst + (size_t)sl.item * G->wstride;
+                    HIP_TRY(hipMemcpyAsync(dst, src, (size_t)q * kp * sizeof(double), hipMemcpyDeviceToDevice, st));
+                    HIP_TRY(hipMemcpyAsync(dst + (size_t)q * kp, src + 2 * (size_t)q * kp, 8 * sizeof(double),
+                                           hipMemcpyDeviceToDevice, st));
+                    ok[sl.item] = 1;
+                }
+                ++done;
+                fill(i);
+            }
+            if (group_active(g)) {
+                S.mark_live(g * gs, group_end(g));
+                if ((rc = sub_launch_round(c, g, S.cmds()))) return rc;
+                inflight[g] = true;
+            }
+        }
+        if (!any) break;
+    }
+    for (int g = 0; g < ng; ++g) HIP_TRY(hipStreamSynchronize(c->gstream[g]));
+    return SDCA_OK;
+}
+
 int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int32_t K, float *score_rows,
                      const int64_t *row_off, double *x_out, sdca_locus_stats *st_out)
 {
@@ -1907,8 +2293,18 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
     if ((rc = ensure_gram_ws(c, nslots, m))) return rc;
     if ((rc = upload_gram_alns(c))) return rc;
     const sdca::ResLayout R{m};
-    Solver S(c, nslots, m, true);
     sdca::GramWs *G = c->gram;
+    // subspace warm start: the restricted solves of every listed locus first
+    std::vector<int> sub_nev;
+    std::vector<char> sub_ok;
+    G->warm_rounds = false;
+    if (K > 0 && subspace_wanted(c) && ensure_subspace(c->members[0], c->stream) == SDCA_OK) {
+        if ((rc = ensure_sub_ws(c, K))) return rc;
+        if ((rc = sub_solve(c, loci, K, sub_nev, sub_ok))) return rc;
+        G->warm_rounds = true;
+    }
+    G->sub_nev = sub_nev.empty() ? nullptr : sub_nev.data();
+    Solver S(c, nslots, m, true);
     const int ng = G->ngroups, gs = G->gs;
     std::vector<double> res((size_t)nslots * R.width(), 0.0);
     int next = 0, done = 0, noconv = 0, nonfinite = 0;
@@ -1918,7 +2314,7 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
             return;
         }
         const int item = next++;
-        S.assign(i, item, alns ? alns[item] : 0, loci[item], false);
+        S.assign(i, item, alns ? alns[item] : 0, loci[item], G->warm_rounds && sub_ok[item]);
     };
     auto group_end = [&](int g) { return std::min(nslots, (g + 1) * gs); };
     auto group_active = [&](int g) {
@@ -1975,6 +2371,8 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
         if (!any) break;
     }
     if ((rc = gram_flush(c, 0, true, score_rows, row_off, x_out))) return rc;
+    G->warm_rounds = false;
+    G->sub_nev = nullptr;
     c->timing.total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (nonfinite) return fail(SDCA_E_NONFINITE, "%d loci reached a non-finite objective", nonfinite);
     if (noconv) return fail(SDCA_E_NOCONV, "%d loci stopped without reaching the gradient threshold", noconv);
@@ -2048,6 +2446,8 @@ void sdca_default_opts(sdca_opts *o)
     o->coarse_factor = 4;
     o->fine_factor = 10;
     o->j_threshold = 0.0;
+    o->subspace_rank = 0;
+    o->subspace_threshold = 3e-2;
 }
 
 int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L, int32_t q, int32_t gap,
@@ -2081,6 +2481,9 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
     if (o.batch_loci < 0) return fail(SDCA_E_ARG, "batch_loci must be >= 1 (or 0: the formulation's default)");
     if (o.max_iter < 1 || o.max_ls < 1) return fail(SDCA_E_ARG, "max_iter and max_ls must be >= 1");
     if (!(o.grad_threshold >= 0.0)) return fail(SDCA_E_ARG, "grad_threshold must be >= 0");
+    if (o.subspace_rank != 0 && o.subspace_rank != sdca::kSubRank)
+        return fail(SDCA_E_ARG, "subspace_rank must be 0 (off) or %d (got %d)", sdca::kSubRank, o.subspace_rank);
+    if (o.subspace_rank && !(o.subspace_threshold > 0.0)) return fail(SDCA_E_ARG, "subspace_threshold must be > 0");
     for (int64_t e = 0; e < (int64_t)B * L; ++e)
         if (states[e] >= q) return fail(SDCA_E_ARG, "state %d at element %lld outside [0, %d)", states[e], (long long)e, q);
 
@@ -2545,7 +2948,7 @@ int sdca_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
                         o.nfeval_first = sl.nfeval_first;
                         o.gnorm_J = sl.gnormJ;
                         o.nfeval_fine = sl.nfeval_fine;
-                        o.reserved = 0;
+                        o.nfeval_sub = 0;
                     }
                     noconv += (sl.status == SDCA_ST_ITER_CAP || sl.status == SDCA_ST_LS_FAIL);
                     nonfinite += (sl.status == SDCA_ST_NONFINITE);
@@ -2713,6 +3116,9 @@ int sdca_get_timing(sdca_ctx *c, sdca_timing *t, int32_t reset)
     t->dots_ms = s.dots;
     t->bytes_update = s.bytes_update;
     t->bytes_dots = s.bytes_dots;
+    t->subspace_ms = s.sub;
+    t->n_sub_evals = s.sub_evals;
+    t->flops_sub = s.flops_sub;
     if (reset) c->timing = sdca::Timing();
     return SDCA_OK;
 }
@@ -2745,7 +3151,7 @@ int sdca_multi_create(sdca_multi **out, sdca_ctx *const *ctxs, int32_t n, const 
         return fail(SDCA_E_ARG, "unknown formulation %d", o.formulation);
     if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 8, 10, 16 (got %d)", o.history);
     if (o.batch_loci < 0 || o.max_iter < 1 || o.max_ls < 1 || !(o.grad_threshold >= 0.0) || !(o.j_threshold >= 0.0) ||
-        o.coarse_factor < 0 || o.fine_factor < 0)
+        o.coarse_factor < 0 || o.fine_factor < 0 || (o.subspace_rank != 0 && o.subspace_rank != sdca::kSubRank))
         return fail(SDCA_E_ARG, "invalid solver options");
     for (int a = 0; a < n; ++a) {
         if (!ctxs[a]) return fail(SDCA_E_ARG, "context %d is null", a);

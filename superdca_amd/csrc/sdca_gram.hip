@@ -147,12 +147,19 @@ __device__ __forceinline__ void ld8d(const double *p, double (&r)[8])
 }
 
 __global__ __launch_bounds__(256) void k_dgemm(const double *__restrict__ A, int64_t lda, const double *__restrict__ Bm,
-                                               int64_t ldb, double *__restrict__ C, int64_t ldc, int N, int Kd)
+                                               int64_t ldb, double *__restrict__ C, int64_t ldc, int N, int Kd,
+                                               int kc, int64_t zstride)
 {
     __shared__ double sA[2][DK][DT + DPAD];
     __shared__ double sB[2][DK][DT + DPAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * DT, n0 = blockIdx.x * DT;
+    // split-K slice z: k in [z kc, min(Kd, (z + 1) kc)), partial product into C + z zstride
+    const int kz = blockIdx.z * kc;
+    Kd = min(Kd - kz, kc);
+    A += kz;
+    Bm += kz;
+    C += blockIdx.z * zstride;
     const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
     // staging: thread -> tile row r, k half kh (DK / 2 doubles of each operand per step)
     constexpr int KH = DK / 2;
@@ -255,7 +262,7 @@ __device__ __forceinline__ SlotPtrs slot_ptrs(const SlotCmd &c, double *V, int64
 
 struct StepArgs {
     int mode, pnew;
-    bool accept;
+    bool accept, staged;
     double alpha, pstep, cd, cg;
 };
 
@@ -265,6 +272,7 @@ __device__ __forceinline__ void load_step(const SlotCmd &c, StepArgs &a, double 
     a.mode = c.mode;
     a.pnew = c.pnew < M ? c.pnew : -1;
     a.accept = c.accept != 0;
+    a.staged = c.init_src != 0.f;
     a.alpha = c.alpha;
     a.pstep = c.pstep;
     a.cd = c.cd;
@@ -278,7 +286,8 @@ __device__ __forceinline__ void load_step(const SlotCmd &c, StepArgs &a, double 
 
 // NE elements e[k] of a slot's vectors (coefficients, image entries or fields), as k_update
 // does it for the primal vectors:
-//   INIT    x_c = 0, x_t = 0
+//   INIT    x_c = 0, x_t = 0; with a staged start (init_src, written into x_c by k_gwarm)
+//           x_t = x_c
 //   RETRY   x_t = x_c + alpha d
 //   NEWDIR  x_c += pstep d_old (accept); s_pnew = pstep d_old, y_pnew = g[gk] - g[1-gk];
 //           d = cd d_old + cg g_k + sum_p (cs[p] s_p + cy[p] y_p); x_t = x_c + alpha d
@@ -293,6 +302,17 @@ __device__ __forceinline__ void step_elems(const StepArgs &a, const double (&cs)
 #pragma unroll
     for (int k = 0; k < NE; ++k) dn[k] = gv[k] = 0.0;
     if (a.mode == MODE_INIT) {
+        if (a.staged) {
+            double x[NE];
+#pragma unroll
+            for (int k = 0; k < NE; ++k) x[k] = P.xc[e[k]];
+#pragma unroll
+            for (int k = 0; k < NE; ++k) {
+                xn[k] = x[k];
+                P.xt[e[k]] = x[k];
+            }
+            return;
+        }
 #pragma unroll
         for (int k = 0; k < NE; ++k) {
             xn[k] = 0.0;
@@ -706,7 +726,17 @@ void gemm(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_
           int N, int Kd)
 {
     if (N <= 0) return;
-    hipLaunchKernelGGL(k_dgemm, dim3((N + DT - 1) / DT, M / DT), dim3(256), 0, st, A, lda, Bm, ldb, C, ldc, N, Kd);
+    hipLaunchKernelGGL(k_dgemm, dim3((N + DT - 1) / DT, M / DT, 1), dim3(256), 0, st, A, lda, Bm, ldb, C, ldc, N, Kd,
+                       Kd, (int64_t)0);
+}
+
+void gemm_splitk(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_t ldb, double *C, int64_t ldc,
+                 int M, int N, int Kd, int kc, int64_t zstride)
+{
+    if (N <= 0) return;
+    const int Z = (Kd + kc - 1) / kc;
+    hipLaunchKernelGGL(k_dgemm, dim3((N + DT - 1) / DT, M / DT, Z), dim3(256), 0, st, A, lda, Bm, ldb, C, ldc, N, Kd,
+                       kc, zstride);
 }
 
 #define SDCA_GRAM_DISPATCH(m, CALL) \
@@ -730,6 +760,16 @@ void step(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *a
 #undef SDCA_GSTEP
     hipLaunchKernelGGL(k_gsoftmax, dim3(nchunk, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Gp, ldG, sm_part,
                        nchunk);
+}
+
+void step_vectors(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V,
+                  int64_t vstride, int64_t nvs, double *upd_part, int nchq)
+{
+#define SDCA_GSTEPV(MM)                                                                                              \
+    hipLaunchKernelGGL((k_ghstep<MM>), dim3(ncmd), dim3(64), 0, st, cmds, aln, V, vstride, nvs, upd_part, nchq);     \
+    hipLaunchKernelGGL((k_gstep<MM>), dim3(nchq, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, upd_part, nchq)
+    SDCA_GRAM_DISPATCH(m, SDCA_GSTEPV)
+#undef SDCA_GSTEPV
 }
 
 int dots_chunks(int64_t qB) { return (int)((qB + 256 * kDotsPerThread - 1) / (256 * kDotsPerThread)); }

@@ -76,6 +76,7 @@ struct SlotCmd {
     int32_t accept;   // NEWDIR: 1 = the last trial point becomes the iterate first (x_c += pstep d_old)
     int32_t pnew;     // NEWDIR: pair slot receiving s = pstep d_old, y = g[gk] - g[1-gk]; -1: none
     uint32_t live;    // bit p: pair slot p holds a stored pair
+    int32_t item;     // index of the locus in the solve's list (staged Gram warm starts), -1 if none
     float alpha;      // step of the trial point x_t = x_c + alpha d
     float pstep;      // NEWDIR with accept: step of the accepted trial
     float init_src;   // MODE_INIT: 1 = copy the staged start point, 0 = zero start
@@ -107,6 +108,8 @@ struct Timing {
     int64_t evals = 0, energy_launches = 0, grad_launches = 0;
     double flops_energy = 0, flops_grad = 0, bytes_lbfgs = 0;
     double update = 0, dots = 0, bytes_update = 0, bytes_dots = 0;
+    double sub = 0, flops_sub = 0;
+    int64_t sub_evals = 0;
 };
 
 // One alignment of a solve, as the kernels see it.  A solve covers one alignment (a
@@ -294,6 +297,28 @@ void scatter_warm(hipStream_t st, const int32_t *slot_of, const int32_t *item_of
                   const int32_t *state_col, int64_t n_slot, float *X0, void *XJ, bool xj_bf16, int64_t jstride);
 }  // namespace kern
 
+// Rank of the subspace warm start's basis (sdca_subspace.hip), and the sequences per
+// split-K slice of its gradient product (kSubRank output rows: without slices a launch
+// would hold one row of tiles).
+constexpr int kSubRank = 128;
+constexpr int kSubKc = 256;
+
+// The subspace warm start (sdca_subspace.hip).
+namespace sub {
+void init(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, const double *Q, const double *Minv,
+          int kp, double *aux, int64_t auxstride);
+void pack(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, const double *V, int64_t vstride,
+          int64_t nvs, int kp, const double *aux, int64_t auxstride, double *Ap, double *Wc);
+void softmax(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, const double *V, int64_t vstride,
+             int64_t nvs, int kp, const double *Ep, const double *Wc, double *Gp, int64_t ldG, double *sm_part,
+             int nchunk);
+void grad(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride, int64_t nvs,
+          int kp, const double *aux, int64_t auxstride, const double *sm_part, int nchunk, double *Z, int nz,
+          int64_t zstride, double lamJ2);
+void warm(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride, int64_t nvs,
+          const double *Qt, const double *KQt, const double *Q, int kp, const double *Wst, int64_t wstride);
+}  // namespace sub
+
 // The Gram formulation (sdca_gram.hip): slot vectors x_c, x_t, g[0], g[1], d, then the m
 // pairs s_p and the m pairs y_p, each nvs doubles apart (coefficients, image, fields).
 namespace gram {
@@ -306,11 +331,19 @@ void gram_matrix(hipStream_t st, const uint64_t *planes, int B, int q, int W, in
 // [cmd][nchunk][65] (blocks of 256 sequences).
 void step(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
           int64_t nvs, double *Gp, int64_t ldG, double *upd_part, double *sm_part, int nchunk, int nchq);
+// The trial-point part of step() alone (the subspace warm start's rounds: nchq blocks of 256
+// over the q x k coefficients).
+void step_vectors(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V,
+                  int64_t vstride, int64_t nvs, double *upd_part, int nchq);
 // C[n][m] = sum_k Bm[n][k] A[m][k], FP64 MFMA: m < M (multiple of 128), k < Kd (multiple
 // of 16), n < N stored; Bm rows up to round_up(N, kGemmCols) readable.
 constexpr int kGemmCols = 128;
 void gemm(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_t ldb, double *C, int64_t ldc, int M,
           int N, int Kd);
+// The same in ceil(Kd / kc) split-K slices (kc a multiple of 16): slice z sums k in
+// [z kc, (z + 1) kc) into C + z zstride.
+void gemm_splitk(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_t ldb, double *C, int64_t ldc,
+                 int M, int N, int Kd, int kc, int64_t zstride);
 // Gradient images from Z = K G and the products: dot_part [cmd][nchd][2m + 5], nchd =
 // dots_chunks(q Bp) blocks per command.
 int dots_chunks(int64_t qB);
@@ -369,6 +402,16 @@ struct GramWs {
     hipGraph_t graph[kMaxGroups] = {};
     hipGraphExec_t gexec[kMaxGroups] = {};
     const double *graph_K[kMaxGroups] = {};
+    // subspace warm start: restricted slot vectors (nvs_r doubles, vstride_r per slot), per-slot
+    // constants (aux, auxstride per slot), the packed restricted coefficients Ap and products
+    // Zr (kSubRank per column), W = V_r a_t (64 per command), its descriptor, and the staged
+    // restricted optima Wst (wstride per item, wcap items)
+    int64_t nvs_r = 0, vstride_r = 0, auxstride = 0, wstride = 0, wcap = 0;
+    int nchq_r = 0, nchd_r = 0;
+    double *Vr = nullptr, *aux = nullptr, *Ap = nullptr, *Zr = nullptr, *Wc = nullptr, *Wst = nullptr;
+    GramAln *d_galn_r = nullptr;
+    bool warm_rounds = false;  // the current solve's full rounds stage warm starts (k_gwarm)
+    const int *sub_nev = nullptr;  // the current solve's restricted evaluations per item
     ~GramWs();
 };
 
@@ -513,6 +556,11 @@ struct sdca_ctx {
     // split gram_split) and FP64 weights, and the workspace of the solves it runs
     double *d_gram = nullptr;
     int gram_split = -1;
+    // the dominant eigen-subspace of d_gram (subspace warm start): orthonormal basis Q
+    // (Bp x kSubRank and its transpose), K Q, [K Q | K Q M^-1]^T (2 kSubRank x Bp) and
+    // M^-1 = (Q^T K Q)^-1; sub_rank 0 until built
+    int sub_rank = 0;
+    double *d_subQ = nullptr, *d_subQt = nullptr, *d_subKQ = nullptr, *d_subKQt = nullptr, *d_subMinv = nullptr;
     double *d_w64 = nullptr;
     sdca::GramWs *gram = nullptr;
 

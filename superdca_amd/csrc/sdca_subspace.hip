@@ -1,0 +1,465 @@
+// sdca_subspace.hip -- the subspace warm start of the Gram formulation (kernels).
+//
+// The Gram solve holds a locus' couplings as J = X_r^T c with c the q x B coefficients of
+// the sequences (sdca_gram.hip).  Its L-BFGS iterations are set by the large eigenvalues
+// of the identical-locus counts K: the clade directions.  Before that solve, each locus is
+// solved with c restricted to c = Q a, Q (B x k) an orthonormal basis of K's dominant
+// eigen-subspace (power iteration), a (k x q).  The restricted problem is the same
+// objective on fewer unknowns:
+//   energies   E_b(s) = h(s) + (K_r Q a)(b, s),  K_r Q = K Q - U V_r,  V_r = U^T Q (q x k)
+//              with U the B x q indicator of the locus' own states: (U V_r a)(b, s) =
+//              (V_r a)(x_br, s)
+//   penalty    lambda_J sum_s a_s^T M_r a_s,  M_r = Q^T K_r Q = M - V_r^T V_r,  M = Q^T K Q
+//   gradient   dE/da_s = (K_r Q)^T G_s + 2 lambda_J M_r a_s = b_s + 2 lambda_J M_r a_s,
+//              b_s = (K Q)^T G_s - V_r^T T(s, .),  T(s, a) = sum_{b: x_br = a} G_b(s)
+// and its L-BFGS runs in the primal metric <u, v> = u_h v_h + a_u . M_r a_v, i.e. on the
+// slot vectors of sdca_gram.hip with B replaced by k: coefficients a, image M_r a, fields.
+// The gradient's coefficients are M_r^{-1} b_s + 2 lambda_J a_s, with
+//   M_r^{-1} = M^{-1} + R_r Omega_r R_r^T,  R_r = M^{-1} V_r^T (k x q),
+//   Omega_r = (I - V_r M^{-1} V_r^T)^{-1} (q x q)     (Woodbury: M_r = M - V_r^T V_r),
+// and its image b_s + 2 lambda_J (M_r a)_s.  One restricted evaluation costs two B x k x q
+// products (k_dgemm) where the full one costs a B x B x q product.  The restricted
+// optimum (a*, h*) then starts the full solve at c = Q a*, K_r c = K Q a* - U V_r a*.
+// Everything is FP64.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "sdca_internal.hpp"
+
+namespace sdca {
+namespace {
+
+constexpr int kSmS = 65;  // softmax partials per block, as in sdca_gram.hip: T (8 x 8) and the data term
+
+__device__ __forceinline__ double wsum(double x)
+{
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    return x;  // lane 0
+}
+
+// Per-slot constants of the restricted problem of an INIT command's locus, one block of
+// 256 threads per command:
+//   V_r[a][i] = sum_{b: x_br = a} Q[b][i]          (aux + 0,            q x kp)
+//   R_r[i][a] = sum_j Minv[i][j] V_r[a][j]         (aux + 8 kp,         kp x 8)
+//   Omega_r   = (I - V_r R_r)^{-1}                  (aux + 16 kp,        8 x 8)
+// Q is Bp x kp (row-major), Minv kp x kp (symmetric).  Thread t covers column i = t % kp
+// over half of the sequences (t / kp); kp <= 128.
+__global__ __launch_bounds__(256) void k_sinit(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                               const double *__restrict__ Q, const double *__restrict__ Minv, int kp,
+                                               double *__restrict__ aux, int64_t auxstride)
+{
+    const SlotCmd &c = cmds[blockIdx.x];
+    if (c.mode != MODE_INIT) return;
+    const GramAln &A = aln[c.aln];
+    const int q = A.q, B = A.B;
+    __shared__ double sV[2][8][128];
+    __shared__ double sX[8][8];
+    const int tid = threadIdx.x, i = tid % kp, half = tid / kp;
+    const int nhalf = 256 / kp;  // 2 when kp = 128
+    double acc[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) acc[a] = 0.0;
+    const uint8_t *y = A.statesT + (int64_t)c.locus * B;
+    if (half < nhalf)
+        for (int b = half; b < B; b += nhalf) {
+            const int ya = y[b];
+            const double v = Q[(int64_t)b * kp + i];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) acc[a] += (a == ya) ? v : 0.0;
+        }
+    if (half < 2)
+#pragma unroll
+        for (int a = 0; a < 8; ++a) sV[half][a][i] = acc[a];
+    __syncthreads();
+    double *out = aux + (int64_t)c.slot * auxstride;
+    if (tid < kp) {
+        for (int a = 0; a < 8; ++a) {
+            const double v = a < q ? (nhalf > 1 ? sV[0][a][tid] + sV[1][a][tid] : sV[0][a][tid]) : 0.0;
+            sV[0][a][tid] = v;
+            out[(int64_t)a * kp + tid] = v;
+        }
+    }
+    __syncthreads();
+    // R_r[i][a] = sum_j Minv[i][j] V_r[a][j] (Minv symmetric: row i read as column i)
+    if (tid < kp) {
+        double r[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) r[a] = 0.0;
+        for (int j = 0; j < kp; ++j) {
+            const double mij = Minv[(int64_t)j * kp + tid];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) r[a] += mij * sV[0][a][j];
+        }
+#pragma unroll
+        for (int a = 0; a < 8; ++a) out[8 * (int64_t)kp + (int64_t)tid * 8 + a] = r[a];
+        // X[a][a'] = V_r[a] . R_r[., a'] : partial products of column tid
+#pragma unroll
+        for (int a = 0; a < 8; ++a) sV[1][a][tid] = r[a];
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const int a = tid >> 3, a2 = tid & 7;
+        double x = 0.0;
+        if (a < q && a2 < q)
+            for (int j = 0; j < kp; ++j) x += sV[0][a][j] * sV[1][a2][j];
+        sX[a][a2] = (a == a2 ? 1.0 : 0.0) - x;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        // Gauss-Jordan with partial pivoting on the q x q block (rows of unobserved states are
+        // those of the identity)
+        double Mx[8][8], Iv[8][8];
+        for (int a = 0; a < q; ++a)
+            for (int u = 0; u < q; ++u) {
+                Mx[a][u] = sX[a][u];
+                Iv[a][u] = a == u ? 1.0 : 0.0;
+            }
+        for (int col = 0; col < q; ++col) {
+            int piv = col;
+            for (int rr = col + 1; rr < q; ++rr)
+                if (fabs(Mx[rr][col]) > fabs(Mx[piv][col])) piv = rr;
+            for (int u = 0; u < q; ++u) {
+                double t = Mx[col][u];
+                Mx[col][u] = Mx[piv][u];
+                Mx[piv][u] = t;
+                t = Iv[col][u];
+                Iv[col][u] = Iv[piv][u];
+                Iv[piv][u] = t;
+            }
+            const double inv = 1.0 / Mx[col][col];
+            for (int u = 0; u < q; ++u) {
+                Mx[col][u] *= inv;
+                Iv[col][u] *= inv;
+            }
+            for (int rr = 0; rr < q; ++rr) {
+                if (rr == col) continue;
+                const double f = Mx[rr][col];
+                for (int u = 0; u < q; ++u) {
+                    Mx[rr][u] -= f * Mx[col][u];
+                    Iv[rr][u] -= f * Iv[col][u];
+                }
+            }
+        }
+        double *om = out + 16 * (int64_t)kp;
+        for (int a = 0; a < 8; ++a)
+            for (int u = 0; u < 8; ++u) om[a * 8 + u] = (a < q && u < q) ? Iv[a][u] : 0.0;
+    }
+}
+
+// The round's trial coefficients as the packed operand of the energy product,
+// Ap[row + s][i] = a_t(s, i), and W[command][a][s] = (V_r a_t)(a, s) (8 x 8 per command).
+// One block of 256 per command.
+__global__ __launch_bounds__(256) void k_spack(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                               const double *__restrict__ V, int64_t vstride, int64_t nvs, int kp,
+                                               const double *__restrict__ aux, int64_t auxstride,
+                                               double *__restrict__ Ap, double *__restrict__ Wc)
+{
+    const int j = blockIdx.x;
+    const SlotCmd &c = cmds[j];
+    if (c.mode == MODE_IDLE) return;
+    const int q = aln[c.aln].q;
+    const double *xt = V + (int64_t)c.slot * vstride + gram::kVecXt * nvs;
+    const double *Vr = aux + (int64_t)c.slot * auxstride;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < q * kp; e += 256) {
+        const int s = e / kp, i = e - (e / kp) * kp;
+        Ap[(int64_t)(c.row + s) * kp + i] = xt[e];
+    }
+    // W[a][s] = sum_i V_r[a][i] a_t(s, i): one wave per (a, s) pair at a time
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int p = wave; p < 64; p += 4) {
+        const int a = p >> 3, s = p & 7;
+        double v = 0.0;
+        if (a < q && s < q)
+            for (int i = lane; i < kp; i += 64) v += Vr[(int64_t)a * kp + i] * xt[(int64_t)s * kp + i];
+        v = wsum(v);
+        if (lane == 0) Wc[(int64_t)j * 64 + p] = v;
+    }
+}
+
+// Softmax of the restricted trial points, one thread per sequence b (blocks of 256):
+//   E_b(s) = h_t(s) + Ep[row + s][b] - W[a = x_br][s],  Ep = (K Q) a_t from the product,
+// then G, the T table and the data term exactly as k_gsoftmax forms them (G into the packed
+// operand of the gradient product).  sm_part[command][chunk] = {T(s, a) at 8 s + a, sum f_b}.
+__global__ __launch_bounds__(256) void k_ssoftmax(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                                  const double *__restrict__ V, int64_t vstride, int64_t nvs, int kp,
+                                                  const double *__restrict__ Ep, const double *__restrict__ Wc,
+                                                  double *__restrict__ Gp, int64_t ldG, double *__restrict__ sm_part,
+                                                  int nchunk)
+{
+    const int j = blockIdx.y;
+    const SlotCmd &c = cmds[j];
+    __shared__ double sW[4][kSmS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double *sp = sm_part + ((int64_t)j * nchunk + blockIdx.x) * kSmS;
+    if (c.mode == MODE_IDLE) {
+        if (tid < kSmS) sp[tid] = 0.0;
+        return;
+    }
+    const GramAln &A = aln[c.aln];
+    const int q = A.q, Bp = A.Bp, B = A.B;
+    const double *xt = V + (int64_t)c.slot * vstride + gram::kVecXt * nvs;
+    const double *W = Wc + (int64_t)j * 64;
+    double hs[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) hs[s] = s < q ? xt[2 * (int64_t)q * kp + s] : 0.0;
+    const int b = blockIdx.x * 256 + tid;
+    double G[8];
+    double fb = 0.0;
+    int y = -1;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) G[s] = 0.0;
+    if (b < B) {
+        y = A.statesT[(int64_t)c.locus * B + b];
+        const double wb = A.w[b];
+        double E[8], mx = -INFINITY;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            E[s] = s < q ? hs[s] + Ep[(int64_t)(c.row + s) * ldG + b] - W[y * 8 + s] : 0.0;
+            if (s < q) mx = fmax(mx, E[s]);
+        }
+        double ex[8], z = 0.0, Ey = 0.0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            ex[s] = s < q ? exp(E[s] - mx) : 0.0;
+            z += ex[s];
+            if (s == y) Ey = E[s];
+        }
+        const double lz = mx + log(z);
+        fb = wb * (lz - Ey);
+        const double iz = 1.0 / z;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            if (s >= q) continue;
+            G[s] = wb * (ex[s] * iz - (s == y ? 1.0 : 0.0));
+            Gp[(int64_t)(c.row + s) * ldG + b] = G[s];
+        }
+    } else if (b < Bp) {
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            if (s < q) Gp[(int64_t)(c.row + s) * ldG + b] = 0.0;  // pad sequences: zero
+    }
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (s >= q || u >= q) continue;
+            const double v = wsum(y == u ? G[s] : 0.0);
+            if (lane == 0) sW[wave][s * 8 + u] = v;
+        }
+    fb = wsum(fb);
+    if (lane == 0) sW[wave][64] = fb;
+    __syncthreads();
+    if (tid < kSmS) {
+        const int s = tid >> 3, u = tid & 7;
+        const bool ok = tid == 64 || (s < q && u < q);
+        sp[tid] = ok ? (sW[0][tid] + sW[1][tid]) + (sW[2][tid] + sW[3][tid]) : 0.0;
+    }
+}
+
+// The restricted gradient of every command, one block of kp (<= 128) threads per command.
+// The split-K product gives, per packed row (row + s), the nz partials of
+// [(K Q)^T G_s | (K Q M^-1)^T G_s] (2 kp entries, zstride apart), so that
+//   b_s = sum_z Z_z[row + s][0, kp) - V_r^T T(s, .)        (written back into Z_0)
+//   u_s = sum_z Z_z[row + s][kp, 2kp) - R_r T(s, .)        = M^-1 b_s
+//   g_s = u_s + R_r Omega_r (V_r u_s) + 2 lambda_J a_t(s)   = M_r^-1 b_s + 2 lambda_J a_t(s)
+// into the coefficients of g[gt]; the products pass (k_gdots, reading b with row stride
+// 2 kp, image b + 2 lambda_J M_r a_t) completes it.
+__global__ __launch_bounds__(128) void k_sgrad(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                               double *__restrict__ V, int64_t vstride, int64_t nvs, int kp,
+                                               const double *__restrict__ aux, int64_t auxstride,
+                                               const double *__restrict__ sm_part, int nchunk, double *__restrict__ Z,
+                                               int nz, int64_t zstride, double lamJ2)
+{
+    const int j = blockIdx.x;
+    const SlotCmd &c = cmds[j];
+    if (c.mode == MODE_IDLE || c.gt < 0 || c.gt > 1) return;
+    const int q = aln[c.aln].q;
+    __shared__ double sT[64];
+    __shared__ double su[8][128];
+    __shared__ double sv[8][8];
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        const int s = tid >> 3, u = tid & 7;
+        double t = 0.0;
+        if (s < q && u < q)
+            for (int ch = 0; ch < nchunk; ++ch) t += sm_part[((int64_t)j * nchunk + ch) * kSmS + tid];
+        sT[tid] = t;
+    }
+    __syncthreads();
+    const double *Vr = aux + (int64_t)c.slot * auxstride;
+    const double *Rr = Vr + 8 * (int64_t)kp;
+    const double *Om = Vr + 16 * (int64_t)kp;
+    const int i = tid;
+    const int64_t ld = 2 * (int64_t)kp;
+    double u[8];
+#pragma unroll
+    for (int s = 0; s < 8; ++s) u[s] = 0.0;
+    if (i < kp)
+        for (int s = 0; s < q; ++s) {
+            double bv = 0.0, uv = 0.0;
+            for (int z = 0; z < nz; ++z) {
+                const double *zr = Z + z * zstride + (int64_t)(c.row + s) * ld;
+                bv += zr[i];
+                uv += zr[kp + i];
+            }
+            for (int a = 0; a < q; ++a) {
+                bv -= Vr[(int64_t)a * kp + i] * sT[s * 8 + a];
+                uv -= Rr[(int64_t)i * 8 + a] * sT[s * 8 + a];
+            }
+            Z[(int64_t)(c.row + s) * ld + i] = bv;
+            u[s] = uv;
+            su[s][i] = uv;
+        }
+    __syncthreads();
+    // v[a][s] = V_r[a] . u_s
+    if (tid < 64) {
+        const int a = tid >> 3, s = tid & 7;
+        double v = 0.0;
+        if (a < q && s < q)
+            for (int k = 0; k < kp; ++k) v += Vr[(int64_t)a * kp + k] * su[s][k];
+        sv[a][s] = v;
+    }
+    __syncthreads();
+    if (i < kp) {
+        double *gt = V + (int64_t)c.slot * vstride + (gram::kVecG + c.gt) * nvs;
+        const double *xt = V + (int64_t)c.slot * vstride + gram::kVecXt * nvs;
+        double r[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) r[a] = a < q ? Rr[(int64_t)i * 8 + a] : 0.0;
+        for (int s = 0; s < q; ++s) {
+            double g = u[s];
+            for (int a = 0; a < q; ++a) {
+                double w = 0.0;  // (Omega_r v)(a, s)
+                for (int a2 = 0; a2 < q; ++a2) w += Om[a * 8 + a2] * sv[a2][s];
+                g += r[a] * w;
+            }
+            gt[(int64_t)s * kp + i] = g + lamJ2 * xt[(int64_t)s * kp + i];
+        }
+    }
+}
+
+// Start points of the full solve from restricted optima: for every INIT command with a
+// staged start (init_src != 0), with (a*, h*) at Wst + item * wstride (coefficients
+// a*(s, i) at s * kp + i, then the fields):
+//   c(s, b) = sum_i Q[b][i] a*(s, i),  (K_r c)(s, b) = sum_i KQ[b][i] a*(s, i) - (V_r a*)(x_br, s)
+// into x_c of the command's slot (coefficients, image, fields).  One block of 256 per
+// command; Qt and KQt are kp x Bp (sequences contiguous).
+__global__ __launch_bounds__(256) void k_gwarm(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                               double *__restrict__ V, int64_t vstride, int64_t nvs,
+                                               const double *__restrict__ Qt, const double *__restrict__ KQt,
+                                               const double *__restrict__ Q, int kp, const double *__restrict__ Wst,
+                                               int64_t wstride)
+{
+    const SlotCmd &c = cmds[blockIdx.x];
+    if (c.mode != MODE_INIT || c.init_src == 0.f || c.item < 0) return;
+    const GramAln &A = aln[c.aln];
+    const int q = A.q, B = A.B, Bp = A.Bp;
+    const int64_t qB = (int64_t)q * Bp;
+    __shared__ double sa[8][128];
+    __shared__ double sV[2][8][128];
+    __shared__ double sW[8][8];
+    const int tid = threadIdx.x;
+    const double *ws = Wst + (int64_t)c.item * wstride;
+    for (int e = tid; e < 8 * 128; e += 256) {
+        const int s = e >> 7, i = e & 127;
+        sa[s][i] = (s < q && i < kp) ? ws[(int64_t)s * kp + i] : 0.0;
+    }
+    // V_r[a][i] = sum_{b: x_br = a} Q[b][i] (as k_sinit), then W = V_r a*
+    const uint8_t *y = A.statesT + (int64_t)c.locus * B;
+    const int i = tid % kp, half = tid / kp, nhalf = 256 / kp;
+    double acc[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) acc[a] = 0.0;
+    if (half < nhalf)
+        for (int b = half; b < B; b += nhalf) {
+            const int ya = y[b];
+            const double v = Q[(int64_t)b * kp + i];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) acc[a] += (a == ya) ? v : 0.0;
+        }
+    if (half < 2)
+#pragma unroll
+        for (int a = 0; a < 8; ++a) sV[half][a][i] = acc[a];
+    __syncthreads();
+    if (tid < 64) {
+        const int a = tid >> 3, s = tid & 7;
+        double v = 0.0;
+        if (a < q && s < q)
+            for (int k = 0; k < kp; ++k) v += (nhalf > 1 ? sV[0][a][k] + sV[1][a][k] : sV[0][a][k]) * sa[s][k];
+        sW[a][s] = v;
+    }
+    __syncthreads();
+    double *xc = V + (int64_t)c.slot * vstride + gram::kVecXc * nvs;
+    for (int b = tid; b < Bp; b += 256) {
+        double cv[8], zv[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) cv[s] = zv[s] = 0.0;
+        if (b < B) {
+            for (int k = 0; k < kp; ++k) {
+                const double qk = Qt[(int64_t)k * Bp + b], kq = KQt[(int64_t)k * Bp + b];
+#pragma unroll
+                for (int s = 0; s < 8; ++s) {
+                    cv[s] += qk * sa[s][k];
+                    zv[s] += kq * sa[s][k];
+                }
+            }
+            const int ya = y[b];
+#pragma unroll
+            for (int s = 0; s < 8; ++s) zv[s] -= sW[ya][s];
+        }
+        for (int s = 0; s < q; ++s) {
+            xc[(int64_t)s * Bp + b] = cv[s];
+            xc[qB + (int64_t)s * Bp + b] = zv[s];
+        }
+    }
+    if (tid < 8) xc[2 * qB + tid] = tid < q ? ws[(int64_t)q * kp + tid] : 0.0;
+}
+
+}  // namespace
+
+namespace sub {
+
+void init(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, const double *Q, const double *Minv,
+          int kp, double *aux, int64_t auxstride)
+{
+    if (ncmd <= 0) return;
+    hipLaunchKernelGGL(k_sinit, dim3(ncmd), dim3(256), 0, st, cmds, aln, Q, Minv, kp, aux, auxstride);
+}
+
+void pack(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, const double *V, int64_t vstride,
+          int64_t nvs, int kp, const double *aux, int64_t auxstride, double *Ap, double *Wc)
+{
+    if (ncmd <= 0) return;
+    hipLaunchKernelGGL(k_spack, dim3(ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, kp, aux, auxstride, Ap, Wc);
+}
+
+void softmax(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, const double *V, int64_t vstride,
+             int64_t nvs, int kp, const double *Ep, const double *Wc, double *Gp, int64_t ldG, double *sm_part,
+             int nchunk)
+{
+    if (ncmd <= 0) return;
+    hipLaunchKernelGGL(k_ssoftmax, dim3(nchunk, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, kp, Ep, Wc, Gp,
+                       ldG, sm_part, nchunk);
+}
+
+void grad(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride, int64_t nvs,
+          int kp, const double *aux, int64_t auxstride, const double *sm_part, int nchunk, double *Z, int nz,
+          int64_t zstride, double lamJ2)
+{
+    if (ncmd <= 0) return;
+    hipLaunchKernelGGL(k_sgrad, dim3(ncmd), dim3(128), 0, st, cmds, aln, V, vstride, nvs, kp, aux, auxstride, sm_part,
+                       nchunk, Z, nz, zstride, lamJ2);
+}
+
+void warm(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride, int64_t nvs,
+          const double *Qt, const double *KQt, const double *Q, int kp, const double *Wst, int64_t wstride)
+{
+    if (ncmd <= 0) return;
+    hipLaunchKernelGGL(k_gwarm, dim3(ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Qt, KQt, Q, kp, Wst,
+                       wstride);
+}
+
+}  // namespace sub
+}  // namespace sdca

@@ -45,7 +45,9 @@ P = sd.FORM_PRIMAL
 ACCURATE = [pytest.param(dict(formulation=P, precision=sd.PREC_FP32), id="fp32"),
             pytest.param(dict(formulation=P, precision=sd.PREC_SPLIT_BF16, fine_factor=0), id="split"),
             pytest.param(dict(formulation=P, precision=sd.PREC_SPLIT_BF16, fine_factor=10), id="split+fine"),
-            pytest.param(dict(formulation=sd.FORM_GRAM, j_threshold=JTHR_GRAM), id="gram")]
+            pytest.param(dict(formulation=sd.FORM_GRAM, j_threshold=JTHR_GRAM, subspace_rank=0), id="gram"),
+            # the subspace warm start (alignments of B >= 256; C1 solves cold)
+            pytest.param(dict(formulation=sd.FORM_GRAM, j_threshold=JTHR_GRAM, subspace_rank=128), id="gram+sub")]
 
 
 def with_jthr(kw):

@@ -37,11 +37,13 @@ def test_default_opts_match_reference_defaults():
     assert o.coarse_factor == 4  # split precision: BF16 high parts alone while ||g||_inf >= 4 * threshold
     assert o.j_threshold == 0.0  # SuperDCA's stopping rule alone unless a coupling threshold is asked for
     assert o.fine_factor == 10
+    assert o.subspace_threshold == 3e-2
 
 
 @pytest.mark.parametrize("bad", [{"coarse_factor": -1}, {"pipeline_groups": 3}, {"history": 7},
                                  {"gemm_tn_energy": 192}, {"gemm_gm_grad": 65}, {"precision": 9},
-                                 {"fine_factor": -1}, {"j_threshold": -1e-6}])
+                                 {"fine_factor": -1}, {"j_threshold": -1e-6}, {"subspace_rank": 64},
+                                 {"subspace_rank": 128, "subspace_threshold": 0.0}])
 def test_invalid_options_are_rejected_before_any_device_call(bad):
     # option checks precede device selection, so they hold without a GPU
     st = np.zeros((4, 3), dtype=np.uint8)
