@@ -8,12 +8,15 @@ Gram formulation, the alignment's identical-locus counts built once: reported) a
 when the last score row is in host memory.
 
 Headline (value): the Gram formulation -- each locus' couplings held as J = X_r^T c in
-the span of the data, FP64 evaluations with one B x B x q GEMM per round, cold start --
-stopping at ||g||_2 < 1e-3 and ||g_J||_2 < 3e-5 (2-norms: bounds on the infinity norms of
-SuperDCA's rule from above).  Its score error against the FP64 oracle is measured on the
-64 loci of the committed C4 fixture, solved with the headline settings.
+the span of the data, FP64 evaluations with one B x B x q GEMM per round -- warm-started
+from the locus' solve restricted to the dominant 128-dimensional eigen-subspace of the
+identical-locus counts (two B x 128 x q products per restricted evaluation), stopping at
+||g||_2 < 1e-3 and ||g_J||_2 < 3e-5 (2-norms: bounds on the infinity norms of SuperDCA's
+rule from above).  Its score error against the FP64 oracle is measured on the 64 loci of
+the committed C4 fixture, solved with the headline settings.
 
 Secondary legs on the same GPU:
+  gram-cold  the same loci, the same rule, from a cold start (zero)
   gram-rule  the Gram formulation at ||g||_2 < 1e-3 alone
   primal     split-BF16 contractions over the K_x one-hot columns, ||g||_inf < 1e-3 and
              ||g_J||_inf < 3e-6, cold and warm-started from the network (trained
@@ -50,9 +53,10 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_steps(ctx, order, pos, per_step, steps, label, solve, fvals=None):
+def run_steps(ctx, order, pos, per_step, steps, label, solve, fvals=None, subs=None):
     """Times `steps` solve calls; returns (loci/s, iterations, evaluations, seconds, new pos).
-    The final objective values are appended to `fvals` when given."""
+    The final objective values are appended to `fvals` when given, the subspace warm
+    start's evaluations to `subs`."""
     import superdca_amd as sd
 
     ctx.timing(reset=True)
@@ -67,6 +71,8 @@ def run_steps(ctx, order, pos, per_step, steps, label, solve, fvals=None):
         evals.append(st["nfeval"])
         if fvals is not None:
             fvals.append(st["fval"])
+        if subs is not None and "nfeval_sub" in st:
+            subs.append(st["nfeval_sub"])
         log(f"[bench] {label} step {k}: {converged} converged, {time.perf_counter() - t0:.2f}s")
     elapsed = time.perf_counter() - t0
     return converged / elapsed, np.concatenate(iters), np.concatenate(evals), elapsed, pos
@@ -87,11 +93,16 @@ def kernel_groups(tm, gram, split):
     (k_ghstep, k_gstep, k_gsoftmax) and the products pass (k_gdots, k_greduce) move the
     slot vectors.  Primal: the two split-BF16 (or FP32) contractions and the vector passes."""
     if gram:
-        return {
+        out = {
             "k_dgemm": (tm["gemm_grad_ms"], tm["flops_grad"], "mfma", tm["n_grad_launches"], PEAK["fp64_tflops"]),
             "k_gstep": (tm["update_ms"], tm["bytes_update"], "hbm", tm["n_evals"], PEAK["hbm_gbs"]),
             "k_gdots": (tm["dots_ms"], tm["bytes_dots"], "hbm", tm["n_evals"], PEAK["hbm_gbs"]),
         }
+        if tm.get("n_sub_evals"):
+            # the subspace warm start's rounds (all of their kernels), against the FP64 MFMA peak
+            out["subspace_rounds"] = (tm["subspace_ms"], tm["flops_sub"], "mfma", tm["n_sub_evals"],
+                                      PEAK["fp64_tflops"])
+        return out
     mp = PEAK["bf16_tflops"] if split else PEAK["fp32_tflops"]
     return {
         "k_gemm_nt<EpiGrad>": (tm["gemm_grad_ms"], tm["flops_grad"], "mfma", tm["n_grad_launches"], mp),
@@ -102,8 +113,12 @@ def kernel_groups(tm, gram, split):
 
 
 def roofline(tm, gram, split):
+    """The dominant kernel's roofline.  Gram: k_dgemm, the largest single kernel by device
+    time (the step and products passes are timed as groups of kernels: k_ghstep, k_gstep,
+    k_gsoftmax and the warm-start scatter; k_gdots and k_greduce).  Primal: the largest
+    group by device time."""
     groups = kernel_groups(tm, gram, split)
-    dom = max(groups, key=lambda k: groups[k][0])
+    dom = "k_dgemm" if gram else max(groups, key=lambda k: groups[k][0])
     ms, work, bound, n, peak = groups[dom]
     scale, unit = (1e12, "TFLOP/s") if bound == "mfma" else (1e9, "GB/s")
     achieved = work / (ms * 1e-3) / scale
@@ -211,6 +226,11 @@ def main() -> int:
                     help="headline slot groups: 1, or 2 pipelined on two streams (0: the library's default, 2)")
     ap.add_argument("--j-threshold", type=float, default=3e-5,
                     help="headline coupling threshold on ||g_J||_2 (0: ||g||_2 < 1e-3 alone)")
+    ap.add_argument("--subspace-rank", type=int, default=128,
+                    help="headline: subspace warm start of this rank (128; 0: cold start)")
+    ap.add_argument("--subspace-threshold", type=float, default=3e-2,
+                    help="the restricted solves stop at ||g||_2 below this")
+    ap.add_argument("--no-cold", action="store_true", help="skip the cold-start Gram leg on the headline's loci")
     ap.add_argument("--rule-loci", type=int, default=4096,
                     help="loci of the Gram leg at ||g||_2 < 1e-3 alone (0: skip)")
     ap.add_argument("--single-group-loci", type=int, default=2048,
@@ -256,7 +276,7 @@ def main() -> int:
     log(f"[bench] workload {wl}: B={B} L={L} generated in {time.time() - t:.1f}s")
     order = np.random.default_rng(1234).permutation(L).astype(np.int32)
     need = (args.warmup * per_step // 4 + args.steps * per_step + args.rule_loci + args.single_group_loci +
-            args.primal_loci + args.fp32_loci + 3 * args.primal_slots)
+            args.primal_loci + args.fp32_loci + 3 * args.primal_slots + 64)
     if need > L:
         order = np.resize(order, need)
     pos = 0
@@ -264,7 +284,8 @@ def main() -> int:
     fx = np.load(fx_path) if wl == "C4" and q == 5 and os.path.exists(fx_path) and not args.no_fixture else None
 
     # ---- headline: the Gram formulation, cold start
-    gram = dict(formulation=sd.FORM_GRAM, precision=sd.PREC_SPLIT_BF16, history=args.history, grad_threshold=1e-3)
+    gram = dict(formulation=sd.FORM_GRAM, precision=sd.PREC_SPLIT_BF16, history=args.history, grad_threshold=1e-3,
+                subspace_rank=args.subspace_rank, subspace_threshold=args.subspace_threshold)
     ctx = sd.Context(states, q=q, batch_loci=slots, pipeline_groups=args.groups, j_threshold=jthr, **gram)
     t = time.time()
     w, _ = ctx.reweight(0.9)
@@ -281,7 +302,10 @@ def main() -> int:
         _, wst, _, _ = ctx.solve(order[pos:pos + per_step // 4])
         assert np.all(wst["nfeval"] > 1), "warmup loci stopped at their start point"
         pos += per_step // 4
-    head_v, head_it, head_ev, head_s, pos = run_steps(ctx, order, pos, per_step, args.steps, "gram", ctx.solve)
+    head_from = pos
+    head_f, head_sub = [], []
+    head_v, head_it, head_ev, head_s, pos = run_steps(ctx, order, pos, per_step, args.steps, "gram", ctx.solve, head_f,
+                                                      head_sub)
     head_tm = ctx.timing()
     accuracy = None
     if fx is not None:
@@ -290,6 +314,23 @@ def main() -> int:
         accuracy["converged"] = int((st_fx["status"] == sd.ST_CONVERGED).sum())
         log(f"[bench] headline score error on the fixture: {accuracy}")
     ctx.close()
+
+    # ---- the same loci from a cold start (zero): warm vs cold on the same GPU
+    gram_cold = None
+    if args.subspace_rank > 0 and not args.no_cold:
+        with sd.Context(states, q=q, batch_loci=slots, pipeline_groups=args.groups, j_threshold=jthr,
+                        **{**gram, "subspace_rank": 0}) as cc:
+            cc.set_weights(w)
+            cc.solve(order[pos:pos + 8])
+            pos += 8
+            cold_f = []
+            cv, cit, cev, _, _ = run_steps(cc, order, head_from, per_step, args.steps, "gram-cold", cc.solve, cold_f)
+            ctm = cc.timing()
+        fh, fc = np.concatenate(head_f), np.concatenate(cold_f)
+        frel = np.abs(fh - fc) / np.abs(fc)
+        gram_cold = {"loci_s": round(cv, 3), "evals_median": float(np.median(cev)), "iter_median": float(np.median(cit)),
+                     "rounds": ctm["n_evals"], "device_ms": device_ms(ctm),
+                     "f_warm_vs_cold_rel": {"median": float(np.median(frel)), "max": float(frel.max())}}
 
     # ---- the same with one slot group: the kernels' rates with the GPU to themselves
     single = None
@@ -408,7 +449,13 @@ def main() -> int:
     conv = f"||g||_2<1e-3 and ||g_J||_2<{jthr:g}" if jthr > 0 else "||g||_2<1e-3"
     detail = {
         "formulation": "gram",
+        "start": (f"subspace warm start (rank {args.subspace_rank}, restricted solves to ||g||_2 < "
+                  f"{args.subspace_threshold:g})") if args.subspace_rank > 0 else "cold",
         "convergence": conv,
+        "gram_cold": gram_cold,
+        "warm_over_cold_gram": round(head_v / gram_cold["loci_s"], 3) if gram_cold else None,
+        "evals_sub_median_headline": float(np.median(np.concatenate(head_sub))) if head_sub else None,
+        "subspace_ms_headline": round(head_tm["subspace_ms"], 1),
         "score_error_headline": accuracy,
         "first_call_s": round(first_s, 3) if first_s is not None else None,
         "evals_median_headline": float(np.median(head_ev)),
@@ -427,6 +474,8 @@ def main() -> int:
         "headline_over_primal_cold_split": round(head_v / primal["cold_split_loci_s"], 3) if primal else None,
         "reweight_ms": round(reweight_s * 1e3, 2), "B_eff": beff,
     }
+    start_desc = (f"warm start from the locus' solve in the dominant {args.subspace_rank}-dim eigen-subspace of the "
+                  "identical-locus counts") if args.subspace_rank > 0 else "cold start"
     out = {
         "metric": METRIC,
         "value": round(head_v, 4),
@@ -440,10 +489,10 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"{wl}: B={B} sequences x N={L} loci, q_env={q}, cold start, Gram formulation (FP64), "
-                               f"{per_step} loci per step ({slots} in flight), L-BFGS m={args.history}, {conv}",
+        "config": {"workload": f"{wl}: B={B} sequences x N={L} loci, q_env={q}, {start_desc}, Gram formulation "
+                               f"(FP64), {per_step} loci per step ({slots} in flight), L-BFGS m={args.history}, {conv}",
                    "B": B, "N": L, "q_env": q, "loci_per_step": per_step, "slots": slots, "formulation": "gram",
-                   "precision": "fp64", "start": "cold", "convergence": conv,
+                   "precision": "fp64", "start": start_desc, "convergence": conv,
                    "parallelism": "loci batched on 1 GPU"},
         "roofline": rl,
         "cpu_baseline": cpu,

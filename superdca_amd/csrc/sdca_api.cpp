@@ -1986,7 +1986,7 @@ int ensure_subspace(sdca_ctx *A, hipStream_t st)
         hipError_t e = hipMemcpyAsync(dQt, Qt.data(), Qt.size() * sizeof(double), hipMemcpyHostToDevice, st);
         if (e != hipSuccess) return e;
         // Y^T[i][b] = sum_b' Qt[i][b'] K[b][b']
-        sdca::gram::gemm(st, A->d_gram, Bp, dQt, Bp, dY, Bp, Bp, kp, Bp);
+        sdca::gram::gemm_splitk(st, A->d_gram, Bp, dQt, Bp, dY, Bp, Bp, kp, Bp, Bp, 0);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipMemcpyAsync(out.data(), dY, out.size() * sizeof(double), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -2160,7 +2160,7 @@ int sub_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds)
     sdca::sub::init(st, dc, nc, G->d_galn, A->d_subQ, A->d_subMinv, kp, G->aux, G->auxstride);
     sdca::gram::step_vectors(st, m, dc, nc, G->d_galn_r, G->Vr, G->vstride_r, G->nvs_r, upd, G->nchq_r);
     sdca::sub::pack(st, dc, nc, G->d_galn, G->Vr, G->vstride_r, G->nvs_r, kp, G->aux, G->auxstride, Ap, Wc);
-    sdca::gram::gemm(st, A->d_subKQ, kp, Ap, kp, Ep, G->ldG, A->Bp, col, kp);
+    sdca::gram::gemm_splitk(st, A->d_subKQ, kp, Ap, kp, Ep, G->ldG, A->Bp, col, kp, kp, 0);
     sdca::sub::softmax(st, dc, nc, G->d_galn, G->Vr, G->vstride_r, G->nvs_r, kp, Ep, Wc, Gp, G->ldG, sm, G->nchunk);
     // split-K over the sequences: slice z of the product at Zs + z * zs (Zs after the slot
     // groups' first slices), summed by the gradient kernel into Zr

@@ -341,7 +341,8 @@ constexpr int kGemmCols = 128;
 void gemm(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_t ldb, double *C, int64_t ldc, int M,
           int N, int Kd);
 // The same in ceil(Kd / kc) split-K slices (kc a multiple of 16): slice z sums k in
-// [z kc, (z + 1) kc) into C + z zstride.
+// [z kc, (z + 1) kc) into C + z zstride.  A kernel instance of its own (k_dgemm<1>): the
+// subspace warm start's products.
 void gemm_splitk(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_t ldb, double *C, int64_t ldc,
                  int M, int N, int Kd, int kc, int64_t zstride);
 // Gradient images from Z = K G and the products: dot_part [cmd][nchd][2m + 5], nchd =

@@ -449,6 +449,13 @@ def test_bench_roofline_fields_from_timings():
     assert rl["frac"] == pytest.approx(50.0 / 78.6, rel=1e-4)
     phases, _ = bench.phase_fractions(tg, True, True)
     assert set(phases) == {"k_dgemm", "k_gstep", "k_gdots"}
+    # with the subspace warm start: its rounds as a phase of their own; the roofline stays on
+    # k_dgemm even when the step pass (a group of kernels) takes more device time
+    ts = dict(tg, subspace_ms=1.0 * r, flops_sub=1.0e10 * r, n_sub_evals=r, update_ms=9.0 * r)
+    phases, _ = bench.phase_fractions(ts, True, True)
+    assert set(phases) == {"k_dgemm", "k_gstep", "k_gdots", "subspace_rounds"}
+    assert phases["subspace_rounds"]["rate"] == pytest.approx(1.0e10 / 1.0e-3 / 1e12, rel=1e-3)
+    assert bench.roofline(ts, True, True)["kernel"] == "k_dgemm"
 
 
 def test_every_hip_compile_disables_packed_fp32_and_targets_gfx950():
