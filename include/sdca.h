@@ -50,7 +50,8 @@ typedef struct sdca_opts {
     int32_t precision;       /* SDCA_PREC_* (default SDCA_PREC_FP32) */
     int32_t batch_loci;      /* loci solved concurrently on the GPU (default 128; 0: 4096 for the
                                 Gram formulation, 128 for the primal one) */
-    int32_t history;         /* L-BFGS history m: 1, 3, 5, 8, 10 (default) or 16 */
+    int32_t history;         /* L-BFGS history m: 1, 3, 5, 8, 10 or 16; the Gram formulation also 24
+                                and 32.  0 (default): the formulation's own, 24 for Gram, 10 primal */
     int32_t max_iter;        /* iteration cap per locus (default 2000, plmDCA.hpp:219) */
     int32_t max_ls;          /* function evaluations per line search (default 20) */
     double grad_threshold;   /* stop when ||g||_inf < this (default 1e-3, plmDCA_options.cpp:61) */
@@ -160,6 +161,7 @@ typedef struct sdca_timing {
     double subspace_ms;      /* rounds of the subspace warm start (sdca_opts.subspace_rank) */
     int64_t n_sub_evals;     /* their batched evaluations (rounds) */
     double flops_sub;        /* their algorithmic flops: 4 * B * 128 * q per locus-evaluation */
+    double host_ms;          /* host time stepping the slots' L-BFGS state between rounds (Gram) */
 } sdca_timing;
 
 /* Message of the calling thread's last failure (SuperDCA prints its errors to

@@ -29,8 +29,13 @@
 #include <cstdlib>
 #include <deque>
 #include <map>
+#include <condition_variable>
+#include <functional>
 #include <mutex>
 #include <random>
+#include <thread>
+
+#include <sched.h>
 #include <string>
 #include <vector>
 
@@ -101,7 +106,15 @@ void dfree(T *&p)
 
 inline int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
 
-bool valid_history(int m) { return m == 1 || m == 3 || m == 5 || m == 8 || m == 10 || m == 16; }
+bool valid_history(int m)
+{
+    return m == 0 || m == 1 || m == 3 || m == 5 || m == 8 || m == 10 || m == 16 || m == 24 || m == 32;
+}
+
+// The L-BFGS history of a solve: opts.history, or with 0 the formulation's default (Gram
+// 24: its vector passes are cheap against the evaluations a longer history saves; primal
+// 10).
+int history_for(const sdca_opts &o, bool gram) { return o.history > 0 ? o.history : (gram ? 24 : 10); }
 
 // Loci in flight: opts.batch_loci, or with 0 the formulation's default (Gram slots are
 // small -- 6.1 MB at B = 3000, q = 5 -- so 4096 of them; primal slots hold q x K_x vectors).
@@ -1448,6 +1461,96 @@ private:
     int max_level_;      // LV_FINE with a coupling threshold, else LV_SPLIT
 };
 
+// ------------------------------------------------------- host worker threads
+
+// A fixed set of worker threads for the host's per-slot L-BFGS steps between rounds:
+// run(lo, hi, fn) calls fn(i) for every i in [lo, hi), split into one contiguous range per
+// thread (the calling thread takes the first).  Each slot's step touches that slot's state
+// and command alone, so the ranges are independent; the result is the same as a serial loop.
+class SlotPool {
+public:
+    explicit SlotPool(int nthreads) : nt_(std::max(1, nthreads))
+    {
+        for (int t = 1; t < nt_; ++t) th_.emplace_back([this, t] { worker(t); });
+    }
+    ~SlotPool()
+    {
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            stop_ = true;
+            ++gen_;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void run(int lo, int hi, const std::function<void(int)> &fn)
+    {
+        if (nt_ <= 1 || hi - lo < 256) {
+            for (int i = lo; i < hi; ++i) fn(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> l(mu_);
+            fn_ = &fn;
+            lo_ = lo;
+            hi_ = hi;
+            pending_ = nt_ - 1;
+            ++gen_;
+        }
+        cv_.notify_all();
+        range(0, lo, hi, fn);
+        std::unique_lock<std::mutex> l(mu_);
+        done_.wait(l, [&] { return pending_ == 0; });
+        fn_ = nullptr;
+    }
+    // the worker count for this process: its CPUs (the affinity mask), at most 8
+    static int default_threads()
+    {
+        cpu_set_t set;
+        int n = 1;
+        if (sched_getaffinity(0, sizeof(set), &set) == 0) n = CPU_COUNT(&set);
+        return std::max(1, std::min(8, n));
+    }
+
+private:
+    void range(int t, int lo, int hi, const std::function<void(int)> &fn) const
+    {
+        const int64_t n = hi - lo;
+        const int a = lo + (int)(n * t / nt_), b = lo + (int)(n * (t + 1) / nt_);
+        for (int i = a; i < b; ++i) fn(i);
+    }
+    void worker(int t)
+    {
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(int)> *fn;
+            int lo, hi;
+            {
+                std::unique_lock<std::mutex> l(mu_);
+                cv_.wait(l, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (stop_) return;
+                fn = fn_;
+                lo = lo_;
+                hi = hi_;
+            }
+            range(t, lo, hi, *fn);
+            {
+                std::lock_guard<std::mutex> l(mu_);
+                if (--pending_ == 0) done_.notify_one();
+            }
+        }
+    }
+    int nt_;
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    const std::function<void(int)> *fn_ = nullptr;
+    int lo_ = 0, hi_ = 0, pending_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
 // ------------------------------------------------------------ Gram formulation
 //
 // The solve in the representation of sdca_gram.hip: FP64 slot vectors of coefficients,
@@ -2214,6 +2317,8 @@ int sub_solve(sdca_ctx *c, const int32_t *loci, int32_t K, std::vector<int> &nev
     const int ng = G->ngroups, gs = G->gs;
     const int q = c->members[0]->q;
     std::vector<double> res((size_t)nslots * R.width(), 0.0);
+    std::vector<char> done_flag(nslots, 0);
+    SlotPool pool(SlotPool::default_threads());
     nev.assign(K, 0);
     ok.assign(K, 0);
     int next = 0, done = 0;
@@ -2249,10 +2354,13 @@ int sub_solve(sdca_ctx *c, const int32_t *loci, int32_t K, std::vector<int> &nev
             inflight[g] = false;
             if ((rc = sub_finish_round(c, g, res))) return rc;
             finished.clear();
-            for (int i = g * gs; i < group_end(g); ++i) {
-                if (S.slot(i).phase == PH_IDLE) continue;
-                if (S.process(i, res.data() + (size_t)i * R.width())) finished.push_back(i);
-            }
+            const auto th = std::chrono::steady_clock::now();
+            pool.run(g * gs, group_end(g), [&](int i) {
+                done_flag[i] = S.slot(i).phase != PH_IDLE && S.process(i, res.data() + (size_t)i * R.width());
+            });
+            for (int i = g * gs; i < group_end(g); ++i)
+                if (done_flag[i]) finished.push_back(i);
+            c->timing.host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th).count();
             hipStream_t st = c->gstream[g];
             for (int i : finished) {
                 const Slot &sl = S.slot(i);
@@ -2289,7 +2397,7 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
     for (sdca_ctx *A : c->members)
         if ((rc = ensure_gram_matrix(A, c->stream))) return rc;
     const int nslots = slots_for(c->opts, true, K);
-    const int m = c->opts.history;
+    const int m = history_for(c->opts, true);
     if ((rc = ensure_gram_ws(c, nslots, m))) return rc;
     if ((rc = upload_gram_alns(c))) return rc;
     const sdca::ResLayout R{m};
@@ -2307,6 +2415,8 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
     Solver S(c, nslots, m, true);
     const int ng = G->ngroups, gs = G->gs;
     std::vector<double> res((size_t)nslots * R.width(), 0.0);
+    std::vector<char> done_flag(nslots, 0);
+    SlotPool pool(SlotPool::default_threads());
     int next = 0, done = 0, noconv = 0, nonfinite = 0;
     auto fill = [&](int i) {
         if (next >= K) {
@@ -2343,10 +2453,13 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
             if ((rc = gram_finish_round(c, g, S.cmds(), res))) return rc;
             ++rounds;
             finished.clear();
-            for (int i = g * gs; i < group_end(g); ++i) {
-                if (S.slot(i).phase == PH_IDLE) continue;
-                if (S.process(i, res.data() + (size_t)i * R.width())) finished.push_back(i);
-            }
+            const auto th = std::chrono::steady_clock::now();
+            pool.run(g * gs, group_end(g), [&](int i) {
+                done_flag[i] = S.slot(i).phase != PH_IDLE && S.process(i, res.data() + (size_t)i * R.width());
+            });
+            for (int i = g * gs; i < group_end(g); ++i)
+                if (done_flag[i]) finished.push_back(i);
+            c->timing.host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th).count();
             if (!finished.empty()) {
                 if ((rc = gram_stage(c, g, S, finished, st_out))) return rc;
                 for (int i : finished) {
@@ -2437,7 +2550,7 @@ void sdca_default_opts(sdca_opts *o)
     o->device = 0;
     o->precision = SDCA_PREC_FP32;
     o->batch_loci = 128;
-    o->history = 10;
+    o->history = 0;
     o->max_iter = 2000;
     o->max_ls = 20;
     o->grad_threshold = 1e-3;
@@ -2477,7 +2590,7 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
         return fail(SDCA_E_ARG, "unknown precision %d", o.precision);
     if (o.formulation < SDCA_FORM_AUTO || o.formulation > SDCA_FORM_GRAM)
         return fail(SDCA_E_ARG, "unknown formulation %d", o.formulation);
-    if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 8, 10, 16 (got %d)", o.history);
+    if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be 0 (the formulation's default) or one of 1, 3, 5, 8, 10, 16, 24, 32 (got %d)", o.history);
     if (o.batch_loci < 0) return fail(SDCA_E_ARG, "batch_loci must be >= 1 (or 0: the formulation's default)");
     if (o.max_iter < 1 || o.max_ls < 1) return fail(SDCA_E_ARG, "max_iter and max_ls must be >= 1");
     if (!(o.grad_threshold >= 0.0)) return fail(SDCA_E_ARG, "grad_threshold must be >= 0");
@@ -2650,7 +2763,10 @@ int sdca_eval_batch(sdca_ctx *c, const int32_t *loci, int32_t K, const double *x
         if (loci[k] < 0 || loci[k] >= c->L) return fail(SDCA_E_ARG, "locus %d out of range", loci[k]);
     if (K == 0) return SDCA_OK;
     const int nslots = slots_for(c->opts, false, K);
-    if ((rc = ensure_workspace(c, nslots, c->opts.history))) return rc;
+    if (history_for(c->opts, false) > 16)
+        return fail(SDCA_E_ARG, "history %d: histories above 16 are supported by the Gram formulation only",
+                    c->opts.history);
+    if ((rc = ensure_workspace(c, nslots, history_for(c->opts, false)))) return rc;
     if ((rc = upload_alns(c))) return rc;
     if (!c->d_X0) {
         if ((rc = dalloc(&c->d_X0, (size_t)nslots * c->n_slot))) return rc;
@@ -2753,9 +2869,12 @@ int sdca_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
     if (c->opts.formulation == SDCA_FORM_GRAM && (warm || cw))
         return fail(SDCA_E_ARG, "the Gram formulation solves from a cold start only");
     if (gram_wanted(c, warm || cw)) return gram_solve_batch(c, alns, loci, K, score_rows, row_off, x_out, st_out);
+    if (history_for(c->opts, false) > 16)
+        return fail(SDCA_E_ARG, "history %d: histories above 16 are supported by the Gram formulation only",
+                    c->opts.history);
     const auto t0 = std::chrono::steady_clock::now();
     const int nslots = slots_for(c->opts, false, K);
-    if ((rc = ensure_workspace(c, nslots, c->opts.history))) return rc;
+    if ((rc = ensure_workspace(c, nslots, history_for(c->opts, false)))) return rc;
     if ((rc = upload_alns(c))) return rc;
     const bool any_warm = warm || cw;
     if (any_warm && !c->d_X0) {
@@ -3119,6 +3238,7 @@ int sdca_get_timing(sdca_ctx *c, sdca_timing *t, int32_t reset)
     t->subspace_ms = s.sub;
     t->n_sub_evals = s.sub_evals;
     t->flops_sub = s.flops_sub;
+    t->host_ms = s.host;
     if (reset) c->timing = sdca::Timing();
     return SDCA_OK;
 }
@@ -3149,7 +3269,7 @@ int sdca_multi_create(sdca_multi **out, sdca_ctx *const *ctxs, int32_t n, const 
         return fail(SDCA_E_ARG, "unknown precision %d", o.precision);
     if (o.formulation < SDCA_FORM_AUTO || o.formulation > SDCA_FORM_GRAM)
         return fail(SDCA_E_ARG, "unknown formulation %d", o.formulation);
-    if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be one of 1, 3, 5, 8, 10, 16 (got %d)", o.history);
+    if (!valid_history(o.history)) return fail(SDCA_E_ARG, "history must be 0 (the formulation's default) or one of 1, 3, 5, 8, 10, 16, 24, 32 (got %d)", o.history);
     if (o.batch_loci < 0 || o.max_iter < 1 || o.max_ls < 1 || !(o.grad_threshold >= 0.0) || !(o.j_threshold >= 0.0) ||
         o.coarse_factor < 0 || o.fine_factor < 0 || (o.subspace_rank != 0 && o.subspace_rank != sdca::kSubRank))
         return fail(SDCA_E_ARG, "invalid solver options");

@@ -43,7 +43,7 @@ struct Options {
     bool no_filter = false;
     int output_base = 1, input_base = 1;
     long begin_locus = 0, end_locus = -1;
-    int device = 0, batch_loci = 0, history = 10, q_env = 4, max_iter = 2000;
+    int device = 0, batch_loci = 0, history = 0, q_env = 4, max_iter = 2000;
     int precision = SDCA_PREC_FP32;
     int formulation = SDCA_FORM_AUTO;
     int chunk_loci = 0;  // loci per solve call (0: 4096, or fewer with --norm-of-mean-scoring)
@@ -87,7 +87,7 @@ const char *kUsage =
     "  --output-top arg               Also write the arg largest scores, largest first (.top).\n"
     "  --output-min-score arg         Also write every score >= arg, largest first (.top).\n"
     "GPU:\n"
-    "  --device arg (=0)  --history arg (=10)  --max-iter arg (=2000)\n"
+    "  --device arg (=0)  --history arg (=0: 24 Gram, 10 primal)  --max-iter arg (=2000)\n"
     "  --batch-loci arg (=0)          Loci in flight (0: 4096 for the Gram formulation, 128 otherwise).\n"
     "  --formulation auto|gram|primal (=auto)\n"
     "                                 gram: couplings in the span of the sequences, FP64, one B x B\n"

@@ -287,6 +287,69 @@ __device__ __forceinline__ void load_step(const SlotCmd &c, StepArgs &a, double 
     }
 }
 
+// NEWDIR of step_elems for histories above 16: the same arithmetic in the same order, with
+// the pairs loaded eight at a time (all 2m loads in flight at once would not fit the
+// registers).
+template <int M, int NE>
+__device__ __forceinline__ void step_newdir_chunked(const StepArgs &a, const double (&cs)[M], const double (&cy)[M],
+                                                    const SlotPtrs &P, int64_t nvs, const int64_t (&e)[NE],
+                                                    double (&xn)[NE], double (&dn)[NE], double (&gv)[NE])
+{
+    constexpr int CH = 8;
+    static_assert(M % CH == 0, "history a multiple of the chunk");
+    double dold[NE], xv[NE], g[NE], go[NE], sn[NE], yn[NE], d[NE];
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        dold[k] = P.dv[e[k]];
+        xv[k] = P.xc[e[k]];
+        g[k] = P.gk[e[k]];
+        go[k] = a.pnew >= 0 ? P.gold[e[k]] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        if (a.accept) {
+            xv[k] = fma(a.pstep, dold[k], xv[k]);
+            P.xc[e[k]] = xv[k];
+        }
+        sn[k] = yn[k] = 0.0;
+        if (a.pnew >= 0) {
+            sn[k] = a.pstep * dold[k];
+            yn[k] = g[k] - go[k];
+            P.Sb[(int64_t)a.pnew * nvs + e[k]] = sn[k];
+            P.Yb[(int64_t)a.pnew * nvs + e[k]] = yn[k];
+        }
+        d[k] = a.cd * dold[k] + a.cg * g[k];
+    }
+    for (int p0 = 0; p0 < M; p0 += CH) {
+        double sv[CH][NE], yv[CH][NE];
+#pragma unroll
+        for (int u = 0; u < CH; ++u)
+#pragma unroll
+            for (int k = 0; k < NE; ++k) {
+                const int p = p0 + u;
+                sv[u][k] = (cs[p] != 0.0 && p != a.pnew) ? P.Sb[(int64_t)p * nvs + e[k]] : 0.0;
+                yv[u][k] = (cy[p] != 0.0 && p != a.pnew) ? P.Yb[(int64_t)p * nvs + e[k]] : 0.0;
+            }
+#pragma unroll
+        for (int k = 0; k < NE; ++k)
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const int p = p0 + u;
+                const bool mine = p == a.pnew;
+                d[k] += cs[p] * (mine ? sn[k] : sv[u][k]);
+                d[k] += cy[p] * (mine ? yn[k] : yv[u][k]);
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+        P.dv[e[k]] = d[k];
+        xn[k] = fma(a.alpha, d[k], xv[k]);
+        P.xt[e[k]] = xn[k];
+        dn[k] = d[k];
+        gv[k] = g[k];
+    }
+}
+
 // NE elements e[k] of a slot's vectors (coefficients, image entries or fields), as k_update
 // does it for the primal vectors:
 //   INIT    x_c = 0, x_t = 0; with a staged start (init_src, written into x_c by k_gwarm)
@@ -338,7 +401,12 @@ __device__ __forceinline__ void step_elems(const StepArgs &a, const double (&cs)
         }
         return;
     }
-    double dold[NE], xv[NE], g[NE], go[NE], sv[M][NE], yv[M][NE];
+    if constexpr (M > 16) {
+        step_newdir_chunked<M, NE>(a, cs, cy, P, nvs, e, xn, dn, gv);
+        return;
+    }
+    constexpr int MS = M > 16 ? 1 : M;  // (the branch above serves longer histories)
+    double dold[NE], xv[NE], g[NE], go[NE], sv[MS][NE], yv[MS][NE];
 #pragma unroll
     for (int k = 0; k < NE; ++k) {
         dold[k] = P.dv[e[k]];
@@ -347,7 +415,7 @@ __device__ __forceinline__ void step_elems(const StepArgs &a, const double (&cs)
         go[k] = a.pnew >= 0 ? P.gold[e[k]] : 0.0;
     }
 #pragma unroll
-    for (int p = 0; p < M; ++p)
+    for (int p = 0; p < MS; ++p)
 #pragma unroll
         for (int k = 0; k < NE; ++k) {
             sv[p][k] = (cs[p] != 0.0 && p != a.pnew) ? P.Sb[(int64_t)p * nvs + e[k]] : 0.0;
@@ -368,7 +436,7 @@ __device__ __forceinline__ void step_elems(const StepArgs &a, const double (&cs)
         }
         double d = a.cd * dold[k] + a.cg * g[k];
 #pragma unroll
-        for (int p = 0; p < M; ++p) {
+        for (int p = 0; p < MS; ++p) {
             const bool mine = p == a.pnew;
             d += cs[p] * (mine ? sn : sv[p][k]);
             d += cy[p] * (mine ? yn : yv[p][k]);
@@ -606,23 +674,27 @@ __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds,
         const double xi = xt[e1];
         const double gc = gt[e0];
         const double gki = cur ? gk[e1] : 0.0, di = cur ? dv[e1] : 0.0;
-        double sp[M], yp[M];
-#pragma unroll
-        for (int p = 0; p < M; ++p) {
-            const bool on = (live >> p) & 1u;
-            sp[p] = on ? Sb[(int64_t)p * nvs + e1] : 0.0;
-            yp[p] = on ? Yb[(int64_t)p * nvs + e1] : 0.0;
-        }
+        // the pairs' images, CH at a time (all of them for histories up to 16)
+        constexpr int CH = M > 16 ? 8 : M;
         const double kg = seq ? z - (A.own ? sT[s * 8 + y] : 0.0) + A.lamJ2 * xi : 0.0;
         gt[e1] = kg;
         acc[2 * M] += gc * kg;
         acc[2 * M + 4] += gc * kg;
         acc[2 * M + 1] += gc * gki;
         acc[2 * M + 2] += gc * di;
+        for (int p0 = 0; p0 < M; p0 += CH) {
+            double sp[CH], yp[CH];
 #pragma unroll
-        for (int p = 0; p < M; ++p) {
-            acc[p] += gc * sp[p];
-            acc[M + p] += gc * yp[p];
+            for (int u = 0; u < CH; ++u) {
+                const bool on = (live >> (p0 + u)) & 1u;
+                sp[u] = on ? Sb[(int64_t)(p0 + u) * nvs + e1] : 0.0;
+                yp[u] = on ? Yb[(int64_t)(p0 + u) * nvs + e1] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                acc[p0 + u] += gc * sp[u];
+                acc[M + p0 + u] += gc * yp[u];
+            }
         }
     }
     if (blockIdx.x == 0 && tid < q) {
@@ -750,6 +822,8 @@ void gemm_splitk(hipStream_t st, const double *A, int64_t lda, const double *Bm,
     case 8: CALL(8); break;         \
     case 10: CALL(10); break;       \
     case 16: CALL(16); break;       \
+    case 24: CALL(24); break;       \
+    case 32: CALL(32); break;       \
     default: break;                 \
     }
 

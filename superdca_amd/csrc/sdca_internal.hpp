@@ -25,7 +25,7 @@ namespace sdca {
 
 constexpr int kTile = 128;        // GEMM output tile rows (and narrow tile columns)
 constexpr int kTileWide = 256;    // wide GEMM tile columns; B and K_x are padded to this
-constexpr int kMaxHistory = 16;   // largest L-BFGS history supported
+constexpr int kMaxHistory = 32;   // largest L-BFGS history supported (the primal kernels: 16)
 constexpr int kMaxGroups = 2;     // slot groups whose rounds are pipelined on separate streams
 
 // BF16 <-> FP32 (round to nearest even; NaN stays NaN).
@@ -110,6 +110,7 @@ struct Timing {
     double update = 0, dots = 0, bytes_update = 0, bytes_dots = 0;
     double sub = 0, flops_sub = 0;
     int64_t sub_evals = 0;
+    double host = 0;  // host time stepping the slots' L-BFGS state between rounds
 };
 
 // One alignment of a solve, as the kernels see it.  A solve covers one alignment (a

@@ -167,10 +167,17 @@ def test_solve_accurate_matches_oracle_optimum(c1, c1_optimum, kw):
           f"max rel err (top decile) {rel_top:.3e}, parameters max err {xerr:.3e} of max |x|")
     assert rel_top < 1e-4
     if kw.get("formulation") == sd.FORM_GRAM:
-        # FP64 evaluations: f at the optimum to its rounding, parameters (formed once, with
-        # FP32 products) to FP32 accuracy
+        # FP64 evaluations: f at the optimum to its rounding.  The parameters are as close as
+        # the stopping point allows: the objective is 2 min(lambda_h, lambda_J)-strongly
+        # convex, so ||x - x*||_2 <= ||g(x)||_2 / (2 min(lambda_h, lambda_J)) with the reported
+        # 2-norm of the final gradient; the couplings are formed once with FP32 products
+        # (1e-6 of ||x*||_2 on top).  A wrong layout, sign or scale misses this by orders.
         np.testing.assert_allclose(stats["fval"], c1_optimum["stats"]["fval"], rtol=1e-10)
-        assert xerr < 1e-5
+        mu = 2.0 * min(c1_optimum["lam"])
+        dist = np.linalg.norm(X - c1_optimum["x"], axis=1)
+        bound = stats["gnorm"] / mu + 1e-6 * np.linalg.norm(c1_optimum["x"], axis=1)
+        print(f"  parameters: largest distance / bound {np.max(dist / bound):.3f}")
+        assert np.all(dist <= bound)
 
 
 def test_reference_mode_threshold_and_refill(c1, c1_optimum):
