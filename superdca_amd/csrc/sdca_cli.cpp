@@ -47,6 +47,7 @@ struct Options {
     int precision = SDCA_PREC_FP32;
     int formulation = SDCA_FORM_AUTO;
     int chunk_loci = 0;  // loci per solve call (0: 4096, or fewer with --norm-of-mean-scoring)
+    int subspace_rank = 128;  // Gram: warm start from the dominant eigen-subspace (0: cold)
 };
 
 const char *kUsage =
@@ -93,6 +94,8 @@ const char *kUsage =
     "                                 gram: couplings in the span of the sequences, FP64, one B x B\n"
     "                                 GEMM per evaluation, stopping on 2-norms; primal: the q x K_x\n"
     "                                 parameters (--precision); auto: gram when B <= K_x.\n"
+    "  --subspace-rank arg (=128)     Gram: start each locus from its solve in the dominant 128-dim\n"
+    "                                 eigen-subspace of the identical-locus counts (0: from zero).\n"
     "  --chunk-loci arg (=4096)       Loci per solve call; the scores stream through a file next to\n"
     "                                 the output, so host memory is O(chunk x loci).\n"
     "  --precision fp32|split-bf16 (=fp32)  --q-env arg (=4; 5 keeps four alleles plus gap)\n";
@@ -171,6 +174,7 @@ Options parse_args(int argc, char **argv)
         else if (a == "--batch-loci") o.batch_loci = std::atoi(need(i).c_str());
         else if (a == "--chunk-loci") o.chunk_loci = std::atoi(need(i).c_str());
         else if (a == "--history") o.history = std::atoi(need(i).c_str());
+        else if (a == "--subspace-rank") o.subspace_rank = std::atoi(need(i).c_str());
         else if (a == "--max-iter") o.max_iter = std::atoi(need(i).c_str());
         else if (a == "--q-env") o.q_env = std::atoi(need(i).c_str());
         else if (a == "--formulation") {
@@ -264,6 +268,7 @@ sdca_opts solver_opts(const Options &o)
     so.formulation = o.formulation;
     so.batch_loci = o.batch_loci;  // 0: the library's default for the formulation it picks
     so.history = o.history;
+    so.subspace_rank = o.subspace_rank;
     so.max_iter = o.max_iter;
     so.grad_threshold = o.gradient_threshold;
     so.j_threshold = o.coupling_threshold;
