@@ -162,6 +162,8 @@ typedef struct sdca_timing {
     int64_t n_sub_evals;     /* their batched evaluations (rounds) */
     double flops_sub;        /* their algorithmic flops: 4 * B * 128 * q per locus-evaluation */
     double host_ms;          /* host time stepping the slots' L-BFGS state between rounds (Gram) */
+    double gstep_ms;         /* the Gram step pass k_gstep alone (part of update_ms) */
+    double bytes_gstep;      /* its algorithmic HBM bytes */
 } sdca_timing;
 
 /* Message of the calling thread's last failure (SuperDCA prints its errors to

@@ -85,7 +85,8 @@ class Timing(C.Structure):
                 ("n_grad_launches", C.c_int64), ("flops_energy", C.c_double), ("flops_grad", C.c_double),
                 ("bytes_lbfgs", C.c_double), ("update_ms", C.c_double), ("dots_ms", C.c_double),
                 ("bytes_update", C.c_double), ("bytes_dots", C.c_double), ("subspace_ms", C.c_double),
-                ("n_sub_evals", C.c_int64), ("flops_sub", C.c_double), ("host_ms", C.c_double)]
+                ("n_sub_evals", C.c_int64), ("flops_sub", C.c_double), ("host_ms", C.c_double),
+                ("gstep_ms", C.c_double), ("bytes_gstep", C.c_double)]
 
 
 _lib = None

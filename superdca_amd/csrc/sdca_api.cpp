@@ -242,6 +242,8 @@ sdca::GramWs::~GramWs()
     dfree(d_sloc);
     dfree(d_sld);
     dfree(d_rows);
+    if (h_stage) (void)hipHostFree(h_stage);
+    dfree(d_stage);
     if (h_cmds) (void)hipHostFree(h_cmds);
     if (h_res) (void)hipHostFree(h_res);
     if (h_rows) (void)hipHostFree(h_rows);
@@ -1660,6 +1662,8 @@ int ensure_gram_ws(sdca_ctx *c, int nslots, int m)
         (rc = dalloc(&G->d_sloc, (size_t)G->nmat)) || (rc = dalloc(&G->d_sld, (size_t)G->nmat)) ||
         (rc = dalloc(&G->d_rows, (size_t)G->nmat * gL)))
         return rc;
+    if ((rc = dalloc(&G->d_stage, (size_t)npad * 5))) return rc;
+    HIP_TRY(hipHostMalloc((void **)&G->h_stage, (size_t)npad * 5 * sizeof(int64_t), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&G->h_cmds, (size_t)npad * sizeof(sdca::SlotCmd), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&G->h_res, (size_t)npad * W * sizeof(double), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&G->h_rows, (size_t)G->nmat * gL * sizeof(float), hipHostMallocDefault));
@@ -1738,9 +1742,11 @@ int gram_launch_graph(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
         double *dot = G->dot_part + (size_t)s0 * G->nchd * sdca::ResLayout{m}.ndots(), *res = G->res + (size_t)s0 * W;
         HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         (void)hipMemcpyAsync(dc, hc, n * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st);
-        sdca::gram::step(st, m, dc, n, G->d_galn, G->V, G->vstride, G->nvs, Gp, G->ldG, upd, sm, G->nchunk, G->nchq);
+        sdca::gram::step(st, m, dc, n, G->d_galn, G->V, G->vstride, G->nvs, Gp, G->ldG, upd, sm, G->nchunk, G->nchq,
+                         nullptr, nullptr);
         sdca::gram::gemm(st, A->d_gram, A->Bp, Gp, G->ldG, Zp, G->ldG, A->Bp, n * A->q, A->Bp);
-        sdca::gram::dots(st, m, dc, n, G->d_galn, G->V, G->vstride, G->nvs, Zp, G->ldG, sm, G->nchunk, dot, G->nchd);
+        sdca::gram::dots(st, m, dc, n, G->d_galn, G->V, G->vstride, G->nvs, Zp, G->ldG, sm, G->nchunk, dot, G->nchd,
+                         0);
         sdca::gram::reduce(st, m, n, upd, dot, sm, G->nchunk, G->nchq, G->nchd, res);
         (void)hipMemcpyAsync(G->h_res + (size_t)s0 * W, res, (size_t)n * W * sizeof(double), hipMemcpyDeviceToHost,
                              st);
@@ -1801,7 +1807,8 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
         sdca::sub::warm(st, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, A->d_subQt, A->d_subKQt, A->d_subQ,
                         sdca::kSubRank, G->Wst, G->wstride);
     }
-    sdca::gram::step(st, m, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, Gp, G->ldG, upd, sm, G->nchunk, G->nchq);
+    sdca::gram::step(st, m, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, Gp, G->ldG, upd, sm, G->nchunk, G->nchq,
+                     ev[6], ev[7]);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[1], st));
     for (int a = 0; a < nmem; ++a) {
@@ -1812,7 +1819,8 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[2], st));
-    sdca::gram::dots(st, m, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, Zp, G->ldG, sm, G->nchunk, dot, G->nchd);
+    sdca::gram::dots(st, m, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, Zp, G->ldG, sm, G->nchunk, dot, G->nchd,
+                     0);
     sdca::gram::reduce(st, m, nc, upd, dot, sm, G->nchunk, G->nchq, G->nchd, res);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[3], st));
@@ -1845,6 +1853,7 @@ int gram_finish_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
         t.lbfgs += elapsed(ev[0], ev[3]);  // the whole replayed round
     } else {
         const double upd = elapsed(ev[0], ev[1]), gm = elapsed(ev[1], ev[2]), dts = elapsed(ev[2], ev[3]);
+        t.gstep += elapsed(ev[6], ev[7]);
         t.update += upd;
         t.grad += gm;
         t.dots += dts;
@@ -1852,7 +1861,7 @@ int gram_finish_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
         t.grad_launches += 1;
     }
     t.evals += 1;
-    double flops = 0.0, bu = 0.0, bd = 0.0;
+    double flops = 0.0, bu = 0.0, bd = 0.0, bgs = 0.0;
     for (int i : lst) {
         const sdca::SlotCmd &k = cmds[i];
         if (k.mode == sdca::MODE_IDLE) continue;
@@ -1874,6 +1883,8 @@ int gram_finish_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
         }
         // + the softmax: x_t read, the beliefs and the gradient's coefficients written
         bu += 8.0 * half * (2.0 * acc + 4.0);
+        // k_gstep alone: the accesses of both halves
+        bgs += 8.0 * half * 2.0 * acc;
         // products pass: Z, the trial image and gradient coefficients read, its image written,
         // and the images of g_k, d and the live pairs
         bd += 8.0 * half * (4.0 + (k.mode == sdca::MODE_INIT ? 0.0 : 2.0 + 2.0 * npairs));
@@ -1881,6 +1892,7 @@ int gram_finish_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     t.flops_grad += flops;
     t.bytes_update += bu;
     t.bytes_dots += bd;
+    t.bytes_gstep += bgs;
     t.bytes_lbfgs += bu + bd;
     return SDCA_OK;
 }
@@ -1892,16 +1904,23 @@ int gram_stage(sdca_ctx *c, int g, const Solver &S, const std::vector<int> &fin,
 {
     sdca::GramWs *G = c->gram;
     hipStream_t st = c->gstream[g];
-    for (int i : fin) {
+    const int nf = (int)fin.size();
+    int64_t *meta = G->h_stage + (size_t)g * G->gs * 5;
+    int64_t maxlen = 0;
+    for (int k = 0; k < nf; ++k) {
+        const int i = fin[k];
         const Slot &sl = S.slot(i);
         const sdca_ctx *A = c->members[sl.aln];
         const int64_t qB = (int64_t)A->q * A->Bp;
         const size_t u = G->pend.size();
         if ((int)u >= G->pcap) return fail(SDCA_E_STATE, "pending buffer full");
-        const double *src = G->V + (size_t)i * G->vstride + (sl.from_xt ? sdca::gram::kVecXt : sdca::gram::kVecXc) * G->nvs;
-        double *dst = G->Pend + u * G->nvs;
-        HIP_TRY(hipMemcpyAsync(dst, src, (size_t)qB * sizeof(double), hipMemcpyDeviceToDevice, st));
-        HIP_TRY(hipMemcpyAsync(dst + 2 * qB, src + 2 * qB, 8 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        // coefficients and fields of x_t or x_c into the pending buffer
+        meta[k] = (int64_t)i * G->vstride + (sl.from_xt ? sdca::gram::kVecXt : sdca::gram::kVecXc) * G->nvs;
+        meta[nf + k] = (int64_t)u * G->nvs;
+        meta[2 * nf + k] = qB;
+        meta[3 * nf + k] = 2 * qB;
+        meta[4 * nf + k] = 2 * qB;
+        maxlen = std::max(maxlen, qB);
         G->pend.push_back({sl.item, sl.r, sl.aln});
         if (st_out) {
             sdca_locus_stats &o = st_out[sl.item];
@@ -1915,6 +1934,12 @@ int gram_stage(sdca_ctx *c, int g, const Solver &S, const std::vector<int> &fin,
             o.nfeval_fine = 0;
             o.nfeval_sub = G->sub_nev ? G->sub_nev[sl.item] : 0;
         }
+    }
+    if (nf > 0) {
+        int64_t *dmeta = G->d_stage + (size_t)g * G->gs * 5;
+        HIP_TRY(hipMemcpyAsync(dmeta, meta, (size_t)nf * 5 * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        sdca::gram::gather(st, G->V, G->Pend, dmeta, nf, maxlen, 8);
+        HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipEventRecord(G->stage_ev[g], st));
     return SDCA_OK;
@@ -2273,7 +2298,7 @@ int sub_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds)
     sdca::sub::grad(st, dc, nc, G->d_galn, G->Vr, G->vstride_r, G->nvs_r, kp, G->aux, G->auxstride, sm, G->nchunk, Zr,
                     nz, zs, 2.0 * A->lambda_J);
     sdca::gram::dots(st, m, dc, nc, G->d_galn_r, G->Vr, G->vstride_r, G->nvs_r, Zr, 2 * kp, sm, G->nchunk, dot,
-                     G->nchd_r);
+                     G->nchd_r, 1);
     sdca::gram::reduce(st, m, nc, upd, dot, sm, G->nchunk, G->nchq_r, G->nchd_r, res);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(ev[5], st));
@@ -2362,18 +2387,30 @@ int sub_solve(sdca_ctx *c, const int32_t *loci, int32_t K, std::vector<int> &nev
                 if (done_flag[i]) finished.push_back(i);
             c->timing.host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th).count();
             hipStream_t st = c->gstream[g];
+            // restricted optima (coefficients, then the fields) into the staged starts
+            int64_t *meta = G->h_stage + (size_t)g * G->gs * 5;
+            std::vector<int> keep;
+            for (int i : finished)
+                if (S.slot(i).status != SDCA_ST_NONFINITE) keep.push_back(i);
+            const int nk = (int)keep.size();
+            for (int k = 0; k < nk; ++k) {
+                const Slot &sl = S.slot(keep[k]);
+                meta[k] = (int64_t)keep[k] * G->vstride_r +
+                          (sl.from_xt ? sdca::gram::kVecXt : sdca::gram::kVecXc) * G->nvs_r;
+                meta[nk + k] = (int64_t)sl.item * G->wstride;
+                meta[2 * nk + k] = (int64_t)q * kp;
+                meta[3 * nk + k] = 2 * (int64_t)q * kp;
+                meta[4 * nk + k] = (int64_t)q * kp;
+                ok[sl.item] = 1;
+            }
+            if (nk > 0) {
+                int64_t *dmeta = G->d_stage + (size_t)g * G->gs * 5;
+                HIP_TRY(hipMemcpyAsync(dmeta, meta, (size_t)nk * 5 * sizeof(int64_t), hipMemcpyHostToDevice, st));
+                sdca::gram::gather(st, G->Vr, G->Wst, dmeta, nk, (int64_t)q * kp, 8);
+                HIP_TRY(hipGetLastError());
+            }
             for (int i : finished) {
-                const Slot &sl = S.slot(i);
-                nev[sl.item] = sl.nfeval;
-                if (sl.status != SDCA_ST_NONFINITE) {
-                    const double *src = G->Vr + (size_t)i * G->vstride_r +
-                                        (sl.from_xt ? sdca::gram::kVecXt : sdca::gram::kVecXc) * G->nvs_r;
-                    double *dst = G->Wst + (size_t)sl.item * G->wstride;
-                    HIP_TRY(hipMemcpyAsync(dst, src, (size_t)q * kp * sizeof(double), hipMemcpyDeviceToDevice, st));
-                    HIP_TRY(hipMemcpyAsync(dst + (size_t)q * kp, src + 2 * (size_t)q * kp, 8 * sizeof(double),
-                                           hipMemcpyDeviceToDevice, st));
-                    ok[sl.item] = 1;
-                }
+                nev[S.slot(i).item] = S.slot(i).nfeval;
                 ++done;
                 fill(i);
             }
@@ -3239,6 +3276,8 @@ int sdca_get_timing(sdca_ctx *c, sdca_timing *t, int32_t reset)
     t->n_sub_evals = s.sub_evals;
     t->flops_sub = s.flops_sub;
     t->host_ms = s.host;
+    t->gstep_ms = s.gstep;
+    t->bytes_gstep = s.bytes_gstep;
     if (reset) c->timing = sdca::Timing();
     return SDCA_OK;
 }

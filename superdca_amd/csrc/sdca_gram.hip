@@ -451,8 +451,9 @@ __device__ __forceinline__ void step_elems(const StepArgs &a, const double (&cs)
 
 // The fields of every command's trial point (one wave per command): the field entries
 // step first, so that the softmax of k_gsoftmax reads the final h of x_t.
-// upd_part[command][nchq] = {0, |h_t|^2, d_h . g_k,h}.
-template <int M>
+// upd_part[command][nchq] = {0, |h_t|^2, d_h . g_k,h}.  PART (here and in k_gstep, k_gdots)
+// tells the full solve's launches (0) from the subspace warm start's (1) in a profile.
+template <int M, int PART>
 __global__ __launch_bounds__(64) void k_ghstep(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
                                                double *__restrict__ V, int64_t vstride, int64_t nvs,
                                                double *__restrict__ upd_part, int nchq)
@@ -488,7 +489,7 @@ __global__ __launch_bounds__(64) void k_ghstep(const SlotCmd *__restrict__ cmds,
 // Trial point of every command, one thread per coefficient (s, b) and its image entry
 // (flat over the q Bp coefficients; blocks of 256).  upd_part[command][chunk] =
 // {c_t . K_r c_t, 0, d . g_k} with d . g_k = c_d . (K_r c_gk).
-template <int M>
+template <int M, int PART>
 __global__ __launch_bounds__(256) void k_gstep(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
                                                double *__restrict__ V, int64_t vstride, int64_t nvs,
                                                double *__restrict__ upd_part, int nchq)
@@ -622,7 +623,7 @@ __global__ __launch_bounds__(256) void k_gsoftmax(const SlotCmd *__restrict__ cm
 // part[command][chunk][2M + 5] = {g_t.s_p, g_t.y_p, g_t.g_t, g_t.g_k, g_t.d, 0, |g_t,J|^2}
 // in the primal metric (coefficients of one against the image of the other).
 constexpr int kDotsPerThread = 4;
-template <int M>
+template <int M, int PART>
 __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
                                                double *__restrict__ V, int64_t vstride, int64_t nvs,
                                                const double *__restrict__ Zp, int64_t ldG,
@@ -786,9 +787,29 @@ __global__ __launch_bounds__(256) void k_gsplit(const double *__restrict__ V, co
         hpart[(int64_t)u * 8 + threadIdx.x] = threadIdx.x < q ? src[2 * (int64_t)q * Bp + threadIdx.x] : 0.0;
 }
 
+// Copies of finished loci's vectors (gram::gather): block y is item u.
+__global__ __launch_bounds__(256) void k_gather(const double *__restrict__ src, double *__restrict__ dst,
+                                                const int64_t *__restrict__ meta, int n, int len2)
+{
+    const int u = blockIdx.y;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const double *s = src + meta[u];
+    double *d = dst + meta[n + u];
+    const int64_t len1 = meta[2 * n + u];
+    if (t < len1) d[t] = s[t];
+    else if (t < len1 + len2) d[meta[4 * n + u] + (t - len1)] = s[meta[3 * n + u] + (t - len1)];
+}
+
 }  // namespace
 
 namespace gram {
+
+void gather(hipStream_t st, const double *src, double *dst, const int64_t *meta, int n, int64_t maxlen1, int len2)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_gather, dim3((unsigned)((maxlen1 + len2 + 255) / 256), n), dim3(256), 0, st, src, dst, meta,
+                       n, len2);
+}
 
 void gram_matrix(hipStream_t st, const uint64_t *planes, int B, int q, int W, int j0, int64_t ldk, double *K)
 {
@@ -828,11 +849,15 @@ void gemm_splitk(hipStream_t st, const double *A, int64_t lda, const double *Bm,
     }
 
 void step(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
-          int64_t nvs, double *Gp, int64_t ldG, double *upd_part, double *sm_part, int nchunk, int nchq)
+          int64_t nvs, double *Gp, int64_t ldG, double *upd_part, double *sm_part, int nchunk, int nchq,
+          hipEvent_t ev_gstep0, hipEvent_t ev_gstep1)
 {
 #define SDCA_GSTEP(MM)                                                                                               \
-    hipLaunchKernelGGL((k_ghstep<MM>), dim3(ncmd), dim3(64), 0, st, cmds, aln, V, vstride, nvs, upd_part, nchq);     \
-    hipLaunchKernelGGL((k_gstep<MM>), dim3(nchq, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, upd_part, nchq)
+    hipLaunchKernelGGL((k_ghstep<MM, 0>), dim3(ncmd), dim3(64), 0, st, cmds, aln, V, vstride, nvs, upd_part, nchq);  \
+    if (ev_gstep0) (void)hipEventRecord(ev_gstep0, st);                                                              \
+    hipLaunchKernelGGL((k_gstep<MM, 0>), dim3(nchq, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, upd_part,   \
+                       nchq);                                                                                        \
+    if (ev_gstep1) (void)hipEventRecord(ev_gstep1, st)
     SDCA_GRAM_DISPATCH(m, SDCA_GSTEP)
 #undef SDCA_GSTEP
     hipLaunchKernelGGL(k_gsoftmax, dim3(nchunk, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Gp, ldG, sm_part,
@@ -843,8 +868,9 @@ void step_vectors(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const Gr
                   int64_t vstride, int64_t nvs, double *upd_part, int nchq)
 {
 #define SDCA_GSTEPV(MM)                                                                                              \
-    hipLaunchKernelGGL((k_ghstep<MM>), dim3(ncmd), dim3(64), 0, st, cmds, aln, V, vstride, nvs, upd_part, nchq);     \
-    hipLaunchKernelGGL((k_gstep<MM>), dim3(nchq, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, upd_part, nchq)
+    hipLaunchKernelGGL((k_ghstep<MM, 1>), dim3(ncmd), dim3(64), 0, st, cmds, aln, V, vstride, nvs, upd_part, nchq);  \
+    hipLaunchKernelGGL((k_gstep<MM, 1>), dim3(nchq, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, upd_part,   \
+                       nchq)
     SDCA_GRAM_DISPATCH(m, SDCA_GSTEPV)
 #undef SDCA_GSTEPV
 }
@@ -852,11 +878,16 @@ void step_vectors(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const Gr
 int dots_chunks(int64_t qB) { return (int)((qB + 256 * kDotsPerThread - 1) / (256 * kDotsPerThread)); }
 
 void dots(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
-          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, int nchunk, double *dot_part, int nchd)
+          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, int nchunk, double *dot_part, int nchd,
+          int part)
 {
-#define SDCA_GDOTS(MM)                                                                                            \
-    hipLaunchKernelGGL((k_gdots<MM>), dim3(nchd, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Zp, ldG,   \
-                       sm_part, nchunk, dot_part, nchd)
+#define SDCA_GDOTS(MM)                                                                                               \
+    if (part == 0)                                                                                                   \
+        hipLaunchKernelGGL((k_gdots<MM, 0>), dim3(nchd, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Zp,     \
+                           ldG, sm_part, nchunk, dot_part, nchd);                                                    \
+    else                                                                                                             \
+        hipLaunchKernelGGL((k_gdots<MM, 1>), dim3(nchd, ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Zp,     \
+                           ldG, sm_part, nchunk, dot_part, nchd)
     SDCA_GRAM_DISPATCH(m, SDCA_GDOTS)
 #undef SDCA_GDOTS
 }

@@ -108,6 +108,7 @@ struct Timing {
     int64_t evals = 0, energy_launches = 0, grad_launches = 0;
     double flops_energy = 0, flops_grad = 0, bytes_lbfgs = 0;
     double update = 0, dots = 0, bytes_update = 0, bytes_dots = 0;
+    double gstep = 0, bytes_gstep = 0;  // k_gstep alone (Gram full solve)
     double sub = 0, flops_sub = 0;
     int64_t sub_evals = 0;
     double host = 0;  // host time stepping the slots' L-BFGS state between rounds
@@ -327,11 +328,18 @@ constexpr int kVecXc = 0, kVecXt = 1, kVecG = 2, kVecD = 4, kVecPairs = 5;
 inline int64_t vec_len(int q, int Bp) { return 2 * (int64_t)q * Bp + 8; }
 // Identical-locus counts over loci [j0, L) from bit-planes (k_bitplanes layout).
 void gram_matrix(hipStream_t st, const uint64_t *planes, int B, int q, int W, int j0, int64_t ldk, double *K);
+// For u < n, with meta = [src offsets | dst offsets | len1 | src2 | dst2] (n entries each):
+// meta[2n + u] doubles from src + meta[u] to dst + meta[n + u], then len2 doubles from
+// src + meta[u] + meta[3n + u] to dst + meta[n + u] + meta[4n + u] (finished loci into the
+// pending or staged-start buffers; maxlen1 bounds the first lengths).
+void gather(hipStream_t st, const double *src, double *dst, const int64_t *meta, int n, int64_t maxlen1, int len2);
 // Trial points and softmax of a round's commands: G into Gp (column row + s of ldG),
 // partials upd_part [cmd][nchq + 1][3] (blocks of 256 coefficients) and sm_part
 // [cmd][nchunk][65] (blocks of 256 sequences).
+// Events ev_gstep0/1 (or null) are recorded around the k_gstep launch.
 void step(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
-          int64_t nvs, double *Gp, int64_t ldG, double *upd_part, double *sm_part, int nchunk, int nchq);
+          int64_t nvs, double *Gp, int64_t ldG, double *upd_part, double *sm_part, int nchunk, int nchq,
+          hipEvent_t ev_gstep0, hipEvent_t ev_gstep1);
 // The trial-point part of step() alone (the subspace warm start's rounds: nchq blocks of 256
 // over the q x k coefficients).
 void step_vectors(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V,
@@ -349,8 +357,10 @@ void gemm_splitk(hipStream_t st, const double *A, int64_t lda, const double *Bm,
 // Gradient images from Z = K G and the products: dot_part [cmd][nchd][2m + 5], nchd =
 // dots_chunks(q Bp) blocks per command.
 int dots_chunks(int64_t qB);
+// part: 0 the full solve, 1 the subspace warm start (kernel instances of their own).
 void dots(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride,
-          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, int nchunk, double *dot_part, int nchd);
+          int64_t nvs, const double *Zp, int64_t ldG, const double *sm_part, int nchunk, double *dot_part, int nchd,
+          int part);
 void reduce(hipStream_t st, int m, int ncmd, const double *upd_part, const double *dot_part, const double *sm_part,
             int nchunk, int nchq, int nchd, double *res);
 // Coefficients of n finished loci (V + src_off[u]) as contraction operands (rows[u]).
@@ -375,6 +385,9 @@ struct GramWs {
     double *upd_part = nullptr, *sm_part = nullptr, *dot_part = nullptr, *res = nullptr;
     SlotCmd *d_cmds = nullptr, *h_cmds = nullptr;  // h_cmds, h_res: pinned
     double *h_res = nullptr;
+    // staging of finished loci: per group, 5 gs entries of gram::gather metadata (pinned and
+    // device), one gather kernel per round instead of two copies per locus
+    int64_t *h_stage = nullptr, *d_stage = nullptr;
     GramAln *d_galn = nullptr;
     AlnDev *d_aln = nullptr;
     std::vector<int> cmd_slot[kMaxGroups];  // per group: slot of each listed command of its round in flight

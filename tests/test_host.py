@@ -441,21 +441,25 @@ def test_bench_roofline_fields_from_timings():
     assert weighted == pytest.approx(sum(f * m for f, m in zip(fr, ms)) / sum(ms), abs=1e-3)
     # the Gram formulation: the FP64 GEMM against the FP64 MFMA peak, traffic per flop
     tg = dict(tm, gemm_energy_ms=0.0, flops_energy=0.0, n_energy_launches=0, gemm_grad_ms=4.0 * r,
-              flops_grad=2.0e11 * r)
+              flops_grad=2.0e11 * r, gstep_ms=1.0 * r, bytes_gstep=0.5 * tm["bytes_update"])
     rl = bench.roofline(tg, True, True)
     assert rl["kernel"] == "k_dgemm" and rl["bound"] == "mfma" and rl["peak"] == 78.6
     assert rl["traffic"] == pytest.approx(bench.HBM_BYTES_PER_FLOP["k_dgemm"] * 2.0e11, rel=1e-6)
     assert rl["achieved"] == pytest.approx(2.0e11 / 4.0e-3 / 1e12, rel=1e-4)
     assert rl["frac"] == pytest.approx(50.0 / 78.6, rel=1e-4)
     phases, _ = bench.phase_fractions(tg, True, True)
-    assert set(phases) == {"k_dgemm", "k_gstep", "k_gdots"}
-    # with the subspace warm start: its rounds as a phase of their own; the roofline stays on
-    # k_dgemm even when the step pass (a group of kernels) takes more device time
+    assert set(phases) == {"k_dgemm", "k_gstep", "k_gsoftmax", "k_gdots"}
+    # with the subspace warm start: its rounds as a phase of their own; the roofline names
+    # the single kernel (k_dgemm or k_gstep) with the larger device time, never a group
     ts = dict(tg, subspace_ms=1.0 * r, flops_sub=1.0e10 * r, n_sub_evals=r, update_ms=9.0 * r)
     phases, _ = bench.phase_fractions(ts, True, True)
-    assert set(phases) == {"k_dgemm", "k_gstep", "k_gdots", "subspace_rounds"}
+    assert set(phases) == {"k_dgemm", "k_gstep", "k_gsoftmax", "k_gdots", "subspace_rounds"}
     assert phases["subspace_rounds"]["rate"] == pytest.approx(1.0e10 / 1.0e-3 / 1e12, rel=1e-3)
     assert bench.roofline(ts, True, True)["kernel"] == "k_dgemm"
+    tk = dict(ts, gstep_ms=6.0 * r)
+    rl = bench.roofline(tk, True, True)
+    assert rl["kernel"] == "k_gstep" and rl["bound"] == "hbm" and rl["peak"] == 8000.0
+    assert rl["achieved"] == pytest.approx(0.5 * tm["bytes_update"] / (6.0 * r * 1e-3) / 1e9, rel=1e-4)
 
 
 def test_every_hip_compile_disables_packed_fp32_and_targets_gfx950():
