@@ -225,7 +225,7 @@ def main() -> int:
     ap.add_argument("--loci-per-step", type=int, default=0, help="default: 16384 at C4, 1024 otherwise")
     ap.add_argument("--slots", type=int, default=0, help="headline loci in flight (default 4096 at C4, 512 otherwise)")
     ap.add_argument("--history", type=int, default=0,
-                    help="L-BFGS history (0: the formulation's default, 24 Gram and 10 primal)")
+                    help="L-BFGS history (0: the formulation's default, 32 Gram and 10 primal)")
     ap.add_argument("--groups", type=int, default=0,
                     help="headline slot groups: 1, or 2 pipelined on two streams (0: the library's default, 2)")
     ap.add_argument("--j-threshold", type=float, default=3e-5,
@@ -494,7 +494,7 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"{wl}: B={B} sequences x N={L} loci, q_env={q}, {start_desc}, Gram formulation "
-                               f"(FP64), {per_step} loci per step ({slots} in flight), L-BFGS m={args.history or 24}, {conv}",
+                               f"(FP64), {per_step} loci per step ({slots} in flight), L-BFGS m={args.history or 32}, {conv}",
                    "B": B, "N": L, "q_env": q, "loci_per_step": per_step, "slots": slots, "formulation": "gram",
                    "precision": "fp64", "start": start_desc, "convergence": conv,
                    "parallelism": "loci batched on 1 GPU"},

@@ -51,7 +51,7 @@ typedef struct sdca_opts {
     int32_t batch_loci;      /* loci solved concurrently on the GPU (default 128; 0: 4096 for the
                                 Gram formulation, 128 for the primal one) */
     int32_t history;         /* L-BFGS history m: 1, 3, 5, 8, 10 or 16; the Gram formulation also 24
-                                and 32.  0 (default): the formulation's own, 24 for Gram, 10 primal */
+                                and 32.  0 (default): the formulation's own, 32 for Gram, 10 primal */
     int32_t max_iter;        /* iteration cap per locus (default 2000, plmDCA.hpp:219) */
     int32_t max_ls;          /* function evaluations per line search (default 20) */
     double grad_threshold;   /* stop when ||g||_inf < this (default 1e-3, plmDCA_options.cpp:61) */

@@ -112,9 +112,9 @@ bool valid_history(int m)
 }
 
 // The L-BFGS history of a solve: opts.history, or with 0 the formulation's default (Gram
-// 24: its vector passes are cheap against the evaluations a longer history saves; primal
+// 32: its vector passes are cheap against the evaluations a longer history saves; primal
 // 10).
-int history_for(const sdca_opts &o, bool gram) { return o.history > 0 ? o.history : (gram ? 24 : 10); }
+int history_for(const sdca_opts &o, bool gram) { return o.history > 0 ? o.history : (gram ? 32 : 10); }
 
 // Loci in flight: opts.batch_loci, or with 0 the formulation's default (Gram slots are
 // small -- 6.1 MB at B = 3000, q = 5 -- so 4096 of them; primal slots hold q x K_x vectors).

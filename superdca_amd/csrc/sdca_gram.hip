@@ -61,6 +61,26 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NV], double *out)
     for (int k = threadIdx.x; k < NV; k += blockDim.x) out[k] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
 }
 
+// Sums 2 H values over a block of 256 threads: v[0, H) into outA[0, H), v[H, 2H) into
+// outB[0, H).  The caller synchronises the block before the next call.
+template <int H>
+__device__ __forceinline__ void block_sum_store_split(double (&v)[2 * H], double *outA, double *outB)
+{
+    __shared__ double red[4][2 * H];
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 2 * H; ++k) {
+        const double x = wave_sum(v[k]);
+        if ((threadIdx.x & 63) == 0) red[wave][k] = x;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 2 * H; k += blockDim.x) {
+        const double t = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+        if (k < H) outA[k] = t;
+        else outB[k - H] = t;
+    }
+}
+
 // ------------------------------------------------------------------ Gram matrix
 
 // K[i][j] = sum_v popcount(P_i^v & P_j^v) over the loci [j0, L) (bits of loci below j0
@@ -288,12 +308,13 @@ __device__ __forceinline__ void load_step(const SlotCmd &c, StepArgs &a, double 
 }
 
 // NEWDIR of step_elems for histories above 16: the same arithmetic in the same order, with
-// the pairs loaded eight at a time (all 2m loads in flight at once would not fit the
-// registers).
+// the pairs streamed eight at a time through a loop that is not unrolled (their
+// coefficients read from the command), so that a thread holds one chunk of pairs instead
+// of 2m values and more waves fit a SIMD.
 template <int M, int NE>
-__device__ __forceinline__ void step_newdir_chunked(const StepArgs &a, const double (&cs)[M], const double (&cy)[M],
-                                                    const SlotPtrs &P, int64_t nvs, const int64_t (&e)[NE],
-                                                    double (&xn)[NE], double (&dn)[NE], double (&gv)[NE])
+__device__ __forceinline__ void step_newdir_chunked(const StepArgs &a, const SlotCmd &c, const SlotPtrs &P,
+                                                    int64_t nvs, const int64_t (&e)[NE], double (&xn)[NE],
+                                                    double (&dn)[NE], double (&gv)[NE])
 {
     constexpr int CH = 8;
     static_assert(M % CH == 0, "history a multiple of the chunk");
@@ -320,24 +341,29 @@ __device__ __forceinline__ void step_newdir_chunked(const StepArgs &a, const dou
         }
         d[k] = a.cd * dold[k] + a.cg * g[k];
     }
+#pragma unroll 1
     for (int p0 = 0; p0 < M; p0 += CH) {
-        double sv[CH][NE], yv[CH][NE];
+        double sv[CH][NE], yv[CH][NE], csv[CH], cyv[CH];
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            csv[u] = c.cs[p0 + u];
+            cyv[u] = c.cy[p0 + u];
+        }
 #pragma unroll
         for (int u = 0; u < CH; ++u)
 #pragma unroll
             for (int k = 0; k < NE; ++k) {
                 const int p = p0 + u;
-                sv[u][k] = (cs[p] != 0.0 && p != a.pnew) ? P.Sb[(int64_t)p * nvs + e[k]] : 0.0;
-                yv[u][k] = (cy[p] != 0.0 && p != a.pnew) ? P.Yb[(int64_t)p * nvs + e[k]] : 0.0;
+                sv[u][k] = (csv[u] != 0.0 && p != a.pnew) ? P.Sb[(int64_t)p * nvs + e[k]] : 0.0;
+                yv[u][k] = (cyv[u] != 0.0 && p != a.pnew) ? P.Yb[(int64_t)p * nvs + e[k]] : 0.0;
             }
 #pragma unroll
         for (int k = 0; k < NE; ++k)
 #pragma unroll
             for (int u = 0; u < CH; ++u) {
-                const int p = p0 + u;
-                const bool mine = p == a.pnew;
-                d[k] += cs[p] * (mine ? sn[k] : sv[u][k]);
-                d[k] += cy[p] * (mine ? yn[k] : yv[u][k]);
+                const bool mine = p0 + u == a.pnew;
+                d[k] += csv[u] * (mine ? sn[k] : sv[u][k]);
+                d[k] += cyv[u] * (mine ? yn[k] : yv[u][k]);
             }
     }
 #pragma unroll
@@ -361,7 +387,7 @@ __device__ __forceinline__ void step_newdir_chunked(const StepArgs &a, const dou
 // the arithmetic, so that a thread has all of its element's vectors in flight at once.
 // The operations are linear, so an image entry follows its coefficients exactly.
 template <int M, int NE>
-__device__ __forceinline__ void step_elems(const StepArgs &a, const double (&cs)[M], const double (&cy)[M],
+__device__ __forceinline__ void step_elems(const StepArgs &a, const SlotCmd &c, const double (&cs)[M], const double (&cy)[M],
                                            const SlotPtrs &P, int64_t nvs, const int64_t (&e)[NE],
                                            double (&xn)[NE], double (&dn)[NE], double (&gv)[NE])
 {
@@ -402,7 +428,7 @@ __device__ __forceinline__ void step_elems(const StepArgs &a, const double (&cs)
         return;
     }
     if constexpr (M > 16) {
-        step_newdir_chunked<M, NE>(a, cs, cy, P, nvs, e, xn, dn, gv);
+        step_newdir_chunked<M, NE>(a, c, P, nvs, e, xn, dn, gv);
         return;
     }
     constexpr int MS = M > 16 ? 1 : M;  // (the branch above serves longer histories)
@@ -471,7 +497,7 @@ __global__ __launch_bounds__(64) void k_ghstep(const SlotCmd *__restrict__ cmds,
             const SlotPtrs P = slot_ptrs(c, V, vstride, nvs, M);
             const int64_t e[1] = {2 * (int64_t)A.q * A.Bp + s};
             double xn[1], dn[1], g[1];
-            step_elems<M, 1>(a, cs, cy, P, nvs, e, xn, dn, g);
+            step_elems<M, 1>(a, c, cs, cy, P, nvs, e, xn, dn, g);
             xsqh = xn[0] * xn[0];
             dgh = dn[0] * g[0];
         }
@@ -515,8 +541,8 @@ __global__ __launch_bounds__(256) void k_gstep(const SlotCmd *__restrict__ cmds,
         // that the pass can share a CU with the other slot group's GEMM)
         const int64_t ec[1] = {idx}, ei[1] = {idx + qB};
         double xc[1], dc[1], gc[1], xi[1], di[1], gi[1];
-        step_elems<M, 1>(a, cs, cy, P, nvs, ec, xc, dc, gc);
-        step_elems<M, 1>(a, cs, cy, P, nvs, ei, xi, di, gi);
+        step_elems<M, 1>(a, c, cs, cy, P, nvs, ec, xc, dc, gc);
+        step_elems<M, 1>(a, c, cs, cy, P, nvs, ei, xi, di, gi);
         red[0] = xc[0] * xi[0];
         red[2] = dc[0] * gi[0];
     }
@@ -623,6 +649,97 @@ __global__ __launch_bounds__(256) void k_gsoftmax(const SlotCmd *__restrict__ cm
 // part[command][chunk][2M + 5] = {g_t.s_p, g_t.y_p, g_t.g_t, g_t.g_k, g_t.d, 0, |g_t,J|^2}
 // in the primal metric (coefficients of one against the image of the other).
 constexpr int kDotsPerThread = 4;
+
+// k_gdots for histories above 16: the same sums in the same order per thread, with the
+// pairs' products reduced eight pairs at a time through a loop that is not unrolled, so
+// that a thread holds 16 partial sums instead of 2m.
+template <int M>
+__device__ __forceinline__ void gdots_chunked(const SlotCmd &c, const GramAln &A, const double *xt, double *gt,
+                                              const double *gk, const double *dv, const double *Sb,
+                                              const double *Yb, int64_t nvs, const double *Zp, int64_t ldG,
+                                              const double *sT, bool cur, uint32_t live, double *out)
+{
+    constexpr int CH = 8;
+    static_assert(M % CH == 0, "history a multiple of the chunk");
+    const int q = A.q, Bp = A.Bp, B = A.B;
+    const int64_t qB = (int64_t)q * Bp;
+    const int tid = threadIdx.x;
+    double tail[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // g.g, g.g_k, g.d, 0, |g_J|^2
+    double gcv[kDotsPerThread];
+    int64_t e1v[kDotsPerThread];
+#pragma unroll
+    for (int k = 0; k < kDotsPerThread; ++k) {
+        gcv[k] = 0.0;
+        e1v[k] = -1;
+        const int64_t idx = (int64_t)blockIdx.x * (256 * kDotsPerThread) + k * 256 + tid;
+        if (idx >= qB) continue;
+        const int s = (int)(idx / Bp), b = (int)(idx - (int64_t)s * Bp);
+        const int64_t e0 = idx, e1 = idx + qB;
+        const bool seq = b < B;
+        const double z = seq ? Zp[(int64_t)(c.row + s) * ldG + b] : 0.0;
+        const int y = seq ? A.statesT[(int64_t)c.locus * B + b] : 0;
+        const double xi = xt[e1];
+        const double gc = gt[e0];
+        const double gki = cur ? gk[e1] : 0.0, di = cur ? dv[e1] : 0.0;
+        const double kg = seq ? z - (A.own ? sT[s * 8 + y] : 0.0) + A.lamJ2 * xi : 0.0;
+        gt[e1] = kg;
+        tail[0] += gc * kg;
+        tail[4] += gc * kg;
+        tail[1] += gc * gki;
+        tail[2] += gc * di;
+        gcv[k] = gc;
+        e1v[k] = e1;
+    }
+    const bool fields = blockIdx.x == 0 && tid < q;
+    double gh = 0.0;
+    int64_t eh = 0;
+    if (fields) {
+        // the fields: g_h(s) = sum_a T(s, a) + 2 lambda_h h_t(s)
+        const int s = tid;
+        for (int u = 0; u < q; ++u) gh += sT[s * 8 + u];
+        eh = 2 * qB + s;
+        gh += A.lamh2 * xt[eh];
+        gt[eh] = gh;
+        tail[0] += gh * gh;
+        if (cur) {
+            tail[1] += gh * gk[eh];
+            tail[2] += gh * dv[eh];
+        }
+    }
+#pragma unroll 1
+    for (int p0 = 0; p0 < M; p0 += CH) {
+        double a[2 * CH];
+#pragma unroll
+        for (int u = 0; u < 2 * CH; ++u) a[u] = 0.0;
+#pragma unroll
+        for (int k = 0; k < kDotsPerThread; ++k) {
+            if (e1v[k] < 0) continue;
+            double sp[CH], yp[CH];
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const bool on = (live >> (p0 + u)) & 1u;
+                sp[u] = on ? Sb[(int64_t)(p0 + u) * nvs + e1v[k]] : 0.0;
+                yp[u] = on ? Yb[(int64_t)(p0 + u) * nvs + e1v[k]] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                a[u] += gcv[k] * sp[u];
+                a[CH + u] += gcv[k] * yp[u];
+            }
+        }
+        if (fields && cur)
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                if (!((live >> (p0 + u)) & 1u)) continue;
+                a[u] += gh * Sb[(int64_t)(p0 + u) * nvs + eh];
+                a[CH + u] += gh * Yb[(int64_t)(p0 + u) * nvs + eh];
+            }
+        block_sum_store_split<CH>(a, out + p0, out + M + p0);
+        __syncthreads();
+    }
+    block_sum_store<5>(tail, out + 2 * M);
+}
+
 template <int M, int PART>
 __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
                                                double *__restrict__ V, int64_t vstride, int64_t nvs,
@@ -663,6 +780,10 @@ __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds,
     __syncthreads();
     const bool cur = c.mode != MODE_INIT;  // a current gradient and a direction exist
     const uint32_t live = cur ? c.live : 0u;
+    if constexpr (M > 16) {
+        gdots_chunked<M>(c, A, xt, gt, gk, dv, Sb, Yb, nvs, Zp, ldG, sT, cur, live, out);
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < kDotsPerThread; ++k) {
         const int64_t idx = (int64_t)blockIdx.x * (256 * kDotsPerThread) + k * 256 + tid;

@@ -32,7 +32,7 @@ def test_default_opts_match_reference_defaults():
     o = sd.default_opts()
     assert o.grad_threshold == 1e-3  # plmDCA_options.cpp:61
     assert o.max_iter == 2000        # plmDCA.hpp:219
-    assert o.history == 0  # the formulation's default: 24 Gram, 10 primal
+    assert o.history == 0  # the formulation's default: 32 Gram, 10 primal
     assert o.precision == sd.PREC_FP32
     assert o.coarse_factor == 4  # split precision: BF16 high parts alone while ||g||_inf >= 4 * threshold
     assert o.j_threshold == 0.0  # SuperDCA's stopping rule alone unless a coupling threshold is asked for
