@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -1004,6 +1005,7 @@ sdca::SlotCmd idle_cmd()
     k.locus = -1;
     k.level = sdca::LV_SPLIT;
     k.item = -1;
+    k.wrow = -1;
     return k;
 }
 
@@ -1774,7 +1776,7 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     lst.clear();
     sdca::SlotCmd *hc = G->h_cmds + s0;
     std::vector<int> col0(nmem, 0), ncol(nmem, 0);
-    int col = 0;
+    int col = 0, nwcols = 0;
     for (int a = 0; a < nmem; ++a) {
         const sdca_ctx *A = c->members[a];
         col0[a] = col;
@@ -1787,6 +1789,11 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
             k.q = A->q;
             k.Kx = (int)A->Kx;
             k.Kxp = (int)A->Kxp;
+            k.wrow = -1;
+            if (G->warm_rounds && k.mode == sdca::MODE_INIT && k.init_src != 0.f && k.item >= 0) {
+                k.wrow = nwcols;
+                nwcols += A->q;
+            }
             lst.push_back(i);
             col += A->q;
         }
@@ -1802,10 +1809,11 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     double *dot = G->dot_part + (size_t)s0 * G->nchd * sdca::ResLayout{m}.ndots(), *res = G->res + (size_t)s0 * W;
     HIP_TRY(hipMemcpyAsync(dc, hc, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(ev[0], st));
-    if (G->warm_rounds) {  // staged starts of the round's new loci into their x_c
+    if (G->warm_rounds && nwcols > 0) {  // staged starts of the round's new loci into their x_c
         const sdca_ctx *A = c->members[0];
-        sdca::sub::warm(st, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, A->d_subQt, A->d_subKQt, A->d_subQ,
-                        sdca::kSubRank, G->Wst, G->wstride);
+        sdca::sub::warm(st, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, A->d_subQ, A->d_subKQ, sdca::kSubRank,
+                        A->Bp, G->Wst, G->wstride, G->Ap + (size_t)g * G->colcap * sdca::kSubRank, nwcols, Gp, Zp,
+                        G->ldG);
     }
     sdca::gram::step(st, m, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, Gp, G->ldG, upd, sm, G->nchunk, G->nchq,
                      ev[6], ev[7]);
@@ -2454,13 +2462,22 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
     std::vector<double> res((size_t)nslots * R.width(), 0.0);
     std::vector<char> done_flag(nslots, 0);
     SlotPool pool(SlotPool::default_threads());
+    // Order of the full solves.  After the restricted solves, loci start in decreasing order
+    // of their restricted evaluations, which predict the full ones (C4: correlation 0.8):
+    // the longest solves start first, so the last rounds are not left to a few slow loci
+    // (longest-processing-time-first scheduling).  Every locus' result is the same in any
+    // order: its rounds depend on its own slot only.
+    std::vector<int> order(K);
+    std::iota(order.begin(), order.end(), 0);
+    if (G->warm_rounds)
+        std::stable_sort(order.begin(), order.end(), [&](int u, int v) { return sub_nev[u] > sub_nev[v]; });
     int next = 0, done = 0, noconv = 0, nonfinite = 0;
     auto fill = [&](int i) {
         if (next >= K) {
             S.set_idle(i);
             return;
         }
-        const int item = next++;
+        const int item = order[next++];
         S.assign(i, item, alns ? alns[item] : 0, loci[item], G->warm_rounds && sub_ok[item]);
     };
     auto group_end = [&](int g) { return std::min(nslots, (g + 1) * gs); };

@@ -167,7 +167,7 @@ __device__ __forceinline__ void ld8d(const double *p, double (&r)[8])
 }
 
 // PART tells the launches apart in a profile: 0 the full solve's Z = K G, 1 the subspace
-// warm start's products (split-K) and its basis.
+// warm start's products (split-K) and its basis, 2 its start points (Q a*, K Q a*).
 template <int PART>
 __global__ __launch_bounds__(256) void k_dgemm(const double *__restrict__ A, int64_t lda, const double *__restrict__ Bm,
                                                int64_t ldb, double *__restrict__ C, int64_t ldc, int N, int Kd,
@@ -944,6 +944,14 @@ void gemm(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_
 {
     if (N <= 0) return;
     hipLaunchKernelGGL(k_dgemm<0>, dim3((N + DT - 1) / DT, M / DT, 1), dim3(256), 0, st, A, lda, Bm, ldb, C, ldc, N,
+                       Kd, Kd, (int64_t)0);
+}
+
+void gemm_warm(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_t ldb, double *C, int64_t ldc,
+               int M, int N, int Kd)
+{
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_dgemm<2>, dim3((N + DT - 1) / DT, M / DT, 1), dim3(256), 0, st, A, lda, Bm, ldb, C, ldc, N,
                        Kd, Kd, (int64_t)0);
 }
 

@@ -77,6 +77,7 @@ struct SlotCmd {
     int32_t pnew;     // NEWDIR: pair slot receiving s = pstep d_old, y = g[gk] - g[1-gk]; -1: none
     uint32_t live;    // bit p: pair slot p holds a stored pair
     int32_t item;     // index of the locus in the solve's list (staged Gram warm starts), -1 if none
+    int32_t wrow;     // MODE_INIT from a staged start: first of its q columns in the round's start product, else -1
     float alpha;      // step of the trial point x_t = x_c + alpha d
     float pstep;      // NEWDIR with accept: step of the accepted trial
     float init_src;   // MODE_INIT: 1 = copy the staged start point, 0 = zero start
@@ -317,8 +318,12 @@ void softmax(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, 
 void grad(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride, int64_t nvs,
           int kp, const double *aux, int64_t auxstride, const double *sm_part, int nchunk, double *Z, int nz,
           int64_t zstride, double lamJ2);
+// Staged starts of the round's new loci (commands with wrow >= 0, nwcols = their columns):
+// their coefficients a* into Ap (rows wrow + s), c = Q a* into Gp and K Q a* into Zp (columns
+// wrow + s of ldG, FP64 MFMA over Bp rows), then x_c = (c, K Q a* - U (U^T c), h*).
 void warm(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride, int64_t nvs,
-          const double *Qt, const double *KQt, const double *Q, int kp, const double *Wst, int64_t wstride);
+          const double *Q, const double *KQ, int kp, int Bp, const double *Wst, int64_t wstride, double *Ap,
+          int nwcols, double *Gp, double *Zp, int64_t ldG);
 }  // namespace sub
 
 // The Gram formulation (sdca_gram.hip): slot vectors x_c, x_t, g[0], g[1], d, then the m
@@ -349,6 +354,9 @@ void step_vectors(hipStream_t st, int m, const SlotCmd *cmds, int ncmd, const Gr
 constexpr int kGemmCols = 128;
 void gemm(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_t ldb, double *C, int64_t ldc, int M,
           int N, int Kd);
+// The same as a launch of its own kernel instance (the subspace warm start's start points).
+void gemm_warm(hipStream_t st, const double *A, int64_t lda, const double *Bm, int64_t ldb, double *C, int64_t ldc,
+               int M, int N, int Kd);
 // The same in ceil(Kd / kc) split-K slices (kc a multiple of 16): slice z sums k in
 // [z kc, (z + 1) kc) into C + z zstride.  A kernel instance of its own (k_dgemm<1>): the
 // subspace warm start's products.

@@ -43,40 +43,71 @@ __device__ __forceinline__ double wsum(double x)
 //   V_r[a][i] = sum_{b: x_br = a} Q[b][i]          (aux + 0,            q x kp)
 //   R_r[i][a] = sum_j Minv[i][j] V_r[a][j]         (aux + 8 kp,         kp x 8)
 //   Omega_r   = (I - V_r R_r)^{-1}                  (aux + 16 kp,        8 x 8)
-// Q is Bp x kp (row-major), Minv kp x kp (symmetric).  Thread t covers column i = t % kp
-// over half of the sequences (t / kp); kp <= 128.
+// Q is Bp x kp (row-major), Minv kp x kp (symmetric); kp = kSubRank.
+// V_r: wave w sums the sequences [w n4, (w + 1) n4) (n4 = ceil(B / 4)); lane l holds
+// columns 2l and 2l + 1 of every state.  A sequence's state is the same for the whole wave,
+// so its row goes to the registers of that state, picked by a wave-uniform index; eight
+// rows are loaded ahead.  The four waves' sums are added in a fixed order.
+// Omega_r: Gauss-Jordan with partial pivoting on [X | I] in LDS, one thread per entry.
 __global__ __launch_bounds__(256) void k_sinit(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
-                                               const double *__restrict__ Q, const double *__restrict__ Minv, int kp,
+                                               const double *__restrict__ Q, const double *__restrict__ Minv,
                                                double *__restrict__ aux, int64_t auxstride)
 {
+    constexpr int kp = kSubRank;
+    static_assert(kp == 128, "V_r's lane layout covers 128 columns");
     const SlotCmd &c = cmds[blockIdx.x];
     if (c.mode != MODE_INIT) return;
     const GramAln &A = aln[c.aln];
     const int q = A.q, B = A.B;
-    __shared__ double sV[2][8][128];
+    __shared__ double sP[4][8][kp];
     __shared__ double sX[8][8];
-    const int tid = threadIdx.x, i = tid % kp, half = tid / kp;
-    const int nhalf = 256 / kp;  // 2 when kp = 128
-    double acc[8];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) acc[a] = 0.0;
+    __shared__ double sG[8][16];
+    __shared__ int sPiv;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint8_t *y = A.statesT + (int64_t)c.locus * B;
-    if (half < nhalf)
-        for (int b = half; b < B; b += nhalf) {
-            const int ya = y[b];
-            const double v = Q[(int64_t)b * kp + i];
+    {
+        const int n4 = (B + 3) >> 2;
+        const int b0 = __builtin_amdgcn_readfirstlane(min(B, wave * n4));
+        const int b1 = __builtin_amdgcn_readfirstlane(min(B, (wave + 1) * n4));
+        double ax[8], ay[8];  // columns 2l and 2l + 1 (two arrays: each stays in registers)
 #pragma unroll
-            for (int a = 0; a < 8; ++a) acc[a] += (a == ya) ? v : 0.0;
+        for (int a = 0; a < 8; ++a) ax[a] = ay[a] = 0.0;
+        const double *qc = Q + 2 * lane;
+        int b = b0;
+        for (; b + 8 <= b1; b += 8) {
+            double2 v[8];
+            int ys[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                v[u] = *reinterpret_cast<const double2 *>(qc + (int64_t)(b + u) * kp);
+                ys[u] = y[b + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int a = __builtin_amdgcn_readfirstlane(ys[u]) & 7;
+                ax[a] += v[u].x;
+                ay[a] += v[u].y;
+            }
         }
-    if (half < 2)
+        for (; b < b1; ++b) {
+            const double2 v = *reinterpret_cast<const double2 *>(qc + (int64_t)b * kp);
+            const int a = __builtin_amdgcn_readfirstlane((int)y[b]) & 7;
+            ax[a] += v.x;
+            ay[a] += v.y;
+        }
 #pragma unroll
-        for (int a = 0; a < 8; ++a) sV[half][a][i] = acc[a];
+        for (int a = 0; a < 8; ++a) {
+            sP[wave][a][2 * lane] = ax[a];
+            sP[wave][a][2 * lane + 1] = ay[a];
+        }
+    }
     __syncthreads();
     double *out = aux + (int64_t)c.slot * auxstride;
     if (tid < kp) {
         for (int a = 0; a < 8; ++a) {
-            const double v = a < q ? (nhalf > 1 ? sV[0][a][tid] + sV[1][a][tid] : sV[0][a][tid]) : 0.0;
-            sV[0][a][tid] = v;
+            const double v =
+                a < q ? (sP[0][a][tid] + sP[1][a][tid]) + (sP[2][a][tid] + sP[3][a][tid]) : 0.0;
+            sP[0][a][tid] = v;
             out[(int64_t)a * kp + tid] = v;
         }
     }
@@ -89,61 +120,55 @@ __global__ __launch_bounds__(256) void k_sinit(const SlotCmd *__restrict__ cmds,
         for (int j = 0; j < kp; ++j) {
             const double mij = Minv[(int64_t)j * kp + tid];
 #pragma unroll
-            for (int a = 0; a < 8; ++a) r[a] += mij * sV[0][a][j];
+            for (int a = 0; a < 8; ++a) r[a] += mij * sP[0][a][j];
         }
 #pragma unroll
         for (int a = 0; a < 8; ++a) out[8 * (int64_t)kp + (int64_t)tid * 8 + a] = r[a];
         // X[a][a'] = V_r[a] . R_r[., a'] : partial products of column tid
 #pragma unroll
-        for (int a = 0; a < 8; ++a) sV[1][a][tid] = r[a];
+        for (int a = 0; a < 8; ++a) sP[1][a][tid] = r[a];
     }
     __syncthreads();
     if (tid < 64) {
         const int a = tid >> 3, a2 = tid & 7;
         double x = 0.0;
         if (a < q && a2 < q)
-            for (int j = 0; j < kp; ++j) x += sV[0][a][j] * sV[1][a2][j];
+            for (int j = 0; j < kp; ++j) x += sP[0][a][j] * sP[1][a2][j];
         sX[a][a2] = (a == a2 ? 1.0 : 0.0) - x;
     }
     __syncthreads();
-    if (tid == 0) {
-        // Gauss-Jordan with partial pivoting on the q x q block (rows of unobserved states are
-        // those of the identity)
-        double Mx[8][8], Iv[8][8];
-        for (int a = 0; a < q; ++a)
-            for (int u = 0; u < q; ++u) {
-                Mx[a][u] = sX[a][u];
-                Iv[a][u] = a == u ? 1.0 : 0.0;
-            }
-        for (int col = 0; col < q; ++col) {
+    // Gauss-Jordan on the q x q block (rows of unobserved states are those of the identity)
+    const int rr = (tid >> 4) & 7, u = tid & 15;
+    if (tid < 128) sG[rr][u] = u < 8 ? sX[rr][u] : (u - 8 == rr ? 1.0 : 0.0);
+    __syncthreads();
+    for (int col = 0; col < q; ++col) {
+        if (tid == 0) {
             int piv = col;
-            for (int rr = col + 1; rr < q; ++rr)
-                if (fabs(Mx[rr][col]) > fabs(Mx[piv][col])) piv = rr;
-            for (int u = 0; u < q; ++u) {
-                double t = Mx[col][u];
-                Mx[col][u] = Mx[piv][u];
-                Mx[piv][u] = t;
-                t = Iv[col][u];
-                Iv[col][u] = Iv[piv][u];
-                Iv[piv][u] = t;
-            }
-            const double inv = 1.0 / Mx[col][col];
-            for (int u = 0; u < q; ++u) {
-                Mx[col][u] *= inv;
-                Iv[col][u] *= inv;
-            }
-            for (int rr = 0; rr < q; ++rr) {
-                if (rr == col) continue;
-                const double f = Mx[rr][col];
-                for (int u = 0; u < q; ++u) {
-                    Mx[rr][u] -= f * Mx[col][u];
-                    Iv[rr][u] -= f * Iv[col][u];
-                }
-            }
+            for (int r2 = col + 1; r2 < q; ++r2)
+                if (fabs(sG[r2][col]) > fabs(sG[piv][col])) piv = r2;
+            sPiv = piv;
         }
-        double *om = out + 16 * (int64_t)kp;
-        for (int a = 0; a < 8; ++a)
-            for (int u = 0; u < 8; ++u) om[a * 8 + u] = (a < q && u < q) ? Iv[a][u] : 0.0;
+        __syncthreads();
+        const int piv = sPiv;
+        if (tid < 16 && piv != col) {
+            const double t = sG[col][tid];
+            sG[col][tid] = sG[piv][tid];
+            sG[piv][tid] = t;
+        }
+        __syncthreads();
+        double pr = 0.0, f = 0.0, inv = 0.0;
+        if (tid < 128) {
+            inv = 1.0 / sG[col][col];
+            pr = sG[col][u] * inv;
+            f = sG[rr][col];
+        }
+        __syncthreads();
+        if (tid < 128 && rr < q) sG[rr][u] = rr == col ? pr : sG[rr][u] - f * pr;
+        __syncthreads();
+    }
+    if (tid < 64) {
+        const int a = tid >> 3, a2 = tid & 7;
+        out[16 * (int64_t)kp + tid] = (a < q && a2 < q) ? sG[a][8 + a2] : 0.0;
     }
 }
 
@@ -340,81 +365,79 @@ __global__ __launch_bounds__(128) void k_sgrad(const SlotCmd *__restrict__ cmds,
     }
 }
 
-// Start points of the full solve from restricted optima: for every INIT command with a
-// staged start (init_src != 0), with (a*, h*) at Wst + item * wstride (coefficients
-// a*(s, i) at s * kp + i, then the fields):
-//   c(s, b) = sum_i Q[b][i] a*(s, i),  (K_r c)(s, b) = sum_i KQ[b][i] a*(s, i) - (V_r a*)(x_br, s)
-// into x_c of the command's slot (coefficients, image, fields).  One block of 256 per
-// command; Qt and KQt are kp x Bp (sequences contiguous).
-__global__ __launch_bounds__(256) void k_gwarm(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
-                                               double *__restrict__ V, int64_t vstride, int64_t nvs,
-                                               const double *__restrict__ Qt, const double *__restrict__ KQt,
-                                               const double *__restrict__ Q, int kp, const double *__restrict__ Wst,
-                                               int64_t wstride)
+// The staged restricted optima of the round's new loci as the operand of the start
+// product: Ap[wrow + s][i] = a*(s, i).  One block of 256 per command.
+__global__ __launch_bounds__(256) void k_wpack(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                               const double *__restrict__ Wst, int64_t wstride, int kp,
+                                               double *__restrict__ Ap)
 {
     const SlotCmd &c = cmds[blockIdx.x];
-    if (c.mode != MODE_INIT || c.init_src == 0.f || c.item < 0) return;
+    if (c.wrow < 0) return;
+    const int q = aln[c.aln].q;
+    const double *ws = Wst + (int64_t)c.item * wstride;
+    for (int e = threadIdx.x; e < q * kp; e += 256) Ap[(int64_t)c.wrow * kp + e] = ws[e];
+}
+
+// The full solve's start of a new locus from its restricted optimum (a*, h*), given the
+// start product's columns c_s = Q a*_s (Gp) and (K Q a*)_s (Zp), column wrow + s of ldG:
+//   coefficients  c
+//   image         K_r c = K Q a* - U W,  W[a][s] = (V_r a*)(a, s) = sum_{b: x_br = a} c_s[b]
+//   fields        h*
+// into x_c.  One block of 256 per command; W from per-thread sums in LDS (thread t sums
+// state column s = t / 32 over the sequences b = t % 32 (mod 32)), reduced in a fixed order.
+__global__ __launch_bounds__(256) void k_gwarm(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                               double *__restrict__ V, int64_t vstride, int64_t nvs,
+                                               const double *__restrict__ Gp, const double *__restrict__ Zp,
+                                               int64_t ldG, int kp, const double *__restrict__ Wst, int64_t wstride)
+{
+    const SlotCmd &c = cmds[blockIdx.x];
+    if (c.wrow < 0) return;
     const GramAln &A = aln[c.aln];
     const int q = A.q, B = A.B, Bp = A.Bp;
     const int64_t qB = (int64_t)q * Bp;
-    __shared__ double sa[8][128];
-    __shared__ double sV[2][8][128];
+    __shared__ double sAcc[256][9];
     __shared__ double sW[8][8];
     const int tid = threadIdx.x;
-    const double *ws = Wst + (int64_t)c.item * wstride;
-    for (int e = tid; e < 8 * 128; e += 256) {
-        const int s = e >> 7, i = e & 127;
-        sa[s][i] = (s < q && i < kp) ? ws[(int64_t)s * kp + i] : 0.0;
-    }
-    // V_r[a][i] = sum_{b: x_br = a} Q[b][i] (as k_sinit), then W = V_r a*
     const uint8_t *y = A.statesT + (int64_t)c.locus * B;
-    const int i = tid % kp, half = tid / kp, nhalf = 256 / kp;
-    double acc[8];
+    const double *cc = Gp + (int64_t)c.wrow * ldG, *kc = Zp + (int64_t)c.wrow * ldG;
 #pragma unroll
-    for (int a = 0; a < 8; ++a) acc[a] = 0.0;
-    if (half < nhalf)
-        for (int b = half; b < B; b += nhalf) {
-            const int ya = y[b];
-            const double v = Q[(int64_t)b * kp + i];
+    for (int a = 0; a < 9; ++a) sAcc[tid][a] = 0.0;
+    {
+        const int s = tid >> 5, g = tid & 31;
+        if (s < q) {
+            const double *cs = cc + (int64_t)s * ldG;
+            int b = g;
+            for (; b + 96 < B; b += 128) {  // four sequences' loads ahead of the sums
+                double v[4];
+                int ys[4];
 #pragma unroll
-            for (int a = 0; a < 8; ++a) acc[a] += (a == ya) ? v : 0.0;
+                for (int u = 0; u < 4; ++u) {
+                    v[u] = cs[b + 32 * u];
+                    ys[u] = y[b + 32 * u] & 7;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) sAcc[tid][ys[u]] += v[u];
+            }
+            for (; b < B; b += 32) sAcc[tid][y[b] & 7] += cs[b];
         }
-    if (half < 2)
-#pragma unroll
-        for (int a = 0; a < 8; ++a) sV[half][a][i] = acc[a];
+    }
     __syncthreads();
     if (tid < 64) {
         const int a = tid >> 3, s = tid & 7;
-        double v = 0.0;
-        if (a < q && s < q)
-            for (int k = 0; k < kp; ++k) v += (nhalf > 1 ? sV[0][a][k] + sV[1][a][k] : sV[0][a][k]) * sa[s][k];
-        sW[a][s] = v;
+        double w = 0.0;
+        for (int g = 0; g < 32; ++g) w += sAcc[s * 32 + g][a];
+        sW[a][s] = w;
     }
     __syncthreads();
     double *xc = V + (int64_t)c.slot * vstride + gram::kVecXc * nvs;
-    for (int b = tid; b < Bp; b += 256) {
-        double cv[8], zv[8];
-#pragma unroll
-        for (int s = 0; s < 8; ++s) cv[s] = zv[s] = 0.0;
-        if (b < B) {
-            for (int k = 0; k < kp; ++k) {
-                const double qk = Qt[(int64_t)k * Bp + b], kq = KQt[(int64_t)k * Bp + b];
-#pragma unroll
-                for (int s = 0; s < 8; ++s) {
-                    cv[s] += qk * sa[s][k];
-                    zv[s] += kq * sa[s][k];
-                }
-            }
-            const int ya = y[b];
-#pragma unroll
-            for (int s = 0; s < 8; ++s) zv[s] -= sW[ya][s];
+    for (int s = 0; s < q; ++s)
+#pragma unroll 4
+        for (int b = tid; b < Bp; b += 256) {
+            const bool in = b < B;
+            xc[(int64_t)s * Bp + b] = in ? cc[(int64_t)s * ldG + b] : 0.0;
+            xc[qB + (int64_t)s * Bp + b] = in ? kc[(int64_t)s * ldG + b] - sW[y[b] & 7][s] : 0.0;
         }
-        for (int s = 0; s < q; ++s) {
-            xc[(int64_t)s * Bp + b] = cv[s];
-            xc[qB + (int64_t)s * Bp + b] = zv[s];
-        }
-    }
-    if (tid < 8) xc[2 * qB + tid] = tid < q ? ws[(int64_t)q * kp + tid] : 0.0;
+    if (tid < 8) xc[2 * qB + tid] = tid < q ? Wst[(int64_t)c.item * wstride + (int64_t)q * kp + tid] : 0.0;
 }
 
 }  // namespace
@@ -425,7 +448,8 @@ void init(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, con
           int kp, double *aux, int64_t auxstride)
 {
     if (ncmd <= 0) return;
-    hipLaunchKernelGGL(k_sinit, dim3(ncmd), dim3(256), 0, st, cmds, aln, Q, Minv, kp, aux, auxstride);
+    (void)kp;  // k_sinit is built for kp = kSubRank, the only rank the host allocates
+    hipLaunchKernelGGL(k_sinit, dim3(ncmd), dim3(256), 0, st, cmds, aln, Q, Minv, aux, auxstride);
 }
 
 void pack(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, const double *V, int64_t vstride,
@@ -454,10 +478,14 @@ void grad(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, dou
 }
 
 void warm(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride, int64_t nvs,
-          const double *Qt, const double *KQt, const double *Q, int kp, const double *Wst, int64_t wstride)
+          const double *Q, const double *KQ, int kp, int Bp, const double *Wst, int64_t wstride, double *Ap,
+          int nwcols, double *Gp, double *Zp, int64_t ldG)
 {
-    if (ncmd <= 0) return;
-    hipLaunchKernelGGL(k_gwarm, dim3(ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Qt, KQt, Q, kp, Wst,
+    if (ncmd <= 0 || nwcols <= 0) return;
+    hipLaunchKernelGGL(k_wpack, dim3(ncmd), dim3(256), 0, st, cmds, aln, Wst, wstride, kp, Ap);
+    gram::gemm_warm(st, Q, kp, Ap, kp, Gp, ldG, Bp, nwcols, kp);
+    gram::gemm_warm(st, KQ, kp, Ap, kp, Zp, ldG, Bp, nwcols, kp);
+    hipLaunchKernelGGL(k_gwarm, dim3(ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Gp, Zp, ldG, kp, Wst,
                        wstride);
 }
 
