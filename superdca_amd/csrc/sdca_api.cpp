@@ -1630,8 +1630,8 @@ int ensure_gram_ws(sdca_ctx *c, int nslots, int m)
     G->vstride = (sdca::gram::kVecPairs + 2 * m) * G->nvs;
     G->ldG = gBp;
     G->nchunk = gBp / 256;
-    G->nchq = gq * gBp / 256;
-    G->nchd = sdca::gram::dots_chunks((int64_t)gq * gBp);
+    G->nchq = sdca::gram::rows(gq) * gBp / 256;  // blocks of 256 over the coordinates of a vector half
+    G->nchd = sdca::gram::dots_chunks((int64_t)sdca::gram::rows(gq) * gBp);
     G->nmat = std::min(nslots, 128);
     G->ngroups = ng;
     G->gs = (nslots + ng - 1) / ng;
@@ -1724,7 +1724,7 @@ int gram_launch_graph(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
         sdca::SlotCmd &k = hc[i - s0];
         k = cmds[i];
         k.slot = i;
-        k.row = (i - s0) * A->q;
+        k.row = (i - s0) * sdca::gram::rows(A->q);
         k.aln = 0;
         k.q = A->q;
         k.Kx = (int)A->Kx;
@@ -1746,7 +1746,7 @@ int gram_launch_graph(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
         (void)hipMemcpyAsync(dc, hc, n * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st);
         sdca::gram::step(st, m, dc, n, G->d_galn, G->V, G->vstride, G->nvs, Gp, G->ldG, upd, sm, G->nchunk, G->nchq,
                          nullptr, nullptr);
-        sdca::gram::gemm(st, A->d_gram, A->Bp, Gp, G->ldG, Zp, G->ldG, A->Bp, n * A->q, A->Bp);
+        sdca::gram::gemm(st, A->d_gram, A->Bp, Gp, G->ldG, Zp, G->ldG, A->Bp, n * sdca::gram::rows(A->q), A->Bp);
         sdca::gram::dots(st, m, dc, n, G->d_galn, G->V, G->vstride, G->nvs, Zp, G->ldG, sm, G->nchunk, dot, G->nchd,
                          0);
         sdca::gram::reduce(st, m, n, upd, dot, sm, G->nchunk, G->nchq, G->nchd, res);
@@ -1792,10 +1792,10 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
             k.wrow = -1;
             if (G->warm_rounds && k.mode == sdca::MODE_INIT && k.init_src != 0.f && k.item >= 0) {
                 k.wrow = nwcols;
-                nwcols += A->q;
+                nwcols += sdca::gram::rows(A->q);
             }
             lst.push_back(i);
-            col += A->q;
+            col += sdca::gram::rows(A->q);
         }
         ncol[a] = col - col0[a];
     }
@@ -1874,8 +1874,8 @@ int gram_finish_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
         const sdca::SlotCmd &k = cmds[i];
         if (k.mode == sdca::MODE_IDLE) continue;
         const sdca_ctx *A = c->members[k.aln];
-        const double half = (double)A->q * A->Bp;  // coefficients (or image entries) of a vector
-        flops += 2.0 * A->B * (double)A->B * A->q;
+        const double half = (double)sdca::gram::rows(A->q) * A->Bp;  // coordinates (or image entries) of a vector
+        flops += 2.0 * A->B * (double)A->B * sdca::gram::rows(A->q);
         int npairs = 0;
         for (int p = 0; p < G->m; ++p) npairs += (k.live >> p) & 1u;
         // vector accesses (coefficients and image) of the trial-point pass, 8 bytes each
@@ -1919,7 +1919,7 @@ int gram_stage(sdca_ctx *c, int g, const Solver &S, const std::vector<int> &fin,
         const int i = fin[k];
         const Slot &sl = S.slot(i);
         const sdca_ctx *A = c->members[sl.aln];
-        const int64_t qB = (int64_t)A->q * A->Bp;
+        const int64_t qB = (int64_t)sdca::gram::rows(A->q) * A->Bp;
         const size_t u = G->pend.size();
         if ((int)u >= G->pcap) return fail(SDCA_E_STATE, "pending buffer full");
         // coefficients and fields of x_t or x_c into the pending buffer
@@ -2226,8 +2226,8 @@ int ensure_sub_ws(sdca_ctx *c, int32_t K)
         G->nvs_r = sdca::gram::vec_len(G->gq, kp);
         G->vstride_r = (sdca::gram::kVecPairs + 2 * G->m) * G->nvs_r;
         G->auxstride = 16 * kp + 64;
-        G->nchq_r = (int)((G->gq * kp + 255) / 256);
-        G->nchd_r = sdca::gram::dots_chunks((int64_t)G->gq * kp);
+        G->nchq_r = (int)((sdca::gram::rows(G->gq) * kp + 255) / 256);
+        G->nchd_r = sdca::gram::dots_chunks((int64_t)sdca::gram::rows(G->gq) * kp);
         const int64_t ncols = G->colcap * G->ngroups;
         const int npad = G->gs * G->ngroups;
         if ((rc = dalloc(&G->Vr, (size_t)G->slots * G->vstride_r)) || (rc = dalloc(&G->aux, (size_t)G->slots * G->auxstride)) ||
@@ -2249,7 +2249,7 @@ int ensure_sub_ws(sdca_ctx *c, int32_t K)
     g.lamJ2 = 2.0 * A->lambda_J;
     g.lamh2 = 2.0 * A->lambda_h;
     HIP_TRY(hipMemcpy(G->d_galn_r, &g, sizeof(g), hipMemcpyHostToDevice));
-    G->wstride = (int64_t)G->gq * kp + 8;
+    G->wstride = (int64_t)sdca::gram::rows(G->gq) * kp + 8;
     if (G->wcap < K) {
         dfree(G->Wst);
         if ((rc = dalloc(&G->Wst, (size_t)std::max(K, 1) * G->wstride))) return rc;
@@ -2279,7 +2279,7 @@ int sub_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds)
         k.row = col;
         k.q = A->q;
         lst.push_back(i);
-        col += A->q;
+        col += sdca::gram::rows(A->q);
     }
     const int nc = (int)lst.size();
     if (nc == 0) return SDCA_OK;
@@ -2333,7 +2333,7 @@ int sub_finish_round(sdca_ctx *c, int g, std::vector<double> &res)
     const sdca_ctx *A = c->members[0];
     c->timing.sub += elapsed(G->ev[g][4], G->ev[g][5]);
     c->timing.sub_evals += 1;
-    c->timing.flops_sub += 4.0 * A->B * (double)sdca::kSubRank * A->q * (double)lst.size();
+    c->timing.flops_sub += 4.0 * A->B * (double)sdca::kSubRank * sdca::gram::rows(A->q) * (double)lst.size();
     return SDCA_OK;
 }
 
@@ -2348,7 +2348,7 @@ int sub_solve(sdca_ctx *c, const int32_t *loci, int32_t K, std::vector<int> &nev
     const SolverLimits lim{c->opts.subspace_threshold, 0.0, c->opts.max_iter};
     Solver S(c, nslots, m, true, &lim);
     const int ng = G->ngroups, gs = G->gs;
-    const int q = c->members[0]->q;
+    const int R1 = sdca::gram::rows(c->members[0]->q);  // coordinate rows of a restricted vector half
     std::vector<double> res((size_t)nslots * R.width(), 0.0);
     std::vector<char> done_flag(nslots, 0);
     SlotPool pool(SlotPool::default_threads());
@@ -2406,15 +2406,15 @@ int sub_solve(sdca_ctx *c, const int32_t *loci, int32_t K, std::vector<int> &nev
                 meta[k] = (int64_t)keep[k] * G->vstride_r +
                           (sl.from_xt ? sdca::gram::kVecXt : sdca::gram::kVecXc) * G->nvs_r;
                 meta[nk + k] = (int64_t)sl.item * G->wstride;
-                meta[2 * nk + k] = (int64_t)q * kp;
-                meta[3 * nk + k] = 2 * (int64_t)q * kp;
-                meta[4 * nk + k] = (int64_t)q * kp;
+                meta[2 * nk + k] = (int64_t)R1 * kp;
+                meta[3 * nk + k] = 2 * (int64_t)R1 * kp;
+                meta[4 * nk + k] = (int64_t)R1 * kp;
                 ok[sl.item] = 1;
             }
             if (nk > 0) {
                 int64_t *dmeta = G->d_stage + (size_t)g * G->gs * 5;
                 HIP_TRY(hipMemcpyAsync(dmeta, meta, (size_t)nk * 5 * sizeof(int64_t), hipMemcpyHostToDevice, st));
-                sdca::gram::gather(st, G->Vr, G->Wst, dmeta, nk, (int64_t)q * kp, 8);
+                sdca::gram::gather(st, G->Vr, G->Wst, dmeta, nk, (int64_t)R1 * kp, 8);
                 HIP_TRY(hipGetLastError());
             }
             for (int i : finished) {

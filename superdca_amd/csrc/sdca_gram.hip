@@ -18,10 +18,13 @@
 // it by the locus' own column, T(s, a) = sum_{b: x_br = a} G_b(s), subtracted after the GEMM.
 // All vectors and reductions are FP64.
 //
-// Slot vectors (FP64, a command uses the first 2 q Bp + 8 entries of each):
-//   [0, q Bp)          coefficients c(s, b) at s * Bp + b (pads b >= B stay 0)
-//   [q Bp, 2 q Bp)     the image (K_r c)(s, b), same layout
-//   [2 q Bp, + q)      the fields h(s)
+// The coefficient columns sum to zero over the states; vectors hold their R = q - 1
+// coordinates in an orthonormal basis of the zero-sum vectors (gram::to_contrast), so that
+// the GEMM and every vector pass run over R rows instead of q.
+// Slot vectors (FP64, a command uses the first 2 R Bp + 8 entries of each):
+//   [0, R Bp)          coordinates c(k, b) at k * Bp + b (pads b >= B stay 0)
+//   [R Bp, 2 R Bp)     the image (K_r c)(k, b), same layout
+//   [2 R Bp, + q)      the fields h(s)
 // in the order x_c, x_t, g[0], g[1], d, s_0..s_{m-1}, y_0..y_{m-1}.
 #include <hip/hip_runtime.h>
 
@@ -495,7 +498,7 @@ __global__ __launch_bounds__(64) void k_ghstep(const SlotCmd *__restrict__ cmds,
             double cs[M], cy[M];
             load_step<M>(c, a, cs, cy);
             const SlotPtrs P = slot_ptrs(c, V, vstride, nvs, M);
-            const int64_t e[1] = {2 * (int64_t)A.q * A.Bp + s};
+            const int64_t e[1] = {2 * (int64_t)gram::rows(A.q) * A.Bp + s};
             double xn[1], dn[1], g[1];
             step_elems<M, 1>(a, c, cs, cy, P, nvs, e, xn, dn, g);
             xsqh = xn[0] * xn[0];
@@ -512,8 +515,8 @@ __global__ __launch_bounds__(64) void k_ghstep(const SlotCmd *__restrict__ cmds,
     }
 }
 
-// Trial point of every command, one thread per coefficient (s, b) and its image entry
-// (flat over the q Bp coefficients; blocks of 256).  upd_part[command][chunk] =
+// Trial point of every command, one thread per coordinate (k, b) and its image entry
+// (flat over the R Bp coordinates; blocks of 256).  upd_part[command][chunk] =
 // {c_t . K_r c_t, 0, d . g_k} with d . g_k = c_d . (K_r c_gk).
 template <int M, int PART>
 __global__ __launch_bounds__(256) void k_gstep(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
@@ -529,7 +532,7 @@ __global__ __launch_bounds__(256) void k_gstep(const SlotCmd *__restrict__ cmds,
         return;
     }
     const GramAln &A = aln[c.aln];
-    const int64_t qB = (int64_t)A.q * A.Bp;
+    const int64_t qB = (int64_t)gram::rows(A.q) * A.Bp;
     StepArgs a;
     double cs[M], cy[M];
     load_step<M>(c, a, cs, cy);
@@ -552,8 +555,9 @@ __global__ __launch_bounds__(256) void k_gstep(const SlotCmd *__restrict__ cmds,
 // Softmax of every command, one thread per sequence b (blocks of 256 sequences):
 //   E_b(s) = h_t(s) + (K_r c_t)(s, b), lz = log sum_s exp E_b(s) (max-shifted),
 //   f_b = w_b (lz - E_b(x_br)), G_b(s) = w_b (P_b(s) - [s == x_br])
-// (plmDCA_objective.hpp:86-151), G into the packed GEMM operand (column row + s, b
-// contiguous) and the trial gradient's coefficients G + 2 lambda_J c_t into g[gt].
+// (plmDCA_objective.hpp:86-151), the image taken from its coordinates to states first.
+// G's coordinates into the packed GEMM operand (column row + k, b contiguous) and the
+// trial gradient's coordinates G + 2 lambda_J c_t into g[gt].
 // sm_part[command][chunk] = {T(s, a) at 8 s + a, sum f_b}.
 __global__ __launch_bounds__(256) void k_gsoftmax(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
                                                   double *__restrict__ V, int64_t vstride, int64_t nvs,
@@ -570,8 +574,8 @@ __global__ __launch_bounds__(256) void k_gsoftmax(const SlotCmd *__restrict__ cm
         return;
     }
     const GramAln &A = aln[c.aln];
-    const int q = A.q, Bp = A.Bp, B = A.B;
-    const int64_t qB = (int64_t)q * Bp;
+    const int q = A.q, R = gram::rows(q), Bp = A.Bp, B = A.B;
+    const int64_t qB = (int64_t)R * Bp;
     double *base = V + (int64_t)c.slot * vstride;
     const double *xt = base + kVecXt * nvs;
     double *gt = base + (kVecG + (c.gt == 1)) * nvs;
@@ -581,8 +585,8 @@ __global__ __launch_bounds__(256) void k_gsoftmax(const SlotCmd *__restrict__ cm
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
         hs[s] = s < q ? xt[2 * qB + s] : 0.0;  // from k_ghstep
-        xtc[s] = (s < q && valid) ? xt[(int64_t)s * Bp + b] : 0.0;
-        xti[s] = (s < q && valid) ? xt[qB + (int64_t)s * Bp + b] : 0.0;
+        xtc[s] = (s < R && valid) ? xt[(int64_t)s * Bp + b] : 0.0;
+        xti[s] = (s < R && valid) ? xt[qB + (int64_t)s * Bp + b] : 0.0;
     }
     double G[8];
     double fb = 0.0;
@@ -592,10 +596,12 @@ __global__ __launch_bounds__(256) void k_gsoftmax(const SlotCmd *__restrict__ cm
     if (valid && b < B) {
         y = A.statesT[(int64_t)c.locus * B + b];
         const double wb = A.w[b];
+        double img[8];
+        gram::from_contrast(xti, q, img);  // (K_r c_t)(s, b) per state
         double E[8], mx = -INFINITY;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            E[s] = hs[s] + xti[s];
+            E[s] = hs[s] + img[s];
             if (s < q) mx = fmax(mx, E[s]);
         }
         double ex[8], z = 0.0, Ey = 0.0;
@@ -609,18 +615,21 @@ __global__ __launch_bounds__(256) void k_gsoftmax(const SlotCmd *__restrict__ cm
         fb = wb * (lz - Ey);
         const double iz = 1.0 / z;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            if (s >= q) continue;
-            G[s] = wb * (ex[s] * iz - (s == y ? 1.0 : 0.0));
-            Gp[(int64_t)(c.row + s) * ldG + b] = G[s];
-            gt[(int64_t)s * Bp + b] = G[s] + A.lamJ2 * xtc[s];
+        for (int s = 0; s < 8; ++s) G[s] = s < q ? wb * (ex[s] * iz - (s == y ? 1.0 : 0.0)) : 0.0;
+        double Gc[8];
+        gram::to_contrast(G, q, Gc);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k >= R) continue;
+            Gp[(int64_t)(c.row + k) * ldG + b] = Gc[k];
+            gt[(int64_t)k * Bp + b] = Gc[k] + A.lamJ2 * xtc[k];
         }
     } else if (valid) {
 #pragma unroll
-        for (int s = 0; s < 8; ++s)
-            if (s < q) {
-                gt[(int64_t)s * Bp + b] = 0.0;  // pad sequences (the GEMM reads them: zero)
-                Gp[(int64_t)(c.row + s) * ldG + b] = 0.0;
+        for (int k = 0; k < 8; ++k)
+            if (k < R) {
+                gt[(int64_t)k * Bp + b] = 0.0;  // pad sequences (the GEMM reads them: zero)
+                Gp[(int64_t)(c.row + k) * ldG + b] = 0.0;
             }
     }
     // T(s, a) = sum over the block's sequences with x_br = a of G_b(s), and the data term
@@ -642,10 +651,11 @@ __global__ __launch_bounds__(256) void k_gsoftmax(const SlotCmd *__restrict__ cm
     }
 }
 
-// The trial gradient's image and its products, flat over the q Bp coefficients (s, b) of a
+// The trial gradient's image and its products, flat over the R Bp coordinates (k, b) of a
 // command, blocks of 1024 (four per thread, 256 apart):
-//   (K_r g_t)(s, b) = Z(s, b) - T(s, x_br) + 2 lambda_J (K_r c_t)(s, b), Z = K G (the GEMM),
-// stored into g[gt]; the fields' gradient sum_b G_b(s) + 2 lambda_h h_t(s) (block 0).
+//   (K_r g_t)(k, b) = Z(k, b) - T^(k, x_br) + 2 lambda_J (K_r c_t)(k, b), Z = K G (the GEMM),
+// T^ the coordinates over s of T(s, a), stored into g[gt]; the fields' gradient
+// sum_b G_b(s) + 2 lambda_h h_t(s) (block 0).
 // part[command][chunk][2M + 5] = {g_t.s_p, g_t.y_p, g_t.g_t, g_t.g_k, g_t.d, 0, |g_t,J|^2}
 // in the primal metric (coefficients of one against the image of the other).
 constexpr int kDotsPerThread = 4;
@@ -662,7 +672,7 @@ __device__ __forceinline__ void gdots_chunked(const SlotCmd &c, const GramAln &A
     constexpr int CH = 8;
     static_assert(M % CH == 0, "history a multiple of the chunk");
     const int q = A.q, Bp = A.Bp, B = A.B;
-    const int64_t qB = (int64_t)q * Bp;
+    const int64_t qB = (int64_t)gram::rows(q) * Bp;
     const int tid = threadIdx.x;
     double tail[5] = {0.0, 0.0, 0.0, 0.0, 0.0};  // g.g, g.g_k, g.d, 0, |g_J|^2
     double gcv[kDotsPerThread];
@@ -681,7 +691,7 @@ __device__ __forceinline__ void gdots_chunked(const SlotCmd &c, const GramAln &A
         const double xi = xt[e1];
         const double gc = gt[e0];
         const double gki = cur ? gk[e1] : 0.0, di = cur ? dv[e1] : 0.0;
-        const double kg = seq ? z - (A.own ? sT[s * 8 + y] : 0.0) + A.lamJ2 * xi : 0.0;
+        const double kg = seq ? z - (A.own ? sT[64 + s * 8 + y] : 0.0) + A.lamJ2 * xi : 0.0;
         gt[e1] = kg;
         tail[0] += gc * kg;
         tail[4] += gc * kg;
@@ -750,7 +760,7 @@ __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds,
     constexpr int NV = 2 * M + 5;
     const int j = blockIdx.y;
     const SlotCmd &c = cmds[j];
-    __shared__ double sT[64];
+    __shared__ double sT[128];
     const int tid = threadIdx.x;
     double acc[NV];
 #pragma unroll
@@ -762,7 +772,7 @@ __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds,
     }
     const GramAln &A = aln[c.aln];
     const int q = A.q, Bp = A.Bp, B = A.B;
-    const int64_t qB = (int64_t)q * Bp;
+    const int64_t qB = (int64_t)gram::rows(q) * Bp;
     double *base = V + (int64_t)c.slot * vstride;
     const double *xt = base + kVecXt * nvs;
     double *gt = base + (kVecG + c.gt) * nvs;
@@ -770,12 +780,23 @@ __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds,
     const double *dv = base + kVecD * nvs;
     const double *Sb = base + kVecPairs * nvs;
     const double *Yb = Sb + (int64_t)M * nvs;
+    // sT[8 s + a] = T(s, a); sT[64 + 8 k + a] = its coordinates over s (the own column's
+    // term of the image, row k)
     if (tid < 64) {
         const int s = tid >> 3, u = tid & 7;
         double t = 0.0;
         if (s < q && u < q)
             for (int ch = 0; ch < nchunk; ++ch) t += sm_part[((int64_t)j * nchunk + ch) * kSm + tid];
         sT[tid] = t;
+    }
+    __syncthreads();
+    if (tid < 8) {
+        double col[8], w[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) col[s] = sT[s * 8 + tid];
+        gram::to_contrast(col, q, w);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sT[64 + k * 8 + tid] = w[k];
     }
     __syncthreads();
     const bool cur = c.mode != MODE_INIT;  // a current gradient and a direction exist
@@ -798,7 +819,7 @@ __global__ __launch_bounds__(256) void k_gdots(const SlotCmd *__restrict__ cmds,
         const double gki = cur ? gk[e1] : 0.0, di = cur ? dv[e1] : 0.0;
         // the pairs' images, CH at a time (all of them for histories up to 16)
         constexpr int CH = M > 16 ? 8 : M;
-        const double kg = seq ? z - (A.own ? sT[s * 8 + y] : 0.0) + A.lamJ2 * xi : 0.0;
+        const double kg = seq ? z - (A.own ? sT[64 + s * 8 + y] : 0.0) + A.lamJ2 * xi : 0.0;
         gt[e1] = kg;
         acc[2 * M] += gc * kg;
         acc[2 * M + 4] += gc * kg;
@@ -874,8 +895,9 @@ __global__ __launch_bounds__(256) void k_greduce(int m, const double *__restrict
 }
 
 // Coefficients of finished loci as the A operand of the coupling contraction J = X_r^T c:
-// item u's q rows from packed row rows[u] (FP32, or three BF16 parts hi + mid + lo), and
-// its fields into hpart[u][0..8) (the contraction's field column).
+// item u's q state rows (from its R coordinate rows) from packed row rows[u] (FP32, or
+// three BF16 parts hi + mid + lo), and its fields into hpart[u][0..8) (the contraction's
+// field column).
 __global__ __launch_bounds__(256) void k_gsplit(const double *__restrict__ V, const int64_t *__restrict__ src_off,
                                                 const int32_t *__restrict__ rows, const int32_t *__restrict__ alns,
                                                 const GramAln *__restrict__ aln, int64_t ldm, float *__restrict__ G32,
@@ -884,12 +906,18 @@ __global__ __launch_bounds__(256) void k_gsplit(const double *__restrict__ V, co
 {
     const int u = blockIdx.y;
     const GramAln &A = aln[alns[u]];
-    const int q = A.q, Bp = A.Bp;
+    const int q = A.q, R = gram::rows(q), Bp = A.Bp;
     const double *src = V + src_off[u];
     const int b = blockIdx.x * 256 + threadIdx.x;
     if (b < Bp) {
-        for (int s = 0; s < q; ++s) {
-            const double v = src[(int64_t)s * Bp + b];
+        double w[8], cs[8];  // the coordinates, then the coefficients per state
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w[k] = k < R ? src[(int64_t)k * Bp + b] : 0.0;
+        gram::from_contrast(w, q, cs);
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            if (s >= q) continue;
+            const double v = cs[s];
             const int64_t o = (int64_t)(rows[u] + s) * ldm + b;
             const float f = (float)v;
             if (G32) {
@@ -905,7 +933,7 @@ __global__ __launch_bounds__(256) void k_gsplit(const double *__restrict__ V, co
         }
     }
     if (blockIdx.x == 0 && threadIdx.x < 8)
-        hpart[(int64_t)u * 8 + threadIdx.x] = threadIdx.x < q ? src[2 * (int64_t)q * Bp + threadIdx.x] : 0.0;
+        hpart[(int64_t)u * 8 + threadIdx.x] = threadIdx.x < q ? src[2 * (int64_t)R * Bp + threadIdx.x] : 0.0;
 }
 
 // Copies of finished loci's vectors (gram::gather): block y is item u.

@@ -330,7 +330,49 @@ void warm(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, dou
 // pairs s_p and the m pairs y_p, each nvs doubles apart (coefficients, image, fields).
 namespace gram {
 constexpr int kVecXc = 0, kVecXt = 1, kVecG = 2, kVecD = 4, kVecPairs = 5;
-inline int64_t vec_len(int q, int Bp) { return 2 * (int64_t)q * Bp + 8; }
+
+// Zero-sum contrasts.  Every coefficient column c(., b) of the Gram solve sums to zero over
+// the q states: the gradient's coefficients G_b(s) = w_b (P_b(s) - [x_br == s]) do, every
+// iterate started from zero is a combination of gradients, and the optimum is
+// c* = -G*/(2 lambda_J).  The solve's vectors therefore hold the q - 1 coordinates
+// w = U^T c of c in the orthonormal (Helmert) basis of the zero-sum vectors of R^q:
+//   U[s][k] = h_k (s <= k),  -(k + 1) h_k (s = k + 1),  0 (s > k + 1),  h_k = 1/sqrt((k+1)(k+2)).
+// With U^T U = I the primal products c_u . (K_r c_v) and all norms are unchanged and K_r
+// acts on each coordinate row as on each state row: the GEMM, the vector passes and the
+// pairs have q - 1 rows instead of q.  Energies and beliefs are formed per state
+// (from_contrast), gradients return to coordinates (to_contrast).
+inline __host__ __device__ int rows(int q) { return q - 1; }
+// Slot vector: coefficients [0, R Bp), image [R Bp, 2 R Bp), fields [2 R Bp, + q), R = q - 1.
+inline int64_t vec_len(int q, int Bp) { return 2 * (int64_t)rows(q) * Bp + 8; }
+
+// w = U^T v for a q-vector v (entries s >= q ignored; w[k] = 0 for k >= q - 1).
+__device__ __forceinline__ void to_contrast(const double (&v)[8], int q, double (&w)[8])
+{
+    constexpr double h[7] = {0.7071067811865475, 0.4082482904638631, 0.2886751345948129, 0.22360679774997896,
+                             0.18257418583505536, 0.1543033499620919, 0.1336306209562122};
+    double run = 0.0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        run += v[k];
+        w[k] = k < q - 1 ? h[k] * (run - (double)(k + 1) * v[k + 1]) : 0.0;
+    }
+    w[7] = 0.0;
+}
+
+// v = U w, the zero-sum q-vector of coordinates w[0, q - 1) (v[s] = 0 for s >= q).
+__device__ __forceinline__ void from_contrast(const double (&w)[8], int q, double (&v)[8])
+{
+    constexpr double h[7] = {0.7071067811865475, 0.4082482904638631, 0.2886751345948129, 0.22360679774997896,
+                             0.18257418583505536, 0.1543033499620919, 0.1336306209562122};
+    double suf = 0.0;  // sum_{k >= s} h_k w[k]
+#pragma unroll
+    for (int s = 7; s >= 0; --s) {
+        if (s < 7 && s < q - 1) suf += h[s] * w[s];
+        double v_s = suf;
+        if (s > 0 && s < q) v_s -= (double)s * h[s - 1] * w[s - 1];
+        v[s] = s < q ? v_s : 0.0;
+    }
+}
 // Identical-locus counts over loci [j0, L) from bit-planes (k_bitplanes layout).
 void gram_matrix(hipStream_t st, const uint64_t *planes, int B, int q, int W, int j0, int64_t ldk, double *K);
 // For u < n, with meta = [src offsets | dst offsets | len1 | src2 | dst2] (n entries each):

@@ -172,8 +172,8 @@ __global__ __launch_bounds__(256) void k_sinit(const SlotCmd *__restrict__ cmds,
     }
 }
 
-// The round's trial coefficients as the packed operand of the energy product,
-// Ap[row + s][i] = a_t(s, i), and W[command][a][s] = (V_r a_t)(a, s) (8 x 8 per command).
+// The round's trial coordinates as the packed operand of the energy product,
+// Ap[row + k][i] = a_t(k, i), and W[command][a][k] = (V_r a_t)(a, k) (8 x 8 per command).
 // One block of 256 per command.
 __global__ __launch_bounds__(256) void k_spack(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
                                                const double *__restrict__ V, int64_t vstride, int64_t nvs, int kp,
@@ -183,30 +183,27 @@ __global__ __launch_bounds__(256) void k_spack(const SlotCmd *__restrict__ cmds,
     const int j = blockIdx.x;
     const SlotCmd &c = cmds[j];
     if (c.mode == MODE_IDLE) return;
-    const int q = aln[c.aln].q;
+    const int q = aln[c.aln].q, R = gram::rows(q);
     const double *xt = V + (int64_t)c.slot * vstride + gram::kVecXt * nvs;
     const double *Vr = aux + (int64_t)c.slot * auxstride;
     const int tid = threadIdx.x;
-    for (int e = tid; e < q * kp; e += 256) {
-        const int s = e / kp, i = e - (e / kp) * kp;
-        Ap[(int64_t)(c.row + s) * kp + i] = xt[e];
-    }
-    // W[a][s] = sum_i V_r[a][i] a_t(s, i): one wave per (a, s) pair at a time
+    for (int e = tid; e < R * kp; e += 256) Ap[(int64_t)c.row * kp + e] = xt[e];  // rows row + k
+    // W[a][k] = sum_i V_r[a][i] a_t(k, i): one wave per (a, k) pair at a time
     const int lane = tid & 63, wave = tid >> 6;
     for (int p = wave; p < 64; p += 4) {
-        const int a = p >> 3, s = p & 7;
+        const int a = p >> 3, k = p & 7;
         double v = 0.0;
-        if (a < q && s < q)
-            for (int i = lane; i < kp; i += 64) v += Vr[(int64_t)a * kp + i] * xt[(int64_t)s * kp + i];
+        if (a < q && k < R)
+            for (int i = lane; i < kp; i += 64) v += Vr[(int64_t)a * kp + i] * xt[(int64_t)k * kp + i];
         v = wsum(v);
         if (lane == 0) Wc[(int64_t)j * 64 + p] = v;
     }
 }
 
 // Softmax of the restricted trial points, one thread per sequence b (blocks of 256):
-//   E_b(s) = h_t(s) + Ep[row + s][b] - W[a = x_br][s],  Ep = (K Q) a_t from the product,
-// then G, the T table and the data term exactly as k_gsoftmax forms them (G into the packed
-// operand of the gradient product).  sm_part[command][chunk] = {T(s, a) at 8 s + a, sum f_b}.
+//   E_b(s) = h_t(s) + U (Ep[row + k][b] - W[a = x_br][k])_k,  Ep = (K Q) a_t from the
+// product, U the coordinates' basis, then G, the T table and the data term exactly as
+// k_gsoftmax forms them (G's coordinates into the packed operand of the gradient product).  sm_part[command][chunk] = {T(s, a) at 8 s + a, sum f_b}.
 __global__ __launch_bounds__(256) void k_ssoftmax(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
                                                   const double *__restrict__ V, int64_t vstride, int64_t nvs, int kp,
                                                   const double *__restrict__ Ep, const double *__restrict__ Wc,
@@ -223,12 +220,12 @@ __global__ __launch_bounds__(256) void k_ssoftmax(const SlotCmd *__restrict__ cm
         return;
     }
     const GramAln &A = aln[c.aln];
-    const int q = A.q, Bp = A.Bp, B = A.B;
+    const int q = A.q, R = gram::rows(q), Bp = A.Bp, B = A.B;
     const double *xt = V + (int64_t)c.slot * vstride + gram::kVecXt * nvs;
     const double *W = Wc + (int64_t)j * 64;
     double hs[8];
 #pragma unroll
-    for (int s = 0; s < 8; ++s) hs[s] = s < q ? xt[2 * (int64_t)q * kp + s] : 0.0;
+    for (int s = 0; s < 8; ++s) hs[s] = s < q ? xt[2 * (int64_t)R * kp + s] : 0.0;
     const int b = blockIdx.x * 256 + tid;
     double G[8];
     double fb = 0.0;
@@ -238,10 +235,14 @@ __global__ __launch_bounds__(256) void k_ssoftmax(const SlotCmd *__restrict__ cm
     if (b < B) {
         y = A.statesT[(int64_t)c.locus * B + b];
         const double wb = A.w[b];
+        double ec[8], es[8];  // (K_r Q a_t)(b, .): coordinates, then states
+#pragma unroll
+        for (int k = 0; k < 8; ++k) ec[k] = k < R ? Ep[(int64_t)(c.row + k) * ldG + b] - W[y * 8 + k] : 0.0;
+        gram::from_contrast(ec, q, es);
         double E[8], mx = -INFINITY;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            E[s] = s < q ? hs[s] + Ep[(int64_t)(c.row + s) * ldG + b] - W[y * 8 + s] : 0.0;
+            E[s] = s < q ? hs[s] + es[s] : 0.0;
             if (s < q) mx = fmax(mx, E[s]);
         }
         double ex[8], z = 0.0, Ey = 0.0;
@@ -255,15 +256,16 @@ __global__ __launch_bounds__(256) void k_ssoftmax(const SlotCmd *__restrict__ cm
         fb = wb * (lz - Ey);
         const double iz = 1.0 / z;
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            if (s >= q) continue;
-            G[s] = wb * (ex[s] * iz - (s == y ? 1.0 : 0.0));
-            Gp[(int64_t)(c.row + s) * ldG + b] = G[s];
-        }
+        for (int s = 0; s < 8; ++s) G[s] = s < q ? wb * (ex[s] * iz - (s == y ? 1.0 : 0.0)) : 0.0;
+        double Gc[8];
+        gram::to_contrast(G, q, Gc);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < R) Gp[(int64_t)(c.row + k) * ldG + b] = Gc[k];
     } else if (b < Bp) {
 #pragma unroll
-        for (int s = 0; s < 8; ++s)
-            if (s < q) Gp[(int64_t)(c.row + s) * ldG + b] = 0.0;  // pad sequences: zero
+        for (int k = 0; k < 8; ++k)
+            if (k < R) Gp[(int64_t)(c.row + k) * ldG + b] = 0.0;  // pad sequences: zero
     }
 #pragma unroll
     for (int s = 0; s < 8; ++s)
@@ -300,8 +302,8 @@ __global__ __launch_bounds__(128) void k_sgrad(const SlotCmd *__restrict__ cmds,
     const int j = blockIdx.x;
     const SlotCmd &c = cmds[j];
     if (c.mode == MODE_IDLE || c.gt < 0 || c.gt > 1) return;
-    const int q = aln[c.aln].q;
-    __shared__ double sT[64];
+    const int q = aln[c.aln].q, R = gram::rows(q);
+    __shared__ double sT[64], sTc[64];
     __shared__ double su[8][128];
     __shared__ double sv[8][8];
     const int tid = threadIdx.x;
@@ -313,6 +315,15 @@ __global__ __launch_bounds__(128) void k_sgrad(const SlotCmd *__restrict__ cmds,
         sT[tid] = t;
     }
     __syncthreads();
+    if (tid < 8) {  // sTc[8 k + a]: the coordinates over s of T(s, a)
+        double col[8], w[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) col[s] = sT[s * 8 + tid];
+        gram::to_contrast(col, q, w);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sTc[k * 8 + tid] = w[k];
+    }
+    __syncthreads();
     const double *Vr = aux + (int64_t)c.slot * auxstride;
     const double *Rr = Vr + 8 * (int64_t)kp;
     const double *Om = Vr + 16 * (int64_t)kp;
@@ -322,7 +333,7 @@ __global__ __launch_bounds__(128) void k_sgrad(const SlotCmd *__restrict__ cmds,
 #pragma unroll
     for (int s = 0; s < 8; ++s) u[s] = 0.0;
     if (i < kp)
-        for (int s = 0; s < q; ++s) {
+        for (int s = 0; s < R; ++s) {  // coordinate rows
             double bv = 0.0, uv = 0.0;
             for (int z = 0; z < nz; ++z) {
                 const double *zr = Z + z * zstride + (int64_t)(c.row + s) * ld;
@@ -330,8 +341,8 @@ __global__ __launch_bounds__(128) void k_sgrad(const SlotCmd *__restrict__ cmds,
                 uv += zr[kp + i];
             }
             for (int a = 0; a < q; ++a) {
-                bv -= Vr[(int64_t)a * kp + i] * sT[s * 8 + a];
-                uv -= Rr[(int64_t)i * 8 + a] * sT[s * 8 + a];
+                bv -= Vr[(int64_t)a * kp + i] * sTc[s * 8 + a];
+                uv -= Rr[(int64_t)i * 8 + a] * sTc[s * 8 + a];
             }
             Z[(int64_t)(c.row + s) * ld + i] = bv;
             u[s] = uv;
@@ -342,7 +353,7 @@ __global__ __launch_bounds__(128) void k_sgrad(const SlotCmd *__restrict__ cmds,
     if (tid < 64) {
         const int a = tid >> 3, s = tid & 7;
         double v = 0.0;
-        if (a < q && s < q)
+        if (a < q && s < R)
             for (int k = 0; k < kp; ++k) v += Vr[(int64_t)a * kp + k] * su[s][k];
         sv[a][s] = v;
     }
@@ -353,7 +364,7 @@ __global__ __launch_bounds__(128) void k_sgrad(const SlotCmd *__restrict__ cmds,
         double r[8];
 #pragma unroll
         for (int a = 0; a < 8; ++a) r[a] = a < q ? Rr[(int64_t)i * 8 + a] : 0.0;
-        for (int s = 0; s < q; ++s) {
+        for (int s = 0; s < R; ++s) {
             double g = u[s];
             for (int a = 0; a < q; ++a) {
                 double w = 0.0;  // (Omega_r v)(a, s)
@@ -366,16 +377,16 @@ __global__ __launch_bounds__(128) void k_sgrad(const SlotCmd *__restrict__ cmds,
 }
 
 // The staged restricted optima of the round's new loci as the operand of the start
-// product: Ap[wrow + s][i] = a*(s, i).  One block of 256 per command.
+// product: Ap[wrow + k][i] = a*(k, i).  One block of 256 per command.
 __global__ __launch_bounds__(256) void k_wpack(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
                                                const double *__restrict__ Wst, int64_t wstride, int kp,
                                                double *__restrict__ Ap)
 {
     const SlotCmd &c = cmds[blockIdx.x];
     if (c.wrow < 0) return;
-    const int q = aln[c.aln].q;
+    const int R = gram::rows(aln[c.aln].q);
     const double *ws = Wst + (int64_t)c.item * wstride;
-    for (int e = threadIdx.x; e < q * kp; e += 256) Ap[(int64_t)c.wrow * kp + e] = ws[e];
+    for (int e = threadIdx.x; e < R * kp; e += 256) Ap[(int64_t)c.wrow * kp + e] = ws[e];
 }
 
 // The full solve's start of a new locus from its restricted optimum (a*, h*), given the
@@ -393,8 +404,8 @@ __global__ __launch_bounds__(256) void k_gwarm(const SlotCmd *__restrict__ cmds,
     const SlotCmd &c = cmds[blockIdx.x];
     if (c.wrow < 0) return;
     const GramAln &A = aln[c.aln];
-    const int q = A.q, B = A.B, Bp = A.Bp;
-    const int64_t qB = (int64_t)q * Bp;
+    const int q = A.q, R = gram::rows(q), B = A.B, Bp = A.Bp;
+    const int64_t qB = (int64_t)R * Bp;
     __shared__ double sAcc[256][9];
     __shared__ double sW[8][8];
     const int tid = threadIdx.x;
@@ -404,7 +415,7 @@ __global__ __launch_bounds__(256) void k_gwarm(const SlotCmd *__restrict__ cmds,
     for (int a = 0; a < 9; ++a) sAcc[tid][a] = 0.0;
     {
         const int s = tid >> 5, g = tid & 31;
-        if (s < q) {
+        if (s < R) {
             const double *cs = cc + (int64_t)s * ldG;
             int b = g;
             for (; b + 96 < B; b += 128) {  // four sequences' loads ahead of the sums
@@ -430,14 +441,14 @@ __global__ __launch_bounds__(256) void k_gwarm(const SlotCmd *__restrict__ cmds,
     }
     __syncthreads();
     double *xc = V + (int64_t)c.slot * vstride + gram::kVecXc * nvs;
-    for (int s = 0; s < q; ++s)
+    for (int s = 0; s < R; ++s)
 #pragma unroll 4
         for (int b = tid; b < Bp; b += 256) {
             const bool in = b < B;
             xc[(int64_t)s * Bp + b] = in ? cc[(int64_t)s * ldG + b] : 0.0;
             xc[qB + (int64_t)s * Bp + b] = in ? kc[(int64_t)s * ldG + b] - sW[y[b] & 7][s] : 0.0;
         }
-    if (tid < 8) xc[2 * qB + tid] = tid < q ? Wst[(int64_t)c.item * wstride + (int64_t)q * kp + tid] : 0.0;
+    if (tid < 8) xc[2 * qB + tid] = tid < q ? Wst[(int64_t)c.item * wstride + (int64_t)R * kp + tid] : 0.0;
 }
 
 }  // namespace
