@@ -206,6 +206,7 @@ sdca_ctx::~sdca_ctx()
             if (e) (void)hipEventDestroy(e);
     for (int g = 0; g < sdca::kMaxGroups; ++g)
         if (gstream[g] && gstream[g] != stream) (void)hipStreamDestroy(gstream[g]);
+    if (fstream) (void)hipStreamDestroy(fstream);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -257,6 +258,10 @@ sdca::GramWs::~GramWs()
         for (auto &e : g)
             if (e) (void)hipEventDestroy(e);
     for (auto &e : stage_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : pend_free)
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : fev)
         if (e) (void)hipEventDestroy(e);
 }
 
@@ -1656,22 +1661,26 @@ int ensure_gram_ws(sdca_ctx *c, int nslots, int m)
     const size_t mel = (size_t)G->mrows * gBp;
     if ((rc = dalloc(&G->Mhi, mel)) || (rc = dalloc(&G->Mlo, mel)) || (rc = dalloc(&G->Ml3, mel))) return rc;
     G->pcap = G->nmat + G->gs;
-    if ((rc = dalloc(&G->Pend, (size_t)G->pcap * G->nvs))) return rc;
+    if ((rc = dalloc(&G->Pend, (size_t)2 * G->pcap * G->nvs))) return rc;
+    // score rows of one flush: a flush of more loci than this runs synchronously
+    G->nhrows = std::min(G->pcap, 4 * G->nmat);
     if ((rc = dalloc(&G->Mout, (size_t)G->nmat * nitem)) || (rc = dalloc(&G->mhpart, (size_t)G->nmat * 8)) ||
         (rc = dalloc(&G->d_mcmds, (size_t)G->nmat)) || (rc = dalloc(&G->d_mgroups, (size_t)nmem)) ||
         (rc = dalloc(&G->d_moff, (size_t)G->nmat)) || (rc = dalloc(&G->d_soff, (size_t)G->nmat)) ||
         (rc = dalloc(&G->d_mrow, (size_t)G->nmat)) || (rc = dalloc(&G->d_maln, (size_t)G->nmat)) ||
         (rc = dalloc(&G->d_sloc, (size_t)G->nmat)) || (rc = dalloc(&G->d_sld, (size_t)G->nmat)) ||
-        (rc = dalloc(&G->d_rows, (size_t)G->nmat * gL)))
+        (rc = dalloc(&G->d_rows, (size_t)G->nhrows * gL)))
         return rc;
     if ((rc = dalloc(&G->d_stage, (size_t)npad * 5))) return rc;
     HIP_TRY(hipHostMalloc((void **)&G->h_stage, (size_t)npad * 5 * sizeof(int64_t), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&G->h_cmds, (size_t)npad * sizeof(sdca::SlotCmd), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void **)&G->h_res, (size_t)npad * W * sizeof(double), hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void **)&G->h_rows, (size_t)G->nmat * gL * sizeof(float), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void **)&G->h_rows, (size_t)G->nhrows * gL * sizeof(float), hipHostMallocDefault));
     for (auto &g : G->ev)
         for (auto &e : g) HIP_TRY(hipEventCreate(&e));
     for (auto &e : G->stage_ev) HIP_TRY(hipEventCreate(&e));
+    for (auto &e : G->pend_free) HIP_TRY(hipEventCreate(&e));
+    for (auto &e : G->fev) HIP_TRY(hipEventCreate(&e));
     G->pend.clear();
     G->pend.reserve(G->pcap);
     HIP_TRY(hipDeviceSynchronize());
@@ -1924,7 +1933,7 @@ int gram_stage(sdca_ctx *c, int g, const Solver &S, const std::vector<int> &fin,
         if ((int)u >= G->pcap) return fail(SDCA_E_STATE, "pending buffer full");
         // coefficients and fields of x_t or x_c into the pending buffer
         meta[k] = (int64_t)i * G->vstride + (sl.from_xt ? sdca::gram::kVecXt : sdca::gram::kVecXc) * G->nvs;
-        meta[nf + k] = (int64_t)u * G->nvs;
+        meta[nf + k] = ((int64_t)G->phalf * G->pcap + (int64_t)u) * G->nvs;
         meta[2 * nf + k] = qB;
         meta[3 * nf + k] = 2 * qB;
         meta[4 * nf + k] = 2 * qB;
@@ -1946,6 +1955,7 @@ int gram_stage(sdca_ctx *c, int g, const Solver &S, const std::vector<int> &fin,
     if (nf > 0) {
         int64_t *dmeta = G->d_stage + (size_t)g * G->gs * 5;
         HIP_TRY(hipMemcpyAsync(dmeta, meta, (size_t)nf * 5 * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamWaitEvent(st, G->pend_free[G->phalf], 0));  // the half's last flush has read it
         sdca::gram::gather(st, G->V, G->Pend, dmeta, nf, maxlen, 8);
         HIP_TRY(hipGetLastError());
     }
@@ -1953,23 +1963,47 @@ int gram_stage(sdca_ctx *c, int g, const Solver &S, const std::vector<int> &fin,
     return SDCA_OK;
 }
 
-// Couplings, score rows and x_out of the pending loci, nmat per batch on group g's stream
-// (after the other groups' staging copies); all of them once nmat have gathered, or when
-// force is set.
-int gram_flush(sdca_ctx *c, int g, bool force, float *score_rows, const int64_t *row_off, double *x_out)
+// The score rows of the last flush into score_rows (or dropped, when score_rows is null),
+// once its batches have run.
+int gram_consume(sdca_ctx *c, float *score_rows, const int64_t *row_off)
+{
+    sdca::GramWs *G = c->gram;
+    if (!G->finflight) return SDCA_OK;
+    G->finflight = false;
+    HIP_TRY(hipStreamSynchronize(c->fstream));
+    c->timing.score += elapsed(G->fev[0], G->fev[1]);
+    if (score_rows)
+        for (const sdca::GramWs::RowOut &o : G->fout) {
+            const int La = c->members[o.aln]->L;
+            const int64_t off = row_off ? row_off[o.item] : (int64_t)o.item * La;
+            std::memcpy(score_rows + off, G->h_rows + (size_t)o.hrow * G->gL, (size_t)La * 4);
+        }
+    G->fout.clear();
+    return SDCA_OK;
+}
+
+// Couplings, score rows and x_out of the pending loci (one half of Pend), nmat per batch on
+// the coupling stream after every group's staging copies; all of them once nmat have
+// gathered, or when force is set.  The rounds go on meanwhile: the rows are copied out at
+// the next flush or by gram_consume at the end of the solve.  With x_out, or more pending
+// loci than the row buffers hold, each batch is waited for and copied out at once.
+int gram_flush(sdca_ctx *c, bool force, float *score_rows, const int64_t *row_off, double *x_out)
 {
     sdca::GramWs *G = c->gram;
     if (G->pend.empty() || (!force && (int)G->pend.size() < G->nmat)) return SDCA_OK;
-    hipStream_t st = c->gstream[g];
-    hipEvent_t *ev = G->ev[g];
-    for (int h = 0; h < G->ngroups; ++h)
-        if (h != g) HIP_TRY(hipStreamWaitEvent(st, G->stage_ev[h], 0));
+    int rc;
+    if ((rc = gram_consume(c, score_rows, row_off))) return rc;  // its row buffers are reused
+    hipStream_t st = c->fstream;
+    for (int h = 0; h < G->ngroups; ++h) HIP_TRY(hipStreamWaitEvent(st, G->stage_ev[h], 0));
+    const bool async = !x_out && (int)G->pend.size() <= G->nhrows;
     const int nmem = (int)c->members.size();
     const int gm = c->opts.gemm_gm_grad > 0 ? c->opts.gemm_gm_grad : 2;
     const int tn = c->opts.gemm_tn_grad > 0 ? c->opts.gemm_tn_grad : 256;
     const size_t n = (size_t)c->L * c->q * c->q + c->q;
     std::vector<float> xdev(x_out ? G->n_item : 0);
     const std::vector<sdca::GramWs::Pending> &P = G->pend;
+    const int64_t pbase = (int64_t)G->phalf * G->pcap;
+    HIP_TRY(hipEventRecord(G->fev[0], st));
     for (size_t u0 = 0; u0 < P.size(); u0 += G->nmat) {
         const int nb = (int)std::min<size_t>(G->nmat, P.size() - u0);
         // pending entries of the batch by alignment
@@ -1991,7 +2025,7 @@ int gram_flush(sdca_ctx *c, int g, bool force, float *score_rows, const int64_t 
             int v = u;
             for (; v < nb && P[order[v]].aln == a; ++v) {
                 const sdca::GramWs::Pending &pe = P[order[v]];
-                moff[v] = (int64_t)order[v] * G->nvs;
+                moff[v] = (pbase + order[v]) * G->nvs;
                 mrow[v] = row;
                 maln[v] = a;
                 sloc[v] = pe.r;
@@ -2031,20 +2065,25 @@ int gram_flush(sdca_ctx *c, int g, bool force, float *score_rows, const int64_t 
         HIP_TRY(hipMemcpyAsync(G->d_mcmds, mc.data(), nb * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
         HIP_TRY(hipMemcpyAsync(G->d_mgroups, groups.data(), groups.size() * sizeof(sdca::GemmGroup),
                                hipMemcpyHostToDevice, st));
-        HIP_TRY(hipEventRecord(ev[4], st));
+        // the batch's score rows: at its place among the flush's rows, or at the start
+        const int64_t hrow0 = async ? (int64_t)u0 : 0;
+        float *drows = G->d_rows + (size_t)hrow0 * G->gL;
         sdca::gram::split(st, G->d_moff, G->d_mrow, G->d_maln, nb, G->nchunk, G->d_galn, G->Pend, G->gBp, nullptr,
                           G->Mhi, G->Mlo, G->Ml3, G->mhpart);
         sdca::kern::gemm_materialize(st, SDCA_PREC_SPLIT_BF16, G->d_mgroups, (int)groups.size(), grid, tn, G->M32,
                                      G->Mhi, G->Mlo, G->Ml3, G->gBp, G->d_aln, G->d_mcmds, G->Mout, G->n_item,
                                      G->mhpart);
-        sdca::kern::score(st, G->Mout, G->d_soff, G->d_sld, G->d_sloc, G->d_maln, nb, G->gL, G->gL, G->d_aln,
-                          G->d_rows);
+        sdca::kern::score(st, G->Mout, G->d_soff, G->d_sld, G->d_sloc, G->d_maln, nb, G->gL, G->gL, G->d_aln, drows);
         HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(ev[5], st));
+        if (async) {
+            for (int v = 0; v < nb; ++v) G->fout.push_back({P[order[v]].item, P[order[v]].aln, hrow0 + v});
+            continue;
+        }
+        HIP_TRY(hipEventRecord(G->fev[1], st));
         if (score_rows)
             HIP_TRY(hipMemcpyAsync(G->h_rows, G->d_rows, (size_t)nb * G->gL * 4, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
-        c->timing.score += elapsed(ev[4], ev[5]);
+        c->timing.score += elapsed(G->fev[0], G->fev[1]);
         for (int v = 0; v < nb; ++v) {
             const sdca::GramWs::Pending &pe = P[order[v]];
             const int La = c->members[pe.aln]->L;
@@ -2057,8 +2096,17 @@ int gram_flush(sdca_ctx *c, int g, bool force, float *score_rows, const int64_t 
                 dev_to_ref(c, pe.r, xdev.data(), x_out + (size_t)pe.item * n);
             }
         }
+        HIP_TRY(hipEventRecord(G->fev[0], st));
+    }
+    HIP_TRY(hipEventRecord(G->pend_free[G->phalf], st));
+    if (async) {
+        HIP_TRY(hipEventRecord(G->fev[1], st));
+        if (score_rows)
+            HIP_TRY(hipMemcpyAsync(G->h_rows, G->d_rows, P.size() * (size_t)G->gL * 4, hipMemcpyDeviceToHost, st));
+        G->finflight = true;
     }
     G->pend.clear();
+    G->phalf ^= 1;
     return SDCA_OK;
 }
 
@@ -2447,6 +2495,13 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
     if ((rc = upload_gram_alns(c))) return rc;
     const sdca::ResLayout R{m};
     sdca::GramWs *G = c->gram;
+    // nothing of an earlier call that returned early: its flush in flight, its pending loci
+    if (G->finflight) {
+        HIP_TRY(hipStreamSynchronize(c->fstream));
+        G->finflight = false;
+        G->fout.clear();
+    }
+    G->pend.clear();
     // subspace warm start: the restricted solves of every listed locus first
     std::vector<int> sub_nev;
     std::vector<char> sub_ok;
@@ -2528,7 +2583,7 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
             }
             // a full batch of finished loci: couplings and score rows (while the other group's
             // round runs)
-            if ((rc = gram_flush(c, g, false, score_rows, row_off, x_out))) return rc;
+            if ((rc = gram_flush(c, false, score_rows, row_off, x_out))) return rc;
             if (group_active(g)) {
                 S.mark_live(g * gs, group_end(g));
                 if ((rc = gram_launch_round(c, g, S.cmds()))) return rc;
@@ -2537,7 +2592,8 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
         }
         if (!any) break;
     }
-    if ((rc = gram_flush(c, 0, true, score_rows, row_off, x_out))) return rc;
+    if ((rc = gram_flush(c, true, score_rows, row_off, x_out))) return rc;
+    if ((rc = gram_consume(c, score_rows, row_off))) return rc;
     G->warm_rounds = false;
     G->sub_nev = nullptr;
     c->timing.total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2577,6 +2633,7 @@ int create_streams(sdca_ctx *c)
     c->gstream[0] = c->stream;
     for (int g = 1; g < sdca::kMaxGroups; ++g)
         if (hipStreamCreate(&c->gstream[g]) != hipSuccess) return fail(SDCA_E_HIP, "hipStreamCreate failed");
+    if (hipStreamCreate(&c->fstream) != hipSuccess) return fail(SDCA_E_HIP, "hipStreamCreate failed");
     for (auto &g : c->gev)
         for (auto &e : g)
             if (hipEventCreate(&e) != hipSuccess) return fail(SDCA_E_HIP, "hipEventCreate failed");

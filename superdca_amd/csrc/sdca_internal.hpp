@@ -448,10 +448,22 @@ struct GramWs {
     struct Pending {
         int item, r, aln;
     };
-    double *Pend = nullptr;
-    int pcap = 0;
+    double *Pend = nullptr;  // two halves of pcap entries: new entries go to half phalf
+    int pcap = 0, phalf = 0;
     std::vector<Pending> pend;
     hipEvent_t stage_ev[kMaxGroups] = {};
+    // a flush reads one half of Pend on c->fstream while the rounds go on; pend_free[h]
+    // follows the last flush of half h (gathers into h wait for it).  Its score rows (up to
+    // nhrows, d_rows to pinned h_rows) are copied out at the next flush or at the end of the
+    // solve: fout lists them, fev brackets the flush.
+    struct RowOut {
+        int item, aln;
+        int64_t hrow;
+    };
+    hipEvent_t pend_free[2] = {}, fev[2] = {};
+    std::vector<RowOut> fout;
+    bool finflight = false;
+    int nhrows = 0;
     int nmat = 0, mrows = 0;
     int64_t n_item = 0;
     float *M32 = nullptr, *Mout = nullptr;
@@ -631,6 +643,7 @@ struct sdca_ctx {
 
     hipEvent_t ev[8] = {};
     hipStream_t gstream[sdca::kMaxGroups] = {};  // gstream[0] is stream
+    hipStream_t fstream = nullptr;  // the Gram solve's coupling batches of finished loci
     hipEvent_t gev[sdca::kMaxGroups][9] = {};  // [8]: a group's score rows are in h_rows
     sdca::Timing timing;
 
