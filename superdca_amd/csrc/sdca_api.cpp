@@ -218,6 +218,8 @@ sdca::GramWs::~GramWs()
     dfree(Zr);
     dfree(Wc);
     dfree(Wst);
+    dfree(Cst);
+    dfree(Kst);
     dfree(d_galn_r);
     dfree(V);
     dfree(Gp);
@@ -1010,7 +1012,6 @@ sdca::SlotCmd idle_cmd()
     k.locus = -1;
     k.level = sdca::LV_SPLIT;
     k.item = -1;
-    k.wrow = -1;
     return k;
 }
 
@@ -1785,7 +1786,7 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     lst.clear();
     sdca::SlotCmd *hc = G->h_cmds + s0;
     std::vector<int> col0(nmem, 0), ncol(nmem, 0);
-    int col = 0, nwcols = 0;
+    int col = 0;
     for (int a = 0; a < nmem; ++a) {
         const sdca_ctx *A = c->members[a];
         col0[a] = col;
@@ -1798,11 +1799,6 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
             k.q = A->q;
             k.Kx = (int)A->Kx;
             k.Kxp = (int)A->Kxp;
-            k.wrow = -1;
-            if (G->warm_rounds && k.mode == sdca::MODE_INIT && k.init_src != 0.f && k.item >= 0) {
-                k.wrow = nwcols;
-                nwcols += sdca::gram::rows(A->q);
-            }
             lst.push_back(i);
             col += sdca::gram::rows(A->q);
         }
@@ -1818,12 +1814,8 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     double *dot = G->dot_part + (size_t)s0 * G->nchd * sdca::ResLayout{m}.ndots(), *res = G->res + (size_t)s0 * W;
     HIP_TRY(hipMemcpyAsync(dc, hc, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(ev[0], st));
-    if (G->warm_rounds && nwcols > 0) {  // staged starts of the round's new loci into their x_c
-        const sdca_ctx *A = c->members[0];
-        sdca::sub::warm(st, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, A->d_subQ, A->d_subKQ, sdca::kSubRank,
-                        A->Bp, G->Wst, G->wstride, G->Ap + (size_t)g * G->colcap * sdca::kSubRank, nwcols, Gp, Zp,
-                        G->ldG);
-    }
+    if (G->warm_rounds)  // staged starts of the round's new loci into their x_c
+        sdca::sub::warm(st, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, G->Cst, G->Kst, G->Wst + G->wfld);
     sdca::gram::step(st, m, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, Gp, G->ldG, upd, sm, G->nchunk, G->nchq,
                      ev[6], ev[7]);
     HIP_TRY(hipGetLastError());
@@ -2119,6 +2111,9 @@ int gram_flush(sdca_ctx *c, bool force, float *score_rows, const int64_t *row_of
 // optima (a*, h*) are staged per locus; the full solve then starts each locus at
 // c = Q a*, K_r c = K Q a* - U V_r a* (k_gwarm) instead of zero.
 
+// Device memory for the start products of one solve call's loci (subspace warm start).
+constexpr double kMaxStartBytes = 64.0 * (1ull << 30);
+
 bool subspace_wanted(const sdca_ctx *c)
 {
     if (c->opts.subspace_rank <= 0 || c->members.size() != 1 || c->scan_split > 0) return false;
@@ -2297,10 +2292,20 @@ int ensure_sub_ws(sdca_ctx *c, int32_t K)
     g.lamJ2 = 2.0 * A->lambda_J;
     g.lamh2 = 2.0 * A->lambda_h;
     HIP_TRY(hipMemcpy(G->d_galn_r, &g, sizeof(g), hipMemcpyHostToDevice));
-    G->wstride = (int64_t)sdca::gram::rows(G->gq) * kp + 8;
+    // staged optima: the coordinate rows of every item (the start product reads whole tiles
+    // of kGemmCols rows), then 8 fields per item; their start products
+    const int R = sdca::gram::rows(A->q);
+    const int64_t wrows = round_up((int64_t)std::max(K, 1) * R, sdca::gram::kGemmCols) + sdca::gram::kGemmCols;
+    G->wfld = wrows * kp;
     if (G->wcap < K) {
         dfree(G->Wst);
-        if ((rc = dalloc(&G->Wst, (size_t)std::max(K, 1) * G->wstride))) return rc;
+        dfree(G->Cst);
+        dfree(G->Kst);
+        G->wcap = 0;
+        const size_t nst = (size_t)std::max(K, 1) * R * A->Bp;
+        if ((rc = dalloc(&G->Wst, (size_t)G->wfld + (size_t)std::max(K, 1) * 8)) || (rc = dalloc(&G->Cst, nst)) ||
+            (rc = dalloc(&G->Kst, nst)))
+            return rc;
         G->wcap = K;
     }
     return SDCA_OK;
@@ -2385,7 +2390,8 @@ int sub_finish_round(sdca_ctx *c, int g, std::vector<double> &res)
     return SDCA_OK;
 }
 
-// The restricted solves of items [0, K): their optima into G->Wst (item * wstride), their
+// The restricted solves of items [0, K): their optima into G->Wst (coordinates of item i at
+// rows [i R, (i + 1) R), fields at wfld + 8 i), their
 // evaluations into nev, and whether each may start the full solve into ok.
 int sub_solve(sdca_ctx *c, const int32_t *loci, int32_t K, std::vector<int> &nev, std::vector<char> &ok)
 {
@@ -2453,10 +2459,10 @@ int sub_solve(sdca_ctx *c, const int32_t *loci, int32_t K, std::vector<int> &nev
                 const Slot &sl = S.slot(keep[k]);
                 meta[k] = (int64_t)keep[k] * G->vstride_r +
                           (sl.from_xt ? sdca::gram::kVecXt : sdca::gram::kVecXc) * G->nvs_r;
-                meta[nk + k] = (int64_t)sl.item * G->wstride;
+                meta[nk + k] = (int64_t)sl.item * R1 * kp;
                 meta[2 * nk + k] = (int64_t)R1 * kp;
                 meta[3 * nk + k] = 2 * (int64_t)R1 * kp;
-                meta[4 * nk + k] = (int64_t)R1 * kp;
+                meta[4 * nk + k] = G->wfld + (int64_t)sl.item * 8 - (int64_t)sl.item * R1 * kp;
                 ok[sl.item] = 1;
             }
             if (nk > 0) {
@@ -2506,9 +2512,18 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
     std::vector<int> sub_nev;
     std::vector<char> sub_ok;
     G->warm_rounds = false;
-    if (K > 0 && subspace_wanted(c) && ensure_subspace(c->members[0], c->stream) == SDCA_OK) {
+    // (the start products take 16 R Bp bytes per locus: batches beyond kMaxStartBytes start cold)
+    const bool sub_fits = (double)K * 16.0 * sdca::gram::rows(c->members[0]->q) * c->members[0]->Bp <= kMaxStartBytes;
+    if (K > 0 && sub_fits && subspace_wanted(c) && ensure_subspace(c->members[0], c->stream) == SDCA_OK) {
         if ((rc = ensure_sub_ws(c, K))) return rc;
         if ((rc = sub_solve(c, loci, K, sub_nev, sub_ok))) return rc;
+        // the start products of every restricted optimum at once (loci whose restricted solve
+        // failed start from zero: their columns are not read)
+        const sdca_ctx *A = c->members[0];
+        sdca::sub::starts(c->gstream[0], A->d_subQ, A->d_subKQ, sdca::kSubRank, A->Bp, G->Wst,
+                          (int64_t)K * sdca::gram::rows(A->q), G->Cst, G->Kst);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(c->gstream[0]));
         G->warm_rounds = true;
     }
     G->sub_nev = sub_nev.empty() ? nullptr : sub_nev.data();

@@ -77,7 +77,6 @@ struct SlotCmd {
     int32_t pnew;     // NEWDIR: pair slot receiving s = pstep d_old, y = g[gk] - g[1-gk]; -1: none
     uint32_t live;    // bit p: pair slot p holds a stored pair
     int32_t item;     // index of the locus in the solve's list (staged Gram warm starts), -1 if none
-    int32_t wrow;     // MODE_INIT from a staged start: first of its q columns in the round's start product, else -1
     float alpha;      // step of the trial point x_t = x_c + alpha d
     float pstep;      // NEWDIR with accept: step of the accepted trial
     float init_src;   // MODE_INIT: 1 = copy the staged start point, 0 = zero start
@@ -318,12 +317,15 @@ void softmax(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, 
 void grad(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride, int64_t nvs,
           int kp, const double *aux, int64_t auxstride, const double *sm_part, int nchunk, double *Z, int nz,
           int64_t zstride, double lamJ2);
-// Staged starts of the round's new loci (commands with wrow >= 0, nwcols = their columns):
-// their coefficients a* into Ap (rows wrow + s), c = Q a* into Gp and K Q a* into Zp (columns
-// wrow + s of ldG, FP64 MFMA over Bp rows), then x_c = (c, K Q a* - U (U^T c), h*).
+// The start products of every staged restricted optimum: with a* of item i at rows
+// [i R, (i + 1) R) of Wst (kp each), c = Q a* into Cst and K Q a* into Kst (R Bp per item,
+// FP64 MFMA; nrows = K R rows).
+void starts(hipStream_t st, const double *Q, const double *KQ, int kp, int Bp, const double *Wst, int64_t nrows,
+            double *Cst, double *Kst);
+// Staged starts of the round's new loci (INIT commands with init_src != 0):
+// x_c = (c, K Q a* - U (U^T c), h*) from Cst, Kst and the fields Hst (8 per item).
 void warm(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride, int64_t nvs,
-          const double *Q, const double *KQ, int kp, int Bp, const double *Wst, int64_t wstride, double *Ap,
-          int nwcols, double *Gp, double *Zp, int64_t ldG);
+          const double *Cst, const double *Kst, const double *Hst);
 }  // namespace sub
 
 // The Gram formulation (sdca_gram.hip): slot vectors x_c, x_t, g[0], g[1], d, then the m
@@ -482,10 +484,12 @@ struct GramWs {
     // subspace warm start: restricted slot vectors (nvs_r doubles, vstride_r per slot), per-slot
     // constants (aux, auxstride per slot), the packed restricted coefficients Ap and products
     // Zr (kSubRank per column), W = V_r a_t (64 per command), its descriptor, and the staged
-    // restricted optima Wst (wstride per item, wcap items)
-    int64_t nvs_r = 0, vstride_r = 0, auxstride = 0, wstride = 0, wcap = 0;
+    // restricted optima Wst (wcap items) with their start products Cst, Kst
+    int64_t nvs_r = 0, vstride_r = 0, auxstride = 0, wcap = 0;
+    int64_t wfld = 0;  // Wst: a* of item i at rows [i R, (i + 1) R) of kSubRank, its fields at wfld + 8 i
     int nchq_r = 0, nchd_r = 0;
     double *Vr = nullptr, *aux = nullptr, *Ap = nullptr, *Zr = nullptr, *Wc = nullptr, *Wst = nullptr;
+    double *Cst = nullptr, *Kst = nullptr;  // per item: Q a* and K Q a* (R x Bp each)
     GramAln *d_galn_r = nullptr;
     bool warm_rounds = false;  // the current solve's full rounds stage warm starts (k_gwarm)
     const int *sub_nev = nullptr;  // the current solve's restricted evaluations per item

@@ -376,33 +376,21 @@ __global__ __launch_bounds__(128) void k_sgrad(const SlotCmd *__restrict__ cmds,
     }
 }
 
-// The staged restricted optima of the round's new loci as the operand of the start
-// product: Ap[wrow + k][i] = a*(k, i).  One block of 256 per command.
-__global__ __launch_bounds__(256) void k_wpack(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
-                                               const double *__restrict__ Wst, int64_t wstride, int kp,
-                                               double *__restrict__ Ap)
-{
-    const SlotCmd &c = cmds[blockIdx.x];
-    if (c.wrow < 0) return;
-    const int R = gram::rows(aln[c.aln].q);
-    const double *ws = Wst + (int64_t)c.item * wstride;
-    for (int e = threadIdx.x; e < R * kp; e += 256) Ap[(int64_t)c.wrow * kp + e] = ws[e];
-}
-
-// The full solve's start of a new locus from its restricted optimum (a*, h*), given the
-// start product's columns c_s = Q a*_s (Gp) and (K Q a*)_s (Zp), column wrow + s of ldG:
-//   coefficients  c
-//   image         K_r c = K Q a* - U W,  W[a][s] = (V_r a*)(a, s) = sum_{b: x_br = a} c_s[b]
-//   fields        h*
-// into x_c.  One block of 256 per command; W from per-thread sums in LDS (thread t sums
-// state column s = t / 32 over the sequences b = t % 32 (mod 32)), reduced in a fixed order.
+// The full solve's start of a new locus from its restricted optimum (a*, h*), given its
+// start products c = Q a* (Cst) and K Q a* (Kst), R x Bp each:
+//   coordinates  c
+//   image        K_r c = K Q a* - U W,  W[a][k] = (V_r a*)(a, k) = sum_{b: x_br = a} c(k, b)
+//   fields       h*
+// into x_c.  One block of 256 per INIT command with a staged start; W from per-thread sums
+// in LDS (thread t sums row k = t / 32 over the sequences b = t % 32 (mod 32)), reduced in a
+// fixed order.
 __global__ __launch_bounds__(256) void k_gwarm(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
                                                double *__restrict__ V, int64_t vstride, int64_t nvs,
-                                               const double *__restrict__ Gp, const double *__restrict__ Zp,
-                                               int64_t ldG, int kp, const double *__restrict__ Wst, int64_t wstride)
+                                               const double *__restrict__ Cst, const double *__restrict__ Kst,
+                                               const double *__restrict__ Hst)
 {
     const SlotCmd &c = cmds[blockIdx.x];
-    if (c.wrow < 0) return;
+    if (c.mode != MODE_INIT || c.init_src == 0.f || c.item < 0) return;
     const GramAln &A = aln[c.aln];
     const int q = A.q, R = gram::rows(q), B = A.B, Bp = A.Bp;
     const int64_t qB = (int64_t)R * Bp;
@@ -410,7 +398,8 @@ __global__ __launch_bounds__(256) void k_gwarm(const SlotCmd *__restrict__ cmds,
     __shared__ double sW[8][8];
     const int tid = threadIdx.x;
     const uint8_t *y = A.statesT + (int64_t)c.locus * B;
-    const double *cc = Gp + (int64_t)c.wrow * ldG, *kc = Zp + (int64_t)c.wrow * ldG;
+    const int64_t ldG = Bp;
+    const double *cc = Cst + (int64_t)c.item * qB, *kc = Kst + (int64_t)c.item * qB;
 #pragma unroll
     for (int a = 0; a < 9; ++a) sAcc[tid][a] = 0.0;
     {
@@ -448,7 +437,7 @@ __global__ __launch_bounds__(256) void k_gwarm(const SlotCmd *__restrict__ cmds,
             xc[(int64_t)s * Bp + b] = in ? cc[(int64_t)s * ldG + b] : 0.0;
             xc[qB + (int64_t)s * Bp + b] = in ? kc[(int64_t)s * ldG + b] - sW[y[b] & 7][s] : 0.0;
         }
-    if (tid < 8) xc[2 * qB + tid] = tid < q ? Wst[(int64_t)c.item * wstride + (int64_t)R * kp + tid] : 0.0;
+    if (tid < 8) xc[2 * qB + tid] = tid < q ? Hst[(int64_t)c.item * 8 + tid] : 0.0;
 }
 
 }  // namespace
@@ -488,16 +477,19 @@ void grad(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, dou
                        nchunk, Z, nz, zstride, lamJ2);
 }
 
-void warm(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride, int64_t nvs,
-          const double *Q, const double *KQ, int kp, int Bp, const double *Wst, int64_t wstride, double *Ap,
-          int nwcols, double *Gp, double *Zp, int64_t ldG)
+void starts(hipStream_t st, const double *Q, const double *KQ, int kp, int Bp, const double *Wst, int64_t nrows,
+            double *Cst, double *Kst)
 {
-    if (ncmd <= 0 || nwcols <= 0) return;
-    hipLaunchKernelGGL(k_wpack, dim3(ncmd), dim3(256), 0, st, cmds, aln, Wst, wstride, kp, Ap);
-    gram::gemm_warm(st, Q, kp, Ap, kp, Gp, ldG, Bp, nwcols, kp);
-    gram::gemm_warm(st, KQ, kp, Ap, kp, Zp, ldG, Bp, nwcols, kp);
-    hipLaunchKernelGGL(k_gwarm, dim3(ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Gp, Zp, ldG, kp, Wst,
-                       wstride);
+    if (nrows <= 0) return;
+    gram::gemm_warm(st, Q, kp, Wst, kp, Cst, Bp, Bp, (int)nrows, kp);
+    gram::gemm_warm(st, KQ, kp, Wst, kp, Kst, Bp, Bp, (int)nrows, kp);
+}
+
+void warm(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride, int64_t nvs,
+          const double *Cst, const double *Kst, const double *Hst)
+{
+    if (ncmd <= 0) return;
+    hipLaunchKernelGGL(k_gwarm, dim3(ncmd), dim3(256), 0, st, cmds, aln, V, vstride, nvs, Cst, Kst, Hst);
 }
 
 }  // namespace sub
