@@ -1786,22 +1786,28 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     lst.clear();
     sdca::SlotCmd *hc = G->h_cmds + s0;
     std::vector<int> col0(nmem, 0), ncol(nmem, 0);
-    int col = 0;
+    int col = 0, nwarm = 0;
+    auto staged = [&](const sdca::SlotCmd &k) {
+        return G->warm_rounds && k.mode == sdca::MODE_INIT && k.init_src != 0.f && k.item >= 0;
+    };
     for (int a = 0; a < nmem; ++a) {
         const sdca_ctx *A = c->members[a];
         col0[a] = col;
-        for (int i = s0; i < s1; ++i) {
-            if (cmds[i].mode == sdca::MODE_IDLE || cmds[i].aln != a) continue;
-            sdca::SlotCmd &k = hc[lst.size()];
-            k = cmds[i];
-            k.slot = i;
-            k.row = col;
-            k.q = A->q;
-            k.Kx = (int)A->Kx;
-            k.Kxp = (int)A->Kxp;
-            lst.push_back(i);
-            col += sdca::gram::rows(A->q);
-        }
+        // (warm rounds, one alignment) the staged starts first: the start kernel covers them alone
+        for (int pass = 0; pass < 2; ++pass)
+            for (int i = s0; i < s1; ++i) {
+                if (cmds[i].mode == sdca::MODE_IDLE || cmds[i].aln != a || staged(cmds[i]) != (pass == 0)) continue;
+                nwarm += pass == 0;
+                sdca::SlotCmd &k = hc[lst.size()];
+                k = cmds[i];
+                k.slot = i;
+                k.row = col;
+                k.q = A->q;
+                k.Kx = (int)A->Kx;
+                k.Kxp = (int)A->Kxp;
+                lst.push_back(i);
+                col += sdca::gram::rows(A->q);
+            }
         ncol[a] = col - col0[a];
     }
     const int nc = (int)lst.size();
@@ -1814,8 +1820,8 @@ int gram_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
     double *dot = G->dot_part + (size_t)s0 * G->nchd * sdca::ResLayout{m}.ndots(), *res = G->res + (size_t)s0 * W;
     HIP_TRY(hipMemcpyAsync(dc, hc, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(ev[0], st));
-    if (G->warm_rounds)  // staged starts of the round's new loci into their x_c
-        sdca::sub::warm(st, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, G->Cst, G->Kst, G->Wst + G->wfld);
+    if (nwarm > 0)  // staged starts of the round's new loci into their x_c
+        sdca::sub::warm(st, dc, nwarm, G->d_galn, G->V, G->vstride, G->nvs, G->Cst, G->Kst, G->Wst + G->wfld);
     sdca::gram::step(st, m, dc, nc, G->d_galn, G->V, G->vstride, G->nvs, Gp, G->ldG, upd, sm, G->nchunk, G->nchq,
                      ev[6], ev[7]);
     HIP_TRY(hipGetLastError());
@@ -2323,17 +2329,20 @@ int sub_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds)
     std::vector<int> &lst = G->cmd_slot[g];
     lst.clear();
     sdca::SlotCmd *hc = G->h_cmds + s0;
-    int col = 0;
-    for (int i = s0; i < s1; ++i) {
-        if (cmds[i].mode == sdca::MODE_IDLE) continue;
-        sdca::SlotCmd &k = hc[lst.size()];
-        k = cmds[i];
-        k.slot = i;
-        k.row = col;
-        k.q = A->q;
-        lst.push_back(i);
-        col += sdca::gram::rows(A->q);
-    }
+    int col = 0, ninit = 0;
+    // INIT commands first: the per-slot constants are formed for them alone
+    for (int pass = 0; pass < 2; ++pass)
+        for (int i = s0; i < s1; ++i) {
+            if (cmds[i].mode == sdca::MODE_IDLE || (cmds[i].mode == sdca::MODE_INIT) != (pass == 0)) continue;
+            sdca::SlotCmd &k = hc[lst.size()];
+            k = cmds[i];
+            k.slot = i;
+            k.row = col;
+            k.q = A->q;
+            lst.push_back(i);
+            col += sdca::gram::rows(A->q);
+            ninit += pass == 0;
+        }
     const int nc = (int)lst.size();
     if (nc == 0) return SDCA_OK;
     hipStream_t st = c->gstream[g];
@@ -2346,7 +2355,7 @@ int sub_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds)
     double *dot = G->dot_part + (size_t)s0 * G->nchd * sdca::ResLayout{m}.ndots(), *res = G->res + (size_t)s0 * W;
     HIP_TRY(hipMemcpyAsync(dc, hc, nc * sizeof(sdca::SlotCmd), hipMemcpyHostToDevice, st));
     HIP_TRY(hipEventRecord(ev[4], st));
-    sdca::sub::init(st, dc, nc, G->d_galn, A->d_subQ, A->d_subMinv, kp, G->aux, G->auxstride);
+    sdca::sub::init(st, dc, ninit, G->d_galn, A->d_subQ, A->d_subMinv, kp, G->aux, G->auxstride);
     sdca::gram::step_vectors(st, m, dc, nc, G->d_galn_r, G->Vr, G->vstride_r, G->nvs_r, upd, G->nchq_r);
     sdca::sub::pack(st, dc, nc, G->d_galn, G->Vr, G->vstride_r, G->nvs_r, kp, G->aux, G->auxstride, Ap, Wc);
     sdca::gram::gemm_splitk(st, A->d_subKQ, kp, Ap, kp, Ep, G->ldG, A->Bp, col, kp, kp, 0);

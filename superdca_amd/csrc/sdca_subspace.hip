@@ -213,6 +213,8 @@ __global__ __launch_bounds__(256) void k_ssoftmax(const SlotCmd *__restrict__ cm
     const int j = blockIdx.y;
     const SlotCmd &c = cmds[j];
     __shared__ double sW[4][kSmS];
+    __shared__ double sGb[256][8];
+    __shared__ int sYb[256];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     double *sp = sm_part + ((int64_t)j * nchunk + blockIdx.x) * kSmS;
     if (c.mode == MODE_IDLE) {
@@ -267,14 +269,19 @@ __global__ __launch_bounds__(256) void k_ssoftmax(const SlotCmd *__restrict__ cm
         for (int k = 0; k < 8; ++k)
             if (k < R) Gp[(int64_t)(c.row + k) * ldG + b] = 0.0;  // pad sequences: zero
     }
+    // T(s, a) = sum over the block's sequences with x_br = a of G_b(s): the block's beliefs
+    // through LDS, then lane 8 s + a of wave w sums its 64 sequences in order; the data term
 #pragma unroll
-    for (int s = 0; s < 8; ++s)
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            if (s >= q || u >= q) continue;
-            const double v = wsum(y == u ? G[s] : 0.0);
-            if (lane == 0) sW[wave][s * 8 + u] = v;
-        }
+    for (int s = 0; s < 8; ++s) sGb[tid][s] = G[s];
+    sYb[tid] = y;
+    __syncthreads();
+    {
+        const int s = lane >> 3, u = lane & 7, b0 = wave * 64;
+        double t = 0.0;
+#pragma unroll 8
+        for (int i = 0; i < 64; ++i) t += sYb[b0 + i] == u ? sGb[b0 + i][s] : 0.0;
+        sW[wave][lane] = t;
+    }
     fb = wsum(fb);
     if (lane == 0) sW[wave][64] = fb;
     __syncthreads();
