@@ -83,8 +83,8 @@ def run_steps(ctx, order, pos, per_step, steps, label, solve, fvals=None, subs=N
 # device copy (fetch x2.000, write x1.000); the profiles hold the commands and the
 # per-dispatch tables.  k_gstep covers the step pass (k_gstep + k_gsoftmax).
 HBM_PER_ALGORITHMIC_BYTE = {"k_update": 13.34 / 13.10, "k_dots": 10.51 / 10.50,
-                            "k_gstep": 10.728 / 10.418, "k_gdots": 5.674 / 5.543}
-HBM_BYTES_PER_FLOP = {"k_dgemm": 1.8777e9 / 0.09641e12}
+                            "k_gstep": 12.618 / 12.253, "k_gdots": 6.689 / 6.523}
+HBM_BYTES_PER_FLOP = {"k_dgemm": 1.757e9 / 0.1134e12}
 
 
 def kernel_groups(tm, gram, split):
