@@ -1994,6 +1994,9 @@ int gram_flush(sdca_ctx *c, bool force, float *score_rows, const int64_t *row_of
     hipStream_t st = c->fstream;
     for (int h = 0; h < G->ngroups; ++h) HIP_TRY(hipStreamWaitEvent(st, G->stage_ev[h], 0));
     const bool async = !x_out && (int)G->pend.size() <= G->nhrows;
+    // without x_out the contraction runs on the R coordinate rows of each locus (k_score
+    // takes each block back to states); x_out needs the q state rows
+    const int coords = x_out ? 0 : 1;
     const int nmem = (int)c->members.size();
     const int gm = c->opts.gemm_gm_grad > 0 ? c->opts.gemm_gm_grad : 2;
     const int tn = c->opts.gemm_tn_grad > 0 ? c->opts.gemm_tn_grad : 256;
@@ -2043,12 +2046,13 @@ int gram_flush(sdca_ctx *c, bool force, float *score_rows, const int64_t *row_of
                 k.Kx = (int)A->Kx;
                 k.Kxp = (int)A->Kxp;
                 k.gt = 0;
-                row += A->q;
+                row += coords ? sdca::gram::rows(A->q) : A->q;
             }
             const int rend = (int)round_up(row, sdca::kTile);
             parts.row[1] = parts.row[2] = rend;
             parts.cmd[1] = parts.cmd[2] = parts.cmd[3] = v;
             groups.push_back(make_group(c, a, parts, rend, false, 1, 0, gm, tn));
+            if (coords) groups.back().q = sdca::gram::rows(A->q);  // rows per item
             row = rend;
             u = v;
         }
@@ -2067,11 +2071,12 @@ int gram_flush(sdca_ctx *c, bool force, float *score_rows, const int64_t *row_of
         const int64_t hrow0 = async ? (int64_t)u0 : 0;
         float *drows = G->d_rows + (size_t)hrow0 * G->gL;
         sdca::gram::split(st, G->d_moff, G->d_mrow, G->d_maln, nb, G->nchunk, G->d_galn, G->Pend, G->gBp, nullptr,
-                          G->Mhi, G->Mlo, G->Ml3, G->mhpart);
+                          G->Mhi, G->Mlo, G->Ml3, G->mhpart, coords);
         sdca::kern::gemm_materialize(st, SDCA_PREC_SPLIT_BF16, G->d_mgroups, (int)groups.size(), grid, tn, G->M32,
                                      G->Mhi, G->Mlo, G->Ml3, G->gBp, G->d_aln, G->d_mcmds, G->Mout, G->n_item,
                                      G->mhpart);
-        sdca::kern::score(st, G->Mout, G->d_soff, G->d_sld, G->d_sloc, G->d_maln, nb, G->gL, G->gL, G->d_aln, drows);
+        sdca::kern::score(st, G->Mout, G->d_soff, G->d_sld, G->d_sloc, G->d_maln, nb, G->gL, G->gL, G->d_aln, drows,
+                          coords);
         HIP_TRY(hipGetLastError());
         if (async) {
             for (int v = 0; v < nb; ++v) G->fout.push_back({P[order[v]].item, P[order[v]].aln, hrow0 + v});
@@ -3170,7 +3175,7 @@ int sdca_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
                 HIP_TRY(hipMemcpyAsync(d_ld, h_ld, nf * 4, hipMemcpyHostToDevice, st));
                 HIP_TRY(hipMemcpyAsync(d_al, h_al, nf * 4, hipMemcpyHostToDevice, st));
                 HIP_TRY(hipEventRecord(ev[6], st));
-                sdca::kern::score(st, c->d_Xtr, d_off, d_ld, d_sl, d_al, nf, c->ws_L, c->ws_L, c->d_aln, d_rows);
+                sdca::kern::score(st, c->d_Xtr, d_off, d_ld, d_sl, d_al, nf, c->ws_L, c->ws_L, c->d_aln, d_rows, 0);
                 HIP_TRY(hipGetLastError());
                 HIP_TRY(hipEventRecord(ev[7], st));
                 if (score_rows) {

@@ -901,14 +901,14 @@ __global__ __launch_bounds__(256) void k_greduce(int m, const double *__restrict
 }
 
 // Coefficients of finished loci as the A operand of the coupling contraction J = X_r^T c:
-// item u's q state rows (from its R coordinate rows) from packed row rows[u] (FP32, or
-// three BF16 parts hi + mid + lo), and its fields into hpart[u][0..8) (the contraction's
-// field column).
+// item u's q state rows (from its R coordinate rows), or with coords its R coordinate rows
+// as they are, from packed row rows[u] (FP32, or three BF16 parts hi + mid + lo), and its
+// fields into hpart[u][0..8) (the contraction's field column).
 __global__ __launch_bounds__(256) void k_gsplit(const double *__restrict__ V, const int64_t *__restrict__ src_off,
                                                 const int32_t *__restrict__ rows, const int32_t *__restrict__ alns,
                                                 const GramAln *__restrict__ aln, int64_t ldm, float *__restrict__ G32,
                                                 uint16_t *__restrict__ Ghi, uint16_t *__restrict__ Glo,
-                                                uint16_t *__restrict__ Gl3, double *__restrict__ hpart)
+                                                uint16_t *__restrict__ Gl3, double *__restrict__ hpart, int coords)
 {
     const int u = blockIdx.y;
     const GramAln &A = aln[alns[u]];
@@ -919,10 +919,16 @@ __global__ __launch_bounds__(256) void k_gsplit(const double *__restrict__ V, co
         double w[8], cs[8];  // the coordinates, then the coefficients per state
 #pragma unroll
         for (int k = 0; k < 8; ++k) w[k] = k < R ? src[(int64_t)k * Bp + b] : 0.0;
-        gram::from_contrast(w, q, cs);
+        if (coords) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) cs[k] = w[k];
+        } else {
+            gram::from_contrast(w, q, cs);
+        }
+        const int nrow = coords ? R : q;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            if (s >= q) continue;
+            if (s >= nrow) continue;
             const double v = cs[s];
             const int64_t o = (int64_t)(rows[u] + s) * ldm + b;
             const float f = (float)v;
@@ -1064,10 +1070,10 @@ void reduce(hipStream_t st, int m, int ncmd, const double *upd_part, const doubl
 
 void split(hipStream_t st, const int64_t *src_off, const int32_t *rows, const int32_t *alns, int n, int nchunk,
            const GramAln *aln, const double *V, int64_t ldm, float *G32, uint16_t *Ghi, uint16_t *Glo,
-           uint16_t *Gl3, double *hpart)
+           uint16_t *Gl3, double *hpart, int coords)
 {
     hipLaunchKernelGGL(k_gsplit, dim3(nchunk, n), dim3(256), 0, st, V, src_off, rows, alns, aln, ldm, G32, Ghi, Glo,
-                       Gl3, hpart);
+                       Gl3, hpart, coords);
 }
 
 }  // namespace gram

@@ -246,7 +246,7 @@ void reduce_results(hipStream_t st, int m, int ncmd, const double *upd_part, con
 // (rows src_ld[k] floats apart), locus loci[k] of alignment alns[k], and writes its row of
 // that alignment's L entries at rows + k * ldrow.
 void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *src_ld, const int32_t *loci,
-           const int32_t *alns, int nscore, int Lmax, int64_t ldrow, const AlnDev *d_aln, float *rows);
+           const int32_t *alns, int nscore, int Lmax, int64_t ldrow, const AlnDev *d_aln, float *rows, int coords);
 // Couplings J = X_r^T c of finished Gram-formulation loci (the gradient contraction
 // without its L2 term): item u's q x Kxp device vector at out + u * n_item, its field
 // column from hpart[u][0..8).
@@ -418,7 +418,7 @@ void reduce(hipStream_t st, int m, int ncmd, const double *upd_part, const doubl
 // Coefficients of n finished loci (V + src_off[u]) as contraction operands (rows[u]).
 void split(hipStream_t st, const int64_t *src_off, const int32_t *rows, const int32_t *alns, int n, int nchunk,
            const GramAln *aln, const double *V, int64_t ldm, float *G32, uint16_t *Ghi, uint16_t *Glo,
-           uint16_t *Gl3, double *hpart);
+           uint16_t *Gl3, double *hpart, int coords);
 }  // namespace gram
 
 // Workspace of Gram-formulation solves (sdca_api.cpp), sized for `slots` loci in flight,

@@ -974,14 +974,15 @@ __global__ void k_reduce(int m, const double *__restrict__ upd_part, const doubl
 // ---------------------------------------------------------------- scoring
 
 // Score row of a solved locus r (plmDCA.hpp:378-392): for n != r the block
-// M[a][s] = J_r[n](a,s) (zero for states a not observed at n), the gauge
+// M[a][s] = J_r[n](a,s) (zero for states a not observed at n; with coords, the solution
+// holds the q - 1 zero-sum coordinates of J_r[n](a, .) per row, gram::to_contrast), the gauge
 // G[i][k] = ((M[i][k] - rowmean[k]) - colmean[i]) + mean(rowmean) of
 // plmDCA_utility.hpp:57-83 (on M^T for n < r), and the Frobenius norm over
 // i, k != gap (Matrix_math.hpp:134-178), accumulated in FP64 and stored as float.
 __global__ __launch_bounds__(256) void k_score(const float *__restrict__ xsrc, const int64_t *__restrict__ src_off,
                                                const int32_t *__restrict__ src_ld, const int32_t *__restrict__ loci,
                                                const int32_t *__restrict__ alns, int64_t ldrow,
-                                               const AlnDev *__restrict__ aln, float *__restrict__ rows)
+                                               const AlnDev *__restrict__ aln, float *__restrict__ rows, int coords)
 {
     const int k = blockIdx.y;
     const int n = blockIdx.x * 256 + threadIdx.x;
@@ -1001,10 +1002,20 @@ __global__ __launch_bounds__(256) void k_score(const float *__restrict__ xsrc, c
     const bool tr = n < r;
     for (int a = 0; a < q; ++a) {
         const int col = state_col[(int64_t)n * q + a];
+        double js[8];  // J_r[n](a, s) over s
+        if (coords) {
+            double w[8];
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk)
+                w[kk] = (col >= 0 && kk < gram::rows(q)) ? (double)x[(int64_t)kk * Kxp + col] : 0.0;
+            gram::from_contrast(w, q, js);
+        } else {
+#pragma unroll
+            for (int s = 0; s < 8; ++s) js[s] = (col >= 0 && s < q) ? (double)x[(int64_t)s * Kxp + col] : 0.0;
+        }
         for (int s = 0; s < q; ++s) {
-            const double v = col >= 0 ? (double)x[(int64_t)s * Kxp + col] : 0.0;
-            if (tr) A[s][a] = v;
-            else A[a][s] = v;
+            if (tr) A[s][a] = js[s];
+            else A[a][s] = js[s];
         }
     }
     double m1[8], m2[8];
@@ -1168,10 +1179,10 @@ void reduce_results(hipStream_t st, int m, int ncmd, const double *upd_part, con
 }
 
 void score(hipStream_t st, const float *xsrc, const int64_t *src_off, const int32_t *src_ld, const int32_t *loci,
-           const int32_t *alns, int nscore, int Lmax, int64_t ldrow, const AlnDev *d_aln, float *rows)
+           const int32_t *alns, int nscore, int Lmax, int64_t ldrow, const AlnDev *d_aln, float *rows, int coords)
 {
     hipLaunchKernelGGL(k_score, dim3(cdiv(Lmax, 256), nscore), dim3(256), 0, st, xsrc, src_off, src_ld, loci, alns,
-                       ldrow, d_aln, rows);
+                       ldrow, d_aln, rows, coords);
 }
 
 }  // namespace kern
