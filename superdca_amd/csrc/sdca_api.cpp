@@ -2239,17 +2239,6 @@ int ensure_subspace(sdca_ctx *A, hipStream_t st)
             Q[(size_t)b * kp + i] = Qt[(size_t)i * Bp + b];
             KQ[(size_t)b * kp + i] = KQt[(size_t)i * Bp + b];
         }
-    // the gradient product's operand: rows [0, kp) (K Q)^T, rows [kp, 2 kp) (K Q M^-1)^T =
-    // M^-1 (K Q)^T (M^-1 symmetric)
-    KQt.resize((size_t)2 * kp * Bp, 0.0);
-    for (int i = 0; i < kp; ++i) {
-        double *dst = KQt.data() + (size_t)(kp + i) * Bp;
-        for (int j = 0; j < kp; ++j) {
-            const double mij = Mi[(size_t)i * kp + j];
-            const double *src = KQt.data() + (size_t)j * Bp;
-            for (int b = 0; b < B; ++b) dst[b] += mij * src[b];
-        }
-    }
     dfree(A->d_subQ);
     dfree(A->d_subQt);
     dfree(A->d_subKQ);
@@ -2286,7 +2275,7 @@ int ensure_sub_ws(sdca_ctx *c, int32_t K)
         const int npad = G->gs * G->ngroups;
         if ((rc = dalloc(&G->Vr, (size_t)G->slots * G->vstride_r)) || (rc = dalloc(&G->aux, (size_t)G->slots * G->auxstride)) ||
             (rc = dalloc(&G->Ap, (size_t)ncols * kp)) ||
-            (rc = dalloc(&G->Zr, (size_t)ncols * 2 * kp * ((G->gBp + sdca::kSubKc - 1) / sdca::kSubKc))) ||
+            (rc = dalloc(&G->Zr, (size_t)ncols * kp * std::max<int64_t>(2, (G->gBp + sdca::kSubKc - 1) / sdca::kSubKc))) ||
             (rc = dalloc(&G->Wc, (size_t)npad * 64)) || (rc = dalloc(&G->d_galn_r, 1)))
             return rc;
     }
@@ -2354,7 +2343,7 @@ int sub_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds)
     hipEvent_t *ev = G->ev[g];
     sdca::SlotCmd *dc = G->d_cmds + s0;
     double *Gp = G->Gp + (size_t)g * G->colcap * G->ldG, *Ep = G->Zp + (size_t)g * G->colcap * G->ldG;
-    double *Ap = G->Ap + (size_t)g * G->colcap * kp, *Zr = G->Zr + (size_t)g * G->colcap * 2 * kp;
+    double *Ap = G->Ap + (size_t)g * G->colcap * kp, *Zr = G->Zr + (size_t)g * G->colcap * kp;
     double *Wc = G->Wc + (size_t)s0 * 64;
     double *upd = G->upd_part + (size_t)s0 * (G->nchq + 1) * 3, *sm = G->sm_part + (size_t)s0 * G->nchunk * 65;
     double *dot = G->dot_part + (size_t)s0 * G->nchd * sdca::ResLayout{m}.ndots(), *res = G->res + (size_t)s0 * W;
@@ -2366,13 +2355,13 @@ int sub_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds)
     sdca::gram::gemm_splitk(st, A->d_subKQ, kp, Ap, kp, Ep, G->ldG, A->Bp, col, kp, kp, 0);
     sdca::sub::softmax(st, dc, nc, G->d_galn, G->Vr, G->vstride_r, G->nvs_r, kp, Ep, Wc, Gp, G->ldG, sm, G->nchunk);
     // split-K over the sequences: slice z of the product at Zs + z * zs (Zs after the slot
-    // groups' first slices), summed by the gradient kernel into Zr
+    // groups' first slices), summed by the gradient kernels into Zr
     const int nz = (A->Bp + sdca::kSubKc - 1) / sdca::kSubKc;
-    const int64_t zs = G->colcap * G->ngroups * (int64_t)(2 * kp);
-    sdca::gram::gemm_splitk(st, A->d_subKQt, A->Bp, Gp, G->ldG, Zr, 2 * kp, 2 * kp, col, A->Bp, sdca::kSubKc, zs);
+    const int64_t zs = G->colcap * G->ngroups * (int64_t)kp;
+    sdca::gram::gemm_splitk(st, A->d_subKQt, A->Bp, Gp, G->ldG, Zr, kp, kp, col, A->Bp, sdca::kSubKc, zs);
     sdca::sub::grad(st, dc, nc, G->d_galn, G->Vr, G->vstride_r, G->nvs_r, kp, G->aux, G->auxstride, sm, G->nchunk, Zr,
-                    nz, zs, 2.0 * A->lambda_J);
-    sdca::gram::dots(st, m, dc, nc, G->d_galn_r, G->Vr, G->vstride_r, G->nvs_r, Zr, 2 * kp, sm, G->nchunk, dot,
+                    nz, zs, A->d_subMinv, col, 2.0 * A->lambda_J);
+    sdca::gram::dots(st, m, dc, nc, G->d_galn_r, G->Vr, G->vstride_r, G->nvs_r, Zr, kp, sm, G->nchunk, dot,
                      G->nchd_r, 1);
     sdca::gram::reduce(st, m, nc, upd, dot, sm, G->nchunk, G->nchq_r, G->nchd_r, res);
     HIP_TRY(hipGetLastError());

@@ -314,9 +314,11 @@ void pack(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, con
 void softmax(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, const double *V, int64_t vstride,
              int64_t nvs, int kp, const double *Ep, const double *Wc, double *Gp, int64_t ldG, double *sm_part,
              int nchunk);
+// The restricted gradients from the nz split-K slices of (K Q)^T G at Z (row stride kp, zstride
+// apart; slice 0 receives b, slice 1 u = M^-1 b over the round's ncol columns).
 void grad(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride, int64_t nvs,
           int kp, const double *aux, int64_t auxstride, const double *sm_part, int nchunk, double *Z, int nz,
-          int64_t zstride, double lamJ2);
+          int64_t zstride, const double *Minv, int ncol, double lamJ2);
 // The start products of every staged restricted optimum: with a* of item i at rows
 // [i R, (i + 1) R) of Wst (kp each), c = Q a* into Cst and K Q a* into Kst (R Bp per item,
 // FP64 MFMA; nrows = K R rows).
@@ -638,7 +640,7 @@ struct sdca_ctx {
     double *d_gram = nullptr;
     int gram_split = -1;
     // the dominant eigen-subspace of d_gram (subspace warm start): orthonormal basis Q
-    // (Bp x kSubRank and its transpose), K Q, [K Q | K Q M^-1]^T (2 kSubRank x Bp) and
+    // (Bp x kSubRank and its transpose), K Q and its transpose (kSubRank x Bp) and
     // M^-1 = (Q^T K Q)^-1; sub_rank 0 until built
     int sub_rank = 0;
     double *d_subQ = nullptr, *d_subQt = nullptr, *d_subKQ = nullptr, *d_subKQt = nullptr, *d_subMinv = nullptr;

@@ -292,27 +292,17 @@ __global__ __launch_bounds__(256) void k_ssoftmax(const SlotCmd *__restrict__ cm
     }
 }
 
-// The restricted gradient of every command, one block of kp (<= 128) threads per command.
-// The split-K product gives, per packed row (row + s), the nz partials of
-// [(K Q)^T G_s | (K Q M^-1)^T G_s] (2 kp entries, zstride apart), so that
-//   b_s = sum_z Z_z[row + s][0, kp) - V_r^T T(s, .)        (written back into Z_0)
-//   u_s = sum_z Z_z[row + s][kp, 2kp) - R_r T(s, .)        = M^-1 b_s
-//   g_s = u_s + R_r Omega_r (V_r u_s) + 2 lambda_J a_t(s)   = M_r^-1 b_s + 2 lambda_J a_t(s)
-// into the coefficients of g[gt]; the products pass (k_gdots, reading b with row stride
-// 2 kp, image b + 2 lambda_J M_r a_t) completes it.
-__global__ __launch_bounds__(128) void k_sgrad(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
-                                               double *__restrict__ V, int64_t vstride, int64_t nvs, int kp,
-                                               const double *__restrict__ aux, int64_t auxstride,
-                                               const double *__restrict__ sm_part, int nchunk, double *__restrict__ Z,
-                                               int nz, int64_t zstride, double lamJ2)
+// The restricted gradient of every command, in two kernels of one block of kp (<= 128)
+// threads per command around the product u = M^-1 b.  The split-K product gives, per
+// packed row (row + k), the nz partials of (K Q)^T G_k (kp entries, zstride apart), so that
+//   k_sgradb:  b_k = sum_z Z_z[row + k] - V_r^T T^(k, .)          (written back into Z_0)
+//   (product)  u_k = M^-1 b_k                                      (into U)
+//   k_sgradg:  g_k = u_k + R_r Omega_r (V_r u_k) + 2 lambda_J a_t(k) = M_r^-1 b_k + 2 lambda_J a_t(k)
+// into the coordinates of g[gt], with T^ the coordinates of T over s; the products pass
+// (k_gdots, reading b with row stride kp, image b + 2 lambda_J M_r a_t) completes it.
+__device__ __forceinline__ void sgrad_tables(const SlotCmd &c, int j, int q, const double *sm_part, int nchunk,
+                                             double *sT, double *sTc)
 {
-    const int j = blockIdx.x;
-    const SlotCmd &c = cmds[j];
-    if (c.mode == MODE_IDLE || c.gt < 0 || c.gt > 1) return;
-    const int q = aln[c.aln].q, R = gram::rows(q);
-    __shared__ double sT[64], sTc[64];
-    __shared__ double su[8][128];
-    __shared__ double sv[8][8];
     const int tid = threadIdx.x;
     if (tid < 64) {
         const int s = tid >> 3, u = tid & 7;
@@ -331,38 +321,63 @@ __global__ __launch_bounds__(128) void k_sgrad(const SlotCmd *__restrict__ cmds,
         for (int k = 0; k < 8; ++k) sTc[k * 8 + tid] = w[k];
     }
     __syncthreads();
+}
+
+__global__ __launch_bounds__(128) void k_sgradb(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                                int kp, const double *__restrict__ aux, int64_t auxstride,
+                                                const double *__restrict__ sm_part, int nchunk,
+                                                double *__restrict__ Z, int nz, int64_t zstride)
+{
+    const int j = blockIdx.x;
+    const SlotCmd &c = cmds[j];
+    if (c.mode == MODE_IDLE || c.gt < 0 || c.gt > 1) return;
+    const int q = aln[c.aln].q, R = gram::rows(q);
+    __shared__ double sT[64], sTc[64];
+    sgrad_tables(c, j, q, sm_part, nchunk, sT, sTc);
+    const double *Vr = aux + (int64_t)c.slot * auxstride;
+    const int i = threadIdx.x;
+    if (i < kp)
+        for (int k = 0; k < R; ++k) {
+            double bv = 0.0;
+            for (int z = 0; z < nz; ++z) bv += Z[z * zstride + (int64_t)(c.row + k) * kp + i];
+            for (int a = 0; a < q; ++a) bv -= Vr[(int64_t)a * kp + i] * sTc[k * 8 + a];
+            Z[(int64_t)(c.row + k) * kp + i] = bv;
+        }
+}
+
+__global__ __launch_bounds__(128) void k_sgradg(const SlotCmd *__restrict__ cmds, const GramAln *__restrict__ aln,
+                                                double *__restrict__ V, int64_t vstride, int64_t nvs, int kp,
+                                                const double *__restrict__ aux, int64_t auxstride,
+                                                const double *__restrict__ U, double lamJ2)
+{
+    const int j = blockIdx.x;
+    const SlotCmd &c = cmds[j];
+    if (c.mode == MODE_IDLE || c.gt < 0 || c.gt > 1) return;
+    const int q = aln[c.aln].q, R = gram::rows(q);
+    __shared__ double su[8][128];
+    __shared__ double sv[8][8];
+    const int tid = threadIdx.x;
     const double *Vr = aux + (int64_t)c.slot * auxstride;
     const double *Rr = Vr + 8 * (int64_t)kp;
     const double *Om = Vr + 16 * (int64_t)kp;
     const int i = tid;
-    const int64_t ld = 2 * (int64_t)kp;
     double u[8];
 #pragma unroll
-    for (int s = 0; s < 8; ++s) u[s] = 0.0;
+    for (int k = 0; k < 8; ++k) u[k] = 0.0;
     if (i < kp)
-        for (int s = 0; s < R; ++s) {  // coordinate rows
-            double bv = 0.0, uv = 0.0;
-            for (int z = 0; z < nz; ++z) {
-                const double *zr = Z + z * zstride + (int64_t)(c.row + s) * ld;
-                bv += zr[i];
-                uv += zr[kp + i];
-            }
-            for (int a = 0; a < q; ++a) {
-                bv -= Vr[(int64_t)a * kp + i] * sTc[s * 8 + a];
-                uv -= Rr[(int64_t)i * 8 + a] * sTc[s * 8 + a];
-            }
-            Z[(int64_t)(c.row + s) * ld + i] = bv;
-            u[s] = uv;
-            su[s][i] = uv;
+        for (int k = 0; k < R; ++k) {
+            const double uv = U[(int64_t)(c.row + k) * kp + i];
+            u[k] = uv;
+            su[k][i] = uv;
         }
     __syncthreads();
-    // v[a][s] = V_r[a] . u_s
+    // v[a][k] = V_r[a] . u_k
     if (tid < 64) {
-        const int a = tid >> 3, s = tid & 7;
+        const int a = tid >> 3, k = tid & 7;
         double v = 0.0;
-        if (a < q && s < R)
-            for (int k = 0; k < kp; ++k) v += Vr[(int64_t)a * kp + k] * su[s][k];
-        sv[a][s] = v;
+        if (a < q && k < R)
+            for (int t = 0; t < kp; ++t) v += Vr[(int64_t)a * kp + t] * su[k][t];
+        sv[a][k] = v;
     }
     __syncthreads();
     if (i < kp) {
@@ -371,14 +386,14 @@ __global__ __launch_bounds__(128) void k_sgrad(const SlotCmd *__restrict__ cmds,
         double r[8];
 #pragma unroll
         for (int a = 0; a < 8; ++a) r[a] = a < q ? Rr[(int64_t)i * 8 + a] : 0.0;
-        for (int s = 0; s < R; ++s) {
-            double g = u[s];
+        for (int k = 0; k < R; ++k) {
+            double g = u[k];
             for (int a = 0; a < q; ++a) {
-                double w = 0.0;  // (Omega_r v)(a, s)
-                for (int a2 = 0; a2 < q; ++a2) w += Om[a * 8 + a2] * sv[a2][s];
+                double w = 0.0;  // (Omega_r v)(a, k)
+                for (int a2 = 0; a2 < q; ++a2) w += Om[a * 8 + a2] * sv[a2][k];
                 g += r[a] * w;
             }
-            gt[(int64_t)s * kp + i] = g + lamJ2 * xt[(int64_t)s * kp + i];
+            gt[(int64_t)k * kp + i] = g + lamJ2 * xt[(int64_t)k * kp + i];
         }
     }
 }
@@ -477,11 +492,16 @@ void softmax(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, 
 
 void grad(hipStream_t st, const SlotCmd *cmds, int ncmd, const GramAln *aln, double *V, int64_t vstride, int64_t nvs,
           int kp, const double *aux, int64_t auxstride, const double *sm_part, int nchunk, double *Z, int nz,
-          int64_t zstride, double lamJ2)
+          int64_t zstride, const double *Minv, int ncol, double lamJ2)
 {
     if (ncmd <= 0) return;
-    hipLaunchKernelGGL(k_sgrad, dim3(ncmd), dim3(128), 0, st, cmds, aln, V, vstride, nvs, kp, aux, auxstride, sm_part,
-                       nchunk, Z, nz, zstride, lamJ2);
+    hipLaunchKernelGGL(k_sgradb, dim3(ncmd), dim3(128), 0, st, cmds, aln, kp, aux, auxstride, sm_part, nchunk, Z, nz,
+                       zstride);
+    // U = M^-1 b into the second slice (read by k_sgradb already)
+    double *U = Z + zstride;
+    gram::gemm_splitk(st, Minv, kp, Z, kp, U, kp, kp, ncol, kp, kp, 0);
+    hipLaunchKernelGGL(k_sgradg, dim3(ncmd), dim3(128), 0, st, cmds, aln, V, vstride, nvs, kp, aux, auxstride, U,
+                       lamJ2);
 }
 
 void starts(hipStream_t st, const double *Q, const double *KQ, int kp, int Bp, const double *Wst, int64_t nrows,
