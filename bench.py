@@ -223,7 +223,7 @@ def main() -> int:
     ap.add_argument("--q", type=int, default=0,
                     help="state envelope (softmax width): default 5 ({A,C,G,T,-}) at C4, 4 (SuperDCA's) otherwise")
     ap.add_argument("--loci-per-step", type=int, default=0, help="default: 16384 at C4, 1024 otherwise")
-    ap.add_argument("--slots", type=int, default=0, help="headline loci in flight (default 4096 at C4, 512 otherwise)")
+    ap.add_argument("--slots", type=int, default=0, help="headline loci in flight (default 3072 at C4, 512 otherwise)")
     ap.add_argument("--history", type=int, default=0,
                     help="L-BFGS history (0: the formulation's default, 32 Gram and 10 primal)")
     ap.add_argument("--groups", type=int, default=0,
@@ -270,7 +270,7 @@ def main() -> int:
     wl = args.workload
     q = args.q or (5 if wl == "C4" else 4)
     per_step = args.loci_per_step or (16384 if wl == "C4" else 1024)
-    slots = args.slots or (4096 if wl == "C4" else 512)
+    slots = args.slots or (3072 if wl == "C4" else 512)
     jthr = args.j_threshold
 
     t = time.time()
@@ -339,7 +339,7 @@ def main() -> int:
     # ---- the same with one slot group: the kernels' rates with the GPU to themselves
     single = None
     if args.single_group_loci > 0:
-        with sd.Context(states, q=q, batch_loci=min(slots, args.single_group_loci), pipeline_groups=1,
+        with sd.Context(states, q=q, batch_loci=min(slots // 2, args.single_group_loci), pipeline_groups=1,
                         j_threshold=jthr, **gram) as c1:
             c1.set_weights(w)
             c1.solve(order[pos:pos + 8])
@@ -347,7 +347,7 @@ def main() -> int:
             v1, _, _, _, pos = run_steps(c1, order, pos, args.single_group_loci, 1, "gram-one-group", c1.solve)
             tm1 = c1.timing()
         ph1, ph1w = phase_fractions(tm1, True, True)
-        single = {"loci_s": round(v1, 3), "slots": min(slots, args.single_group_loci), "phases": ph1,
+        single = {"loci_s": round(v1, 3), "slots": min(slots // 2, args.single_group_loci), "phases": ph1,
                   "time_weighted_frac": ph1w, "device_ms": device_ms(tm1)}
 
     # ---- the Gram formulation at ||g||_2 < 1e-3 alone

@@ -48,7 +48,7 @@ typedef struct sdca_ctx sdca_ctx;
 typedef struct sdca_opts {
     int32_t device;          /* HIP device ordinal (default 0) */
     int32_t precision;       /* SDCA_PREC_* (default SDCA_PREC_FP32) */
-    int32_t batch_loci;      /* loci solved concurrently on the GPU (default 128; 0: 4096 for the
+    int32_t batch_loci;      /* loci solved concurrently on the GPU (default 128; 0: 3072 for the
                                 Gram formulation, 128 for the primal one) */
     int32_t history;         /* L-BFGS history m: 1, 3, 5, 8, 10 or 16; the Gram formulation also 24
                                 and 32.  0 (default): the formulation's own, 32 for Gram, 10 primal */

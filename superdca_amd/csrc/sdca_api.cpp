@@ -121,7 +121,7 @@ int history_for(const sdca_opts &o, bool gram) { return o.history > 0 ? o.histor
 // small -- 6.1 MB at B = 3000, q = 5 -- so 4096 of them; primal slots hold q x K_x vectors).
 int slots_for(const sdca_opts &o, bool gram, int K)
 {
-    const int b = o.batch_loci > 0 ? o.batch_loci : (gram ? 4096 : 128);
+    const int b = o.batch_loci > 0 ? o.batch_loci : (gram ? 3072 : 128);
     return std::min(b, K);
 }
 

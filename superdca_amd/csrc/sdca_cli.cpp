@@ -89,7 +89,7 @@ const char *kUsage =
     "  --output-min-score arg         Also write every score >= arg, largest first (.top).\n"
     "GPU:\n"
     "  --device arg (=0)  --history arg (=0: 32 Gram, 10 primal)  --max-iter arg (=2000)\n"
-    "  --batch-loci arg (=0)          Loci in flight (0: 4096 for the Gram formulation, 128 otherwise).\n"
+    "  --batch-loci arg (=0)          Loci in flight (0: 3072 for the Gram formulation, 128 otherwise).\n"
     "  --formulation auto|gram|primal (=auto)\n"
     "                                 gram: couplings in the span of the sequences, FP64, one B x B\n"
     "                                 GEMM per evaluation, stopping on 2-norms; primal: the q x K_x\n"
