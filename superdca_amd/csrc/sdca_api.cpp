@@ -1657,8 +1657,8 @@ int ensure_gram_ws(sdca_ctx *c, int nslots, int m)
         (rc = dalloc(&G->res, (size_t)npad * W)) || (rc = dalloc(&G->d_cmds, (size_t)npad)) ||
         (rc = dalloc(&G->d_galn, (size_t)nmem)) || (rc = dalloc(&G->d_aln, (size_t)nmem)))
         return rc;
-    // the coupling contraction of finished loci: three BF16 parts (hi + mid + lo, the FP32
-    // product) at either precision setting
+    // the coupling contraction of finished loci: two BF16 parts (hi + lo, 16 bits of each
+    // coefficient) at either precision setting
     const size_t mel = (size_t)G->mrows * gBp;
     if ((rc = dalloc(&G->Mhi, mel)) || (rc = dalloc(&G->Mlo, mel)) || (rc = dalloc(&G->Ml3, mel))) return rc;
     G->pcap = G->nmat + G->gs;
@@ -2049,8 +2049,10 @@ int gram_flush(sdca_ctx *c, bool force, float *score_rows, const int64_t *row_of
                 row += coords ? sdca::gram::rows(A->q) : A->q;
             }
             const int rend = (int)round_up(row, sdca::kTile);
-            parts.row[1] = parts.row[2] = rend;
-            parts.cmd[1] = parts.cmd[2] = parts.cmd[3] = v;
+            parts.row[1] = parts.row[0];  // two BF16 parts (hi + lo) for every row
+            parts.row[2] = rend;
+            parts.cmd[1] = parts.cmd[0];
+            parts.cmd[2] = parts.cmd[3] = v;
             groups.push_back(make_group(c, a, parts, rend, false, 1, 0, gm, tn));
             if (coords) groups.back().q = sdca::gram::rows(A->q);  // rows per item
             row = rend;
@@ -2071,7 +2073,7 @@ int gram_flush(sdca_ctx *c, bool force, float *score_rows, const int64_t *row_of
         const int64_t hrow0 = async ? (int64_t)u0 : 0;
         float *drows = G->d_rows + (size_t)hrow0 * G->gL;
         sdca::gram::split(st, G->d_moff, G->d_mrow, G->d_maln, nb, G->nchunk, G->d_galn, G->Pend, G->gBp, nullptr,
-                          G->Mhi, G->Mlo, G->Ml3, G->mhpart, coords);
+                          G->Mhi, G->Mlo, nullptr, G->mhpart, coords);
         sdca::kern::gemm_materialize(st, SDCA_PREC_SPLIT_BF16, G->d_mgroups, (int)groups.size(), grid, tn, G->M32,
                                      G->Mhi, G->Mlo, G->Ml3, G->gBp, G->d_aln, G->d_mcmds, G->Mout, G->n_item,
                                      G->mhpart);

@@ -902,8 +902,8 @@ __global__ __launch_bounds__(256) void k_greduce(int m, const double *__restrict
 
 // Coefficients of finished loci as the A operand of the coupling contraction J = X_r^T c:
 // item u's q state rows (from its R coordinate rows), or with coords its R coordinate rows
-// as they are, from packed row rows[u] (FP32, or three BF16 parts hi + mid + lo), and its
-// fields into hpart[u][0..8) (the contraction's field column).
+// as they are, from packed row rows[u] (FP32, or BF16 parts hi + mid, and lo unless Gl3 is
+// null), and its fields into hpart[u][0..8) (the contraction's field column).
 __global__ __launch_bounds__(256) void k_gsplit(const double *__restrict__ V, const int64_t *__restrict__ src_off,
                                                 const int32_t *__restrict__ rows, const int32_t *__restrict__ alns,
                                                 const GramAln *__restrict__ aln, int64_t ldm, float *__restrict__ G32,
@@ -940,7 +940,7 @@ __global__ __launch_bounds__(256) void k_gsplit(const double *__restrict__ V, co
                 const uint16_t l = f2bf(r1);
                 Ghi[o] = h;
                 Glo[o] = l;
-                Gl3[o] = f2bf(r1 - bf2f(l));
+                if (Gl3) Gl3[o] = f2bf(r1 - bf2f(l));
             }
         }
     }
