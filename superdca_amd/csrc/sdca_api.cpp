@@ -1658,7 +1658,7 @@ int ensure_gram_ws(sdca_ctx *c, int nslots, int m)
         (rc = dalloc(&G->d_galn, (size_t)nmem)) || (rc = dalloc(&G->d_aln, (size_t)nmem)))
         return rc;
     // the coupling contraction of finished loci: two BF16 parts (hi + lo, 16 bits of each
-    // coefficient) at either precision setting
+    // coordinate) for the scores, three (hi + mid + lo) for x_out, at either precision setting
     const size_t mel = (size_t)G->mrows * gBp;
     if ((rc = dalloc(&G->Mhi, mel)) || (rc = dalloc(&G->Mlo, mel)) || (rc = dalloc(&G->Ml3, mel))) return rc;
     G->pcap = G->nmat + G->gs;
@@ -1994,8 +1994,9 @@ int gram_flush(sdca_ctx *c, bool force, float *score_rows, const int64_t *row_of
     hipStream_t st = c->fstream;
     for (int h = 0; h < G->ngroups; ++h) HIP_TRY(hipStreamWaitEvent(st, G->stage_ev[h], 0));
     const bool async = !x_out && (int)G->pend.size() <= G->nhrows;
-    // without x_out the contraction runs on the R coordinate rows of each locus (k_score
-    // takes each block back to states); x_out needs the q state rows
+    // without x_out the contraction runs on the R coordinate rows of each locus in two BF16
+    // parts (k_score takes each block back to states); x_out needs the q state rows, formed
+    // from three parts (FP32 products: the parameters to about 1e-8 relative)
     const int coords = x_out ? 0 : 1;
     const int nmem = (int)c->members.size();
     const int gm = c->opts.gemm_gm_grad > 0 ? c->opts.gemm_gm_grad : 2;
@@ -2049,9 +2050,10 @@ int gram_flush(sdca_ctx *c, bool force, float *score_rows, const int64_t *row_of
                 row += coords ? sdca::gram::rows(A->q) : A->q;
             }
             const int rend = (int)round_up(row, sdca::kTile);
-            parts.row[1] = parts.row[0];  // two BF16 parts (hi + lo) for every row
+            // scores: two BF16 parts (hi + lo) for every row; x_out: three (FP32 products)
+            parts.row[1] = coords ? parts.row[0] : rend;
             parts.row[2] = rend;
-            parts.cmd[1] = parts.cmd[0];
+            parts.cmd[1] = coords ? parts.cmd[0] : v;
             parts.cmd[2] = parts.cmd[3] = v;
             groups.push_back(make_group(c, a, parts, rend, false, 1, 0, gm, tn));
             if (coords) groups.back().q = sdca::gram::rows(A->q);  // rows per item
@@ -2073,7 +2075,7 @@ int gram_flush(sdca_ctx *c, bool force, float *score_rows, const int64_t *row_of
         const int64_t hrow0 = async ? (int64_t)u0 : 0;
         float *drows = G->d_rows + (size_t)hrow0 * G->gL;
         sdca::gram::split(st, G->d_moff, G->d_mrow, G->d_maln, nb, G->nchunk, G->d_galn, G->Pend, G->gBp, nullptr,
-                          G->Mhi, G->Mlo, nullptr, G->mhpart, coords);
+                          G->Mhi, G->Mlo, coords ? nullptr : G->Ml3, G->mhpart, coords);
         sdca::kern::gemm_materialize(st, SDCA_PREC_SPLIT_BF16, G->d_mgroups, (int)groups.size(), grid, tn, G->M32,
                                      G->Mhi, G->Mlo, G->Ml3, G->gBp, G->d_aln, G->d_mcmds, G->Mout, G->n_item,
                                      G->mhpart);
