@@ -598,7 +598,7 @@ inline size_t pair_bytes(const sdca_ctx *c) { return c->opts.precision == SDCA_P
 // kernels compete for the same CUs, and at the C4 shape one group is faster.
 int choose_groups(const sdca_ctx *c, int nslots)
 {
-    if (c->opts.pipeline_groups != 2 || c->members.size() != 1) return 1;
+    if (c->opts.pipeline_groups < 2 || c->members.size() != 1) return 1;  // (4 means 2 here)
     const bool even = nslots % 2 == 0 && ((int64_t)(nslots / 2) * c->q) % sdca::kTile == 0;
     return even ? 2 : 1;
 }
@@ -1618,7 +1618,8 @@ int ensure_gram_ws(sdca_ctx *c, int nslots, int m)
     const int nmem = (int)c->members.size();
     // two slot groups (their rounds overlap: one group's GEMM with the other's vector passes
     // and host steps) unless pipeline_groups is 1 or the batch is small
-    const int ng = (c->opts.pipeline_groups == 1 || nslots < 64) ? 1 : 2;
+    const int pg = c->opts.pipeline_groups;
+    const int ng = (pg == 1 || nslots < 64) ? 1 : (pg == 4 && nslots >= 256 ? 4 : 2);
     sdca::GramWs *G = c->gram;
     if (G && G->slots == nslots && G->m == m && G->gq == gq && G->gBp == gBp && G->gL == gL && G->nmem == nmem &&
         G->n_item == nitem && G->ngroups == ng)
@@ -2717,8 +2718,8 @@ int sdca_ctx_create(sdca_ctx **out, const uint8_t *states, int32_t B, int32_t L,
     if (o.fine_factor < 0 || o.fine_factor > 1000000)
         return fail(SDCA_E_ARG, "fine_factor must be 0 (no fine level) or a factor >= 1 (got %d)", o.fine_factor);
     if (!(o.j_threshold >= 0.0)) return fail(SDCA_E_ARG, "j_threshold must be >= 0");
-    if (o.pipeline_groups < 0 || o.pipeline_groups > sdca::kMaxGroups)
-        return fail(SDCA_E_ARG, "pipeline_groups must be 0, 1 or 2");
+    if (o.pipeline_groups < 0 || o.pipeline_groups > sdca::kMaxGroups || o.pipeline_groups == 3)
+        return fail(SDCA_E_ARG, "pipeline_groups must be 0, 1, 2 or 4");
     if (o.precision != SDCA_PREC_FP32 && o.precision != SDCA_PREC_SPLIT_BF16)
         return fail(SDCA_E_ARG, "unknown precision %d", o.precision);
     if (o.formulation < SDCA_FORM_AUTO || o.formulation > SDCA_FORM_GRAM)

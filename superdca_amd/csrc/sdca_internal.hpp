@@ -26,7 +26,7 @@ namespace sdca {
 constexpr int kTile = 128;        // GEMM output tile rows (and narrow tile columns)
 constexpr int kTileWide = 256;    // wide GEMM tile columns; B and K_x are padded to this
 constexpr int kMaxHistory = 32;   // largest L-BFGS history supported (the primal kernels: 16)
-constexpr int kMaxGroups = 2;     // slot groups whose rounds are pipelined on separate streams
+constexpr int kMaxGroups = 4;     // slot groups whose rounds are pipelined on separate streams
 
 // BF16 <-> FP32 (round to nearest even; NaN stays NaN).
 __device__ __forceinline__ uint16_t f2bf(float x)
