@@ -76,6 +76,24 @@ def test_subspace_results_independent_of_locus_order(c2_q5):
         np.testing.assert_array_equal(s1[k][perm], s2[k])
 
 
+@pytest.mark.parametrize("groups", [1, 4])
+def test_slot_groups_give_identical_results(c2_q5, groups):
+    """One, two (the default) or four slot groups on as many streams: every locus' rounds
+    depend on its own slot only, so the results are bit-identical."""
+    st, w = c2_q5
+    loci = np.arange(0, st.shape[1], 17, dtype=np.int32)
+    out = []
+    for g in (2, groups):
+        with sd.Context(st, q=5, batch_loci=256, pipeline_groups=g, subspace_rank=128, **GRAM) as ctx:
+            ctx.set_weights(w)
+            rows, stats, _, rc = ctx.solve(loci)
+        assert rc == sd.SDCA_OK, rc
+        out.append((rows, stats))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for k in ("fval", "nfeval", "nfeval_sub", "iter"):
+        np.testing.assert_array_equal(out[0][1][k], out[1][1][k])
+
+
 def test_subspace_skipped_below_256_sequences(c1):
     """B = 200 < 2 x 128: the alignment solves from zero, bit-identical to the cold start."""
     st = c1["states"]
