@@ -96,7 +96,10 @@ typedef struct sdca_opts {
                                 once per alignment); the restricted solve costs two B x 128 x q
                                 products per evaluation instead of one B x B x q product.  The
                                 optimum is unchanged (strictly convex); the full solve's iterations
-                                drop.  0 (default): cold start from zero */
+                                drop.  Loci start from zero instead (nfeval_sub 0) when B < 256, when
+                                the counts' restricted matrix Q^T K Q is singular (too few loci for
+                                the rank) or when a call's start products would exceed 64 GiB.
+                                0 (default): cold start from zero */
     double subspace_threshold; /* the restricted solve stops at ||g||_2 < this in its own metric
                                 (default 3e-2), after at most max_iter iterations */
 } sdca_opts;

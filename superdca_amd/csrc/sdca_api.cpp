@@ -2164,6 +2164,7 @@ int ensure_subspace(sdca_ctx *A, hipStream_t st)
 {
     constexpr int kp = sdca::kSubRank;
     if (A->sub_rank == kp) return SDCA_OK;
+    if (A->sub_rank < 0) return SDCA_E_STATE;  // these counts had no basis: not tried again
     const int B = A->B, Bp = A->Bp;
     std::vector<double> Qt((size_t)kp * Bp, 0.0), KQt((size_t)kp * Bp, 0.0);
     std::mt19937_64 gen(0x5d5ca11);
@@ -2213,7 +2214,10 @@ int ensure_subspace(sdca_ctx *A, hipStream_t st)
     for (int j = 0; j < kp; ++j) {
         double d = M[(size_t)j * kp + j];
         for (int k = 0; k < j; ++k) d -= Lc[(size_t)j * kp + k] * Lc[(size_t)j * kp + k];
-        if (!(d > 0.0)) return fail(SDCA_E_STATE, "Q^T K Q is not positive definite");
+        if (!(d > 0.0)) {
+            A->sub_rank = -1;
+            return fail(SDCA_E_STATE, "Q^T K Q is not positive definite");
+        }
         Lc[(size_t)j * kp + j] = std::sqrt(d);
         for (int i = j + 1; i < kp; ++i) {
             double v = M[(size_t)i * kp + j];

@@ -641,7 +641,8 @@ struct sdca_ctx {
     int gram_split = -1;
     // the dominant eigen-subspace of d_gram (subspace warm start): orthonormal basis Q
     // (Bp x kSubRank and its transpose), K Q and its transpose (kSubRank x Bp) and
-    // M^-1 = (Q^T K Q)^-1; sub_rank 0 until built
+    // M^-1 = (Q^T K Q)^-1; sub_rank 0 until built, -1 when these counts have none (Q^T K Q
+    // singular: too few loci for the rank)
     int sub_rank = 0;
     double *d_subQ = nullptr, *d_subQt = nullptr, *d_subKQ = nullptr, *d_subKQt = nullptr, *d_subMinv = nullptr;
     double *d_w64 = nullptr;
