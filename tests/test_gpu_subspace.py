@@ -108,3 +108,26 @@ def test_subspace_skipped_below_256_sequences(c1):
     np.testing.assert_array_equal(out[0][0], out[1][0])
     np.testing.assert_array_equal(out[0][1]["fval"], out[1][1]["fval"])
     assert np.all(out[1][1]["nfeval_sub"] == 0)
+
+
+def test_subspace_skipped_when_its_basis_is_singular():
+    """B = 300 sequences but only 60 loci: the identical-locus counts have too few
+    directions for a 128-dimensional basis (Q^T K Q singular), so the loci start from zero,
+    bit-identical to the cold start, on this call and the next."""
+    d = synth.generate(B=300, N=60, clades=8, pairs=3, seed=12)
+    st, _ = sd.prepare_alignment(d["chars"])
+    loci = np.arange(0, st.shape[1], 7, dtype=np.int32)
+    out = []
+    for rank in (0, 128):
+        with sd.Context(st, batch_loci=32, subspace_rank=rank, **GRAM) as ctx:
+            w, _ = ctx.reweight(0.9)
+            ctx.set_weights(w)
+            for _ in range(2):
+                rows, stats, _, rc = ctx.solve(loci)
+                assert rc == sd.SDCA_OK, rc
+                out.append((rows, stats))
+    for rows, stats in out[1:]:
+        np.testing.assert_array_equal(out[0][0], rows)
+        np.testing.assert_array_equal(out[0][1]["fval"], stats["fval"])
+        assert np.all(stats["nfeval_sub"] == 0)
+
