@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <numeric>
 #include <chrono>
 #include <cmath>
@@ -1561,6 +1562,51 @@ private:
     bool stop_ = false;
 };
 
+// Host threads of a Gram solve: with four slot groups, two threads step two groups each
+// (one host thread does not step four groups as fast as the device runs their rounds);
+// otherwise the calling thread steps them all.  The first error stops the other thread.
+class HostThreads {
+public:
+    HostThreads(int ngroups, int device) : ng_(ngroups), nt_(ngroups == 4 ? 2 : 1), device_(device) {}
+
+    // fn(g0, g1, msg) steps groups [g0, g1) and returns an sdca status (msg: its error text).
+    void run(const std::function<int(int, int, std::string &)> &fn)
+    {
+        const int per = ng_ / nt_;
+        std::vector<int> rc(nt_, SDCA_OK);
+        std::vector<std::string> msg(nt_);
+        auto body = [&](int t) {
+            if (t > 0 && hipSetDevice(device_) != hipSuccess) {
+                rc[t] = SDCA_E_HIP;
+                msg[t] = "hipSetDevice failed on a host thread";
+            } else {
+                rc[t] = fn(t * per, (t + 1) * per, msg[t]);
+                if (rc[t] != SDCA_OK && msg[t].empty()) msg[t] = g_err;  // this thread's error text
+            }
+            if (rc[t] != SDCA_OK) stop_ = true;
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt_; ++t) th.emplace_back(body, t);
+        body(0);
+        for (std::thread &x : th) x.join();
+        for (int t = 0; t < nt_; ++t)
+            if (rc[t] != SDCA_OK) {
+                rc_ = rc[t];
+                msg_ = msg[t];
+                break;
+            }
+    }
+    bool stopped() const { return stop_.load(); }
+    int result() const { return rc_; }
+    const std::string &message() const { return msg_; }
+
+private:
+    int ng_, nt_, device_;
+    std::atomic<bool> stop_{false};
+    int rc_ = SDCA_OK;
+    std::string msg_;
+};
+
 // ------------------------------------------------------------ Gram formulation
 //
 // The solve in the representation of sdca_gram.hip: FP64 slot vectors of coefficients,
@@ -1863,6 +1909,7 @@ int gram_finish_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds
         o[R.gJinf()] = std::sqrt(std::max(0.0, o[R.gJinf()]));
     }
     hipEvent_t *ev = G->ev[g];
+    std::lock_guard<std::mutex> lk(G->mu);  // the timing is shared by the host threads
     auto &t = c->timing;
     const bool graphed = c->opts.graph_rounds && c->members.size() == 1 && !G->warm_rounds;
     if (graphed) {
@@ -1963,20 +2010,30 @@ int gram_stage(sdca_ctx *c, int g, const Solver &S, const std::vector<int> &fin,
 }
 
 // The score rows of the last flush into score_rows (or dropped, when score_rows is null),
-// once its batches have run.
-int gram_consume(sdca_ctx *c, float *score_rows, const int64_t *row_off)
+// once its batches have run; the copy in pieces of rows on the pool's threads (about 48 MB
+// per 128 loci at C4: one thread would hold the rounds up for milliseconds).
+int gram_consume(sdca_ctx *c, float *score_rows, const int64_t *row_off, SlotPool *pool)
 {
     sdca::GramWs *G = c->gram;
     if (!G->finflight) return SDCA_OK;
     G->finflight = false;
     HIP_TRY(hipStreamSynchronize(c->fstream));
     c->timing.score += elapsed(G->fev[0], G->fev[1]);
-    if (score_rows)
-        for (const sdca::GramWs::RowOut &o : G->fout) {
-            const int La = c->members[o.aln]->L;
+    if (score_rows) {
+        constexpr int kPieces = 8;
+        const std::vector<sdca::GramWs::RowOut> &F = G->fout;
+        auto piece = [&](int u) {
+            const sdca::GramWs::RowOut &o = F[u / kPieces];
+            const int La = c->members[o.aln]->L, p = u % kPieces;
             const int64_t off = row_off ? row_off[o.item] : (int64_t)o.item * La;
-            std::memcpy(score_rows + off, G->h_rows + (size_t)o.hrow * G->gL, (size_t)La * 4);
-        }
+            const int64_t a = (int64_t)La * p / kPieces, b = (int64_t)La * (p + 1) / kPieces;
+            std::memcpy(score_rows + off + a, G->h_rows + (size_t)o.hrow * G->gL + a, (size_t)(b - a) * 4);
+        };
+        const int n = (int)F.size() * kPieces;
+        if (pool) pool->run(0, n, piece);
+        else
+            for (int u = 0; u < n; ++u) piece(u);
+    }
     G->fout.clear();
     return SDCA_OK;
 }
@@ -1986,12 +2043,12 @@ int gram_consume(sdca_ctx *c, float *score_rows, const int64_t *row_off)
 // gathered, or when force is set.  The rounds go on meanwhile: the rows are copied out at
 // the next flush or by gram_consume at the end of the solve.  With x_out, or more pending
 // loci than the row buffers hold, each batch is waited for and copied out at once.
-int gram_flush(sdca_ctx *c, bool force, float *score_rows, const int64_t *row_off, double *x_out)
+int gram_flush(sdca_ctx *c, bool force, float *score_rows, const int64_t *row_off, double *x_out, SlotPool *pool)
 {
     sdca::GramWs *G = c->gram;
     if (G->pend.empty() || (!force && (int)G->pend.size() < G->nmat)) return SDCA_OK;
     int rc;
-    if ((rc = gram_consume(c, score_rows, row_off))) return rc;  // its row buffers are reused
+    if ((rc = gram_consume(c, score_rows, row_off, pool))) return rc;  // its row buffers are reused
     hipStream_t st = c->fstream;
     for (int h = 0; h < G->ngroups; ++h) HIP_TRY(hipStreamWaitEvent(st, G->stage_ev[h], 0));
     const bool async = !x_out && (int)G->pend.size() <= G->nhrows;
@@ -2396,6 +2453,7 @@ int sub_finish_round(sdca_ctx *c, int g, std::vector<double> &res)
         o[R.gJinf()] = std::sqrt(std::max(0.0, o[R.gJinf()]));
     }
     const sdca_ctx *A = c->members[0];
+    std::lock_guard<std::mutex> lk(G->mu);
     c->timing.sub += elapsed(G->ev[g][4], G->ev[g][5]);
     c->timing.sub_evals += 1;
     c->timing.flops_sub += 4.0 * A->B * (double)sdca::kSubRank * sdca::gram::rows(A->q) * (double)lst.size();
@@ -2417,7 +2475,6 @@ int sub_solve(sdca_ctx *c, const int32_t *loci, int32_t K, std::vector<int> &nev
     const int R1 = sdca::gram::rows(c->members[0]->q);  // coordinate rows of a restricted vector half
     std::vector<double> res((size_t)nslots * R.width(), 0.0);
     std::vector<char> done_flag(nslots, 0);
-    SlotPool pool(SlotPool::default_threads());
     nev.assign(K, 0);
     ok.assign(K, 0);
     int next = 0, done = 0;
@@ -2437,65 +2494,76 @@ int sub_solve(sdca_ctx *c, const int32_t *loci, int32_t K, std::vector<int> &nev
     };
     int rc;
     for (int i = 0; i < nslots; ++i) fill(i);
-    bool inflight[sdca::kMaxGroups] = {};
-    for (int g = 0; g < ng; ++g) {
-        if (!group_active(g)) continue;
-        S.mark_live(g * gs, group_end(g));
-        if ((rc = sub_launch_round(c, g, S.cmds()))) return rc;
-        inflight[g] = true;
-    }
-    std::vector<int> finished;
-    while (done < K) {
-        bool any = false;
-        for (int g = 0; g < ng; ++g) {
-            if (!inflight[g]) continue;
-            any = true;
-            inflight[g] = false;
-            if ((rc = sub_finish_round(c, g, res))) return rc;
-            finished.clear();
-            const auto th = std::chrono::steady_clock::now();
-            pool.run(g * gs, group_end(g), [&](int i) {
-                done_flag[i] = S.slot(i).phase != PH_IDLE && S.process(i, res.data() + (size_t)i * R.width());
-            });
-            for (int i = g * gs; i < group_end(g); ++i)
-                if (done_flag[i]) finished.push_back(i);
-            c->timing.host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th).count();
-            hipStream_t st = c->gstream[g];
-            // restricted optima (coefficients, then the fields) into the staged starts
-            int64_t *meta = G->h_stage + (size_t)g * G->gs * 5;
-            std::vector<int> keep;
-            for (int i : finished)
-                if (S.slot(i).status != SDCA_ST_NONFINITE) keep.push_back(i);
-            const int nk = (int)keep.size();
-            for (int k = 0; k < nk; ++k) {
-                const Slot &sl = S.slot(keep[k]);
-                meta[k] = (int64_t)keep[k] * G->vstride_r +
-                          (sl.from_xt ? sdca::gram::kVecXt : sdca::gram::kVecXc) * G->nvs_r;
-                meta[nk + k] = (int64_t)sl.item * R1 * kp;
-                meta[2 * nk + k] = (int64_t)R1 * kp;
-                meta[3 * nk + k] = 2 * (int64_t)R1 * kp;
-                meta[4 * nk + k] = G->wfld + (int64_t)sl.item * 8 - (int64_t)sl.item * R1 * kp;
-                ok[sl.item] = 1;
-            }
-            if (nk > 0) {
-                int64_t *dmeta = G->d_stage + (size_t)g * G->gs * 5;
-                HIP_TRY(hipMemcpyAsync(dmeta, meta, (size_t)nk * 5 * sizeof(int64_t), hipMemcpyHostToDevice, st));
-                sdca::gram::gather(st, G->Vr, G->Wst, dmeta, nk, (int64_t)R1 * kp, 8);
-                HIP_TRY(hipGetLastError());
-            }
-            for (int i : finished) {
-                nev[S.slot(i).item] = S.slot(i).nfeval;
-                ++done;
-                fill(i);
-            }
-            if (group_active(g)) {
-                S.mark_live(g * gs, group_end(g));
-                if ((rc = sub_launch_round(c, g, S.cmds()))) return rc;
-                inflight[g] = true;
-            }
+    HostThreads ht(ng, c->device);
+    ht.run([&](int g0, int g1, std::string &) -> int {
+        int rc;
+        SlotPool pool(SlotPool::default_threads());
+        bool inflight[sdca::kMaxGroups] = {};
+        for (int g = g0; g < g1; ++g) {
+            if (!group_active(g)) continue;
+            S.mark_live(g * gs, group_end(g));
+            if ((rc = sub_launch_round(c, g, S.cmds()))) return rc;
+            inflight[g] = true;
         }
-        if (!any) break;
-    }
+        std::vector<int> finished, keep;
+        while (!ht.stopped()) {
+            bool any = false;
+            for (int g = g0; g < g1; ++g) {
+                if (!inflight[g]) continue;
+                any = true;
+                inflight[g] = false;
+                if ((rc = sub_finish_round(c, g, res))) return rc;
+                finished.clear();
+                const auto th = std::chrono::steady_clock::now();
+                pool.run(g * gs, group_end(g), [&](int i) {
+                    done_flag[i] = S.slot(i).phase != PH_IDLE && S.process(i, res.data() + (size_t)i * R.width());
+                });
+                for (int i = g * gs; i < group_end(g); ++i)
+                    if (done_flag[i]) finished.push_back(i);
+                const double hms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th).count();
+                hipStream_t st = c->gstream[g];
+                // restricted optima (coordinates, then the fields) into the staged starts
+                int64_t *meta = G->h_stage + (size_t)g * G->gs * 5;
+                keep.clear();
+                for (int i : finished)
+                    if (S.slot(i).status != SDCA_ST_NONFINITE) keep.push_back(i);
+                const int nk = (int)keep.size();
+                for (int k = 0; k < nk; ++k) {
+                    const Slot &sl = S.slot(keep[k]);
+                    meta[k] = (int64_t)keep[k] * G->vstride_r +
+                              (sl.from_xt ? sdca::gram::kVecXt : sdca::gram::kVecXc) * G->nvs_r;
+                    meta[nk + k] = (int64_t)sl.item * R1 * kp;
+                    meta[2 * nk + k] = (int64_t)R1 * kp;
+                    meta[3 * nk + k] = 2 * (int64_t)R1 * kp;
+                    meta[4 * nk + k] = G->wfld + (int64_t)sl.item * 8 - (int64_t)sl.item * R1 * kp;
+                    ok[sl.item] = 1;
+                }
+                if (nk > 0) {
+                    int64_t *dmeta = G->d_stage + (size_t)g * G->gs * 5;
+                    HIP_TRY(hipMemcpyAsync(dmeta, meta, (size_t)nk * 5 * sizeof(int64_t), hipMemcpyHostToDevice, st));
+                    sdca::gram::gather(st, G->Vr, G->Wst, dmeta, nk, (int64_t)R1 * kp, 8);
+                    HIP_TRY(hipGetLastError());
+                }
+                {
+                    std::lock_guard<std::mutex> lk(G->mu);
+                    c->timing.host += hms;
+                    for (int i : finished) {
+                        nev[S.slot(i).item] = S.slot(i).nfeval;
+                        ++done;
+                        fill(i);
+                    }
+                }
+                if (group_active(g)) {
+                    S.mark_live(g * gs, group_end(g));
+                    if ((rc = sub_launch_round(c, g, S.cmds()))) return rc;
+                    inflight[g] = true;
+                }
+            }
+            if (!any) break;
+        }
+        return SDCA_OK;
+    });
+    if ((rc = ht.result())) return fail(rc, "%s", ht.message().c_str());
     for (int g = 0; g < ng; ++g) HIP_TRY(hipStreamSynchronize(c->gstream[g]));
     return SDCA_OK;
 }
@@ -2543,7 +2611,6 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
     const int ng = G->ngroups, gs = G->gs;
     std::vector<double> res((size_t)nslots * R.width(), 0.0);
     std::vector<char> done_flag(nslots, 0);
-    SlotPool pool(SlotPool::default_threads());
     // Order of the full solves.  After the restricted solves, loci start in decreasing order
     // of their restricted evaluations, which predict the full ones (C4: correlation 0.8):
     // the longest solves start first, so the last rounds are not left to a few slow loci
@@ -2569,58 +2636,73 @@ int gram_solve_batch(sdca_ctx *c, const int32_t *alns, const int32_t *loci, int3
         return false;
     };
     for (int i = 0; i < nslots; ++i) fill(i);
-    bool inflight[sdca::kMaxGroups] = {};
-    for (int g = 0; g < ng; ++g) {
-        if (!group_active(g)) continue;
-        S.mark_live(g * gs, group_end(g));
-        if ((rc = gram_launch_round(c, g, S.cmds()))) return rc;
-        inflight[g] = true;
-    }
     int64_t rounds = 0;
-    std::vector<int> finished;
-    // the groups' rounds alternate: while the host steps one group's line searches (and
-    // forms its finished loci's couplings), the other group's round runs on the device
-    while (done < K) {
-        bool any = false;
-        for (int g = 0; g < ng; ++g) {
-            if (!inflight[g]) continue;
-            any = true;
-            inflight[g] = false;
-            if ((rc = gram_finish_round(c, g, S.cmds(), res))) return rc;
-            ++rounds;
-            finished.clear();
-            const auto th = std::chrono::steady_clock::now();
-            pool.run(g * gs, group_end(g), [&](int i) {
-                done_flag[i] = S.slot(i).phase != PH_IDLE && S.process(i, res.data() + (size_t)i * R.width());
-            });
-            for (int i = g * gs; i < group_end(g); ++i)
-                if (done_flag[i]) finished.push_back(i);
-            c->timing.host += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th).count();
-            if (!finished.empty()) {
-                if ((rc = gram_stage(c, g, S, finished, st_out))) return rc;
-                for (int i : finished) {
-                    const Slot &sl = S.slot(i);
-                    noconv += (sl.status == SDCA_ST_ITER_CAP || sl.status == SDCA_ST_LS_FAIL);
-                    nonfinite += (sl.status == SDCA_ST_NONFINITE);
-                    ++done;
-                    fill(i);
-                }
-                if (c->opts.verbose)
-                    fprintf(stderr, "sdca: %d / %d loci solved after %lld rounds\n", done, K, (long long)rounds);
-            }
-            // a full batch of finished loci: couplings and score rows (while the other group's
-            // round runs)
-            if ((rc = gram_flush(c, false, score_rows, row_off, x_out))) return rc;
-            if (group_active(g)) {
-                S.mark_live(g * gs, group_end(g));
-                if ((rc = gram_launch_round(c, g, S.cmds()))) return rc;
-                inflight[g] = true;
-            }
+    // The groups' rounds alternate: while the host steps one group's line searches (and
+    // forms its finished loci's couplings), the other group's round runs on the device.
+    // With four groups, two host threads step two groups each.
+    HostThreads ht(ng, c->device);
+    ht.run([&](int g0, int g1, std::string &msg) -> int {
+        int rc;
+        SlotPool pool(SlotPool::default_threads());
+        bool inflight[sdca::kMaxGroups] = {};
+        for (int g = g0; g < g1; ++g) {
+            if (!group_active(g)) continue;
+            S.mark_live(g * gs, group_end(g));
+            if ((rc = gram_launch_round(c, g, S.cmds()))) return (msg = g_err, rc);
+            inflight[g] = true;
         }
-        if (!any) break;
+        std::vector<int> finished;
+        while (!ht.stopped()) {
+            bool any = false;
+            for (int g = g0; g < g1; ++g) {
+                if (!inflight[g]) continue;
+                any = true;
+                inflight[g] = false;
+                if ((rc = gram_finish_round(c, g, S.cmds(), res))) return (msg = g_err, rc);
+                finished.clear();
+                const auto th = std::chrono::steady_clock::now();
+                pool.run(g * gs, group_end(g), [&](int i) {
+                    done_flag[i] = S.slot(i).phase != PH_IDLE && S.process(i, res.data() + (size_t)i * R.width());
+                });
+                for (int i = g * gs; i < group_end(g); ++i)
+                    if (done_flag[i]) finished.push_back(i);
+                const double hms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th).count();
+                {
+                    std::lock_guard<std::mutex> lk(G->mu);
+                    c->timing.host += hms;
+                    ++rounds;
+                    if (!finished.empty()) {
+                        if ((rc = gram_stage(c, g, S, finished, st_out))) return (msg = g_err, rc);
+                        for (int i : finished) {
+                            const Slot &sl = S.slot(i);
+                            noconv += (sl.status == SDCA_ST_ITER_CAP || sl.status == SDCA_ST_LS_FAIL);
+                            nonfinite += (sl.status == SDCA_ST_NONFINITE);
+                            ++done;
+                            fill(i);
+                        }
+                        if (c->opts.verbose)
+                            fprintf(stderr, "sdca: %d / %d loci solved after %lld rounds\n", done, K, (long long)rounds);
+                    }
+                    // a full batch of finished loci: couplings and score rows (on their own
+                    // stream, while the rounds go on)
+                    if ((rc = gram_flush(c, false, score_rows, row_off, x_out, &pool))) return (msg = g_err, rc);
+                }
+                if (group_active(g)) {
+                    S.mark_live(g * gs, group_end(g));
+                    if ((rc = gram_launch_round(c, g, S.cmds()))) return (msg = g_err, rc);
+                    inflight[g] = true;
+                }
+            }
+            if (!any) break;
+        }
+        return SDCA_OK;
+    });
+    if ((rc = ht.result())) return fail(rc, "%s", ht.message().c_str());
+    {
+        SlotPool pool(SlotPool::default_threads());
+        if ((rc = gram_flush(c, true, score_rows, row_off, x_out, &pool))) return rc;
+        if ((rc = gram_consume(c, score_rows, row_off, &pool))) return rc;
     }
-    if ((rc = gram_flush(c, true, score_rows, row_off, x_out))) return rc;
-    if ((rc = gram_consume(c, score_rows, row_off))) return rc;
     G->warm_rounds = false;
     G->sub_nev = nullptr;
     c->timing.total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

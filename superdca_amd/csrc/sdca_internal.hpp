@@ -17,6 +17,7 @@
 
 #include <cstdint>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/sdca.h"
@@ -494,6 +495,9 @@ struct GramWs {
     double *Cst = nullptr, *Kst = nullptr;  // per item: Q a* and K Q a* (R x Bp each)
     GramAln *d_galn_r = nullptr;
     bool warm_rounds = false;  // the current solve's full rounds stage warm starts (k_gwarm)
+    // with four groups two host threads step two groups each: mu guards what they share
+    // (the locus queue, the counters and timing, the pending buffer and coupling batches)
+    std::mutex mu;
     const int *sub_nev = nullptr;  // the current solve's restricted evaluations per item
     ~GramWs();
 };
