@@ -2345,6 +2345,15 @@ int ensure_sub_ws(sdca_ctx *c, int32_t K)
             (rc = dalloc(&G->Wc, (size_t)npad * 64)) || (rc = dalloc(&G->d_galn_r, 1)))
             return rc;
     }
+    // restricted groups: each of the full solve's groups halved (slots and columns) when
+    // four fit, so that two host threads keep four short rounds in flight
+    G->rng = G->ngroups;
+    if (G->ngroups >= 2 && 2 * G->ngroups <= sdca::kMaxGroups && G->slots >= 256) {
+        const int rgs = (G->slots + 2 * G->ngroups - 1) / (2 * G->ngroups);
+        if (round_up((int64_t)rgs * G->gq, sdca::gram::kGemmCols) <= G->colcap / 2) G->rng = 2 * G->ngroups;
+    }
+    G->rgs = (G->slots + G->rng - 1) / G->rng;
+    G->rcc = G->colcap * G->ngroups / G->rng;
     // the restricted descriptor: vectors of q x kp coefficients, every "sequence" valid, no
     // own-column term in the products pass (the gradient kernel subtracts it)
     sdca::GramAln g{};
@@ -2385,7 +2394,7 @@ int sub_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds)
     sdca::GramWs *G = c->gram;
     const sdca_ctx *A = c->members[0];
     const int m = G->m, W = sdca::ResLayout{m}.width();
-    const int s0 = g * G->gs, s1 = std::min<int>((int)cmds.size(), s0 + G->gs);
+    const int s0 = g * G->rgs, s1 = std::min<int>((int)cmds.size(), s0 + G->rgs);
     std::vector<int> &lst = G->cmd_slot[g];
     lst.clear();
     sdca::SlotCmd *hc = G->h_cmds + s0;
@@ -2408,8 +2417,8 @@ int sub_launch_round(sdca_ctx *c, int g, const std::vector<sdca::SlotCmd> &cmds)
     hipStream_t st = c->gstream[g];
     hipEvent_t *ev = G->ev[g];
     sdca::SlotCmd *dc = G->d_cmds + s0;
-    double *Gp = G->Gp + (size_t)g * G->colcap * G->ldG, *Ep = G->Zp + (size_t)g * G->colcap * G->ldG;
-    double *Ap = G->Ap + (size_t)g * G->colcap * kp, *Zr = G->Zr + (size_t)g * G->colcap * kp;
+    double *Gp = G->Gp + (size_t)g * G->rcc * G->ldG, *Ep = G->Zp + (size_t)g * G->rcc * G->ldG;
+    double *Ap = G->Ap + (size_t)g * G->rcc * kp, *Zr = G->Zr + (size_t)g * G->rcc * kp;
     double *Wc = G->Wc + (size_t)s0 * 64;
     double *upd = G->upd_part + (size_t)s0 * (G->nchq + 1) * 3, *sm = G->sm_part + (size_t)s0 * G->nchunk * 65;
     double *dot = G->dot_part + (size_t)s0 * G->nchd * sdca::ResLayout{m}.ndots(), *res = G->res + (size_t)s0 * W;
@@ -2445,7 +2454,7 @@ int sub_finish_round(sdca_ctx *c, int g, std::vector<double> &res)
     HIP_TRY(hipStreamSynchronize(c->gstream[g]));
     const sdca::ResLayout R{G->m};
     const int W = R.width();
-    const double *hr = G->h_res + (size_t)g * G->gs * W;
+    const double *hr = G->h_res + (size_t)g * G->rgs * W;
     for (size_t k = 0; k < lst.size(); ++k) {
         double *o = res.data() + (size_t)lst[k] * W;
         std::memcpy(o, hr + k * W, W * sizeof(double));
@@ -2471,7 +2480,7 @@ int sub_solve(sdca_ctx *c, const int32_t *loci, int32_t K, std::vector<int> &nev
     const sdca::ResLayout R{m};
     const SolverLimits lim{c->opts.subspace_threshold, 0.0, c->opts.max_iter};
     Solver S(c, nslots, m, true, &lim);
-    const int ng = G->ngroups, gs = G->gs;
+    const int ng = G->rng, gs = G->rgs;
     const int R1 = sdca::gram::rows(c->members[0]->q);  // coordinate rows of a restricted vector half
     std::vector<double> res((size_t)nslots * R.width(), 0.0);
     std::vector<char> done_flag(nslots, 0);
@@ -2523,7 +2532,7 @@ int sub_solve(sdca_ctx *c, const int32_t *loci, int32_t K, std::vector<int> &nev
                 const double hms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - th).count();
                 hipStream_t st = c->gstream[g];
                 // restricted optima (coordinates, then the fields) into the staged starts
-                int64_t *meta = G->h_stage + (size_t)g * G->gs * 5;
+                int64_t *meta = G->h_stage + (size_t)g * gs * 5;
                 keep.clear();
                 for (int i : finished)
                     if (S.slot(i).status != SDCA_ST_NONFINITE) keep.push_back(i);
@@ -2539,7 +2548,7 @@ int sub_solve(sdca_ctx *c, const int32_t *loci, int32_t K, std::vector<int> &nev
                     ok[sl.item] = 1;
                 }
                 if (nk > 0) {
-                    int64_t *dmeta = G->d_stage + (size_t)g * G->gs * 5;
+                    int64_t *dmeta = G->d_stage + (size_t)g * gs * 5;
                     HIP_TRY(hipMemcpyAsync(dmeta, meta, (size_t)nk * 5 * sizeof(int64_t), hipMemcpyHostToDevice, st));
                     sdca::gram::gather(st, G->Vr, G->Wst, dmeta, nk, (int64_t)R1 * kp, 8);
                     HIP_TRY(hipGetLastError());

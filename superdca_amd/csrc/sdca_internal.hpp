@@ -434,6 +434,10 @@ struct GramWs {
     // rows and columns start at its first slot (or column) times the per-slot (per-column) size
     int ngroups = 1, gs = 0;
     int64_t colcap = 0;
+    // the subspace warm start's rounds are short: its groups (rng of rgs slots and rcc
+    // columns each, the same buffers) may be twice as many, so that more of them overlap
+    int rng = 1, rgs = 0;
+    int64_t rcc = 0;
     int64_t nvs = 0, vstride = 0, ldG = 0;
     double *V = nullptr;                  // slot vectors
     double *Gp = nullptr, *Zp = nullptr;  // packed GEMM operand G and product K G, ldG per column
