@@ -64,7 +64,8 @@ typedef struct sdca_opts {
     int32_t gemm_tn_grad;    /* > 0: force the gradient output tile width, 128 or 256 (no tuning) */
     int32_t pipeline_groups; /* 2: two slot groups whose rounds overlap on two streams (4: four,
                                 Gram formulation only); 1: one; 0: the formulation's default (Gram
-                                two, primal one) */
+                                two, primal one).  With two, the subspace warm start's short
+                                restricted rounds run in four groups of half the size */
     int32_t coarse_factor;   /* split-BF16: a locus' GEMMs use the BF16 high parts alone (one MFMA
                                 pass) while its ||g||_inf >= coarse_factor * grad_threshold, then
                                 its iterate is evaluated again in full and it finishes at split
