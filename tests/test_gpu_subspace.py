@@ -95,6 +95,25 @@ def test_slot_groups_give_identical_results(c2_q5, groups):
         np.testing.assert_array_equal(out[0][1][k], out[1][1][k])
 
 
+@pytest.mark.parametrize("slots", [257, 300])
+def test_restricted_groups_with_uneven_slot_counts(c2_q5, slots):
+    """Slot counts that four restricted groups do not divide evenly (257: three groups of 65
+    slots and one of 62; 300: four of 75): the same results as one group."""
+    st, w = c2_q5
+    loci = np.arange(0, st.shape[1], 13, dtype=np.int32)  # 385 loci
+    out = []
+    for g in (2, 1):
+        with sd.Context(st, q=5, batch_loci=slots, pipeline_groups=g, subspace_rank=128, **GRAM) as ctx:
+            ctx.set_weights(w)
+            rows, stats, _, rc = ctx.solve(loci)
+        assert rc == sd.SDCA_OK, rc
+        assert stats["nfeval_sub"].sum() > 0
+        out.append((rows, stats))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    for k in ("fval", "nfeval", "nfeval_sub", "iter"):
+        np.testing.assert_array_equal(out[0][1][k], out[1][1][k])
+
+
 def test_subspace_skipped_below_256_sequences(c1):
     """B = 200 < 2 x 128: the alignment solves from zero, bit-identical to the cold start."""
     st = c1["states"]
